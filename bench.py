@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""ISPH projection-step benchmark on one MI355X.
+
+Workload: the urban street canyon at 2M particles -- domain 2.0 x 1.0 x 0.5 m
+at dx = 0.008 m, 4x2 cuboid buildings (0.1 x 0.1 x 0.15 m, 0.2 m spacing)
+carried by IBM markers, log-law inflow (u* = 0.3 m/s, z0 = 0.001 m), MOST
+ground layer with Obukhov length L = +50 m, Smagorinsky LES (Cs = 0.12),
+dt = 5e-5 s, BiCGSTAB PPE at relative tolerance 1e-6.  The same steps run in
+FP64 and in MIXED precision; ``value`` is the MIXED throughput.
+
+particle-steps/s = n * K / wall time of K timed steps (after W warm-up steps,
+the first of which runs the neighbour search), state resident in HBM.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("urban-sph_a100_amd", "oracle"):
+    sys.path.insert(0, os.path.join(ROOT, sub))
+
+import numpy as np  # noqa: E402
+
+METRIC = "ISPH particle-steps/s @2M (mixed vs FP64, same PPE tol); p50/p99 case latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, dt, tol):
+    prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=precision)
+    case = urbansph.Case(prm, len(P))
+    case.upload(P, None, nrm)
+    count = 0
+    t_setup = time.perf_counter()
+    for _ in range(warmup):
+        case.step(count, dt)
+        case.sync()
+        count += 1
+    t_setup = time.perf_counter() - t_setup
+    lat = []
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        case.step(count, dt)
+        case.sync()
+        lat.append((time.perf_counter() - a) * 1e3)
+        iters.append(case.ppe_report().iterations)
+        count += 1
+    wall = time.perf_counter() - t0
+    # one more pass with per-kernel HIP-event timing (not part of the timed loop)
+    case.profile(True)
+    prof_steps = 2
+    for _ in range(prof_steps):
+        case.step(count, dt)
+        count += 1
+    case.sync()
+    prof = case.profile_report()
+    case.profile(False)
+    rep = case.ppe_report()
+    out = dict(value=len(P) * steps / wall, ms_per_step=wall / steps * 1e3, lat=lat, iters=iters,
+               prof=prof, prof_steps=prof_steps, rel_res=rep.rel_residual, refinements=rep.refinements,
+               warmup_s=t_setup, dev_bytes=case.device_bytes(), pres=case.field("PRES"), ux=case.field("UX"))
+    case.close()
+    return out
+
+
+def cpu_baseline(cases, abi, dt, tol, max_seconds=30.0):
+    """The FP64 oracle (OpenMP) on a bounded sample of the same workload: a
+    0.4 x 0.32 x 0.5 m slab of the canyon at the same dx with one building."""
+    import oracle
+    threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    P, nrm, geom = cases.canyon(dx=0.008, Lx=0.4, Ly=0.32, Lz=0.5, buildings=[(0.2, 0.16)])
+    prm = cases.canyon_params(geom, oracle.default_params(), dt=dt, tol=tol)
+    o = oracle.Oracle(prm, len(P))
+    o.upload(P, None, nrm)
+    o.step(0, dt)                       # neighbour search + first solve (not timed)
+    times = []
+    count = 1
+    t_all = time.perf_counter()
+    while count <= 5 and time.perf_counter() - t_all < max_seconds:
+        a = time.perf_counter()
+        o.step(count, dt)
+        times.append(time.perf_counter() - a)
+        count += 1
+    o.close()
+    med = float(np.median(times))
+    return dict(value=len(P) / med, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"canyon slab 0.4x0.32x0.5 m, dx=0.008, one building, n={len(P)}, "
+                       f"FP64 BiCGSTAB tol {tol:g}, median of {len(times)} steps after 1 warm-up")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--dx", type=float, default=0.008)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--dt", type=float, default=5e-5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(multi-GPU halo exchange is design-only)", file=sys.stderr)
+        sys.exit(2)
+    args.warmup = max(1, args.warmup)
+
+    import abi
+    import cases
+    import urbansph
+    urbansph.lib()
+
+    t = time.perf_counter()
+    P, nrm, geom = cases.canyon(dx=args.dx)
+    n = len(P)
+    log(f"[bench] canyon n={n} (fluid {geom['n_fluid']}, wall {geom['n_wall']}, lid {geom['n_lid']}, "
+        f"markers {geom['n_marker']}) built in {time.perf_counter() - t:.1f}s")
+
+    res = {}
+    for name, prec in (("fp64", abi.FP64), ("mixed", abi.MIXED)):
+        if args.only and args.only != name:
+            continue
+        log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps}")
+        res[name] = run_precision(cases, abi, urbansph, P, nrm, geom, prec, args.steps, args.warmup, args.dt,
+                                  args.tol)
+        r = res[name]
+        log(f"[bench] {name}: {r['value'] / 1e6:.2f} M particle-steps/s, {r['ms_per_step']:.1f} ms/step, "
+            f"PPE iters {r['iters']}, refinements {r['refinements']}, rel res {r['rel_res']:.2e}")
+
+    main_name = "mixed" if "mixed" in res else "fp64"
+    m = res[main_name]
+    # dominant kernel: the SpMV of the PPE solve
+    sp = m["prof"].get("spmv")
+    roof = None
+    if sp:
+        launches, ms, byts = sp
+        avg_s = ms / launches / 1e3
+        achieved = byts / launches / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "spmv", "avg_us": round(avg_s * 1e6, 2),
+                "alg_bytes_per_launch": round(byts / launches)}
+    tot_ms = sum(v[1] for v in m["prof"].values())
+    breakdown = {k: round(v[1] / m["prof_steps"], 3) for k, v in sorted(m["prof"].items(), key=lambda kv: -kv[1][1])}
+    lat = np.array(m["lat"])
+    line = {
+        "metric": METRIC,
+        "value": round(m["value"], 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(m["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair sums + f32 inner Krylov, f64 accumulation/dots/refinement" if main_name == "mixed" else "f64",
+        "data": "synthetic (generated street-canyon deck, no dataset)",
+        "config": {"workload": f"urban street canyon 2.0x1.0x0.5 m, dx={args.dx} m, 4x2 IBM buildings, "
+                               f"MOST L=+50 m, SSM Cs=0.12, dt={args.dt} s, BiCGSTAB tol {args.tol:g}",
+                   "particles": n, "precision": main_name},
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "what": "per-step latency of one resident case"},
+        "ppe_iters_per_step": m["iters"],
+        "kernel_ms_per_step": breakdown,
+        "roofline": roof,
+    }
+    if "fp64" in res:
+        line["fp64"] = {"value": round(res["fp64"]["value"], 1), "ms_per_step": round(res["fp64"]["ms_per_step"], 3),
+                        "ppe_iters_per_step": res["fp64"]["iters"]}
+        if "mixed" in res:
+            line["mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+            a, b = res["mixed"]["pres"], res["fp64"]["pres"]
+            line["mixed_vs_fp64_rel_l2"] = {
+                "pres": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                "ux": float(np.linalg.norm(res["mixed"]["ux"] - res["fp64"]["ux"]) / np.linalg.norm(res["fp64"]["ux"])),
+                "note": "after warmup+steps+2 steps of each run"}
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline(cases, abi, args.dt, args.tol)
+        log(f"[bench] cpu baseline {line['cpu_baseline']['value'] / 1e3:.1f} k particle-steps/s "
+            f"on {line['cpu_baseline']['cores']} threads")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

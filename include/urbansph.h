@@ -1,0 +1,293 @@
+/*
+ * urbansph.h -- C ABI of the MI355X-native ISPH projection step.
+ *
+ * This is the drop-in boundary for the reference's one-step entry point
+ *   SOPHIA_single_ISPH(...)                      ISPH_Calc.cuh:4-233
+ * and for the buffer ownership that its caller, the time loop, holds
+ *   ISPH_Calc(...)  allocations / upload / loop  ISPH_Calc.cuh:238-577
+ * The reference has no FFI; its host C++ calls the step directly with raw
+ * device pointers, host globals (vii/vif, function_init.cuh:1-163) and
+ * __constant__ mirrors.  Here the caller owns one opaque context per case
+ * (or per batch of cases) and passes plain pointers and sizes.  No torch or
+ * HIP types appear in any signature.
+ *
+ * All entry points return 0 on success and a negative SPH_E_* code on
+ * failure; sph_last_error() then holds a message.  The reference checks no
+ * errors at all (it never calls cudaGetLastError on the path).
+ *
+ * Threading: one host thread per context at a time.  Every call is ordered
+ * on the context's own HIP stream; calls that return host data synchronise
+ * that stream.
+ */
+#ifndef URBANSPH_H
+#define URBANSPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Particle records.  Field order and widths are those of the reference's AoS
+ * structs with Real=float and int_t=uint_t=int (Variable_Type.cuh:2-4):
+ *   part1  class_Cuda_Particle_Array.cuh:11-67   152 bytes
+ *   part2  class_Cuda_Particle_Array.cuh:69-85    52 bytes
+ *   part3  class_Cuda_Particle_Array.cuh:87-112  164 bytes
+ * A reference caller can pass its part1/part2/part3 arrays unchanged.
+ * ------------------------------------------------------------------------- */
+typedef struct sph_part1 {
+    int   i_type, buffer_type, MOST_buffer, p_type;
+    float x, y, z;
+    float ux, uy, uz;
+    float m, h, temp, pres, pres0, rho, flt_s, enthalpy, concn, concentration;
+    float k_turb, e_turb;
+    float eli, vol;
+    float m_ref, h_ref;
+    float vis_t;
+    float fbx, fby, fbz;
+    float fmx, fmy, hmz;
+    float float1, float2, float3;
+    int   ncell, source;
+} sph_part1;
+
+typedef struct sph_part2 {
+    float rho_ref, SR;
+    float x0, y0, z0;
+    float ux0, uy0, uz0;
+    float rho0, drho0, temp0, concn0, enthalpy0;
+} sph_part2;
+
+typedef struct sph_part3 {
+    float drho, dconcn, denthalpy;
+    float ftotalx, ftotaly, ftotalz, ftotal;
+    float fpx, fpy, fpz, fbody;
+    float dtemp, dk_turb, de_trub, cc;
+    float nx, ny, nz;
+    float Cm[3][3];
+    float inv_cm_xx, inv_cm_yy, inv_cm_zz;
+    float inv_cm_xy, inv_cm_yz, inv_cm_zx;
+    float Gxx, Gxy, Gyx, Gyy;
+    float L11, L12, L21, L22;
+} sph_part3;
+
+/* ---------------------------------------------------------------------------
+ * Option values (Parameters.cuh:26-31,33-34,58-61,64-69,73-76)
+ * ------------------------------------------------------------------------- */
+enum { SPH_GAUSSIAN = 0, SPH_QUINTIC = 1, SPH_QUARTIC = 2,
+       SPH_WENDLAND2 = 3, SPH_WENDLAND4 = 4, SPH_WENDLAND6 = 5 };
+enum { SPH_EULER = 0, SPH_PRE_COR = 1 };
+enum { SPH_FV_NONE = 0, SPH_FV_CLEARY = 1, SPH_FV_MONAGHAN = 2 };
+enum { SPH_LAMINAR = 0, SPH_SSM = 4, SPH_DDF = 5 };
+/* PPE solver: REF_JACOBI = the reference's single sweep of KERNEL_PPE3D
+ * (functions_PPE.cuh:2-175, ISPH_Calc.cuh:137) made deterministic (every row
+ * reads the pressures of the previous iterate); BICGSTAB = Jacobi-
+ * preconditioned BiCGSTAB on the same operator to a relative residual. */
+enum { SPH_PPE_REF_JACOBI = 0, SPH_PPE_BICGSTAB = 1 };
+/* Precision: FP64 = every field and sum in double.  MIXED = interaction
+ * sums and the inner PPE solve in float with double accumulation; Krylov
+ * dots, residual norms and an outer iterative-refinement loop in double. */
+enum { SPH_FP64 = 0, SPH_MIXED = 1 };
+/* MOST wall model (functions_MOST.cuh:175-322): REFERENCE = bulk-Richardson
+ * Newton iteration on the Obukhov length; PRESCRIBED_L = the length is given. */
+enum { SPH_MOST_REFERENCE = 0, SPH_MOST_PRESCRIBED_L = 1 };
+
+/* ---------------------------------------------------------------------------
+ * Solver parameters.  The integer/real knobs are the vii/vif entries the step
+ * reads (function_init.cuh:1-81, parsed at :166-463); the remaining members
+ * are compile-time constants of the reference made runtime (Parameters.cuh,
+ * functions_PROP.cuh:121-131, functions_NNPS.cuh:4-14).  sph_params_default()
+ * fills every member with the reference's value.
+ * ------------------------------------------------------------------------- */
+typedef struct sph_params {
+    /* vii */
+    int    dim;                 /* only 3 is supported                       */
+    int    kernel_type;         /* SPH_GAUSSIAN .. SPH_WENDLAND6              */
+    int    time_type;           /* SPH_EULER / SPH_PRE_COR                    */
+    int    freq_filt;           /* Shepard filter period, must be > 0         */
+    int    fp_solve;            /* pressure force                             */
+    int    fv_solve;            /* SPH_FV_*                                   */
+    int    fg_solve;            /* gravity                                    */
+    int    con_solve;           /* temperature equation                       */
+    int    boussinesq_solve;
+    int    kgc_solve;           /* only 0 (NO) is accepted, see DESIGN        */
+    int    turbulence_model;    /* SPH_LAMINAR / SPH_SSM / SPH_DDF            */
+    int    concn_solve;         /* concentration diffusion                    */
+    int    noslip_bc;
+    int    artificial_viscosity;/* 1 = always on, as functions_PPE.cuh:268    */
+    /* vif */
+    double kappa;               /* support radius factor k in k*h             */
+    double u_limit;             /* velocity limiter, functions_TIME_ISPH:182  */
+    /* constants made runtime */
+    double mu_fluid;            /* viscosity2(): p_type 1 and others          */
+    double mu_wall;             /* viscosity2(): p_type 0 and 1000            */
+    double gravity;             /* Gravitational_CONST                        */
+    double T_ini, T_con, T_start_t, T_con_t, tempb;
+    double Cs_sgs, Ck_sgs, Ce_sgs;
+    double rho_air;             /* IBM penalty density (Rho_air)              */
+    double av_alpha, av_beta, av_c;   /* artificial viscosity 0.05,0.05,30   */
+    double solid_k, air_k, solid_cp, air_cp, solid_rho;  /* con_solve only   */
+    double diff_coef;           /* diffusion_coefficient() (concn_solve)      */
+    /* open-boundary geometry (functions_NNPS.cuh:4-14, functions_BC.cuh:732) */
+    double L2, L3, L_left, L_right;
+    double outlet_copy_dx;      /* source plane at L3 - outlet_copy_dx        */
+    double copy_tol;            /* exact-position match radius (0.001 m)      */
+    double neumann_ghost_r;     /* Neumann inlet ghost match radius (0.1 m)   */
+    /* MOST */
+    int    most_mode;
+    double most_z0, most_z0h, most_L;
+    double most_ghost_dz, most_ghost_r;   /* heat-flux deposit: z+5 m, 0.1 m  */
+    /* IBM */
+    double ibm_ut_factor;       /* target tangential speed = f * u_caseH(z)   */
+    /* PPE solve */
+    int    ppe_solver;          /* SPH_PPE_*                                  */
+    int    precision;           /* SPH_FP64 / SPH_MIXED                       */
+    double ppe_tol;             /* relative residual ||b-Ax||/||b||           */
+    int    ppe_maxit;           /* max BiCGSTAB iterations per solve          */
+    double ppe_inner_tol;       /* MIXED: relative tol of each inner solve    */
+    int    ppe_max_refine;      /* MIXED: max outer refinement steps          */
+} sph_params;
+
+/* ---------------------------------------------------------------------------
+ * Errors
+ * ------------------------------------------------------------------------- */
+enum {
+    SPH_OK = 0,
+    SPH_E_ARG = -1,          /* invalid argument / configuration            */
+    SPH_E_HIP = -2,          /* HIP runtime or kernel launch failure        */
+    SPH_E_NOMEM = -3,        /* device allocation failed / budget exceeded  */
+    SPH_E_STATE = -4,        /* call out of order (e.g. step before upload) */
+    SPH_E_NOCONV = -5,       /* PPE did not reach ppe_tol within ppe_maxit  */
+    SPH_E_NONFINITE = -6,    /* NaN/Inf detected in the state               */
+    SPH_E_UNSUPPORTED = -7   /* option the build does not implement         */
+};
+
+/* ---------------------------------------------------------------------------
+ * Field identifiers for sph_get_field_f64 / sph_get_field_i32: state in the
+ * device (cell-sorted) order.  P_* = persistent state (P1/P2 after the step),
+ * W_* = step working copies (SP1/SP2), Q_* = per-step scratch (P3).
+ * ------------------------------------------------------------------------- */
+enum {
+    SPH_F_X = 0, SPH_F_Y, SPH_F_Z, SPH_F_UX, SPH_F_UY, SPH_F_UZ,
+    SPH_F_M, SPH_F_H, SPH_F_TEMP, SPH_F_PRES, SPH_F_PRES0, SPH_F_RHO,
+    SPH_F_FLT_S, SPH_F_CONCN, SPH_F_CONCENTRATION, SPH_F_K_TURB,
+    SPH_F_E_TURB, SPH_F_VIS_T, SPH_F_FBX, SPH_F_FBY, SPH_F_FBZ,
+    SPH_F_FMX, SPH_F_FMY, SPH_F_HMZ, SPH_F_FLOAT1, SPH_F_FLOAT2,
+    SPH_F_FLOAT3, SPH_F_VOL, SPH_F_ENTHALPY,
+    /* part2 */
+    SPH_F_RHO_REF, SPH_F_SR, SPH_F_X0, SPH_F_Y0, SPH_F_Z0,
+    SPH_F_UX0, SPH_F_UY0, SPH_F_UZ0, SPH_F_TEMP0, SPH_F_CONCN0,
+    /* part3 (scratch of the last step) */
+    SPH_F_FTOTALX, SPH_F_FTOTALY, SPH_F_FTOTALZ, SPH_F_FTOTAL,
+    SPH_F_FPX, SPH_F_FPY, SPH_F_FPZ, SPH_F_DTEMP, SPH_F_DCONCN,
+    SPH_F_INV_CM_XX, SPH_F_INV_CM_YY, SPH_F_INV_CM_ZZ,
+    SPH_F_INV_CM_XY, SPH_F_INV_CM_YZ, SPH_F_INV_CM_ZX,
+    SPH_F_NX, SPH_F_NY, SPH_F_NZ,
+    SPH_F_COUNT_F64
+};
+enum {
+    SPH_I_I_TYPE = 0, SPH_I_BUFFER_TYPE, SPH_I_MOST_BUFFER, SPH_I_P_TYPE,
+    SPH_I_NCELL, SPH_I_SOURCE, SPH_I_COUNT
+};
+
+/* Domain and cell grid as the reference sizes it (ISPH.cuh:75-101,
+ * function_init.cuh:678-708, functions_NNPS.cuh:534-546).  All float. */
+typedef struct sph_domain {
+    float x_min, x_max, y_min, y_max, z_min, z_max;
+    float dcell;
+    int   NI, NJ, NK;
+    int   num_cells;            /* NI*NJ*NK + 1 (last = dummy key)           */
+} sph_domain;
+
+/* PPE report of the last step. */
+typedef struct sph_ppe_report {
+    int    iterations;          /* BiCGSTAB iterations (inner, summed)       */
+    int    refinements;         /* MIXED: outer refinement steps             */
+    int    unknowns;            /* rows of the solve                         */
+    long long nnz;              /* stored off-diagonal entries               */
+    double rel_residual;        /* final ||b-Ax||/||b|| (true, FP64)         */
+    int    converged;
+} sph_ppe_report;
+
+typedef struct sph_ctx sph_ctx;
+
+/* Fills every member with the reference's value (see sph_params comments). */
+void sph_params_default(sph_params* p);
+
+/* Creates a context for one case with room for n_part2 particles
+ * (num_part2, ISPH.cuh:111-128).  Validates the parameters: dim==3,
+ * freq_filt>0, kgc_solve==0, kappa>0, supported option values. */
+int sph_create(const sph_params* prm, int n_part2, sph_ctx** out);
+
+/* Host AoS -> device.  P2 may be NULL (the reference uploads zeros,
+ * ISPH_Calc.cuh:351).  ib_normals (3 floats per particle, may be NULL) are
+ * the outward normals of IBM markers (p_type >= 1000); they persist for the
+ * life of the case.  Sizes the domain and cell grid from the positions and
+ * resets the step state: the next sph_step must pass count == 0, which runs
+ * the neighbour search (ISPH_Calc.cuh:23-52). */
+int sph_upload(sph_ctx* c, const sph_part1* P1, const sph_part2* P2,
+               const float* ib_normals, int n);
+
+/* One projection step == SOPHIA_single_ISPH (ISPH_Calc.cuh:4-233) for the
+ * given step counter, time-step and time.  Asynchronous: returns after
+ * enqueueing (PPE convergence is checked at the next synchronising call
+ * unless the solve itself needs host decisions, which BiCGSTAB does). */
+int sph_step(sph_ctx* c, int count, double dt, double time);
+
+/* Device -> host AoS, in the device (cell-sorted) order.  Any pointer may be
+ * NULL.  Doubles are rounded to float. */
+int sph_download(sph_ctx* c, sph_part1* P1, sph_part2* P2, sph_part3* P3, int n);
+
+/* One field in full precision, device order.  n must equal the particle
+ * count. */
+int sph_get_field_f64(sph_ctx* c, int field, double* out, int n);
+int sph_get_field_i32(sph_ctx* c, int field, int* out, int n);
+
+/* Neighbour-search products of the last count==0 step: sorted cell keys and
+ * the sort permutation (n each), and cell start/end (num_cells each; empty
+ * cells have start -1 and end 0).  Any pointer may be NULL. */
+int sph_nnps_debug(sph_ctx* c, int* keys, int* perm, int* cell_start,
+                   int* cell_end);
+
+/* Neighbour lists of the count==0 search, CSR in device order: for particle
+ * i, entries [offsets[i], offsets[i+1]) are ascending neighbour indices.
+ * kind 0 = the dcell-query window list (r < kappa*max(hi,hj), self
+ * included); kind 1 = the binning-window list (0 < r < kappa*hi).
+ * Call with nbr == NULL to get the entry count in *total. */
+int sph_neighbors(sph_ctx* c, int kind, int* offsets, int* nbr, long long* total);
+
+int sph_domain_info(const sph_ctx* c, sph_domain* d);
+int sph_ppe_info(const sph_ctx* c, sph_ppe_report* r);
+
+/* Max |u|, max %drho and max |f| over fluid particles (kernel_copy_max +
+ * 3x DeviceReduce::Max, functions_OUTPUT.cuh:19-33, ISPH_Calc.cuh:504-514). */
+int sph_stats(sph_ctx* c, float* umax, float* drho_pct_max, float* fmax);
+
+/* Blocks until all enqueued work of the context is done; reports deferred
+ * errors (non-convergence, non-finite state). */
+int sph_synchronize(sph_ctx* c);
+
+/* Per-kernel timing with HIP events on the context's stream.  enable=1
+ * starts recording (adds an event pair around every launch); the report
+ * lists, per kernel name, launches, total milliseconds and algorithmic
+ * bytes moved.  Returns the number of kernels written (<= max). */
+int sph_profile_enable(sph_ctx* c, int enable);
+typedef struct sph_kernel_time {
+    char   name[48];
+    long long launches;
+    double total_ms;
+    double alg_bytes;           /* sum over launches of algorithmic bytes    */
+} sph_kernel_time;
+int sph_profile_report(sph_ctx* c, sph_kernel_time* out, int max);
+
+/* Device memory held by the context, bytes. */
+long long sph_device_bytes(const sph_ctx* c);
+
+const char* sph_last_error(const sph_ctx* c);
+void sph_destroy(sph_ctx* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* URBANSPH_H */

@@ -1,0 +1,219 @@
+"""Parity of the HIP library with the CPU oracle, through the C ABI.
+
+NNPS products (cell keys, permutation, cell start/end, neighbour sets) must be
+bit-exact.  FP64 fields must match within 1e-10 relative L2 (the FP64 path
+restates the same IEEE expressions, so most fields match exactly).  MIXED
+fields must match the FP64 oracle within the tolerance stated per test.
+"""
+import numpy as np
+import pytest
+
+import abi
+import cases
+import oracle
+import urbansph
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FIELDS = ["UX", "UY", "UZ", "PRES", "TEMP", "VIS_T", "FLT_S", "FBX", "FBY", "FBZ", "FMX", "FMY", "HMZ",
+          "FLOAT1", "FLOAT2", "FLOAT3", "CONCENTRATION", "CONCN", "K_TURB", "E_TURB", "RHO_REF", "SR",
+          "UX0", "UY0", "UZ0", "FTOTALX", "FTOTALY", "FTOTALZ", "FTOTAL", "FPX", "FPY", "FPZ", "DTEMP",
+          "DCONCN", "INV_CM_XX", "INV_CM_YY", "INV_CM_ZZ", "INV_CM_XY", "INV_CM_YZ", "INV_CM_ZX"]
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm(a - b)
+    s = np.linalg.norm(b)
+    return d / s if s > 0 else d
+
+
+def run_pair(P, prm, normals=None, steps=1, dt=1e-4):
+    n = len(P)
+    print(f"[parity] n={n} solver={prm.ppe_solver} precision={prm.precision} steps={steps}", flush=True)
+    o = oracle.Oracle(prm, n)
+    o.upload(P, None, normals)
+    g = urbansph.Case(prm, n)
+    g.upload(P, None, normals)
+    for c in range(steps):
+        rc = o.step(c, dt)
+        assert rc == 0, f"oracle step rc={rc}"
+        g.step(c, dt)
+    g.sync()
+    return o, g
+
+
+def assert_nnps_equal(o, g):
+    ko, po, so, eo = o.nnps()
+    kg, pg, sg, eg = g.nnps()
+    np.testing.assert_array_equal(kg, ko)
+    np.testing.assert_array_equal(pg, po)
+    np.testing.assert_array_equal(sg, so)
+    np.testing.assert_array_equal(eg, eo)
+    for kind in (0, 1):
+        oo, no = o.neighbors(kind)
+        og, ng = g.neighbors(kind)
+        np.testing.assert_array_equal(og, oo)
+        np.testing.assert_array_equal(ng, no)
+
+
+def assert_fields(o, g, tol, fields=FIELDS, report=None):
+    worst = {}
+    for f in fields:
+        a, b = g.field(f), o.field(f)
+        assert np.all(np.isfinite(a)), f
+        e = rel_l2(a, b)
+        worst[f] = e
+        assert e <= tol, f"{f}: rel L2 {e:.3e} > {tol:.1e}"
+    if report is not None:
+        report.update(worst)
+    return worst
+
+
+# ------------------------------------------------------------------ FP64 --
+@pytest.mark.parametrize("solver", [abi.PPE_BICGSTAB, abi.PPE_REF_JACOBI])
+def test_box_fp64(solver):
+    P, _ = cases.box(nf=12, walls=3)
+    prm = cases.box_params(urbansph.default_params(), tol=1e-8)
+    prm.update(ppe_solver=solver)
+    o, g = run_pair(P, prm, steps=3)
+    assert_nnps_equal(o, g)
+    assert_fields(o, g, FP64_TOL)
+    ro, rg = o.ppe_report(), g.ppe_report()
+    assert rg.iterations == ro.iterations
+    assert rg.converged == 1
+    for a, b in zip(g.stats(), o.stats()):
+        assert a == pytest.approx(b, rel=1e-6, abs=1e-12)
+
+
+def test_canyon_fp64():
+    P, nrm, geom = cases.small_canyon(dx=0.01)
+    prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-8)
+    o, g = run_pair(P, prm, normals=nrm, steps=2, dt=5e-5)
+    assert_nnps_equal(o, g)
+    assert_fields(o, g, FP64_TOL)
+    assert g.ppe_report().iterations == o.ppe_report().iterations
+
+
+def test_canyon_options_fp64():
+    """Remaining branches: MOST Richardson/Newton branch with a warm surface,
+    temperature and concentration transport, Deardorff LES, Monaghan
+    viscosity, gravity, Euler stepping."""
+    P, nrm, geom = cases.small_canyon(dx=0.012)
+    P["temp"] = np.where(P["p_type"] == 1, 302.0, 300.0)
+    P["concn"] = np.where(P["x"] < 0.1, 1.0, 0.0)
+    P["k_turb"] = 0.01
+    P["source"] = (np.arange(len(P)) % 97 == 0).astype(np.int32)
+    for variant in range(3):
+        prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-9)
+        prm.update(most_mode=abi.MOST_REFERENCE, con_solve=1, concn_solve=1, T_con=301.0)
+        if variant == 0:
+            prm.update(turbulence_model=abi.DDF)
+        elif variant == 1:
+            prm.update(fv_solve=abi.FV_MONAGHAN, fg_solve=1)
+        else:
+            prm.update(time_type=abi.EULER, boussinesq_solve=1, ppe_solver=abi.PPE_REF_JACOBI)
+        o, g = run_pair(P, prm, normals=nrm, steps=2, dt=5e-5)
+        # MOST uses log/atan whose last bit may differ between libm and the
+        # device library, so this case is held to the FP64 tolerance only
+        assert_fields(o, g, FP64_TOL)
+
+
+@pytest.mark.parametrize("ktype", [abi.GAUSSIAN, abi.QUINTIC, abi.QUARTIC, abi.WENDLAND4, abi.WENDLAND6])
+def test_kernel_types_fp64(ktype):
+    P, _ = cases.box(nf=8, walls=3)
+    prm = cases.box_params(urbansph.default_params(), tol=1e-9)
+    prm.update(kernel_type=ktype, ppe_solver=abi.PPE_REF_JACOBI)
+    o, g = run_pair(P, prm, steps=2)
+    assert_fields(o, g, FP64_TOL)
+
+
+def test_nnps_random_cloud():
+    import test_oracle_nnps as t
+    P = t._random_cloud(4000, seed=11)
+    prm = urbansph.default_params()
+    o, g = run_pair(P, prm, steps=1)
+    assert_nnps_equal(o, g)
+    assert_fields(o, g, FP64_TOL)
+
+
+# ------------------------------------------------------------------ MIXED --
+MIXED_TOL = 1e-5
+
+
+@pytest.mark.parametrize("which", ["box", "canyon"])
+def test_mixed_vs_fp64_oracle(which):
+    if which == "box":
+        P, nrm = cases.box(nf=12, walls=3)
+        prm = cases.box_params(urbansph.default_params(), tol=1e-6)
+        dt = 1e-4
+    else:
+        P, nrm, geom = cases.small_canyon(dx=0.01)
+        prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-6)
+        dt = 5e-5
+    o = oracle.Oracle(prm, len(P))
+    o.upload(P, None, nrm)
+    o.step(0, dt)
+    prm.update(precision=abi.MIXED)
+    g = urbansph.Case(prm, len(P))
+    g.upload(P, None, nrm)
+    g.step(0, dt)
+    g.sync()
+    assert_nnps_equal(o, g)
+    rep = g.ppe_report()
+    assert rep.converged == 1 and rep.rel_residual <= 1e-6
+    errs = {}
+    for f in ("PRES", "UX", "UY", "UZ"):
+        errs[f] = rel_l2(g.field(f), o.field(f))
+    print("[mixed]", which, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for f, e in errs.items():
+        assert e <= MIXED_TOL, f"{f}: {e:.3e}"
+
+
+# ------------------------------------------------------------ ABI paths --
+def test_error_paths():
+    P, _ = cases.box(nf=4, walls=1)
+    prm = cases.box_params(urbansph.default_params())
+    g = urbansph.Case(prm, len(P))
+    with pytest.raises(urbansph.SPHError) as e:
+        g.step(0, 1e-4)
+    assert e.value.rc == abi.E_STATE
+    g.upload(P)
+    with pytest.raises(urbansph.SPHError) as e:
+        g.step(1, 1e-4)
+    assert e.value.rc == abi.E_STATE
+    with pytest.raises(urbansph.SPHError) as e:
+        g.upload(np.concatenate([P, P]))
+    assert e.value.rc == abi.E_ARG
+    bad = cases.box_params(urbansph.default_params())
+    bad.update(freq_filt=0)
+    with pytest.raises(urbansph.SPHError) as e:
+        urbansph.Case(bad, 10)
+    assert e.value.rc == abi.E_ARG
+    bad = cases.box_params(urbansph.default_params())
+    bad.update(kgc_solve=1)
+    with pytest.raises(urbansph.SPHError) as e:
+        urbansph.Case(bad, 10)
+    assert e.value.rc == abi.E_UNSUPPORTED
+    # non-convergence is reported, not hidden
+    prm2 = cases.box_params(urbansph.default_params(), tol=1e-14)
+    prm2.update(ppe_maxit=2)
+    g2 = urbansph.Case(prm2, len(P))
+    g2.upload(P)
+    with pytest.raises(urbansph.SPHError) as e:
+        g2.step(0, 1e-4)
+    assert e.value.rc == abi.E_NOCONV
+
+
+def test_download_roundtrip():
+    P, _ = cases.box(nf=6, walls=2)
+    prm = cases.box_params(urbansph.default_params())
+    g = urbansph.Case(prm, len(P))
+    g.upload(P)
+    g.step(0, 1e-4)
+    P1, P2, P3 = g.download()
+    np.testing.assert_array_equal(P1["ux"], g.field("UX").astype(np.float32))
+    np.testing.assert_array_equal(P1["pres"], g.field("PRES").astype(np.float32))
+    np.testing.assert_array_equal(P3["fpx"], g.field("FPX").astype(np.float32))
+    keys, perm, _, _ = g.nnps()
+    np.testing.assert_array_equal(P1["x"], P["x"][perm])

@@ -1,0 +1,426 @@
+// Neighbour search of the count==0 step on MI355X (ISPH_Calc.cuh:23-52):
+//   KERNEL_index_particle_to_cell  functions_NNPS.cuh:879-907  -> k_hash
+//   cub::DeviceRadixSort::SortPairs ISPH_Calc.cuh:33           -> radix_sort
+//   KERNEL_reorder (start/end)     functions_NNPS.cuh:813-838  -> k_cell_bounds
+//   KERNEL_reorder (gather)        functions_NNPS.cuh:840-843  -> permute
+// plus the per-particle neighbour list that the per-step passes walk
+// (the reference rescans the cell window in every pass).
+//
+// The sort is a stable least-significant-digit radix sort over only the
+// ceil(log2(num_cells+1)) key bits, 8 bits per pass.  Each 256-thread tile
+// of 2048 keys ranks its keys with wave64 ballots (a match-any over the
+// digit bits) and LDS per-wave digit counters, so equal digits keep input
+// order and the result equals a stable sort.
+#include "flags.h"
+#include "sph_internal.h"
+#include "sph_math.h"
+
+namespace sph {
+
+// ------------------------------------------------------------------------
+// KERNEL_set_ncell, functions_NNPS.cuh:943-955
+__global__ void __launch_bounds__(BLOCK) k_set_ncell(int n, const int* i_type, const double* h, const double* h_ref,
+                                                     int* ncell) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i_type[i] == 3) return;
+    ncell[i] = (int)(1.0 * ceilf(2.0f / (float)(h_ref[i] / h[i])));
+}
+
+// KERNEL_index_particle_to_cell, functions_NNPS.cuh:879-907 (float binning)
+__global__ void __launch_bounds__(BLOCK) k_hash(int n, sph_domain d, const int* i_type, const double* x,
+                                                const double* y, const double* z, int* keys, int* vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int key;
+    if (i_type[i] == 3) key = d.num_cells;
+    else {
+        Cell c = bin_cell(d, x[i], y[i], z[i]);
+        key = c.i + d.NI * c.j + d.NI * d.NJ * c.k;
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// ------------------------------------------------------------------------
+// radix sort
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = BLOCK * RS_ITEMS;
+
+__global__ void __launch_bounds__(BLOCK) k_rs_hist(const int* keys, int n, int shift, int ntiles, int* hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    long long base = (long long)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        long long idx = base + k * BLOCK + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_rs_scatter(const int* kin, const int* vin, int* kout, int* vout, int n,
+                                                      int shift, int ntiles, const int* offs) {
+    __shared__ int cnt[256];
+    __shared__ int wc[BLOCK / 64][256];
+    __shared__ int boff[256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    cnt[tid] = 0;
+    boff[tid] = offs[tid * ntiles + blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < BLOCK / 64; q++) wc[q][tid] = 0;
+    __syncthreads();
+    long long base = (long long)blockIdx.x * RS_TILE;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int k = 0; k < RS_ITEMS; k++) {
+        long long idx = base + k * BLOCK + tid;
+        bool valid = idx < n;
+        int key = valid ? kin[idx] : 0;
+        int val = valid ? vin[idx] : 0;
+        int dg = valid ? ((key >> shift) & 255) : 256;
+        unsigned long long peers = ~0ull;
+#pragma unroll
+        for (int b = 0; b < 9; b++) {
+            int bit = (dg >> b) & 1;
+            unsigned long long bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        int wrank = __popcll(peers & lt);
+        int wcount = __popcll(peers);
+        int leader = __ffsll((long long)peers) - 1;
+        if (valid && lane == leader) wc[w][dg] = wcount;
+        __syncthreads();
+        if (valid) {
+            int pre = cnt[dg];
+            for (int q = 0; q < w; q++) pre += wc[q][dg];
+            int pos = boff[dg] + pre + wrank;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        int add = 0;
+#pragma unroll
+        for (int q = 0; q < BLOCK / 64; q++) { add += wc[q][tid]; wc[q][tid] = 0; }
+        cnt[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------
+// exclusive scans (device-wide), 2048 elements per 256-thread block
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_scan_block(const T* in, T* out, long long n, T* sums) {
+    __shared__ T sh[BLOCK];
+    const int t = threadIdx.x;
+    long long base = (long long)blockIdx.x * RS_TILE + (long long)t * RS_ITEMS;
+    T v[RS_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        long long idx = base + k;
+        v[k] = idx < n ? in[idx] : (T)0;
+        s += v[k];
+    }
+    sh[t] = s;
+    __syncthreads();
+    for (int off = 1; off < BLOCK; off <<= 1) {
+        T a = t >= off ? sh[t - off] : (T)0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    T run = sh[t] - s;
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; k++) {
+        long long idx = base + k;
+        if (idx < n) out[idx] = run;
+        run += v[k];
+    }
+    if (t == BLOCK - 1) sums[blockIdx.x] = sh[t];
+}
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_scan_add(T* out, long long n, const T* offs) {
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) out[idx] += offs[idx / RS_TILE];
+}
+
+template <class T>
+static int exclusive_scan(sph_ctx* c, const T* in, T* out, long long n, T* total_out) {
+    long long nb = (n + RS_TILE - 1) / RS_TILE;
+    if (nb < 1) nb = 1;
+    T* sums = nullptr;
+    T* sums_scan = nullptr;
+    SPH_TRY(dalloc(c, &sums, nb + 1));
+    hipLaunchKernelGGL(k_scan_block<T>, dim3((unsigned)nb), dim3(BLOCK), 0, c->stream, in, out, n, sums);
+    SPH_TRY(launch_check(c, "scan_block"));
+    T total = 0;
+    if (nb > 1) {
+        SPH_TRY(dalloc(c, &sums_scan, nb + 1));
+        SPH_TRY(exclusive_scan<T>(c, sums, sums_scan, nb, &total));
+        hipLaunchKernelGGL(k_scan_add<T>, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, out, n, (const T*)sums_scan);
+        SPH_TRY(launch_check(c, "scan_add"));
+        dfree(c, sums_scan);
+    } else {
+        SPH_HIP(c, hipMemcpyAsync(&total, sums, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    dfree(c, sums);
+    if (total_out) *total_out = total;
+    return 0;
+}
+int exclusive_scan_i32(sph_ctx* c, const int* in, int* out, long long n, int* total) {
+    return exclusive_scan<int>(c, in, out, n, total);
+}
+int exclusive_scan_i64(sph_ctx* c, const long long* in, long long* out, long long n, long long* total) {
+    return exclusive_scan<long long>(c, in, out, n, total);
+}
+
+static int radix_sort(sph_ctx* c, int max_key) {
+    const int n = c->n;
+    int bits = 1;
+    while ((1LL << bits) <= (long long)max_key) bits++;
+    int passes = (bits + 7) / 8;
+    int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    int hist_n = 256 * ntiles;
+    int* hist = nullptr;
+    int* offs = nullptr;
+    SPH_TRY(dalloc(c, &hist, hist_n));
+    SPH_TRY(dalloc(c, &offs, hist_n));
+    int *kin = c->keys, *vin = c->perm, *kout = c->keys_tmp, *vout = c->perm_tmp;
+    for (int p = 0; p < passes; p++) {
+        int shift = 8 * p;
+        SPH_TRY(prof_begin(c, "radix_hist", 4.0 * n));
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(BLOCK), 0, c->stream, (const int*)kin, n, shift, ntiles, hist);
+        SPH_TRY(launch_check(c, "radix_hist"));
+        SPH_TRY(prof_end(c));
+        SPH_TRY(exclusive_scan_i32(c, hist, offs, hist_n, nullptr));
+        SPH_TRY(prof_begin(c, "radix_scatter", 16.0 * n));
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(BLOCK), 0, c->stream, (const int*)kin, (const int*)vin,
+                           kout, vout, n, shift, ntiles, (const int*)offs);
+        SPH_TRY(launch_check(c, "radix_scatter"));
+        SPH_TRY(prof_end(c));
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != c->keys) {   // odd pass count: result is in the tmp buffers
+        std::swap(c->keys, c->keys_tmp);
+        std::swap(c->perm, c->perm_tmp);
+    }
+    dfree(c, hist);
+    dfree(c, offs);
+    return 0;
+}
+
+// ------------------------------------------------------------------------
+// cell start / end, KERNEL_reorder functions_NNPS.cuh:825-838
+__global__ void __launch_bounds__(BLOCK) k_fill_i32(int* a, long long n, int v) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void __launch_bounds__(BLOCK) k_cell_bounds(int n, const int* keys, int* gstr, int* gend) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int h = keys[i];
+    if (i == 0 || h != keys[i - 1]) {
+        gstr[h] = i;
+        if (i > 0) gend[keys[i - 1]] = i;
+    }
+    if (i == n - 1) gend[h] = i + 1;
+}
+
+// gather one array through the permutation: out[i] = in[perm[i]]
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_gather(int n, const int* perm, const T* in, T* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[perm[i]];
+}
+
+// ------------------------------------------------------------------------
+// Neighbour list: the union of the dcell-query window and the binning
+// window of particle i (both +-ncell), scanned in increasing cell index so
+// entries come out in ascending j, each tagged with the flags of flags.h.
+template <bool FILL>
+__device__ __forceinline__ int nl_scan(int i, const Grid& g, const double* x, const double* y, const double* z,
+                                       const double* h, const int* ncell, double kappa, unsigned* out) {
+    const sph_domain& d = g.d;
+    double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+    int nc = ncell[i];
+    Cell q = query_cell(d, xi, yi, zi);
+    Cell b = bin_cell(d, xi, yi, zi);
+    int ilo = min(q.i, b.i) - nc, ihi = max(q.i, b.i) + nc;
+    int jlo = min(q.j, b.j) - nc, jhi = max(q.j, b.j) + nc;
+    int klo = min(q.k, b.k) - nc, khi = max(q.k, b.k) + nc;
+    const double khi_r = kappa * hi, khalf = kappa * (0.5 * hi);
+    int cnt = 0;
+    for (int ck = klo; ck <= khi; ck++)
+        for (int cj = jlo; cj <= jhi; cj++)
+            for (int ci = ilo; ci <= ihi; ci++) {
+                if (!cell_ok(d, ci, cj, ck)) continue;
+                bool inD = abs(ci - q.i) <= nc && abs(cj - q.j) <= nc && abs(ck - q.k) <= nc;
+                bool inB = abs(ci - b.i) <= nc && abs(cj - b.j) <= nc && abs(ck - b.k) <= nc;
+                if (!inD && !inB) continue;
+                int cell = ci + d.NI * cj + d.NI * d.NJ * ck;
+                int s = g.gstr[cell];
+                if (s == -1) continue;
+                int e = g.gend[cell];
+                for (int j = s; j < e; j++) {
+                    double r = dist3<double>(xi, yi, zi, x[j], y[j], z[j]);
+                    double kmax = kappa * fmax(hi, h[j]);
+                    double tp = r + 1e-20, tm = r - 1e-20;
+                    unsigned f = 0;
+                    if (inD) {
+                        if (tp < kmax) f |= F_DP;
+                        if (tm < kmax) f |= F_DM;
+                        if (tp < khi_r) f |= F_HP;
+                        if (tm < khi_r) f |= F_HM;
+                    }
+                    if (inB) {
+                        if (r > 0 && r < khi_r) f |= F_B;
+                        if (tp < khalf) f |= F_BH;
+                    }
+                    if (f) {
+                        if (FILL) out[(size_t)cnt * SLICE] = (unsigned)j | f;
+                        cnt++;
+                    }
+                }
+            }
+    return cnt;
+}
+__global__ void __launch_bounds__(BLOCK) k_nl_count(int n, Grid g, const double* x, const double* y, const double* z,
+                                                    const double* h, const int* ncell, double kappa, int* cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cnt[i] = nl_scan<false>(i, g, x, y, z, h, ncell, kappa, nullptr);
+}
+__global__ void __launch_bounds__(BLOCK) k_nl_fill(int n, Grid g, const double* x, const double* y, const double* z,
+                                                   const double* h, const int* ncell, double kappa,
+                                                   const long long* slice_off, unsigned* ent) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned* out = ent + slice_off[i >> 6] + (i & 63);
+    nl_scan<true>(i, g, x, y, z, h, ncell, kappa, out);
+}
+__global__ void __launch_bounds__(BLOCK) k_slice_width(int n, int nslices, const int* cnt, long long* w) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslices) return;
+    int m = 0;
+    for (int l = 0; l < SLICE; l++) {
+        int i = s * SLICE + l;
+        if (i < n) m = max(m, cnt[i]);
+    }
+    w[s] = (long long)m * SLICE;
+}
+
+// ------------------------------------------------------------------------
+static int permute_state(sph_ctx* c) {
+    const int n = c->n;
+    double* tmp = nullptr;
+    int* itmp = nullptr;
+    SPH_TRY(dalloc(c, &tmp, n));
+    SPH_TRY(dalloc(c, &itmp, n));
+    auto gd = [&](double*& a) -> int {
+        if (!a) return 0;
+        hipLaunchKernelGGL(k_gather<double>, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->perm,
+                           (const double*)a, tmp);
+        SPH_TRY(launch_check(c, "permute"));
+        std::swap(a, tmp);
+        return 0;
+    };
+    auto gi = [&](int*& a) -> int {
+        hipLaunchKernelGGL(k_gather<int>, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->perm,
+                           (const int*)a, itmp);
+        SPH_TRY(launch_check(c, "permute"));
+        std::swap(a, itmp);
+        return 0;
+    };
+    SPH_TRY(prof_begin(c, "permute", 0));
+    for (int f = 0; f < SPH_F_COUNT_F64 + 3; f++) SPH_TRY(gd(c->pf[f]));
+    for (int f = 0; f < SPH_I_COUNT; f++) SPH_TRY(gi(c->pi[f]));
+    for (int f = 0; f < 10; f++) SPH_TRY(gd(c->p2[f]));
+    for (int f = 15; f < 18; f++) SPH_TRY(gd(c->q[f]));
+    SPH_TRY(prof_end(c));
+    dfree(c, tmp);
+    dfree(c, itmp);
+    return 0;
+}
+
+static int build_neighbor_list(sph_ctx* c) {
+    const int n = c->n;
+    int nslices = (n + SLICE - 1) / SLICE;
+    dfree(c, c->nl_cnt); c->nl_cnt = nullptr;
+    dfree(c, c->nl_slice_off); c->nl_slice_off = nullptr;
+    dfree(c, c->nl_ent); c->nl_ent = nullptr;
+    SPH_TRY(dalloc(c, &c->nl_cnt, n));
+    SPH_TRY(dalloc(c, &c->nl_slice_off, nslices + 1));
+    long long* w = nullptr;
+    SPH_TRY(dalloc(c, &w, nslices + 1));
+    Grid g = c->grid();
+    const double *x = c->pf[SPH_F_X], *y = c->pf[SPH_F_Y], *z = c->pf[SPH_F_Z], *h = c->pf[SPH_F_H];
+    SPH_TRY(prof_begin(c, "nl_count", 0));
+    hipLaunchKernelGGL(k_nl_count, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
+                       (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, c->nl_cnt);
+    SPH_TRY(launch_check(c, "nl_count"));
+    SPH_TRY(prof_end(c));
+    hipLaunchKernelGGL(k_slice_width, dim3(grid1(nslices)), dim3(BLOCK), 0, c->stream, n, nslices,
+                       (const int*)c->nl_cnt, w);
+    SPH_TRY(launch_check(c, "slice_width"));
+    long long total = 0;
+    SPH_TRY(exclusive_scan_i64(c, w, c->nl_slice_off, nslices, &total));
+    dfree(c, w);
+    c->nl_total = total;
+    SPH_TRY(dalloc(c, &c->nl_ent, (size_t)total + 1));
+    SPH_TRY(prof_begin(c, "nl_fill", 0));
+    hipLaunchKernelGGL(k_nl_fill, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
+                       (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, (const long long*)c->nl_slice_off, c->nl_ent);
+    SPH_TRY(launch_check(c, "nl_fill"));
+    SPH_TRY(prof_end(c));
+    return 0;
+}
+
+// Full count==0 search on the current persistent state (any order).
+int nnps_run(sph_ctx* c) {
+    const int n = c->n;
+    const sph_domain& d = c->dom;
+    if (!c->keys) {
+        SPH_TRY(dalloc(c, &c->keys, n));
+        SPH_TRY(dalloc(c, &c->perm, n));
+        SPH_TRY(dalloc(c, &c->keys_tmp, n));
+        SPH_TRY(dalloc(c, &c->perm_tmp, n));
+    }
+    if (!c->gstr) {
+        SPH_TRY(dalloc(c, &c->gstr, (size_t)d.num_cells + 1));
+        SPH_TRY(dalloc(c, &c->gend, (size_t)d.num_cells + 1));
+    }
+    SPH_TRY(prof_begin(c, "hash", (3.0 * 8 + 4 + 8) * n));
+    hipLaunchKernelGGL(k_hash, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, d, (const int*)c->pi[SPH_I_I_TYPE],
+                       (const double*)c->pf[SPH_F_X], (const double*)c->pf[SPH_F_Y], (const double*)c->pf[SPH_F_Z],
+                       c->keys, c->perm);
+    SPH_TRY(launch_check(c, "hash"));
+    SPH_TRY(prof_end(c));
+    SPH_TRY(radix_sort(c, d.num_cells));
+    SPH_TRY(prof_begin(c, "cell_bounds", 4.0 * n + 8.0 * (d.num_cells + 1)));
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid1((long long)d.num_cells + 1)), dim3(BLOCK), 0, c->stream, c->gstr,
+                       (long long)d.num_cells + 1, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid1((long long)d.num_cells + 1)), dim3(BLOCK), 0, c->stream, c->gend,
+                       (long long)d.num_cells + 1, 0);
+    hipLaunchKernelGGL(k_cell_bounds, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->keys, c->gstr,
+                       c->gend);
+    SPH_TRY(launch_check(c, "cell_bounds"));
+    SPH_TRY(prof_end(c));
+    SPH_TRY(permute_state(c));
+    SPH_TRY(build_neighbor_list(c));
+    c->searched = true;
+    return 0;
+}
+
+int set_ncell_run(sph_ctx* c) {
+    const int n = c->n;
+    hipLaunchKernelGGL(k_set_ncell, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->pi[SPH_I_I_TYPE],
+                       (const double*)c->pf[SPH_F_H], (const double*)c->pf[SPH_F_COUNT_F64 + 2],
+                       c->pi[SPH_I_NCELL]);
+    return launch_check(c, "set_ncell");
+}
+
+}  // namespace sph

@@ -1,0 +1,1006 @@
+// Per-step physics passes of the ISPH projection step on MI355X.
+//
+// One thread per particle over structure-of-arrays state.  Neighbour passes
+// walk the per-particle neighbour list built once by the count==0 search
+// (the reference rescans the 5^3 cell window every pass; in its Eulerian mode
+// positions never move, ISPH_Calc.cuh:23-63).  Each list entry carries
+// precomputed FP64 window/radius flags, so a pass selects exactly the
+// reference's neighbour set for that kernel in both precision modes.
+// Passes whose search is centred on a ghost node (Neumann inlet mirror,
+// periodic copy, outlet copy, MOST deposit) scan the cell window directly.
+//
+// R is the pair-math type (double: FP64 mode, float: mixed mode).  Sums are
+// always accumulated in double.  Differences of positions and velocities are
+// taken in double and then rounded to R, so the float path loses no
+// significance to cancellation.
+#include "flags.h"
+#include "sph_internal.h"
+#include "sph_math.h"
+
+namespace sph {
+
+struct NLRow { const unsigned* p; int cnt; };
+__device__ __forceinline__ NLRow nl_row(const NList& nl, int i) {
+    return {nl.ent + nl.slice_off[i >> 6] + (i & 63), nl.cnt[i]};
+}
+#define NB_AT(row, k) ((row).p[(size_t)(k) * SLICE])
+
+// ------------------------------------------------------------------------
+// Step start: SP1 <- P1 and SP2 <- P2 for the mutable fields, P3 <- 0
+// (ISPH_Calc.cuh:53-63).
+struct CopySet {
+    const double* src[MF_COUNT + 10];
+    double* dst[MF_COUNT + 10];
+    double* zero[15];
+};
+__global__ void __launch_bounds__(BLOCK) k_step_begin(int n, CopySet cs) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int f = 0; f < MF_COUNT + 10; f++) cs.dst[f][i] = cs.src[f][i];
+#pragma unroll
+    for (int f = 0; f < 15; f++) cs.zero[f][i] = 0.0;
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_clc_gradient_correction_3D, functions_KNL.cuh:216-324
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_grad_corr(int n, KConst kc, PView S, QView Q, NList nl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ptype_i = S.p_type[i];
+    R hi = (R)S.h[i];
+    R tmp_A = calc_tmpA<R>(kc, hi);
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    double tmpxx = 0, tmpyy = 0, tmpzz = 0, tmpxy = 0, tmpyz = 0, tmpzx = 0;
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_B)) continue;
+        int j = e & NB_IDX;
+        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz);
+        R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
+        R mj = (R)S.m[j], rhoj = (R)S.rho[j];
+        R mtd = mj * tdwij;
+        R rtd = (R)1.0 / (rhoj * tdist);
+        R txx = -mtd * dx * dx; txx *= rtd;
+        R txy = -mtd * dy * dx; txy *= rtd;
+        R tyy = -mtd * dy * dy; tyy *= rtd;
+        R tzx = -mtd * dx * dz; tzx *= rtd;
+        R tyz = -mtd * dy * dz; tyz *= rtd;
+        R tzz = -mtd * dz * dz; tzz *= rtd;
+        R f = (R)(S.p_type[j] == 1);
+        tmpxx += (double)(txx * f); tmpxy += (double)(txy * f); tmpyy += (double)(tyy * f);
+        tmpzx += (double)(tzx * f); tmpyz += (double)(tyz * f); tmpzz += (double)(tzz * f);
+    }
+    double tmpcmd = tmpxx * (tmpyy * tmpzz - tmpyz * tmpyz);
+    tmpcmd -= tmpxy * (tmpxy * tmpzz - tmpyz * tmpzx);
+    tmpcmd += tmpzx * (tmpxy * tmpyz - tmpyy * tmpzx);
+    if ((fabs(tmpcmd) > 2.2204460492503131e-16) && (ptype_i == 1)) {
+        double rtcmd = 1.0 / tmpcmd;
+        Q.inv_xx[i] = (tmpyy * tmpzz - tmpyz * tmpyz) * rtcmd;
+        Q.inv_xy[i] = (tmpzx * tmpyz - tmpxy * tmpzz) * rtcmd;
+        Q.inv_zx[i] = (tmpxy * tmpyz - tmpzx * tmpyy) * rtcmd;
+        Q.inv_yy[i] = (tmpxx * tmpzz - tmpzx * tmpzx) * rtcmd;
+        Q.inv_yz[i] = (tmpzx * tmpxy - tmpxx * tmpyz) * rtcmd;
+        Q.inv_zz[i] = (tmpxx * tmpyy - tmpxy * tmpxy) * rtcmd;
+    } else {
+        Q.inv_xx[i] = 1.0; Q.inv_xy[i] = 0.0; Q.inv_zx[i] = 0.0;
+        Q.inv_yy[i] = 1.0; Q.inv_yz[i] = 0.0; Q.inv_zz[i] = 1.0;
+    }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_clc_prep3D, functions_PREP.cuh:287-488
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_prep(int n, KConst kc, sph_params pr, PView S, P2View S2, QView Q,
+                                                NList nl, int count, double dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.i_type[i] > 2) return;
+    if (S.p_type[i] >= 1000) return;
+    if (S.p_type[i] <= 0) return;
+    double ki = fmax(S.k_turb[i], 0.0);
+    double nui = S.vis_t[i];
+    R gc11 = (R)Q.inv_xx[i], gc12 = (R)Q.inv_xy[i], gc22 = (R)Q.inv_yy[i];
+    R gc23 = (R)Q.inv_yz[i], gc31 = (R)Q.inv_zx[i], gc33 = (R)Q.inv_zz[i];
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    double uxi = S.ux[i], uyi = S.uy[i], uzi = S.uz[i];
+    double hi_d = S.h[i];
+    R hi = (R)hi_d;
+    R tmp_A = calc_tmpA<R>(kc, hi);
+    double tmp_flt = 0.0, tmp_sigma = 0.0;
+    double st11 = 0, st12 = 0, st22 = 0, st23 = 0, st31 = 0, st33 = 0;
+    const bool filt = (count % pr.freq_filt) == 0;
+    const bool les = (pr.fv_solve != 0) && (pr.turbulence_model != SPH_LAMINAR);
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_HP)) continue;
+        int j = e & NB_IDX;
+        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        R mj = (R)S.m[j], rhoj = (R)S.rho[j];
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
+        R tmp_wij = kernel_w<R>(kc, tmp_A, hi, tdist);
+        R tmp_dwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
+        R tdwx = tmp_dwij * dx / tdist;
+        R tdwy = tmp_dwij * dy / tdist;
+        R tdwz = tmp_dwij * dz / tdist;
+        if (filt) tmp_flt += (double)(mj / rhoj * tmp_wij);
+        if (les) {
+            tmp_sigma += (double)(tdist * tdist * tmp_wij * mj / rhoj);
+            if (S.p_type[j] == 1) {
+                R tdwxc = gc11 * tdwx + gc12 * tdwy + gc31 * tdwz;
+                R tdwyc = gc12 * tdwx + gc22 * tdwy + gc23 * tdwz;
+                R tdwzc = gc31 * tdwx + gc23 * tdwy + gc33 * tdwz;
+                R dux = (R)(S.ux[j] - uxi), duy = (R)(S.uy[j] - uyi), duz = (R)(S.uz[j] - uzi);
+                st11 += (double)(mj / rhoj * dux * tdwxc);
+                st12 += (double)((R)0.5 * mj / rhoj * (dux * tdwyc + duy * tdwxc));
+                st22 += (double)(mj / rhoj * duy * tdwyc);
+                st23 += (double)((R)0.5 * mj / rhoj * (duy * tdwzc + duz * tdwyc));
+                st31 += (double)((R)0.5 * mj / rhoj * (dux * tdwzc + duz * tdwxc));
+                st33 += (double)(mj / rhoj * duz * tdwzc);
+            }
+        }
+    }
+    if (les) {
+        double tmp_SR = 2 * (st11 * st11 + st22 * st22 + st33 * st33 + 2 * st12 * st12 + 2 * st31 * st31 + 2 * st23 * st23);
+        tmp_SR = sqrt(tmp_SR);
+        S2.SR[i] = tmp_SR;
+        tmp_sigma = 0.3333 * tmp_sigma;
+        tmp_sigma = sqrt(tmp_sigma);
+        double tmp_cl = tmp_sigma > hi_d ? tmp_sigma : hi_d;
+        if (pr.turbulence_model == SPH_SSM) {
+            double tvis_t = pr.Cs_sgs * pr.Cs_sgs * tmp_cl * tmp_cl * tmp_SR;
+            if (S.buffer_type[i] != 1) S.vis_t[i] = tvis_t;
+        } else if (pr.turbulence_model == SPH_DDF) {
+            double p = nui * tmp_SR * tmp_SR;
+            double b = 0.0;
+            double eps = pr.Ce_sgs * ki * sqrt(ki) / tmp_cl;
+            double dk = p + b - eps + 0.0;
+            ki += dk * dt;
+            ki = ki > 1e-4 ? ki : 1e-4;
+            double tvis_t = pr.Ck_sgs * tmp_cl * sqrt(ki);
+            if (S.buffer_type[i] != 1) { S.vis_t[i] = tvis_t; S.k_turb[i] = ki; S.e_turb[i] = eps; }
+        }
+    }
+    if (count == 0) S2.rho_ref[i] = S.rho[i];
+    if (filt) S.flt_s[i] = tmp_flt;
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_boundary3D, functions_BC.cuh:100-190 (wall velocity)
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_boundary3d(int n, KConst kc, sph_params pr, PView S, NList nl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.i_type[i] > 2) return;
+    int p_type_i = S.p_type[i];
+    if (p_type_i > 0) return;
+    R hi = (R)S.h[i];
+    R tmp_A = calc_tmpA<R>(kc, hi);
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    double uxi = S.ux[i], uyi = S.uy[i], uzi = S.uz[i];
+    double tmpx = 0.0, tmpy = 0.0, tmpz = 0.0, flt = 1.0e-10;
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_HM)) continue;
+        int j = e & NB_IDX;
+        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+        R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
+        int p_type_j = S.p_type[j];
+        if (pr.noslip_bc == 1) {
+            if ((p_type_j > 0) && (p_type_j != 9)) {
+                tmpx += (double)((R)S.ux[j] * twij);
+                tmpy += (double)((R)S.uy[j] * twij);
+                tmpz += (double)((R)S.uz[j] * twij);
+                flt += (double)twij;
+            }
+        }
+    }
+    if (pr.noslip_bc == 1 && (p_type_i == 0 || p_type_i == 9)) {
+        S.ux[i] = 2 * uxi * (p_type_i == 9) - tmpx / flt;
+        S.uy[i] = 2 * uyi * (p_type_i == 9) - tmpy / flt;
+        S.uz[i] = 2 * uzi * (p_type_i == 9) - tmpz / flt;
+    }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_MOST_boundary3D, functions_MOST.cuh:175-322.  Part 1: the
+// surface-layer fluxes of each MOST particle; the neighbour heat deposit is
+// stored per particle for the gather in part 2.
+__global__ void __launch_bounds__(BLOCK) k_most_self(int n, sph_params pr, PView S, double* dep, int count, double dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.MOST_buffer[i] != 1) { dep[i] = 0.0; return; }
+    const double KVK = 0.4;
+    double zi = S.z[i];
+    double uxi = S.ux[i], uyi = S.uy[i];
+    double tempi = S.temp[i], rhoi = S.rho[i], mi = S.m[i];
+    double space = S.h[i] / 1.5;
+    double z0 = pr.most_z0, z0h = pr.most_z0h;
+    double u_star = 0.0, theta_star = 0.0, OB_length;
+    double theta_surface = wall_temperature(pr, dt, count);
+    double wind_speed = sqrt(uxi * uxi + uyi * uyi);
+    wind_speed = fmax(0.001, fmin(wind_speed, 100.0));
+    if (pr.most_mode == SPH_MOST_PRESCRIBED_L) {
+        OB_length = pr.most_L;
+        u_star = KVK * wind_speed / Phi_m(OB_length, zi, z0);
+        theta_star = KVK * (tempi - theta_surface) / Phi_h(OB_length, zi, z0h);
+    } else {
+        double Ri_b = (pr.gravity / tempi) * (tempi - theta_surface) * zi / (wind_speed * wind_speed + 1e-10);
+        if (fabs(Ri_b) > 1e-3) {
+            OB_length = zi / Ri_b;
+            OB_length = fmax(-500.0, fmin(OB_length, 500.0));
+            for (int iter = 0; iter < 200; iter++) {
+                double ol_m = OB_length, ol_l = 0.999 * ol_m, ol_u = 1.001 * ol_m;
+                double pm = Phi_m(ol_m, zi, z0);
+                double fL = Ri_b - (zi / ol_m) * Phi_h(ol_m, zi, z0h) / (pm * pm);
+                double dfL = ((-(zi / ol_u) * Phi_h(ol_u, zi, z0h) / (Phi_m(ol_u, zi, z0) * Phi_m(ol_u, zi, z0))) -
+                              (-(zi / ol_l) * Phi_h(ol_l, zi, z0h) / (Phi_m(ol_l, zi, z0) * Phi_m(ol_l, zi, z0)))) /
+                             (ol_u - ol_l);
+                OB_length -= 1.0 * fL / dfL;
+                if (fabs(OB_length) > 1e-6) {
+                    if (fabs((OB_length - ol_m) / OB_length) < 1e-5) break;
+                    else if (fabs(OB_length - ol_m) < 1e-5) break;
+                }
+            }
+            u_star = KVK * wind_speed / Phi_m(OB_length, zi, z0);
+            theta_star = KVK * (tempi - theta_surface) / Phi_h(OB_length, zi, z0h);
+        } else {
+            OB_length = 1e6;
+            u_star = KVK * wind_speed / log(zi / z0);
+            theta_star = KVK * (tempi - theta_surface) / log(zi / z0h);
+        }
+    }
+    double momentum_x = -rhoi * u_star * (uxi * u_star / wind_speed);
+    double momentum_y = -rhoi * u_star * (uyi * u_star / wind_speed);
+    double shf = -rhoi * pr.air_cp * u_star * theta_star * 100.0;
+    dep[i] = 0.5 * shf / (space * rhoi * pr.air_cp);
+    S.fmx[i] = 1.0 * momentum_x * space * space / mi;
+    S.fmy[i] = 1.0 * momentum_y * space * space / mi;
+    S.float1[i] = u_star; S.float2[i] = theta_star; S.float3[i] = OB_length;
+}
+
+// Part 2: every particle j gathers the deposits of the MOST particles i whose
+// ghost node (x_i, y_i, z_i + dz) lies within ghost_r of j and whose 3x3x3
+// ghost window holds j's cell, in ascending i, with i's own deposit applied
+// after its neighbour deposits (functions_MOST.cuh:286-316 serialised).
+__global__ void __launch_bounds__(BLOCK) k_most_deposit(int n, sph_params pr, PView S, QView Q, Grid g,
+                                                        const double* dep, const int* keys) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const sph_domain& d = g.d;
+    const bool self_most = S.MOST_buffer[j] == 1;
+    double xj = S.x[j], yj = S.y[j], zj = S.z[j];
+    double acc = 0.0, last = 0.0;
+    bool any = false, self_done = !self_most;
+    int key_j = keys[j];
+    int cj_i = key_j % d.NI, cj_j = (key_j / d.NI) % d.NJ, cj_k = key_j / (d.NI * d.NJ);
+    double r = pr.most_ghost_r;
+    double px = xj, py = yj, pz = zj - pr.most_ghost_dz;
+    double mg = 1e-6 + 1e-6 * r;
+    bool box_ok = !(pz + r + mg < (double)d.z_min || pz - r - mg > (double)d.z_max ||
+                    px + r + mg < (double)d.x_min || px - r - mg > (double)d.x_max ||
+                    py + r + mg < (double)d.y_min || py - r - mg > (double)d.y_max);
+    if (box_ok && key_j < d.num_cells - 1) {
+        int i0 = bin_axis((float)(px - r - mg), d.x_min, d.x_max, d.NI), i1 = bin_axis((float)(px + r + mg), d.x_min, d.x_max, d.NI);
+        int j0 = bin_axis((float)(py - r - mg), d.y_min, d.y_max, d.NJ), j1 = bin_axis((float)(py + r + mg), d.y_min, d.y_max, d.NJ);
+        int k0 = bin_axis((float)(pz - r - mg), d.z_min, d.z_max, d.NK), k1 = bin_axis((float)(pz + r + mg), d.z_min, d.z_max, d.NK);
+        for (int ck = k0; ck <= k1; ck++)
+            for (int cj = j0; cj <= j1; cj++)
+                for (int ci = i0; ci <= i1; ci++) {
+                    int cell = ci + d.NI * cj + d.NI * d.NJ * ck;
+                    int s = g.gstr[cell];
+                    if (s == -1) continue;
+                    int e = g.gend[cell];
+                    for (int i = s; i < e; i++) {
+                        if (!self_done && i > j) { acc += dep[j]; last = dep[j]; any = true; self_done = true; }
+                        if (S.MOST_buffer[i] != 1) continue;
+                        double xg = S.x[i], yg = S.y[i], zg = S.z[i] + pr.most_ghost_dz;
+                        Cell c = query_cell(d, xg, yg, zg);
+                        bool inwin = abs(cj_i - c.i) <= 1 && abs(cj_j - c.j) <= 1 && abs(cj_k - c.k) <= 1;
+                        double tdist = dist3<double>(xg, yg, zg, xj, yj, zj) - 1e-20;
+                        if (inwin && tdist < pr.most_ghost_r) { acc += dep[i]; last = dep[i]; any = true; }
+                        if (i == j) { acc += dep[j]; last = dep[j]; any = true; self_done = true; }
+                    }
+                }
+    }
+    if (!self_done) { acc += dep[j]; last = dep[j]; any = true; }
+    if (any) { Q.dtemp[j] += acc; S.hmz[j] = last; }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_Neumann_boundary3D, functions_BC.cuh:290-446.  Pressures are read
+// from a snapshot (pold) taken before the launch.
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_neumann(int n, KConst kc, sph_params pr, PView A, Grid g,
+                                                   const double* pold) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (A.p_type[i] == 1) return;
+    const sph_domain& d = g.d;
+    R hi = (R)(2.0 * A.h[i]);
+    R tmp_A = calc_tmpA<R>(kc, hi);
+    R search_range = (R)kc.kappa * hi;
+    double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    double grad_px = 0, grad_py = 0, grad_pz = 0, tpres = 0, tflt_s = 0, pres_g = 0;
+    double tux = 0, tuy = 0, tuz = 0;
+    int bt = A.buffer_type[i];
+    double xg = (bt == 1 || bt == -1) ? 2.0 * pr.L2 - xi : xi;
+    double yg = yi, zg = zi;
+    Cell c = query_cell(d, xg, yg, zg);
+    for (int z = -1; z <= 1; z++)
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
+                int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
+                int s = g.gstr[cell];
+                if (s == -1) continue;
+                int e = g.gend[cell];
+                for (int j = s; j < e; j++) {
+                    R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+                    R dist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+                    if (dist < (R)pr.neumann_ghost_r) pres_g = pold[j];
+                }
+            }
+    int nc = A.ncell[i];
+    for (int z = -nc; z <= nc; z++)
+        for (int y = -nc; y <= nc; y++)
+            for (int x = -nc; x <= nc; x++) {
+                if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
+                int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
+                int s = g.gstr[cell];
+                if (s == -1) continue;
+                int e = g.gend[cell];
+                for (int j = s; j < e; j++) {
+                    if (A.p_type[j] >= 1000) continue;
+                    R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+                    R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+                    if (!(tdist < search_range)) continue;
+                    R mj = (R)A.m[j], rhoj = (R)A.rho[j];
+                    double presj = pold[j];
+                    R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
+                    R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
+                    R tdwx = dx / tdist * tdwij, tdwy = dy / tdist * tdwij, tdwz = dz / tdist * tdwij;
+                    R neighbor = (R)((A.p_type[j] == 1) * (A.buffer_type[j] == 0));
+                    tpres += (double)(mj / rhoj * (R)presj * twij * neighbor);
+                    tux += (double)(mj / rhoj * (R)A.ux[j] * twij * neighbor);
+                    tuy += (double)(mj / rhoj * (R)A.uy[j] * twij * neighbor);
+                    tuz += (double)(mj / rhoj * (R)A.uz[j] * twij * neighbor);
+                    tflt_s += (double)(mj / rhoj * twij * neighbor);
+                    R dp = (R)(presj - pres_g);
+                    grad_px += (double)(mj / rhoj * dp * tdwx * neighbor);
+                    grad_py += (double)(mj / rhoj * dp * tdwy * neighbor);
+                    grad_pz += (double)(mj / rhoj * dp * tdwz * neighbor);
+                }
+            }
+    if (tflt_s < 1e-6) {
+        A.pres[i] = 0.0;
+    } else {
+        A.pres[i] = tpres / tflt_s;
+        if (A.p_type[i] == -1) { A.ux[i] = tux / tflt_s; A.uy[i] = tuy / tflt_s; A.uz[i] = tuz / tflt_s; }
+        if (bt == 1 || bt == -1) A.pres[i] = pres_g + (xi - xg) * grad_px + (yi - yg) * grad_py + (zi - zg) * grad_pz;
+    }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_advection_force3D, functions_PPE.cuh:191-480
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_advection(int n, KConst kc, sph_params pr, PView S, QView Q, NList nl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.i_type[i] > 2) return;
+    int ptypei = S.p_type[i];
+    if (ptypei == 1000) return;
+    if (ptypei <= 0) return;
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    double uxi = S.ux[i], uyi = S.uy[i], uzi = S.uz[i];
+    double tempi = S.temp[i];
+    R hi = (R)S.h[i], mi = (R)S.m[i], rhoi = (R)S.rho[i];
+    double hi_d = S.h[i];
+    double eli = S.eli[i];
+    R tmp_A = calc_tmpA<R>(kc, hi);
+    R kci = 0, cpi = 1, concni = 0, diffi = 0;
+    if (pr.con_solve) {
+        kci = (R)(pr.solid_k * (ptypei == 0 || ptypei == 1000) + pr.air_k * (ptypei == 1 || ptypei == -1));
+        cpi = (R)(pr.solid_cp * (ptypei == 0 || ptypei == 1000) + pr.air_cp * (ptypei == 1 || ptypei == -1));
+    }
+    if (pr.concn_solve) { concni = (R)S.concn[i]; diffi = (R)pr.diff_coef; }
+    R visi = (R)(viscosity2(pr, ptypei) + S.vis_t[i] * S.rho[i]);
+    double betai = 1.0 / pr.T_ini;
+    double tmpx = 0, tmpy = 0, tmpz = 0, tmp_Rc = 0, tmp_Rd = 0;
+    double eulerx = 0, eulery = 0, eulerz = 0, eulert = 0, tmp_temp = 0, tmp_flt = 0;
+    const R Alpha = (R)pr.av_alpha, Beta = (R)pr.av_beta, soundspeed = (R)pr.av_c;
+    const R ome = (R)(1 - eli);
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_DP)) continue;
+        int j = e & NB_IDX;
+        int ptypej = S.p_type[j];
+        if (ptypej >= 1000) continue;
+        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
+        R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
+        R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
+        R tdwx = tdwij * dx / tdist;
+        R tdwy = tdwij * dy / tdist;
+        R tdwz = tdwij * dz / tdist;
+        R mj = (R)S.m[j], rhoj = (R)S.rho[j];
+        double hj_d = S.h[j];
+        R dux = (R)(uxi - S.ux[j]), duy = (R)(uyi - S.uy[j]), duz = (R)(uzi - S.uz[j]);   // u_i - u_j
+        if (pr.fv_solve == SPH_FV_CLEARY) {
+            R visj = (R)(viscosity2(pr, ptypej) + S.vis_t[j] * S.rho[j]);
+            R C_v = dx * tdwx + dy * tdwy + dz * tdwz;
+            C_v *= (visi * visj) / (visi + visj);
+            C_v *= (R)4 * (mj / (rhoi * rhoj));
+            C_v /= tdist;
+            C_v /= tdist;
+            tmpx += (double)(C_v * dux); tmpy += (double)(C_v * duy); tmpz += (double)(C_v * duz);
+        } else if (pr.fv_solve == SPH_FV_MONAGHAN) {
+            R uij_xij = dux * dx + duy * dy + duz * dz;
+            R visj = (R)(viscosity2(pr, ptypej) + S.vis_t[j] * S.rho[j]);
+            R phi_ij = uij_xij;
+            phi_ij /= (tdist * tdist + (R)0.01 * hi * hi);
+            R P_ij = (R)(-2.0) * (R)(3 + 2.0) * phi_ij * mj / rhoj;
+            R C_v = (R)2.0 * (visi * visj) / (visi + visj - (R)1e-20);
+            R nui = C_v / rhoi;
+            P_ij *= nui;
+            if (ptypei <= 0 || ptypei >= 1000) P_ij = 0;
+            if (ptypej <= 0 || ptypei >= 1000) P_ij = 0;
+            tmpx += (double)(-(P_ij)*tdwx); tmpy += (double)(-(P_ij)*tdwy); tmpz += (double)(-(P_ij)*tdwz);
+        }
+        if (eli < 1.0) {
+            R Rux = (R)uxi, Ruy = (R)uyi, Ruz = (R)uzi;
+            R vx = -dux, vy = -duy, vz = -duz;   // u_j - u_i
+            eulerx += (double)((Rux * vx * tdwx + Ruy * vx * tdwy + Ruz * vx * tdwz) * mj / rhoj * ome);
+            eulery += (double)((Rux * vy * tdwx + Ruy * vy * tdwy + Ruz * vy * tdwz) * mj / rhoj * ome);
+            eulerz += (double)((Rux * vz * tdwx + Ruy * vz * tdwy + Ruz * vz * tdwz) * mj / rhoj * ome);
+            if (pr.con_solve)
+                eulert -= (double)((R)2.0 * mj / (rhoi + rhoj) * (R)(S.temp[j] - tempi) * (dux * tdwx + duy * tdwy + duz * tdwz) * ome);
+            if (pr.concn_solve) {
+                R dc = (R)(S.concn[j] - (double)concni);
+                tmp_Rd -= (double)((Rux * dc * tdwx + Ruy * dc * tdwy + Ruz * dc * tdwz) * mj / rhoj * ome);
+            }
+        }
+        if (pr.con_solve) {
+            R kcj = (R)(pr.solid_k * (ptypej == 0 || ptypej == 1000) + pr.air_k * (ptypej == 1 || ptypej == -1));
+            R rhoj_temp = (ptypej == 1 || ptypej == -1) ? rhoj : (R)pr.solid_rho;
+            R mj_temp = mj * rhoj_temp / rhoj;
+            R sum_con_H = (R)4.0 * mj_temp * kcj * kci * (R)(tempi - S.temp[j]) * tdwij;
+            sum_con_H /= (tdist + (R)1e-10) * rhoi * rhoj_temp * (kci + kcj);
+            tmp_Rc += (double)sum_con_H;
+        }
+        if (pr.concn_solve) {
+            R concnj = (R)S.concn[j];
+            R diffj = (R)pr.diff_coef;
+            R tmprd = mi * (diffi * rhoi + diffj * rhoj);
+            tmprd *= (dx * tdwx + dy * tdwy + dz * tdwz);
+            tmprd /= (rhoi * rhoj * (tdist * tdist + (R)0.01 * hi * hi));
+            R tmprd1 = tmprd * (concni - concnj) * (R)(ptypei == ptypej);
+            tmp_Rd += (double)tmprd1;
+        }
+        if (pr.boussinesq_solve) {
+            tmp_temp += (double)((mj / rhoj) * (R)S.temp[j] * twij);
+            tmp_flt += (double)((mj / rhoj) * twij);
+        }
+        if (pr.artificial_viscosity) {
+            R uij_xij = dux * dx + duy * dy + duz * dz;
+            if (uij_xij < 0) {
+                R h_ij = (R)((hi_d + hj_d) * 0.5);
+                R phi_ij = h_ij * uij_xij;
+                phi_ij /= (tdist * tdist + (R)0.01 * h_ij * h_ij);
+                R P_ij = mi * phi_ij * (-Alpha * soundspeed + Beta * phi_ij);
+                P_ij /= (rhoi + rhoj);
+                P_ij *= (R)0.5;
+                tmpx += (double)(-(P_ij)*tdwx); tmpy += (double)(-(P_ij)*tdwy); tmpz += (double)(-(P_ij)*tdwz);
+            }
+        }
+    }
+    if (pr.fg_solve) tmpz += -pr.gravity;
+    if ((pr.boussinesq_solve) && (ptypei > 0) && (ptypei < 1000))
+        tmpz = betai * pr.gravity * (tmp_temp / tmp_flt - pr.tempb);
+    Q.ftotalx[i] = tmpx - eulerx;
+    Q.ftotaly[i] = tmpy - eulery;
+    Q.ftotalz[i] = tmpz - eulerz;
+    if (pr.con_solve) Q.dtemp[i] += tmp_Rc / (double)cpi - eulert;
+    Q.ftotal[i] = sqrt(tmpx * tmpx + tmpy * tmpy + tmpz * tmpz);
+    if (pr.concn_solve) Q.dconcn[i] = tmp_Rd;
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_clc_projection, functions_TIME_ISPH.cuh:2-112 (predictor)
+__global__ void __launch_bounds__(BLOCK) k_projection(int n, sph_params pr, PView S, P2View S2, QView Q, double dt,
+                                                      double time) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.i_type[i] >= 3) return;
+    if (S.p_type[i] >= 1000) return;
+    int buffer_type = S.buffer_type[i], MOST = S.MOST_buffer[i], p_typei = S.p_type[i];
+    if (p_typei == 9) {
+        S2.x0[i] = S.x[i]; S2.y0[i] = S.y[i]; S2.z0[i] = S.z[i];
+        S.ux[i] = 1.0; S.uy[i] = 0; S.uz[i] = 0;
+        S2.ux0[i] = 1.0; S2.uy0[i] = 0; S2.uz0[i] = 0;
+    } else {
+        double tux0 = S.ux[i], tuy0 = S.uy[i], tuz0 = S.uz[i];
+        double tuxp, tuyp, tuzp;
+        if (p_typei > 0) {
+            double b = (double)(buffer_type == 0 || MOST == 1);
+            double tdux = Q.ftotalx[i] * b + S.fmx[i];
+            double tduy = Q.ftotaly[i] * b + S.fmy[i];
+            double tduz = Q.ftotalz[i] * b;
+            tuxp = tux0 + tdux * (dt);
+            tuyp = tuy0 + tduy * (dt);
+            tuzp = tuz0 + tduz * (dt);
+        } else {
+            tuxp = tux0; tuyp = tuy0; tuzp = tuz0;
+        }
+        S.ux[i] = tuxp; S.uy[i] = tuyp; S.uz[i] = tuzp;
+        S2.x0[i] = S.x[i]; S2.y0[i] = S.y[i]; S2.z0[i] = S.z[i];
+        S2.ux0[i] = tux0; S2.uy0[i] = tuy0; S2.uz0[i] = tuz0;
+    }
+    if (time == 0) S2.rho_ref[i] = S.rho[i];
+    if (pr.con_solve == 1) S2.temp0[i] = S.temp[i];
+    if (pr.concn_solve == 1) {
+        double tconcn = S.concn[i];
+        S2.concn0[i] = tconcn;
+        S.concn[i] = tconcn + Q.dconcn[i] * (dt);
+    }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_periodic_boundary3D, functions_BC.cuh:1015-1154 (reads the snapshot
+// arrays; last match inside the first cell that has one).
+__global__ void __launch_bounds__(BLOCK) k_periodic(int n, sph_params pr, PView A, Grid g, const double* oux,
+                                                    const double* ouy, const double* ouz, const double* opres,
+                                                    const double* otemp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int bt = A.buffer_type[i];
+    if (bt == 0) return;
+    if (A.p_type[i] >= 1000) return;
+    double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    double xg, yg, zg;
+    if (bt == 3) { xg = xi; yg = yi + pr.L_right - pr.L_left; zg = zi; }
+    else if (bt == 4) { xg = xi; yg = yi - pr.L_right + pr.L_left; zg = zi; }
+    else return;
+    const sph_domain& d = g.d;
+    int nc = A.ncell[i];
+    Cell c = query_cell(d, xg, yg, zg);
+    int found = -1;
+    for (int z = -nc; z <= nc && found < 0; z++)
+        for (int y = -nc; y <= nc && found < 0; y++)
+            for (int x = -nc; x <= nc && found < 0; x++) {
+                if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
+                int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
+                int s = g.gstr[cell];
+                if (s == -1) continue;
+                int e = g.gend[cell];
+                for (int j = s; j < e; j++) {
+                    double tdist = dist3<double>(xg, yg, zg, A.x[j], A.y[j], A.z[j]) - 1e-20;
+                    if (tdist < pr.copy_tol) found = j;
+                }
+            }
+    if (found >= 0) {
+        A.ux[i] = oux[found]; A.uy[i] = ouy[found]; A.uz[i] = ouz[found];
+        A.pres[i] = opres[found]; A.temp[i] = otemp[found];
+    } else {
+        A.ux[i] = 0; A.uy[i] = 0; A.uz[i] = 0; A.pres[i] = 0; A.temp[i] = 0;
+    }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_freeslip_boundary3D, functions_BC.cuh:1156-1271
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_freeslip(int n, KConst kc, PView A, NList nl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (A.p_type[i] != -1) return;
+    R hi = (R)A.h[i];
+    R tmp_A = calc_tmpA<R>(kc, hi);
+    double xg = A.x[i], yg = A.y[i], zg = A.z[i];
+    double tux = 0, tuy = 0, tuz = 0, tpres = 0, ttemp = 0, tflt_s = 0;
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_HM)) continue;
+        int j = e & NB_IDX;
+        R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+        R mj = (R)A.m[j], rhoj = (R)A.rho[j];
+        R neighbor = (R)(A.p_type[j] == 1);
+        R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
+        tux += (double)((R)A.ux[j] * (mj / rhoj) * twij * neighbor);
+        tuy += (double)((R)A.uy[j] * (mj / rhoj) * twij * neighbor);
+        tuz += (double)((R)A.uz[j] * (mj / rhoj) * twij * neighbor);
+        tpres += (double)((R)A.pres[j] * (mj / rhoj) * twij * neighbor);
+        ttemp += (double)((R)A.temp[j] * (mj / rhoj) * twij * neighbor);
+        tflt_s += (double)((mj / rhoj) * twij * neighbor);
+    }
+    if (tflt_s > 1e-5) {
+        A.ux[i] = tux / tflt_s; A.uy[i] = tuy / tflt_s; A.uz[i] = tuz / tflt_s;
+        A.pres[i] = tpres / tflt_s; A.temp[i] = ttemp / tflt_s;
+    }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_donothing_buffer3D, functions_BC.cuh:673-761 (snapshot reads)
+__global__ void __launch_bounds__(BLOCK) k_outlet(int n, sph_params pr, PView A, Grid g, const double* oux,
+                                                  const double* ouy, const double* ouz, const double* opres,
+                                                  const double* otemp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (A.buffer_type[i] != 2) return;
+    const sph_domain& d = g.d;
+    double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    Cell c = query_cell(d, xi, yi, zi);
+    int nc = 2 * A.ncell[i];
+    double xg = pr.L3 - pr.outlet_copy_dx;
+    int found = -1;
+    for (int z = -nc; z <= nc; z++)
+        for (int y = -nc; y <= nc; y++)
+            for (int x = -nc; x <= nc; x++) {
+                if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
+                int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
+                int s = g.gstr[cell];
+                if (s == -1) continue;
+                int e = g.gend[cell];
+                for (int j = s; j < e; j++) {
+                    double tdist = dist3<double>(xg, yi, zi, A.x[j], A.y[j], A.z[j]) - 1e-20;
+                    if (A.p_type[j] < 1000 && tdist < pr.copy_tol) found = j;
+                }
+            }
+    if (found >= 0) {
+        A.pres[i] = opres[found]; A.ux[i] = oux[found]; A.uy[i] = ouy[found]; A.uz[i] = ouz[found];
+        A.temp[i] = otemp[found];
+    } else {
+        A.pres[i] = 0; A.ux[i] = 0; A.uy[i] = 0; A.uz[i] = 0; A.temp[i] = 0;
+    }
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_pressureforce3D, functions_PPE.cuh:482-647
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_pressure_force(int n, KConst kc, sph_params pr, PView S, QView Q, NList nl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.i_type[i] > 2) return;
+    if (S.p_type[i] >= 1000) return;
+    if (S.p_type[i] <= 0) return;
+    if (S.buffer_type[i] != 0) return;
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    double hi_d = S.h[i];
+    double pi_d = S.pres[i] - S.pres0[i];
+    R pi = (R)pi_d, rhoi = (R)S.rho[i];
+    double tmpx = 0, tmpy = 0, tmpz = 0;
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_DM)) continue;
+        int j = e & NB_IDX;
+        if (S.p_type[j] >= 1000) continue;
+        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+        R hij = (R)(0.5 * (hi_d + S.h[j]));
+        R tmp_Aij = calc_tmpA<R>(kc, hij);
+        R tdwij = kernel_dw<R>(kc, tmp_Aij, hij, tdist);
+        R tdwx = tdwij * dx / tdist;
+        R tdwy = tdwij * dy / tdist;
+        R tdwz = tdwij * dz / tdist;
+        R mj = (R)S.m[j], rhoj = (R)S.rho[j];
+        R pj = (R)(S.pres[j] - S.pres0[j]);
+        if (pr.fp_solve) {
+            const R Pb = (R)0.0;
+            R C_p = (R)(-2.0) * mj / (rhoi * rhoj) * ((pi + Pb) * rhoj + (pj + Pb) * rhoi) / (rhoi + rhoj);
+            tmpx += (double)(C_p * tdwx); tmpy += (double)(C_p * tdwy); tmpz += (double)(C_p * tdwz);
+        }
+    }
+    Q.fpx[i] = tmpx; Q.fpy[i] = tmpy; Q.fpz[i] = tmpz;
+}
+
+// ------------------------------------------------------------------------
+// IBM_force_interpolation3D, functions_IBM.cuh:342-490
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_ibm_force(int n, KConst kc, sph_params pr, PView S, QView Q, NList nl,
+                                                     double dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.p_type[i] < 1000) return;
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    R tmp_h = (R)(0.5 * S.h[i]);
+    R tmp_A = calc_tmpA<R>(kc, tmp_h);
+    double tmpux = 0, tmpuy = 0, tmpuz = 0, filt = 0;
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_BH)) continue;
+        int j = e & NB_IDX;
+        if (S.i_type[j] == 4 || S.p_type[j] != 1) continue;
+        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
+        R twij = kernel_w<R>(kc, tmp_A, tmp_h, tdist);
+        R tmprho = (R)S.m[j] / (R)S.rho[j];
+        tmpux += (double)(((R)S.ux[j]) * tmprho * twij);
+        tmpuy += (double)(((R)S.uy[j]) * tmprho * twij);
+        tmpuz += (double)(((R)S.uz[j]) * tmprho * twij);
+        filt += (double)(tmprho * twij);
+    }
+    S.concentration[i] = filt;
+    double ufx = tmpux / (filt + 1e-10), ufy = tmpuy / (filt + 1e-10), ufz = tmpuz / (filt + 1e-10);
+    double nx = Q.nx[i], ny = Q.ny[i], nz = Q.nz[i];
+    double nmag = sqrt(nx * nx + ny * ny + nz * nz) + 1e-20;
+    nx = nx / nmag; ny = ny / nmag; nz = nz / nmag;
+    double un = ufx * nx + ufy * ny + ufz * nz;
+    double utx = ufx - un * nx, uty = ufy - un * ny, utz = ufz - un * nz;
+    double Ubtx = pr.ibm_ut_factor * interp_ux_caseH(zi, 1.0);
+    double Ubty = 0.0, Ubtz = 0.0;
+    double beta_t = 1.0;
+    double corrx = (-un) * nx + beta_t * (Ubtx - utx);
+    double corry = (-un) * ny + beta_t * (Ubty - uty);
+    double corrz = (-un) * nz + beta_t * (Ubtz - utz);
+    S.fbx[i] = pr.rho_air * corrx / dt;
+    S.fby[i] = pr.rho_air * corry / dt;
+    S.fbz[i] = pr.rho_air * corrz / dt;
+}
+
+// IBM_spreading_interpolation3D, functions_IBM.cuh:492-613
+template <class R>
+__global__ void __launch_bounds__(BLOCK) k_ibm_spread(int n, KConst kc, PView S, NList nl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.p_type[i] != 1) return;
+    if (S.buffer_type[i] == 2) return;
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    R tmp_h = (R)(0.5 * S.h[i]);
+    R tmp_A = calc_tmpA<R>(kc, tmp_h);
+    double filt = 0, fb_x = 0, fb_y = 0, fb_z = 0;
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt; k++) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_BH)) continue;
+        int j = e & NB_IDX;
+        if (S.i_type[j] == 4 || S.p_type[j] != 1000) continue;
+        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
+        R twij = kernel_w<R>(kc, tmp_A, tmp_h, tdist);
+        R tmprho = (R)S.m[j] / (R)S.rho[j];
+        fb_x += (double)((R)S.fbx[j] * twij * tmprho);
+        fb_y += (double)((R)S.fby[j] * twij * tmprho);
+        fb_z += (double)((R)S.fbz[j] * twij * tmprho);
+        filt += (double)(tmprho * twij);
+    }
+    if (filt < 1e-12) {
+        S.fbx[i] = 0.0; S.fby[i] = 0.0; S.fbz[i] = 0.0; S.concentration[i] = 0.0;
+        return;
+    }
+    S.fbx[i] = fb_x / (filt + 1e-10);
+    S.fby[i] = fb_y / (filt + 1e-10);
+    S.fbz[i] = fb_z / (filt + 1e-10);
+    S.concentration[i] = filt;
+}
+
+// ------------------------------------------------------------------------
+// KERNEL_time_update_projection, functions_TIME_ISPH.cuh:115-296 (corrector)
+// and IBM_corrector, functions_IBM.cuh:615-725 (only its copies are live).
+__global__ void __launch_bounds__(BLOCK) k_time_update(int n, sph_params pr, PView S, PView P, P2View S2, P2View P2,
+                                                       QView Q, double dt, int count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (S.p_type[i] >= 1000) {   // IBM_corrector (no i_type filter there)
+        P.fbx[i] = S.fbx[i]; P.fby[i] = S.fby[i]; P.fbz[i] = S.fbz[i];
+        P.concentration[i] = S.concentration[i]; P.vol[i] = S.vol[i]; P.pres[i] = S.pres[i];
+        return;
+    }
+    if (S.i_type[i] >= 3) return;
+    int buffer_type = S.buffer_type[i], p_type_i = S.p_type[i];
+    if (p_type_i == 9) {
+        P.ux[i] = S.ux[i]; P.uy[i] = S.uy[i]; P.uz[i] = S.uz[i];
+    } else {
+        double tux0, tuy0, tuz0, dux_dt, duy_dt, duz_dt;
+        if (p_type_i > 0) {
+            tux0 = S.ux[i]; tuy0 = S.uy[i]; tuz0 = S.uz[i];
+            if (buffer_type == 1 || buffer_type == -1) { tux0 = S2.ux0[i]; tuy0 = S2.uy0[i]; tuz0 = S2.uz0[i]; }
+            double b = (double)(buffer_type == 0);
+            dux_dt = Q.fpx[i] * b + S.fbx[i] / S.rho[i] + S.fmx[i];
+            duy_dt = Q.fpy[i] * b + S.fby[i] / S.rho[i] + S.fmy[i];
+            duz_dt = Q.fpz[i] * b + S.fbz[i] / S.rho[i];
+        } else {
+            tux0 = S2.ux0[i]; tuy0 = S2.uy0[i]; tuz0 = S2.uz0[i];
+            dux_dt = duy_dt = duz_dt = 0.0;
+        }
+        double uxc = tux0 + dux_dt * (dt), uyc = tuy0 + duy_dt * (dt), uzc = tuz0 + duz_dt * (dt);
+        if ((uxc * uxc + uyc * uyc + uzc * uzc) >= pr.u_limit * pr.u_limit) { uxc = tux0; uyc = tuy0; uzc = tuz0; }
+        P.ux[i] = uxc; P.uy[i] = uyc; P.uz[i] = uzc;
+    }
+    if (pr.con_solve == 1) P.temp[i] = S2.temp0[i] + Q.dtemp[i] * dt * (buffer_type == 0);
+    else { P.enthalpy[i] = S.enthalpy[i]; P.temp[i] = S.temp[i]; }
+    if (pr.concn_solve == 1) {
+        double cc = S2.concn0[i] + Q.dconcn[i] * dt;
+        P.concn[i] = cc < 0.0 ? 0.0 : cc;
+    } else P.concn[i] = S.concn[i];
+    P.pres[i] = S.pres[i]; P.flt_s[i] = S.flt_s[i];
+    P.k_turb[i] = S.k_turb[i]; P.e_turb[i] = S.e_turb[i];
+    P2.rho_ref[i] = S2.rho_ref[i]; P.vis_t[i] = S.vis_t[i];
+    P.fbx[i] = S.fbx[i]; P.fby[i] = S.fby[i]; P.fbz[i] = S.fbz[i];
+    P.fmx[i] = S.fmx[i]; P.fmy[i] = S.fmy[i]; P.hmz[i] = S.hmz[i];
+    P.float1[i] = S.float1[i]; P.float2[i] = S.float2[i]; P.float3[i] = S.float3[i];
+    P.concentration[i] = S.concentration[i];
+    if (P.source[i] == 1) P.concn[i] += 0.5 * dt;
+    if (P.p_type[i] == 0) P.temp[i] = wall_temperature(pr, dt, count);
+}
+
+// ------------------------------------------------------------------------
+// kernel_copy_max + 3 x DeviceReduce::Max, functions_OUTPUT.cuh:19-33
+__device__ __forceinline__ unsigned f2ord(float f) {
+    unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+__global__ void __launch_bounds__(BLOCK) k_stats(int n, PView P, P2View S2, QView Q, unsigned* out) {
+    __shared__ unsigned s[3][BLOCK];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float a = 0, b = 0, c = 0;
+    if (i < n && !(P.p_type[i] == 0 || P.i_type[i] >= 2)) {
+        a = (float)sqrt(P.ux[i] * P.ux[i] + P.uy[i] * P.uy[i] + P.uz[i] * P.uz[i]);
+        b = (float)((P.rho[i] - S2.rho_ref[i]) / S2.rho_ref[i] * 100.0);
+        c = (float)Q.ftotal[i];
+    }
+    unsigned ua = i < n ? f2ord(a) : 0u, ub = i < n ? f2ord(b) : 0u, uc = i < n ? f2ord(c) : 0u;
+    s[0][threadIdx.x] = ua; s[1][threadIdx.x] = ub; s[2][threadIdx.x] = uc;
+    __syncthreads();
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 3; q++) s[q][threadIdx.x] = max(s[q][threadIdx.x], s[q][threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) atomicMax(&out[q], s[q][0]);
+}
+__global__ void k_stats_final(const unsigned* in, float* out) {
+    if (threadIdx.x < 3) out[threadIdx.x] = ord2f(in[threadIdx.x]);
+}
+
+// ========================================================================
+// Host orchestration
+// ========================================================================
+static KConst kconst(const sph_ctx* c) { return KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15}; }
+
+static int snapshot(sph_ctx* c, const PView& A) {
+    const double* src[5] = {A.ux, A.uy, A.uz, A.pres, A.temp};
+    for (int f = 0; f < 5; f++)
+        SPH_HIP(c, hipMemcpyAsync(c->snap[f], src[f], sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+#define LAUNCH(name, bytes, kern, ...)                                                  \
+    do {                                                                               \
+        SPH_TRY(prof_begin(c, name, (double)(bytes)));                                 \
+        hipLaunchKernelGGL(kern, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, __VA_ARGS__); \
+        SPH_TRY(launch_check(c, name));                                                \
+        SPH_TRY(prof_end(c));                                                          \
+    } while (0)
+
+static int run_neumann(sph_ctx* c, const PView& A) {
+    int n = c->n;
+    SPH_TRY(snapshot(c, A));
+    if (c->prm.precision == SPH_MIXED)
+        LAUNCH("neumann", 0, k_neumann<float>, n, kconst(c), c->prm, A, c->grid(), (const double*)c->snap[3]);
+    else
+        LAUNCH("neumann", 0, k_neumann<double>, n, kconst(c), c->prm, A, c->grid(), (const double*)c->snap[3]);
+    return 0;
+}
+static int run_outlet(sph_ctx* c, const PView& A) {
+    int n = c->n;
+    SPH_TRY(snapshot(c, A));
+    LAUNCH("outlet", 0, k_outlet, n, c->prm, A, c->grid(), (const double*)c->snap[0], (const double*)c->snap[1],
+           (const double*)c->snap[2], (const double*)c->snap[3], (const double*)c->snap[4]);
+    return 0;
+}
+static int run_periodic(sph_ctx* c, const PView& A) {
+    int n = c->n;
+    SPH_TRY(snapshot(c, A));
+    LAUNCH("periodic", 0, k_periodic, n, c->prm, A, c->grid(), (const double*)c->snap[0], (const double*)c->snap[1],
+           (const double*)c->snap[2], (const double*)c->snap[3], (const double*)c->snap[4]);
+    return 0;
+}
+static int run_freeslip(sph_ctx* c, const PView& A) {
+    int n = c->n;
+    if (c->prm.precision == SPH_MIXED) LAUNCH("freeslip", 0, k_freeslip<float>, n, kconst(c), A, c->nlist());
+    else LAUNCH("freeslip", 0, k_freeslip<double>, n, kconst(c), A, c->nlist());
+    return 0;
+}
+
+// Everything of SOPHIA_single_ISPH before the pressure solve
+// (ISPH_Calc.cuh:20-132).  The count==0 neighbour search has already run.
+int step_physics_front(sph_ctx* c, int count, double dt, double time) {
+    const int n = c->n;
+    const sph_params& pr = c->prm;
+    const bool mixed = pr.precision == SPH_MIXED;
+    PView P = c->pview(), S = c->sview();
+    P2View S2 = c->s2view();
+    QView Q = c->qview();
+    NList nl = c->nlist();
+    KConst kc = kconst(c);
+    // SP1 <- P1, SP2 <- P2, P3 <- 0 (normals persist), set_ncell
+    CopySet cs{};
+    for (int f = 0; f < MF_COUNT; f++) {
+        static const int map[MF_COUNT] = {SPH_F_UX, SPH_F_UY, SPH_F_UZ, SPH_F_PRES, SPH_F_TEMP, SPH_F_VIS_T,
+                                          SPH_F_K_TURB, SPH_F_E_TURB, SPH_F_FLT_S, SPH_F_FMX, SPH_F_FMY,
+                                          SPH_F_HMZ, SPH_F_FLOAT1, SPH_F_FLOAT2, SPH_F_FLOAT3, SPH_F_FBX,
+                                          SPH_F_FBY, SPH_F_FBZ, SPH_F_CONCENTRATION, SPH_F_CONCN,
+                                          SPH_F_ENTHALPY};
+        cs.src[f] = c->pf[map[f]];
+        cs.dst[f] = c->sm[f];
+    }
+    for (int f = 0; f < 10; f++) { cs.src[MF_COUNT + f] = c->p2[f]; cs.dst[MF_COUNT + f] = c->s2[f]; }
+    for (int f = 0; f < 15; f++) cs.zero[f] = c->q[f];
+    LAUNCH("step_begin", 0, k_step_begin, n, cs);
+
+    if (mixed) LAUNCH("grad_corr", 0, k_grad_corr<float>, n, kc, S, Q, nl);
+    else LAUNCH("grad_corr", 0, k_grad_corr<double>, n, kc, S, Q, nl);
+    if (mixed) LAUNCH("prep", 0, k_prep<float>, n, kc, pr, S, S2, Q, nl, count, dt);
+    else LAUNCH("prep", 0, k_prep<double>, n, kc, pr, S, S2, Q, nl, count, dt);
+    if (pr.noslip_bc == 1) {
+        if (mixed) LAUNCH("boundary3d", 0, k_boundary3d<float>, n, kc, pr, S, nl);
+        else LAUNCH("boundary3d", 0, k_boundary3d<double>, n, kc, pr, S, nl);
+    }
+    double* dep = c->snap[4];   // scratch: MOST deposits (snapshots are not live here)
+    LAUNCH("most_self", 0, k_most_self, n, pr, S, dep, count, dt);
+    LAUNCH("most_deposit", 0, k_most_deposit, n, pr, S, Q, c->grid(), (const double*)dep, (const int*)c->keys);
+    if (count == 0) SPH_TRY(run_neumann(c, S));
+    if (mixed) LAUNCH("advection", 0, k_advection<float>, n, kc, pr, S, Q, nl);
+    else LAUNCH("advection", 0, k_advection<double>, n, kc, pr, S, Q, nl);
+    if (pr.time_type == SPH_PRE_COR) LAUNCH("projection", 0, k_projection, n, pr, S, S2, Q, dt, time);
+    SPH_TRY(run_periodic(c, S));
+    SPH_TRY(run_freeslip(c, S));
+    return 0;
+}
+
+// Everything after the pressure solve (ISPH_Calc.cuh:143-214).
+int step_physics_back(sph_ctx* c, int count, double dt, double time) {
+    (void)time;
+    const int n = c->n;
+    const sph_params& pr = c->prm;
+    const bool mixed = pr.precision == SPH_MIXED;
+    PView P = c->pview(), S = c->sview();
+    P2View S2 = c->s2view(), P2 = c->p2view();
+    QView Q = c->qview();
+    NList nl = c->nlist();
+    KConst kc = kconst(c);
+    SPH_TRY(run_neumann(c, S));
+    SPH_TRY(run_outlet(c, S));
+    if (mixed) LAUNCH("pressure_force", 0, k_pressure_force<float>, n, kc, pr, S, Q, nl);
+    else LAUNCH("pressure_force", 0, k_pressure_force<double>, n, kc, pr, S, Q, nl);
+    if (mixed) LAUNCH("ibm_force", 0, k_ibm_force<float>, n, kc, pr, S, Q, nl, dt);
+    else LAUNCH("ibm_force", 0, k_ibm_force<double>, n, kc, pr, S, Q, nl, dt);
+    if (mixed) LAUNCH("ibm_spread", 0, k_ibm_spread<float>, n, kc, S, nl);
+    else LAUNCH("ibm_spread", 0, k_ibm_spread<double>, n, kc, S, nl);
+    LAUNCH("time_update", 0, k_time_update, n, pr, S, P, S2, P2, Q, dt, count);
+    SPH_TRY(run_neumann(c, P));
+    SPH_TRY(run_outlet(c, P));
+    SPH_TRY(run_periodic(c, P));
+    SPH_TRY(run_freeslip(c, P));
+    return 0;
+}
+
+int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax_) {
+    const int n = c->n;
+    unsigned* acc = (unsigned*)c->stat_buf;
+    SPH_HIP(c, hipMemsetAsync(acc, 0, 3 * sizeof(unsigned), c->stream));
+    LAUNCH("stats", 0, k_stats, n, c->pview(), c->s2view(), c->qview(), acc);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, c->stream, (const unsigned*)acc, c->stat_buf + 4);
+    SPH_TRY(launch_check(c, "stats_final"));
+    float h[3];
+    SPH_HIP(c, hipMemcpyAsync(h, c->stat_buf + 4, 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    *umax = h[0]; *drho = h[1]; *fmax_ = h[2];
+    return 0;
+}
+
+}  // namespace sph
