@@ -1,0 +1,66 @@
+"""Committed golden fixtures (tests/golden/*.npz, written by
+tests/golden/make_golden.py): the oracle must reproduce them bit for bit
+(CPU), and the HIP library must match them (GPU, FP64 tolerance)."""
+import ctypes as C
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import abi
+import oracle
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FILES = sorted(glob.glob(os.path.join(HERE, "*.npz")))
+
+
+def _load(path):
+    z = np.load(path, allow_pickle=False)
+    prm = abi.Params.from_buffer_copy(z["params"].tobytes())
+    nrm = z["normals"] if "normals" in z.files else None
+    return z, prm, nrm
+
+
+def test_fixtures_present():
+    assert len(FILES) >= 3
+
+
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_oracle_reproduces_fixture(path):
+    z, prm, nrm = _load(path)
+    o = oracle.Oracle(prm, len(z["P1"]))
+    o.upload(z["P1"], None, nrm)
+    for c in range(int(z["steps"])):
+        assert o.step(c, float(z["dt"])) == 0
+    keys, perm, gs, ge = o.nnps()
+    np.testing.assert_array_equal(perm, z["perm"])
+    np.testing.assert_array_equal(keys, z["keys"])
+    np.testing.assert_array_equal(gs, z["cell_start"])
+    for k in z.files:
+        if k.startswith("f_"):
+            np.testing.assert_array_equal(o.field(k[2:]), z[k], err_msg=k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FILES, ids=[os.path.basename(f) for f in FILES])
+def test_gpu_matches_fixture(path):
+    import urbansph
+    z, prm, nrm = _load(path)
+    print(f"[golden] {os.path.basename(path)} n={len(z['P1'])}", flush=True)
+    g = urbansph.Case(prm, len(z["P1"]))
+    g.upload(z["P1"], None, nrm)
+    for c in range(int(z["steps"])):
+        g.step(c, float(z["dt"]))
+    g.sync()
+    keys, perm, gs, ge = g.nnps()
+    np.testing.assert_array_equal(perm, z["perm"])
+    np.testing.assert_array_equal(keys, z["keys"])
+    np.testing.assert_array_equal(gs, z["cell_start"])
+    np.testing.assert_array_equal(ge, z["cell_end"])
+    for k in z.files:
+        if k.startswith("f_"):
+            a, b = g.field(k[2:]), z[k]
+            d = np.linalg.norm(a - b)
+            s = np.linalg.norm(b)
+            assert (d / s if s > 0 else d) <= 1e-10, k

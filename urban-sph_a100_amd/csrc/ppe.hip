@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(BLOCK) k_kr_fin(int op, long long nb, const do
     }
     double A = canon_partials(pa, nb, sh, lvl);
     double B = 0.0;
-    if (op == FIN_INIT || op == FIN_OMEGA || op == FIN_R) B = canon_partials(pb, nb, sh, lvl);
+    if (op == FIN_INIT || op == FIN_OMEGA || op == FIN_R || op == FIN_NORM) B = canon_partials(pb, nb, sh, lvl);
     if (threadIdx.x != 0) return;
     const double tol = sc[SC_TOL];
     switch (op) {
