@@ -70,6 +70,19 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
     return out
 
 
+def measured_traffic(precision):
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from a 1 GiB copy);
+    None when the summary is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "spmv_canyon2m.md")
+    if not os.path.exists(path):
+        return None
+    key = "mixed" if precision == "mixed" else "fp64"
+    m = re.search(r"traffic_bytes_per_launch_%s_spmv: (\d+)" % key, open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(cases, abi, dt, tol, max_seconds=30.0):
     """The FP64 oracle (OpenMP) on a bounded sample of the same workload: a
     0.4 x 0.32 x 0.5 m slab of the canyon at the same dx with one building."""
@@ -105,7 +118,18 @@ def main():
     ap.add_argument("--dt", type=float, default=5e-5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run only a 1 GiB device-to-device copy kernel (for scaling HBM counters)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_read": a.numel() * 4, "copy_calib_bytes_written": a.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
               f"(multi-GPU halo exchange is design-only)", file=sys.stderr)
@@ -144,7 +168,7 @@ def main():
         avg_s = ms / launches / 1e3
         achieved = byts / launches / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(main_name),
                 "kernel": "spmv", "avg_us": round(avg_s * 1e6, 2),
                 "alg_bytes_per_launch": round(byts / launches)}
     tot_ms = sum(v[1] for v in m["prof"].values())

@@ -145,7 +145,8 @@ typedef struct sph_params {
     int    precision;           /* SPH_FP64 / SPH_MIXED                       */
     double ppe_tol;             /* relative residual ||b-Ax||/||b||           */
     int    ppe_maxit;           /* max BiCGSTAB iterations per solve          */
-    double ppe_inner_tol;       /* MIXED: relative tol of each inner solve    */
+    double ppe_inner_tol;       /* MIXED: floor of each inner solve's relative
+                                   tolerance (each aims at ppe_tol absolute)  */
     int    ppe_max_refine;      /* MIXED: max outer refinement steps          */
 } sph_params;
 

@@ -19,6 +19,18 @@
 
 namespace sph {
 
+// Bin-cell range that contains every particle within distance r of p
+// (binning is monotone in each coordinate; a small margin absorbs the
+// double -> float rounding of the bounds).
+__device__ __forceinline__ void ball_bins(const sph_domain& d, double px, double py, double pz, double r,
+                                          Cell& lo, Cell& hi) {
+    const double m = 1e-9 + 1e-6 * r;
+    lo = Cell{bin_axis((float)(px - r - m), d.x_min, d.x_max, d.NI), bin_axis((float)(py - r - m), d.y_min, d.y_max, d.NJ),
+              bin_axis((float)(pz - r - m), d.z_min, d.z_max, d.NK)};
+    hi = Cell{bin_axis((float)(px + r + m), d.x_min, d.x_max, d.NI), bin_axis((float)(py + r + m), d.y_min, d.y_max, d.NJ),
+              bin_axis((float)(pz + r + m), d.z_min, d.z_max, d.NK)};
+}
+
 struct NLRow { const unsigned* p; int cnt; };
 __device__ __forceinline__ NLRow nl_row(const NList& nl, int i) {
     return {nl.ent + nl.slice_off[i >> 6] + (i & 63), nl.cnt[i]};
@@ -333,7 +345,9 @@ __global__ void __launch_bounds__(BLOCK) k_neumann(int n, KConst kc, sph_params 
     double xg = (bt == 1 || bt == -1) ? 2.0 * pr.L2 - xi : xi;
     double yg = yi, zg = zi;
     Cell c = query_cell(d, xg, yg, zg);
-    for (int z = -1; z <= 1; z++)
+    // the ghost pressure and the gradient only enter the inlet formula
+    const bool inlet = (bt == 1 || bt == -1);
+    for (int z = -1; z <= 1 && inlet; z++)
         for (int y = -1; y <= 1; y++)
             for (int x = -1; x <= 1; x++) {
                 if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
@@ -372,10 +386,12 @@ __global__ void __launch_bounds__(BLOCK) k_neumann(int n, KConst kc, sph_params 
                     tuy += (double)(mj / rhoj * (R)A.uy[j] * twij * neighbor);
                     tuz += (double)(mj / rhoj * (R)A.uz[j] * twij * neighbor);
                     tflt_s += (double)(mj / rhoj * twij * neighbor);
-                    R dp = (R)(presj - pres_g);
-                    grad_px += (double)(mj / rhoj * dp * tdwx * neighbor);
-                    grad_py += (double)(mj / rhoj * dp * tdwy * neighbor);
-                    grad_pz += (double)(mj / rhoj * dp * tdwz * neighbor);
+                    if (inlet) {
+                        R dp = (R)(presj - pres_g);
+                        grad_px += (double)(mj / rhoj * dp * tdwx * neighbor);
+                        grad_py += (double)(mj / rhoj * dp * tdwy * neighbor);
+                        grad_pz += (double)(mj / rhoj * dp * tdwz * neighbor);
+                    }
                 }
             }
     if (tflt_s < 1e-6) {
@@ -572,9 +588,13 @@ __global__ void __launch_bounds__(BLOCK) k_periodic(int n, sph_params pr, PView 
     int nc = A.ncell[i];
     Cell c = query_cell(d, xg, yg, zg);
     int found = -1;
-    for (int z = -nc; z <= nc && found < 0; z++)
-        for (int y = -nc; y <= nc && found < 0; y++)
-            for (int x = -nc; x <= nc && found < 0; x++) {
+    // only cells inside the bin range of the copy_tol ball can hold a match
+    // (the first such cell in window order is unchanged)
+    Cell lo, hi;
+    ball_bins(d, xg, yg, zg, pr.copy_tol, lo, hi);
+    for (int z = max(-nc, lo.k - c.k); z <= min(nc, hi.k - c.k) && found < 0; z++)
+        for (int y = max(-nc, lo.j - c.j); y <= min(nc, hi.j - c.j) && found < 0; y++)
+            for (int x = max(-nc, lo.i - c.i); x <= min(nc, hi.i - c.i) && found < 0; x++) {
                 if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
                 int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
                 int s = g.gstr[cell];
@@ -641,9 +661,14 @@ __global__ void __launch_bounds__(BLOCK) k_outlet(int n, sph_params pr, PView A,
     int nc = 2 * A.ncell[i];
     double xg = pr.L3 - pr.outlet_copy_dx;
     int found = -1;
-    for (int z = -nc; z <= nc; z++)
-        for (int y = -nc; y <= nc; y++)
-            for (int x = -nc; x <= nc; x++) {
+    // A match lies within copy_tol of (xg, yi, zi); binning is monotone, so
+    // only window cells inside the bin range of that ball can hold one.
+    // Skipping the others leaves the last match in window order unchanged.
+    Cell lo, hi;
+    ball_bins(d, xg, yi, zi, pr.copy_tol, lo, hi);
+    for (int z = max(-nc, lo.k - c.k); z <= min(nc, hi.k - c.k); z++)
+        for (int y = max(-nc, lo.j - c.j); y <= min(nc, hi.j - c.j); y++)
+            for (int x = max(-nc, lo.i - c.i); x <= min(nc, hi.i - c.i); x++) {
                 if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
                 int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
                 int s = g.gstr[cell];

@@ -65,34 +65,77 @@ struct PPEState {
 };
 
 // ------------------------------------------------------------------------
-// canonical block reduction (256 values, one per thread) -> thread 0
+// Canonical reductions.  A chunk of 256 values is summed by the pairwise
+// tree t[i] += t[i+off], off = 128, 64, ..., 1.  One wave64 evaluates it in
+// registers: lane l holds a[l], a[l+64], a[l+128], a[l+192]; the two upper
+// levels are (a[l]+a[l+128]) + (a[l+64]+a[l+192]); the last six are
+// shuffles with offsets 32..1.  The pairing is exactly the tree's, so the
+// result is bit-identical to any other evaluation of the same tree.
+__device__ __forceinline__ double wave_tree(double a0, double a1, double a2, double a3) {
+    double v = (a0 + a2) + (a1 + a3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+// one value per thread of a 256-thread block -> canonical sum on thread 0
 __device__ __forceinline__ double block_canon(double v, double* sh) {
     const int t = threadIdx.x;
     sh[t] = v;
     __syncthreads();
-#pragma unroll
-    for (int off = 128; off > 0; off >>= 1) {
-        if (t < off) sh[t] += sh[t + off];
-        __syncthreads();
-    }
-    double r = sh[0];
+    double r = 0.0;
+    if (t < 64) r = wave_tree(sh[t], sh[t + 64], sh[t + 128], sh[t + 192]);
     __syncthreads();
     return r;
 }
-// One 256-thread block reduces nb partials canonically (nb <= 65536).
-__device__ double canon_partials(const double* part, long long nb, double* sh, double* lvl) {
+// two sums at once (waves 0 and 1)
+__device__ __forceinline__ void block_canon2(double a, double b, double* sh, double* ra, double* rb) {
     const int t = threadIdx.x;
-    if (nb <= 256) return block_canon(t < nb ? part[t] : 0.0, sh);
-    int nc = (int)((nb + 255) / 256);
-    for (int c = 0; c < nc; c++) {
-        long long idx = (long long)c * 256 + t;
-        double v = block_canon(idx < nb ? part[idx] : 0.0, sh);
-        if (t == 0) lvl[c] = v;
+    sh[t] = a;
+    sh[BLOCK + t] = b;
+    __syncthreads();
+    if (t < 64) *ra = wave_tree(sh[t], sh[t + 64], sh[t + 128], sh[t + 192]);
+    else if (t < 128) {
+        const int l = t - 64;
+        *rb = wave_tree(sh[BLOCK + l], sh[BLOCK + l + 64], sh[BLOCK + l + 128], sh[BLOCK + l + 192]);
     }
     __syncthreads();
-    double v = t < nc ? lvl[t] : 0.0;
+}
+__device__ __forceinline__ double chunk_from_global(const double* part, long long base, long long nb, int lane) {
+    double a[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        long long idx = base + lane + 64 * q;
+        a[q] = idx < nb ? part[idx] : 0.0;
+    }
+    return wave_tree(a[0], a[1], a[2], a[3]);
+}
+// One 256-thread block reduces nb partials canonically (nb <= 65536); the
+// result is valid on thread 0.
+__device__ double canon_partials(const double* part, long long nb, double* lvl) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (nb <= 256) {
+        double r = 0.0;
+        if (w == 0) r = chunk_from_global(part, 0, nb, lane);
+        return r;
+    }
+    int nc = (int)((nb + 255) / 256);
+    for (int c = w; c < nc; c += (int)(blockDim.x >> 6)) {
+        double v = chunk_from_global(part, (long long)c * 256, nb, lane);
+        if (lane == 0) lvl[c] = v;
+    }
     __syncthreads();
-    return block_canon(v, sh);
+    double r = 0.0;
+    if (w == 0) {
+        double a[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int idx = lane + 64 * q;
+            a[q] = idx < nc ? lvl[idx] : 0.0;
+        }
+        r = wave_tree(a[0], a[1], a[2], a[3]);
+    }
+    __syncthreads();
+    return r;
 }
 
 // ------------------------------------------------------------------------
@@ -226,10 +269,31 @@ __device__ __forceinline__ double spmv_row(int r, int nr, const long long* sl_of
                                            const V* vals, const double* diag, const X* x) {
     const long long base = sl_off[r >> 6] + (r & 63);
     const int m = cnt[r];
+    const int* cp = cols + base;
+    const V* vp = vals + base;
     double s = 0.0;
-    for (int k = 0; k < m; k++) {
-        long long e = base + (long long)k * SLICE;
-        s += (double)vals[e] * (double)x[cols[e]];
+    int k = 0;
+    // matrix entries stream through once: nontemporal loads keep the
+    // caches for the x gathers; four independent loads in flight per lane
+    for (; k + 4 <= m; k += 4) {
+        const int c0 = __builtin_nontemporal_load(cp + (size_t)(k + 0) * SLICE);
+        const int c1 = __builtin_nontemporal_load(cp + (size_t)(k + 1) * SLICE);
+        const int c2 = __builtin_nontemporal_load(cp + (size_t)(k + 2) * SLICE);
+        const int c3 = __builtin_nontemporal_load(cp + (size_t)(k + 3) * SLICE);
+        const V v0 = __builtin_nontemporal_load(vp + (size_t)(k + 0) * SLICE);
+        const V v1 = __builtin_nontemporal_load(vp + (size_t)(k + 1) * SLICE);
+        const V v2 = __builtin_nontemporal_load(vp + (size_t)(k + 2) * SLICE);
+        const V v3 = __builtin_nontemporal_load(vp + (size_t)(k + 3) * SLICE);
+        const X x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        s += (double)v0 * (double)x0;
+        s += (double)v1 * (double)x1;
+        s += (double)v2 * (double)x2;
+        s += (double)v3 * (double)x3;
+    }
+    for (; k < m; k++) {
+        const int c0 = __builtin_nontemporal_load(cp + (size_t)k * SLICE);
+        const V v0 = __builtin_nontemporal_load(vp + (size_t)k * SLICE);
+        s += (double)v0 * (double)x[c0];
     }
     return diag[r] * (double)x[r] - s;
 }
@@ -239,7 +303,7 @@ template <class V, class X>
 __global__ void __launch_bounds__(BLOCK) k_kr_init(int nr, const long long* sl_off, const int* cnt, const int* cols,
                                                    const V* vals, const double* diag, const X* rhs, const X* x,
                                                    X* r, X* rhat, double* pa, double* pb) {
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[2 * BLOCK];
     int row = blockIdx.x * blockDim.x + threadIdx.x;
     double a = 0.0, b = 0.0;
     if (row < nr) {
@@ -250,9 +314,10 @@ __global__ void __launch_bounds__(BLOCK) k_kr_init(int nr, const long long* sl_o
         a = (double)rhs[row] * (double)rhs[row];
         b = (double)rr * (double)rr;
     }
-    double ta = block_canon(a, sh);
-    double tb = block_canon(b, sh);
-    if (threadIdx.x == 0) { pa[blockIdx.x] = ta; pb[blockIdx.x] = tb; }
+    double ta = 0.0, tb = 0.0;
+    block_canon2(a, b, sh, &ta, &tb);
+    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
+    if (threadIdx.x == 64) pb[blockIdx.x] = tb;
 }
 
 // p = r (it==1) or r + beta (p - omega v); phat = p / diag
@@ -275,7 +340,7 @@ __global__ void __launch_bounds__(BLOCK) k_kr_spmv(int nr, const double* sc, con
                                                    const int* cols, const V* vals, const double* diag, const X* x,
                                                    X* y, const X* d, double* pa, double* pb) {
     GATE(sc);
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[2 * BLOCK];
     int row = blockIdx.x * blockDim.x + threadIdx.x;
     double a = 0.0, b = 0.0;
     if (row < nr) {
@@ -288,12 +353,15 @@ __global__ void __launch_bounds__(BLOCK) k_kr_spmv(int nr, const double* sc, con
             a = (double)d[row] * (double)yy;    // rhat.v
         }
     }
-    double ta = block_canon(a, sh);
     if (TWO) {
-        double tb = block_canon(b, sh);
-        if (threadIdx.x == 0) pb[blockIdx.x] = tb;
+        double ta = 0.0, tb = 0.0;
+        block_canon2(a, b, sh, &ta, &tb);
+        if (threadIdx.x == 0) pa[blockIdx.x] = ta;
+        if (threadIdx.x == 64) pb[blockIdx.x] = tb;
+    } else {
+        double ta = block_canon(a, sh);
+        if (threadIdx.x == 0) pa[blockIdx.x] = ta;
     }
-    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
 }
 
 // s = r - alpha v ; shat = s / diag ; partial s.s
@@ -321,7 +389,7 @@ __global__ void __launch_bounds__(BLOCK) k_kr_xr(int nr, const double* sc, XS* x
                                                  const X* s, const X* t, X* r, const X* rhat, double* pa, double* pb) {
     const bool half = sc[SC_HALF] == 1.0;
     if (sc[SC_DONE] != 0.0 && !half) return;
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[2 * BLOCK];
     int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (half) {
         if (row < nr) x[row] = (XS)((double)x[row] + sc[SC_ALPHA] * (double)phat[row]);
@@ -335,16 +403,18 @@ __global__ void __launch_bounds__(BLOCK) k_kr_xr(int nr, const double* sc, XS* x
         a = (double)rv * (double)rv;
         b = (double)rhat[row] * (double)rv;
     }
-    double ta = block_canon(a, sh);
-    double tb = block_canon(b, sh);
-    if (threadIdx.x == 0) { pa[blockIdx.x] = ta; pb[blockIdx.x] = tb; }
+    double ta = 0.0, tb = 0.0;
+    block_canon2(a, b, sh, &ta, &tb);
+    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
+    if (threadIdx.x == 64) pb[blockIdx.x] = tb;
 }
 
 enum FinOp { FIN_INIT, FIN_ALPHA, FIN_S, FIN_OMEGA, FIN_R, FIN_NORM };
-__global__ void __launch_bounds__(BLOCK) k_kr_fin(int op, long long nb, const double* pa, const double* pb,
+constexpr int FIN_BLOCK = 1024;
+__global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(int op, long long nb, const double* pa, const double* pb,
                                                   double* sc) {
-    __shared__ double sh[BLOCK];
     __shared__ double lvl[256];
+    __shared__ double resB;
     const bool half = sc[SC_HALF] == 1.0;
     if (op == FIN_R) {
         if (sc[SC_DONE] != 0.0 && !half) return;
@@ -355,9 +425,12 @@ __global__ void __launch_bounds__(BLOCK) k_kr_fin(int op, long long nb, const do
         if (threadIdx.x == 0) sc[SC_HALF] = 2.0;
         return;
     }
-    double A = canon_partials(pa, nb, sh, lvl);
+    double A = canon_partials(pa, nb, lvl);
     double B = 0.0;
-    if (op == FIN_INIT || op == FIN_OMEGA || op == FIN_R || op == FIN_NORM) B = canon_partials(pb, nb, sh, lvl);
+    if (op == FIN_INIT || op == FIN_OMEGA || op == FIN_R || op == FIN_NORM) {
+        B = canon_partials(pb, nb, lvl);
+        if (threadIdx.x == 0) resB = B;
+    }
     if (threadIdx.x != 0) return;
     const double tol = sc[SC_TOL];
     switch (op) {
@@ -408,7 +481,7 @@ __global__ void __launch_bounds__(BLOCK) k_kr_fin(int op, long long nb, const do
 __global__ void __launch_bounds__(BLOCK) k_outer_res(int nr, const long long* sl_off, const int* cnt, const int* cols,
                                                      const float* vals, const double* diag, const double* rhs,
                                                      const double* x, float* r32, double* pa, double* pb) {
-    __shared__ double sh[BLOCK];
+    __shared__ double sh[2 * BLOCK];
     int row = blockIdx.x * blockDim.x + threadIdx.x;
     double a = 0.0, b = 0.0;
     if (row < nr) {
@@ -418,9 +491,10 @@ __global__ void __launch_bounds__(BLOCK) k_outer_res(int nr, const long long* sl
         a = rr * rr;
         b = rhs[row] * rhs[row];
     }
-    double ta = block_canon(a, sh);
-    double tb = block_canon(b, sh);
-    if (threadIdx.x == 0) { pa[blockIdx.x] = ta; pb[blockIdx.x] = tb; }
+    double ta = 0.0, tb = 0.0;
+    block_canon2(a, b, sh, &ta, &tb);
+    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
+    if (threadIdx.x == 64) pb[blockIdx.x] = tb;
 }
 __global__ void __launch_bounds__(BLOCK) k_add_correction(int nr, double* x, const float* e) {
     int row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -524,6 +598,14 @@ static int read_scalars(sph_ctx* c) {
     return 0;
 }
 
+#define KF(op)                                                                                     \
+    do {                                                                                           \
+        SPH_TRY(prof_begin(c, "kr_fin", 0.0));                                                     \
+        hipLaunchKernelGGL(k_kr_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, (int)(op), P->nb,      \
+                           (const double*)P->partA, (const double*)P->partB, sc);                  \
+        SPH_TRY(launch_check(c, "kr_fin"));                                                        \
+        SPH_TRY(prof_end(c));                                                                      \
+    } while (0)
 #define KL(name, bytes, grid, kern, ...)                                              \
     do {                                                                             \
         SPH_TRY(prof_begin(c, name, (double)(bytes)));                               \
@@ -549,7 +631,7 @@ static int bicgstab(sph_ctx* c, const V* vals, const X* rhs, XS* xs, X* xinit, X
     SPH_HIP(c, hipMemcpyAsync(sc + SC_MAXIT, c->hscal + SC_MAXIT, sizeof(double), hipMemcpyHostToDevice, c->stream));
     KL("kr_init", nnzb, g, (k_kr_init<V, X>), nr, (const long long*)P->sl_off, (const int*)P->cnt,
        (const int*)P->cols, vals, (const double*)P->diag, rhs, (const X*)xinit, r, rhat, P->partA, P->partB);
-    KL("kr_fin", 0, 1, k_kr_fin, (int)FIN_INIT, P->nb, (const double*)P->partA, (const double*)P->partB, sc);
+    KF(FIN_INIT);
     SPH_TRY(read_scalars(c));
     if (c->hscal[SC_DONE] != 0.0) {
         if (c->hscal[SC_ZEROX] != 0.0)
@@ -565,17 +647,17 @@ static int bicgstab(sph_ctx* c, const V* vals, const X* rhs, XS* xs, X* xinit, X
             KL("spmv", nnzb, g, (k_kr_spmv<V, X, false>), nr, (const double*)sc, (const long long*)P->sl_off,
                (const int*)P->cnt, (const int*)P->cols, vals, (const double*)P->diag, (const X*)phat, v,
                (const X*)rhat, P->partA, P->partB);
-            KL("kr_fin", 0, 1, k_kr_fin, (int)FIN_ALPHA, P->nb, (const double*)P->partA, (const double*)P->partB, sc);
+            KF(FIN_ALPHA);
             KL("kr_s", 0, g, k_kr_s<X>, nr, (const double*)sc, (const X*)r, (const X*)v, s, shat,
                (const double*)P->diag, P->partA);
-            KL("kr_fin", 0, 1, k_kr_fin, (int)FIN_S, P->nb, (const double*)P->partA, (const double*)P->partB, sc);
+            KF(FIN_S);
             KL("spmv", nnzb, g, (k_kr_spmv<V, X, true>), nr, (const double*)sc, (const long long*)P->sl_off,
                (const int*)P->cnt, (const int*)P->cols, vals, (const double*)P->diag, (const X*)shat, t,
                (const X*)s, P->partA, P->partB);
-            KL("kr_fin", 0, 1, k_kr_fin, (int)FIN_OMEGA, P->nb, (const double*)P->partA, (const double*)P->partB, sc);
+            KF(FIN_OMEGA);
             KL("kr_xr", 0, g, (k_kr_xr<X, XS>), nr, (const double*)sc, xs, (const X*)phat, (const X*)shat,
                (const X*)s, (const X*)t, r, (const X*)rhat, P->partA, P->partB);
-            KL("kr_fin", 0, 1, k_kr_fin, (int)FIN_R, P->nb, (const double*)P->partA, (const double*)P->partB, sc);
+            KF(FIN_R);
         }
         SPH_TRY(read_scalars(c));
         if (c->hscal[SC_DONE] != 0.0) break;
@@ -636,16 +718,18 @@ int ppe_solve(sph_ctx* c, double dt) {
             KL("outer_res", (double)P->nnz * 8.0 + (double)nr * 28.0, g, k_outer_res, nr, (const long long*)P->sl_off,
                (const int*)P->cnt, (const int*)P->cols, (const float*)P->vals32, (const double*)P->diag,
                (const double*)P->rhs, (const double*)P->x, P->frhs, P->partA, P->partB);
-            KL("kr_fin", 0, 1, k_kr_fin, (int)FIN_NORM, P->nb, (const double*)P->partA, (const double*)P->partB, sc);
+            KF(FIN_NORM);
             SPH_TRY(read_scalars(c));
             double rn = c->hscal[SC_RNORM], bn = c->hscal[SC_BNORM];
             rel = bn > 0 ? rn / bn : 0.0;
             if (rn <= pr.ppe_tol * bn) { conv = true; break; }
             if (refine == pr.ppe_max_refine) break;
             hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(BLOCK), 0, c->stream, (long long)nr, P->fx);
+            // aim the correction at the final absolute tolerance, but not
+            // below what a float Krylov solve reliably attains
+            double inner = std::max(pr.ppe_inner_tol, 0.5 * pr.ppe_tol * bn / rn);
             SPH_TRY((bicgstab<float, float, float>(c, P->vals32, P->frhs, P->fx, P->fx, P->fr, P->frhat, P->fp,
-                                                    P->fphat, P->fv, P->fs, P->fshat, P->ft, pr.ppe_inner_tol,
-                                                    pr.ppe_maxit)));
+                                                    P->fphat, P->fv, P->fs, P->fshat, P->ft, inner, pr.ppe_maxit)));
             total_it += (int)c->hscal[SC_ITERS];
             KL("add_corr", 0, g, k_add_correction, nr, P->x, (const float*)P->fx);
         }
