@@ -1,15 +1,23 @@
 #!/usr/bin/env python3
 """ISPH projection-step benchmark on one MI355X.
 
-Workload: the urban street canyon at 2M particles -- domain 2.0 x 1.0 x 0.5 m
-at dx = 0.008 m, 4x2 cuboid buildings (0.1 x 0.1 x 0.15 m, 0.2 m spacing)
-carried by IBM markers, log-law inflow (u* = 0.3 m/s, z0 = 0.001 m), MOST
-ground layer with Obukhov length L = +50 m, Smagorinsky LES (Cs = 0.12),
-dt = 5e-5 s, BiCGSTAB PPE at relative tolerance 1e-6.  The same steps run in
-FP64 and in MIXED precision; ``value`` is the MIXED throughput.
+Headline workload: the urban street canyon at 2M particles -- domain
+2.0 x 1.0 x 0.5 m at dx = 0.008 m, 4x2 cuboid buildings (0.1 x 0.1 x 0.15 m,
+0.2 m spacing) carried by IBM markers, log-law inflow (u* = 0.3 m/s,
+z0 = 0.001 m), MOST ground layer with Obukhov length L = +50 m, Smagorinsky
+LES (Cs = 0.12), dt = 5e-5 s, BiCGSTAB PPE at relative tolerance 1e-6.  The
+same steps run in FP64 and in MIXED precision; ``value`` is the MIXED
+throughput.
 
 particle-steps/s = n * K / wall time of K timed steps (after W warm-up steps,
-the first of which runs the neighbour search), state resident in HBM.
+the first of which runs the neighbour search and the next four of which time
+the autotuner's candidate workgroup sizes), state resident in HBM.
+
+Two more legs ride along: a batched ensemble (64 canyon cases of 2e4..3e5
+particles stepped as one launch sequence, against the same cases stepped one
+at a time) and a serving run (a seeded Poisson stream of canyon requests
+through the continuous-batching server under a 240 GB budget), which gives
+the p50/p99 case latency.
 """
 import argparse
 import json
@@ -35,6 +43,7 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
     prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=precision)
     case = urbansph.Case(prm, len(P))
     case.upload(P, None, nrm)
+    log(f"[bench]   uploaded n={len(P)}")
     count = 0
     t_setup = time.perf_counter()
     for _ in range(warmup):
@@ -65,7 +74,8 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
     rep = case.ppe_report()
     out = dict(value=len(P) * steps / wall, ms_per_step=wall / steps * 1e3, lat=lat, iters=iters,
                prof=prof, prof_steps=prof_steps, rel_res=rep.rel_residual, refinements=rep.refinements,
-               warmup_s=t_setup, dev_bytes=case.device_bytes(), pres=case.field("PRES"), ux=case.field("UX"))
+               warmup_s=t_setup, dev_bytes=case.device_bytes(), pres=case.field("PRES"), ux=case.field("UX"),
+               tuning=case.tuning())
     case.close()
     return out
 
@@ -81,6 +91,107 @@ def measured_traffic(precision):
     key = "mixed" if precision == "mixed" else "fp64"
     m = re.search(r"traffic_bytes_per_launch_%s_spmv: (\d+)" % key, open(path).read())
     return int(m.group(1)) if m else None
+
+
+def run_batched(cases, abi, urbansph, steps, warmup, dt, tol, n_cases=64):
+    """The 64-case ensemble (counts log-uniform in [2e4, 3e5], inflow scale
+    U(0.5, 2), building height U(0.05, 0.2), seed 7) stepped as one batch,
+    then the same cases stepped one at a time with sph_step."""
+    ens = cases.ensemble(n_cases=n_cases)
+    ntot = sum(len(e[0]) for e in ens)
+    log(f"[bench] batched: {n_cases} cases, {ntot} particles")
+    cs = []
+    for P, nrm, geom, info in ens:
+        prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED)
+        c = urbansph.Case(prm, len(P))
+        c.upload(P, None, nrm)
+        cs.append(c)
+    b = urbansph.Batch()
+    k = len(cs)
+    count = 0
+    for _ in range(warmup):
+        b.step(cs, [count] * k, [dt] * k)
+        b.sync()
+        count += 1
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step(cs, [count] * k, [dt] * k)
+        count += 1
+    b.sync()
+    wall = time.perf_counter() - t0
+    iters = [c.ppe_report().iterations for c in cs]
+    out = dict(value=ntot * steps / wall, ms_per_step=wall / steps * 1e3, particles=ntot, cases=k,
+               tuning=b.tuning(), ppe_iters_max=max(iters), ppe_iters_min=min(iters),
+               dev_bytes=sum(c.device_bytes() for c in cs))
+    log(f"[bench] batched: {out['value'] / 1e6:.2f} M particle-steps/s, {out['ms_per_step']:.1f} ms/step")
+    # the same cases one at a time (their own engines; block size 256)
+    seq_steps = 2
+    for c in cs:
+        c.options(graphs=True, autotune=False)
+    for c in cs:
+        c.step(count, dt)
+    for c in cs:
+        c.sync()
+    count += 1
+    t0 = time.perf_counter()
+    for _ in range(seq_steps):
+        for c in cs:
+            c.step(count, dt)
+            c.sync()
+        count += 1
+    wall = time.perf_counter() - t0
+    out["sequential_value"] = ntot * seq_steps / wall
+    out["speedup_vs_sequential"] = out["value"] / out["sequential_value"]
+    log(f"[bench] sequential: {out['sequential_value'] / 1e6:.2f} M particle-steps/s")
+    b.close()
+    for c in cs:
+        c.close()
+    return out
+
+
+def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=24, rate=4.0):
+    """A seeded Poisson stream of canyon requests through the server: wall
+    clock, requests submitted at their arrival times, one co-batched step per
+    tick.  Latency = completion - arrival."""
+    reqs = cases.serving_requests(n_req=n_req, rate=rate)
+    decks = []
+    for r in reqs:
+        P, nrm, geom = cases.sized_canyon(r["target"], r["u_scale"], r["bh"])
+        prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED, L=r["L"])
+        decks.append((P, nrm, prm))
+    log(f"[bench] serving: {n_req} requests at {rate}/s, sizes {min(len(d[0]) for d in decks)}.."
+        f"{max(len(d[0]) for d in decks)}, steps {min(r['steps'] for r in reqs)}..{max(r['steps'] for r in reqs)}")
+    srv = urbansph.Server(budget_bytes=int(budget), max_cases=256)
+    t0 = srv.clock()
+    ids = []
+    nxt = 0
+    max_running = 0
+    while nxt < len(reqs) or srv.status().running or srv.status().queued:
+        now = srv.clock() - t0
+        while nxt < len(reqs) and reqs[nxt]["t_arrival"] <= now:
+            P, nrm, prm = decks[nxt]
+            ids.append(srv.submit(prm, P, reqs[nxt]["steps"], dt, normals=nrm, t_arrival=t0 + reqs[nxt]["t_arrival"]))
+            nxt += 1
+        st = srv.status()
+        if not st.running and not st.queued:
+            time.sleep(max(0.0, reqs[nxt]["t_arrival"] - (srv.clock() - t0)))
+            continue
+        srv.tick()
+        max_running = max(max_running, srv.status().running)
+    infos = [srv.request(i) for i in ids]
+    lat = np.array([i.t_finish - i.t_arrival for i in infos])
+    wait = np.array([i.t_admit - i.t_arrival for i in infos])
+    st = srv.status()
+    span = max(i.t_finish for i in infos) - (t0 + reqs[0]["t_arrival"])
+    out = dict(requests=n_req, rate_per_s=rate, failed=sum(i.state != abi.REQ_DONE for i in infos),
+               p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+               admit_wait_p99_ms=float(np.percentile(wait, 99) * 1e3),
+               value=st.particle_steps / span, ticks=st.ticks, max_running=max_running,
+               budget_bytes=int(budget))
+    log(f"[bench] serving: p50 {out['p50_ms']:.0f} ms, p99 {out['p99_ms']:.0f} ms, "
+        f"{out['value'] / 1e6:.2f} M particle-steps/s over {span:.1f} s, max {max_running} running")
+    srv.close()
+    return out
 
 
 def cpu_baseline(cases, abi, dt, tol, max_seconds=30.0):
@@ -112,11 +223,15 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--dx", type=float, default=0.008)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--dt", type=float, default=5e-5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a 1 GiB device-to-device copy kernel (for scaling HBM counters)")
@@ -171,7 +286,6 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(main_name),
                 "kernel": "spmv", "avg_us": round(avg_s * 1e6, 2),
                 "alg_bytes_per_launch": round(byts / launches)}
-    tot_ms = sum(v[1] for v in m["prof"].values())
     breakdown = {k: round(v[1] / m["prof_steps"], 3) for k, v in sorted(m["prof"].items(), key=lambda kv: -kv[1][1])}
     lat = np.array(m["lat"])
     line = {
@@ -190,8 +304,10 @@ def main():
         "config": {"workload": f"urban street canyon 2.0x1.0x0.5 m, dx={args.dx} m, 4x2 IBM buildings, "
                                f"MOST L=+50 m, SSM Cs=0.12, dt={args.dt} s, BiCGSTAB tol {args.tol:g}",
                    "particles": n, "precision": main_name},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
-                       "what": "per-step latency of one resident case"},
+        "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                            "p99": round(float(np.percentile(lat, 99)), 3),
+                            "what": "per-step latency of the resident 2M case"},
+        "tuning": m["tuning"],
         "ppe_iters_per_step": m["iters"],
         "kernel_ms_per_step": breakdown,
         "roofline": roof,
@@ -206,6 +322,29 @@ def main():
                 "pres": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
                 "ux": float(np.linalg.norm(res["mixed"]["ux"] - res["fp64"]["ux"]) / np.linalg.norm(res["fp64"]["ux"])),
                 "note": "after warmup+steps+2 steps of each run"}
+    if not args.no_batch:
+        bt = run_batched(cases, abi, urbansph, args.batch_steps, 5, args.dt, args.tol)
+        line["batched"] = {
+            "workload": "64 canyon cases, particle counts log-uniform in [2e4, 3e5] (seed 7), inflow scale U(0.5, 2), "
+                        "building height U(0.05, 0.2), dx=0.008 m, mixed precision, one launch sequence per step",
+            "particles": bt["particles"], "value": round(bt["value"], 1), "unit": "particle-steps/s",
+            "ms_per_step": round(bt["ms_per_step"], 3), "steps": args.batch_steps, "warmup": 5,
+            "sequential_value": round(bt["sequential_value"], 1),
+            "speedup_vs_sequential": round(bt["speedup_vs_sequential"], 3),
+            "ppe_iters_range": [bt["ppe_iters_min"], bt["ppe_iters_max"]], "tuning": bt["tuning"],
+            "device_bytes": bt["dev_bytes"]}
+    if not args.no_serve:
+        sv = run_serving(cases, abi, urbansph, args.dt, args.tol, n_req=args.serve_requests)
+        line["latency_ms"] = {"p50": round(sv["p50_ms"], 1), "p99": round(sv["p99_ms"], 1),
+                              "what": "serving: completion - arrival per canyon request"}
+        line["serving"] = {
+            "workload": f"{sv['requests']} canyon requests, Poisson arrivals {sv['rate_per_s']}/s, counts log-uniform "
+                        "in [1e4, 2e5], 10..40 steps, L in {-50, +50, inf} m, mixed precision, 240 GB budget "
+                        "(sizes and step counts scaled down 10x from the 2e6-particle / 500-step request mix "
+                        "so that the offered load stays below one GPU's capacity)",
+            "value": round(sv["value"], 1), "unit": "particle-steps/s", "p50_ms": round(sv["p50_ms"], 1),
+            "p99_ms": round(sv["p99_ms"], 1), "admit_wait_p99_ms": round(sv["admit_wait_p99_ms"], 1),
+            "ticks": sv["ticks"], "max_running": sv["max_running"], "failed": sv["failed"]}
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(cases, abi, args.dt, args.tol)

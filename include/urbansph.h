@@ -231,10 +231,26 @@ int sph_upload(sph_ctx* c, const sph_part1* P1, const sph_part2* P2,
                const float* ib_normals, int n);
 
 /* One projection step == SOPHIA_single_ISPH (ISPH_Calc.cuh:4-233) for the
- * given step counter, time-step and time.  Asynchronous: returns after
- * enqueueing (PPE convergence is checked at the next synchronising call
- * unless the solve itself needs host decisions, which BiCGSTAB does). */
+ * given step counter, time-step and time.  Asynchronous apart from the
+ * BiCGSTAB convergence polls: returns after enqueueing the last pass.
+ * Returns SPH_E_NOCONV, after completing the step, when the PPE solve did
+ * not reach ppe_tol within ppe_maxit (the reference checks nothing). */
 int sph_step(sph_ctx* c, int count, double dt, double time);
+
+/* Options of the context's step engine.  use_graphs: replay each BiCGSTAB
+ * iteration as one captured HIP graph (default 1).  autotune: run the
+ * neighbour passes at 64/128/256/512 threads per workgroup on the first
+ * four steps of each size class and keep the fastest (default 1).  Neither
+ * changes any result. */
+int sph_set_engine_options(sph_ctx* c, int use_graphs, int autotune);
+
+/* Workgroup sizes the autotuner picked: one row of 1 + SPH_TUNE_PASSES ints
+ * per size class (floor(log2(particles in the step)), then the size of each
+ * tunable pass: grad_corr, prep, boundary3d, neumann, advection, freeslip,
+ * pressure_force, ibm_force, ibm_spread, ppe_assemble; 0 while that class is
+ * still being timed).  Returns the number of rows written (<= max_rows). */
+#define SPH_TUNE_PASSES 10
+int sph_tuning(const sph_ctx* c, int* out, int max_rows);
 
 /* Device -> host AoS, in the device (cell-sorted) order.  Any pointer may be
  * NULL.  Doubles are rounded to float. */
@@ -287,6 +303,87 @@ long long sph_device_bytes(const sph_ctx* c);
 
 const char* sph_last_error(const sph_ctx* c);
 void sph_destroy(sph_ctx* c);
+
+/* ---- Batches: independent cases stepped as one launch sequence ----------
+ * No counterpart in the reference, which runs one case per GPU thread
+ * (ISPH.cuh:159-162).  A batch steps contexts made by sph_create together:
+ * every pass is one launch over the particles of all listed cases, the
+ * Krylov solves of all cases share their launches, and each case keeps its
+ * own scalars, tolerance and convergence.  A case's results are bitwise the
+ * same as when it is stepped alone with sph_step.  A context belongs to no
+ * batch: any subset of contexts may be stepped together, one batch step at
+ * a time per context. */
+typedef struct sph_batch sph_batch;
+int sph_batch_create(sph_batch** out);
+
+/* One step of each listed case with its own count, dt and time.  The cases
+ * must share precision and ppe_solver.  Returns 0, the first error, or
+ * SPH_E_NOCONV after completing the step when a case's PPE did not converge
+ * (sph_ppe_info of each case says which). */
+int sph_batch_step(sph_batch* b, sph_ctx* const* cases, int ncases, const int* count,
+                   const double* dt, const double* time);
+int sph_batch_synchronize(sph_batch* b);
+int sph_batch_set_options(sph_batch* b, int use_graphs, int autotune);
+int sph_batch_profile_enable(sph_batch* b, int enable);
+int sph_batch_profile_report(sph_batch* b, sph_kernel_time* out, int max);
+int sph_batch_tuning(const sph_batch* b, int* out, int max_rows);
+const char* sph_batch_last_error(const sph_batch* b);
+void sph_batch_destroy(sph_batch* b);
+
+/* ---- Serving: continuous batching of scenario requests ------------------
+ * No counterpart in the reference (its ISPH_Calc runs one deck to the end).
+ * Requests (a particle deck, a step count, dt) are queued on submit,
+ * admitted in arrival order while their estimated device memory fits the
+ * HBM budget and fewer than max_cases run, kept resident, and advanced one
+ * co-batched step per tick (sph_batch_step over every running case).  A
+ * finished request frees its memory in the same tick, so queued requests
+ * join at the next one.  Every case computes bitwise what it computes alone.
+ * All requests of one server share precision and ppe_solver.
+ * dry_run_rate > 0: no GPU is used; each tick advances a simulated clock by
+ * (particles of the running cases) / dry_run_rate seconds, for testing and
+ * capacity planning of the admission policy. */
+enum { SPH_REQ_QUEUED = 0, SPH_REQ_RUNNING = 1, SPH_REQ_DONE = 2, SPH_REQ_FAILED = 3 };
+typedef struct sph_server sph_server;
+typedef struct sph_server_info {
+    int queued, running, finished;
+    long long reserved_bytes;       /* device memory held for running cases  */
+    long long budget_bytes;
+    long long ticks;                /* co-batched steps so far               */
+    long long particle_steps;       /* sum over ticks of running particles   */
+    double clock;                   /* seconds since creation (or simulated) */
+} sph_server_info;
+typedef struct sph_request_info {
+    int state;                      /* SPH_REQ_*                             */
+    int rc;                         /* 0 or the error that failed it         */
+    int n, steps, steps_done, noconv_steps;
+    long long ppe_iterations;       /* summed over its steps                 */
+    long long est_bytes;            /* admission estimate                    */
+    double t_arrival, t_admit, t_finish;   /* server clock; -1 = not yet     */
+} sph_request_info;
+
+/* Upper bound of the device memory of one case, used for admission; the
+ * measured footprint replaces it after the case's first step. */
+long long sph_estimate_bytes(const sph_params* prm, int n);
+int sph_server_create(long long hbm_budget_bytes, int max_cases, double dry_run_rate, sph_server** out);
+/* Copies the deck; t_arrival < 0 means now (server clock). */
+int sph_server_submit(sph_server* s, const sph_params* prm, const sph_part1* P1, const float* ib_normals,
+                      int n, int steps, double dt, double t_arrival, long long* id_out);
+/* Admit what fits, run one co-batched step, retire finished requests (their
+ * ids go to finished[], up to max_finished; *n_finished counts all). */
+int sph_server_tick(sph_server* s, long long* finished, int max_finished, int* n_finished);
+/* Dry run only: move the simulated clock forward to t (idle time). */
+int sph_server_idle_until(sph_server* s, double t);
+double sph_server_clock(const sph_server* s);
+int sph_server_status(const sph_server* s, sph_server_info* info);
+int sph_server_request(const sph_server* s, long long id, sph_request_info* info);
+/* Final FP64 field of a finished request in input order: SPH_F_UX, _UY, _UZ
+ * or _PRES. */
+int sph_server_result(const sph_server* s, long long id, int field, double* out, int n);
+/* Final state of a finished request as reference records, input order. */
+int sph_server_result_part1(const sph_server* s, long long id, sph_part1* out, int n);
+int sph_server_release(sph_server* s, long long id);
+const char* sph_server_last_error(const sph_server* s);
+void sph_server_destroy(sph_server* s);
 
 #ifdef __cplusplus
 }

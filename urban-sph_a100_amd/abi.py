@@ -117,6 +117,22 @@ class KernelTime(C.Structure):
                 ("total_ms", C.c_double), ("alg_bytes", C.c_double)]
 
 
+class ServerInfo(C.Structure):
+    _fields_ = [("queued", C.c_int), ("running", C.c_int), ("finished", C.c_int),
+                ("reserved_bytes", C.c_longlong), ("budget_bytes", C.c_longlong),
+                ("ticks", C.c_longlong), ("particle_steps", C.c_longlong), ("clock", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("rc", C.c_int), ("n", C.c_int), ("steps", C.c_int),
+                ("steps_done", C.c_int), ("noconv_steps", C.c_int), ("ppe_iterations", C.c_longlong),
+                ("est_bytes", C.c_longlong), ("t_arrival", C.c_double), ("t_admit", C.c_double),
+                ("t_finish", C.c_double)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
 def ptr(a):
     """Raw data pointer of a contiguous numpy array (or None)."""
     if a is None:

@@ -226,6 +226,17 @@ def small_canyon(dx=0.01, **kw):
     return canyon(**args)
 
 
+def sized_canyon(target, u_scale=1.0, bh=0.15, dx=0.008):
+    """A one-building canyon whose domain length is chosen so that it holds
+    about ``target`` particles at spacing dx (cross-section 0.24 m wide,
+    0.06 m of air above the roof)."""
+    Ly, Lz = 0.24, max(0.16, round((bh + 0.06) / dx) * dx)
+    per_x = (Ly / dx) * (Lz / dx + 6)
+    Lx = max(0.2, round(target / per_x) * dx)
+    return canyon(dx=dx, Lx=Lx, Ly=Ly, Lz=Lz, buildings=[(0.5 * Lx, 0.5 * Ly)],
+                  bsize=(0.05, 0.05, bh), u_scale=u_scale / 4.0)
+
+
 def ensemble(n_cases=64, lo=2e4, hi=3e5, seed=7, dx=0.008):
     """Independent canyon cases of uneven size: particle counts log-uniform in
     [lo, hi], inflow scale U(0.5, 2.0) and building height U(0.05, 0.2), all
@@ -237,10 +248,27 @@ def ensemble(n_cases=64, lo=2e4, hi=3e5, seed=7, dx=0.008):
         target = math.exp(math.log(lo) + r[c, 0] * (math.log(hi) - math.log(lo)))
         uscale = 0.5 + 1.5 * r[c, 1]
         bh = 0.05 + 0.15 * r[c, 2]
-        Ly, Lz = 0.24, max(0.16, round((bh + 0.06) / dx) * dx)
-        per_x = (Ly / dx) * (Lz / dx + 6)
-        Lx = max(0.2, round(target / per_x) * dx)
-        p, nrm, g = canyon(dx=dx, Lx=Lx, Ly=Ly, Lz=Lz, buildings=[(0.5 * Lx, 0.5 * Ly)],
-                           bsize=(0.05, 0.05, bh), u_scale=uscale / 4.0)
+        p, nrm, g = sized_canyon(target, uscale, bh, dx)
         out.append((p, nrm, g, dict(target=target, u_scale=uscale, bh=bh)))
+    return out
+
+
+STABILITY_L = (-50.0, 50.0, 1.0e6)   # unstable, stable, neutral (|L| -> inf)
+
+
+def serving_requests(n_req=24, rate=4.0, lo=1e4, hi=2e5, steps_lo=10, steps_hi=40, seed=9):
+    """A Poisson stream of canyon scenario requests: exponential
+    inter-arrival times at ``rate`` per second, particle counts log-uniform in
+    [lo, hi], step counts uniform in [steps_lo, steps_hi], inflow scale
+    U(0.5, 2.0) and Obukhov length drawn from STABILITY_L, all from one seed.
+    Returns a list of dicts (t_arrival, target, steps, u_scale, L, bh)."""
+    r = uniform(seed, 6 * n_req).reshape(n_req, 6)
+    t = 0.0
+    out = []
+    for k in range(n_req):
+        t += -math.log(1.0 - r[k, 0]) / rate
+        target = math.exp(math.log(lo) + r[k, 1] * (math.log(hi) - math.log(lo)))
+        steps = steps_lo + int(r[k, 2] * (steps_hi - steps_lo + 1))
+        out.append(dict(t_arrival=t, target=target, steps=min(steps, steps_hi), u_scale=0.5 + 1.5 * r[k, 3],
+                        L=STABILITY_L[min(2, int(3 * r[k, 4]))], bh=0.05 + 0.15 * r[k, 5]))
     return out

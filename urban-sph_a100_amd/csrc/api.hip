@@ -13,11 +13,6 @@
 
 namespace sph {
 
-int fail(sph_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
-    return code;
-}
-
 int dmalloc(sph_ctx* c, void** p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) bytes = 8;
@@ -35,50 +30,6 @@ void dfree(sph_ctx* c, void* p) {
     size_t sz = 0;
     if (hipMemPtrGetInfo(p, &sz) == hipSuccess) c->bytes -= (long long)sz;
     (void)hipFree(p);
-}
-
-int launch_check(sph_ctx* c, const char* what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(c, SPH_E_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
-    return 0;
-}
-
-int prof_begin(sph_ctx* c, const char* name, double alg_bytes) {
-    if (!c->prof.on) return 0;
-    int rec = -1;
-    for (size_t k = 0; k < c->prof.recs.size(); k++)
-        if (c->prof.recs[k].name == name) { rec = (int)k; break; }
-    if (rec < 0) {
-        c->prof.recs.push_back(Profile::Rec{name, 0, 0.0, 0.0});
-        rec = (int)c->prof.recs.size() - 1;
-    }
-    Profile::Pending p{rec, nullptr, nullptr, alg_bytes};
-    SPH_HIP(c, hipEventCreate(&p.a));
-    SPH_HIP(c, hipEventCreate(&p.b));
-    SPH_HIP(c, hipEventRecord(p.a, c->stream));
-    c->prof.pending.push_back(p);
-    return 0;
-}
-int prof_end(sph_ctx* c) {
-    if (!c->prof.on) return 0;
-    SPH_HIP(c, hipEventRecord(c->prof.pending.back().b, c->stream));
-    return 0;
-}
-static int prof_collect(sph_ctx* c) {
-    if (c->prof.pending.empty()) return 0;
-    SPH_HIP(c, hipStreamSynchronize(c->stream));
-    for (auto& p : c->prof.pending) {
-        float ms = 0.f;
-        SPH_HIP(c, hipEventElapsedTime(&ms, p.a, p.b));
-        auto& r = c->prof.recs[p.rec];
-        r.launches++;
-        r.ms += ms;
-        r.bytes += p.bytes;
-        (void)hipEventDestroy(p.a);
-        (void)hipEventDestroy(p.b);
-    }
-    c->prof.pending.clear();
-    return 0;
 }
 
 // AoS (reference layout) -> SoA (same order)
@@ -151,8 +102,6 @@ __global__ void __launch_bounds__(BLOCK) k_pack(int n, PView P, P2View S2, QView
         C[i] = q;
     }
 }
-
-int set_ncell_run(sph_ctx* c);   // nnps.hip
 
 }  // namespace sph
 
@@ -258,7 +207,11 @@ int sph_create(const sph_params* prm, int n_part2, sph_ctx** out) {
     c->prm = *prm;
     c->cap = n_part2;
     c->c_pi15 = std::pow(3.141592653, 1.5);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SPH_E_HIP; }
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SPH_E_HIP; }
+    c->stream = c->own_stream;
+    c->solo = new Eng();
+    c->profp = &c->solo->prof;
+    if (eng_init(*c->solo, c->own_stream) != 0) { sph_destroy(c); return SPH_E_HIP; }
     const size_t N = (size_t)n_part2;
     auto A = [&](double** p) { return dalloc(c, p, N); };
     int e = 0;
@@ -275,9 +228,7 @@ int sph_create(const sph_params* prm, int n_part2, sph_ctx** out) {
     for (int f = 0; f < 10; f++) e = e ? e : A(&c->s2[f]);
     for (int f = 0; f < 18; f++) e = e ? e : A(&c->q[f]);
     for (int f = 0; f < 5; f++) e = e ? e : A(&c->snap[f]);
-    e = e ? e : dalloc(c, &c->dscal, 64);
     e = e ? e : dalloc(c, &c->stat_buf, 16);
-    if (!e && hipHostMalloc((void**)&c->hscal, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) e = SPH_E_NOMEM;
     if (e) { sph_destroy(c); return e; }
     *out = c;
     return 0;
@@ -348,30 +299,49 @@ int sph_upload(sph_ctx* c, const sph_part1* P1, const sph_part2* P2, const float
     ppe_free(c);
     c->uploaded = true;
     c->searched = false;
+    c->generation++;
     return 0;
 }
 
 int sph_step(sph_ctx* c, int count, double dt, double time) {
     if (!c) return SPH_E_ARG;
-    if (!c->uploaded) return fail(c, SPH_E_STATE, "sph_step before sph_upload");
-    if (count < 0) return fail(c, SPH_E_ARG, "count < 0");
-    if (count > 0 && !c->searched) return fail(c, SPH_E_STATE, "the first step after sph_upload must have count == 0");
-    if (!(dt > 0)) return fail(c, SPH_E_ARG, "dt must be > 0");
-    SPH_TRY(set_ncell_run(c));
-    if (count == 0) {
-        SPH_TRY(nnps_run(c));
-        ppe_free(c);
-    }
-    SPH_TRY(step_physics_front(c, count, dt, time));
-    SPH_TRY(ppe_solve(c, dt));
-    SPH_TRY(step_physics_back(c, count, dt, time));
-    return 0;
+    sph_ctx* one[1] = {c};
+    int rc = engine_step(*c->solo, one, 1, &count, &dt, &time);
+    if (rc && !c->solo->err.empty()) c->err = c->solo->err;
+    return rc;
 }
 
 int sph_synchronize(sph_ctx* c) {
     if (!c) return SPH_E_ARG;
     SPH_HIP(c, hipStreamSynchronize(c->stream));
-    return prof_collect(c);
+    int rc = prof_collect(*c->solo);
+    if (rc) c->err = c->solo->err;
+    return rc;
+}
+
+int sph_set_engine_options(sph_ctx* c, int use_graphs, int autotune) {
+    if (!c) return SPH_E_ARG;
+    c->solo->use_graphs = use_graphs != 0;
+    c->solo->autotune = autotune != 0;
+    return 0;
+}
+
+static int tuning_rows(const Eng& e, int* out, int max_rows) {
+    if (!out || max_rows <= 0) return 0;
+    int k = 0;
+    for (auto& t : e.tune) {
+        if (k >= max_rows) break;
+        int* row = out + (size_t)k * (1 + TK_COUNT);
+        row[0] = t.first;
+        for (int q = 0; q < TK_COUNT; q++) row[1 + q] = t.second.phase >= TUNE_CANDIDATES ? t.second.block[q] : 0;
+        k++;
+    }
+    return k;
+}
+
+int sph_tuning(const sph_ctx* c, int* out, int max_rows) {
+    if (!c) return SPH_E_ARG;
+    return tuning_rows(*c->solo, out, max_rows);
 }
 
 int sph_download(sph_ctx* c, sph_part1* P1, sph_part2* P2, sph_part3* P3, int n) {
@@ -477,19 +447,17 @@ int sph_stats(sph_ctx* c, float* umax, float* drho_pct_max, float* fmax_) {
     return stats_run(c, umax, drho_pct_max, fmax_);
 }
 
-int sph_profile_enable(sph_ctx* c, int enable) {
-    if (!c) return SPH_E_ARG;
-    if (!enable) SPH_TRY(prof_collect(c));
-    c->prof.on = enable != 0;
-    if (enable) c->prof.recs.clear();
+static int profile_enable(Eng& e, int enable) {
+    if (!enable) SPH_TRY(prof_collect(e));
+    e.prof.on = enable != 0;
+    if (enable) e.prof.recs.clear();
     return 0;
 }
-
-int sph_profile_report(sph_ctx* c, sph_kernel_time* out, int max) {
-    if (!c) return SPH_E_ARG;
-    SPH_TRY(prof_collect(c));
+static int profile_report(Eng& e, sph_kernel_time* out, int max) {
+    SPH_HIP(&e, hipStreamSynchronize(e.st));
+    SPH_TRY(prof_collect(e));
     int k = 0;
-    for (auto& r : c->prof.recs) {
+    for (auto& r : e.prof.recs) {
         if (k >= max || !out) break;
         sph_kernel_time& t = out[k++];
         std::memset(&t, 0, sizeof(t));
@@ -501,14 +469,82 @@ int sph_profile_report(sph_ctx* c, sph_kernel_time* out, int max) {
     return k;
 }
 
+int sph_profile_enable(sph_ctx* c, int enable) {
+    if (!c) return SPH_E_ARG;
+    return profile_enable(*c->solo, enable);
+}
+
+int sph_profile_report(sph_ctx* c, sph_kernel_time* out, int max) {
+    if (!c) return SPH_E_ARG;
+    return profile_report(*c->solo, out, max);
+}
+
 long long sph_device_bytes(const sph_ctx* c) { return c ? c->bytes : 0; }
+
+// ------------------------------------------------------------ batches ----
+struct sph_batch {
+    Eng e;
+};
+
+int sph_batch_create(sph_batch** out) {
+    if (!out) return SPH_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SPH_E_HIP;
+    sph_batch* b = new sph_batch();
+    if (eng_init(b->e, nullptr) != 0) { eng_free(b->e); delete b; return SPH_E_HIP; }
+    *out = b;
+    return 0;
+}
+
+int sph_batch_step(sph_batch* b, sph_ctx* const* cases, int ncases, const int* count, const double* dt,
+                   const double* time) {
+    if (!b) return SPH_E_ARG;
+    return engine_step(b->e, cases, ncases, count, dt, time);
+}
+
+int sph_batch_synchronize(sph_batch* b) {
+    if (!b) return SPH_E_ARG;
+    SPH_HIP(&b->e, hipStreamSynchronize(b->e.st));
+    return prof_collect(b->e);
+}
+
+int sph_batch_set_options(sph_batch* b, int use_graphs, int autotune) {
+    if (!b) return SPH_E_ARG;
+    b->e.use_graphs = use_graphs != 0;
+    b->e.autotune = autotune != 0;
+    return 0;
+}
+
+int sph_batch_profile_enable(sph_batch* b, int enable) {
+    if (!b) return SPH_E_ARG;
+    return profile_enable(b->e, enable);
+}
+
+int sph_batch_profile_report(sph_batch* b, sph_kernel_time* out, int max) {
+    if (!b) return SPH_E_ARG;
+    return profile_report(b->e, out, max);
+}
+
+int sph_batch_tuning(const sph_batch* b, int* out, int max_rows) {
+    if (!b) return SPH_E_ARG;
+    return tuning_rows(b->e, out, max_rows);
+}
+
+const char* sph_batch_last_error(const sph_batch* b) { return b ? b->e.err.c_str() : "null batch"; }
+
+void sph_batch_destroy(sph_batch* b) {
+    if (!b) return;
+    eng_free(b->e);
+    delete b;
+}
 
 const char* sph_last_error(const sph_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 void sph_destroy(sph_ctx* c) {
     if (!c) return;
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto& p : c->prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+    if (c->solo) { eng_free(*c->solo); delete c->solo; c->solo = nullptr; }
     ppe_free(c);
     for (auto p : c->pf) dfree(c, p);
     for (auto p : c->pi) dfree(c, p);
@@ -518,10 +554,9 @@ void sph_destroy(sph_ctx* c) {
     for (auto p : c->q) dfree(c, p);
     for (auto p : c->snap) dfree(c, p);
     void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->nl_slice_off, c->nl_cnt,
-                    c->nl_ent, c->dscal, c->stat_buf};
+                    c->nl_ent, c->stat_buf};
     for (auto p : rest) dfree(c, p);
-    if (c->hscal) (void)hipHostFree(c->hscal);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
 

@@ -1,0 +1,405 @@
+// Step engine of the MI355X ISPH library: the projection step of one case
+// or of many independent cases as one launch sequence.
+//
+// Every pass is launched once over the particles (or PPE rows) of all cases
+// of the step.  A segment table maps each workgroup to (case, first index);
+// segments start at multiples of the workgroup size, so no workgroup mixes
+// two cases and a case's arithmetic, its reductions included, is the same
+// alone and in any batch.  Case descriptors (buffer pointers, parameters,
+// count / dt / time) go to the device once per step from pinned staging
+// memory; the Krylov iterations read their scalars and controls from device
+// memory, which lets one captured HIP graph replay any iteration.
+//
+// Autotuner: the neighbour passes run at 64, 128, 256 and 512 threads per
+// workgroup on four consecutive steps of each batch size class (log2 of the
+// batch's particle count), timed with HIP events; afterwards each pass uses
+// the size that was fastest per call.  One thread per particle and no
+// cross-thread arithmetic in those passes: the choice never changes results.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "sph_internal.h"
+
+namespace sph {
+
+// ---------------------------------------------------------------- profile
+static int prof_rec(Profile& P, const char* name) {
+    for (size_t k = 0; k < P.recs.size(); k++)
+        if (P.recs[k].name == name) return (int)k;
+    P.recs.push_back(Profile::Rec{name, 0, 0.0, 0.0});
+    return (int)P.recs.size() - 1;
+}
+template <class T>
+static int prof_begin_on(T* owner, Profile& P, hipStream_t st, const char* name, double bytes) {
+    Profile::Pending p{prof_rec(P, name), nullptr, nullptr, bytes};
+    SPH_HIP(owner, hipEventCreate(&p.a));
+    SPH_HIP(owner, hipEventCreate(&p.b));
+    SPH_HIP(owner, hipEventRecord(p.a, st));
+    P.pending.push_back(p);
+    return 0;
+}
+int prof_begin(Eng& e, const char* name, double bytes) {
+    if (!e.prof.on) return 0;
+    return prof_begin_on(&e, e.prof, e.st, name, bytes);
+}
+int prof_end(Eng& e) {
+    if (!e.prof.on) return 0;
+    SPH_HIP(&e, hipEventRecord(e.prof.pending.back().b, e.st));
+    return 0;
+}
+int prof_begin(sph_ctx* c, const char* name, double bytes) {
+    if (!c->profp || !c->profp->on) return 0;
+    return prof_begin_on(c, *c->profp, c->stream, name, bytes);
+}
+int prof_end(sph_ctx* c) {
+    if (!c->profp || !c->profp->on) return 0;
+    SPH_HIP(c, hipEventRecord(c->profp->pending.back().b, c->stream));
+    return 0;
+}
+int prof_collect(Eng& e) {
+    for (auto& p : e.prof.pending) {
+        float ms = 0.f;
+        SPH_HIP(&e, hipEventSynchronize(p.b));
+        SPH_HIP(&e, hipEventElapsedTime(&ms, p.a, p.b));
+        auto& r = e.prof.recs[p.rec];
+        r.launches++;
+        r.ms += ms;
+        r.bytes += p.bytes;
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    e.prof.pending.clear();
+    return 0;
+}
+
+// ---------------------------------------------------------------- tuning
+static Eng::Tune& tune_for(Eng& e, int cls) {
+    for (auto& t : e.tune)
+        if (t.first == cls) return t.second;
+    Eng::Tune t{};
+    for (int k = 0; k < TK_COUNT; k++) t.block[k] = BLOCK;
+    e.tune.emplace_back(cls, t);
+    return e.tune.back().second;
+}
+int tuned_block(Eng& e, int kid) {
+    if (!e.autotune) return BLOCK;
+    Eng::Tune& t = tune_for(e, e.tune_class);
+    if (t.phase < TUNE_CANDIDATES) return TUNE_BLOCKS[t.phase];
+    return t.block[kid];
+}
+int tune_begin(Eng& e, int kid, int block) {
+    e.tune_open = false;
+    if (!e.autotune) return 0;
+    Eng::Tune& t = tune_for(e, e.tune_class);
+    if (t.phase >= TUNE_CANDIDATES || TUNE_BLOCKS[t.phase] != block) return 0;
+    Eng::TuneEv ev{e.tune_class, kid, t.phase, nullptr, nullptr};
+    SPH_HIP(&e, hipEventCreate(&ev.a));
+    SPH_HIP(&e, hipEventCreate(&ev.b));
+    SPH_HIP(&e, hipEventRecord(ev.a, e.st));
+    e.tune_ev.push_back(ev);
+    e.tune_open = true;
+    return 0;
+}
+int tune_end(Eng& e) {
+    if (!e.tune_open) return 0;
+    e.tune_open = false;
+    SPH_HIP(&e, hipEventRecord(e.tune_ev.back().b, e.st));
+    return 0;
+}
+// Fold the timings of the previous step into its size class; after the
+// last candidate, pick each pass's fastest size per call.
+static int tune_fold(Eng& e) {
+    if (e.tune_ev.empty()) return 0;
+    std::vector<int> classes;
+    for (auto& ev : e.tune_ev) {
+        float ms = 0.f;
+        SPH_HIP(&e, hipEventSynchronize(ev.b));
+        SPH_HIP(&e, hipEventElapsedTime(&ms, ev.a, ev.b));
+        Eng::Tune& t = tune_for(e, ev.cls);
+        t.ms[ev.kid][ev.cand] += ms;
+        t.calls[ev.kid][ev.cand]++;
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+        if (std::find(classes.begin(), classes.end(), ev.cls) == classes.end()) classes.push_back(ev.cls);
+    }
+    e.tune_ev.clear();
+    for (int cls : classes) {
+        Eng::Tune& t = tune_for(e, cls);
+        if (t.phase >= TUNE_CANDIDATES) continue;
+        if (++t.phase < TUNE_CANDIDATES) continue;
+        for (int k = 0; k < TK_COUNT; k++) {
+            int best = -1;
+            double best_ms = 0.0;
+            for (int q = 0; q < TUNE_CANDIDATES; q++) {
+                if (t.calls[k][q] == 0) continue;
+                double per = t.ms[k][q] / t.calls[k][q];
+                if (best < 0 || per < best_ms) { best = q; best_ms = per; }
+            }
+            t.block[k] = best < 0 ? BLOCK : TUNE_BLOCKS[best];
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- shapes
+int shape(Eng& e, int kind, int block, Shape* out) {
+    const int nc = (int)e.cs.size();
+    auto units = [&](int k) -> long long {
+        const sph_ctx* c = e.cs[k];
+        return kind == 0 ? c->n : ppe_view(c).nr;
+    };
+    out->block = block;
+    if (nc == 1) {
+        out->grid = (unsigned)((units(0) + block - 1) / block);
+        out->tab = nullptr;
+        return 0;
+    }
+    Eng::Tab* t = nullptr;
+    for (auto& x : e.tabs)
+        if (x.kind == kind && x.block == block) { t = &x; break; }
+    if (t && t->valid) {
+        out->grid = t->nblk;
+        out->tab = t->d;
+        return 0;
+    }
+    std::vector<int2> h;
+    for (int k = 0; k < nc; k++) {
+        const long long u = units(k);
+        for (long long f = 0; f < u; f += block) h.push_back(make_int2(k, (int)f));
+    }
+    if (!t) {
+        e.tabs.push_back(Eng::Tab{});
+        t = &e.tabs.back();
+        t->kind = kind;
+        t->block = block;
+    }
+    if (h.size() > t->cap) {
+        if (t->d) (void)hipFree(t->d);
+        t->d = nullptr;
+        t->cap = std::max<size_t>(h.size(), 2 * t->cap);
+        hipError_t er = hipMalloc(&t->d, sizeof(int2) * t->cap);
+        if (er != hipSuccess) {
+            t->d = nullptr;
+            t->cap = 0;
+            (void)hipGetLastError();
+            return fail(&e, SPH_E_NOMEM, "segment table allocation failed");
+        }
+    }
+    // a new table is used only by launches after this point on the engine's
+    // stream; tables of an older composition were drained before reuse
+    if (!h.empty()) SPH_HIP(&e, hipMemcpy(t->d, h.data(), sizeof(int2) * h.size(), hipMemcpyHostToDevice));
+    t->nblk = (unsigned)h.size();
+    t->valid = true;
+    out->grid = t->nblk;
+    out->tab = t->d;
+    return 0;
+}
+
+// ---------------------------------------------------------------- engine
+int eng_init(Eng& e, hipStream_t st) {
+    if (st) e.st = st;
+    else {
+        SPH_HIP(&e, hipStreamCreateWithFlags(&e.st, hipStreamNonBlocking));
+        e.own_stream = true;
+    }
+    SPH_HIP(&e, hipEventCreateWithFlags(&e.up_ev, hipEventDisableTiming));
+    SPH_HIP(&e, hipEventRecord(e.up_ev, e.st));
+    return 0;
+}
+
+static void drop_graphs(Eng& e) {
+    for (auto& g : e.iter_graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+void eng_free(Eng& e) {
+    if (e.st) (void)hipStreamSynchronize(e.st);
+    (void)prof_collect(e);
+    for (auto& ev : e.tune_ev) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
+    e.tune_ev.clear();
+    drop_graphs(e);
+    for (auto& t : e.tabs)
+        if (t.d) (void)hipFree(t.d);
+    e.tabs.clear();
+    if (e.d_desc) (void)hipFree(e.d_desc);
+    if (e.d_sc) (void)hipFree(e.d_sc);
+    if (e.d_ctl) (void)hipFree(e.d_ctl);
+    if (e.h_desc) (void)hipHostFree(e.h_desc);
+    if (e.h_sc) (void)hipHostFree(e.h_sc);
+    if (e.h_ctl) (void)hipHostFree(e.h_ctl);
+    e.d_desc = nullptr; e.d_sc = nullptr; e.d_ctl = nullptr;
+    e.h_desc = nullptr; e.h_sc = nullptr; e.h_ctl = nullptr;
+    e.cap = 0;
+    if (e.up_ev) (void)hipEventDestroy(e.up_ev);
+    e.up_ev = nullptr;
+    if (e.own_stream && e.st) (void)hipStreamDestroy(e.st);
+    e.st = nullptr;
+}
+
+static int eng_reserve(Eng& e, int nc) {
+    if (nc <= e.cap) return 0;
+    SPH_HIP(&e, hipStreamSynchronize(e.st));
+    const int cap = std::max(nc, 2 * e.cap);
+    if (e.d_desc) (void)hipFree(e.d_desc);
+    if (e.d_sc) (void)hipFree(e.d_sc);
+    if (e.d_ctl) (void)hipFree(e.d_ctl);
+    if (e.h_desc) (void)hipHostFree(e.h_desc);
+    if (e.h_sc) (void)hipHostFree(e.h_sc);
+    if (e.h_ctl) (void)hipHostFree(e.h_ctl);
+    e.d_desc = nullptr; e.d_sc = nullptr; e.d_ctl = nullptr;
+    e.h_desc = nullptr; e.h_sc = nullptr; e.h_ctl = nullptr;
+    e.cap = 0;
+    bool ok = hipMalloc(&e.d_desc, sizeof(CaseDesc) * cap) == hipSuccess &&
+              hipMalloc(&e.d_sc, sizeof(double) * SCAL_COUNT * cap) == hipSuccess &&
+              hipMalloc(&e.d_ctl, sizeof(double) * CTL_COUNT * cap) == hipSuccess &&
+              hipHostMalloc((void**)&e.h_desc, sizeof(CaseDesc) * cap, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&e.h_sc, sizeof(double) * SCAL_COUNT * cap, hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&e.h_ctl, sizeof(double) * CTL_COUNT * cap, hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        return fail(&e, SPH_E_NOMEM, "engine descriptor allocation failed");
+    }
+    std::memset(e.h_ctl, 0, sizeof(double) * CTL_COUNT * cap);
+    std::memset(e.h_sc, 0, sizeof(double) * SCAL_COUNT * cap);
+    SPH_HIP(&e, hipMemcpy(e.d_ctl, e.h_ctl, sizeof(double) * CTL_COUNT * cap, hipMemcpyHostToDevice));
+    SPH_HIP(&e, hipMemcpy(e.d_sc, e.h_sc, sizeof(double) * SCAL_COUNT * cap, hipMemcpyHostToDevice));
+    e.cap = cap;
+    return 0;
+}
+
+static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double time) {
+    CaseDesc& d = e.h_desc[k];
+    std::memset(&d, 0, sizeof(d));
+    d.n = c->n;
+    d.count = count;
+    d.dt = dt;
+    d.time = time;
+    d.kc = KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15};
+    d.pr = c->prm;
+    d.P = c->pview();
+    d.S = c->sview();
+    d.P2 = c->p2view();
+    d.S2 = c->s2view();
+    d.Q = c->qview();
+    d.g = c->grid();
+    d.nl = c->nlist();
+    d.keys = c->keys;
+    for (int f = 0; f < 5; f++) d.snap[f] = c->snap[f];
+    d.ppe = ppe_view(c);
+    d.sc = e.d_sc + (size_t)k * SCAL_COUNT;
+    d.ctl = e.d_ctl + (size_t)k * CTL_COUNT;
+}
+
+static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, const double* dt, const double* time) {
+    // count == 0: the neighbour search and the PPE structure of that case
+    for (int k = 0; k < nc; k++) {
+        sph_ctx* c = cases[k];
+        if (count[k] == 0) {
+            SPH_TRY(set_ncell_run(c));
+            SPH_TRY(nnps_run(c));
+            SPH_TRY(ppe_build(c));
+        } else if (!c->ppe) {
+            SPH_TRY(ppe_build(c));
+        }
+    }
+    // composition of the step: tables and graphs stay valid while it holds
+    std::vector<long long> sig;
+    sig.reserve(3 * (size_t)nc);
+    long long total = 0;
+    for (int k = 0; k < nc; k++) {
+        sig.push_back((long long)(intptr_t)cases[k]);
+        sig.push_back(cases[k]->generation);
+        sig.push_back(cases[k]->n);
+        total += cases[k]->n;
+    }
+    if (sig != e.comp) {
+        SPH_HIP(&e, hipStreamSynchronize(e.st));   // nothing in flight reads the old tables
+        e.comp = sig;
+        for (auto& t : e.tabs) t.valid = false;
+        drop_graphs(e);
+        SPH_TRY(eng_reserve(e, nc));
+    }
+    e.cs.assign(cases, cases + nc);
+    int cls = 0;
+    while ((1LL << (cls + 1)) <= total) cls++;
+    e.tune_class = cls;
+    // descriptors
+    SPH_HIP(&e, hipEventSynchronize(e.up_ev));
+    for (int k = 0; k < nc; k++) fill_desc(e, k, cases[k], count[k], dt[k], time[k]);
+    SPH_HIP(&e, hipMemcpyAsync(e.d_desc, e.h_desc, sizeof(CaseDesc) * nc, hipMemcpyHostToDevice, e.st));
+    SPH_HIP(&e, hipEventRecord(e.up_ev, e.st));
+    Shape sh;
+    SPH_TRY(shape(e, 0, BLOCK, &sh));
+    SPH_TRY(shape(e, 1, BLOCK, &sh));
+    SPH_TRY(step_physics_front(e));
+    const int prc = ppe_solve(e);
+    if (prc != 0 && prc != SPH_E_NOCONV) return prc;
+    SPH_TRY(step_physics_back(e));
+    return prc;
+}
+
+int engine_step(Eng& e, sph_ctx* const* cases, int nc, const int* count, const double* dt, const double* time) {
+    if (nc <= 0 || !cases || !count || !dt || !time) return fail(&e, SPH_E_ARG, "empty batch");
+    for (int k = 0; k < nc; k++) {
+        sph_ctx* c = cases[k];
+        if (!c) return fail(&e, SPH_E_ARG, "null case");
+        for (int q = 0; q < k; q++)
+            if (cases[q] == c) return fail(&e, SPH_E_ARG, "a case appears twice in one step");
+        if (!c->uploaded) return fail(&e, SPH_E_STATE, "sph_step before sph_upload");
+        if (count[k] < 0) return fail(&e, SPH_E_ARG, "count < 0");
+        if (count[k] > 0 && !c->searched)
+            return fail(&e, SPH_E_STATE, "the first step after sph_upload must have count == 0");
+        if (!(dt[k] > 0)) return fail(&e, SPH_E_ARG, "dt must be > 0");
+        if (c->prm.precision != cases[0]->prm.precision || c->prm.ppe_solver != cases[0]->prm.ppe_solver)
+            return fail(&e, SPH_E_ARG, "cases of one step must share precision and ppe_solver");
+    }
+    e.err.clear();
+    SPH_TRY(tune_fold(e));
+    // hand the cases over to the engine's stream
+    std::vector<hipEvent_t> evs;
+    int rc = 0;
+    for (int k = 0; k < nc && !rc; k++) {
+        sph_ctx* c = cases[k];
+        c->err.clear();
+        if (c->own_stream == e.st) continue;
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(ev, c->own_stream) != hipSuccess || hipStreamWaitEvent(e.st, ev, 0) != hipSuccess)
+            rc = fail(&e, SPH_E_HIP, "stream hand-over failed");
+        if (ev) evs.push_back(ev);
+    }
+    for (int k = 0; k < nc; k++) { cases[k]->stream = e.st; cases[k]->profp = &e.prof; }
+    if (!rc) rc = step_body(e, cases, nc, count, dt, time);
+    for (int k = 0; k < nc; k++) { cases[k]->stream = cases[k]->own_stream; cases[k]->profp = &cases[k]->solo->prof; }
+    // the cases' own streams continue after this step
+    hipEvent_t done = nullptr;
+    bool handed = false;
+    for (int k = 0; k < nc; k++) {
+        sph_ctx* c = cases[k];
+        if (c->own_stream == e.st) continue;
+        if (!handed) {
+            handed = true;
+            if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess || hipEventRecord(done, e.st) != hipSuccess) {
+                if (!rc) rc = fail(&e, SPH_E_HIP, "stream hand-back failed");
+                (void)hipStreamSynchronize(e.st);
+                break;
+            }
+        }
+        if (hipStreamWaitEvent(c->own_stream, done, 0) != hipSuccess && !rc) rc = fail(&e, SPH_E_HIP, "stream hand-back failed");
+    }
+    if (done) (void)hipEventDestroy(done);
+    for (hipEvent_t ev : evs) (void)hipEventDestroy(ev);
+    if (rc) {
+        // every case carries the reason; a case-specific one wins
+        std::string first;
+        for (int k = 0; k < nc; k++)
+            if (!cases[k]->err.empty()) { first = cases[k]->err; break; }
+        if (e.err.empty()) e.err = first;
+        for (int k = 0; k < nc; k++)
+            if (cases[k]->err.empty() && rc != SPH_E_NOCONV) cases[k]->err = e.err;
+    }
+    return rc;
+}
+
+}  // namespace sph

@@ -412,6 +412,7 @@ int nnps_run(sph_ctx* c) {
     SPH_TRY(permute_state(c));
     SPH_TRY(build_neighbor_list(c));
     c->searched = true;
+    c->generation++;   // state arrays were swapped for their sorted copies
     return 0;
 }
 

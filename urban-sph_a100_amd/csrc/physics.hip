@@ -19,6 +19,15 @@
 
 namespace sph {
 
+// This thread's particle: case descriptor `cd` and index `i` in the case;
+// threads past the case's end return.
+#define SEG_PARTICLE(sg)                          \
+    int first_;                                   \
+    const int cid_ = seg_case((sg), first_);      \
+    const CaseDesc& cd = (sg).cd[cid_];           \
+    const int i = first_ + (int)threadIdx.x;      \
+    if (i >= cd.n) return;
+
 // Bin-cell range that contains every particle within distance r of p
 // (binning is monotone in each coordinate; a small margin absorbs the
 // double -> float rounding of the bounds).
@@ -39,27 +48,44 @@ __device__ __forceinline__ NLRow nl_row(const NList& nl, int i) {
 
 // ------------------------------------------------------------------------
 // Step start: SP1 <- P1 and SP2 <- P2 for the mutable fields, P3 <- 0
-// (ISPH_Calc.cuh:53-63).
-struct CopySet {
-    const double* src[MF_COUNT + 10];
-    double* dst[MF_COUNT + 10];
-    double* zero[15];
-};
-__global__ void __launch_bounds__(BLOCK) k_step_begin(int n, CopySet cs) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-#pragma unroll
-    for (int f = 0; f < MF_COUNT + 10; f++) cs.dst[f][i] = cs.src[f][i];
-#pragma unroll
-    for (int f = 0; f < 15; f++) cs.zero[f][i] = 0.0;
+// (ISPH_Calc.cuh:53-63), and KERNEL_set_ncell (functions_NNPS.cuh:943-955).
+__global__ void __launch_bounds__(MAX_BLOCK) k_step_begin(Seg sg) {
+    SEG_PARTICLE(sg);
+    const PView P = cd.P, S = cd.S;
+    const P2View P2 = cd.P2, S2 = cd.S2;
+    const QView Q = cd.Q;
+    if (P.i_type[i] != 3) P.ncell[i] = (int)(1.0 * ceilf(2.0f / (float)(P.h_ref[i] / P.h[i])));
+    S.ux[i] = P.ux[i]; S.uy[i] = P.uy[i]; S.uz[i] = P.uz[i]; S.pres[i] = P.pres[i]; S.temp[i] = P.temp[i];
+    S.vis_t[i] = P.vis_t[i]; S.k_turb[i] = P.k_turb[i]; S.e_turb[i] = P.e_turb[i]; S.flt_s[i] = P.flt_s[i];
+    S.fmx[i] = P.fmx[i]; S.fmy[i] = P.fmy[i]; S.hmz[i] = P.hmz[i];
+    S.float1[i] = P.float1[i]; S.float2[i] = P.float2[i]; S.float3[i] = P.float3[i];
+    S.fbx[i] = P.fbx[i]; S.fby[i] = P.fby[i]; S.fbz[i] = P.fbz[i];
+    S.concentration[i] = P.concentration[i]; S.concn[i] = P.concn[i]; S.enthalpy[i] = P.enthalpy[i];
+    S2.rho_ref[i] = P2.rho_ref[i]; S2.SR[i] = P2.SR[i]; S2.x0[i] = P2.x0[i]; S2.y0[i] = P2.y0[i];
+    S2.z0[i] = P2.z0[i]; S2.ux0[i] = P2.ux0[i]; S2.uy0[i] = P2.uy0[i]; S2.uz0[i] = P2.uz0[i];
+    S2.temp0[i] = P2.temp0[i]; S2.concn0[i] = P2.concn0[i];
+    Q.ftotalx[i] = 0.0; Q.ftotaly[i] = 0.0; Q.ftotalz[i] = 0.0; Q.ftotal[i] = 0.0;
+    Q.fpx[i] = 0.0; Q.fpy[i] = 0.0; Q.fpz[i] = 0.0; Q.dtemp[i] = 0.0; Q.dconcn[i] = 0.0;
+    Q.inv_xx[i] = 0.0; Q.inv_yy[i] = 0.0; Q.inv_zz[i] = 0.0; Q.inv_xy[i] = 0.0; Q.inv_yz[i] = 0.0; Q.inv_zx[i] = 0.0;
+}
+
+// Snapshot of the fields a copy/mirror pass reads while it writes them.
+__global__ void __launch_bounds__(MAX_BLOCK) k_snapshot(Seg sg, int which) {
+    SEG_PARTICLE(sg);
+    const PView A = which ? cd.P : cd.S;
+    cd.snap[0][i] = A.ux[i]; cd.snap[1][i] = A.uy[i]; cd.snap[2][i] = A.uz[i];
+    cd.snap[3][i] = A.pres[i]; cd.snap[4][i] = A.temp[i];
 }
 
 // ------------------------------------------------------------------------
 // KERNEL_clc_gradient_correction_3D, functions_KNL.cuh:216-324
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_grad_corr(int n, KConst kc, PView S, QView Q, NList nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_grad_corr(Seg sg) {
+    SEG_PARTICLE(sg);
+    const KConst kc = cd.kc;
+    const PView S = cd.S;
+    const QView Q = cd.Q;
+    const NList nl = cd.nl;
     int ptype_i = S.p_type[i];
     R hi = (R)S.h[i];
     R tmp_A = calc_tmpA<R>(kc, hi);
@@ -106,10 +132,16 @@ __global__ void __launch_bounds__(BLOCK) k_grad_corr(int n, KConst kc, PView S, 
 // ------------------------------------------------------------------------
 // KERNEL_clc_prep3D, functions_PREP.cuh:287-488
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_prep(int n, KConst kc, sph_params pr, PView S, P2View S2, QView Q,
-                                                NList nl, int count, double dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
+    SEG_PARTICLE(sg);
+    const KConst kc = cd.kc;
+    const sph_params pr = cd.pr;
+    const PView S = cd.S;
+    const P2View S2 = cd.S2;
+    const QView Q = cd.Q;
+    const NList nl = cd.nl;
+    const int count = cd.count;
+    const double dt = cd.dt;
     if (S.i_type[i] > 2) return;
     if (S.p_type[i] >= 1000) return;
     if (S.p_type[i] <= 0) return;
@@ -184,9 +216,13 @@ __global__ void __launch_bounds__(BLOCK) k_prep(int n, KConst kc, sph_params pr,
 // ------------------------------------------------------------------------
 // KERNEL_boundary3D, functions_BC.cuh:100-190 (wall velocity)
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_boundary3d(int n, KConst kc, sph_params pr, PView S, NList nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_boundary3d(Seg sg) {
+    SEG_PARTICLE(sg);
+    const sph_params pr = cd.pr;
+    if (pr.noslip_bc != 1) return;   // ISPH_Calc.cuh:83
+    const KConst kc = cd.kc;
+    const PView S = cd.S;
+    const NList nl = cd.nl;
     if (S.i_type[i] > 2) return;
     int p_type_i = S.p_type[i];
     if (p_type_i > 0) return;
@@ -224,9 +260,13 @@ __global__ void __launch_bounds__(BLOCK) k_boundary3d(int n, KConst kc, sph_para
 // KERNEL_MOST_boundary3D, functions_MOST.cuh:175-322.  Part 1: the
 // surface-layer fluxes of each MOST particle; the neighbour heat deposit is
 // stored per particle for the gather in part 2.
-__global__ void __launch_bounds__(BLOCK) k_most_self(int n, sph_params pr, PView S, double* dep, int count, double dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_most_self(Seg sg) {
+    SEG_PARTICLE(sg);
+    const sph_params pr = cd.pr;
+    const PView S = cd.S;
+    double* dep = cd.snap[4];   // the snapshots are not live between the copy passes
+    const int count = cd.count;
+    const double dt = cd.dt;
     if (S.MOST_buffer[i] != 1) { dep[i] = 0.0; return; }
     const double KVK = 0.4;
     double zi = S.z[i];
@@ -281,10 +321,15 @@ __global__ void __launch_bounds__(BLOCK) k_most_self(int n, sph_params pr, PView
 // ghost node (x_i, y_i, z_i + dz) lies within ghost_r of j and whose 3x3x3
 // ghost window holds j's cell, in ascending i, with i's own deposit applied
 // after its neighbour deposits (functions_MOST.cuh:286-316 serialised).
-__global__ void __launch_bounds__(BLOCK) k_most_deposit(int n, sph_params pr, PView S, QView Q, Grid g,
-                                                        const double* dep, const int* keys) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_most_deposit(Seg sg) {
+    SEG_PARTICLE(sg);
+    const int j = i;
+    const sph_params pr = cd.pr;
+    const PView S = cd.S;
+    const QView Q = cd.Q;
+    const Grid g = cd.g;
+    const double* dep = cd.snap[4];
+    const int* keys = cd.keys;
     const sph_domain& d = g.d;
     const bool self_most = S.MOST_buffer[j] == 1;
     double xj = S.x[j], yj = S.y[j], zj = S.z[j];
@@ -329,10 +374,14 @@ __global__ void __launch_bounds__(BLOCK) k_most_deposit(int n, sph_params pr, PV
 // KERNEL_Neumann_boundary3D, functions_BC.cuh:290-446.  Pressures are read
 // from a snapshot (pold) taken before the launch.
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_neumann(int n, KConst kc, sph_params pr, PView A, Grid g,
-                                                   const double* pold) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_neumann(Seg sg, int which, int only_count0) {
+    SEG_PARTICLE(sg);
+    if (only_count0 && cd.count != 0) return;   // the extra pre-advection call, ISPH_Calc.cuh:88-90
+    const KConst kc = cd.kc;
+    const sph_params pr = cd.pr;
+    const PView A = which ? cd.P : cd.S;
+    const Grid g = cd.g;
+    const double* pold = cd.snap[3];
     if (A.p_type[i] == 1) return;
     const sph_domain& d = g.d;
     R hi = (R)(2.0 * A.h[i]);
@@ -406,9 +455,13 @@ __global__ void __launch_bounds__(BLOCK) k_neumann(int n, KConst kc, sph_params 
 // ------------------------------------------------------------------------
 // KERNEL_advection_force3D, functions_PPE.cuh:191-480
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_advection(int n, KConst kc, sph_params pr, PView S, QView Q, NList nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
+    SEG_PARTICLE(sg);
+    const KConst kc = cd.kc;
+    const sph_params pr = cd.pr;
+    const PView S = cd.S;
+    const QView Q = cd.Q;
+    const NList nl = cd.nl;
     if (S.i_type[i] > 2) return;
     int ptypei = S.p_type[i];
     if (ptypei == 1000) return;
@@ -530,10 +583,14 @@ __global__ void __launch_bounds__(BLOCK) k_advection(int n, KConst kc, sph_param
 
 // ------------------------------------------------------------------------
 // KERNEL_clc_projection, functions_TIME_ISPH.cuh:2-112 (predictor)
-__global__ void __launch_bounds__(BLOCK) k_projection(int n, sph_params pr, PView S, P2View S2, QView Q, double dt,
-                                                      double time) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_projection(Seg sg) {
+    SEG_PARTICLE(sg);
+    const sph_params pr = cd.pr;
+    if (pr.time_type != SPH_PRE_COR) return;   // ISPH_Calc.cuh:105
+    const PView S = cd.S;
+    const P2View S2 = cd.S2;
+    const QView Q = cd.Q;
+    const double dt = cd.dt, time = cd.time;
     if (S.i_type[i] >= 3) return;
     if (S.p_type[i] >= 1000) return;
     int buffer_type = S.buffer_type[i], MOST = S.MOST_buffer[i], p_typei = S.p_type[i];
@@ -571,11 +628,12 @@ __global__ void __launch_bounds__(BLOCK) k_projection(int n, sph_params pr, PVie
 // ------------------------------------------------------------------------
 // KERNEL_periodic_boundary3D, functions_BC.cuh:1015-1154 (reads the snapshot
 // arrays; last match inside the first cell that has one).
-__global__ void __launch_bounds__(BLOCK) k_periodic(int n, sph_params pr, PView A, Grid g, const double* oux,
-                                                    const double* ouy, const double* ouz, const double* opres,
-                                                    const double* otemp) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_periodic(Seg sg, int which) {
+    SEG_PARTICLE(sg);
+    const sph_params pr = cd.pr;
+    const PView A = which ? cd.P : cd.S;
+    const Grid g = cd.g;
+    const double *oux = cd.snap[0], *ouy = cd.snap[1], *ouz = cd.snap[2], *opres = cd.snap[3], *otemp = cd.snap[4];
     int bt = A.buffer_type[i];
     if (bt == 0) return;
     if (A.p_type[i] >= 1000) return;
@@ -616,9 +674,11 @@ __global__ void __launch_bounds__(BLOCK) k_periodic(int n, sph_params pr, PView 
 // ------------------------------------------------------------------------
 // KERNEL_freeslip_boundary3D, functions_BC.cuh:1156-1271
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_freeslip(int n, KConst kc, PView A, NList nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_freeslip(Seg sg, int which) {
+    SEG_PARTICLE(sg);
+    const KConst kc = cd.kc;
+    const PView A = which ? cd.P : cd.S;
+    const NList nl = cd.nl;
     if (A.p_type[i] != -1) return;
     R hi = (R)A.h[i];
     R tmp_A = calc_tmpA<R>(kc, hi);
@@ -649,11 +709,12 @@ __global__ void __launch_bounds__(BLOCK) k_freeslip(int n, KConst kc, PView A, N
 
 // ------------------------------------------------------------------------
 // KERNEL_donothing_buffer3D, functions_BC.cuh:673-761 (snapshot reads)
-__global__ void __launch_bounds__(BLOCK) k_outlet(int n, sph_params pr, PView A, Grid g, const double* oux,
-                                                  const double* ouy, const double* ouz, const double* opres,
-                                                  const double* otemp) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_outlet(Seg sg, int which) {
+    SEG_PARTICLE(sg);
+    const sph_params pr = cd.pr;
+    const PView A = which ? cd.P : cd.S;
+    const Grid g = cd.g;
+    const double *oux = cd.snap[0], *ouy = cd.snap[1], *ouz = cd.snap[2], *opres = cd.snap[3], *otemp = cd.snap[4];
     if (A.buffer_type[i] != 2) return;
     const sph_domain& d = g.d;
     double xi = A.x[i], yi = A.y[i], zi = A.z[i];
@@ -690,9 +751,13 @@ __global__ void __launch_bounds__(BLOCK) k_outlet(int n, sph_params pr, PView A,
 // ------------------------------------------------------------------------
 // KERNEL_pressureforce3D, functions_PPE.cuh:482-647
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_pressure_force(int n, KConst kc, sph_params pr, PView S, QView Q, NList nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
+    SEG_PARTICLE(sg);
+    const KConst kc = cd.kc;
+    const sph_params pr = cd.pr;
+    const PView S = cd.S;
+    const QView Q = cd.Q;
+    const NList nl = cd.nl;
     if (S.i_type[i] > 2) return;
     if (S.p_type[i] >= 1000) return;
     if (S.p_type[i] <= 0) return;
@@ -730,10 +795,14 @@ __global__ void __launch_bounds__(BLOCK) k_pressure_force(int n, KConst kc, sph_
 // ------------------------------------------------------------------------
 // IBM_force_interpolation3D, functions_IBM.cuh:342-490
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_ibm_force(int n, KConst kc, sph_params pr, PView S, QView Q, NList nl,
-                                                     double dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_ibm_force(Seg sg) {
+    SEG_PARTICLE(sg);
+    const KConst kc = cd.kc;
+    const sph_params pr = cd.pr;
+    const PView S = cd.S;
+    const QView Q = cd.Q;
+    const NList nl = cd.nl;
+    const double dt = cd.dt;
     if (S.p_type[i] < 1000) return;
     double xi = S.x[i], yi = S.y[i], zi = S.z[i];
     R tmp_h = (R)(0.5 * S.h[i]);
@@ -774,9 +843,11 @@ __global__ void __launch_bounds__(BLOCK) k_ibm_force(int n, KConst kc, sph_param
 
 // IBM_spreading_interpolation3D, functions_IBM.cuh:492-613
 template <class R>
-__global__ void __launch_bounds__(BLOCK) k_ibm_spread(int n, KConst kc, PView S, NList nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_ibm_spread(Seg sg) {
+    SEG_PARTICLE(sg);
+    const KConst kc = cd.kc;
+    const PView S = cd.S;
+    const NList nl = cd.nl;
     if (S.p_type[i] != 1) return;
     if (S.buffer_type[i] == 2) return;
     double xi = S.x[i], yi = S.y[i], zi = S.z[i];
@@ -811,10 +882,14 @@ __global__ void __launch_bounds__(BLOCK) k_ibm_spread(int n, KConst kc, PView S,
 // ------------------------------------------------------------------------
 // KERNEL_time_update_projection, functions_TIME_ISPH.cuh:115-296 (corrector)
 // and IBM_corrector, functions_IBM.cuh:615-725 (only its copies are live).
-__global__ void __launch_bounds__(BLOCK) k_time_update(int n, sph_params pr, PView S, PView P, P2View S2, P2View P2,
-                                                       QView Q, double dt, int count) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void __launch_bounds__(MAX_BLOCK) k_time_update(Seg sg) {
+    SEG_PARTICLE(sg);
+    const sph_params pr = cd.pr;
+    const PView S = cd.S, P = cd.P;
+    const P2View S2 = cd.S2, P2 = cd.P2;
+    const QView Q = cd.Q;
+    const double dt = cd.dt;
+    const int count = cd.count;
     if (S.p_type[i] >= 1000) {   // IBM_corrector (no i_type filter there)
         P.fbx[i] = S.fbx[i]; P.fby[i] = S.fby[i]; P.fbz[i] = S.fbz[i];
         P.concentration[i] = S.concentration[i]; P.vol[i] = S.vol[i]; P.pres[i] = S.pres[i];
@@ -892,125 +967,93 @@ __global__ void k_stats_final(const unsigned* in, float* out) {
 }
 
 // ========================================================================
-// Host orchestration
+// Host orchestration: every launch covers all cases of the engine's step.
+// Per-case switches (noslip_bc, time_type, the count==0 Neumann call) are
+// tested inside the kernels, so one launch sequence serves any mix.
 // ========================================================================
-static KConst kconst(const sph_ctx* c) { return KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15}; }
 
-static int snapshot(sph_ctx* c, const PView& A) {
-    const double* src[5] = {A.ux, A.uy, A.uz, A.pres, A.temp};
-    for (int f = 0; f < 5; f++)
-        SPH_HIP(c, hipMemcpyAsync(c->snap[f], src[f], sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
-    return 0;
-}
-
-#define LAUNCH(name, bytes, kern, ...)                                                  \
-    do {                                                                               \
-        SPH_TRY(prof_begin(c, name, (double)(bytes)));                                 \
-        hipLaunchKernelGGL(kern, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, __VA_ARGS__); \
-        SPH_TRY(launch_check(c, name));                                                \
-        SPH_TRY(prof_end(c));                                                          \
+// launch over the particles of every case; tunable passes (kid >= 0) run at
+// the workgroup size the autotuner picked for this batch size
+#define SEGL(name, kid, kern, ...)                                                                   \
+    do {                                                                                            \
+        const int blk_ = (kid) >= 0 ? tuned_block(e, (kid)) : BLOCK;                                \
+        Shape sh_;                                                                                  \
+        SPH_TRY(shape(e, 0, blk_, &sh_));                                                           \
+        if (sh_.grid > 0) {                                                                         \
+            SPH_TRY(prof_begin(e, name, 0.0));                                                      \
+            if ((kid) >= 0) SPH_TRY(tune_begin(e, (kid), blk_));                                    \
+            hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(blk_), 0, e.st, Seg{e.d_desc, sh_.tab}, ##__VA_ARGS__); \
+            SPH_TRY(launch_check(&e, name));                                                        \
+            if ((kid) >= 0) SPH_TRY(tune_end(e));                                                   \
+            SPH_TRY(prof_end(e));                                                                   \
+        }                                                                                           \
     } while (0)
 
-static int run_neumann(sph_ctx* c, const PView& A) {
-    int n = c->n;
-    SPH_TRY(snapshot(c, A));
-    if (c->prm.precision == SPH_MIXED)
-        LAUNCH("neumann", 0, k_neumann<float>, n, kconst(c), c->prm, A, c->grid(), (const double*)c->snap[3]);
-    else
-        LAUNCH("neumann", 0, k_neumann<double>, n, kconst(c), c->prm, A, c->grid(), (const double*)c->snap[3]);
+static bool mixed_of(const Eng& e) { return e.cs[0]->prm.precision == SPH_MIXED; }
+
+// which: 0 = step working copy (SP1), 1 = persistent state (P1)
+static int run_neumann(Eng& e, int which, int only_count0) {
+    SEGL("snapshot", -1, k_snapshot, which);
+    if (mixed_of(e)) SEGL("neumann", TK_NEUMANN, k_neumann<float>, which, only_count0);
+    else SEGL("neumann", TK_NEUMANN, k_neumann<double>, which, only_count0);
     return 0;
 }
-static int run_outlet(sph_ctx* c, const PView& A) {
-    int n = c->n;
-    SPH_TRY(snapshot(c, A));
-    LAUNCH("outlet", 0, k_outlet, n, c->prm, A, c->grid(), (const double*)c->snap[0], (const double*)c->snap[1],
-           (const double*)c->snap[2], (const double*)c->snap[3], (const double*)c->snap[4]);
+static int run_outlet(Eng& e, int which) {
+    SEGL("snapshot", -1, k_snapshot, which);
+    SEGL("outlet", -1, k_outlet, which);
     return 0;
 }
-static int run_periodic(sph_ctx* c, const PView& A) {
-    int n = c->n;
-    SPH_TRY(snapshot(c, A));
-    LAUNCH("periodic", 0, k_periodic, n, c->prm, A, c->grid(), (const double*)c->snap[0], (const double*)c->snap[1],
-           (const double*)c->snap[2], (const double*)c->snap[3], (const double*)c->snap[4]);
+static int run_periodic(Eng& e, int which) {
+    SEGL("snapshot", -1, k_snapshot, which);
+    SEGL("periodic", -1, k_periodic, which);
     return 0;
 }
-static int run_freeslip(sph_ctx* c, const PView& A) {
-    int n = c->n;
-    if (c->prm.precision == SPH_MIXED) LAUNCH("freeslip", 0, k_freeslip<float>, n, kconst(c), A, c->nlist());
-    else LAUNCH("freeslip", 0, k_freeslip<double>, n, kconst(c), A, c->nlist());
+static int run_freeslip(Eng& e, int which) {
+    if (mixed_of(e)) SEGL("freeslip", TK_FREESLIP, k_freeslip<float>, which);
+    else SEGL("freeslip", TK_FREESLIP, k_freeslip<double>, which);
     return 0;
 }
 
 // Everything of SOPHIA_single_ISPH before the pressure solve
-// (ISPH_Calc.cuh:20-132).  The count==0 neighbour search has already run.
-int step_physics_front(sph_ctx* c, int count, double dt, double time) {
-    const int n = c->n;
-    const sph_params& pr = c->prm;
-    const bool mixed = pr.precision == SPH_MIXED;
-    PView P = c->pview(), S = c->sview();
-    P2View S2 = c->s2view();
-    QView Q = c->qview();
-    NList nl = c->nlist();
-    KConst kc = kconst(c);
-    // SP1 <- P1, SP2 <- P2, P3 <- 0 (normals persist), set_ncell
-    CopySet cs{};
-    for (int f = 0; f < MF_COUNT; f++) {
-        static const int map[MF_COUNT] = {SPH_F_UX, SPH_F_UY, SPH_F_UZ, SPH_F_PRES, SPH_F_TEMP, SPH_F_VIS_T,
-                                          SPH_F_K_TURB, SPH_F_E_TURB, SPH_F_FLT_S, SPH_F_FMX, SPH_F_FMY,
-                                          SPH_F_HMZ, SPH_F_FLOAT1, SPH_F_FLOAT2, SPH_F_FLOAT3, SPH_F_FBX,
-                                          SPH_F_FBY, SPH_F_FBZ, SPH_F_CONCENTRATION, SPH_F_CONCN,
-                                          SPH_F_ENTHALPY};
-        cs.src[f] = c->pf[map[f]];
-        cs.dst[f] = c->sm[f];
-    }
-    for (int f = 0; f < 10; f++) { cs.src[MF_COUNT + f] = c->p2[f]; cs.dst[MF_COUNT + f] = c->s2[f]; }
-    for (int f = 0; f < 15; f++) cs.zero[f] = c->q[f];
-    LAUNCH("step_begin", 0, k_step_begin, n, cs);
-
-    if (mixed) LAUNCH("grad_corr", 0, k_grad_corr<float>, n, kc, S, Q, nl);
-    else LAUNCH("grad_corr", 0, k_grad_corr<double>, n, kc, S, Q, nl);
-    if (mixed) LAUNCH("prep", 0, k_prep<float>, n, kc, pr, S, S2, Q, nl, count, dt);
-    else LAUNCH("prep", 0, k_prep<double>, n, kc, pr, S, S2, Q, nl, count, dt);
-    if (pr.noslip_bc == 1) {
-        if (mixed) LAUNCH("boundary3d", 0, k_boundary3d<float>, n, kc, pr, S, nl);
-        else LAUNCH("boundary3d", 0, k_boundary3d<double>, n, kc, pr, S, nl);
-    }
-    double* dep = c->snap[4];   // scratch: MOST deposits (snapshots are not live here)
-    LAUNCH("most_self", 0, k_most_self, n, pr, S, dep, count, dt);
-    LAUNCH("most_deposit", 0, k_most_deposit, n, pr, S, Q, c->grid(), (const double*)dep, (const int*)c->keys);
-    if (count == 0) SPH_TRY(run_neumann(c, S));
-    if (mixed) LAUNCH("advection", 0, k_advection<float>, n, kc, pr, S, Q, nl);
-    else LAUNCH("advection", 0, k_advection<double>, n, kc, pr, S, Q, nl);
-    if (pr.time_type == SPH_PRE_COR) LAUNCH("projection", 0, k_projection, n, pr, S, S2, Q, dt, time);
-    SPH_TRY(run_periodic(c, S));
-    SPH_TRY(run_freeslip(c, S));
+// (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
+int step_physics_front(Eng& e) {
+    const bool mixed = mixed_of(e);
+    bool any_count0 = false;
+    for (size_t k = 0; k < e.cs.size(); k++) any_count0 |= e.h_desc[k].count == 0;
+    SEGL("step_begin", -1, k_step_begin);
+    if (mixed) SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<float>);
+    else SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<double>);
+    if (mixed) SEGL("prep", TK_PREP, k_prep<float>);
+    else SEGL("prep", TK_PREP, k_prep<double>);
+    if (mixed) SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<float>);
+    else SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<double>);
+    SEGL("most_self", -1, k_most_self);
+    SEGL("most_deposit", -1, k_most_deposit);
+    if (any_count0) SPH_TRY(run_neumann(e, 0, 1));
+    if (mixed) SEGL("advection", TK_ADVECTION, k_advection<float>);
+    else SEGL("advection", TK_ADVECTION, k_advection<double>);
+    SEGL("projection", -1, k_projection);
+    SPH_TRY(run_periodic(e, 0));
+    SPH_TRY(run_freeslip(e, 0));
     return 0;
 }
 
 // Everything after the pressure solve (ISPH_Calc.cuh:143-214).
-int step_physics_back(sph_ctx* c, int count, double dt, double time) {
-    (void)time;
-    const int n = c->n;
-    const sph_params& pr = c->prm;
-    const bool mixed = pr.precision == SPH_MIXED;
-    PView P = c->pview(), S = c->sview();
-    P2View S2 = c->s2view(), P2 = c->p2view();
-    QView Q = c->qview();
-    NList nl = c->nlist();
-    KConst kc = kconst(c);
-    SPH_TRY(run_neumann(c, S));
-    SPH_TRY(run_outlet(c, S));
-    if (mixed) LAUNCH("pressure_force", 0, k_pressure_force<float>, n, kc, pr, S, Q, nl);
-    else LAUNCH("pressure_force", 0, k_pressure_force<double>, n, kc, pr, S, Q, nl);
-    if (mixed) LAUNCH("ibm_force", 0, k_ibm_force<float>, n, kc, pr, S, Q, nl, dt);
-    else LAUNCH("ibm_force", 0, k_ibm_force<double>, n, kc, pr, S, Q, nl, dt);
-    if (mixed) LAUNCH("ibm_spread", 0, k_ibm_spread<float>, n, kc, S, nl);
-    else LAUNCH("ibm_spread", 0, k_ibm_spread<double>, n, kc, S, nl);
-    LAUNCH("time_update", 0, k_time_update, n, pr, S, P, S2, P2, Q, dt, count);
-    SPH_TRY(run_neumann(c, P));
-    SPH_TRY(run_outlet(c, P));
-    SPH_TRY(run_periodic(c, P));
-    SPH_TRY(run_freeslip(c, P));
+int step_physics_back(Eng& e) {
+    const bool mixed = mixed_of(e);
+    SPH_TRY(run_neumann(e, 0, 0));
+    SPH_TRY(run_outlet(e, 0));
+    if (mixed) SEGL("pressure_force", TK_PRESSURE, k_pressure_force<float>);
+    else SEGL("pressure_force", TK_PRESSURE, k_pressure_force<double>);
+    if (mixed) SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<float>);
+    else SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<double>);
+    if (mixed) SEGL("ibm_spread", TK_IBM_SPREAD, k_ibm_spread<float>);
+    else SEGL("ibm_spread", TK_IBM_SPREAD, k_ibm_spread<double>);
+    SEGL("time_update", -1, k_time_update);
+    SPH_TRY(run_neumann(e, 1, 0));
+    SPH_TRY(run_outlet(e, 1));
+    SPH_TRY(run_periodic(e, 1));
+    SPH_TRY(run_freeslip(e, 1));
     return 0;
 }
 
@@ -1018,7 +1061,8 @@ int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax_) {
     const int n = c->n;
     unsigned* acc = (unsigned*)c->stat_buf;
     SPH_HIP(c, hipMemsetAsync(acc, 0, 3 * sizeof(unsigned), c->stream));
-    LAUNCH("stats", 0, k_stats, n, c->pview(), c->s2view(), c->qview(), acc);
+    hipLaunchKernelGGL(k_stats, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, c->pview(), c->s2view(), c->qview(), acc);
+    SPH_TRY(launch_check(c, "stats"));
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, c->stream, (const unsigned*)acc, c->stat_buf + 4);
     SPH_TRY(launch_check(c, "stats_final"));
     float h[3];

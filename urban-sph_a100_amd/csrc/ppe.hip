@@ -17,12 +17,18 @@
 // The column structure is built once per neighbour search; values, diagonal
 // and right-hand side are assembled every step from the neighbour list.
 //
+// Batching: every kernel covers the rows of all cases of the step (segment
+// table, 256-row blocks that never straddle two cases).  Each case has its
+// own scalars, controls (tolerance, iteration cap, skip) and chunk partials,
+// and one finalize block per case advances its scalars, so a case's iterate
+// sequence does not depend on which other cases share the launches.
+//
 // Reductions: every dot product is the canonical tree of 256-element chunks
 // (one 256-thread block per chunk, pairwise offsets 128..1, chunk partials
-// reduced the same way), so results are bit-reproducible and independent of
-// which other cases run.  Scalars live on the device; kernels after
-// convergence return immediately, so the host polls only every few
-// iterations and the iterate sequence matches a sequential implementation.
+// reduced the same way), so results are bit-reproducible.  Scalars live on
+// the device; kernels of a case that has converged return immediately, so
+// the host polls only every few iterations and the iterate sequence matches
+// a sequential implementation.
 //
 // MIXED precision: matrix values and Krylov vectors of an inner BiCGSTAB are
 // float, every SpMV row sum and every dot is accumulated in double; an outer
@@ -36,7 +42,7 @@ namespace sph {
 
 enum Scal {
     SC_RHO, SC_RHO_OLD, SC_ALPHA, SC_OMEGA, SC_BETA, SC_BNORM, SC_RNORM, SC_SNORM, SC_TOL,
-    SC_DONE, SC_CONV, SC_HALF, SC_IT, SC_ITERS, SC_MAXIT, SC_ZEROX, SC_COUNT = 32
+    SC_DONE, SC_CONV, SC_HALF, SC_IT, SC_ITERS, SC_MAXIT, SC_ZEROX, SC_COUNT = SCAL_COUNT
 };
 
 struct PPEState {
@@ -56,12 +62,13 @@ struct PPEState {
     double* x = nullptr;        // solution q (double)
     double* pnew = nullptr;     // Jacobi output
     // Krylov work vectors (double set and float set)
-    double *r, *rhat, *p, *phat, *v, *s, *shat, *t;
-    float *fr, *frhat, *fp, *fphat, *fv, *fs, *fshat, *ft, *fx, *frhs;
+    double *r = nullptr, *rhat = nullptr, *p = nullptr, *phat = nullptr, *v = nullptr, *s = nullptr,
+           *shat = nullptr, *t = nullptr;
+    float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fphat = nullptr, *fv = nullptr, *fs = nullptr,
+          *fshat = nullptr, *ft = nullptr, *fx = nullptr, *frhs = nullptr;
     double* partA = nullptr;
     double* partB = nullptr;
-    long long nb = 0;
-    int last_iters = 16;
+    int nb = 0;
 };
 
 // ------------------------------------------------------------------------
@@ -87,18 +94,21 @@ __device__ __forceinline__ double block_canon(double v, double* sh) {
     __syncthreads();
     return r;
 }
-// two sums at once (waves 0 and 1)
-__device__ __forceinline__ void block_canon2(double a, double b, double* sh, double* ra, double* rb) {
+// two sums at once (waves 0 and 1): .a valid on thread 0, .b on thread 64
+struct Pair { double a, b; };
+__device__ __forceinline__ Pair block_canon2(double a, double b, double* sh) {
     const int t = threadIdx.x;
     sh[t] = a;
     sh[BLOCK + t] = b;
     __syncthreads();
-    if (t < 64) *ra = wave_tree(sh[t], sh[t + 64], sh[t + 128], sh[t + 192]);
+    Pair r{0.0, 0.0};
+    if (t < 64) r.a = wave_tree(sh[t], sh[t + 64], sh[t + 128], sh[t + 192]);
     else if (t < 128) {
         const int l = t - 64;
-        *rb = wave_tree(sh[BLOCK + l], sh[BLOCK + l + 64], sh[BLOCK + l + 128], sh[BLOCK + l + 192]);
+        r.b = wave_tree(sh[BLOCK + l], sh[BLOCK + l + 64], sh[BLOCK + l + 128], sh[BLOCK + l + 192]);
     }
     __syncthreads();
+    return r;
 }
 __device__ __forceinline__ double chunk_from_global(const double* part, long long base, long long nb, int lane) {
     double a[4];
@@ -139,7 +149,7 @@ __device__ double canon_partials(const double* part, long long nb, double* lvl) 
 }
 
 // ------------------------------------------------------------------------
-// structure
+// structure (per case, once per neighbour search)
 __global__ void __launch_bounds__(BLOCK) k_ppe_flag(int n, PView S, int* flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -185,16 +195,46 @@ __global__ void __launch_bounds__(BLOCK) k_row_slice_width(int nr, int nslices, 
     w[s] = (long long)m * SLICE;
 }
 
+// This thread's PPE row: case descriptor `cd`, row index `row` in the case
+// and the case-local 256-row chunk `chunk` of the block.
+#define SEG_ROW(sg)                               \
+    int first_;                                   \
+    const int cid_ = seg_case((sg), first_);      \
+    const CaseDesc& cd = (sg).cd[cid_];           \
+    const int row = first_ + (int)threadIdx.x;    \
+    const int chunk = first_ / BLOCK;             \
+    (void)chunk;
+
+__device__ __forceinline__ const double* vals_of(const PPEView& v, double*) { return v.vals64; }
+__device__ __forceinline__ const float* vals_of(const PPEView& v, float*) { return v.vals32; }
+__device__ __forceinline__ double* wvals_of(const PPEView& v, double*) { return v.vals64; }
+__device__ __forceinline__ float* wvals_of(const PPEView& v, float*) { return v.vals32; }
+
+// Krylov vector set of one precision (x and rhs: the system being solved)
+template <class X>
+struct KSet { X *r, *rhat, *p, *phat, *v, *s, *shat, *t, *x; const X* rhs; };
+__device__ __forceinline__ KSet<double> kset(const PPEView& v, double*) {
+    return {v.r, v.rhat, v.p, v.phat, v.v, v.s, v.shat, v.t, v.x, v.rhs};
+}
+__device__ __forceinline__ KSet<float> kset(const PPEView& v, float*) {
+    return {v.fr, v.frhat, v.fp, v.fphat, v.fv, v.fs, v.fshat, v.ft, v.fx, v.frhs};
+}
+
 // ------------------------------------------------------------------------
 // per-step assembly, functions_PPE.cuh:2-160
 template <class R, class V>
-__global__ void __launch_bounds__(BLOCK) k_ppe_assemble(int nr, const int* rows, const int* rowof, KConst kc, PView S,
-                                                        QView Q, NList nl, const long long* sl_off, V* vals,
-                                                        double* diag, double* rhs, double* x0, double* pnew,
-                                                        double dt, int jacobi) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nr) return;
-    int i = rows[r];
+__global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) {
+    SEG_ROW(sg);
+    const PPEView pp = cd.ppe;
+    if (row >= pp.nr) return;
+    const int r = row;
+    const KConst kc = cd.kc;
+    const PView S = cd.S;
+    const QView Q = cd.Q;
+    const NList nl = cd.nl;
+    const double dt = cd.dt;
+    const int* rowof = pp.rowof;
+    int i = pp.rows[r];
     R hi = (R)(1.0 * S.h[i]);
     R tmp_A = calc_tmpA<R>(kc, hi);
     double xi = S.x[i], yi = S.y[i], zi = S.z[i];
@@ -203,12 +243,12 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_assemble(int nr, const int* rows,
     R gc11 = (R)Q.inv_xx[i], gc12 = (R)Q.inv_xy[i], gc22 = (R)Q.inv_yy[i];
     R gc23 = (R)Q.inv_yz[i], gc31 = (R)Q.inv_zx[i], gc33 = (R)Q.inv_zz[i];
     double bix = 0, biy = 0, biz = 0, Aij = 0, AijPij = 0, kn = 0;
-    const unsigned* row = nl.ent + nl.slice_off[i >> 6] + (i & 63);
+    const unsigned* nrow = nl.ent + nl.slice_off[i >> 6] + (i & 63);
     int m = nl.cnt[i];
-    V* out = vals + sl_off[r >> 6] + (r & 63);
+    V* out = jacobi ? nullptr : wvals_of(pp, (V*)nullptr) + pp.sl_off[r >> 6] + (r & 63);
     int slot = 0;
     for (int k = 0; k < m; k++) {
-        unsigned e = row[(size_t)k * SLICE];
+        unsigned e = nrow[(size_t)k * SLICE];
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
         if (S.p_type[j] >= 1000 || S.i_type[j] == 4) continue;
@@ -243,29 +283,32 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_assemble(int nr, const int* rows,
     double bi = 1.0 * (bix + biy + biz + 0.0) / dt;
     double q0 = S.pres[i] - S.pres0[i];
     if (jacobi) {
-        pnew[r] = S.pres0[i] + (bi + AijPij) / Aij;
+        pp.pnew[r] = S.pres0[i] + (bi + AijPij) / Aij;
     } else {
-        diag[r] = Aij;
-        rhs[r] = bi + kn;
-        x0[r] = q0;
+        pp.diag[r] = Aij;
+        pp.rhs[r] = bi + kn;
+        pp.x[r] = q0;
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_ppe_scatter(int nr, const int* rows, const double* pres0, const double* src,
-                                                       int add_p0, double* pres) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nr) return;
-    int i = rows[r];
-    pres[i] = add_p0 ? pres0[i] + src[r] : src[r];
+// pressure <- Jacobi sweep output, or p0 + solved increment
+__global__ void __launch_bounds__(BLOCK) k_ppe_scatter(Seg sg, int jacobi) {
+    SEG_ROW(sg);
+    const PPEView pp = cd.ppe;
+    if (row >= pp.nr) return;
+    const PView S = cd.S;
+    const int i = pp.rows[row];
+    S.pres[i] = jacobi ? pp.pnew[row] : S.pres0[i] + pp.x[row];
 }
 
 // ------------------------------------------------------------------------
-// Krylov kernels.  X = vector type, V = matrix value type.  Every kernel is
-// gated on the device flag SC_DONE.
-#define GATE(sc) if ((sc)[SC_DONE] != 0.0) return
+// Krylov kernels.  X = vector type, V = matrix value type.  Every kernel of
+// a case returns at once when the case's solve is done or skipped.
+#define GATE(cd) if ((cd).sc[SC_DONE] != 0.0) return
+#define SKIPPED(cd) ((cd).ctl[CTL_SKIP] != 0.0)
 
 template <class V, class X>
-__device__ __forceinline__ double spmv_row(int r, int nr, const long long* sl_off, const int* cnt, const int* cols,
+__device__ __forceinline__ double spmv_row(int r, const long long* sl_off, const int* cnt, const int* cols,
                                            const V* vals, const double* diag, const X* x) {
     const long long base = sl_off[r >> 6] + (r & 63);
     const int m = cnt[r];
@@ -300,51 +343,66 @@ __device__ __forceinline__ double spmv_row(int r, int nr, const long long* sl_of
 
 // r = rhs - A x ; rhat = r ; partials rhs.rhs (A) and r.r (B)
 template <class V, class X>
-__global__ void __launch_bounds__(BLOCK) k_kr_init(int nr, const long long* sl_off, const int* cnt, const int* cols,
-                                                   const V* vals, const double* diag, const X* rhs, const X* x,
-                                                   X* r, X* rhat, double* pa, double* pb) {
+__global__ void __launch_bounds__(BLOCK) k_kr_init(Seg sg) {
     __shared__ double sh[2 * BLOCK];
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    SEG_ROW(sg);
+    if (SKIPPED(cd)) return;
+    const PPEView pp = cd.ppe;
+    const KSet<X> K = kset(pp, (X*)nullptr);
     double a = 0.0, b = 0.0;
-    if (row < nr) {
-        double y = spmv_row<V, X>(row, nr, sl_off, cnt, cols, vals, diag, x);
-        X rr = (X)((double)rhs[row] - y);
-        r[row] = rr;
-        rhat[row] = rr;
-        a = (double)rhs[row] * (double)rhs[row];
+    if (row < pp.nr) {
+        double y = spmv_row<V, X>(row, pp.sl_off, pp.cnt, pp.cols, vals_of(pp, (V*)nullptr), pp.diag, K.x);
+        X rr = (X)((double)K.rhs[row] - y);
+        K.r[row] = rr;
+        K.rhat[row] = rr;
+        a = (double)K.rhs[row] * (double)K.rhs[row];
         b = (double)rr * (double)rr;
     }
-    double ta = 0.0, tb = 0.0;
-    block_canon2(a, b, sh, &ta, &tb);
-    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
-    if (threadIdx.x == 64) pb[blockIdx.x] = tb;
+    const Pair t2 = block_canon2(a, b, sh);
+    if (threadIdx.x == 0) pp.partA[chunk] = t2.a;
+    if (threadIdx.x == 64) pp.partB[chunk] = t2.b;
+}
+
+// x = 0 when the right-hand side vanished (FIN_INIT)
+template <class X>
+__global__ void __launch_bounds__(BLOCK) k_kr_zerox(Seg sg) {
+    SEG_ROW(sg);
+    if (SKIPPED(cd) || cd.sc[SC_ZEROX] == 0.0) return;
+    const PPEView pp = cd.ppe;
+    if (row < pp.nr) kset(pp, (X*)nullptr).x[row] = (X)0;
 }
 
 // p = r (it==1) or r + beta (p - omega v); phat = p / diag
 template <class X>
-__global__ void __launch_bounds__(BLOCK) k_kr_p(int nr, const double* sc, const X* r, const X* v, X* p, X* phat,
-                                                const double* diag) {
-    GATE(sc);
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= nr) return;
+__global__ void __launch_bounds__(BLOCK) k_kr_p(Seg sg) {
+    SEG_ROW(sg);
+    GATE(cd);
+    const PPEView pp = cd.ppe;
+    if (row >= pp.nr) return;
+    const KSet<X> K = kset(pp, (X*)nullptr);
+    const double* sc = cd.sc;
     X pv;
-    if (sc[SC_IT] == 1.0) pv = r[row];
-    else pv = (X)((double)r[row] + sc[SC_BETA] * ((double)p[row] - sc[SC_OMEGA] * (double)v[row]));
-    p[row] = pv;
-    phat[row] = (X)((double)pv / diag[row]);
+    if (sc[SC_IT] == 1.0) pv = K.r[row];
+    else pv = (X)((double)K.r[row] + sc[SC_BETA] * ((double)K.p[row] - sc[SC_OMEGA] * (double)K.v[row]));
+    K.p[row] = pv;
+    K.phat[row] = (X)((double)pv / pp.diag[row]);
 }
 
-// y = A x ; partial d.y (A)   [and y.y (B) when two dots are needed]
+// TWO = false: v = A phat, partial rhat.v (A)
+// TWO = true : t = A shat, partials t.t (A) and t.s (B)
 template <class V, class X, bool TWO>
-__global__ void __launch_bounds__(BLOCK) k_kr_spmv(int nr, const double* sc, const long long* sl_off, const int* cnt,
-                                                   const int* cols, const V* vals, const double* diag, const X* x,
-                                                   X* y, const X* d, double* pa, double* pb) {
-    GATE(sc);
+__global__ void __launch_bounds__(BLOCK) k_kr_spmv(Seg sg) {
     __shared__ double sh[2 * BLOCK];
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    SEG_ROW(sg);
+    GATE(cd);
+    const PPEView pp = cd.ppe;
+    const KSet<X> K = kset(pp, (X*)nullptr);
+    const X* x = TWO ? K.shat : K.phat;
+    X* y = TWO ? K.t : K.v;
+    const X* d = TWO ? K.s : K.rhat;
     double a = 0.0, b = 0.0;
-    if (row < nr) {
-        X yy = (X)spmv_row<V, X>(row, nr, sl_off, cnt, cols, vals, diag, x);
+    if (row < pp.nr) {
+        X yy = (X)spmv_row<V, X>(row, pp.sl_off, pp.cnt, pp.cols, vals_of(pp, (V*)nullptr), pp.diag, x);
         y[row] = yy;
         if (TWO) {
             a = (double)yy * (double)yy;        // t.t
@@ -354,67 +412,77 @@ __global__ void __launch_bounds__(BLOCK) k_kr_spmv(int nr, const double* sc, con
         }
     }
     if (TWO) {
-        double ta = 0.0, tb = 0.0;
-        block_canon2(a, b, sh, &ta, &tb);
-        if (threadIdx.x == 0) pa[blockIdx.x] = ta;
-        if (threadIdx.x == 64) pb[blockIdx.x] = tb;
+        const Pair t2 = block_canon2(a, b, sh);
+        if (threadIdx.x == 0) pp.partA[chunk] = t2.a;
+        if (threadIdx.x == 64) pp.partB[chunk] = t2.b;
     } else {
         double ta = block_canon(a, sh);
-        if (threadIdx.x == 0) pa[blockIdx.x] = ta;
+        if (threadIdx.x == 0) pp.partA[chunk] = ta;
     }
 }
 
 // s = r - alpha v ; shat = s / diag ; partial s.s
 template <class X>
-__global__ void __launch_bounds__(BLOCK) k_kr_s(int nr, const double* sc, const X* r, const X* v, X* s, X* shat,
-                                                const double* diag, double* pa) {
-    GATE(sc);
+__global__ void __launch_bounds__(BLOCK) k_kr_s(Seg sg) {
     __shared__ double sh[BLOCK];
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    SEG_ROW(sg);
+    GATE(cd);
+    const PPEView pp = cd.ppe;
+    const KSet<X> K = kset(pp, (X*)nullptr);
     double a = 0.0;
-    if (row < nr) {
-        X sv = (X)((double)r[row] - sc[SC_ALPHA] * (double)v[row]);
-        s[row] = sv;
-        shat[row] = (X)((double)sv / diag[row]);
+    if (row < pp.nr) {
+        X sv = (X)((double)K.r[row] - cd.sc[SC_ALPHA] * (double)K.v[row]);
+        K.s[row] = sv;
+        K.shat[row] = (X)((double)sv / pp.diag[row]);
         a = (double)sv * (double)sv;
     }
     double ta = block_canon(a, sh);
-    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
+    if (threadIdx.x == 0) pp.partA[chunk] = ta;
 }
 
 // x = x + alpha phat + omega shat ; r = s - omega t ; partials r.r, rhat.r
 // (after a half-step exit only x = x + alpha phat)
-template <class X, class XS>
-__global__ void __launch_bounds__(BLOCK) k_kr_xr(int nr, const double* sc, XS* x, const X* phat, const X* shat,
-                                                 const X* s, const X* t, X* r, const X* rhat, double* pa, double* pb) {
+template <class X>
+__global__ void __launch_bounds__(BLOCK) k_kr_xr(Seg sg) {
+    __shared__ double sh[2 * BLOCK];
+    SEG_ROW(sg);
+    const double* sc = cd.sc;
     const bool half = sc[SC_HALF] == 1.0;
     if (sc[SC_DONE] != 0.0 && !half) return;
-    __shared__ double sh[2 * BLOCK];
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const PPEView pp = cd.ppe;
+    const KSet<X> K = kset(pp, (X*)nullptr);
     if (half) {
-        if (row < nr) x[row] = (XS)((double)x[row] + sc[SC_ALPHA] * (double)phat[row]);
+        if (row < pp.nr) K.x[row] = (X)((double)K.x[row] + sc[SC_ALPHA] * (double)K.phat[row]);
         return;
     }
     double a = 0.0, b = 0.0;
-    if (row < nr) {
-        x[row] = (XS)((double)x[row] + sc[SC_ALPHA] * (double)phat[row] + sc[SC_OMEGA] * (double)shat[row]);
-        X rv = (X)((double)s[row] - sc[SC_OMEGA] * (double)t[row]);
-        r[row] = rv;
+    if (row < pp.nr) {
+        K.x[row] = (X)((double)K.x[row] + sc[SC_ALPHA] * (double)K.phat[row] + sc[SC_OMEGA] * (double)K.shat[row]);
+        X rv = (X)((double)K.s[row] - sc[SC_OMEGA] * (double)K.t[row]);
+        K.r[row] = rv;
         a = (double)rv * (double)rv;
-        b = (double)rhat[row] * (double)rv;
+        b = (double)K.rhat[row] * (double)rv;
     }
-    double ta = 0.0, tb = 0.0;
-    block_canon2(a, b, sh, &ta, &tb);
-    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
-    if (threadIdx.x == 64) pb[blockIdx.x] = tb;
+    const Pair t2 = block_canon2(a, b, sh);
+    if (threadIdx.x == 0) pp.partA[chunk] = t2.a;
+    if (threadIdx.x == 64) pp.partB[chunk] = t2.b;
 }
 
+// One block per case: reduce the case's chunk partials canonically and
+// advance its scalars.
 enum FinOp { FIN_INIT, FIN_ALPHA, FIN_S, FIN_OMEGA, FIN_R, FIN_NORM };
 constexpr int FIN_BLOCK = 1024;
-__global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(int op, long long nb, const double* pa, const double* pb,
-                                                  double* sc) {
+__global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(const CaseDesc* cds, int op) {
     __shared__ double lvl[256];
-    __shared__ double resB;
+    const CaseDesc& cd = cds[blockIdx.x];
+    double* sc = cd.sc;
+    const double* ctl = cd.ctl;
+    if (ctl[CTL_SKIP] != 0.0) {
+        if (op == FIN_INIT && threadIdx.x == 0) {
+            sc[SC_DONE] = 1.0; sc[SC_HALF] = 0.0; sc[SC_CONV] = 0.0; sc[SC_ITERS] = 0.0; sc[SC_ZEROX] = 0.0;
+        }
+        return;
+    }
     const bool half = sc[SC_HALF] == 1.0;
     if (op == FIN_R) {
         if (sc[SC_DONE] != 0.0 && !half) return;
@@ -425,14 +493,12 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(int op, long long nb, cons
         if (threadIdx.x == 0) sc[SC_HALF] = 2.0;
         return;
     }
-    double A = canon_partials(pa, nb, lvl);
+    const long long nb = cd.ppe.nb;
+    double A = canon_partials(cd.ppe.partA, nb, lvl);
     double B = 0.0;
-    if (op == FIN_INIT || op == FIN_OMEGA || op == FIN_R || op == FIN_NORM) {
-        B = canon_partials(pb, nb, lvl);
-        if (threadIdx.x == 0) resB = B;
-    }
+    if (op == FIN_INIT || op == FIN_OMEGA || op == FIN_R || op == FIN_NORM) B = canon_partials(cd.ppe.partB, nb, lvl);
     if (threadIdx.x != 0) return;
-    const double tol = sc[SC_TOL];
+    const double tol = ctl[CTL_TOL];
     switch (op) {
     case FIN_INIT: {
         double bnorm = sqrt(A), rnorm = sqrt(B);
@@ -466,7 +532,7 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(int op, long long nb, cons
             sc[SC_RHO_OLD] = sc[SC_RHO];
             sc[SC_RHO] = rho_new;
             sc[SC_IT] = sc[SC_IT] + 1.0;
-            if (sc[SC_IT] > sc[SC_MAXIT]) { sc[SC_DONE] = 1.0; sc[SC_ITERS] = sc[SC_MAXIT]; }
+            if (sc[SC_IT] > ctl[CTL_MAXIT]) { sc[SC_DONE] = 1.0; sc[SC_ITERS] = ctl[CTL_MAXIT]; }
             else if (rho_new == 0.0) { sc[SC_DONE] = 1.0; sc[SC_ITERS] = sc[SC_IT]; }
             else sc[SC_BETA] = (sc[SC_RHO] / sc[SC_RHO_OLD]) * (sc[SC_ALPHA] / sc[SC_OMEGA]);
         }
@@ -478,27 +544,34 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(int op, long long nb, cons
 
 // outer residual of the mixed solve: r = rhs - A x (double x, float values,
 // double accumulation); partials r.r (A) and rhs.rhs (B); float copy of r
-__global__ void __launch_bounds__(BLOCK) k_outer_res(int nr, const long long* sl_off, const int* cnt, const int* cols,
-                                                     const float* vals, const double* diag, const double* rhs,
-                                                     const double* x, float* r32, double* pa, double* pb) {
+__global__ void __launch_bounds__(BLOCK) k_outer_res(Seg sg) {
     __shared__ double sh[2 * BLOCK];
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+    SEG_ROW(sg);
+    if (SKIPPED(cd)) return;
+    const PPEView pp = cd.ppe;
     double a = 0.0, b = 0.0;
-    if (row < nr) {
-        double y = spmv_row<float, double>(row, nr, sl_off, cnt, cols, vals, diag, x);
-        double rr = rhs[row] - y;
-        r32[row] = (float)rr;
+    if (row < pp.nr) {
+        double y = spmv_row<float, double>(row, pp.sl_off, pp.cnt, pp.cols, pp.vals32, pp.diag, pp.x);
+        double rr = pp.rhs[row] - y;
+        pp.frhs[row] = (float)rr;
         a = rr * rr;
-        b = rhs[row] * rhs[row];
+        b = pp.rhs[row] * pp.rhs[row];
     }
-    double ta = 0.0, tb = 0.0;
-    block_canon2(a, b, sh, &ta, &tb);
-    if (threadIdx.x == 0) pa[blockIdx.x] = ta;
-    if (threadIdx.x == 64) pb[blockIdx.x] = tb;
+    const Pair t2 = block_canon2(a, b, sh);
+    if (threadIdx.x == 0) pp.partA[chunk] = t2.a;
+    if (threadIdx.x == 64) pp.partB[chunk] = t2.b;
 }
-__global__ void __launch_bounds__(BLOCK) k_add_correction(int nr, double* x, const float* e) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row < nr) x[row] = x[row] + (double)e[row];
+__global__ void __launch_bounds__(BLOCK) k_add_correction(Seg sg) {
+    SEG_ROW(sg);
+    if (SKIPPED(cd)) return;
+    const PPEView pp = cd.ppe;
+    if (row < pp.nr) pp.x[row] = pp.x[row] + (double)pp.fx[row];
+}
+__global__ void __launch_bounds__(BLOCK) k_zero_fx(Seg sg) {
+    SEG_ROW(sg);
+    if (SKIPPED(cd)) return;
+    const PPEView pp = cd.ppe;
+    if (row < pp.nr) pp.fx[row] = 0.0f;
 }
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_zero(long long n, T* a) {
@@ -509,8 +582,6 @@ __global__ void __launch_bounds__(BLOCK) k_zero(long long n, T* a) {
 // ========================================================================
 // host side
 // ========================================================================
-static KConst kconst(const sph_ctx* c) { return KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15}; }
-
 void ppe_free(sph_ctx* c) {
     PPEState* P = c->ppe;
     if (!P) return;
@@ -520,12 +591,15 @@ void ppe_free(sph_ctx* c) {
     for (void* q : ptrs) dfree(c, q);
     delete P;
     c->ppe = nullptr;
+    c->generation++;
 }
 
-static int ppe_build(sph_ctx* c) {
+// Rows and column structure of a case's PPE from its neighbour list.
+int ppe_build(sph_ctx* c) {
     ppe_free(c);
     PPEState* P = new PPEState();
     c->ppe = P;
+    c->generation++;
     const int n = c->n;
     PView S = c->sview();
     int* flag = nullptr;
@@ -542,6 +616,7 @@ static int ppe_build(sph_ctx* c) {
     hipLaunchKernelGGL(k_ppe_rows, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)flag, (const int*)scan,
                        P->rowof, P->rows);
     SPH_TRY(launch_check(c, "ppe_rows"));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
     dfree(c, flag);
     dfree(c, scan);
     if (nr == 0) return 0;
@@ -592,161 +667,273 @@ static int ppe_build(sph_ctx* c) {
     return 0;
 }
 
-static int read_scalars(sph_ctx* c) {
-    SPH_HIP(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
-    SPH_HIP(c, hipStreamSynchronize(c->stream));
+PPEView ppe_view(const sph_ctx* c) {
+    PPEView v{};
+    const PPEState* P = c->ppe;
+    if (!P) return v;
+    v.nr = P->nr; v.nb = P->nb;
+    v.rows = P->rows; v.rowof = P->rowof; v.cnt = P->cnt; v.sl_off = P->sl_off; v.cols = P->cols;
+    v.vals64 = P->vals64; v.vals32 = P->vals32; v.diag = P->diag; v.rhs = P->rhs; v.x = P->x; v.pnew = P->pnew;
+    v.r = P->r; v.rhat = P->rhat; v.p = P->p; v.phat = P->phat; v.v = P->v; v.s = P->s; v.shat = P->shat; v.t = P->t;
+    v.fr = P->fr; v.frhat = P->frhat; v.fp = P->fp; v.fphat = P->fphat; v.fv = P->fv; v.fs = P->fs;
+    v.fshat = P->fshat; v.ft = P->ft; v.fx = P->fx; v.frhs = P->frhs;
+    v.partA = P->partA; v.partB = P->partB;
+    return v;
+}
+
+static int nr_of(const sph_ctx* c) { return c->ppe ? c->ppe->nr : 0; }
+
+static int read_scalars(Eng& e) {
+    SPH_HIP(&e, hipMemcpyAsync(e.h_sc, e.d_sc, sizeof(double) * SC_COUNT * e.cs.size(), hipMemcpyDeviceToHost, e.st));
+    SPH_HIP(&e, hipStreamSynchronize(e.st));
+    return 0;
+}
+// solve controls of every case: the staging buffer is free once the last
+// upload from it has run
+static int ctl_wait(Eng& e) {
+    SPH_HIP(&e, hipEventSynchronize(e.up_ev));
+    return 0;
+}
+static int ctl_upload(Eng& e) {
+    SPH_HIP(&e, hipMemcpyAsync(e.d_ctl, e.h_ctl, sizeof(double) * CTL_COUNT * e.cs.size(), hipMemcpyHostToDevice, e.st));
+    SPH_HIP(&e, hipEventRecord(e.up_ev, e.st));
     return 0;
 }
 
+// launch over the PPE rows of every case, 256-row blocks
+#define RL(name, bytes, kern, ...)                                                                   \
+    do {                                                                                            \
+        Shape sh_;                                                                                  \
+        SPH_TRY(shape(e, 1, BLOCK, &sh_));                                                          \
+        if (sh_.grid > 0) {                                                                         \
+            SPH_TRY(prof_begin(e, name, (double)(bytes)));                                          \
+            hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(BLOCK), 0, e.st, Seg{e.d_desc, sh_.tab}, ##__VA_ARGS__); \
+            SPH_TRY(launch_check(&e, name));                                                        \
+            SPH_TRY(prof_end(e));                                                                   \
+        }                                                                                           \
+    } while (0)
 #define KF(op)                                                                                     \
     do {                                                                                           \
-        SPH_TRY(prof_begin(c, "kr_fin", 0.0));                                                     \
-        hipLaunchKernelGGL(k_kr_fin, dim3(1), dim3(FIN_BLOCK), 0, c->stream, (int)(op), P->nb,      \
-                           (const double*)P->partA, (const double*)P->partB, sc);                  \
-        SPH_TRY(launch_check(c, "kr_fin"));                                                        \
-        SPH_TRY(prof_end(c));                                                                      \
-    } while (0)
-#define KL(name, bytes, grid, kern, ...)                                              \
-    do {                                                                             \
-        SPH_TRY(prof_begin(c, name, (double)(bytes)));                               \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, c->stream, __VA_ARGS__); \
-        SPH_TRY(launch_check(c, name));                                              \
-        SPH_TRY(prof_end(c));                                                        \
+        SPH_TRY(prof_begin(e, "kr_fin", 0.0));                                                     \
+        hipLaunchKernelGGL(k_kr_fin, dim3((unsigned)e.cs.size()), dim3(FIN_BLOCK), 0, e.st,        \
+                           (const CaseDesc*)e.d_desc, (int)(op));                                  \
+        SPH_TRY(launch_check(&e, "kr_fin"));                                                       \
+        SPH_TRY(prof_end(e));                                                                      \
     } while (0)
 
-// BiCGSTAB on A q = rhs (X vectors, V values), solution in xs (XS type).
-// Returns iterations; sets converged / residual in the host scalar mirror.
-template <class V, class X, class XS>
-static int bicgstab(sph_ctx* c, const V* vals, const X* rhs, XS* xs, X* xinit, X* r, X* rhat, X* p, X* phat, X* v,
-                    X* s, X* shat, X* t, double tol, int maxit) {
-    PPEState* P = c->ppe;
-    const int nr = P->nr;
-    const unsigned g = grid1(nr);
-    double* sc = c->dscal;
-    const double nnzb = (double)P->nnz * (4.0 + sizeof(V)) + (double)nr * (8.0 + 2.0 * sizeof(X));
-    // scalars: tol, maxit
-    c->hscal[SC_TOL] = tol;
-    c->hscal[SC_MAXIT] = (double)maxit;
-    SPH_HIP(c, hipMemcpyAsync(sc + SC_TOL, c->hscal + SC_TOL, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    SPH_HIP(c, hipMemcpyAsync(sc + SC_MAXIT, c->hscal + SC_MAXIT, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    KL("kr_init", nnzb, g, (k_kr_init<V, X>), nr, (const long long*)P->sl_off, (const int*)P->cnt,
-       (const int*)P->cols, vals, (const double*)P->diag, rhs, (const X*)xinit, r, rhat, P->partA, P->partB);
+// one BiCGSTAB iteration of every unfinished case
+template <class V, class X>
+static int kr_iteration(Eng& e, double spmv_bytes) {
+    RL("kr_p", 0, k_kr_p<X>);
+    RL("spmv", spmv_bytes, (k_kr_spmv<V, X, false>));
+    KF(FIN_ALPHA);
+    RL("kr_s", 0, k_kr_s<X>);
+    KF(FIN_S);
+    RL("spmv", spmv_bytes, (k_kr_spmv<V, X, true>));
+    KF(FIN_OMEGA);
+    RL("kr_xr", 0, k_kr_xr<X>);
+    KF(FIN_R);
+    return 0;
+}
+
+// The iteration as a HIP graph, captured once per composition of the step
+// (its launches read everything else from the device descriptors).
+template <class V, class X>
+static int kr_graph(Eng& e, int gi, double spmv_bytes, hipGraphExec_t* out) {
+    if (!e.iter_graph[gi]) {
+        Shape sh;
+        SPH_TRY(shape(e, 1, BLOCK, &sh));   // segment table exists before the capture
+        hipGraph_t g = nullptr;
+        SPH_HIP(&e, hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
+        int rc = kr_iteration<V, X>(e, spmv_bytes);
+        hipError_t er = hipStreamEndCapture(e.st, &g);
+        if (rc) {
+            if (er == hipSuccess && g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        SPH_HIP(&e, er);
+        er = hipGraphInstantiate(&e.iter_graph[gi], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (er != hipSuccess) e.iter_graph[gi] = nullptr;
+        SPH_HIP(&e, er);
+    }
+    *out = e.iter_graph[gi];
+    return 0;
+}
+
+// BiCGSTAB of every case whose control says not to skip it, on the vector
+// set X (double: the FP64 solve; float: the inner solve of the mixed mode)
+// with the tolerances and iteration caps already in the controls.
+// iters[k] receives case k's iteration count.
+template <class V, class X>
+static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters) {
+    const int nc = (int)e.cs.size();
+    double spmv_bytes = 0.0;
+    int maxit = 0;
+    for (int k = 0; k < nc; k++) {
+        if (e.h_ctl[k * CTL_COUNT + CTL_SKIP] != 0.0) continue;
+        const PPEState* P = e.cs[k]->ppe;
+        spmv_bytes += (double)P->nnz * (4.0 + sizeof(V)) + (double)P->nr * (8.0 + 2.0 * sizeof(X));
+        maxit = std::max(maxit, (int)e.h_ctl[k * CTL_COUNT + CTL_MAXIT]);
+    }
+    RL("kr_init", spmv_bytes, (k_kr_init<V, X>));
     KF(FIN_INIT);
-    SPH_TRY(read_scalars(c));
-    if (c->hscal[SC_DONE] != 0.0) {
-        if (c->hscal[SC_ZEROX] != 0.0)
-            hipLaunchKernelGGL(k_zero<XS>, dim3(g), dim3(BLOCK), 0, c->stream, (long long)nr, xs);
-        return 0;
-    }
+    RL("kr_zerox", 0, k_kr_zerox<X>);
+    SPH_TRY(read_scalars(e));
+    auto all_done = [&]() {
+        for (int k = 0; k < nc; k++)
+            if (e.h_ctl[k * CTL_COUNT + CTL_SKIP] == 0.0 && e.h_sc[k * SC_COUNT + SC_DONE] == 0.0) return false;
+        return true;
+    };
+    const bool graphs = e.use_graphs && !e.prof.on;
+    int& pred = e.pred_iters[gi][std::min(pred_slot, Eng::PRED_SLOTS - 1)];
     int launched = 0;
-    int batch = std::max(2, std::min(32, P->last_iters / 3));
-    while (launched < maxit) {
-        for (int b = 0; b < batch && launched < maxit; b++, launched++) {
-            KL("kr_p", 0, g, k_kr_p<X>, nr, (const double*)sc, (const X*)r, (const X*)v, p, phat,
-               (const double*)P->diag);
-            KL("spmv", nnzb, g, (k_kr_spmv<V, X, false>), nr, (const double*)sc, (const long long*)P->sl_off,
-               (const int*)P->cnt, (const int*)P->cols, vals, (const double*)P->diag, (const X*)phat, v,
-               (const X*)rhat, P->partA, P->partB);
-            KF(FIN_ALPHA);
-            KL("kr_s", 0, g, k_kr_s<X>, nr, (const double*)sc, (const X*)r, (const X*)v, s, shat,
-               (const double*)P->diag, P->partA);
-            KF(FIN_S);
-            KL("spmv", nnzb, g, (k_kr_spmv<V, X, true>), nr, (const double*)sc, (const long long*)P->sl_off,
-               (const int*)P->cnt, (const int*)P->cols, vals, (const double*)P->diag, (const X*)shat, t,
-               (const X*)s, P->partA, P->partB);
-            KF(FIN_OMEGA);
-            KL("kr_xr", 0, g, (k_kr_xr<X, XS>), nr, (const double*)sc, xs, (const X*)phat, (const X*)shat,
-               (const X*)s, (const X*)t, r, (const X*)rhat, P->partA, P->partB);
-            KF(FIN_R);
+    while (!all_done() && launched < maxit) {
+        // run straight to just before the expected end, then poll often;
+        // without a prediction grow the batches geometrically
+        int batch;
+        if (pred > 0) batch = launched + 4 < (pred * 9) / 10 ? (pred * 9) / 10 - launched : 2;
+        else batch = std::max(4, launched / 2);
+        batch = std::min(batch, maxit - launched);
+        if (graphs) {
+            hipGraphExec_t g = nullptr;
+            SPH_TRY((kr_graph<V, X>(e, gi, spmv_bytes, &g)));
+            for (int b = 0; b < batch; b++) SPH_HIP(&e, hipGraphLaunch(g, e.st));
+        } else {
+            for (int b = 0; b < batch; b++) SPH_TRY((kr_iteration<V, X>(e, spmv_bytes)));
         }
-        SPH_TRY(read_scalars(c));
-        if (c->hscal[SC_DONE] != 0.0) break;
-        batch = std::min(32, batch * 2);
+        launched += batch;
+        SPH_TRY(read_scalars(e));
     }
-    if (c->hscal[SC_DONE] == 0.0) {   // launched maxit iterations without the flag
-        c->hscal[SC_ITERS] = maxit;
+    int most = 0;
+    iters.assign(nc, 0);
+    for (int k = 0; k < nc; k++) {
+        if (e.h_ctl[k * CTL_COUNT + CTL_SKIP] != 0.0) continue;
+        const double* sc = e.h_sc + k * SC_COUNT;
+        iters[k] = sc[SC_DONE] != 0.0 ? (int)sc[SC_ITERS] : (int)e.h_ctl[k * CTL_COUNT + CTL_MAXIT];
+        most = std::max(most, iters[k]);
     }
-    P->last_iters = std::max(4, (int)c->hscal[SC_ITERS]);
+    if (launched > 0) pred = std::max(4, most);
     return 0;
 }
 
-int ppe_solve(sph_ctx* c, double dt) {
-    if (!c->ppe) SPH_TRY(ppe_build(c));
-    PPEState* P = c->ppe;
-    const int nr = P->nr;
-    c->rep = sph_ppe_report{};
-    c->rep.unknowns = nr;
-    c->rep.nnz = P->nnz;
-    if (nr == 0) { c->rep.converged = 1; return 0; }
-    const sph_params& pr = c->prm;
-    const bool mixed = pr.precision == SPH_MIXED;
-    const bool jacobi = pr.ppe_solver == SPH_PPE_REF_JACOBI;
-    PView S = c->sview();
-    QView Q = c->qview();
-    NList nl = c->nlist();
-    KConst kc = kconst(c);
-    const unsigned g = grid1(nr);
-    const double asm_bytes = (double)nr * (15.0 * 8) + (double)P->nnz * (4.0 + (mixed ? 4.0 : 8.0));
-    if (mixed)
-        KL("ppe_assemble", asm_bytes, g, (k_ppe_assemble<float, float>), nr, (const int*)P->rows, (const int*)P->rowof,
-           kc, S, Q, nl, (const long long*)P->sl_off, P->vals32, P->diag, P->rhs, P->x, P->pnew, dt, jacobi ? 1 : 0);
-    else
-        KL("ppe_assemble", asm_bytes, g, (k_ppe_assemble<double, double>), nr, (const int*)P->rows,
-           (const int*)P->rowof, kc, S, Q, nl, (const long long*)P->sl_off, P->vals64, P->diag, P->rhs, P->x,
-           P->pnew, dt, jacobi ? 1 : 0);
+// PPE of every case of the step: assembly, then one Jacobi sweep or the
+// Krylov solve, then the pressures.
+int ppe_solve(Eng& e) {
+    const int nc = (int)e.cs.size();
+    const sph_params& pr0 = e.cs[0]->prm;
+    const bool mixed = pr0.precision == SPH_MIXED;
+    const bool jacobi = pr0.ppe_solver == SPH_PPE_REF_JACOBI;
+    double asm_bytes = 0.0;
+    long long rows_total = 0;
+    for (sph_ctx* c : e.cs) {
+        c->rep = sph_ppe_report{};
+        c->rep.unknowns = nr_of(c);
+        c->rep.nnz = c->ppe ? c->ppe->nnz : 0;
+        if (c->rep.unknowns == 0) c->rep.converged = 1;
+        rows_total += c->rep.unknowns;
+        asm_bytes += (double)c->rep.unknowns * (15.0 * 8) + (double)c->rep.nnz * (4.0 + (mixed ? 4.0 : 8.0));
+    }
+    if (rows_total == 0) return 0;
+    {
+        const int blk = tuned_block(e, TK_ASSEMBLE);
+        Shape sh;
+        SPH_TRY(shape(e, 1, blk, &sh));
+        SPH_TRY(prof_begin(e, "ppe_assemble", asm_bytes));
+        SPH_TRY(tune_begin(e, TK_ASSEMBLE, blk));
+        if (mixed)
+            hipLaunchKernelGGL((k_ppe_assemble<float, float>), dim3(sh.grid), dim3(blk), 0, e.st, Seg{e.d_desc, sh.tab},
+                               jacobi ? 1 : 0);
+        else
+            hipLaunchKernelGGL((k_ppe_assemble<double, double>), dim3(sh.grid), dim3(blk), 0, e.st,
+                               Seg{e.d_desc, sh.tab}, jacobi ? 1 : 0);
+        SPH_TRY(launch_check(&e, "ppe_assemble"));
+        SPH_TRY(tune_end(e));
+        SPH_TRY(prof_end(e));
+    }
     if (jacobi) {
-        KL("ppe_scatter", 0, g, k_ppe_scatter, nr, (const int*)P->rows, (const double*)S.pres0,
-           (const double*)P->pnew, 0, S.pres);
-        c->rep.iterations = 1;
-        c->rep.converged = 1;
-        c->rep.rel_residual = -1.0;
+        RL("ppe_scatter", 0, k_ppe_scatter, 1);
+        for (sph_ctx* c : e.cs) {
+            if (c->rep.unknowns == 0) continue;
+            c->rep.iterations = 1;
+            c->rep.converged = 1;
+            c->rep.rel_residual = -1.0;
+        }
         return 0;
     }
-    int rc = 0;
+    std::vector<int> iters;
     if (!mixed) {
-        SPH_TRY((bicgstab<double, double, double>(c, P->vals64, P->rhs, P->x, P->x, P->r, P->rhat, P->p, P->phat,
-                                                   P->v, P->s, P->shat, P->t, pr.ppe_tol, pr.ppe_maxit)));
-        c->rep.iterations = (int)c->hscal[SC_ITERS];
-        c->rep.converged = c->hscal[SC_CONV] != 0.0;
-        c->rep.rel_residual = c->hscal[SC_BNORM] > 0 ? c->hscal[SC_RNORM] / c->hscal[SC_BNORM] : 0.0;
-    } else {
-        double* sc = c->dscal;
-        int total_it = 0, refine = 0;
-        bool conv = false;
-        double rel = 0.0;
-        for (refine = 0; refine <= pr.ppe_max_refine; refine++) {
-            KL("outer_res", (double)P->nnz * 8.0 + (double)nr * 28.0, g, k_outer_res, nr, (const long long*)P->sl_off,
-               (const int*)P->cnt, (const int*)P->cols, (const float*)P->vals32, (const double*)P->diag,
-               (const double*)P->rhs, (const double*)P->x, P->frhs, P->partA, P->partB);
-            KF(FIN_NORM);
-            SPH_TRY(read_scalars(c));
-            double rn = c->hscal[SC_RNORM], bn = c->hscal[SC_BNORM];
-            rel = bn > 0 ? rn / bn : 0.0;
-            if (rn <= pr.ppe_tol * bn) { conv = true; break; }
-            if (refine == pr.ppe_max_refine) break;
-            hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(BLOCK), 0, c->stream, (long long)nr, P->fx);
-            // aim the correction at the final absolute tolerance, but not
-            // below what a float Krylov solve reliably attains
-            double inner = std::max(pr.ppe_inner_tol, 0.5 * pr.ppe_tol * bn / rn);
-            SPH_TRY((bicgstab<float, float, float>(c, P->vals32, P->frhs, P->fx, P->fx, P->fr, P->frhat, P->fp,
-                                                    P->fphat, P->fv, P->fs, P->fshat, P->ft, inner, pr.ppe_maxit)));
-            total_it += (int)c->hscal[SC_ITERS];
-            KL("add_corr", 0, g, k_add_correction, nr, P->x, (const float*)P->fx);
+        SPH_TRY(ctl_wait(e));
+        for (int k = 0; k < nc; k++) {
+            double* ct = e.h_ctl + k * CTL_COUNT;
+            ct[CTL_TOL] = e.cs[k]->prm.ppe_tol;
+            ct[CTL_MAXIT] = e.cs[k]->prm.ppe_maxit;
+            ct[CTL_SKIP] = nr_of(e.cs[k]) == 0 ? 1.0 : 0.0;
         }
-        c->rep.iterations = total_it;
-        c->rep.refinements = refine;
-        c->rep.converged = conv;
-        c->rep.rel_residual = rel;
+        SPH_TRY(ctl_upload(e));
+        SPH_TRY((bicgstab<double, double>(e, 0, 0, iters)));
+        for (int k = 0; k < nc; k++) {
+            sph_ctx* c = e.cs[k];
+            if (c->rep.unknowns == 0) continue;
+            const double* sc = e.h_sc + k * SC_COUNT;
+            c->rep.iterations = iters[k];
+            c->rep.converged = sc[SC_CONV] != 0.0;
+            c->rep.rel_residual = sc[SC_BNORM] > 0 ? sc[SC_RNORM] / sc[SC_BNORM] : 0.0;
+        }
+    } else {
+        // outer FP64 iterative refinement around float inner solves; each
+        // case refines until its own tolerance is met
+        std::vector<char> active(nc);
+        std::vector<int> refine(nc, 0);
+        for (int k = 0; k < nc; k++) active[k] = nr_of(e.cs[k]) > 0;
+        double res_bytes = 0.0;
+        for (sph_ctx* c : e.cs) res_bytes += (double)c->rep.nnz * 8.0 + (double)c->rep.unknowns * 28.0;
+        for (int outer = 0;; outer++) {
+            bool any = false;
+            for (int k = 0; k < nc; k++) any |= active[k] != 0;
+            if (!any) break;
+            SPH_TRY(ctl_wait(e));
+            for (int k = 0; k < nc; k++) e.h_ctl[k * CTL_COUNT + CTL_SKIP] = active[k] ? 0.0 : 1.0;
+            SPH_TRY(ctl_upload(e));
+            RL("outer_res", res_bytes, k_outer_res);
+            KF(FIN_NORM);
+            SPH_TRY(read_scalars(e));
+            any = false;
+            SPH_TRY(ctl_wait(e));
+            for (int k = 0; k < nc; k++) {
+                if (!active[k]) continue;
+                sph_ctx* c = e.cs[k];
+                const sph_params& pr = c->prm;
+                const double* sc = e.h_sc + k * SC_COUNT;
+                const double rn = sc[SC_RNORM], bn = sc[SC_BNORM];
+                c->rep.rel_residual = bn > 0 ? rn / bn : 0.0;
+                c->rep.refinements = refine[k];
+                double* ct = e.h_ctl + k * CTL_COUNT;
+                if (rn <= pr.ppe_tol * bn) { c->rep.converged = 1; active[k] = 0; }
+                else if (refine[k] == pr.ppe_max_refine) active[k] = 0;
+                else {
+                    // aim the correction at the final absolute tolerance, but
+                    // not below what a float Krylov solve reliably attains
+                    ct[CTL_TOL] = std::max(pr.ppe_inner_tol, 0.5 * pr.ppe_tol * bn / rn);
+                    ct[CTL_MAXIT] = pr.ppe_maxit;
+                    any = true;
+                }
+                ct[CTL_SKIP] = active[k] ? 0.0 : 1.0;
+            }
+            SPH_TRY(ctl_upload(e));
+            if (!any) break;
+            RL("zero_fx", 0, k_zero_fx);
+            SPH_TRY((bicgstab<float, float>(e, 1, outer, iters)));
+            for (int k = 0; k < nc; k++)
+                if (active[k]) { e.cs[k]->rep.iterations += iters[k]; refine[k]++; }
+            RL("add_corr", 0, k_add_correction);
+        }
     }
-    KL("ppe_scatter", 0, g, k_ppe_scatter, nr, (const int*)P->rows, (const double*)S.pres0, (const double*)P->x, 1,
-       S.pres);
-    if (!c->rep.converged) rc = fail(c, SPH_E_NOCONV, "PPE did not converge within ppe_maxit");
+    RL("ppe_scatter", 0, k_ppe_scatter, 0);
+    int rc = 0;
+    for (sph_ctx* c : e.cs)
+        if (!c->rep.converged) rc = fail(c, SPH_E_NOCONV, "PPE did not converge within ppe_maxit");
     return rc;
-}
-
-int reduce_finalize(sph_ctx* c, const double* part, long long nb, double* dst) {
-    (void)c; (void)part; (void)nb; (void)dst;
-    return 0;
 }
 
 }  // namespace sph

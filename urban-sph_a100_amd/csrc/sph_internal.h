@@ -64,7 +64,119 @@ struct Profile {
     std::vector<Pending> pending;
 };
 
+struct KConst {
+    int type;          // SPH_GAUSSIAN..SPH_WENDLAND6
+    double kappa;
+    double c_pi15;     // pow(PI, 1.5), evaluated once on the host
+};
+
+// PPE storage of one case (ppe.hip).  Rows are the unknowns (fluid
+// particles); SELL-64 over rows.
+struct PPEView {
+    int nr;                       // unknowns
+    int nb;                       // 256-row chunks = reduction partials
+    const int* rows;              // row -> particle
+    const int* rowof;             // particle -> row or -1
+    const int* cnt;               // entries per row (off-diagonal)
+    const long long* sl_off;      // first entry of each 64-row slice
+    const int* cols;
+    double* vals64;
+    float* vals32;
+    double *diag, *rhs, *x, *pnew;
+    double *r, *rhat, *p, *phat, *v, *s, *shat, *t;
+    float *fr, *frhat, *fp, *fphat, *fv, *fs, *fshat, *ft, *fx, *frhs;
+    double *partA, *partB;
+};
+
+// Everything a pass needs to know about one case.  The engine keeps one per
+// case of a step in device memory; kernels find theirs through the segment
+// table of the launch.
+struct CaseDesc {
+    int n, count;
+    double dt, time;
+    KConst kc;
+    sph_params pr;
+    PView P, S;          // persistent state, step working copy
+    P2View P2, S2;
+    QView Q;
+    Grid g;
+    NList nl;
+    const int* keys;     // sorted cell keys of the count==0 search
+    double* snap[5];     // ux, uy, uz, pres, temp snapshots (snap[4] doubles as MOST deposits)
+    PPEView ppe;
+    double* sc;          // Krylov scalars, computed on the device
+    const double* ctl;   // solve controls written by the host: tol, maxit, skip
+};
+enum Ctl { CTL_TOL, CTL_MAXIT, CTL_SKIP, CTL_COUNT = 4 };
+constexpr int SCAL_COUNT = 32;   // Krylov scalars per case
+
+// A segmented launch: block b works on case tab[b].x from index tab[b].y on
+// (index = particle or PPE row).  tab == nullptr means one case, block b at
+// b * blockDim.x.  Segments of different cases never share a block, so a
+// case's blocks and their chunk partials are the same alone and in a batch.
+struct Seg {
+    const CaseDesc* cd;
+    const int2* tab;
+};
+__device__ __forceinline__ int seg_case(const Seg& s, int& first) {
+    if (s.tab == nullptr) { first = (int)(blockIdx.x * blockDim.x); return 0; }
+    const int2 t = s.tab[blockIdx.x];
+    first = t.y;
+    return t.x;
+}
+
+// launch tuning: the neighbour passes whose workgroup size the autotuner picks
+enum TuneKernel {
+    TK_GRAD_CORR, TK_PREP, TK_BOUNDARY, TK_NEUMANN, TK_ADVECTION, TK_FREESLIP, TK_PRESSURE, TK_IBM_FORCE,
+    TK_IBM_SPREAD, TK_ASSEMBLE, TK_COUNT
+};
+static_assert(TK_COUNT == SPH_TUNE_PASSES, "tunable passes of the public tuning report");
+constexpr int TUNE_CANDIDATES = 4;           // 64, 128, 256, 512 threads
+constexpr int TUNE_BLOCKS[TUNE_CANDIDATES] = {64, 128, 256, 512};
+constexpr int MAX_BLOCK = 512;
+
 struct PPEState;   // ppe.hip
+
+// The step engine: one stream, one profile, the cases of the current step
+// and their device descriptors.  A single sph_step runs a batch of one.
+struct Eng {
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    Profile prof;
+    std::string err;
+    std::vector<sph_ctx*> cs;
+    std::vector<long long> comp;            // composition signature of cs
+    // descriptors (pinned staging, device copy) and per-case scalars
+    CaseDesc* h_desc = nullptr;
+    CaseDesc* d_desc = nullptr;
+    double* h_sc = nullptr;  double* d_sc = nullptr;
+    double* h_ctl = nullptr; double* d_ctl = nullptr;
+    int cap = 0;
+    hipEvent_t up_ev = nullptr;             // last upload from the staging buffers
+    // segment tables: kind 0 = particles, 1 = PPE rows; one per block size
+    struct Tab { int kind = 0, block = 0; unsigned nblk = 0; int2* d = nullptr; size_t cap = 0; bool valid = false; };
+    std::vector<Tab> tabs;
+    // autotuner: per size class (log2 of the batch's particle count)
+    struct Tune {
+        int phase = 0;
+        int block[TK_COUNT];
+        double ms[TK_COUNT][TUNE_CANDIDATES];
+        int calls[TK_COUNT][TUNE_CANDIDATES];
+    };
+    std::vector<std::pair<int, Tune>> tune;
+    bool autotune = true;
+    struct TuneEv { int cls, kid, cand; hipEvent_t a, b; };
+    std::vector<TuneEv> tune_ev;
+    bool tune_open = false;
+    int tune_class = 0;
+    // graph of one BiCGSTAB iteration per vector precision, valid for comp
+    bool use_graphs = true;
+    hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
+    // Krylov iterations of the last solves (per vector precision and
+    // refinement pass): the host polls for convergence only near them
+    static constexpr int PRED_SLOTS = 4;
+    int pred_iters[2][PRED_SLOTS] = {};
+};
 
 }  // namespace sph
 
@@ -74,9 +186,13 @@ struct sph_ctx {
     bool uploaded = false, searched = false;
     sph_domain dom{};
     double c_pi15 = 0.0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // stream of the current call (the engine's during a step)
+    hipStream_t own_stream = nullptr;   // stream of single-case calls
+    sph::Profile* profp = nullptr;      // profile of the engine driving the step
+    sph::Eng* solo = nullptr;           // engine of sph_step (a batch of one)
     std::string err;
     long long bytes = 0;
+    long long generation = 0;           // bumped whenever device buffers move
 
     // persistent state
     double* pf[SPH_F_COUNT_F64 + 3] = {};  // P1 fields in SPH_F_* numbering, then eli, m_ref, h_ref
@@ -87,11 +203,6 @@ struct sph_ctx {
     double* q[18] = {};                  // part3 scratch + normals (last 3 persistent)
     double* snap[5] = {};                // snapshot buffers (ux, uy, uz, pres, temp)
 
-    // upload staging (original order)
-    void* aos1 = nullptr;
-    void* aos2 = nullptr;
-    float* nrm_in = nullptr;
-
     // neighbour search
     int* keys = nullptr;  int* perm = nullptr;
     int* keys_tmp = nullptr; int* perm_tmp = nullptr;
@@ -101,15 +212,10 @@ struct sph_ctx {
     long long nl_total = 0;
     int* scan_tmp = nullptr; size_t scan_tmp_n = 0;
 
-    // reductions / stats
-    double* red_part = nullptr; size_t red_part_n = 0;
-    double* dscal = nullptr;       // device scalars
-    double* hscal = nullptr;       // pinned host mirror
-    float* stat_buf = nullptr;
+    float* stat_buf = nullptr;           // stats reduction scratch
 
     sph::PPEState* ppe = nullptr;
     sph_ppe_report rep{};
-    sph::Profile prof;
 
     sph::PView pview() const;
     sph::PView sview() const;
@@ -123,7 +229,11 @@ struct sph_ctx {
 namespace sph {
 
 // error helpers ------------------------------------------------------------
-int fail(sph_ctx* c, int code, const std::string& msg);
+template <class T>
+int fail(T* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
 #define SPH_HIP(ctx, call)                                                          \
     do {                                                                            \
         hipError_t e_ = (call);                                                     \
@@ -141,27 +251,47 @@ template <class T>
 int dalloc(sph_ctx* c, T** p, size_t count) { return dmalloc(c, (void**)p, count * sizeof(T)); }
 void dfree(sph_ctx* c, void* p);
 
-// profiling around launches
+// profiling around launches (the profile of the driving engine)
 int prof_begin(sph_ctx* c, const char* name, double alg_bytes);
 int prof_end(sph_ctx* c);
-int launch_check(sph_ctx* c, const char* what);
+int prof_begin(Eng& e, const char* name, double alg_bytes);
+int prof_end(Eng& e);
+int prof_collect(Eng& e);
+template <class T>
+int launch_check(T* c, const char* what) {
+    hipError_t er = hipGetLastError();
+    (void)c;
+    if (er != hipSuccess) return fail(c, SPH_E_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(er));
+    return 0;
+}
 
 inline unsigned grid1(long long n, int block = BLOCK) { return (unsigned)((n + block - 1) / block); }
 
 // nnps.hip
 int nnps_run(sph_ctx* c);
+int set_ncell_run(sph_ctx* c);
 int exclusive_scan_i32(sph_ctx* c, const int* in, int* out, long long n, int* total_out);
 int exclusive_scan_i64(sph_ctx* c, const long long* in, long long* out, long long n, long long* total_out);
 
+// engine.hip: the step over a set of cases, launch shapes, tuning
+int eng_init(Eng& e, hipStream_t st);   // st == nullptr: the engine creates its own stream
+void eng_free(Eng& e);
+int engine_step(Eng& e, sph_ctx* const* cases, int nc, const int* count, const double* dt, const double* time);
+struct Shape { unsigned grid; int block; const int2* tab; };
+int shape(Eng& e, int kind, int block, Shape* out);
+int tuned_block(Eng& e, int kid);
+int tune_begin(Eng& e, int kid, int block);
+int tune_end(Eng& e);
+
 // physics.hip
-int step_physics_front(sph_ctx* c, int count, double dt, double time);
-int step_physics_back(sph_ctx* c, int count, double dt, double time);
+int step_physics_front(Eng& e);
+int step_physics_back(Eng& e);
 int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax);
 
 // ppe.hip
-int ppe_solve(sph_ctx* c, double dt);
+int ppe_build(sph_ctx* c);
+PPEView ppe_view(const sph_ctx* c);
+int ppe_solve(Eng& e);   // every case of the step; 0, or SPH_E_NOCONV when a case did not converge
 void ppe_free(sph_ctx* c);
-// canonical reduction of partials already in c->red_part (count nb) into *dst
-int reduce_finalize(sph_ctx* c, const double* part, long long nb, double* dst);
 
 }  // namespace sph

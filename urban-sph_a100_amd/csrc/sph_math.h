@@ -8,15 +8,9 @@
 
 #include <hip/hip_runtime.h>
 
-#include "../../include/urbansph.h"
+#include "sph_internal.h"
 
 namespace sph {
-
-struct KConst {
-    int type;          // SPH_GAUSSIAN..SPH_WENDLAND6
-    double kappa;
-    double c_pi15;     // pow(PI, 1.5), evaluated once on the host
-};
 
 constexpr double PI_REF = 3.141592653;   // Parameters.cuh:41
 

@@ -45,6 +45,35 @@ def lib():
     L.sph_last_error.restype = C.c_char_p
     L.sph_last_error.argtypes = [vp]
     L.sph_destroy.argtypes = [vp]
+    L.sph_set_engine_options.argtypes = [vp, C.c_int, C.c_int]
+    L.sph_tuning.argtypes = [vp, vp, C.c_int]
+    L.sph_batch_create.argtypes = [C.POINTER(vp)]
+    L.sph_batch_step.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+    L.sph_batch_synchronize.argtypes = [vp]
+    L.sph_batch_set_options.argtypes = [vp, C.c_int, C.c_int]
+    L.sph_batch_profile_enable.argtypes = [vp, C.c_int]
+    L.sph_batch_profile_report.argtypes = [vp, C.POINTER(abi.KernelTime), C.c_int]
+    L.sph_batch_tuning.argtypes = [vp, vp, C.c_int]
+    L.sph_batch_last_error.restype = C.c_char_p
+    L.sph_batch_last_error.argtypes = [vp]
+    L.sph_batch_destroy.argtypes = [vp]
+    L.sph_estimate_bytes.restype = C.c_longlong
+    L.sph_estimate_bytes.argtypes = [C.POINTER(abi.Params), C.c_int]
+    L.sph_server_create.argtypes = [C.c_longlong, C.c_int, C.c_double, C.POINTER(vp)]
+    L.sph_server_submit.argtypes = [vp, C.POINTER(abi.Params), vp, vp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.POINTER(C.c_longlong)]
+    L.sph_server_tick.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+    L.sph_server_idle_until.argtypes = [vp, C.c_double]
+    L.sph_server_clock.restype = C.c_double
+    L.sph_server_clock.argtypes = [vp]
+    L.sph_server_status.argtypes = [vp, C.POINTER(abi.ServerInfo)]
+    L.sph_server_request.argtypes = [vp, C.c_longlong, C.POINTER(abi.RequestInfo)]
+    L.sph_server_result.argtypes = [vp, C.c_longlong, C.c_int, vp, C.c_int]
+    L.sph_server_result_part1.argtypes = [vp, C.c_longlong, vp, C.c_int]
+    L.sph_server_release.argtypes = [vp, C.c_longlong]
+    L.sph_server_last_error.restype = C.c_char_p
+    L.sph_server_last_error.argtypes = [vp]
+    L.sph_server_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -54,8 +83,29 @@ EXPORTED = [
     "sph_get_field_f64", "sph_get_field_i32", "sph_nnps_debug", "sph_neighbors",
     "sph_domain_info", "sph_ppe_info", "sph_stats", "sph_synchronize",
     "sph_profile_enable", "sph_profile_report", "sph_device_bytes",
-    "sph_last_error", "sph_destroy",
+    "sph_last_error", "sph_destroy", "sph_set_engine_options", "sph_tuning",
+    "sph_batch_create", "sph_batch_step", "sph_batch_synchronize", "sph_batch_set_options",
+    "sph_batch_profile_enable", "sph_batch_profile_report", "sph_batch_tuning",
+    "sph_batch_last_error", "sph_batch_destroy", "sph_estimate_bytes", "sph_server_create",
+    "sph_server_submit", "sph_server_tick", "sph_server_idle_until", "sph_server_clock",
+    "sph_server_status", "sph_server_request", "sph_server_result", "sph_server_result_part1",
+    "sph_server_release", "sph_server_last_error", "sph_server_destroy",
 ]
+
+TUNE_PASSES = ["grad_corr", "prep", "boundary3d", "neumann", "advection", "freeslip", "pressure_force",
+               "ibm_force", "ibm_spread", "ppe_assemble"]
+
+
+def _tuning_rows(fn, h):
+    out = np.zeros((64, 1 + len(TUNE_PASSES)), np.int32)
+    k = fn(h, abi.ptr(out), 64)
+    return {int(r[0]): dict(zip(TUNE_PASSES, (int(v) for v in r[1:]))) for r in out[:max(k, 0)]}
+
+
+def _profile_rows(buf, k):
+    if k < 0:
+        raise SPHError(k, "profile_report")
+    return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms, buf[i].alg_bytes) for i in range(k)}
 
 
 def default_params():
@@ -158,10 +208,13 @@ class Case:
 
     def profile_report(self):
         buf = (abi.KernelTime * 128)()
-        k = self.L.sph_profile_report(self.h, buf, 128)
-        if k < 0:
-            raise SPHError(k, "profile_report")
-        return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms, buf[i].alg_bytes) for i in range(k)}
+        return _profile_rows(buf, self.L.sph_profile_report(self.h, buf, 128))
+
+    def options(self, graphs=True, autotune=True):
+        self._check(self.L.sph_set_engine_options(self.h, int(graphs), int(autotune)), "engine_options")
+
+    def tuning(self):
+        return _tuning_rows(self.L.sph_tuning, self.h)
 
     def device_bytes(self):
         return self.L.sph_device_bytes(self.h)
@@ -169,6 +222,137 @@ class Case:
     def close(self):
         if getattr(self, "h", None):
             self.L.sph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent cases stepped together (one sph_batch)."""
+
+    def __init__(self):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.sph_batch_create(C.byref(h))
+        if rc:
+            raise SPHError(rc, "sph_batch_create failed")
+        self.h = h
+
+    def step(self, cases, counts, dts, times=None):
+        """One step of every case in ``cases`` (Case objects).  Raises
+        SPHError on failure; rc SPH_E_NOCONV (-5) is raised after the step
+        completed."""
+        k = len(cases)
+        hs = (C.c_void_p * k)(*[c.h for c in cases])
+        cnt = np.ascontiguousarray(counts, np.int32)
+        dt = np.ascontiguousarray(dts, np.float64)
+        tm = cnt * dt if times is None else np.ascontiguousarray(times, np.float64)
+        rc = self.L.sph_batch_step(self.h, hs, k, abi.ptr(cnt), abi.ptr(dt), abi.ptr(tm))
+        if rc:
+            raise SPHError(rc, f"sph_batch_step: {self.L.sph_batch_last_error(self.h).decode()}")
+
+    def sync(self):
+        rc = self.L.sph_batch_synchronize(self.h)
+        if rc:
+            raise SPHError(rc, f"sph_batch_synchronize: {self.L.sph_batch_last_error(self.h).decode()}")
+
+    def options(self, graphs=True, autotune=True):
+        self.L.sph_batch_set_options(self.h, int(graphs), int(autotune))
+
+    def profile(self, on=True):
+        self.L.sph_batch_profile_enable(self.h, 1 if on else 0)
+
+    def profile_report(self):
+        buf = (abi.KernelTime * 128)()
+        return _profile_rows(buf, self.L.sph_batch_profile_report(self.h, buf, 128))
+
+    def tuning(self):
+        return _tuning_rows(self.L.sph_batch_tuning, self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sph_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def estimate_bytes(params, n):
+    return lib().sph_estimate_bytes(C.byref(params), int(n))
+
+
+class Server:
+    """Continuous-batching request loop (one sph_server).  dry_run_rate > 0
+    simulates the GPU at that many particle-steps/s."""
+
+    def __init__(self, budget_bytes, max_cases=256, dry_run_rate=0.0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.sph_server_create(int(budget_bytes), int(max_cases), float(dry_run_rate), C.byref(h))
+        if rc:
+            raise SPHError(rc, "sph_server_create failed")
+        self.h = h
+        self._fin = np.zeros(4096, np.int64)
+
+    def _check(self, rc, what):
+        if rc:
+            raise SPHError(rc, f"{what}: {self.L.sph_server_last_error(self.h).decode()}")
+
+    def submit(self, params, P1, steps, dt, normals=None, t_arrival=-1.0):
+        P1 = np.ascontiguousarray(P1)
+        assert P1.dtype == abi.PART1
+        nrm = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32)
+        rid = C.c_longlong()
+        self._check(self.L.sph_server_submit(self.h, C.byref(params), abi.ptr(P1), abi.ptr(nrm), len(P1),
+                                             int(steps), float(dt), float(t_arrival), C.byref(rid)), "submit")
+        return rid.value
+
+    def tick(self):
+        """One co-batched step; returns the ids that finished in it."""
+        nf = C.c_int()
+        self._check(self.L.sph_server_tick(self.h, abi.ptr(self._fin), len(self._fin), C.byref(nf)), "tick")
+        return [int(x) for x in self._fin[:min(nf.value, len(self._fin))]]
+
+    def idle_until(self, t):
+        self.L.sph_server_idle_until(self.h, float(t))
+
+    def clock(self):
+        return self.L.sph_server_clock(self.h)
+
+    def status(self):
+        i = abi.ServerInfo()
+        self._check(self.L.sph_server_status(self.h, C.byref(i)), "status")
+        return i
+
+    def request(self, rid):
+        i = abi.RequestInfo()
+        self._check(self.L.sph_server_request(self.h, int(rid), C.byref(i)), "request")
+        return i
+
+    def result(self, rid, name, n):
+        out = np.empty(n, np.float64)
+        self._check(self.L.sph_server_result(self.h, int(rid), abi.F[name], abi.ptr(out), n), "result")
+        return out
+
+    def result_part1(self, rid, n):
+        out = np.zeros(n, abi.PART1)
+        self._check(self.L.sph_server_result_part1(self.h, int(rid), abi.ptr(out), n), "result_part1")
+        return out
+
+    def release(self, rid):
+        self._check(self.L.sph_server_release(self.h, int(rid)), "release")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sph_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
