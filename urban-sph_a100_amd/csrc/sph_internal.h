@@ -118,9 +118,20 @@ struct Seg {
     const CaseDesc* cd;
     const int2* tab;
 };
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b mod 8).  Logical block order is remapped so that each XCD works on one
+// contiguous range of blocks: neighbouring rows, whose gathers touch the
+// same particles, then share one L2.  A bijection on [0, gridDim.x).
+constexpr unsigned NUM_XCD = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    const unsigned per = nb / NUM_XCD, rem = nb % NUM_XCD, x = b % NUM_XCD, k = b / NUM_XCD;
+    return x * per + min(x, rem) + k;
+}
 __device__ __forceinline__ int seg_case(const Seg& s, int& first) {
-    if (s.tab == nullptr) { first = (int)(blockIdx.x * blockDim.x); return 0; }
-    const int2 t = s.tab[blockIdx.x];
+    const unsigned lb = xcd_block();
+    if (s.tab == nullptr) { first = (int)(lb * blockDim.x); return 0; }
+    const int2 t = s.tab[lb];
     first = t.y;
     return t.x;
 }
