@@ -148,6 +148,9 @@ typedef struct sph_params {
     double ppe_inner_tol;       /* MIXED: floor of each inner solve's relative
                                    tolerance (each aims at ppe_tol absolute)  */
     int    ppe_max_refine;      /* MIXED: max outer refinement steps          */
+    int    ppe_index16;         /* MIXED: 16-bit PPE column indices where they
+                                   fit (1, default) or 32-bit (0); the results
+                                   are identical, only the bytes moved differ */
 } sph_params;
 
 /* ---------------------------------------------------------------------------
@@ -351,6 +354,9 @@ typedef struct sph_server_info {
     long long ticks;                /* co-batched steps so far               */
     long long particle_steps;       /* sum over ticks of running particles   */
     double clock;                   /* seconds since creation (or simulated) */
+    double admit_s, step_s, retire_s;  /* host seconds spent admitting (create +
+                                       upload), in batch steps, retiring (result
+                                       copies + free)                           */
 } sph_server_info;
 typedef struct sph_request_info {
     int state;                      /* SPH_REQ_*                             */

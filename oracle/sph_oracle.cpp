@@ -1412,7 +1412,7 @@ void oracle_params_default(sph_params* p) {
     p->most_ghost_dz = 5.0; p->most_ghost_r = 0.1;
     p->ibm_ut_factor = 0.2;
     p->ppe_solver = SPH_PPE_REF_JACOBI; p->precision = SPH_FP64; p->ppe_tol = 1e-8; p->ppe_maxit = 5000;
-    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20;
+    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1;
 }
 
 void* oracle_create(const sph_params* prm, int n) {

@@ -199,3 +199,13 @@ def test_server_results_equal_alone():
     st = srv.status()
     assert st.reserved_bytes == 0 and st.running == 0
     srv.close()
+
+
+def test_index16_changes_nothing():
+    """16-bit PPE column indices (MIXED) give bitwise the 32-bit result."""
+    spec = _specs(abi.MIXED)[3]
+    P, nrm, prm, dt = spec
+    prm32 = type(prm).from_buffer_copy(prm)
+    prm32.update(ppe_index16=0)
+    ref = _alone((P, nrm, prm32, dt), 3)
+    _assert_same(_alone(spec, 3), ref, "index16")

@@ -86,7 +86,7 @@ class Params(C.Structure):
             "ibm_ut_factor")]
         + [("ppe_solver", C.c_int), ("precision", C.c_int),
            ("ppe_tol", C.c_double), ("ppe_maxit", C.c_int),
-           ("ppe_inner_tol", C.c_double), ("ppe_max_refine", C.c_int)]
+           ("ppe_inner_tol", C.c_double), ("ppe_max_refine", C.c_int), ("ppe_index16", C.c_int)]
     )
 
     def update(self, **kw):
@@ -120,7 +120,8 @@ class KernelTime(C.Structure):
 class ServerInfo(C.Structure):
     _fields_ = [("queued", C.c_int), ("running", C.c_int), ("finished", C.c_int),
                 ("reserved_bytes", C.c_longlong), ("budget_bytes", C.c_longlong),
-                ("ticks", C.c_longlong), ("particle_steps", C.c_longlong), ("clock", C.c_double)]
+                ("ticks", C.c_longlong), ("particle_steps", C.c_longlong), ("clock", C.c_double),
+                ("admit_s", C.c_double), ("step_s", C.c_double), ("retire_s", C.c_double)]
 
 
 class RequestInfo(C.Structure):

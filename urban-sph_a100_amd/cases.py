@@ -109,12 +109,16 @@ def log_law(z, u_star=0.3, z0=0.001, kappa_vk=0.4):
 
 
 def canyon(dx=0.008, Lx=2.0, Ly=1.0, Lz=0.5, buildings=None, bsize=(0.1, 0.1, 0.15),
-           walls=3, lid=3, buffers=3, u_star=0.3, z0=0.001, rho=1.225, u_scale=1.0):
+           walls=3, lid=3, buffers=3, u_star=0.3, z0=0.001, rho=1.225, u_scale=1.0, fill_buildings=True):
     """Urban canyon deck.  Fluid sites at ((i+.5)dx, (j+.5)dx, (k+.5)dx) fill
     [0,Lx)x[0,Ly)x(0,Lz) minus the buildings; ``walls`` ground layers
     (p_type 0) below z=0; ``lid`` free-slip layers (p_type -1) above Lz;
     buildings as IBM marker shells (p_type 1000) on their four sides and roof,
-    outward normals returned separately.  Inlet (buffer 1) is x < L2, outlet
+    outward normals returned separately.  The building volume keeps its fluid
+    sites at rest (fill_buildings, the immersed-boundary setting: the markers'
+    forcing, not a void, carries the wall); fill_buildings=False cuts them
+    out, which leaves an unconstrained void that the projection cannot
+    handle (the flow next to the faces blows up within a few steps).  Inlet (buffer 1) is x < L2, outlet
     (buffer 2) x >= L3, periodic side buffers (3/4) y < L_left and y > L_right,
     each ``buffers`` layers wide (functions_NNPS.cuh:253-275).  The first fluid
     layer is the MOST layer (MOST_buffer = 1).  Velocity: log law with u*, z0
@@ -130,10 +134,14 @@ def canyon(dx=0.008, Lx=2.0, Ly=1.0, Lz=0.5, buildings=None, bsize=(0.1, 0.1, 0.
     y = (j.ravel() + 0.5) * dx
     z = (k.ravel() + 0.5) * dx
     keep = np.ones(x.shape, dtype=bool)
+    solid = np.zeros(x.shape, dtype=bool)
     for (xc, yc) in buildings:
         inside = (np.abs(x - xc) < 0.5 * bx) & (np.abs(y - yc) < 0.5 * by) & (z < bh)
-        keep &= ~inside
+        solid |= inside
+    if not fill_buildings:
+        keep &= ~solid
     fx, fy, fz = x[keep], y[keep], z[keep]
+    fsolid = solid[keep]
     first_layer = (k.ravel()[keep] == 0)
     # ground walls and lid
     kw, jw, iw = np.meshgrid(np.arange(walls), np.arange(ny), np.arange(nx), indexing="ij")
@@ -171,7 +179,7 @@ def canyon(dx=0.008, Lx=2.0, Ly=1.0, Lz=0.5, buildings=None, bsize=(0.1, 0.1, 0.
     _fill_common(p, dx, rho)
     # velocity: log law for fluid and lid
     u = np.zeros(n)
-    u[:nfl] = u_scale * log_law(fz, u_star, z0)
+    u[:nfl] = np.where(fsolid, 0.0, u_scale * log_law(fz, u_star, z0))
     u[nfl + nw:nfl + nw + nl] = u_scale * log_law(lz, u_star, z0)
     p["ux"] = u
     # buffers (open_boundary > 0 branch, functions_NNPS.cuh:251-276)

@@ -45,6 +45,11 @@ enum Scal {
     SC_DONE, SC_CONV, SC_HALF, SC_IT, SC_ITERS, SC_MAXIT, SC_ZEROX, SC_COUNT = SCAL_COUNT
 };
 
+// 16-bit column entries: 10 offset bits, up to 64 window groups per slice
+constexpr int EC_OFF_BITS = 10;
+constexpr int EC_OFF_MASK = (1 << EC_OFF_BITS) - 1;
+constexpr int EC_MAX_GROUPS = 64;
+
 struct PPEState {
     int nr = 0;                 // unknowns
     int nslices = 0;
@@ -55,12 +60,15 @@ struct PPEState {
     int* cnt = nullptr;         // entries per row
     long long* sl_off = nullptr;
     int* cols = nullptr;
+    unsigned short* ecols = nullptr;
+    int* gbase = nullptr;
     double* vals64 = nullptr;
     float* vals32 = nullptr;
     double* diag = nullptr;
     double* rhs = nullptr;
     double* x = nullptr;        // solution q (double)
     double* pnew = nullptr;     // Jacobi output
+    float* fdiag = nullptr;
     // Krylov work vectors (double set and float set)
     double *r = nullptr, *rhat = nullptr, *p = nullptr, *phat = nullptr, *v = nullptr, *s = nullptr,
            *shat = nullptr, *t = nullptr;
@@ -195,6 +203,47 @@ __global__ void __launch_bounds__(BLOCK) k_row_slice_width(int nr, int nslices, 
     w[s] = (long long)m * SLICE;
 }
 
+// 16-bit columns of the MIXED matrix, one 64-thread block per 64-row slice.
+// The distinct aligned 1024-column windows that the slice's entries touch
+// become its groups, found in ascending order by repeated wave minima (each
+// row's columns ascend, so every lane walks its row once).  Entry k of row r
+// becomes (group << 10) | (col & 1023); the slice's group g starts at column
+// gbase[64 * slice + g].  More than 64 groups: *overflow = 1 and the case
+// keeps 32-bit columns.
+__global__ void __launch_bounds__(SLICE) k_ecols_build(int nr, const long long* sl_off, const int* cnt,
+                                                       const int* cols, unsigned short* ecols, int* gbase,
+                                                       int* overflow) {
+    __shared__ int gk[EC_MAX_GROUPS];
+    const int sl = blockIdx.x, l = threadIdx.x, r = sl * SLICE + l;
+    const int m = r < nr ? cnt[r] : 0;
+    const int* cp = cols + sl_off[sl] + l;
+    int p = 0, ng = 0, cur = -1;
+    while (true) {
+        while (p < m && (cp[(size_t)p * SLICE] >> EC_OFF_BITS) <= cur) p++;
+        int k = p < m ? (cp[(size_t)p * SLICE] >> EC_OFF_BITS) : INT_MAX;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) k = min(k, __shfl_xor(k, off, 64));
+        if (k == INT_MAX) break;
+        if (ng < EC_MAX_GROUPS && l == 0) gk[ng] = k;
+        ng++;
+        cur = k;
+    }
+    __syncthreads();
+    if (ng > EC_MAX_GROUPS) {
+        if (l == 0) *overflow = 1;
+        return;
+    }
+    gbase[(size_t)sl * SLICE + l] = l < ng ? gk[l] << EC_OFF_BITS : 0;
+    unsigned short* ep = ecols + sl_off[sl] + l;
+    int g = 0;
+    for (int k2 = 0; k2 < m; k2++) {
+        const int c = cp[(size_t)k2 * SLICE];
+        const int key = c >> EC_OFF_BITS;
+        while (g + 1 < ng && gk[g] < key) g++;
+        ep[(size_t)k2 * SLICE] = (unsigned short)((g << EC_OFF_BITS) | (c & EC_OFF_MASK));
+    }
+}
+
 // This thread's PPE row: case descriptor `cd`, row index `row` in the case
 // and the case-local 256-row chunk `chunk` of the block.
 #define SEG_ROW(sg)                               \
@@ -213,6 +262,9 @@ __device__ __forceinline__ float* wvals_of(const PPEView& v, float*) { return v.
 // Krylov vector set of one precision (x and rhs: the system being solved)
 template <class X>
 struct KSet { X *r, *rhat, *p, *phat, *v, *s, *shat, *t, *x; const X* rhs; };
+// diagonal used with each vector set
+__device__ __forceinline__ const double* kdiag(const PPEView& v, double*) { return v.diag; }
+__device__ __forceinline__ const float* kdiag(const PPEView& v, float*) { return v.fdiag; }
 __device__ __forceinline__ KSet<double> kset(const PPEView& v, double*) {
     return {v.r, v.rhat, v.p, v.phat, v.v, v.s, v.shat, v.t, v.x, v.rhs};
 }
@@ -286,6 +338,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         pp.pnew[r] = S.pres0[i] + (bi + AijPij) / Aij;
     } else {
         pp.diag[r] = Aij;
+        if (pp.fdiag) pp.fdiag[r] = (float)Aij;
         pp.rhs[r] = bi + kn;
         pp.x[r] = q0;
     }
@@ -307,17 +360,13 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_scatter(Seg sg, int jacobi) {
 #define GATE(cd) if ((cd).sc[SC_DONE] != 0.0) return
 #define SKIPPED(cd) ((cd).ctl[CTL_SKIP] != 0.0)
 
+// Off-diagonal part of one row, sum_k vals[k] * x[col[k]], accumulated in
+// double.  Matrix entries stream through once: nontemporal loads keep the
+// caches for the x gathers; four independent loads in flight per lane.
 template <class V, class X>
-__device__ __forceinline__ double spmv_row(int r, const long long* sl_off, const int* cnt, const int* cols,
-                                           const V* vals, const double* diag, const X* x) {
-    const long long base = sl_off[r >> 6] + (r & 63);
-    const int m = cnt[r];
-    const int* cp = cols + base;
-    const V* vp = vals + base;
+__device__ __forceinline__ double offdiag_i32(const int* cp, const V* vp, int m, const X* x) {
     double s = 0.0;
     int k = 0;
-    // matrix entries stream through once: nontemporal loads keep the
-    // caches for the x gathers; four independent loads in flight per lane
     for (; k + 4 <= m; k += 4) {
         const int c0 = __builtin_nontemporal_load(cp + (size_t)(k + 0) * SLICE);
         const int c1 = __builtin_nontemporal_load(cp + (size_t)(k + 1) * SLICE);
@@ -338,20 +387,77 @@ __device__ __forceinline__ double spmv_row(int r, const long long* sl_off, const
         const V v0 = __builtin_nontemporal_load(vp + (size_t)k * SLICE);
         s += (double)v0 * (double)x[c0];
     }
-    return diag[r] * (double)x[r] - s;
+    return s;
+}
+// The same over 16-bit columns: entry e = (group << EC_OFF_BITS) | offset,
+// column = wb[group] + offset, wb = the slice's group bases (in LDS).
+template <class X>
+__device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const float* vp, int m, const int* wb,
+                                              const X* x) {
+    double s = 0.0;
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+        const int e0 = __builtin_nontemporal_load(ep + (size_t)(k + 0) * SLICE);
+        const int e1 = __builtin_nontemporal_load(ep + (size_t)(k + 1) * SLICE);
+        const int e2 = __builtin_nontemporal_load(ep + (size_t)(k + 2) * SLICE);
+        const int e3 = __builtin_nontemporal_load(ep + (size_t)(k + 3) * SLICE);
+        const float v0 = __builtin_nontemporal_load(vp + (size_t)(k + 0) * SLICE);
+        const float v1 = __builtin_nontemporal_load(vp + (size_t)(k + 1) * SLICE);
+        const float v2 = __builtin_nontemporal_load(vp + (size_t)(k + 2) * SLICE);
+        const float v3 = __builtin_nontemporal_load(vp + (size_t)(k + 3) * SLICE);
+        const X x0 = x[wb[e0 >> EC_OFF_BITS] + (e0 & EC_OFF_MASK)];
+        const X x1 = x[wb[e1 >> EC_OFF_BITS] + (e1 & EC_OFF_MASK)];
+        const X x2 = x[wb[e2 >> EC_OFF_BITS] + (e2 & EC_OFF_MASK)];
+        const X x3 = x[wb[e3 >> EC_OFF_BITS] + (e3 & EC_OFF_MASK)];
+        s += (double)v0 * (double)x0;
+        s += (double)v1 * (double)x1;
+        s += (double)v2 * (double)x2;
+        s += (double)v3 * (double)x3;
+    }
+    for (; k < m; k++) {
+        const int e0 = __builtin_nontemporal_load(ep + (size_t)k * SLICE);
+        const float v0 = __builtin_nontemporal_load(vp + (size_t)k * SLICE);
+        s += (double)v0 * (double)x[wb[e0 >> EC_OFF_BITS] + (e0 & EC_OFF_MASK)];
+    }
+    return s;
+}
+
+// Group bases of the block's slices into LDS (every thread of the block
+// takes part; call before the row guard).
+__device__ __forceinline__ void load_group_bases(const PPEView& pp, int first, int* wbase) {
+    if (pp.ecols) wbase[threadIdx.x] = pp.gbase[first + (int)threadIdx.x];
+    __syncthreads();
+}
+
+// y_r = diag_r x_r - offdiag_r(x); dg = the diagonal of this operator
+template <class V, class X, class DG>
+__device__ __forceinline__ double spmv_row(const PPEView& pp, int r, const V* vals, const DG* dg, const X* x,
+                                           const int* wbase) {
+    const long long base = pp.sl_off[r >> 6] + (r & 63);
+    const int m = pp.cnt[r];
+    double s;
+    if constexpr (sizeof(V) == 4) {
+        if (pp.ecols) s = offdiag_u16<X>(pp.ecols + base, (const float*)vals + base, m, wbase + (threadIdx.x & ~63), x);
+        else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
+    } else {
+        s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
+    }
+    return (double)dg[r] * (double)x[r] - s;
 }
 
 // r = rhs - A x ; rhat = r ; partials rhs.rhs (A) and r.r (B)
 template <class V, class X>
 __global__ void __launch_bounds__(BLOCK) k_kr_init(Seg sg) {
     __shared__ double sh[2 * BLOCK];
+    __shared__ int wbase[BLOCK];
     SEG_ROW(sg);
     if (SKIPPED(cd)) return;
     const PPEView pp = cd.ppe;
+    load_group_bases(pp, first_, wbase);
     const KSet<X> K = kset(pp, (X*)nullptr);
     double a = 0.0, b = 0.0;
     if (row < pp.nr) {
-        double y = spmv_row<V, X>(row, pp.sl_off, pp.cnt, pp.cols, vals_of(pp, (V*)nullptr), pp.diag, K.x);
+        double y = spmv_row(pp, row, vals_of(pp, (V*)nullptr), kdiag(pp, (X*)nullptr), K.x, wbase);
         X rr = (X)((double)K.rhs[row] - y);
         K.r[row] = rr;
         K.rhat[row] = rr;
@@ -385,7 +491,7 @@ __global__ void __launch_bounds__(BLOCK) k_kr_p(Seg sg) {
     if (sc[SC_IT] == 1.0) pv = K.r[row];
     else pv = (X)((double)K.r[row] + sc[SC_BETA] * ((double)K.p[row] - sc[SC_OMEGA] * (double)K.v[row]));
     K.p[row] = pv;
-    K.phat[row] = (X)((double)pv / pp.diag[row]);
+    K.phat[row] = (X)((double)pv / (double)kdiag(pp, (X*)nullptr)[row]);
 }
 
 // TWO = false: v = A phat, partial rhat.v (A)
@@ -393,16 +499,18 @@ __global__ void __launch_bounds__(BLOCK) k_kr_p(Seg sg) {
 template <class V, class X, bool TWO>
 __global__ void __launch_bounds__(BLOCK) k_kr_spmv(Seg sg) {
     __shared__ double sh[2 * BLOCK];
+    __shared__ int wbase[BLOCK];
     SEG_ROW(sg);
     GATE(cd);
     const PPEView pp = cd.ppe;
+    load_group_bases(pp, first_, wbase);
     const KSet<X> K = kset(pp, (X*)nullptr);
     const X* x = TWO ? K.shat : K.phat;
     X* y = TWO ? K.t : K.v;
     const X* d = TWO ? K.s : K.rhat;
     double a = 0.0, b = 0.0;
     if (row < pp.nr) {
-        X yy = (X)spmv_row<V, X>(row, pp.sl_off, pp.cnt, pp.cols, vals_of(pp, (V*)nullptr), pp.diag, x);
+        X yy = (X)spmv_row(pp, row, vals_of(pp, (V*)nullptr), kdiag(pp, (X*)nullptr), x, wbase);
         y[row] = yy;
         if (TWO) {
             a = (double)yy * (double)yy;        // t.t
@@ -433,7 +541,7 @@ __global__ void __launch_bounds__(BLOCK) k_kr_s(Seg sg) {
     if (row < pp.nr) {
         X sv = (X)((double)K.r[row] - cd.sc[SC_ALPHA] * (double)K.v[row]);
         K.s[row] = sv;
-        K.shat[row] = (X)((double)sv / pp.diag[row]);
+        K.shat[row] = (X)((double)sv / (double)kdiag(pp, (X*)nullptr)[row]);
         a = (double)sv * (double)sv;
     }
     double ta = block_canon(a, sh);
@@ -546,12 +654,14 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(const CaseDesc* cds, int o
 // double accumulation); partials r.r (A) and rhs.rhs (B); float copy of r
 __global__ void __launch_bounds__(BLOCK) k_outer_res(Seg sg) {
     __shared__ double sh[2 * BLOCK];
+    __shared__ int wbase[BLOCK];
     SEG_ROW(sg);
     if (SKIPPED(cd)) return;
     const PPEView pp = cd.ppe;
+    load_group_bases(pp, first_, wbase);
     double a = 0.0, b = 0.0;
     if (row < pp.nr) {
-        double y = spmv_row<float, double>(row, pp.sl_off, pp.cnt, pp.cols, pp.vals32, pp.diag, pp.x);
+        double y = spmv_row(pp, row, (const float*)pp.vals32, (const double*)pp.diag, (const double*)pp.x, wbase);
         double rr = pp.rhs[row] - y;
         pp.frhs[row] = (float)rr;
         a = rr * rr;
@@ -585,8 +695,8 @@ __global__ void __launch_bounds__(BLOCK) k_zero(long long n, T* a) {
 void ppe_free(sph_ctx* c) {
     PPEState* P = c->ppe;
     if (!P) return;
-    void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->vals64, P->vals32, P->diag, P->rhs, P->x,
-                    P->pnew, P->r, P->rhat, P->p, P->phat, P->v, P->s, P->shat, P->t, P->fr, P->frhat, P->fp,
+    void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->ecols, P->gbase, P->vals64, P->vals32,
+                    P->diag, P->fdiag, P->rhs, P->x, P->pnew, P->r, P->rhat, P->p, P->phat, P->v, P->s, P->shat, P->t, P->fr, P->frhat, P->fp,
                     P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->partA, P->partB};
     for (void* q : ptrs) dfree(c, q);
     delete P;
@@ -638,20 +748,43 @@ int ppe_build(sph_ctx* c) {
     hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
                        (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols);
     SPH_TRY(launch_check(c, "ppe_cols"));
+    P->nb = (nr + BLOCK - 1) / BLOCK;
+    const bool mixed = c->prm.precision == SPH_MIXED;
+    if (mixed && c->prm.ppe_index16) {
+        // 16-bit columns where every slice fits 64 windows of 1024 columns
+        int* ovf = nullptr;
+        SPH_TRY(dalloc(c, &P->ecols, P->nnz_slots + 1));
+        SPH_TRY(dalloc(c, &P->gbase, (size_t)P->nb * BLOCK));
+        SPH_TRY(dalloc(c, &ovf, 1));
+        hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(BLOCK), 0, c->stream, 1LL, ovf);
+        hipLaunchKernelGGL(k_zero<int>, dim3(grid1((long long)P->nb * BLOCK)), dim3(BLOCK), 0, c->stream,
+                           (long long)P->nb * BLOCK, P->gbase);
+        hipLaunchKernelGGL(k_ecols_build, dim3(P->nslices), dim3(SLICE), 0, c->stream, nr, (const long long*)P->sl_off,
+                           (const int*)P->cnt, (const int*)P->cols, P->ecols, P->gbase, ovf);
+        SPH_TRY(launch_check(c, "ecols_build"));
+        int hovf = 0;
+        SPH_HIP(c, hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(c, ovf);
+        if (hovf) {
+            dfree(c, P->ecols); P->ecols = nullptr;
+            dfree(c, P->gbase); P->gbase = nullptr;
+        }
+    }
     // real nnz (for reports)
     std::vector<int> hc(nr);
     SPH_HIP(c, hipMemcpyAsync(hc.data(), P->cnt, sizeof(int) * nr, hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
     P->nnz = 0;
     for (int v : hc) P->nnz += v;
-    const bool mixed = c->prm.precision == SPH_MIXED;
+    if (P->ecols) { dfree(c, P->cols); P->cols = nullptr; }   // the 16-bit copy replaces them
     if (mixed) SPH_TRY(dalloc(c, &P->vals32, P->nnz_slots + 1));
     else SPH_TRY(dalloc(c, &P->vals64, P->nnz_slots + 1));
     SPH_TRY(dalloc(c, &P->diag, nr));
+    if (mixed) SPH_TRY(dalloc(c, &P->fdiag, nr));
     SPH_TRY(dalloc(c, &P->rhs, nr));
     SPH_TRY(dalloc(c, &P->x, nr));
     SPH_TRY(dalloc(c, &P->pnew, nr));
-    P->nb = (nr + BLOCK - 1) / BLOCK;
     if (P->nb > 65536) return fail(c, SPH_E_UNSUPPORTED, "PPE larger than 16.7M unknowns");
     SPH_TRY(dalloc(c, &P->partA, P->nb));
     SPH_TRY(dalloc(c, &P->partB, P->nb));
@@ -673,6 +806,7 @@ PPEView ppe_view(const sph_ctx* c) {
     if (!P) return v;
     v.nr = P->nr; v.nb = P->nb;
     v.rows = P->rows; v.rowof = P->rowof; v.cnt = P->cnt; v.sl_off = P->sl_off; v.cols = P->cols;
+    v.ecols = P->ecols; v.gbase = P->gbase; v.fdiag = P->fdiag;
     v.vals64 = P->vals64; v.vals32 = P->vals32; v.diag = P->diag; v.rhs = P->rhs; v.x = P->x; v.pnew = P->pnew;
     v.r = P->r; v.rhat = P->rhat; v.p = P->p; v.phat = P->phat; v.v = P->v; v.s = P->s; v.shat = P->shat; v.t = P->t;
     v.fr = P->fr; v.frhat = P->frhat; v.fp = P->fp; v.fphat = P->fphat; v.fv = P->fv; v.fs = P->fs;
@@ -772,8 +906,15 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters) {
     int maxit = 0;
     for (int k = 0; k < nc; k++) {
         if (e.h_ctl[k * CTL_COUNT + CTL_SKIP] != 0.0) continue;
+        // compulsory bytes of one SpMV launch: every stored entry (column
+        // index, value) once, and per row its count, diagonal, own x, the
+        // output and the dot partner (plus the slice group bases of 16-bit
+        // columns); the x gathers of the neighbours are cache traffic
         const PPEState* P = e.cs[k]->ppe;
-        spmv_bytes += (double)P->nnz * (4.0 + sizeof(V)) + (double)P->nr * (8.0 + 2.0 * sizeof(X));
+        const bool i16 = P->ecols != nullptr && sizeof(V) == 4;
+        const double diag_b = sizeof(X) == 4 ? 4.0 : 8.0;
+        spmv_bytes += (double)P->nnz * ((i16 ? 2.0 : 4.0) + sizeof(V)) +
+                      (double)P->nr * (4.0 + diag_b + 3.0 * sizeof(X) + (i16 ? 4.0 : 0.0));
         maxit = std::max(maxit, (int)e.h_ctl[k * CTL_COUNT + CTL_MAXIT]);
     }
     RL("kr_init", spmv_bytes, (k_kr_init<V, X>));

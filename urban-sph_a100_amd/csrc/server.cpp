@@ -52,6 +52,7 @@ struct sph_server {
     long long next_id = 1;
     long long reserved = 0;
     long long steps_done = 0, particle_steps = 0;
+    double t_admit = 0.0, t_step = 0.0, t_retire = 0.0;   // host seconds per phase
     sph_batch* batch = nullptr;
     std::deque<Request*> queue;
     std::vector<Request*> active;
@@ -110,8 +111,13 @@ int collect(sph_server* s, Request* r) {
     return 0;
 }
 
+double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 // FIFO admission while the head fits the budget and a slot is free.
 int admit(sph_server* s) {
+    const double t0 = wall();
     while (!s->queue.empty() && (int)s->active.size() < s->max_cases) {
         Request* r = s->queue.front();
         if (s->reserved + r->est > s->budget) break;
@@ -132,6 +138,7 @@ int admit(sph_server* s) {
         r->state = SPH_REQ_RUNNING;
         s->active.push_back(r);
     }
+    s->t_admit += wall() - t0;
     return 0;
 }
 
@@ -210,6 +217,7 @@ int sph_server_tick(sph_server* s, long long* finished, int max_finished, int* n
     const int k = (int)s->active.size();
     long long ps = 0;
     for (Request* r : s->active) ps += r->n;
+    const double t_step0 = wall();
     if (s->dry) {
         s->sim_clock += (double)ps / s->dry_rate;
     } else {
@@ -246,6 +254,8 @@ int sph_server_tick(sph_server* s, long long* finished, int max_finished, int* n
     }
     s->steps_done++;
     s->particle_steps += ps;
+    const double t_ret0 = wall();
+    s->t_step += t_ret0 - t_step0;
     int nf = 0;
     std::vector<Request*> keep;
     for (Request* r : s->active) {
@@ -262,6 +272,7 @@ int sph_server_tick(sph_server* s, long long* finished, int max_finished, int* n
         nf++;
     }
     s->active.swap(keep);
+    s->t_retire += wall() - t_ret0;
     if (n_finished) *n_finished = nf;
     return 0;
 }
@@ -286,6 +297,9 @@ int sph_server_status(const sph_server* s, sph_server_info* info) {
     info->ticks = s->steps_done;
     info->particle_steps = s->particle_steps;
     info->clock = s->now();
+    info->admit_s = s->t_admit;
+    info->step_s = s->t_step;
+    info->retire_s = s->t_retire;
     return 0;
 }
 

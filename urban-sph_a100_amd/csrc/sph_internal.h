@@ -79,10 +79,13 @@ struct PPEView {
     const int* rowof;             // particle -> row or -1
     const int* cnt;               // entries per row (off-diagonal)
     const long long* sl_off;      // first entry of each 64-row slice
-    const int* cols;
+    const int* cols;              // int32 columns (null when the case uses ecols)
+    const unsigned short* ecols;  // MIXED: 16-bit columns (window group, offset), or null
+    const int* gbase;             // MIXED: first column of each window group, 64 per slice
     double* vals64;
     float* vals32;
     double *diag, *rhs, *x, *pnew;
+    float* fdiag;                 // MIXED: float diagonal of the inner (float) operator
     double *r, *rhat, *p, *phat, *v, *s, *shat, *t;
     float *fr, *frhat, *fp, *fphat, *fv, *fs, *fshat, *ft, *fx, *frhs;
     double *partA, *partB;
