@@ -121,7 +121,7 @@ def canyon(dx=0.008, Lx=2.0, Ly=1.0, Lz=0.5, buildings=None, bsize=(0.1, 0.1, 0.
     handle (the flow next to the faces blows up within a few steps).  Inlet (buffer 1) is x < L2, outlet
     (buffer 2) x >= L3, periodic side buffers (3/4) y < L_left and y > L_right,
     each ``buffers`` layers wide (functions_NNPS.cuh:253-275).  The first fluid
-    layer is the MOST layer (MOST_buffer = 1).  Velocity: log law with u*, z0
+    layer outside the buildings is the MOST layer (MOST_buffer = 1).  Velocity: log law with u*, z0
     (scaled by u_scale) for fluid and lid particles.
 
     Returns (part1 array, normals (n,3) float32, geometry dict)."""
@@ -201,7 +201,7 @@ def canyon(dx=0.008, Lx=2.0, Ly=1.0, Lz=0.5, buildings=None, bsize=(0.1, 0.1, 0.
     p["buffer_type"] = bt
     p["i_type"] = it
     most = np.zeros(n, np.int32)
-    most[:nfl][first_layer & (bt[:nfl] < 3)] = 1
+    most[:nfl][first_layer & (bt[:nfl] < 3) & ~fsolid] = 1   # open ground only, not under a building
     p["MOST_buffer"] = most
     normals = np.zeros((n, 3), np.float32)
     normals[nfl + nw + nl:] = nrm
