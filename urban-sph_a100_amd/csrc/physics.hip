@@ -420,7 +420,10 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_neumann(Seg sg, int which, int on
                 if (s == -1) continue;
                 int e = g.gend[cell];
                 for (int j = s; j < e; j++) {
-                    if (A.p_type[j] >= 1000) continue;
+                    // only fluid, non-buffer neighbours carry weight (the
+                    // reference multiplies the others by 0; skipping those
+                    // exact zeros leaves every sum bit for bit the same)
+                    if (A.p_type[j] != 1 || A.buffer_type[j] != 0) continue;
                     R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
                     R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
                     if (!(tdist < search_range)) continue;
@@ -429,17 +432,16 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_neumann(Seg sg, int which, int on
                     R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
                     R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
                     R tdwx = dx / tdist * tdwij, tdwy = dy / tdist * tdwij, tdwz = dz / tdist * tdwij;
-                    R neighbor = (R)((A.p_type[j] == 1) * (A.buffer_type[j] == 0));
-                    tpres += (double)(mj / rhoj * (R)presj * twij * neighbor);
-                    tux += (double)(mj / rhoj * (R)A.ux[j] * twij * neighbor);
-                    tuy += (double)(mj / rhoj * (R)A.uy[j] * twij * neighbor);
-                    tuz += (double)(mj / rhoj * (R)A.uz[j] * twij * neighbor);
-                    tflt_s += (double)(mj / rhoj * twij * neighbor);
+                    tpres += (double)(mj / rhoj * (R)presj * twij);
+                    tux += (double)(mj / rhoj * (R)A.ux[j] * twij);
+                    tuy += (double)(mj / rhoj * (R)A.uy[j] * twij);
+                    tuz += (double)(mj / rhoj * (R)A.uz[j] * twij);
+                    tflt_s += (double)(mj / rhoj * twij);
                     if (inlet) {
                         R dp = (R)(presj - pres_g);
-                        grad_px += (double)(mj / rhoj * dp * tdwx * neighbor);
-                        grad_py += (double)(mj / rhoj * dp * tdwy * neighbor);
-                        grad_pz += (double)(mj / rhoj * dp * tdwz * neighbor);
+                        grad_px += (double)(mj / rhoj * dp * tdwx);
+                        grad_py += (double)(mj / rhoj * dp * tdwy);
+                        grad_pz += (double)(mj / rhoj * dp * tdwz);
                     }
                 }
             }
@@ -689,17 +691,19 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_freeslip(Seg sg, int which) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_HM)) continue;
         int j = e & NB_IDX;
+        // fluid neighbours only (the reference weights the others by an
+        // exact 0, which leaves the sums unchanged)
+        if (A.p_type[j] != 1) continue;
         R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
         R mj = (R)A.m[j], rhoj = (R)A.rho[j];
-        R neighbor = (R)(A.p_type[j] == 1);
         R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
-        tux += (double)((R)A.ux[j] * (mj / rhoj) * twij * neighbor);
-        tuy += (double)((R)A.uy[j] * (mj / rhoj) * twij * neighbor);
-        tuz += (double)((R)A.uz[j] * (mj / rhoj) * twij * neighbor);
-        tpres += (double)((R)A.pres[j] * (mj / rhoj) * twij * neighbor);
-        ttemp += (double)((R)A.temp[j] * (mj / rhoj) * twij * neighbor);
-        tflt_s += (double)((mj / rhoj) * twij * neighbor);
+        tux += (double)((R)A.ux[j] * (mj / rhoj) * twij);
+        tuy += (double)((R)A.uy[j] * (mj / rhoj) * twij);
+        tuz += (double)((R)A.uz[j] * (mj / rhoj) * twij);
+        tpres += (double)((R)A.pres[j] * (mj / rhoj) * twij);
+        ttemp += (double)((R)A.temp[j] * (mj / rhoj) * twij);
+        tflt_s += (double)((mj / rhoj) * twij);
     }
     if (tflt_s > 1e-5) {
         A.ux[i] = tux / tflt_s; A.uy[i] = tuy / tflt_s; A.uz[i] = tuz / tflt_s;

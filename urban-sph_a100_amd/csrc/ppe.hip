@@ -50,6 +50,15 @@ constexpr int EC_OFF_BITS = 10;
 constexpr int EC_OFF_MASK = (1 << EC_OFF_BITS) - 1;
 constexpr int EC_MAX_GROUPS = 64;
 
+// Matrix layout: sliced ELLPACK over 64-row slices, one wave64 per slice,
+// entries column-major within a slice: entry k of row r lives at
+//     sl_off[r / 64] + k * 64 + r % 64,
+// so the wave's loads of one slot are one contiguous block.
+__host__ __device__ __forceinline__ long long prow(const long long* sl_off, int r) {
+    return sl_off[r >> 6] + (r & 63);
+}
+__host__ __device__ __forceinline__ long long pslot(int k) { return (long long)k * SLICE; }
+
 struct PPEState {
     int nr = 0;                 // unknowns
     int nslices = 0;
@@ -178,7 +187,7 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_struct(int nr, const int* rows, c
     const unsigned* row = nl.ent + nl.slice_off[i >> 6] + (i & 63);
     int m = nl.cnt[i];
     int slot = 0;
-    int* out = FILL ? cols + sl_off[r >> 6] + (r & 63) : nullptr;
+    int* out = FILL ? cols + prow(sl_off, r) : nullptr;
     for (int k = 0; k < m; k++) {
         unsigned e = row[(size_t)k * SLICE];
         if (!(e & F_DP)) continue;
@@ -187,7 +196,7 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_struct(int nr, const int* rows, c
         if (j == i) continue;
         int rj = rowof[j];
         if (rj < 0) continue;
-        if (FILL) out[(size_t)slot * SLICE] = rj;
+        if (FILL) out[pslot(slot)] = rj;
         slot++;
     }
     if (!FILL) cnt[r] = slot;
@@ -219,8 +228,8 @@ __global__ void __launch_bounds__(SLICE) k_ecols_build(int nr, const long long* 
     const int* cp = cols + sl_off[sl] + l;
     int p = 0, ng = 0, cur = -1;
     while (true) {
-        while (p < m && (cp[(size_t)p * SLICE] >> EC_OFF_BITS) <= cur) p++;
-        int k = p < m ? (cp[(size_t)p * SLICE] >> EC_OFF_BITS) : INT_MAX;
+        while (p < m && (cp[pslot(p)] >> EC_OFF_BITS) <= cur) p++;
+        int k = p < m ? (cp[pslot(p)] >> EC_OFF_BITS) : INT_MAX;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) k = min(k, __shfl_xor(k, off, 64));
         if (k == INT_MAX) break;
@@ -237,10 +246,10 @@ __global__ void __launch_bounds__(SLICE) k_ecols_build(int nr, const long long* 
     unsigned short* ep = ecols + sl_off[sl] + l;
     int g = 0;
     for (int k2 = 0; k2 < m; k2++) {
-        const int c = cp[(size_t)k2 * SLICE];
+        const int c = cp[pslot(k2)];
         const int key = c >> EC_OFF_BITS;
         while (g + 1 < ng && gk[g] < key) g++;
-        ep[(size_t)k2 * SLICE] = (unsigned short)((g << EC_OFF_BITS) | (c & EC_OFF_MASK));
+        ep[pslot(k2)] = (unsigned short)((g << EC_OFF_BITS) | (c & EC_OFF_MASK));
     }
 }
 
@@ -297,7 +306,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     double bix = 0, biy = 0, biz = 0, Aij = 0, AijPij = 0, kn = 0;
     const unsigned* nrow = nl.ent + nl.slice_off[i >> 6] + (i & 63);
     int m = nl.cnt[i];
-    V* out = jacobi ? nullptr : wvals_of(pp, (V*)nullptr) + pp.sl_off[r >> 6] + (r & 63);
+    V* out = jacobi ? nullptr : wvals_of(pp, (V*)nullptr) + prow(pp.sl_off, r);
     int slot = 0;
     for (int k = 0; k < m; k++) {
         unsigned e = nrow[(size_t)k * SLICE];
@@ -325,7 +334,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         if (jacobi) AijPij += ccd * qj;
         if (j != i) {
             if (rowof[j] >= 0) {
-                if (!jacobi) out[(size_t)slot * SLICE] = (V)cc;
+                if (!jacobi) out[pslot(slot)] = (V)cc;
                 slot++;
             } else {
                 kn += ccd * qj;
@@ -363,62 +372,57 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_scatter(Seg sg, int jacobi) {
 // Off-diagonal part of one row, sum_k vals[k] * x[col[k]], accumulated in
 // double.  Matrix entries stream through once: nontemporal loads keep the
 // caches for the x gathers; four independent loads in flight per lane.
+// Off-diagonal part of one row, sum_k vals[k] * x[col[k]], accumulated in
+// double in entry order.  Matrix entries stream through once: nontemporal
+// loads keep the caches for the x gathers; four independent loads in flight
+// per lane.
 template <class V, class X>
 __device__ __forceinline__ double offdiag_i32(const int* cp, const V* vp, int m, const X* x) {
     double s = 0.0;
     int k = 0;
     for (; k + 4 <= m; k += 4) {
-        const int c0 = __builtin_nontemporal_load(cp + (size_t)(k + 0) * SLICE);
-        const int c1 = __builtin_nontemporal_load(cp + (size_t)(k + 1) * SLICE);
-        const int c2 = __builtin_nontemporal_load(cp + (size_t)(k + 2) * SLICE);
-        const int c3 = __builtin_nontemporal_load(cp + (size_t)(k + 3) * SLICE);
-        const V v0 = __builtin_nontemporal_load(vp + (size_t)(k + 0) * SLICE);
-        const V v1 = __builtin_nontemporal_load(vp + (size_t)(k + 1) * SLICE);
-        const V v2 = __builtin_nontemporal_load(vp + (size_t)(k + 2) * SLICE);
-        const V v3 = __builtin_nontemporal_load(vp + (size_t)(k + 3) * SLICE);
+        const int c0 = __builtin_nontemporal_load(cp + pslot(k + 0));
+        const int c1 = __builtin_nontemporal_load(cp + pslot(k + 1));
+        const int c2 = __builtin_nontemporal_load(cp + pslot(k + 2));
+        const int c3 = __builtin_nontemporal_load(cp + pslot(k + 3));
+        const V v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
+        const V v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
+        const V v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
+        const V v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
         const X x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
         s += (double)v0 * (double)x0;
         s += (double)v1 * (double)x1;
         s += (double)v2 * (double)x2;
         s += (double)v3 * (double)x3;
     }
-    for (; k < m; k++) {
-        const int c0 = __builtin_nontemporal_load(cp + (size_t)k * SLICE);
-        const V v0 = __builtin_nontemporal_load(vp + (size_t)k * SLICE);
-        s += (double)v0 * (double)x[c0];
-    }
+    for (; k < m; k++) s += (double)__builtin_nontemporal_load(vp + pslot(k)) * (double)x[__builtin_nontemporal_load(cp + pslot(k))];
     return s;
 }
 // The same over 16-bit columns: entry e = (group << EC_OFF_BITS) | offset,
 // column = wb[group] + offset, wb = the slice's group bases (in LDS).
+__device__ __forceinline__ int ecol(const int* wb, int e) { return wb[e >> EC_OFF_BITS] + (e & EC_OFF_MASK); }
 template <class X>
 __device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const float* vp, int m, const int* wb,
                                               const X* x) {
     double s = 0.0;
     int k = 0;
     for (; k + 4 <= m; k += 4) {
-        const int e0 = __builtin_nontemporal_load(ep + (size_t)(k + 0) * SLICE);
-        const int e1 = __builtin_nontemporal_load(ep + (size_t)(k + 1) * SLICE);
-        const int e2 = __builtin_nontemporal_load(ep + (size_t)(k + 2) * SLICE);
-        const int e3 = __builtin_nontemporal_load(ep + (size_t)(k + 3) * SLICE);
-        const float v0 = __builtin_nontemporal_load(vp + (size_t)(k + 0) * SLICE);
-        const float v1 = __builtin_nontemporal_load(vp + (size_t)(k + 1) * SLICE);
-        const float v2 = __builtin_nontemporal_load(vp + (size_t)(k + 2) * SLICE);
-        const float v3 = __builtin_nontemporal_load(vp + (size_t)(k + 3) * SLICE);
-        const X x0 = x[wb[e0 >> EC_OFF_BITS] + (e0 & EC_OFF_MASK)];
-        const X x1 = x[wb[e1 >> EC_OFF_BITS] + (e1 & EC_OFF_MASK)];
-        const X x2 = x[wb[e2 >> EC_OFF_BITS] + (e2 & EC_OFF_MASK)];
-        const X x3 = x[wb[e3 >> EC_OFF_BITS] + (e3 & EC_OFF_MASK)];
+        const int e0 = __builtin_nontemporal_load(ep + pslot(k + 0));
+        const int e1 = __builtin_nontemporal_load(ep + pslot(k + 1));
+        const int e2 = __builtin_nontemporal_load(ep + pslot(k + 2));
+        const int e3 = __builtin_nontemporal_load(ep + pslot(k + 3));
+        const float v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
+        const float v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
+        const float v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
+        const float v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
+        const X x0 = x[ecol(wb, e0)], x1 = x[ecol(wb, e1)], x2 = x[ecol(wb, e2)], x3 = x[ecol(wb, e3)];
         s += (double)v0 * (double)x0;
         s += (double)v1 * (double)x1;
         s += (double)v2 * (double)x2;
         s += (double)v3 * (double)x3;
     }
-    for (; k < m; k++) {
-        const int e0 = __builtin_nontemporal_load(ep + (size_t)k * SLICE);
-        const float v0 = __builtin_nontemporal_load(vp + (size_t)k * SLICE);
-        s += (double)v0 * (double)x[wb[e0 >> EC_OFF_BITS] + (e0 & EC_OFF_MASK)];
-    }
+    for (; k < m; k++)
+        s += (double)__builtin_nontemporal_load(vp + pslot(k)) * (double)x[ecol(wb, __builtin_nontemporal_load(ep + pslot(k)))];
     return s;
 }
 
@@ -433,7 +437,7 @@ __device__ __forceinline__ void load_group_bases(const PPEView& pp, int first, i
 template <class V, class X, class DG>
 __device__ __forceinline__ double spmv_row(const PPEView& pp, int r, const V* vals, const DG* dg, const X* x,
                                            const int* wbase) {
-    const long long base = pp.sl_off[r >> 6] + (r & 63);
+    const long long base = prow(pp.sl_off, r);
     const int m = pp.cnt[r];
     double s;
     if constexpr (sizeof(V) == 4) {
