@@ -237,13 +237,23 @@ def main():
                     help="run only a 1 GiB device-to-device copy kernel (for scaling HBM counters)")
     args = ap.parse_args()
     if args.copy_calib:
+        # 1 GiB device-to-device copy: the reference for scaling the HBM
+        # byte counters, and the achievable copy bandwidth of this HBM
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_bytes_read": a.numel() * 4, "copy_calib_bytes_written": a.numel() * 4}))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / 10
+        print(json.dumps({"copy_calib_bytes_read": a.numel() * 4, "copy_calib_bytes_written": a.numel() * 4,
+                          "copy_GBps_read_plus_write": round(2 * a.numel() * 4 / sec / 1e9, 1)}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
