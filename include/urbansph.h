@@ -151,6 +151,12 @@ typedef struct sph_params {
     int    ppe_index16;         /* MIXED: 16-bit PPE column indices where they
                                    fit (1, default) or 32-bit (0); the results
                                    are identical, only the bytes moved differ */
+    int    ppe_reorder;         /* MIXED: number the PPE unknowns in lattice
+                                   (z, y, x) order for contiguous SpMV gathers
+                                   (1, default) or keep the particles' cell
+                                   order (0)                                  */
+    int    apr_h_change;        /* h_change (vii[36]): refined children get
+                                   h/2 (1, default) or keep h (0)             */
 } sph_params;
 
 /* ---------------------------------------------------------------------------
@@ -300,6 +306,19 @@ typedef struct sph_kernel_time {
     double alg_bytes;           /* sum over launches of algorithmic bytes    */
 } sph_kernel_time;
 int sph_profile_report(sph_ctx* c, sph_kernel_time* out, int max);
+
+/* Adaptive particle refinement, the splitting half of Particle_refinement_cell
+ * (functions_APR_etc.cuh:354-375; KERNEL_APR_condition3D_cell and
+ * KERNEL_cell_APR_3D, functions_APR.cuh:270-500).  target_level (n values,
+ * device order, or NULL to keep the current ones, initially 0) is the
+ * reference's M_num; children inherit it.  Each pass splits, in every cell,
+ * the first particle whose target exceeds its level log8(m_ref/m) by >= 0.75
+ * into 8 children (m/8, vol/8, h/2, offsets +-cbrt(m/rho)/4), appended after
+ * the current particles; the parent becomes a dummy (i_type 3).  The search
+ * runs again after every pass; passes stop early when none splits.  *n_out
+ * receives the new particle count (dummies included).  Needs a search (a
+ * count == 0 step) beforehand and room in the context (n_part2). */
+int sph_apr_refine(sph_ctx* c, const double* target_level, int max_passes, int* n_out);
 
 /* Device memory held by the context, bytes. */
 long long sph_device_bytes(const sph_ctx* c);

@@ -33,6 +33,7 @@ def _load():
     lib.oracle_create.argtypes = [C.POINTER(abi.Params), C.c_int]
     lib.oracle_upload.argtypes = [vp, vp, vp, vp, C.c_int]
     lib.oracle_step.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+    lib.oracle_apr_refine.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
     lib.oracle_get_field_f64.argtypes = [vp, C.c_int, vp, C.c_int]
     lib.oracle_get_field_i32.argtypes = [vp, C.c_int, vp, C.c_int]
     lib.oracle_nnps.argtypes = [vp, vp, vp, vp, vp]
@@ -98,6 +99,17 @@ class Oracle:
         if time is None:
             time = count * dt
         return lib.oracle_step(self.h, int(count), float(dt), float(time))
+
+    def apr_refine(self, target=None, max_passes=1):
+        """Refinement passes; target: per-particle target level in the
+        current (cell-sorted) order, or None to keep the current targets."""
+        t = None if target is None else np.ascontiguousarray(target, np.float64)
+        n_out = C.c_int(0)
+        rc = lib.oracle_apr_refine(self.h, abi.ptr(t), int(max_passes), C.byref(n_out))
+        if rc:
+            raise RuntimeError(f"oracle_apr_refine rc={rc}")
+        self.n = n_out.value
+        return self.n
 
     def field(self, name):
         out = np.empty(self.n, np.float64)

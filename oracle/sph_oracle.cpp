@@ -85,6 +85,7 @@ struct Oracle {
     std::vector<P3d> P3;
     std::vector<double> nx, ny, nz;            // persistent IB normals
     std::vector<int> keys, perm, gstr, gend;
+    std::vector<double> apr_target;            // APR target level (the reference's M_num)
     // last PPE system (BICGSTAB), rows in ascending particle order
     std::vector<int> sys_rows, sys_rowptr, sys_cols;
     std::vector<double> sys_diag, sys_rhs, sys_vals, sys_x0, sys_x;
@@ -384,6 +385,11 @@ void nnps(Oracle& o) {
     }
     o.SP1 = s1; o.SP2 = s2; o.P1 = s1; o.P2 = s2;
     o.nx = tnx; o.ny = tny; o.nz = tnz;
+    if ((int)o.apr_target.size() == n) {
+        std::vector<double> t(n);
+        for (int i = 0; i < n; i++) t[i] = o.apr_target[idx[i]];
+        o.apr_target = t;
+    }
     o.searched = true;
 }
 
@@ -1320,6 +1326,77 @@ void ibm_corrector(Oracle& o) {
 }
 
 // SOPHIA_single_ISPH, ISPH_Calc.cuh:4-233
+// ------------------------------------------------------------ APR --------
+// Cube root by Newton's method on the mantissa, 8 iterations from 1.
+double apr_cbrt(double v) {
+    int e = 0;
+    double m = frexp(v, &e);
+    const int r = ((e % 3) + 3) % 3;
+    m = ldexp(m, r);
+    e -= r;
+    double y = 1.0;
+    for (int it = 0; it < 8; it++) y = y - (y * y * y - m) / (3.0 * y * y);
+    return ldexp(y, e / 3);
+}
+
+// Splitting passes of Particle_refinement_cell (functions_APR_etc.cuh:354-375):
+// the condition of KERNEL_APR_condition3D_cell (functions_APR.cuh:270-330;
+// the first particle of a cell with M_num - log8(m_ref/m) >= 0.75), the
+// host prefix sum Cell_index, and KERNEL_cell_APR_3D (functions_APR.cuh:
+// 420-500: 8 children at +-cbrt(V)/4, m/8, vol/8, h/2, at n + 8*rank; the
+// parent becomes i_type 3).  The search runs again after each pass.
+int apr_refine(Oracle& o, const double* target, int max_passes, int* n_out) {
+    if (!o.searched) return SPH_E_STATE;
+    if ((int)o.apr_target.size() != o.n) o.apr_target.assign(o.n, 0.0);
+    if (target) for (int i = 0; i < o.n; i++) o.apr_target[i] = target[i];
+    for (int pass = 0; pass < max_passes; pass++) {
+        const int ncells = o.d.num_cells - 1;
+        std::vector<int> pick;
+        for (int cell = 0; cell < ncells; cell++) {
+            if (o.gstr[cell] == -1) continue;
+            for (int j = o.gstr[cell]; j < o.gend[cell]; j++) {
+                const P1d& p = o.P1[j];
+                double level = log(p.m_ref / p.m) / log(8.0);
+                if (o.apr_target[j] - level >= 0.75) {
+                    if (!(p.p_type != 1 && o.apr_target[j] > 2.0)) pick.push_back(j);
+                    break;
+                }
+            }
+        }
+        const int ns = (int)pick.size();
+        if (ns == 0) break;
+        if ((long long)o.n + 8LL * ns > o.cap) return SPH_E_NOMEM;
+        const int n0 = o.n;
+        o.n += 8 * ns;
+        o.P1.resize(o.n); o.P2.resize(o.n);
+        o.nx.resize(o.n); o.ny.resize(o.n); o.nz.resize(o.n); o.apr_target.resize(o.n);
+        for (int k = 0; k < ns; k++) {
+            const int j = pick[k];
+            const P1d par = o.P1[j];
+            const double a = 0.25 * apr_cbrt(par.m / par.rho);
+            for (int q = 0; q < 8; q++) {
+                const int c = n0 + 8 * k + q;
+                P1d ch = par;
+                ch.x = q < 4 ? par.x + a : par.x - a;
+                ch.y = (q & 2) ? par.y - a : par.y + a;
+                ch.z = (q & 1) ? par.z - a : par.z + a;
+                ch.m = 0.125 * par.m;
+                ch.vol = 0.125 * par.vol;
+                if (o.prm.apr_h_change) ch.h = 0.5 * par.h;
+                o.P1[c] = ch;
+                o.P2[c] = o.P2[j];
+                o.nx[c] = o.nx[j]; o.ny[c] = o.ny[j]; o.nz[c] = o.nz[j];
+                o.apr_target[c] = o.apr_target[j];
+            }
+            o.P1[j].i_type = 3;
+        }
+        set_ncell(o);
+        nnps(o);
+    }
+    if (n_out) *n_out = o.n;
+    return 0;
+}
+
 int step(Oracle& o, int count, double dt, double time) {
     const sph_params& pr = o.prm;
     set_ncell(o);
@@ -1412,7 +1489,7 @@ void oracle_params_default(sph_params* p) {
     p->most_ghost_dz = 5.0; p->most_ghost_r = 0.1;
     p->ibm_ut_factor = 0.2;
     p->ppe_solver = SPH_PPE_REF_JACOBI; p->precision = SPH_FP64; p->ppe_tol = 1e-8; p->ppe_maxit = 5000;
-    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1;
+    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1;
 }
 
 void* oracle_create(const sph_params* prm, int n) {
@@ -1455,6 +1532,12 @@ int oracle_upload(void* h, const sph_part1* P1, const sph_part2* P2, const float
     o.uploaded = true;
     o.searched = false;
     return 0;
+}
+
+int oracle_apr_refine(void* h, const double* target, int max_passes, int* n_out) {
+    Oracle& o = *(Oracle*)h;
+    if (!o.uploaded) return SPH_E_STATE;
+    return apr_refine(o, target, max_passes, n_out);
 }
 
 int oracle_step(void* h, int count, double dt, double time) {

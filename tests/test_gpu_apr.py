@@ -1,0 +1,96 @@
+"""Adaptive particle refinement through the C ABI against the oracle:
+the same particles split in every pass, the searches after refinement are
+bit-exact, and the refined case steps to the FP64 tolerance."""
+import numpy as np
+import pytest
+
+import abi
+import cases
+import oracle
+import urbansph
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["UX", "UY", "UZ", "PRES", "VIS_T", "FTOTALX", "FPX", "FPZ", "FBX", "INV_CM_XX", "X", "Y", "Z", "M", "H"]
+
+
+def rel_l2(a, b):
+    s = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / s if s > 0 else np.linalg.norm(a - b)
+
+
+def _targets(case, geom_buildings, r1, r2):
+    x, y, z = case.field("X"), case.field("Y"), case.field("Z")
+    return cases.apr_targets(x, y, z, case.ifield("P_TYPE"), case.ifield("I_TYPE"), geom_buildings, r1, r2)
+
+
+@pytest.mark.parametrize("passes", [1, 40])
+def test_apr_canyon_fp64(passes):
+    P, nrm, geom = cases.small_canyon(dx=0.02)
+    prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-10)
+    cap = 40 * len(P)
+    o = oracle.Oracle(prm, cap)
+    o.upload(P, None, nrm)
+    g = urbansph.Case(prm, cap)
+    g.upload(P, None, nrm)
+    dt = 5e-5
+    assert o.step(0, dt) == 0
+    g.step(0, dt)
+    t = _targets(g, geom["buildings"], 0.05, 0.03)
+    print(f"[apr] n={len(P)} targets: {int((t == 1).sum())} at level 1, {int((t == 2).sum())} at level 2", flush=True)
+    n_o = o.apr_refine(t, max_passes=passes)
+    n_g = g.apr_refine(t, max_passes=passes)
+    print(f"[apr] passes={passes}: n {len(P)} -> {n_g}", flush=True)
+    assert n_g == n_o and n_g > len(P)
+    ko, po, so, eo = o.nnps()
+    kg, pg, sg, eg = g.nnps()
+    np.testing.assert_array_equal(kg, ko)
+    np.testing.assert_array_equal(pg, po)
+    np.testing.assert_array_equal(sg, so)
+    np.testing.assert_array_equal(eg, eo)
+    for kind in (0, 1):
+        oo, no = o.neighbors(kind)
+        og, ng = g.neighbors(kind)
+        np.testing.assert_array_equal(og, oo)
+        np.testing.assert_array_equal(ng, no)
+    for c in (1, 2):
+        assert o.step(c, dt) == 0
+        g.step(c, dt)
+    g.sync()
+    live = g.ifield("I_TYPE") != 3
+    for f in FIELDS:
+        a, b = g.field(f)[live], o.field(f)[live]
+        assert np.all(np.isfinite(a)), f
+        e = rel_l2(a, b)
+        assert e <= 1e-10, f"{f}: {e:.3e}"
+
+
+def test_apr_mixed_runs():
+    P, nrm, geom = cases.small_canyon(dx=0.02)
+    prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-6, precision=abi.MIXED)
+    g = urbansph.Case(prm, 40 * len(P))
+    g.upload(P, None, nrm)
+    g.step(0, 5e-5)
+    n = g.apr_refine(_targets(g, geom["buildings"], 0.05, 0.03), max_passes=40)
+    for c in (1, 2, 3):
+        g.step(c, 5e-5)
+    g.sync()
+    r = g.ppe_report()
+    assert r.converged == 1 and r.rel_residual <= 1e-6
+    live = g.ifield("I_TYPE") != 3
+    assert np.all(np.isfinite(g.field("PRES")[live]))
+    assert n > len(P)
+
+
+def test_apr_needs_room_and_a_search():
+    P, _ = cases.box(nf=6, walls=3)
+    prm = cases.box_params(urbansph.default_params())
+    g = urbansph.Case(prm, len(P) + 8)
+    g.upload(P)
+    with pytest.raises(urbansph.SPHError) as e:
+        g.apr_refine(np.ones(len(P)), max_passes=1)
+    assert e.value.rc == abi.E_STATE
+    g.step(0, 1e-4)
+    with pytest.raises(urbansph.SPHError) as e:
+        g.apr_refine(np.ones(len(P)), max_passes=1)
+    assert e.value.rc == abi.E_NOMEM

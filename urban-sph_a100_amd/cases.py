@@ -206,7 +206,8 @@ def canyon(dx=0.008, Lx=2.0, Ly=1.0, Lz=0.5, buildings=None, bsize=(0.1, 0.1, 0.
     normals = np.zeros((n, 3), np.float32)
     normals[nfl + nw + nl:] = nrm
     geom = dict(L2=L2, L3=L3, L_left=L_left, L_right=L_right, outlet_copy_dx=dx,
-                n_fluid=nfl, n_wall=nw, n_lid=nl, n_marker=nm, dx=dx)
+                n_fluid=nfl, n_wall=nw, n_lid=nl, n_marker=nm, dx=dx,
+                buildings=[(xc, yc, bx, by, bh) for (xc, yc) in buildings])
     return p, normals, geom
 
 
@@ -232,6 +233,27 @@ def small_canyon(dx=0.01, **kw):
                 bsize=(0.05, 0.05, 0.08))
     args.update(kw)
     return canyon(**args)
+
+
+def apr_targets(x, y, z, p_type, i_type, buildings, r1, r2):
+    """Refinement target levels of the multi-resolution canyon: 2 for active
+    fluid outside the buildings within r2 of a building wall or roof, 1
+    within r1, 0 elsewhere.  ``buildings`` as in the canyon geometry
+    (xc, yc, bx, by, bh)."""
+    x, y, z = (np.asarray(v, np.float64) for v in (x, y, z))
+    d = np.full(x.shape, np.inf)
+    inside = np.zeros(x.shape, bool)
+    for (xc, yc, bx, by, bh) in buildings:
+        ex = np.maximum(np.abs(x - xc) - 0.5 * bx, 0.0)
+        ey = np.maximum(np.abs(y - yc) - 0.5 * by, 0.0)
+        ez = np.maximum(z - bh, 0.0)
+        d = np.minimum(d, np.sqrt(ex * ex + ey * ey + ez * ez))
+        inside |= (np.abs(x - xc) < 0.5 * bx) & (np.abs(y - yc) < 0.5 * by) & (z < bh)
+    active = (np.asarray(p_type) == 1) & (np.asarray(i_type) == 1) & ~inside
+    t = np.zeros(x.shape)
+    t[active & (d < r1)] = 1.0
+    t[active & (d < r2)] = 2.0
+    return t
 
 
 def sized_canyon(target, u_scale=1.0, bh=0.15, dx=0.008):

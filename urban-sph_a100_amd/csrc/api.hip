@@ -172,7 +172,7 @@ void sph_params_default(sph_params* p) {
     p->most_ghost_dz = 5.0; p->most_ghost_r = 0.1;
     p->ibm_ut_factor = 0.2;
     p->ppe_solver = SPH_PPE_REF_JACOBI; p->precision = SPH_FP64; p->ppe_tol = 1e-8; p->ppe_maxit = 5000;
-    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1;
+    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1;
 }
 
 static int validate(const sph_params* p, std::string& why) {
@@ -479,6 +479,14 @@ int sph_profile_report(sph_ctx* c, sph_kernel_time* out, int max) {
     return profile_report(*c->solo, out, max);
 }
 
+int sph_apr_refine(sph_ctx* c, const double* target_level, int max_passes, int* n_out) {
+    if (!c || max_passes < 0) return SPH_E_ARG;
+    if (!c->uploaded || !c->searched)
+        return fail(c, SPH_E_STATE, "refinement needs a neighbour search (a count == 0 step) first");
+    c->err.clear();
+    return apr_refine(c, target_level, max_passes, n_out);
+}
+
 long long sph_device_bytes(const sph_ctx* c) { return c ? c->bytes : 0; }
 
 // ------------------------------------------------------------ batches ----
@@ -554,7 +562,7 @@ void sph_destroy(sph_ctx* c) {
     for (auto p : c->q) dfree(c, p);
     for (auto p : c->snap) dfree(c, p);
     void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->nl_slice_off, c->nl_cnt,
-                    c->nl_ent, c->stat_buf};
+                    c->nl_ent, c->stat_buf, c->apr_target};
     for (auto p : rest) dfree(c, p);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;

@@ -340,6 +340,7 @@ static int permute_state(sph_ctx* c) {
     for (int f = 0; f < SPH_I_COUNT; f++) SPH_TRY(gi(c->pi[f]));
     for (int f = 0; f < 10; f++) SPH_TRY(gd(c->p2[f]));
     for (int f = 15; f < 18; f++) SPH_TRY(gd(c->q[f]));
+    SPH_TRY(gd(c->apr_target));
     SPH_TRY(prof_end(c));
     dfree(c, tmp);
     dfree(c, itmp);
@@ -379,15 +380,18 @@ static int build_neighbor_list(sph_ctx* c) {
     return 0;
 }
 
-// Full count==0 search on the current persistent state (any order).
-int nnps_run(sph_ctx* c) {
+// Full count==0 search on the current persistent state (any order): cell
+// keys, stable sort, cell bounds, permutation of the state and, when
+// build_list, the per-particle neighbour list.
+int nnps_run(sph_ctx* c, bool build_list) {
     const int n = c->n;
     const sph_domain& d = c->dom;
     if (!c->keys) {
-        SPH_TRY(dalloc(c, &c->keys, n));
-        SPH_TRY(dalloc(c, &c->perm, n));
-        SPH_TRY(dalloc(c, &c->keys_tmp, n));
-        SPH_TRY(dalloc(c, &c->perm_tmp, n));
+        // sized for the context's capacity: refinement adds particles
+        SPH_TRY(dalloc(c, &c->keys, c->cap));
+        SPH_TRY(dalloc(c, &c->perm, c->cap));
+        SPH_TRY(dalloc(c, &c->keys_tmp, c->cap));
+        SPH_TRY(dalloc(c, &c->perm_tmp, c->cap));
     }
     if (!c->gstr) {
         SPH_TRY(dalloc(c, &c->gstr, (size_t)d.num_cells + 1));
@@ -410,7 +414,7 @@ int nnps_run(sph_ctx* c) {
     SPH_TRY(launch_check(c, "cell_bounds"));
     SPH_TRY(prof_end(c));
     SPH_TRY(permute_state(c));
-    SPH_TRY(build_neighbor_list(c));
+    if (build_list) SPH_TRY(build_neighbor_list(c));
     c->searched = true;
     c->generation++;   // state arrays were swapped for their sorted copies
     return 0;

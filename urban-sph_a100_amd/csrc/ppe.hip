@@ -34,6 +34,8 @@
 // float, every SpMV row sum and every dot is accumulated in double; an outer
 // iterative-refinement loop computes the residual with double vectors and
 // double accumulation and adds the float correction to a double solution.
+#include <algorithm>
+
 #include "flags.h"
 #include "sph_internal.h"
 #include "sph_math.h"
@@ -71,6 +73,7 @@ struct PPEState {
     int* cols = nullptr;
     unsigned short* ecols = nullptr;
     int* gbase = nullptr;
+    unsigned char* slotmap = nullptr;   // lattice row order: slot of each entry in list order
     double* vals64 = nullptr;
     float* vals32 = nullptr;
     double* diag = nullptr;
@@ -212,6 +215,71 @@ __global__ void __launch_bounds__(BLOCK) k_row_slice_width(int nr, int nslices, 
     w[s] = (long long)m * SLICE;
 }
 
+// Lattice row order of the MIXED matrix.  Rows are renumbered by the
+// (z, y, x) lattice position of their particle, so the 64 rows of a slice
+// are consecutive lattice sites whose k-th neighbours are consecutive too,
+// and the x gathers of a wave fall on a few cache lines.  Only the PPE's
+// unknowns are renumbered; particles keep the reference's cell order.
+__global__ void __launch_bounds__(BLOCK) k_lattice_key(int nr, const int* rows, PView S, double x0, double y0,
+                                                       double z0, double q, long long nx, long long ny,
+                                                       long long* key) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const int i = rows[r];
+    const long long kx = (long long)floor((S.x[i] - x0) / q + 0.5);
+    const long long ky = (long long)floor((S.y[i] - y0) / q + 0.5);
+    const long long kz = (long long)floor((S.z[i] - z0) / q + 0.5);
+    key[r] = (kz * ny + ky) * nx + kx;
+}
+__global__ void __launch_bounds__(BLOCK) k_rowof_from_rows(int nr, const int* rows, int* rowof) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nr) rowof[rows[r]] = r;
+}
+// Columns of each row in ascending (lattice) order, sorted in LDS, and the
+// slot each entry takes, indexed by its rank in the neighbour list (the
+// order in which the assembly visits it).  Rows longer than SORT_CAP set
+// *overflow (the case then keeps cell order).
+constexpr int SORT_CAP = 160;
+__global__ void __launch_bounds__(SLICE) k_ppe_struct_sorted(int nr, const int* rows, const int* rowof, PView S,
+                                                             NList nl, const int* cnt, const long long* sl_off,
+                                                             int* cols, unsigned char* slotmap, int* overflow) {
+    __shared__ int sc[SLICE][SORT_CAP + 1];
+    __shared__ unsigned char so[SLICE][SORT_CAP];
+    const int t = threadIdx.x;
+    const int r = blockIdx.x * SLICE + t;
+    if (r >= nr) return;
+    const int m = cnt[r];
+    if (m > SORT_CAP) { *overflow = 1; return; }
+    const int i = rows[r];
+    const unsigned* row = nl.ent + nl.slice_off[i >> 6] + (i & 63);
+    const int ml = nl.cnt[i];
+    int e = 0;
+    for (int k = 0; k < ml; k++) {
+        unsigned en = row[(size_t)k * SLICE];
+        if (!(en & F_DP)) continue;
+        int j = en & NB_IDX;
+        if (S.p_type[j] >= 1000 || S.i_type[j] == 4) continue;
+        if (j == i) continue;
+        int rj = rowof[j];
+        if (rj < 0) continue;
+        int pos = e;
+        while (pos > 0 && sc[t][pos - 1] > rj) {
+            sc[t][pos] = sc[t][pos - 1];
+            so[t][pos] = so[t][pos - 1];
+            pos--;
+        }
+        sc[t][pos] = rj;
+        so[t][pos] = (unsigned char)e;
+        e++;
+    }
+    int* out = cols + prow(sl_off, r);
+    unsigned char* sm = slotmap + prow(sl_off, r);
+    for (int p2 = 0; p2 < e; p2++) {
+        out[pslot(p2)] = sc[t][p2];
+        sm[pslot(so[t][p2])] = (unsigned char)p2;
+    }
+}
+
 // 16-bit columns of the MIXED matrix, one 64-thread block per 64-row slice.
 // The distinct aligned 1024-column windows that the slice's entries touch
 // become its groups, found in ascending order by repeated wave minima (each
@@ -307,6 +375,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     const unsigned* nrow = nl.ent + nl.slice_off[i >> 6] + (i & 63);
     int m = nl.cnt[i];
     V* out = jacobi ? nullptr : wvals_of(pp, (V*)nullptr) + prow(pp.sl_off, r);
+    const unsigned char* smap = pp.slotmap ? pp.slotmap + prow(pp.sl_off, r) : nullptr;
     int slot = 0;
     for (int k = 0; k < m; k++) {
         unsigned e = nrow[(size_t)k * SLICE];
@@ -334,7 +403,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         if (jacobi) AijPij += ccd * qj;
         if (j != i) {
             if (rowof[j] >= 0) {
-                if (!jacobi) out[pslot(slot)] = (V)cc;
+                if (!jacobi) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
                 slot++;
             } else {
                 kn += ccd * qj;
@@ -699,13 +768,53 @@ __global__ void __launch_bounds__(BLOCK) k_zero(long long n, T* a) {
 void ppe_free(sph_ctx* c) {
     PPEState* P = c->ppe;
     if (!P) return;
-    void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->ecols, P->gbase, P->vals64, P->vals32,
+    void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->ecols, P->gbase, P->slotmap, P->vals64, P->vals32,
                     P->diag, P->fdiag, P->rhs, P->x, P->pnew, P->r, P->rhat, P->p, P->phat, P->v, P->s, P->shat, P->t, P->fr, P->frhat, P->fp,
                     P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->partA, P->partB};
     for (void* q : ptrs) dfree(c, q);
     delete P;
     c->ppe = nullptr;
     c->generation++;
+}
+
+// Renumbers the rows by lattice position (see k_lattice_key).  The lattice
+// spacing is taken as h / 1.5 of the first row's particle; keys are
+// rounded, so jitter below half a spacing leaves the order intact.
+static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
+    const int nr = P->nr;
+    PView S = c->sview();
+    int r0 = 0;
+    double h0 = 0.0;
+    SPH_HIP(c, hipMemcpyAsync(&r0, P->rows, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    SPH_HIP(c, hipMemcpyAsync(&h0, S.h + r0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    const double q = h0 / 1.5;
+    const sph_domain& d = c->dom;
+    const double ex = ((double)d.x_max - d.x_min) / q + 3.0, ey = ((double)d.y_max - d.y_min) / q + 3.0,
+                 ez = ((double)d.z_max - d.z_min) / q + 3.0;
+    if (!(q > 0) || ex * ey * ez > 4.0e18) { *used = false; return 0; }
+    const double x0 = (double)d.x_min - q, y0 = (double)d.y_min - q, z0 = (double)d.z_min - q;
+    long long* key = nullptr;
+    SPH_TRY(dalloc(c, &key, nr));
+    hipLaunchKernelGGL(k_lattice_key, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows, S, x0, y0,
+                       z0, q, (long long)ex, (long long)ey, key);
+    SPH_TRY(launch_check(c, "lattice_key"));
+    std::vector<long long> hk(nr);
+    std::vector<int> hr(nr), order(nr);
+    SPH_HIP(c, hipMemcpyAsync(hk.data(), key, sizeof(long long) * nr, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipMemcpyAsync(hr.data(), P->rows, sizeof(int) * nr, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c, key);
+    for (int r = 0; r < nr; r++) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hk[a] < hk[b]; });
+    std::vector<int> nrows(nr);
+    for (int r = 0; r < nr; r++) nrows[r] = hr[order[r]];
+    SPH_HIP(c, hipMemcpyAsync(P->rows, nrows.data(), sizeof(int) * nr, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_rowof_from_rows, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows, P->rowof);
+    SPH_TRY(launch_check(c, "rowof"));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 // Rows and column structure of a case's PPE from its neighbour list.
@@ -734,6 +843,9 @@ int ppe_build(sph_ctx* c) {
     dfree(c, flag);
     dfree(c, scan);
     if (nr == 0) return 0;
+    const bool mixed = c->prm.precision == SPH_MIXED;
+    bool lattice = mixed && c->prm.ppe_reorder && nr > SLICE;
+    if (lattice) SPH_TRY(lattice_order(c, P, &lattice));
     P->nslices = (nr + SLICE - 1) / SLICE;
     SPH_TRY(dalloc(c, &P->cnt, nr));
     SPH_TRY(dalloc(c, &P->sl_off, P->nslices + 1));
@@ -749,11 +861,26 @@ int ppe_build(sph_ctx* c) {
     dfree(c, w);
     SPH_TRY(dalloc(c, &P->cols, P->nnz_slots + 1));
     hipLaunchKernelGGL(k_zero<int>, dim3(grid1(P->nnz_slots + 1)), dim3(BLOCK), 0, c->stream, P->nnz_slots + 1, P->cols);
-    hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
-                       (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols);
-    SPH_TRY(launch_check(c, "ppe_cols"));
+    if (lattice) {
+        int* ovf = nullptr;
+        SPH_TRY(dalloc(c, &P->slotmap, P->nnz_slots + 1));
+        SPH_TRY(dalloc(c, &ovf, 1));
+        hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(BLOCK), 0, c->stream, 1LL, ovf);
+        hipLaunchKernelGGL(k_ppe_struct_sorted, dim3(P->nslices), dim3(SLICE), 0, c->stream, nr, (const int*)P->rows,
+                           (const int*)P->rowof, S, c->nlist(), (const int*)P->cnt, (const long long*)P->sl_off,
+                           P->cols, P->slotmap, ovf);
+        SPH_TRY(launch_check(c, "ppe_cols_sorted"));
+        int hovf = 0;
+        SPH_HIP(c, hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(c, ovf);
+        if (hovf) return fail(c, SPH_E_UNSUPPORTED, "a PPE row exceeds the lattice-order sort capacity; set ppe_reorder = 0");
+    } else {
+        hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
+                           (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols);
+        SPH_TRY(launch_check(c, "ppe_cols"));
+    }
     P->nb = (nr + BLOCK - 1) / BLOCK;
-    const bool mixed = c->prm.precision == SPH_MIXED;
     if (mixed && c->prm.ppe_index16) {
         // 16-bit columns where every slice fits 64 windows of 1024 columns
         int* ovf = nullptr;
@@ -810,7 +937,7 @@ PPEView ppe_view(const sph_ctx* c) {
     if (!P) return v;
     v.nr = P->nr; v.nb = P->nb;
     v.rows = P->rows; v.rowof = P->rowof; v.cnt = P->cnt; v.sl_off = P->sl_off; v.cols = P->cols;
-    v.ecols = P->ecols; v.gbase = P->gbase; v.fdiag = P->fdiag;
+    v.ecols = P->ecols; v.gbase = P->gbase; v.fdiag = P->fdiag; v.slotmap = P->slotmap;
     v.vals64 = P->vals64; v.vals32 = P->vals32; v.diag = P->diag; v.rhs = P->rhs; v.x = P->x; v.pnew = P->pnew;
     v.r = P->r; v.rhat = P->rhat; v.p = P->p; v.phat = P->phat; v.v = P->v; v.s = P->s; v.shat = P->shat; v.t = P->t;
     v.fr = P->fr; v.frhat = P->frhat; v.fp = P->fp; v.fphat = P->fphat; v.fv = P->fv; v.fs = P->fs;

@@ -82,6 +82,7 @@ struct PPEView {
     const int* cols;              // int32 columns (null when the case uses ecols)
     const unsigned short* ecols;  // MIXED: 16-bit columns (window group, offset), or null
     const int* gbase;             // MIXED: first column of each window group, 64 per slice
+    const unsigned char* slotmap; // MIXED lattice order: slot of each entry, by its rank in the neighbour list
     double* vals64;
     float* vals32;
     double *diag, *rhs, *x, *pnew;
@@ -227,6 +228,7 @@ struct sph_ctx {
     int* scan_tmp = nullptr; size_t scan_tmp_n = 0;
 
     float* stat_buf = nullptr;           // stats reduction scratch
+    double* apr_target = nullptr;        // APR target level per particle (the reference's M_num)
 
     sph::PPEState* ppe = nullptr;
     sph_ppe_report rep{};
@@ -282,7 +284,7 @@ int launch_check(T* c, const char* what) {
 inline unsigned grid1(long long n, int block = BLOCK) { return (unsigned)((n + block - 1) / block); }
 
 // nnps.hip
-int nnps_run(sph_ctx* c);
+int nnps_run(sph_ctx* c, bool build_list = true);
 int set_ncell_run(sph_ctx* c);
 int exclusive_scan_i32(sph_ctx* c, const int* in, int* out, long long n, int* total_out);
 int exclusive_scan_i64(sph_ctx* c, const long long* in, long long* out, long long n, long long* total_out);
@@ -296,6 +298,9 @@ int shape(Eng& e, int kind, int block, Shape* out);
 int tuned_block(Eng& e, int kid);
 int tune_begin(Eng& e, int kid, int block);
 int tune_end(Eng& e);
+
+// apr.hip
+int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out);
 
 // physics.hip
 int step_physics_front(Eng& e);

@@ -46,6 +46,7 @@ def lib():
     L.sph_last_error.argtypes = [vp]
     L.sph_destroy.argtypes = [vp]
     L.sph_set_engine_options.argtypes = [vp, C.c_int, C.c_int]
+    L.sph_apr_refine.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
     L.sph_tuning.argtypes = [vp, vp, C.c_int]
     L.sph_batch_create.argtypes = [C.POINTER(vp)]
     L.sph_batch_step.argtypes = [vp, vp, C.c_int, vp, vp, vp]
@@ -83,7 +84,7 @@ EXPORTED = [
     "sph_get_field_f64", "sph_get_field_i32", "sph_nnps_debug", "sph_neighbors",
     "sph_domain_info", "sph_ppe_info", "sph_stats", "sph_synchronize",
     "sph_profile_enable", "sph_profile_report", "sph_device_bytes",
-    "sph_last_error", "sph_destroy", "sph_set_engine_options", "sph_tuning",
+    "sph_last_error", "sph_destroy", "sph_set_engine_options", "sph_tuning", "sph_apr_refine",
     "sph_batch_create", "sph_batch_step", "sph_batch_synchronize", "sph_batch_set_options",
     "sph_batch_profile_enable", "sph_batch_profile_report", "sph_batch_tuning",
     "sph_batch_last_error", "sph_batch_destroy", "sph_estimate_bytes", "sph_server_create",
@@ -212,6 +213,15 @@ class Case:
 
     def options(self, graphs=True, autotune=True):
         self._check(self.L.sph_set_engine_options(self.h, int(graphs), int(autotune)), "engine_options")
+
+    def apr_refine(self, target=None, max_passes=1):
+        """Refinement passes (sph_apr_refine); target: per-particle target
+        level in device order, or None to keep the current targets."""
+        t = None if target is None else np.ascontiguousarray(target, np.float64)
+        n_out = C.c_int(0)
+        self._check(self.L.sph_apr_refine(self.h, abi.ptr(t), int(max_passes), C.byref(n_out)), "apr_refine")
+        self.n = n_out.value
+        return self.n
 
     def tuning(self):
         return _tuning_rows(self.L.sph_tuning, self.h)
