@@ -149,6 +149,48 @@ def run_batched(cases, abi, urbansph, steps, warmup, dt, tol, n_cases=64):
     return out
 
 
+def run_apr(cases, abi, urbansph, steps, warmup, dt, tol, r1=0.05, r2=0.0125):
+    """The multi-resolution canyon: the 2M canyon refined by APR to level 1
+    (dx 0.004 m) within r1 and level 2 (dx 0.002 m) within r2 of a building
+    wall or roof, mixed precision.  Throughput counts live particles."""
+    P, nrm, geom = cases.canyon(dx=0.008)
+    t_deck = cases.apr_targets(P["x"], P["y"], P["z"], P["p_type"], P["i_type"], geom["buildings"], r1, r2)
+    cap = len(P) + 8 * int((t_deck == 1).sum()) + 72 * int((t_deck == 2).sum()) + 1024
+    prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED)
+    c = urbansph.Case(prm, cap)
+    c.upload(P, None, nrm)
+    c.step(0, dt)
+    c.sync()
+    t = cases.apr_targets(c.field("X"), c.field("Y"), c.field("Z"), c.ifield("P_TYPE"), c.ifield("I_TYPE"),
+                          geom["buildings"], r1, r2)
+    a = time.perf_counter()
+    n = c.apr_refine(t, max_passes=200)
+    t_apr = time.perf_counter() - a
+    live = int((c.ifield("I_TYPE") != 3).sum())
+    h = c.field("H")[c.ifield("I_TYPE") != 3]
+    log(f"[bench] apr: {len(P)} -> {live} live particles ({n} with parents) in {t_apr:.1f} s; "
+        f"h levels {sorted(set(np.round(h / h.max(), 3)))}")
+    count = 1
+    for _ in range(warmup):
+        c.step(count, dt)
+        count += 1
+    c.sync()
+    t0 = time.perf_counter()
+    iters = []
+    for _ in range(steps):
+        c.step(count, dt)
+        c.sync()
+        iters.append(c.ppe_report().iterations)
+        count += 1
+    wall = time.perf_counter() - t0
+    rep = c.ppe_report()
+    out = dict(value=live * steps / wall, ms_per_step=wall / steps * 1e3, particles_live=live, particles_total=n,
+               coarse=len(P), refine_s=t_apr, iters=iters, rel_res=rep.rel_residual, converged=rep.converged)
+    log(f"[bench] apr: {out['value'] / 1e6:.2f} M particle-steps/s, {out['ms_per_step']:.1f} ms/step, iters {iters}")
+    c.close()
+    return out
+
+
 def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=24, rate=4.0):
     """A seeded Poisson stream of canyon requests through the server: wall
     clock, requests submitted at their arrival times, one co-batched step per
@@ -230,6 +272,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-apr", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
@@ -343,6 +386,16 @@ def main():
             "speedup_vs_sequential": round(bt["speedup_vs_sequential"], 3),
             "ppe_iters_range": [bt["ppe_iters_min"], bt["ppe_iters_max"]], "tuning": bt["tuning"],
             "device_bytes": bt["dev_bytes"]}
+    if not args.no_apr:
+        ar = run_apr(cases, abi, urbansph, 5, 4, args.dt, args.tol)
+        line["apr"] = {
+            "workload": "street canyon refined by APR: dx 0.008 m far field, 0.004 m within 0.05 m and 0.002 m within "
+                        "0.0125 m of building walls and roofs, mixed precision",
+            "particles_live": ar["particles_live"], "particles_with_parents": ar["particles_total"],
+            "value": round(ar["value"], 1), "unit": "particle-steps/s (live particles)",
+            "ms_per_step": round(ar["ms_per_step"], 3), "steps": 5, "warmup": 4,
+            "refine_seconds": round(ar["refine_s"], 2), "ppe_iters_per_step": ar["iters"],
+            "converged": ar["converged"]}
     if not args.no_serve:
         sv = run_serving(cases, abi, urbansph, args.dt, args.tol, n_req=args.serve_requests)
         line["latency_ms"] = {"p50": round(sv["p50_ms"], 1), "p99": round(sv["p99_ms"], 1),

@@ -157,6 +157,13 @@ typedef struct sph_params {
                                    order (0)                                  */
     int    apr_h_change;        /* h_change (vii[36]): refined children get
                                    h/2 (1, default) or keep h (0)             */
+    int    ale_positions;       /* 1: the corrector moves particles,
+                                   x += (u0 + u)/2 dt eli for p_type > 0
+                                   (functions_TIME_ISPH.cuh:193-195, computed
+                                   but never stored by the reference), and the
+                                   neighbour search runs every step
+                                   (Lagrangian / ALE); 0 (default): positions
+                                   fixed, search at count == 0 only          */
 } sph_params;
 
 /* ---------------------------------------------------------------------------

@@ -86,6 +86,7 @@ struct Oracle {
     std::vector<double> nx, ny, nz;            // persistent IB normals
     std::vector<int> keys, perm, gstr, gend;
     std::vector<double> apr_target;            // APR target level (the reference's M_num)
+    std::vector<double> xn, yn, zn;            // ALE: positions of the corrector, committed at step end
     // last PPE system (BICGSTAB), rows in ascending particle order
     std::vector<int> sys_rows, sys_rowptr, sys_cols;
     std::vector<double> sys_diag, sys_rhs, sys_vals, sys_x0, sys_x;
@@ -1273,6 +1274,10 @@ void ibm_spread(Oracle& o) {
 void time_update(Oracle& o, double dt, int count) {
     const sph_params& pr = o.prm;
     auto& S = o.SP1; auto& P = o.P1; auto& S2 = o.SP2; auto& P2 = o.P2;
+    if (pr.ale_positions) {
+        o.xn.resize(o.n); o.yn.resize(o.n); o.zn.resize(o.n);
+        for (int i = 0; i < o.n; i++) { o.xn[i] = S[i].x; o.yn[i] = S[i].y; o.zn[i] = S[i].z; }
+    }
     for (int i = 0; i < o.n; i++) {
         if (S[i].i_type >= 3) continue;
         if (S[i].p_type >= 1000) continue;
@@ -1296,6 +1301,13 @@ void time_update(Oracle& o, double dt, int count) {
             double uxc = tux0 + dux_dt * (dt), uyc = tuy0 + duy_dt * (dt), uzc = tuz0 + duz_dt * (dt);
             if ((uxc * uxc + uyc * uyc + uzc * uzc) >= pr.u_limit * pr.u_limit) { uxc = tux0; uyc = tuy0; uzc = tuz0; }
             P[i].ux = uxc; P[i].uy = uyc; P[i].uz = uzc;
+            if (pr.ale_positions) {
+                // functions_TIME_ISPH.cuh:193-195 (computed there, written here)
+                const double mv = (double)(p_type_i > 0) * S[i].eli;
+                o.xn[i] = S[i].x + (S2[i].ux0 + uxc) / 2.0 * (dt) * mv;
+                o.yn[i] = S[i].y + (S2[i].uy0 + uyc) / 2.0 * (dt) * mv;
+                o.zn[i] = S[i].z + (S2[i].uz0 + uzc) / 2.0 * (dt) * mv;
+            }
         }
         if (pr.con_solve == 1) P[i].temp = S2[i].temp0 + q.dtemp * dt * (buffer_type == 0);
         else { P[i].enthalpy = S[i].enthalpy; P[i].temp = S[i].temp; }
@@ -1400,7 +1412,7 @@ int apr_refine(Oracle& o, const double* target, int max_passes, int* n_out) {
 int step(Oracle& o, int count, double dt, double time) {
     const sph_params& pr = o.prm;
     set_ncell(o);
-    if (count == 0) {
+    if (count == 0 || pr.ale_positions) {   // moving particles: a search every step
         nnps(o);
     } else {
         if (!o.searched) return SPH_E_STATE;
@@ -1428,6 +1440,8 @@ int step(Oracle& o, int count, double dt, double time) {
     outlet(o, o.P1);
     periodic(o, o.P1);
     freeslip(o, o.P1);
+    if (pr.ale_positions)
+        for (int i = 0; i < o.n; i++) { o.P1[i].x = o.xn[i]; o.P1[i].y = o.yn[i]; o.P1[i].z = o.zn[i]; }
     return rc;
 }
 
@@ -1490,6 +1504,7 @@ void oracle_params_default(sph_params* p) {
     p->ibm_ut_factor = 0.2;
     p->ppe_solver = SPH_PPE_REF_JACOBI; p->precision = SPH_FP64; p->ppe_tol = 1e-8; p->ppe_maxit = 5000;
     p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1;
+    p->ale_positions = 0;
 }
 
 void* oracle_create(const sph_params* prm, int n) {

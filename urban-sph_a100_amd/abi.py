@@ -87,7 +87,8 @@ class Params(C.Structure):
         + [("ppe_solver", C.c_int), ("precision", C.c_int),
            ("ppe_tol", C.c_double), ("ppe_maxit", C.c_int),
            ("ppe_inner_tol", C.c_double), ("ppe_max_refine", C.c_int), ("ppe_index16", C.c_int),
-           ("ppe_reorder", C.c_int), ("apr_h_change", C.c_int)]
+           ("ppe_reorder", C.c_int), ("apr_h_change", C.c_int),
+           ("ale_positions", C.c_int)]
     )
 
     def update(self, **kw):

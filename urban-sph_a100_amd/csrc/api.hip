@@ -172,7 +172,7 @@ void sph_params_default(sph_params* p) {
     p->most_ghost_dz = 5.0; p->most_ghost_r = 0.1;
     p->ibm_ut_factor = 0.2;
     p->ppe_solver = SPH_PPE_REF_JACOBI; p->precision = SPH_FP64; p->ppe_tol = 1e-8; p->ppe_maxit = 5000;
-    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1;
+    p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1; p->ale_positions = 0;
 }
 
 static int validate(const sph_params* p, std::string& why) {
@@ -562,7 +562,7 @@ void sph_destroy(sph_ctx* c) {
     for (auto p : c->q) dfree(c, p);
     for (auto p : c->snap) dfree(c, p);
     void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->nl_slice_off, c->nl_cnt,
-                    c->nl_ent, c->stat_buf, c->apr_target};
+                    c->nl_ent, c->stat_buf, c->apr_target, c->xnew[0], c->xnew[1], c->xnew[2]};
     for (auto p : rest) dfree(c, p);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;

@@ -287,15 +287,19 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.keys = c->keys;
     for (int f = 0; f < 5; f++) d.snap[f] = c->snap[f];
     d.ppe = ppe_view(c);
+    for (int f = 0; f < 3; f++) d.xnew[f] = c->xnew[f];
     d.sc = e.d_sc + (size_t)k * SCAL_COUNT;
     d.ctl = e.d_ctl + (size_t)k * CTL_COUNT;
 }
 
 static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, const double* dt, const double* time) {
-    // count == 0: the neighbour search and the PPE structure of that case
+    // count == 0 (every step in ALE mode): the neighbour search and the PPE
+    // structure of that case
     for (int k = 0; k < nc; k++) {
         sph_ctx* c = cases[k];
-        if (count[k] == 0) {
+        if (c->prm.ale_positions && !c->xnew[0])
+            for (int f = 0; f < 3; f++) SPH_TRY(dalloc(c, &c->xnew[f], c->cap));
+        if (count[k] == 0 || c->prm.ale_positions) {   // moving particles: a search every step
             SPH_TRY(set_ncell_run(c));
             SPH_TRY(nnps_run(c));
             SPH_TRY(ppe_build(c));

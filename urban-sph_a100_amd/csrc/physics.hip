@@ -67,6 +67,16 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_step_begin(Seg sg) {
     Q.ftotalx[i] = 0.0; Q.ftotaly[i] = 0.0; Q.ftotalz[i] = 0.0; Q.ftotal[i] = 0.0;
     Q.fpx[i] = 0.0; Q.fpy[i] = 0.0; Q.fpz[i] = 0.0; Q.dtemp[i] = 0.0; Q.dconcn[i] = 0.0;
     Q.inv_xx[i] = 0.0; Q.inv_yy[i] = 0.0; Q.inv_zz[i] = 0.0; Q.inv_xy[i] = 0.0; Q.inv_yz[i] = 0.0; Q.inv_zx[i] = 0.0;
+    if (cd.xnew[0]) { cd.xnew[0][i] = P.x[i]; cd.xnew[1][i] = P.y[i]; cd.xnew[2][i] = P.z[i]; }
+}
+
+// ALE: the corrector's positions become the state at the end of the step,
+// after the boundary passes that still search the step's cells.
+__global__ void __launch_bounds__(MAX_BLOCK) k_commit_positions(Seg sg) {
+    SEG_PARTICLE(sg);
+    if (!cd.xnew[0]) return;
+    const PView P = cd.P;
+    P.x[i] = cd.xnew[0][i]; P.y[i] = cd.xnew[1][i]; P.z[i] = cd.xnew[2][i];
 }
 
 // Snapshot of the fields a copy/mirror pass reads while it writes them.
@@ -919,6 +929,13 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_time_update(Seg sg) {
         double uxc = tux0 + dux_dt * (dt), uyc = tuy0 + duy_dt * (dt), uzc = tuz0 + duz_dt * (dt);
         if ((uxc * uxc + uyc * uyc + uzc * uzc) >= pr.u_limit * pr.u_limit) { uxc = tux0; uyc = tuy0; uzc = tuz0; }
         P.ux[i] = uxc; P.uy[i] = uyc; P.uz[i] = uzc;
+        if (cd.xnew[0]) {
+            // functions_TIME_ISPH.cuh:193-195 (computed there, written here)
+            const double mv = (double)(p_type_i > 0) * S.eli[i];
+            cd.xnew[0][i] = S.x[i] + (S2.ux0[i] + uxc) / 2.0 * (dt) * mv;
+            cd.xnew[1][i] = S.y[i] + (S2.uy0[i] + uyc) / 2.0 * (dt) * mv;
+            cd.xnew[2][i] = S.z[i] + (S2.uz0[i] + uzc) / 2.0 * (dt) * mv;
+        }
     }
     if (pr.con_solve == 1) P.temp[i] = S2.temp0[i] + Q.dtemp[i] * dt * (buffer_type == 0);
     else { P.enthalpy[i] = S.enthalpy[i]; P.temp[i] = S.temp[i]; }
@@ -1058,6 +1075,9 @@ int step_physics_back(Eng& e) {
     SPH_TRY(run_outlet(e, 1));
     SPH_TRY(run_periodic(e, 1));
     SPH_TRY(run_freeslip(e, 1));
+    bool ale = false;
+    for (sph_ctx* c : e.cs) ale |= c->prm.ale_positions != 0;
+    if (ale) SEGL("commit_positions", -1, k_commit_positions);
     return 0;
 }
 

@@ -110,6 +110,7 @@ struct CaseDesc {
     PPEView ppe;
     double* sc;          // Krylov scalars, computed on the device
     const double* ctl;   // solve controls written by the host: tol, maxit, skip
+    double* xnew[3];     // ALE positions of the corrector (null unless ale_positions)
 };
 enum Ctl { CTL_TOL, CTL_MAXIT, CTL_SKIP, CTL_COUNT = 4 };
 constexpr int SCAL_COUNT = 32;   // Krylov scalars per case
@@ -229,6 +230,7 @@ struct sph_ctx {
 
     float* stat_buf = nullptr;           // stats reduction scratch
     double* apr_target = nullptr;        // APR target level per particle (the reference's M_num)
+    double* xnew[3] = {};                // ALE: corrector positions, committed at the end of the step
 
     sph::PPEState* ppe = nullptr;
     sph_ppe_report rep{};
