@@ -417,6 +417,98 @@ int sph_server_release(sph_server* s, long long id);
 const char* sph_server_last_error(const sph_server* s);
 void sph_server_destroy(sph_server* s);
 
+/* ---- File formats (host only, no device work) ---------------------------
+ * The reference's solver settings file (solv.txt), particle deck and VTK
+ * output.  None of these touch a context; they fill or read the records
+ * above. */
+
+/* Run controls and the settings of solv.txt that are not step parameters.
+ * Codes follow the reference's globals (Parameters.cuh, function_init.cuh). */
+#define SPH_MAX_PLOT 64
+typedef struct sph_run_config {
+    int    solver_type;         /* 0 WCSPH (the reference's default), 1 ISPH  */
+    int    prop_table;          /* property_table(YES/NO):                    */
+    int    atmos_stab;          /* 0 Unstable, 1 Neutral, 2 Stable            */
+    int    terrain_type;        /* 0 Flat .. 4 Ocean                          */
+    int    structure;           /* structure_solv                             */
+    int    open_boundary;       /* 0 NO, 1 OPEN, 2 PERIODIC                   */
+    int    switch_ptype;
+    int    apr_solve;           /* 0 NO, 1 Cell, 2 Switch, 3 Block            */
+    int    block_config[3];     /* NBI, NBJ, NBK                              */
+    int    nb_cell_type;        /* 0 3X3, 1 5X5                               */
+    int    timestep_update;     /* flag_timestep_update                       */
+    int    freq_output;         /* plot-output                                */
+    int    freq_apr, freq_bapr, freq_temp, freq_h_reset;
+    int    min_iteration, max_iteration;
+    double buffersize, p_ref, dp_th, drho_th, p_relaxation, dt_structure;
+    double dt, time, time_end;  /* time-step, start-time, end-time            */
+    int    num_plot;            /* plot-variables: count and names            */
+    char   plot[SPH_MAX_PLOT][32];
+    char   unknown[256];        /* keywords whose value matched no option,
+                                   space separated (the reference keeps the
+                                   previous value for these, as here)         */
+} sph_run_config;
+
+/* Reads solv.txt: replaces read_solv_input (function_init.cuh:166-463).
+ * Tokens are whitespace separated; each keyword is followed by the
+ * reference's number of filler tokens and then its value.  Step settings
+ * go into *prm (members not named by the file keep their values, except
+ * the solve switches that the reference clears first: fp, fv, fg, con,
+ * boussinesq, kgc, turbulence, concn); everything else goes into *run,
+ * which is reset first.  Values are stored as the reference codes them,
+ * including options sph_create rejects (e.g. viscous-force Violeau = 3).
+ * Returns SPH_E_ARG if the file cannot be read. */
+int sph_read_solv(const char* path, sph_params* prm, sph_run_config* run);
+
+/* Reads a particle deck: replaces read_input (function_init.cuh:550-676)
+ * and gpu_count_particle_numbers2 (function_init.cuh:483-498).  The first
+ * line lists integer column codes separated by tabs (1 x, 2 y, 3 z, 4 ux,
+ * 5 uy, 6 uz, 7 m, 8 p_type, 9 h, 10 temp, 11 pres, 12 rho, 15 concn,
+ * 26 h_ref, 27 m_ref; every other code is read and skipped); then one
+ * whitespace-separated value per column and particle.  Each particle read
+ * gets i_type 1 and zero elsewhere; an incomplete last record is dropped.
+ * With P1 == NULL only counts (*n_out).  SPH_E_NOMEM if more than cap
+ * particles; SPH_E_ARG if the file or its header cannot be read. */
+int sph_read_deck(const char* path, sph_part1* P1, int cap, int* n_out);
+
+/* Writes legacy VTK (version 3.0, BINARY, POLYDATA, big-endian): replaces
+ * save_vtk_bin_single_flag (functions_OUTPUT.cuh:121-616).  Particles with
+ * i_type 1 or 2 are written: POINTS (float x, y, z), then POINT_DATA with
+ * one FIELD array for every table entry whose leading characters match a
+ * label, as the reference's strncmp table does (e.g. "velocity" -> velocity
+ * x3, "smoothing" -> h, "rho_ref" -> rho and rho_ref).  The FIELD count is
+ * the number of arrays written.  P2 and P3 may be NULL when no label needs
+ * them (rho_ref; normal, dtemp), else SPH_E_ARG. */
+int sph_write_vtk(const char* path, const sph_part1* P1, const sph_part2* P2, const sph_part3* P3, int n,
+                  const char* const* labels, int nlabels);
+
+/* Material property tables (temperature T, enthalpy h, conductivity k,
+ * specific heat cp, viscosity mu), one table per p_type, stored back to
+ * back: table i holds entries index[i] .. index[i] + size[i] - 1.  The
+ * reference's capacity is 50 entries and 10 tables (Parameters.cuh:66,
+ * SOPHIA_gpu.cu:56-57); these are larger. */
+#define SPH_TABLE_CAP 512
+#define SPH_MAX_TABLES 32
+typedef struct sph_prop_tables {
+    int   ntables, ndata;
+    int   index[SPH_MAX_TABLES], size[SPH_MAX_TABLES];
+    float T[SPH_TABLE_CAP], h[SPH_TABLE_CAP], k[SPH_TABLE_CAP], cp[SPH_TABLE_CAP], mu[SPH_TABLE_CAP];
+} sph_prop_tables;
+
+/* Reads a property-table file: replaces read_table (function_init.cuh:724-970).
+ * A block "#p_type N" ... "#end" holds lines starting //T, //h, //k, //cp,
+ * //mu, each followed by one line of comma-separated values.  A block's
+ * length is that of the last array read in it; index/size follow the
+ * reference (tables numbered 0 .. ntables-1 in file order).  SPH_E_NOMEM
+ * beyond the capacities; SPH_E_ARG for an unreadable file or a block
+ * without #end. */
+int sph_read_tables(const char* path, sph_prop_tables* t);
+
+/* Table lookup: replaces interp2 (functions_PROP.cuh:8-40).  Piecewise-
+ * linear in float through (x[i], y[i]), extended linearly past both ends;
+ * NaN for size < 2 or a NaN argument. */
+float sph_table_interp(const float* x, const float* y, int size, float v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1603,13 +1603,14 @@ int oracle_domain(void* h, sph_domain* d) {
     return 0;
 }
 
-// Neighbour lists by the window rules: kind 0 = dcell-query window, self
-// included, tdist = r + 1e-20 < kappa*max(hi,hj); kind 1 = binning window,
-// 0 < r < kappa*hi.
+// Neighbour lists of the last search by the window rules: kind 0 =
+// dcell-query window, self included, tdist = r + 1e-20 < kappa*max(hi,hj);
+// kind 1 = binning window, 0 < r < kappa*hi.  Positions are those the search
+// sorted (SP1): in ALE mode the step moves particles after its search.
 int oracle_neighbors(void* h, int kind, int* off, int* nbr, long long* total) {
     Oracle& o = *(Oracle*)h;
     if (!o.searched) return SPH_E_STATE;
-    const auto& S = o.P1;
+    const auto& S = o.SP1;
     std::vector<std::vector<int>> L(o.n);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < o.n; i++) {

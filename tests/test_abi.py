@@ -6,6 +6,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import abi
 import oracle
@@ -64,3 +65,24 @@ def test_params_struct_layout_matches_c(tmp_path):
     assert [int(v) for v in out[1:4]] == [152, 52, 164]
     for m, off in zip(members, out[4:]):
         assert getattr(abi.Params, m).offset == int(off), m
+
+
+@pytest.mark.parametrize("struct,cname", [("RunConfig", "sph_run_config"), ("PropTables", "sph_prop_tables"),
+                                          ("ServerInfo", "sph_server_info"), ("RequestInfo", "sph_request_info")])
+def test_aux_struct_layouts_match_c(tmp_path, struct, cname):
+    S = getattr(abi, struct)
+    members = [n for n, _ in S._fields_]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "urbansph.h"', 'int main(void){',
+             f'printf("%zu\\n", sizeof({cname}));']
+    for m in members:
+        lines.append(f'printf("%zu\\n", offsetof({cname}, {m}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = [int(v) for v in subprocess.check_output([str(exe)]).decode().split()]
+    assert out[0] == C.sizeof(S)
+    for m, off in zip(members, out[1:]):
+        assert getattr(S, m).offset == off, m

@@ -135,6 +135,39 @@ class RequestInfo(C.Structure):
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
+MAX_PLOT = 64
+TABLE_CAP, MAX_TABLES = 512, 32
+
+
+class PropTables(C.Structure):
+    """sph_prop_tables: material property tables, one per p_type."""
+    _fields_ = [("ntables", C.c_int), ("ndata", C.c_int), ("index", C.c_int * MAX_TABLES),
+                ("size", C.c_int * MAX_TABLES)] + [(n, C.c_float * TABLE_CAP) for n in ("T", "h", "k", "cp", "mu")]
+
+
+class RunConfig(C.Structure):
+    """sph_run_config: run controls and solv.txt settings outside sph_params."""
+    _fields_ = [(n, C.c_int) for n in (
+        "solver_type", "prop_table", "atmos_stab", "terrain_type", "structure", "open_boundary",
+        "switch_ptype", "apr_solve")] + [("block_config", C.c_int * 3)] + [(n, C.c_int) for n in (
+        "nb_cell_type", "timestep_update", "freq_output", "freq_apr", "freq_bapr", "freq_temp",
+        "freq_h_reset", "min_iteration", "max_iteration")] + [(n, C.c_double) for n in (
+        "buffersize", "p_ref", "dp_th", "drho_th", "p_relaxation", "dt_structure", "dt", "time",
+        "time_end")] + [("num_plot", C.c_int), ("plot", (C.c_char * 32) * MAX_PLOT), ("unknown", C.c_char * 256)]
+
+    def as_dict(self):
+        d = {}
+        for n, _ in self._fields_:
+            v = getattr(self, n)
+            if n == "block_config":
+                v = list(v)
+            elif n == "plot":
+                v = [v[i].value.decode() for i in range(self.num_plot)]
+            elif n == "unknown":
+                v = v.decode().split()
+            d[n] = v
+        return d
+
 
 def ptr(a):
     """Raw data pointer of a contiguous numpy array (or None)."""

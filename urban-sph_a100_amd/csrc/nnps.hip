@@ -318,8 +318,10 @@ static int permute_state(sph_ctx* c) {
     const int n = c->n;
     double* tmp = nullptr;
     int* itmp = nullptr;
-    SPH_TRY(dalloc(c, &tmp, n));
-    SPH_TRY(dalloc(c, &itmp, n));
+    // each state array is swapped for its gathered copy, so the copies keep
+    // the context's capacity: refinement appends particles past n
+    SPH_TRY(dalloc(c, &tmp, c->cap));
+    SPH_TRY(dalloc(c, &itmp, c->cap));
     auto gd = [&](double*& a) -> int {
         if (!a) return 0;
         hipLaunchKernelGGL(k_gather<double>, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->perm,

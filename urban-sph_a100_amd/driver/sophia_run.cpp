@@ -1,21 +1,19 @@
 // sophia_run -- C++ host driver over the C ABI: the counterpart of the
 // reference's per-GPU time loop ISPH_Calc (ISPH_Calc.cuh:238-577) for one
-// GPU.  It reads a particle deck, uploads it, runs the step loop and prints
-// the reference's periodic statistics line (ISPH_Calc.cuh:500-521).
+// GPU.  It reads the solver settings and a particle deck with the library's
+// readers, uploads the deck, runs the step loop, prints the reference's
+// periodic statistics line (ISPH_Calc.cuh:500-521) and, with --vtk, writes
+// the reference's VTK frames every plot-output steps (ISPH_Calc.cuh:525-540).
 //
-// Deck format: the reference's tab-header particle file
-// (function_init.cuh:550-676): the first line lists integer column codes
-// separated by tabs (1 x, 2 y, 3 z, 4 ux, 5 uy, 6 uz, 7 m, 8 p_type, 9 h,
-// 10 temp, 11 pres, 12 rho, 15 concn, 26 h_ref, 27 m_ref; other codes are
-// read and ignored), then one value per column for each particle.
-//
-// Parameters: "key value" lines naming sph_params members, e.g.
+// Settings: a solv.txt in the reference's keyword format (--solv), then any
+// number of "key value" files naming sph_params members, e.g.
 //   kernel_type 3
 //   ppe_solver 1
 //   ppe_tol 1e-6
-// plus the run controls dt, time, time_end, freq_output.
+// plus the run controls dt, time, time_end, freq_output; later files win.
 //
-// usage: sophia_run deck.txt [params.txt] [--steps N] [--dump out.bin]
+// usage: sophia_run deck.txt [--solv solv.txt] [params.txt ...] [--steps N]
+//                    [--vtk DIR] [--dump out.bin]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -23,65 +21,12 @@
 #include <cstring>
 #include <fstream>
 #include <map>
-#include <sstream>
 #include <string>
 #include <vector>
 
 #include "../../include/urbansph.h"
 
-static bool read_deck(const char* path, std::vector<sph_part1>& P) {
-    std::ifstream f(path);
-    if (!f) return false;
-    std::string header;
-    if (!std::getline(f, header)) return false;
-    std::vector<int> codes;
-    {
-        std::stringstream ss(header);
-        std::string tok;
-        while (std::getline(ss, tok, '\t'))
-            if (!tok.empty()) codes.push_back(std::atoi(tok.c_str()));
-    }
-    if (codes.empty()) return false;
-    std::string v;
-    while (true) {
-        sph_part1 p;
-        std::memset(&p, 0, sizeof(p));
-        size_t k = 0;
-        for (; k < codes.size(); k++) {
-            if (!(f >> v)) break;
-            double d = std::atof(v.c_str());
-            switch (codes[k]) {
-            case 1: p.x = (float)d; break;
-            case 2: p.y = (float)d; break;
-            case 3: p.z = (float)d; break;
-            case 4: p.ux = (float)d; break;
-            case 5: p.uy = (float)d; break;
-            case 6: p.uz = (float)d; break;
-            case 7: p.m = (float)d; break;
-            case 8: p.p_type = std::atoi(v.c_str()); break;
-            case 9: p.h = (float)d; break;
-            case 10: p.temp = (float)d; break;
-            case 11: p.pres = (float)d; break;
-            case 12: p.rho = (float)d; break;
-            case 15: p.concn = (float)d; break;
-            case 26: p.h_ref = (float)d; break;
-            case 27: p.m_ref = (float)d; break;
-            default: break;
-            }
-        }
-        if (k < codes.size()) break;
-        p.i_type = 1;
-        P.push_back(p);
-    }
-    return !P.empty();
-}
-
-struct Run {
-    double dt = 1e-4, time = 0.0, time_end = 1e-3;
-    int freq_output = 10;
-};
-
-static bool read_params(const char* path, sph_params& prm, Run& run) {
+static bool read_params(const char* path, sph_params& prm, sph_run_config& run) {
     std::ifstream f(path);
     if (!f) return false;
     std::map<std::string, int*> ints = {
@@ -110,22 +55,53 @@ static bool read_params(const char* path, sph_params& prm, Run& run) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s deck.txt [params.txt] [--steps N] [--dump out.bin]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s deck.txt [--solv solv.txt] [params.txt ...] [--steps N] [--vtk DIR] "
+                             "[--dump out.bin]\n", argv[0]);
         return 2;
     }
     sph_params prm;
     sph_params_default(&prm);
-    Run run;
+    sph_run_config run;
+    std::memset(&run, 0, sizeof(run));
+    run.dt = 1e-4;
+    run.time_end = 1e-3;
+    run.freq_output = 10;
     int max_steps = -1;
     const char* dump = nullptr;
+    const char* vtk_dir = nullptr;
     for (int a = 2; a < argc; a++) {
         if (!std::strcmp(argv[a], "--steps") && a + 1 < argc) max_steps = std::atoi(argv[++a]);
         else if (!std::strcmp(argv[a], "--dump") && a + 1 < argc) dump = argv[++a];
-        else if (!read_params(argv[a], prm, run)) { std::fprintf(stderr, "cannot read %s\n", argv[a]); return 2; }
+        else if (!std::strcmp(argv[a], "--vtk") && a + 1 < argc) vtk_dir = argv[++a];
+        else if (!std::strcmp(argv[a], "--solv") && a + 1 < argc) {
+            if (sph_read_solv(argv[++a], &prm, &run)) { std::fprintf(stderr, "cannot read %s\n", argv[a]); return 2; }
+            if (run.unknown[0]) std::fprintf(stderr, "solv: unrecognised values for: %s\n", run.unknown);
+        } else if (!read_params(argv[a], prm, run)) { std::fprintf(stderr, "cannot read %s\n", argv[a]); return 2; }
     }
-    std::vector<sph_part1> P;
-    if (!read_deck(argv[1], P)) { std::fprintf(stderr, "cannot read deck %s\n", argv[1]); return 2; }
-    const int n = (int)P.size();
+    if (run.freq_output <= 0) run.freq_output = 10;
+    int n = 0;
+    if (sph_read_deck(argv[1], nullptr, 0, &n) || n == 0) {
+        std::fprintf(stderr, "cannot read deck %s\n", argv[1]);
+        return 2;
+    }
+    std::vector<sph_part1> P(n);
+    if (sph_read_deck(argv[1], P.data(), n, &n)) { std::fprintf(stderr, "cannot read deck %s\n", argv[1]); return 2; }
+    std::vector<const char*> labels;
+    for (int q = 0; q < run.num_plot; q++) labels.push_back(run.plot[q]);
+    if (vtk_dir && labels.empty())
+        for (const char* l : {"p_type", "velocity", "pressure", "vis_t"}) labels.push_back(l);
+    std::vector<sph_part1> O1;
+    std::vector<sph_part2> O2;
+    std::vector<sph_part3> O3;
+    auto save_vtk = [&](sph_ctx* c, int count) {
+        O1.resize(n);
+        O2.resize(n);
+        O3.resize(n);
+        if (sph_download(c, O1.data(), O2.data(), O3.data(), n)) return false;
+        char path[1024];
+        std::snprintf(path, sizeof(path), "%s/fluid_%dstp.vtk", vtk_dir, count);
+        return sph_write_vtk(path, O1.data(), O2.data(), O3.data(), n, labels.data(), (int)labels.size()) == 0;
+    };
     sph_ctx* c = nullptr;
     int rc = sph_create(&prm, n, &c);
     if (rc) { std::fprintf(stderr, "sph_create: %d\n", rc); return 1; }
@@ -154,6 +130,7 @@ int main(int argc, char** argv) {
             double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             std::printf("Simulation time: %5.5f sec\nComputing time: %5.2f sec\n", t, wall);
             std::printf("%d\t tu_max=%5.2f\tftotal_max=%5.5f\n\n", count, umax, fmax);
+            if (vtk_dir && !save_vtk(c, count)) { std::fprintf(stderr, "cannot write VTK in %s\n", vtk_dir); return 1; }
         }
         if (t >= run.time_end || (max_steps > 0 && done >= max_steps)) break;
     }

@@ -75,6 +75,12 @@ def lib():
     L.sph_server_last_error.restype = C.c_char_p
     L.sph_server_last_error.argtypes = [vp]
     L.sph_server_destroy.argtypes = [vp]
+    L.sph_read_solv.argtypes = [C.c_char_p, C.POINTER(abi.Params), C.POINTER(abi.RunConfig)]
+    L.sph_read_deck.argtypes = [C.c_char_p, vp, C.c_int, C.POINTER(C.c_int)]
+    L.sph_write_vtk.argtypes = [C.c_char_p, vp, vp, vp, C.c_int, C.POINTER(C.c_char_p), C.c_int]
+    L.sph_read_tables.argtypes = [C.c_char_p, C.POINTER(abi.PropTables)]
+    L.sph_table_interp.restype = C.c_float
+    L.sph_table_interp.argtypes = [vp, vp, C.c_int, C.c_float]
     _lib = L
     return L
 
@@ -91,6 +97,7 @@ EXPORTED = [
     "sph_server_submit", "sph_server_tick", "sph_server_idle_until", "sph_server_clock",
     "sph_server_status", "sph_server_request", "sph_server_result", "sph_server_result_part1",
     "sph_server_release", "sph_server_last_error", "sph_server_destroy",
+    "sph_read_solv", "sph_read_deck", "sph_write_vtk", "sph_read_tables", "sph_table_interp",
 ]
 
 TUNE_PASSES = ["grad_corr", "prep", "boundary3d", "neumann", "advection", "freeslip", "pressure_force",
@@ -119,6 +126,65 @@ class SPHError(RuntimeError):
     def __init__(self, rc, msg):
         super().__init__(f"rc={rc}: {msg}")
         self.rc = rc
+
+
+def read_solv(path, params=None):
+    """solv.txt -> (sph_params, run settings dict) (function_init.cuh:166-463).
+    Members the file does not name keep their values from `params`
+    (default: the reference defaults)."""
+    prm = abi.Params()
+    C.pointer(prm)[0] = params if params is not None else default_params()
+    run = abi.RunConfig()
+    rc = lib().sph_read_solv(os.fsencode(path), C.byref(prm), C.byref(run))
+    if rc:
+        raise SPHError(rc, f"cannot read {path}")
+    return prm, run.as_dict()
+
+
+def read_deck(path):
+    """Particle deck -> part1 records (function_init.cuh:550-676)."""
+    L = lib()
+    n = C.c_int()
+    rc = L.sph_read_deck(os.fsencode(path), None, 0, C.byref(n))
+    if rc:
+        raise SPHError(rc, f"cannot read deck {path}")
+    P = np.zeros(n.value, abi.PART1)
+    rc = L.sph_read_deck(os.fsencode(path), abi.ptr(P), len(P), C.byref(n))
+    if rc:
+        raise SPHError(rc, f"cannot read deck {path}")
+    return P[:n.value]
+
+
+def read_tables(path):
+    """Property tables -> list of per-p_type dicts of float32 arrays
+    T, h, k, cp, mu (function_init.cuh:724-970)."""
+    t = abi.PropTables()
+    rc = lib().sph_read_tables(os.fsencode(path), C.byref(t))
+    if rc:
+        raise SPHError(rc, f"cannot read tables {path}")
+    out = []
+    for i in range(t.ntables):
+        a, n = t.index[i], t.size[i]
+        out.append({k: np.array(getattr(t, k)[a:a + n], np.float32) for k in ("T", "h", "k", "cp", "mu")})
+    return out
+
+
+def table_interp(x, y, v):
+    """interp2 (functions_PROP.cuh:8-40) on float32 knots."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return np.float32(lib().sph_table_interp(abi.ptr(x), abi.ptr(y), len(x), float(v)))
+
+
+def write_vtk(path, P1, labels, P2=None, P3=None):
+    """Legacy binary VTK of the i_type 1/2 particles (functions_OUTPUT.cuh:121-616)."""
+    P1 = np.ascontiguousarray(P1, abi.PART1)
+    P2 = None if P2 is None else np.ascontiguousarray(P2, abi.PART2)
+    P3 = None if P3 is None else np.ascontiguousarray(P3, abi.PART3)
+    arr = (C.c_char_p * max(len(labels), 1))(*[l.encode() for l in labels])
+    rc = lib().sph_write_vtk(os.fsencode(path), abi.ptr(P1), abi.ptr(P2), abi.ptr(P3), len(P1), arr, len(labels))
+    if rc:
+        raise SPHError(rc, f"cannot write {path}")
 
 
 class Case:
