@@ -111,7 +111,8 @@ typedef struct sph_params {
     int    fg_solve;            /* gravity                                    */
     int    con_solve;           /* temperature equation                       */
     int    boussinesq_solve;
-    int    kgc_solve;           /* only 0 (NO) is accepted, see DESIGN        */
+    int    kgc_solve;           /* only 0 (NO): the reference never builds
+                                   Cm, so KGC would zero the gradients        */
     int    turbulence_model;    /* SPH_LAMINAR / SPH_SSM / SPH_DDF            */
     int    concn_solve;         /* concentration diffusion                    */
     int    noslip_bc;
@@ -154,7 +155,9 @@ typedef struct sph_params {
     int    ppe_reorder;         /* MIXED: number the PPE unknowns in lattice
                                    (z, y, x) order for contiguous SpMV gathers
                                    (1, default) or keep the particles' cell
-                                   order (0)                                  */
+                                   order (0).  A case with a row of more than
+                                   160 entries (coarse particles next to
+                                   refined ones) keeps cell order.            */
     int    apr_h_change;        /* h_change (vii[36]): refined children get
                                    h/2 (1, default) or keep h (0)             */
     int    ale_positions;       /* 1: the corrector moves particles,
@@ -250,7 +253,9 @@ int sph_upload(sph_ctx* c, const sph_part1* P1, const sph_part2* P2,
  * given step counter, time-step and time.  Asynchronous apart from the
  * BiCGSTAB convergence polls: returns after enqueueing the last pass.
  * Returns SPH_E_NOCONV, after completing the step, when the PPE solve did
- * not reach ppe_tol within ppe_maxit (the reference checks nothing). */
+ * not reach ppe_tol within ppe_maxit, and SPH_E_NONFINITE, after completing
+ * it, when the PPE system held NaN or Inf (the solve then stops at once).
+ * The reference checks neither. */
 int sph_step(sph_ctx* c, int count, double dt, double time);
 
 /* Options of the context's step engine.  use_graphs: replay each BiCGSTAB
@@ -347,8 +352,9 @@ int sph_batch_create(sph_batch** out);
 
 /* One step of each listed case with its own count, dt and time.  The cases
  * must share precision and ppe_solver.  Returns 0, the first error, or
- * SPH_E_NOCONV after completing the step when a case's PPE did not converge
- * (sph_ppe_info of each case says which). */
+ * SPH_E_NOCONV / SPH_E_NONFINITE after completing the step when a case's
+ * PPE did not converge or met NaN/Inf (sph_ppe_info of each case says
+ * which). */
 int sph_batch_step(sph_batch* b, sph_ctx* const* cases, int ncases, const int* count,
                    const double* dt, const double* time);
 int sph_batch_synchronize(sph_batch* b);

@@ -31,9 +31,15 @@
  * = identity on a lattice and exact for linear fields; PPE operator row
  * sums, b = 0 for uniform u*, Laplacian of x^2; BiCGSTAB against scipy's
  * sparse LU; Shepard reproduction of constants; hand-built predictor and
- * corrector cases; committed golden fixtures.  Functions that only
- * reference-held data could pin (MOST Newton, IBM target table, DDF) are
- * marked "parity unpinned" in DESIGN.md.
+ * corrector cases; committed golden fixtures; APR split geometry and mass
+ * conservation; ALE trapezoid positions.
+ *
+ * Parity unpinned (no closed form, and the reference ships no vectors):
+ * the MOST Newton branch (functions_MOST.cuh:214-249), the DDF k-equation
+ * (functions_PREP.cuh:462-478), the IBM target-velocity table values
+ * (functions_IBM.cuh:289-306) and the ghost "last match" selection of the
+ * Neumann, periodic and outlet copies (functions_BC.cuh:345-369,732-744,
+ * 1122-1129).  These are checked only as transcriptions.
  */
 #include "../include/urbansph.h"
 
@@ -1052,6 +1058,10 @@ int bicgstab(Oracle& o, const std::vector<double>& rhs, std::vector<double>& x) 
     spmv(rp, cl, vl, dg, x, y);
     for (long i = 0; i < nr; i++) r[i] = rhs[i] - y[i];
     double rnorm = sqrt(dot(r, r));
+    if (!(bnorm <= DBL_MAX) || !(rnorm <= DBL_MAX)) {   // NaN/Inf in the system: no iteration
+        o.rep.rel_residual = rnorm / bnorm; o.rep.converged = 0;
+        return SPH_E_NONFINITE;
+    }
     if (rnorm <= pr.ppe_tol * bnorm) { o.rep.rel_residual = rnorm / bnorm; o.rep.converged = 1; return 0; }
     rhat = r;
     double rho_old = 1.0, alpha = 1.0, omega = 1.0;
@@ -1086,12 +1096,13 @@ int bicgstab(Oracle& o, const std::vector<double>& rhs, std::vector<double>& x) 
         for (long i = 0; i < nr; i++) r[i] = s[i] - omega * t[i];
         rnorm = sqrt(dot(r, r));
         if (rnorm <= pr.ppe_tol * bnorm) { conv = 1; break; }
-        if (omega == 0.0) break;
+        if (omega == 0.0 || !(rnorm <= DBL_MAX)) break;
         rho_old = rho;
     }
     o.rep.iterations = it > pr.ppe_maxit ? pr.ppe_maxit : it;
     o.rep.rel_residual = rnorm / bnorm;
     o.rep.converged = conv;
+    if (!conv && !(rnorm <= DBL_MAX)) return SPH_E_NONFINITE;
     return conv ? 0 : SPH_E_NOCONV;
 }
 

@@ -251,6 +251,28 @@ def test_error_paths():
     assert e.value.rc == abi.E_NOCONV
 
 
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_nonfinite_state_is_reported_at_once(precision):
+    """A NaN velocity makes the PPE right-hand side NaN: the solve ends
+    without iterating (no run to ppe_maxit) and the step says NONFINITE,
+    as the oracle does."""
+    P, _ = cases.box(nf=6, walls=3)
+    fl = np.nonzero(P["p_type"] == 1)[0]
+    P["ux"][fl[len(fl) // 2]] = np.nan
+    prm = cases.box_params(urbansph.default_params(), tol=1e-8, precision=precision)
+    prm.update(ppe_solver=abi.PPE_BICGSTAB)
+    o = oracle.Oracle(prm, len(P))
+    o.upload(P)
+    assert o.step(0, 1e-4) == abi.E_NONFINITE
+    g = urbansph.Case(prm, len(P))
+    g.upload(P)
+    with pytest.raises(urbansph.SPHError) as e:
+        g.step(0, 1e-4)
+    assert e.value.rc == abi.E_NONFINITE
+    r = g.ppe_report()
+    assert r.converged == 0 and r.iterations == 0
+
+
 def test_download_roundtrip():
     P, _ = cases.box(nf=6, walls=2)
     prm = cases.box_params(urbansph.default_params())

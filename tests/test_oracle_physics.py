@@ -253,3 +253,18 @@ def test_ppe_jacobi_is_one_sweep_of_the_operator():
     # compare only rows not overwritten later by the boundary kernels
     p = o2.field("PRES")[s["rows"]]
     np.testing.assert_allclose(p, x1, rtol=1e-10, atol=1e-10 * np.abs(x1).max())
+
+
+def test_nonfinite_state_ends_the_solve_at_once():
+    """A NaN in the state makes the PPE right-hand side NaN: the solve stops
+    without iterating and the step reports SPH_E_NONFINITE."""
+    P, _ = cases.box(nf=6, walls=3)
+    fl = np.nonzero(P["p_type"] == 1)[0]
+    P["ux"][fl[len(fl) // 2]] = np.nan
+    prm = cases.box_params(oracle.default_params(), tol=1e-8)
+    prm.update(ppe_solver=abi.PPE_BICGSTAB)
+    o = oracle.Oracle(prm, len(P))
+    o.upload(P)
+    assert o.step(0, 1e-4) == abi.E_NONFINITE
+    r = o.ppe_report()
+    assert r.converged == 0 and r.iterations == 0 and not np.isfinite(r.rel_residual)

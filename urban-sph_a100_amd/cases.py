@@ -235,11 +235,14 @@ def small_canyon(dx=0.01, **kw):
     return canyon(**args)
 
 
-def apr_targets(x, y, z, p_type, i_type, buildings, r1, r2):
+def apr_targets(x, y, z, p_type, i_type, buildings, r1, r2, most=None):
     """Refinement target levels of the multi-resolution canyon: 2 for active
     fluid outside the buildings within r2 of a building wall or roof, 1
     within r1, 0 elsewhere.  ``buildings`` as in the canyon geometry
-    (xc, yc, bx, by, bh)."""
+    (xc, yc, bx, by, bh).  With ``most`` (the MOST_buffer flags) the MOST
+    wall-model layer stays at its resolution: its children inherit the flag,
+    and two levels down the lowest ones sit at z = z0, where the log law's
+    log(z / z0) vanishes."""
     x, y, z = (np.asarray(v, np.float64) for v in (x, y, z))
     d = np.full(x.shape, np.inf)
     inside = np.zeros(x.shape, bool)
@@ -250,6 +253,8 @@ def apr_targets(x, y, z, p_type, i_type, buildings, r1, r2):
         d = np.minimum(d, np.sqrt(ex * ex + ey * ey + ez * ez))
         inside |= (np.abs(x - xc) < 0.5 * bx) & (np.abs(y - yc) < 0.5 * by) & (z < bh)
     active = (np.asarray(p_type) == 1) & (np.asarray(i_type) == 1) & ~inside
+    if most is not None:
+        active &= np.asarray(most) == 0
     t = np.zeros(x.shape)
     t[active & (d < r1)] = 1.0
     t[active & (d < r2)] = 2.0

@@ -13,23 +13,54 @@
 
 namespace sph {
 
+// Device blocks are recycled within a context: a freed block goes to the
+// context's cache and serves a later request of its size up to 1/8 more.
+// The steps that search every step (moving particles) or after refinement
+// would otherwise pay a hipMalloc/hipFree pair, and hipFree's device-wide
+// wait, per buffer per step.
 int dmalloc(sph_ctx* c, void** p, size_t bytes) {
     *p = nullptr;
-    if (bytes == 0) bytes = 8;
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 256;
+    auto it = c->cache.lower_bound(bytes);
+    if (it != c->cache.end() && it->first <= bytes + bytes / 8) {
+        *p = it->second;
+        c->live[*p] = it->first;
+        c->cache.erase(it);
+        return 0;
+    }
     hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !c->cache.empty()) {
+        (void)hipGetLastError();
+        dcache_release(c);
+        e = hipMalloc(p, bytes);
+    }
     if (e != hipSuccess) {
         *p = nullptr;
         (void)hipGetLastError();
         return fail(c, SPH_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
     }
+    c->live[*p] = bytes;
     c->bytes += (long long)bytes;
     return 0;
 }
 void dfree(sph_ctx* c, void* p) {
     if (!p) return;
-    size_t sz = 0;
-    if (hipMemPtrGetInfo(p, &sz) == hipSuccess) c->bytes -= (long long)sz;
-    (void)hipFree(p);
+    auto it = c->live.find(p);
+    if (it == c->live.end()) {
+        (void)hipFree(p);
+        return;
+    }
+    c->cache.emplace(it->second, p);
+    c->live.erase(it);
+}
+// Returns the cached blocks to HIP (hipFree waits for the device first).
+void dcache_release(sph_ctx* c) {
+    for (auto& b : c->cache) {
+        (void)hipFree(b.second);
+        c->bytes -= (long long)b.first;
+    }
+    c->cache.clear();
 }
 
 // AoS (reference layout) -> SoA (same order)
@@ -564,6 +595,8 @@ void sph_destroy(sph_ctx* c) {
     void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->nl_slice_off, c->nl_cnt,
                     c->nl_ent, c->stat_buf, c->apr_target, c->xnew[0], c->xnew[1], c->xnew[2]};
     for (auto p : rest) dfree(c, p);
+    dcache_release(c);
+    for (auto& b : c->live) (void)hipFree(b.first);   // none expected: every buffer is freed above
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }

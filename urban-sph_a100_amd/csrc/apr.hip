@@ -138,11 +138,11 @@ int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out) {
         ppe_free(c);
         if ((rc = set_ncell_run(c))) break;
         // the next pass needs only the cells; the neighbour list is built
-        // once after the last pass
+        // once after the last pass, on the cells and order of its search
         rc = nnps_run(c, false);
         split_any = true;
     }
-    if (rc == 0 && split_any) rc = nnps_run(c, true);
+    if (rc == 0 && split_any) rc = nnps_list(c);
     dfree(c, flag);
     dfree(c, rank);
     dfree(c, pick);

@@ -338,7 +338,9 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
     SPH_TRY(shape(e, 1, BLOCK, &sh));
     SPH_TRY(step_physics_front(e));
     const int prc = ppe_solve(e);
-    if (prc != 0 && prc != SPH_E_NOCONV) return prc;
+    // a case whose solve did not converge, or met NaN/Inf, still completes
+    // the step with the others; its error is reported at the end
+    if (prc != 0 && prc != SPH_E_NOCONV && prc != SPH_E_NONFINITE) return prc;
     SPH_TRY(step_physics_back(e));
     return prc;
 }
@@ -401,7 +403,7 @@ int engine_step(Eng& e, sph_ctx* const* cases, int nc, const int* count, const d
             if (!cases[k]->err.empty()) { first = cases[k]->err; break; }
         if (e.err.empty()) e.err = first;
         for (int k = 0; k < nc; k++)
-            if (cases[k]->err.empty() && rc != SPH_E_NOCONV) cases[k]->err = e.err;
+            if (cases[k]->err.empty() && rc != SPH_E_NOCONV && rc != SPH_E_NONFINITE) cases[k]->err = e.err;
     }
     return rc;
 }

@@ -382,6 +382,9 @@ static int build_neighbor_list(sph_ctx* c) {
     return 0;
 }
 
+// The neighbour list alone, on the cells and order of the last search.
+int nnps_list(sph_ctx* c) { return build_neighbor_list(c); }
+
 // Full count==0 search on the current persistent state (any order): cell
 // keys, stable sort, cell bounds, permutation of the state and, when
 // build_list, the per-particle neighbour list.

@@ -36,6 +36,9 @@
 // double accumulation and adds the float correction to a double solution.
 #include <algorithm>
 
+#include <cfloat>
+#include <cmath>
+
 #include "flags.h"
 #include "sph_internal.h"
 #include "sph_math.h"
@@ -688,6 +691,7 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(const CaseDesc* cds, int o
         sc[SC_IT] = 1.0; sc[SC_HALF] = 0.0; sc[SC_ZEROX] = 0.0;
         sc[SC_DONE] = 0.0; sc[SC_CONV] = 0.0; sc[SC_ITERS] = 0.0;
         if (bnorm == 0.0) { sc[SC_DONE] = 1.0; sc[SC_CONV] = 1.0; sc[SC_ZEROX] = 1.0; sc[SC_RNORM] = 0.0; }
+        else if (!(bnorm <= DBL_MAX) || !(rnorm <= DBL_MAX)) { sc[SC_DONE] = 1.0; }   // NaN/Inf in the system
         else if (rnorm <= tol * bnorm) { sc[SC_DONE] = 1.0; sc[SC_CONV] = 1.0; }
         else if (B == 0.0) { sc[SC_DONE] = 1.0; sc[SC_ITERS] = 1.0; }
         break;
@@ -707,7 +711,7 @@ __global__ void __launch_bounds__(FIN_BLOCK) k_kr_fin(const CaseDesc* cds, int o
         double rnorm = sqrt(A);
         sc[SC_RNORM] = rnorm;
         if (rnorm <= tol * sc[SC_BNORM]) { sc[SC_DONE] = 1.0; sc[SC_CONV] = 1.0; sc[SC_ITERS] = sc[SC_IT]; }
-        else if (sc[SC_OMEGA] == 0.0) { sc[SC_DONE] = 1.0; sc[SC_ITERS] = sc[SC_IT]; }
+        else if (sc[SC_OMEGA] == 0.0 || !(rnorm <= DBL_MAX)) { sc[SC_DONE] = 1.0; sc[SC_ITERS] = sc[SC_IT]; }
         else {
             double rho_new = B;
             sc[SC_RHO_OLD] = sc[SC_RHO];
@@ -874,7 +878,16 @@ int ppe_build(sph_ctx* c) {
         SPH_HIP(c, hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         SPH_HIP(c, hipStreamSynchronize(c->stream));
         dfree(c, ovf);
-        if (hovf) return fail(c, SPH_E_UNSUPPORTED, "a PPE row exceeds the lattice-order sort capacity; set ppe_reorder = 0");
+        if (hovf) {
+            // a row with more entries than the in-block sort holds (refined
+            // particles next to coarse ones): number this case's rows in
+            // cell order instead
+            const int keep = c->prm.ppe_reorder;
+            c->prm.ppe_reorder = 0;
+            const int rc = ppe_build(c);
+            c->prm.ppe_reorder = keep;
+            return rc;
+        }
     } else {
         hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
                            (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols);
@@ -1089,6 +1102,13 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters) {
     return 0;
 }
 
+// ||r|| / ||b||: 0 for a zero right-hand side, NaN when either norm is not
+// finite (a NaN or Inf in the system)
+static double rel_residual(double rn, double bn) {
+    if (!std::isfinite(rn) || !std::isfinite(bn)) return NAN;
+    return bn > 0 ? rn / bn : 0.0;
+}
+
 // PPE of every case of the step: assembly, then one Jacobi sweep or the
 // Krylov solve, then the pressures.
 int ppe_solve(Eng& e) {
@@ -1150,7 +1170,7 @@ int ppe_solve(Eng& e) {
             const double* sc = e.h_sc + k * SC_COUNT;
             c->rep.iterations = iters[k];
             c->rep.converged = sc[SC_CONV] != 0.0;
-            c->rep.rel_residual = sc[SC_BNORM] > 0 ? sc[SC_RNORM] / sc[SC_BNORM] : 0.0;
+            c->rep.rel_residual = rel_residual(sc[SC_RNORM], sc[SC_BNORM]);
         }
     } else {
         // outer FP64 iterative refinement around float inner solves; each
@@ -1178,11 +1198,11 @@ int ppe_solve(Eng& e) {
                 const sph_params& pr = c->prm;
                 const double* sc = e.h_sc + k * SC_COUNT;
                 const double rn = sc[SC_RNORM], bn = sc[SC_BNORM];
-                c->rep.rel_residual = bn > 0 ? rn / bn : 0.0;
+                c->rep.rel_residual = rel_residual(rn, bn);
                 c->rep.refinements = refine[k];
                 double* ct = e.h_ctl + k * CTL_COUNT;
                 if (rn <= pr.ppe_tol * bn) { c->rep.converged = 1; active[k] = 0; }
-                else if (refine[k] == pr.ppe_max_refine) active[k] = 0;
+                else if (refine[k] == pr.ppe_max_refine || !std::isfinite(rn) || !std::isfinite(bn)) active[k] = 0;
                 else {
                     // aim the correction at the final absolute tolerance, but
                     // not below what a float Krylov solve reliably attains
@@ -1203,8 +1223,13 @@ int ppe_solve(Eng& e) {
     }
     RL("ppe_scatter", 0, k_ppe_scatter, 0);
     int rc = 0;
-    for (sph_ctx* c : e.cs)
-        if (!c->rep.converged) rc = fail(c, SPH_E_NOCONV, "PPE did not converge within ppe_maxit");
+    for (sph_ctx* c : e.cs) {
+        if (c->rep.converged) continue;
+        if (!std::isfinite(c->rep.rel_residual))
+            rc = fail(c, SPH_E_NONFINITE, "non-finite PPE residual: the state holds NaN or Inf");
+        else if (rc != SPH_E_NONFINITE)
+            rc = fail(c, SPH_E_NOCONV, "PPE did not converge within ppe_maxit");
+    }
     return rc;
 }
 

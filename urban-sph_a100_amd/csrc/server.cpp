@@ -10,6 +10,7 @@
 // and batching policy testable and plannable on a host without a GPU.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -232,7 +233,7 @@ int sph_server_tick(sph_server* s, long long* finished, int max_finished, int* n
             tms[q] = r->count * r->dt;
         }
         int rc = sph_batch_step(s->batch, cs.data(), k, cnt.data(), dts.data(), tms.data());
-        if (rc && rc != SPH_E_NOCONV) {
+        if (rc && rc != SPH_E_NOCONV && rc != SPH_E_NONFINITE) {
             std::string why = sph_batch_last_error(s->batch);
             for (Request* r : s->active) { r->rc = rc; retire(s, r, SPH_REQ_FAILED); }
             s->active.clear();
@@ -243,6 +244,8 @@ int sph_server_tick(sph_server* s, long long* finished, int max_finished, int* n
             sph_ppe_info(r->ctx, &rep);
             r->ppe_iterations += rep.iterations;
             if (!rep.converged) r->noconv_steps++;
+            // a case whose state went NaN/Inf ends now, failed; the others go on
+            if (!rep.converged && !std::isfinite(rep.rel_residual)) r->rc = SPH_E_NONFINITE;
             // the real footprint replaces the estimate once the first step has
             // built the neighbour list and the PPE
             if (r->count == 0) {
@@ -260,9 +263,9 @@ int sph_server_tick(sph_server* s, long long* finished, int max_finished, int* n
     std::vector<Request*> keep;
     for (Request* r : s->active) {
         r->count++;
-        if (r->count < r->steps) { keep.push_back(r); continue; }
-        int state = SPH_REQ_DONE;
-        if (!s->dry) {
+        if (r->count < r->steps && r->rc == 0) { keep.push_back(r); continue; }
+        int state = r->rc ? SPH_REQ_FAILED : SPH_REQ_DONE;
+        if (!s->dry && state == SPH_REQ_DONE) {
             int rc = sph_synchronize(r->ctx);
             if (!rc) rc = collect(s, r);
             if (rc) { r->rc = rc; state = SPH_REQ_FAILED; }

@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/urbansph.h"
@@ -207,8 +209,13 @@ struct sph_ctx {
     sph::Profile* profp = nullptr;      // profile of the engine driving the step
     sph::Eng* solo = nullptr;           // engine of sph_step (a batch of one)
     std::string err;
-    long long bytes = 0;
+    long long bytes = 0;                // device memory held: live blocks and cached free blocks
     long long generation = 0;           // bumped whenever device buffers move
+    // device blocks: live ones with their sizes, and freed ones kept for
+    // reuse by later allocations of about the same size (every use of a
+    // block is ordered on the context's stream, so reuse needs no wait)
+    std::unordered_map<void*, size_t> live;
+    std::multimap<size_t, void*> cache;
 
     // persistent state
     double* pf[SPH_F_COUNT_F64 + 3] = {};  // P1 fields in SPH_F_* numbering, then eli, m_ref, h_ref
@@ -268,6 +275,7 @@ int dmalloc(sph_ctx* c, void** p, size_t bytes);
 template <class T>
 int dalloc(sph_ctx* c, T** p, size_t count) { return dmalloc(c, (void**)p, count * sizeof(T)); }
 void dfree(sph_ctx* c, void* p);
+void dcache_release(sph_ctx* c);
 
 // profiling around launches (the profile of the driving engine)
 int prof_begin(sph_ctx* c, const char* name, double alg_bytes);
@@ -287,6 +295,7 @@ inline unsigned grid1(long long n, int block = BLOCK) { return (unsigned)((n + b
 
 // nnps.hip
 int nnps_run(sph_ctx* c, bool build_list = true);
+int nnps_list(sph_ctx* c);
 int set_ncell_run(sph_ctx* c);
 int exclusive_scan_i32(sph_ctx* c, const int* in, int* out, long long n, int* total_out);
 int exclusive_scan_i64(sph_ctx* c, const long long* in, long long* out, long long n, long long* total_out);
