@@ -207,6 +207,48 @@ def run_apr(cases, abi, urbansph, steps, warmup, dt, tol, r1=0.05, r2=0.0125):
     return out
 
 
+def run_ale(cases, abi, urbansph, P, nrm, geom, steps, warmup, dt, tol):
+    """The 2M canyon in Lagrangian/ALE mode (eli = 1 for fluid): the
+    corrector moves the particles and every step runs the neighbour search
+    and rebuilds the PPE structure, mixed precision."""
+    P = P.copy()
+    P["eli"] = np.where(P["p_type"] == 1, 1.0, 0.0).astype(np.float32)
+    prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED)
+    prm.update(ale_positions=1)
+    c = urbansph.Case(prm, len(P))
+    c.upload(P, None, nrm)
+    count = 0
+    for k in range(warmup):
+        a = time.perf_counter()
+        c.step(count, dt)
+        c.sync()
+        log(f"[bench] ale: warmup step {k + 1}/{warmup} {1e3 * (time.perf_counter() - a):.0f} ms, "
+            f"PPE iters {c.ppe_report().iterations}")
+        count += 1
+    t0 = time.perf_counter()
+    iters = []
+    for k in range(steps):
+        c.step(count, dt)
+        c.sync()
+        iters.append(c.ppe_report().iterations)
+        count += 1
+        log(f"[bench] ale: step {k + 1}/{steps} at {time.perf_counter() - t0:.1f} s, PPE iters {iters[-1]}")
+    wall = time.perf_counter() - t0
+    rep = c.ppe_report()
+    c.profile(True)
+    c.step(count, dt)
+    c.sync()
+    prof = c.profile_report()
+    c.profile(False)
+    out = dict(value=len(P) * steps / wall, ms_per_step=wall / steps * 1e3, iters=iters, rel_res=rep.rel_residual,
+               converged=rep.converged,
+               kernel_ms=dict(sorted(((k, round(v[1], 2)) for k, v in prof.items()), key=lambda kv: -kv[1])[:14]))
+    log(f"[bench] ale: {out['value'] / 1e6:.2f} M particle-steps/s, {out['ms_per_step']:.1f} ms/step, iters {iters}")
+    log(f"[bench] ale: kernel ms of one step {out['kernel_ms']}")
+    c.close()
+    return out
+
+
 def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=24, rate=4.0):
     """A seeded Poisson stream of canyon requests through the server: wall
     clock, requests submitted at their arrival times, one co-batched step per
@@ -288,6 +330,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-ale", action="store_true", help="skip the Lagrangian/ALE leg")
     ap.add_argument("--no-apr", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -413,6 +456,14 @@ def main():
             "ms_per_step": round(ar["ms_per_step"], 3), "steps": 3, "warmup": 2,
             "refine_seconds": round(ar["refine_s"], 2), "ppe_iters_per_step": ar["iters"],
             "converged": ar["converged"], "kernel_ms_one_step": ar["kernel_ms"]}
+    if not args.no_ale:
+        al = run_ale(cases, abi, urbansph, P, nrm, geom, 3, 2, args.dt, args.tol)
+        line["ale"] = {
+            "workload": "the same 2M canyon in Lagrangian/ALE mode: fluid particles move (eli = 1), neighbour search "
+                        "and PPE structure rebuilt every step, mixed precision",
+            "particles": n, "value": round(al["value"], 1), "unit": "particle-steps/s",
+            "ms_per_step": round(al["ms_per_step"], 3), "steps": 3, "warmup": 2, "ppe_iters_per_step": al["iters"],
+            "converged": al["converged"], "kernel_ms_one_step": al["kernel_ms"]}
     if not args.no_serve:
         sv = run_serving(cases, abi, urbansph, args.dt, args.tol, n_req=args.serve_requests)
         line["latency_ms"] = {"p50": round(sv["p50_ms"], 1), "p99": round(sv["p99_ms"], 1),

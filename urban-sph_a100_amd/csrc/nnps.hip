@@ -177,8 +177,10 @@ int exclusive_scan_i64(sph_ctx* c, const long long* in, long long* out, long lon
     return exclusive_scan<long long>(c, in, out, n, total);
 }
 
-static int radix_sort(sph_ctx* c, int max_key) {
-    const int n = c->n;
+// Stable LSD radix sort of (key, value) pairs, keys in [0, max_key]; the
+// sorted pairs end in *keys / *vals (the buffers may be swapped with the
+// temporaries).
+int radix_sort_pairs(sph_ctx* c, int** keys, int** vals, int** ktmp, int** vtmp, int n, int max_key) {
     int bits = 1;
     while ((1LL << bits) <= (long long)max_key) bits++;
     int passes = (bits + 7) / 8;
@@ -188,7 +190,7 @@ static int radix_sort(sph_ctx* c, int max_key) {
     int* offs = nullptr;
     SPH_TRY(dalloc(c, &hist, hist_n));
     SPH_TRY(dalloc(c, &offs, hist_n));
-    int *kin = c->keys, *vin = c->perm, *kout = c->keys_tmp, *vout = c->perm_tmp;
+    int *kin = *keys, *vin = *vals, *kout = *ktmp, *vout = *vtmp;
     for (int p = 0; p < passes; p++) {
         int shift = 8 * p;
         SPH_TRY(prof_begin(c, "radix_hist", 4.0 * n));
@@ -204,13 +206,16 @@ static int radix_sort(sph_ctx* c, int max_key) {
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    if (kin != c->keys) {   // odd pass count: result is in the tmp buffers
-        std::swap(c->keys, c->keys_tmp);
-        std::swap(c->perm, c->perm_tmp);
+    if (kin != *keys) {   // odd pass count: result is in the tmp buffers
+        std::swap(*keys, *ktmp);
+        std::swap(*vals, *vtmp);
     }
     dfree(c, hist);
     dfree(c, offs);
     return 0;
+}
+static int radix_sort(sph_ctx* c, int max_key) {
+    return radix_sort_pairs(c, &c->keys, &c->perm, &c->keys_tmp, &c->perm_tmp, c->n, max_key);
 }
 
 // ------------------------------------------------------------------------

@@ -66,7 +66,9 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_step_begin(Seg sg) {
     S2.temp0[i] = P2.temp0[i]; S2.concn0[i] = P2.concn0[i];
     Q.ftotalx[i] = 0.0; Q.ftotaly[i] = 0.0; Q.ftotalz[i] = 0.0; Q.ftotal[i] = 0.0;
     Q.fpx[i] = 0.0; Q.fpy[i] = 0.0; Q.fpz[i] = 0.0; Q.dtemp[i] = 0.0; Q.dconcn[i] = 0.0;
-    Q.inv_xx[i] = 0.0; Q.inv_yy[i] = 0.0; Q.inv_zz[i] = 0.0; Q.inv_xy[i] = 0.0; Q.inv_yz[i] = 0.0; Q.inv_zx[i] = 0.0;
+    if (!cd.gc_fresh) {
+        Q.inv_xx[i] = 0.0; Q.inv_yy[i] = 0.0; Q.inv_zz[i] = 0.0; Q.inv_xy[i] = 0.0; Q.inv_yz[i] = 0.0; Q.inv_zx[i] = 0.0;
+    }
     if (cd.xnew[0]) { cd.xnew[0][i] = P.x[i]; cd.xnew[1][i] = P.y[i]; cd.xnew[2][i] = P.z[i]; }
 }
 
@@ -92,6 +94,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_snapshot(Seg sg, int which) {
 template <class R>
 __global__ void __launch_bounds__(MAX_BLOCK) k_grad_corr(Seg sg) {
     SEG_PARTICLE(sg);
+    if (cd.gc_fresh) return;   // unchanged since the last step of this search
     const KConst kc = cd.kc;
     const PView S = cd.S;
     const QView Q = cd.Q;
@@ -1042,8 +1045,12 @@ int step_physics_front(Eng& e) {
     bool any_count0 = false;
     for (size_t k = 0; k < e.cs.size(); k++) any_count0 |= e.h_desc[k].count == 0;
     SEGL("step_begin", -1, k_step_begin);
-    if (mixed) SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<float>);
-    else SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<double>);
+    bool any_gc = false;
+    for (size_t k = 0; k < e.cs.size(); k++) any_gc |= !e.h_desc[k].gc_fresh;
+    if (any_gc) {
+        if (mixed) SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<float>);
+        else SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<double>);
+    }
     if (mixed) SEGL("prep", TK_PREP, k_prep<float>);
     else SEGL("prep", TK_PREP, k_prep<double>);
     if (mixed) SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<float>);

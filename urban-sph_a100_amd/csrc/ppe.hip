@@ -234,6 +234,24 @@ __global__ void __launch_bounds__(BLOCK) k_lattice_key(int nr, const int* rows, 
     const long long kz = (long long)floor((S.z[i] - z0) / q + 0.5);
     key[r] = (kz * ny + ky) * nx + kx;
 }
+// The same key in 32 bits (lattices of fewer than 2^31 sites), paired with
+// the row's particle for the device radix sort.
+__global__ void __launch_bounds__(BLOCK) k_lattice_key32(int nr, const int* rows, PView S, double x0, double y0,
+                                                         double z0, double q, long long nx, long long ny, int* key,
+                                                         int* val) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const int i = rows[r];
+    const long long kx = (long long)floor((S.x[i] - x0) / q + 0.5);
+    const long long ky = (long long)floor((S.y[i] - y0) / q + 0.5);
+    const long long kz = (long long)floor((S.z[i] - z0) / q + 0.5);
+    key[r] = (int)((kz * ny + ky) * nx + kx);
+    val[r] = i;
+}
+__global__ void __launch_bounds__(BLOCK) k_widen(int n, const int* a, long long* b) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) b[r] = a[r];
+}
 __global__ void __launch_bounds__(BLOCK) k_rowof_from_rows(int nr, const int* rows, int* rowof) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < nr) rowof[rows[r]] = r;
@@ -799,6 +817,28 @@ static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
                  ez = ((double)d.z_max - d.z_min) / q + 3.0;
     if (!(q > 0) || ex * ey * ez > 4.0e18) { *used = false; return 0; }
     const double x0 = (double)d.x_min - q, y0 = (double)d.y_min - q, z0 = (double)d.z_min - q;
+    if (ex * ey * ez < 2.0e9) {
+        // device path: 32-bit keys, stable radix sort of (key, particle)
+        int *key = nullptr, *val = nullptr, *ktmp = nullptr, *vtmp = nullptr;
+        SPH_TRY(dalloc(c, &key, nr));
+        SPH_TRY(dalloc(c, &val, nr));
+        SPH_TRY(dalloc(c, &ktmp, nr));
+        SPH_TRY(dalloc(c, &vtmp, nr));
+        const long long nx = (long long)ex, ny = (long long)ey, nz = (long long)ez;
+        hipLaunchKernelGGL(k_lattice_key32, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows, S,
+                           x0, y0, z0, q, nx, ny, key, val);
+        SPH_TRY(launch_check(c, "lattice_key"));
+        SPH_TRY(radix_sort_pairs(c, &key, &val, &ktmp, &vtmp, nr, (int)(nx * ny * nz - 1)));
+        SPH_HIP(c, hipMemcpyAsync(P->rows, val, sizeof(int) * nr, hipMemcpyDeviceToDevice, c->stream));
+        hipLaunchKernelGGL(k_rowof_from_rows, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
+                           P->rowof);
+        SPH_TRY(launch_check(c, "rowof"));
+        dfree(c, key);
+        dfree(c, val);
+        dfree(c, ktmp);
+        dfree(c, vtmp);
+        return 0;
+    }
     long long* key = nullptr;
     SPH_TRY(dalloc(c, &key, nr));
     hipLaunchKernelGGL(k_lattice_key, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows, S, x0, y0,
@@ -915,12 +955,18 @@ int ppe_build(sph_ctx* c) {
             dfree(c, P->gbase); P->gbase = nullptr;
         }
     }
-    // real nnz (for reports)
-    std::vector<int> hc(nr);
-    SPH_HIP(c, hipMemcpyAsync(hc.data(), P->cnt, sizeof(int) * nr, hipMemcpyDeviceToHost, c->stream));
-    SPH_HIP(c, hipStreamSynchronize(c->stream));
-    P->nnz = 0;
-    for (int v : hc) P->nnz += v;
+    // real nnz (for reports and byte counts): the sum of the row counts
+    {
+        long long *wide = nullptr, *tmp = nullptr, total = 0;
+        SPH_TRY(dalloc(c, &wide, nr));
+        SPH_TRY(dalloc(c, &tmp, nr));
+        hipLaunchKernelGGL(k_widen, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->cnt, wide);
+        SPH_TRY(launch_check(c, "widen"));
+        SPH_TRY(exclusive_scan_i64(c, wide, tmp, nr, &total));
+        dfree(c, wide);
+        dfree(c, tmp);
+        P->nnz = total;
+    }
     if (P->ecols) { dfree(c, P->cols); P->cols = nullptr; }   // the 16-bit copy replaces them
     if (mixed) SPH_TRY(dalloc(c, &P->vals32, P->nnz_slots + 1));
     else SPH_TRY(dalloc(c, &P->vals64, P->nnz_slots + 1));

@@ -99,6 +99,7 @@ struct PPEView {
 // table of the launch.
 struct CaseDesc {
     int n, count;
+    int gc_fresh;        // the correction matrices inv_cm hold this search's values (fixed positions)
     double dt, time;
     KConst kc;
     sph_params pr;
@@ -211,6 +212,7 @@ struct sph_ctx {
     std::string err;
     long long bytes = 0;                // device memory held: live blocks and cached free blocks
     long long generation = 0;           // bumped whenever device buffers move
+    long long gc_gen = -1;              // generation whose gradient correction inv_cm holds
     // device blocks: live ones with their sizes, and freed ones kept for
     // reuse by later allocations of about the same size (every use of a
     // block is ordered on the context's stream, so reuse needs no wait)
@@ -296,6 +298,7 @@ inline unsigned grid1(long long n, int block = BLOCK) { return (unsigned)((n + b
 // nnps.hip
 int nnps_run(sph_ctx* c, bool build_list = true);
 int nnps_list(sph_ctx* c);
+int radix_sort_pairs(sph_ctx* c, int** keys, int** vals, int** ktmp, int** vtmp, int n, int max_key);
 int set_ncell_run(sph_ctx* c);
 int exclusive_scan_i32(sph_ctx* c, const int* in, int* out, long long n, int* total_out);
 int exclusive_scan_i64(sph_ctx* c, const long long* in, long long* out, long long n, long long* total_out);
