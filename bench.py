@@ -39,9 +39,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+GRAPHS = True   # --no-graphs: launch every BiCGSTAB kernel directly (for counter runs)
+
+
 def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, dt, tol):
     prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=precision)
     case = urbansph.Case(prm, len(P))
+    if not GRAPHS:
+        case.options(graphs=False, autotune=True)
     case.upload(P, None, nrm)
     log(f"[bench]   uploaded n={len(P)}")
     count = 0
@@ -331,6 +336,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-ale", action="store_true", help="skip the Lagrangian/ALE leg")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="headline legs without HIP graphs (each Krylov kernel launched directly)")
     ap.add_argument("--no-apr", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -362,6 +369,8 @@ def main():
               f"(multi-GPU halo exchange is design-only)", file=sys.stderr)
         sys.exit(2)
     args.warmup = max(1, args.warmup)
+    global GRAPHS
+    GRAPHS = not args.no_graphs
 
     import abi
     import cases

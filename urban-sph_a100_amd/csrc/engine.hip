@@ -276,6 +276,8 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     // inv_cm depends only on positions, h, m, rho, p_type and the neighbour
     // list; with fixed particles it is the same every step of a search
     d.gc_fresh = !c->prm.ale_positions && c->gc_gen == c->generation;
+    // the PPE values depend on the same inputs and on the row numbering
+    d.vals_fresh = !c->prm.ale_positions && c->vals_gen == c->generation;
     d.dt = dt;
     d.time = time;
     d.kc = KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15};
@@ -342,6 +344,7 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
     SPH_TRY(step_physics_front(e));
     for (int k = 0; k < nc; k++) cases[k]->gc_gen = cases[k]->generation;
     const int prc = ppe_solve(e);
+    for (int k = 0; k < nc; k++) cases[k]->vals_gen = cases[k]->generation;
     // a case whose solve did not converge, or met NaN/Inf, still completes
     // the step with the others; its error is reported at the end
     if (prc != 0 && prc != SPH_E_NOCONV && prc != SPH_E_NONFINITE) return prc;

@@ -373,18 +373,22 @@ __device__ __forceinline__ KSet<float> kset(const PPEView& v, float*) {
 // ------------------------------------------------------------------------
 // per-step assembly, functions_PPE.cuh:2-160
 template <class R, class V>
+// Launched over particles (not rows): a particle's neighbour list is read
+// in particle order, coalesced across the wave, whatever order the rows
+// are numbered in.
 __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) {
     SEG_ROW(sg);
     const PPEView pp = cd.ppe;
-    if (row >= pp.nr) return;
-    const int r = row;
+    if (row >= cd.n) return;
+    const int r = pp.rowof[row];
+    if (r < 0) return;
     const KConst kc = cd.kc;
     const PView S = cd.S;
     const QView Q = cd.Q;
     const NList nl = cd.nl;
     const double dt = cd.dt;
     const int* rowof = pp.rowof;
-    int i = pp.rows[r];
+    const int i = row;
     R hi = (R)(1.0 * S.h[i]);
     R tmp_A = calc_tmpA<R>(kc, hi);
     double xi = S.x[i], yi = S.y[i], zi = S.z[i];
@@ -395,7 +399,9 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     double bix = 0, biy = 0, biz = 0, Aij = 0, AijPij = 0, kn = 0;
     const unsigned* nrow = nl.ent + nl.slice_off[i >> 6] + (i & 63);
     int m = nl.cnt[i];
-    V* out = jacobi ? nullptr : wvals_of(pp, (V*)nullptr) + prow(pp.sl_off, r);
+    // with fixed particles the values of a search are written by its first
+    // step only; later steps compute the right-hand side
+    V* out = (jacobi || cd.vals_fresh) ? nullptr : wvals_of(pp, (V*)nullptr) + prow(pp.sl_off, r);
     const unsigned char* smap = pp.slotmap ? pp.slotmap + prow(pp.sl_off, r) : nullptr;
     int slot = 0;
     for (int k = 0; k < m; k++) {
@@ -424,7 +430,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         if (jacobi) AijPij += ccd * qj;
         if (j != i) {
             if (rowof[j] >= 0) {
-                if (!jacobi) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
+                if (out) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
                 slot++;
             } else {
                 kn += ccd * qj;
@@ -436,8 +442,10 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     if (jacobi) {
         pp.pnew[r] = S.pres0[i] + (bi + AijPij) / Aij;
     } else {
-        pp.diag[r] = Aij;
-        if (pp.fdiag) pp.fdiag[r] = (float)Aij;
+        if (out) {
+            pp.diag[r] = Aij;
+            if (pp.fdiag) pp.fdiag[r] = (float)Aij;
+        }
         pp.rhs[r] = bi + kn;
         pp.x[r] = q0;
     }
@@ -1176,7 +1184,7 @@ int ppe_solve(Eng& e) {
     {
         const int blk = tuned_block(e, TK_ASSEMBLE);
         Shape sh;
-        SPH_TRY(shape(e, 1, blk, &sh));
+        SPH_TRY(shape(e, 0, blk, &sh));   // over particles
         SPH_TRY(prof_begin(e, "ppe_assemble", asm_bytes));
         SPH_TRY(tune_begin(e, TK_ASSEMBLE, blk));
         if (mixed)

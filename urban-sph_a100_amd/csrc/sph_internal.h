@@ -100,6 +100,7 @@ struct PPEView {
 struct CaseDesc {
     int n, count;
     int gc_fresh;        // the correction matrices inv_cm hold this search's values (fixed positions)
+    int vals_fresh;      // the PPE values and diagonal hold this search's values (fixed positions)
     double dt, time;
     KConst kc;
     sph_params pr;
@@ -213,6 +214,7 @@ struct sph_ctx {
     long long bytes = 0;                // device memory held: live blocks and cached free blocks
     long long generation = 0;           // bumped whenever device buffers move
     long long gc_gen = -1;              // generation whose gradient correction inv_cm holds
+    long long vals_gen = -1;            // generation whose PPE values the matrix holds
     // device blocks: live ones with their sizes, and freed ones kept for
     // reuse by later allocations of about the same size (every use of a
     // block is ordered on the context's stream, so reuse needs no wait)
