@@ -149,9 +149,9 @@ typedef struct sph_params {
     double ppe_inner_tol;       /* MIXED: floor of each inner solve's relative
                                    tolerance (each aims at ppe_tol absolute)  */
     int    ppe_max_refine;      /* MIXED: max outer refinement steps          */
-    int    ppe_index16;         /* MIXED: 16-bit PPE column indices where they
-                                   fit (1, default) or 32-bit (0); the results
-                                   are identical, only the bytes moved differ */
+    int    ppe_index16;         /* 16-bit PPE column indices where they fit
+                                   (1, default) or 32-bit (0); the results are
+                                   identical, only the bytes moved differ      */
     int    ppe_reorder;         /* MIXED: number the PPE unknowns in lattice
                                    (z, y, x) order for contiguous SpMV gathers
                                    (1, default) or keep the particles' cell

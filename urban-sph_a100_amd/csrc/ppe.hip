@@ -499,8 +499,8 @@ __device__ __forceinline__ double offdiag_i32(const int* cp, const V* vp, int m,
 // The same over 16-bit columns: entry e = (group << EC_OFF_BITS) | offset,
 // column = wb[group] + offset, wb = the slice's group bases (in LDS).
 __device__ __forceinline__ int ecol(const int* wb, int e) { return wb[e >> EC_OFF_BITS] + (e & EC_OFF_MASK); }
-template <class X>
-__device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const float* vp, int m, const int* wb,
+template <class V, class X>
+__device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const V* vp, int m, const int* wb,
                                               const X* x) {
     double s = 0.0;
     int k = 0;
@@ -509,10 +509,10 @@ __device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const fl
         const int e1 = __builtin_nontemporal_load(ep + pslot(k + 1));
         const int e2 = __builtin_nontemporal_load(ep + pslot(k + 2));
         const int e3 = __builtin_nontemporal_load(ep + pslot(k + 3));
-        const float v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
-        const float v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
-        const float v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
-        const float v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
+        const V v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
+        const V v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
+        const V v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
+        const V v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
         const X x0 = x[ecol(wb, e0)], x1 = x[ecol(wb, e1)], x2 = x[ecol(wb, e2)], x3 = x[ecol(wb, e3)];
         s += (double)v0 * (double)x0;
         s += (double)v1 * (double)x1;
@@ -538,12 +538,8 @@ __device__ __forceinline__ double spmv_row(const PPEView& pp, int r, const V* va
     const long long base = prow(pp.sl_off, r);
     const int m = pp.cnt[r];
     double s;
-    if constexpr (sizeof(V) == 4) {
-        if (pp.ecols) s = offdiag_u16<X>(pp.ecols + base, (const float*)vals + base, m, wbase + (threadIdx.x & ~63), x);
-        else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
-    } else {
-        s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
-    }
+    if (pp.ecols) s = offdiag_u16<V, X>(pp.ecols + base, vals + base, m, wbase + (threadIdx.x & ~63), x);
+    else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
     return (double)dg[r] * (double)x[r] - s;
 }
 
@@ -942,7 +938,7 @@ int ppe_build(sph_ctx* c) {
         SPH_TRY(launch_check(c, "ppe_cols"));
     }
     P->nb = (nr + BLOCK - 1) / BLOCK;
-    if (mixed && c->prm.ppe_index16) {
+    if (c->prm.ppe_index16) {
         // 16-bit columns where every slice fits 64 windows of 1024 columns
         int* ovf = nullptr;
         SPH_TRY(dalloc(c, &P->ecols, P->nnz_slots + 1));
@@ -1109,7 +1105,7 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters) {
         // output and the dot partner (plus the slice group bases of 16-bit
         // columns); the x gathers of the neighbours are cache traffic
         const PPEState* P = e.cs[k]->ppe;
-        const bool i16 = P->ecols != nullptr && sizeof(V) == 4;
+        const bool i16 = P->ecols != nullptr;
         const double diag_b = sizeof(X) == 4 ? 4.0 : 8.0;
         spmv_bytes += (double)P->nnz * ((i16 ? 2.0 : 4.0) + sizeof(V)) +
                       (double)P->nr * (4.0 + diag_b + 3.0 * sizeof(X) + (i16 ? 4.0 : 0.0));
