@@ -228,7 +228,13 @@ typedef struct sph_ppe_report {
     long long nnz;              /* stored off-diagonal entries               */
     double rel_residual;        /* final ||b-Ax||/||b|| (true, FP64)         */
     int    converged;
+    int    ordering;            /* row numbering: SPH_ORDER_*                 */
+    long long slots;            /* stored SELL slots, padding included        */
+    int    index16;             /* 16-bit column indices in use               */
 } sph_ppe_report;
+/* PPE row numbering: the particles' cell order or lattice (z, y, x) order
+ * (see ppe_reorder). */
+enum { SPH_ORDER_CELL = 0, SPH_ORDER_LATTICE = 1 };
 
 typedef struct sph_ctx sph_ctx;
 

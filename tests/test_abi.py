@@ -68,7 +68,8 @@ def test_params_struct_layout_matches_c(tmp_path):
 
 
 @pytest.mark.parametrize("struct,cname", [("RunConfig", "sph_run_config"), ("PropTables", "sph_prop_tables"),
-                                          ("ServerInfo", "sph_server_info"), ("RequestInfo", "sph_request_info")])
+                                          ("ServerInfo", "sph_server_info"), ("RequestInfo", "sph_request_info"),
+                                          ("PPEReport", "sph_ppe_report")])
 def test_aux_struct_layouts_match_c(tmp_path, struct, cname):
     S = getattr(abi, struct)
     members = [n for n, _ in S._fields_]

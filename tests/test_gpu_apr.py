@@ -94,3 +94,28 @@ def test_apr_needs_room_and_a_search():
     with pytest.raises(urbansph.SPHError) as e:
         g.apr_refine(np.ones(len(P)), max_passes=1)
     assert e.value.rc == abi.E_NOMEM
+
+
+
+def test_refined_mixed_case_keeps_cell_order():
+    """A refined case has rows longer than the lattice-order sort holds:
+    MIXED keeps cell order for it, and its 16-bit columns, whose windows are
+    collected in any order, still encode the same matrix (same iteration
+    count and result as 32-bit columns)."""
+    P, nrm, geom = cases.small_canyon(dx=0.02)
+    out = []
+    for i16 in (1, 0):
+        prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-6, precision=abi.MIXED)
+        prm.update(ppe_index16=i16)
+        g = urbansph.Case(prm, 40 * len(P))
+        g.upload(P, None, nrm)
+        g.step(0, 5e-5)
+        g.apr_refine(_targets(g, geom["buildings"], 0.05, 0.03), max_passes=40)
+        g.step(1, 5e-5)
+        g.sync()
+        r = g.ppe_report()
+        out.append((r.ordering, r.index16, r.iterations, g.field("PRES")))
+        g.close()
+    assert out[0][0] == abi.ORDER_CELL and out[0][1] == 1 and out[1][1] == 0
+    assert out[0][2] == out[1][2]
+    np.testing.assert_array_equal(out[0][3], out[1][3])

@@ -111,7 +111,11 @@ class Domain(C.Structure):
 class PPEReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refinements", C.c_int),
                 ("unknowns", C.c_int), ("nnz", C.c_longlong),
-                ("rel_residual", C.c_double), ("converged", C.c_int)]
+                ("rel_residual", C.c_double), ("converged", C.c_int),
+                ("ordering", C.c_int), ("slots", C.c_longlong), ("index16", C.c_int)]
+
+
+ORDER_CELL, ORDER_LATTICE = 0, 1
 
 
 class KernelTime(C.Structure):

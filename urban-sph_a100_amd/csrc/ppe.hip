@@ -54,6 +54,7 @@ enum Scal {
 constexpr int EC_OFF_BITS = 10;
 constexpr int EC_OFF_MASK = (1 << EC_OFF_BITS) - 1;
 constexpr int EC_MAX_GROUPS = 64;
+constexpr int EC_MAX_WORDS = (65536 * 256 >> 10) >> 5;   // window bitmap words for 16.7M unknowns
 
 // Matrix layout: sliced ELLPACK over 64-row slices, one wave64 per slice,
 // entries column-major within a slice: entry k of row r lives at
@@ -69,6 +70,7 @@ struct PPEState {
     int nslices = 0;
     long long nnz_slots = 0;    // SELL storage incl. padding
     long long nnz = 0;          // real entries
+    int ordering = SPH_ORDER_CELL;
     int* rowof = nullptr;       // particle -> row or -1
     int* rows = nullptr;        // row -> particle
     int* cnt = nullptr;         // entries per row
@@ -301,44 +303,63 @@ __global__ void __launch_bounds__(SLICE) k_ppe_struct_sorted(int nr, const int* 
     }
 }
 
-// 16-bit columns of the MIXED matrix, one 64-thread block per 64-row slice.
+// 16-bit columns of the PPE matrix, one 64-thread block per 64-row slice.
 // The distinct aligned 1024-column windows that the slice's entries touch
-// become its groups, found in ascending order by repeated wave minima (each
-// row's columns ascend, so every lane walks its row once).  Entry k of row r
-// becomes (group << 10) | (col & 1023); the slice's group g starts at column
+// become its groups, in ascending order (collected in an LDS bitmap, so a
+// row may list its columns in any order).  Entry k of row r becomes
+// (group << 10) | (col & 1023); the slice's group g starts at column
 // gbase[64 * slice + g].  More than 64 groups: *overflow = 1 and the case
 // keeps 32-bit columns.
 __global__ void __launch_bounds__(SLICE) k_ecols_build(int nr, const long long* sl_off, const int* cnt,
                                                        const int* cols, unsigned short* ecols, int* gbase,
                                                        int* overflow) {
+    // the windows a slice touches, as a bitmap in LDS; rows may list their
+    // columns in any order
+    __shared__ unsigned win[EC_MAX_WORDS];
     __shared__ int gk[EC_MAX_GROUPS];
+    __shared__ int ngs;
     const int sl = blockIdx.x, l = threadIdx.x, r = sl * SLICE + l;
     const int m = r < nr ? cnt[r] : 0;
     const int* cp = cols + sl_off[sl] + l;
-    int p = 0, ng = 0, cur = -1;
-    while (true) {
-        while (p < m && (cp[pslot(p)] >> EC_OFF_BITS) <= cur) p++;
-        int k = p < m ? (cp[pslot(p)] >> EC_OFF_BITS) : INT_MAX;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) k = min(k, __shfl_xor(k, off, 64));
-        if (k == INT_MAX) break;
-        if (ng < EC_MAX_GROUPS && l == 0) gk[ng] = k;
-        ng++;
-        cur = k;
+    const int nwords = (((nr - 1) >> EC_OFF_BITS) >> 5) + 1;
+    for (int w = l; w < nwords; w += SLICE) win[w] = 0u;
+    __syncthreads();
+    for (int k = 0; k < m; k++) {
+        const int key = cp[pslot(k)] >> EC_OFF_BITS;
+        atomicOr(&win[key >> 5], 1u << (key & 31));
     }
     __syncthreads();
+    if (l == 0) {
+        int ng = 0;
+        for (int w = 0; w < nwords; w++) {
+            unsigned b = win[w];
+            while (b) {
+                const int bit = __builtin_ctz(b);
+                b &= b - 1;
+                if (ng < EC_MAX_GROUPS) gk[ng] = (w << 5) + bit;
+                ng++;
+            }
+        }
+        ngs = ng;
+    }
+    __syncthreads();
+    const int ng = ngs;
     if (ng > EC_MAX_GROUPS) {
         if (l == 0) *overflow = 1;
         return;
     }
     gbase[(size_t)sl * SLICE + l] = l < ng ? gk[l] << EC_OFF_BITS : 0;
     unsigned short* ep = ecols + sl_off[sl] + l;
-    int g = 0;
     for (int k2 = 0; k2 < m; k2++) {
         const int c = cp[pslot(k2)];
         const int key = c >> EC_OFF_BITS;
-        while (g + 1 < ng && gk[g] < key) g++;
-        ep[pslot(k2)] = (unsigned short)((g << EC_OFF_BITS) | (c & EC_OFF_MASK));
+        int lo = 0, hi = ng - 1;   // gk is ascending and holds key
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (gk[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        ep[pslot(k2)] = (unsigned short)((lo << EC_OFF_BITS) | (c & EC_OFF_MASK));
     }
 }
 
@@ -865,8 +886,44 @@ static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
     return 0;
 }
 
+// Entry count of every row and the SELL slice offsets for the current row
+// numbering.
+static int ppe_count_and_slices(sph_ctx* c, PPEState* P) {
+    const int nr = P->nr;
+    hipLaunchKernelGGL(k_ppe_struct<false>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
+                       (const int*)P->rowof, c->sview(), c->nlist(), P->cnt, (const long long*)nullptr, (int*)nullptr);
+    SPH_TRY(launch_check(c, "ppe_count"));
+    long long* w = nullptr;
+    SPH_TRY(dalloc(c, &w, P->nslices + 1));
+    hipLaunchKernelGGL(k_row_slice_width, dim3(grid1(P->nslices)), dim3(BLOCK), 0, c->stream, nr, P->nslices,
+                       (const int*)P->cnt, w);
+    SPH_TRY(launch_check(c, "ppe_width"));
+    SPH_TRY(exclusive_scan_i64(c, w, P->sl_off, P->nslices, &P->nnz_slots));
+    dfree(c, w);
+    return 0;
+}
+
+// Real entries (for reports, byte counts and the padding test): the sum of
+// the row counts.
+static int ppe_nnz(sph_ctx* c, PPEState* P) {
+    const int nr = P->nr;
+    long long *wide = nullptr, *tmp = nullptr, total = 0;
+    SPH_TRY(dalloc(c, &wide, nr));
+    SPH_TRY(dalloc(c, &tmp, nr));
+    hipLaunchKernelGGL(k_widen, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->cnt, wide);
+    SPH_TRY(launch_check(c, "widen"));
+    SPH_TRY(exclusive_scan_i64(c, wide, tmp, nr, &total));
+    dfree(c, wide);
+    dfree(c, tmp);
+    P->nnz = total;
+    return 0;
+}
+
 // Rows and column structure of a case's PPE from its neighbour list.
-int ppe_build(sph_ctx* c) {
+static int ppe_build_ordered(sph_ctx* c, bool allow_lattice);
+int ppe_build(sph_ctx* c) { return ppe_build_ordered(c, true); }
+
+static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
     ppe_free(c);
     PPEState* P = new PPEState();
     c->ppe = P;
@@ -891,22 +948,14 @@ int ppe_build(sph_ctx* c) {
     dfree(c, flag);
     dfree(c, scan);
     if (nr == 0) return 0;
+    if ((long long)nr > 65536LL * BLOCK) return fail(c, SPH_E_UNSUPPORTED, "PPE larger than 16.7M unknowns");
     const bool mixed = c->prm.precision == SPH_MIXED;
-    bool lattice = mixed && c->prm.ppe_reorder && nr > SLICE;
+    bool lattice = allow_lattice && mixed && c->prm.ppe_reorder && nr > SLICE;
     if (lattice) SPH_TRY(lattice_order(c, P, &lattice));
     P->nslices = (nr + SLICE - 1) / SLICE;
     SPH_TRY(dalloc(c, &P->cnt, nr));
     SPH_TRY(dalloc(c, &P->sl_off, P->nslices + 1));
-    hipLaunchKernelGGL(k_ppe_struct<false>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
-                       (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)nullptr, (int*)nullptr);
-    SPH_TRY(launch_check(c, "ppe_count"));
-    long long* w = nullptr;
-    SPH_TRY(dalloc(c, &w, P->nslices + 1));
-    hipLaunchKernelGGL(k_row_slice_width, dim3(grid1(P->nslices)), dim3(BLOCK), 0, c->stream, nr, P->nslices,
-                       (const int*)P->cnt, w);
-    SPH_TRY(launch_check(c, "ppe_width"));
-    SPH_TRY(exclusive_scan_i64(c, w, P->sl_off, P->nslices, &P->nnz_slots));
-    dfree(c, w);
+    SPH_TRY(ppe_count_and_slices(c, P));
     SPH_TRY(dalloc(c, &P->cols, P->nnz_slots + 1));
     hipLaunchKernelGGL(k_zero<int>, dim3(grid1(P->nnz_slots + 1)), dim3(BLOCK), 0, c->stream, P->nnz_slots + 1, P->cols);
     if (lattice) {
@@ -924,21 +973,18 @@ int ppe_build(sph_ctx* c) {
         dfree(c, ovf);
         if (hovf) {
             // a row with more entries than the in-block sort holds (refined
-            // particles next to coarse ones): number this case's rows in
+            // particles next to coarse ones): number this case's rows from
             // cell order instead
-            const int keep = c->prm.ppe_reorder;
-            c->prm.ppe_reorder = 0;
-            const int rc = ppe_build(c);
-            c->prm.ppe_reorder = keep;
-            return rc;
+            return ppe_build_ordered(c, false);
         }
+        P->ordering = SPH_ORDER_LATTICE;
     } else {
         hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
                            (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols);
         SPH_TRY(launch_check(c, "ppe_cols"));
     }
     P->nb = (nr + BLOCK - 1) / BLOCK;
-    if (c->prm.ppe_index16) {
+    if (c->prm.ppe_index16 && (long long)nr <= 65536LL * BLOCK) {   // the window bitmap covers 16.7M columns
         // 16-bit columns where every slice fits 64 windows of 1024 columns
         int* ovf = nullptr;
         SPH_TRY(dalloc(c, &P->ecols, P->nnz_slots + 1));
@@ -959,18 +1005,7 @@ int ppe_build(sph_ctx* c) {
             dfree(c, P->gbase); P->gbase = nullptr;
         }
     }
-    // real nnz (for reports and byte counts): the sum of the row counts
-    {
-        long long *wide = nullptr, *tmp = nullptr, total = 0;
-        SPH_TRY(dalloc(c, &wide, nr));
-        SPH_TRY(dalloc(c, &tmp, nr));
-        hipLaunchKernelGGL(k_widen, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->cnt, wide);
-        SPH_TRY(launch_check(c, "widen"));
-        SPH_TRY(exclusive_scan_i64(c, wide, tmp, nr, &total));
-        dfree(c, wide);
-        dfree(c, tmp);
-        P->nnz = total;
-    }
+    SPH_TRY(ppe_nnz(c, P));
     if (P->ecols) { dfree(c, P->cols); P->cols = nullptr; }   // the 16-bit copy replaces them
     if (mixed) SPH_TRY(dalloc(c, &P->vals32, P->nnz_slots + 1));
     else SPH_TRY(dalloc(c, &P->vals64, P->nnz_slots + 1));
@@ -1172,6 +1207,9 @@ int ppe_solve(Eng& e) {
         c->rep = sph_ppe_report{};
         c->rep.unknowns = nr_of(c);
         c->rep.nnz = c->ppe ? c->ppe->nnz : 0;
+        c->rep.ordering = c->ppe ? c->ppe->ordering : SPH_ORDER_CELL;
+        c->rep.slots = c->ppe ? c->ppe->nnz_slots : 0;
+        c->rep.index16 = c->ppe && c->ppe->ecols ? 1 : 0;
         if (c->rep.unknowns == 0) c->rep.converged = 1;
         rows_total += c->rep.unknowns;
         asm_bytes += (double)c->rep.unknowns * (15.0 * 8) + (double)c->rep.nnz * (4.0 + (mixed ? 4.0 : 8.0));
