@@ -177,7 +177,7 @@ QView sph_ctx::qview() const {
     v.nx = q[15]; v.ny = q[16]; v.nz = q[17];
     return v;
 }
-Grid sph_ctx::grid() const { return Grid{dom, gstr, gend}; }
+Grid sph_ctx::grid() const { return Grid{dom, gstr, gend, cstart, keys}; }
 NList sph_ctx::nlist() const { return NList{nl_slice_off, nl_cnt, nl_ent}; }
 
 // ----------------------------------------------------------- C ABI ----
@@ -302,8 +302,8 @@ int sph_upload(sph_ctx* c, const sph_part1* P1, const sph_part2* P2, const float
     long long cells = (long long)d.NI * d.NJ * d.NK + 1;
     if (cells > (1LL << 30)) return fail(c, SPH_E_ARG, "cell grid too large");
     if (c->gstr && cells != (long long)c->dom.num_cells) {
-        dfree(c, c->gstr); dfree(c, c->gend);
-        c->gstr = c->gend = nullptr;
+        dfree(c, c->gstr); dfree(c, c->gend); dfree(c, c->cstart);
+        c->gstr = c->gend = c->cstart = nullptr;
     }
     d.num_cells = (int)cells;
     c->dom = d;
@@ -592,7 +592,7 @@ void sph_destroy(sph_ctx* c) {
     for (auto p : c->s2) dfree(c, p);
     for (auto p : c->q) dfree(c, p);
     for (auto p : c->snap) dfree(c, p);
-    void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->nl_slice_off, c->nl_cnt,
+    void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->cstart, c->nl_slice_off, c->nl_cnt,
                     c->nl_ent, c->stat_buf, c->apr_target, c->xnew[0], c->xnew[1], c->xnew[2]};
     for (auto p : rest) dfree(c, p);
     dcache_release(c);

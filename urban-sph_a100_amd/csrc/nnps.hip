@@ -12,6 +12,8 @@
 // digit bits) and LDS per-wave digit counters, so equal digits keep input
 // order and the result equals a stable sort.
 #include "flags.h"
+#include <climits>
+
 #include "sph_internal.h"
 #include "sph_math.h"
 
@@ -242,6 +244,21 @@ __global__ void __launch_bounds__(BLOCK) k_gather(int n, const int* perm, const 
     if (i < n) out[i] = in[perm[i]];
 }
 
+// First particle of the sorted order whose key is >= cell, for every cell
+// and one past the last: the particles of consecutive cells a..b are
+// [cstart[a], cstart[b + 1]), in the order the cells hold them.
+__global__ void __launch_bounds__(BLOCK) k_cell_start(int ncs, const int* keys, int n, int* cstart) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= ncs) return;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < cell) lo = mid + 1;
+        else hi = mid;
+    }
+    cstart[cell] = lo;
+}
+
 // ------------------------------------------------------------------------
 // Neighbour list: the union of the dcell-query window and the binning
 // window of particle i (both +-ncell), scanned in increasing cell index so
@@ -254,43 +271,50 @@ __device__ __forceinline__ int nl_scan(int i, const Grid& g, const double* x, co
     int nc = ncell[i];
     Cell q = query_cell(d, xi, yi, zi);
     Cell b = bin_cell(d, xi, yi, zi);
-    int ilo = min(q.i, b.i) - nc, ihi = max(q.i, b.i) + nc;
     int jlo = min(q.j, b.j) - nc, jhi = max(q.j, b.j) + nc;
     int klo = min(q.k, b.k) - nc, khi = max(q.k, b.k) + nc;
     const double khi_r = kappa * hi, khalf = kappa * (0.5 * hi);
     int cnt = 0;
-    for (int ck = klo; ck <= khi; ck++)
-        for (int cj = jlo; cj <= jhi; cj++)
-            for (int ci = ilo; ci <= ihi; ci++) {
-                if (!cell_ok(d, ci, cj, ck)) continue;
-                bool inD = abs(ci - q.i) <= nc && abs(cj - q.j) <= nc && abs(ck - q.k) <= nc;
-                bool inB = abs(ci - b.i) <= nc && abs(cj - b.j) <= nc && abs(ck - b.k) <= nc;
+    for (int ck = max(klo, 0); ck <= min(khi, d.NK - 1); ck++)
+        for (int cj = max(jlo, 0); cj <= min(jhi, d.NJ - 1); cj++) {
+            // the cells of this x-row that either window holds are
+            // consecutive, so their particles are one range of the order
+            const bool rowD = abs(cj - q.j) <= nc && abs(ck - q.k) <= nc;
+            const bool rowB = abs(cj - b.j) <= nc && abs(ck - b.k) <= nc;
+            if (!rowD && !rowB) continue;
+            int i0 = INT_MAX, i1 = INT_MIN;
+            if (rowD) { i0 = min(i0, q.i - nc); i1 = max(i1, q.i + nc); }
+            if (rowB) { i0 = min(i0, b.i - nc); i1 = max(i1, b.i + nc); }
+            i0 = max(i0, 0);
+            i1 = min(i1, d.NI - 1);
+            if (i0 > i1) continue;
+            const int base = d.NI * cj + d.NI * d.NJ * ck;
+            const int s = g.cstart[base + i0], e = g.cstart[base + i1 + 1];
+            for (int j = s; j < e; j++) {
+                const int ci = g.keys[j] - base;
+                const bool inD = rowD && abs(ci - q.i) <= nc;
+                const bool inB = rowB && abs(ci - b.i) <= nc;
                 if (!inD && !inB) continue;
-                int cell = ci + d.NI * cj + d.NI * d.NJ * ck;
-                int s = g.gstr[cell];
-                if (s == -1) continue;
-                int e = g.gend[cell];
-                for (int j = s; j < e; j++) {
-                    double r = dist3<double>(xi, yi, zi, x[j], y[j], z[j]);
-                    double kmax = kappa * fmax(hi, h[j]);
-                    double tp = r + 1e-20, tm = r - 1e-20;
-                    unsigned f = 0;
-                    if (inD) {
-                        if (tp < kmax) f |= F_DP;
-                        if (tm < kmax) f |= F_DM;
-                        if (tp < khi_r) f |= F_HP;
-                        if (tm < khi_r) f |= F_HM;
-                    }
-                    if (inB) {
-                        if (r > 0 && r < khi_r) f |= F_B;
-                        if (tp < khalf) f |= F_BH;
-                    }
-                    if (f) {
-                        if (FILL) out[(size_t)cnt * SLICE] = (unsigned)j | f;
-                        cnt++;
-                    }
+                double r = dist3<double>(xi, yi, zi, x[j], y[j], z[j]);
+                double kmax = kappa * fmax(hi, h[j]);
+                double tp = r + 1e-20, tm = r - 1e-20;
+                unsigned f = 0;
+                if (inD) {
+                    if (tp < kmax) f |= F_DP;
+                    if (tm < kmax) f |= F_DM;
+                    if (tp < khi_r) f |= F_HP;
+                    if (tm < khi_r) f |= F_HM;
+                }
+                if (inB) {
+                    if (r > 0 && r < khi_r) f |= F_B;
+                    if (tp < khalf) f |= F_BH;
+                }
+                if (f) {
+                    if (FILL) out[(size_t)cnt * SLICE] = (unsigned)j | f;
+                    cnt++;
                 }
             }
+        }
     return cnt;
 }
 __global__ void __launch_bounds__(BLOCK) k_nl_count(int n, Grid g, const double* x, const double* y, const double* z,
@@ -406,6 +430,7 @@ int nnps_run(sph_ctx* c, bool build_list) {
     if (!c->gstr) {
         SPH_TRY(dalloc(c, &c->gstr, (size_t)d.num_cells + 1));
         SPH_TRY(dalloc(c, &c->gend, (size_t)d.num_cells + 1));
+        SPH_TRY(dalloc(c, &c->cstart, (size_t)d.num_cells + 2));
     }
     SPH_TRY(prof_begin(c, "hash", (3.0 * 8 + 4 + 8) * n));
     hipLaunchKernelGGL(k_hash, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, d, (const int*)c->pi[SPH_I_I_TYPE],
@@ -422,6 +447,9 @@ int nnps_run(sph_ctx* c, bool build_list) {
     hipLaunchKernelGGL(k_cell_bounds, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->keys, c->gstr,
                        c->gend);
     SPH_TRY(launch_check(c, "cell_bounds"));
+    hipLaunchKernelGGL(k_cell_start, dim3(grid1((long long)d.num_cells + 2)), dim3(BLOCK), 0, c->stream,
+                       d.num_cells + 2, (const int*)c->keys, n, c->cstart);
+    SPH_TRY(launch_check(c, "cell_start"));
     SPH_TRY(prof_end(c));
     SPH_TRY(permute_state(c));
     if (build_list) SPH_TRY(build_neighbor_list(c));

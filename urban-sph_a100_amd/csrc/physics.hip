@@ -28,6 +28,20 @@ namespace sph {
     const int i = first_ + (int)threadIdx.x;      \
     if (i >= cd.n) return;
 
+// Particles of the cells (i0..i1, cj, ck) that lie inside the grid, as one
+// range [rg.x, rg.y) of the sorted order; false when the row is outside the
+// grid or the clipped x-range is empty.
+__device__ __forceinline__ bool row_range(const Grid& g, int i0, int i1, int cj, int ck, int2& rg) {
+    const sph_domain& d = g.d;
+    if (cj < 0 || cj > d.NJ - 1 || ck < 0 || ck > d.NK - 1) return false;
+    i0 = max(i0, 0);
+    i1 = min(i1, d.NI - 1);
+    if (i0 > i1) return false;
+    const int base = d.NI * cj + d.NI * d.NJ * ck;
+    rg = make_int2(g.cstart[base + i0], g.cstart[base + i1 + 1]);
+    return true;
+}
+
 // Bin-cell range that contains every particle within distance r of p
 // (binning is monotone in each coordinate; a small margin absorbs the
 // double -> float rounding of the bounds).
@@ -409,30 +423,26 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_neumann(Seg sg, int which, int on
     Cell c = query_cell(d, xg, yg, zg);
     // the ghost pressure and the gradient only enter the inlet formula
     const bool inlet = (bt == 1 || bt == -1);
+    // The window's cells of one x-row are consecutive in cell order, so
+    // their particles form one range of the sorted order, visited in the
+    // same order as cell by cell.
     for (int z = -1; z <= 1 && inlet; z++)
-        for (int y = -1; y <= 1; y++)
-            for (int x = -1; x <= 1; x++) {
-                if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
-                int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
-                int s = g.gstr[cell];
-                if (s == -1) continue;
-                int e = g.gend[cell];
-                for (int j = s; j < e; j++) {
-                    R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
-                    R dist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
-                    if (dist < (R)pr.neumann_ghost_r) pres_g = pold[j];
-                }
+        for (int y = -1; y <= 1; y++) {
+            int2 rg;
+            if (!row_range(g, c.i - 1, c.i + 1, c.j + y, c.k + z, rg)) continue;
+            for (int j = rg.x; j < rg.y; j++) {
+                R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+                R dist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+                if (dist < (R)pr.neumann_ghost_r) pres_g = pold[j];
             }
+        }
     int nc = A.ncell[i];
     for (int z = -nc; z <= nc; z++)
-        for (int y = -nc; y <= nc; y++)
-            for (int x = -nc; x <= nc; x++) {
-                if (!cell_ok(d, c.i + x, c.j + y, c.k + z)) continue;
-                int cell = (c.i + x) + d.NI * (c.j + y) + d.NI * d.NJ * (c.k + z);
-                int s = g.gstr[cell];
-                if (s == -1) continue;
-                int e = g.gend[cell];
-                for (int j = s; j < e; j++) {
+        for (int y = -nc; y <= nc; y++) {
+            int2 rg;
+            if (!row_range(g, c.i - nc, c.i + nc, c.j + y, c.k + z, rg)) continue;
+            {
+                for (int j = rg.x; j < rg.y; j++) {
                     // only fluid, non-buffer neighbours carry weight (the
                     // reference multiplies the others by 0; skipping those
                     // exact zeros leaves every sum bit for bit the same)
@@ -458,6 +468,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_neumann(Seg sg, int which, int on
                     }
                 }
             }
+        }
     if (tflt_s < 1e-6) {
         A.pres[i] = 0.0;
     } else {

@@ -42,6 +42,8 @@ struct Grid {
     sph_domain d;
     const int* gstr;
     const int* gend;
+    const int* cstart;   // first particle (sorted order) with cell key >= c, c = 0 .. num_cells + 1
+    const int* keys;     // sorted cell keys of the search
 };
 // Neighbour list, SELL-64 over particles, ascending j per particle.
 struct NList {
@@ -234,6 +236,7 @@ struct sph_ctx {
     int* keys = nullptr;  int* perm = nullptr;
     int* keys_tmp = nullptr; int* perm_tmp = nullptr;
     int* gstr = nullptr;  int* gend = nullptr;
+    int* cstart = nullptr;
     int* sort_hist = nullptr; int sort_hist_n = 0;
     long long* nl_slice_off = nullptr; int* nl_cnt = nullptr; unsigned* nl_ent = nullptr;
     long long nl_total = 0;
