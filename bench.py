@@ -221,7 +221,7 @@ def run_apr(cases, abi, urbansph, steps, warmup, dt, tol, r1=0.05, r2=0.0125):
     out = dict(value=live * steps / wall, ms_per_step=wall / steps * 1e3, particles_live=live, particles_total=n,
                coarse=len(P), refine_s=t_apr, iters=iters, rel_res=rep.rel_residual, converged=rep.converged,
                ppe_rows=rep.unknowns, ppe_nnz=rep.nnz, ppe_slots=rep.slots, ppe_ordering=rep.ordering,
-               ppe_index16=rep.index16,
+               ppe_index16=rep.index16, ppe_tail_rows=rep.tail_rows,
                kernel_ms=dict(sorted(((k, round(v[1], 2)) for k, v in prof.items()), key=lambda kv: -kv[1])[:12]))
     log(f"[bench] apr: {out['value'] / 1e6:.2f} M particle-steps/s, {out['ms_per_step']:.1f} ms/step, iters {iters}")
     log(f"[bench] apr: kernel ms of one step {out['kernel_ms']}")
@@ -494,7 +494,7 @@ def main():
             "refine_seconds": round(ar["refine_s"], 2), "ppe_iters_per_step": ar["iters"],
             "converged": ar["converged"], "kernel_ms_one_step": ar["kernel_ms"],
             "ppe": {"rows": ar["ppe_rows"], "nnz": ar["ppe_nnz"], "sell_slots": ar["ppe_slots"],
-                    "ordering": ["cell", "lattice"][ar["ppe_ordering"]], "index16": ar["ppe_index16"]}}
+                    "ordering": ["cell", "lattice"][ar["ppe_ordering"]], "index16": ar["ppe_index16"], "tail_rows": ar["ppe_tail_rows"]}}
     if not args.no_ale:
         al = run_ale(cases, abi, urbansph, P, nrm, geom, 3, 2, args.dt, args.tol)
         line["ale"] = {

@@ -231,6 +231,9 @@ typedef struct sph_ppe_report {
     int    ordering;            /* row numbering: SPH_ORDER_*                 */
     long long slots;            /* stored SELL slots, padding included        */
     int    index16;             /* 16-bit column indices in use               */
+    int    tail_rows;           /* MIXED: rows of more than 256 entries, whose
+                                   entries past the 256th are stored
+                                   contiguously and summed by a wave each     */
 } sph_ppe_report;
 /* PPE row numbering: the particles' cell order or lattice (z, y, x) order
  * (see ppe_reorder). */

@@ -112,7 +112,7 @@ class PPEReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refinements", C.c_int),
                 ("unknowns", C.c_int), ("nnz", C.c_longlong),
                 ("rel_residual", C.c_double), ("converged", C.c_int),
-                ("ordering", C.c_int), ("slots", C.c_longlong), ("index16", C.c_int)]
+                ("ordering", C.c_int), ("slots", C.c_longlong), ("index16", C.c_int), ("tail_rows", C.c_int)]
 
 
 ORDER_CELL, ORDER_LATTICE = 0, 1

@@ -147,7 +147,8 @@ int shape(Eng& e, int kind, int block, Shape* out) {
     const int nc = (int)e.cs.size();
     auto units = [&](int k) -> long long {
         const sph_ctx* c = e.cs[k];
-        return kind == 0 ? c->n : ppe_view(c).nr;
+        // kind 0: particles; 1: PPE rows; 2: one 64-lane wave per long PPE row
+        return kind == 0 ? c->n : kind == 1 ? ppe_view(c).nr : 64LL * ppe_view(c).nlong;
     };
     out->block = block;
     if (nc == 1) {

@@ -94,6 +94,14 @@ struct PPEState {
     double* partA = nullptr;
     double* partB = nullptr;
     int nb = 0;
+    int* ltail = nullptr;       // MIXED: long rows (more than LONG_ROW entries) and their tails
+    int* lrows = nullptr;
+    long long* tail_off = nullptr;
+    int* tail_cols = nullptr;
+    float* tail_vals = nullptr;
+    int nlong = 0;
+    long long tail_total = 0;
+    double* lsum = nullptr;
 };
 
 // ------------------------------------------------------------------------
@@ -188,7 +196,8 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_rows(int n, const int* flag, cons
 }
 template <bool FILL>
 __global__ void __launch_bounds__(BLOCK) k_ppe_struct(int nr, const int* rows, const int* rowof, PView S, NList nl,
-                                                      int* cnt, const long long* sl_off, int* cols) {
+                                                      int* cnt, const long long* sl_off, int* cols, const int* ltail,
+                                                      const long long* tail_off, int* tail_cols) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
     int i = rows[r];
@@ -204,7 +213,10 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_struct(int nr, const int* rows, c
         if (j == i) continue;
         int rj = rowof[j];
         if (rj < 0) continue;
-        if (FILL) out[pslot(slot)] = rj;
+        if (FILL) {
+            if (slot < LONG_ROW || !ltail) out[pslot(slot)] = rj;
+            else tail_cols[tail_off[ltail[r]] + (slot - LONG_ROW)] = rj;
+        }
         slot++;
     }
     if (!FILL) cnt[r] = slot;
@@ -250,6 +262,25 @@ __global__ void __launch_bounds__(BLOCK) k_lattice_key32(int nr, const int* rows
     key[r] = (int)((kz * ny + ky) * nx + kx);
     val[r] = i;
 }
+__global__ void __launch_bounds__(BLOCK) k_long_flag(int nr, const int* cnt, int* flag) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nr) flag[r] = cnt[r] > LONG_ROW ? 1 : 0;
+}
+// long-row tables; the row's count becomes its SELL part
+__global__ void __launch_bounds__(BLOCK) k_long_index(int nr, int* cnt, const int* flag, const int* scan, int* lrows,
+                                                      int* ltail, long long* tlen) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    if (flag[r]) {
+        const int u = scan[r];
+        lrows[u] = r;
+        ltail[r] = u;
+        tlen[u] = cnt[r] - LONG_ROW;
+        cnt[r] = LONG_ROW;
+    } else {
+        ltail[r] = -1;
+    }
+}
 __global__ void __launch_bounds__(BLOCK) k_widen(int n, const int* a, long long* b) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < n) b[r] = a[r];
@@ -263,6 +294,8 @@ __global__ void __launch_bounds__(BLOCK) k_rowof_from_rows(int nr, const int* ro
 // order in which the assembly visits it).  Rows longer than SORT_CAP set
 // *overflow (the case then keeps cell order).
 constexpr int SORT_CAP = 160;
+// lattice-ordered rows fit the sort, so they never have tails
+static_assert(LONG_ROW >= SORT_CAP, "a lattice-ordered row must fit the SELL part");
 __global__ void __launch_bounds__(SLICE) k_ppe_struct_sorted(int nr, const int* rows, const int* rowof, PView S,
                                                              NList nl, const int* cnt, const long long* sl_off,
                                                              int* cols, unsigned char* slotmap, int* overflow) {
@@ -451,7 +484,10 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         if (jacobi) AijPij += ccd * qj;
         if (j != i) {
             if (rowof[j] >= 0) {
-                if (out) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
+                if (out) {
+                    if (slot < LONG_ROW || !pp.ltail) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
+                    else pp.tail_vals[pp.tail_off[pp.ltail[r]] + (slot - LONG_ROW)] = (float)cc;
+                }
                 slot++;
             } else {
                 kn += ccd * qj;
@@ -561,10 +597,44 @@ __device__ __forceinline__ double spmv_row(const PPEView& pp, int r, const V* va
     double s;
     if (pp.ecols) s = offdiag_u16<V, X>(pp.ecols + base, vals + base, m, wbase + (threadIdx.x & ~63), x);
     else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
+    if (pp.ltail) {
+        const int u = pp.ltail[r];
+        if (u >= 0) s += pp.lsum[u];
+    }
     return (double)dg[r] * (double)x[r] - s;
 }
 
 // r = rhs - A x ; rhat = r ; partials rhs.rhs (A) and r.r (B)
+// Tail sums of the long rows, one 64-lane wave per row: lane l takes tail
+// entries l, l + 64, ... (contiguous, so every load is coalesced) and a
+// fixed shuffle tree adds the lanes.  WHICH picks the vector the next
+// product multiplies: 0 the Krylov iterate, 1 p-hat, 2 s-hat, 3 the FP64
+// solution of the outer residual.
+template <class V, class X, int WHICH>
+__global__ void __launch_bounds__(BLOCK) k_kr_long(Seg sg) {
+    int first_;
+    const int cid_ = seg_case(sg, first_);
+    const CaseDesc& cd = sg.cd[cid_];
+    const int t = first_ + (int)threadIdx.x;
+    const PPEView pp = cd.ppe;
+    if ((t >> 6) >= pp.nlong) return;   // whole waves
+    if (SKIPPED(cd)) return;
+    if ((WHICH == 1 || WHICH == 2) && cd.sc[SC_DONE] != 0.0) return;
+    const int lane = t & 63, u = t >> 6;
+    const KSet<X> K = kset(pp, (X*)nullptr);
+    const X* x = WHICH == 0 ? K.x : WHICH == 1 ? K.phat : K.shat;
+    const long long e1 = pp.tail_off[u + 1];
+    double s = 0.0;
+    for (long long e = pp.tail_off[u] + lane; e < e1; e += 64) {
+        const int c = pp.tail_cols[e];
+        const double xv = WHICH == 3 ? pp.x[c] : (double)x[c];
+        s += (double)pp.tail_vals[e] * xv;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) pp.lsum[u] = s;
+}
+
 template <class V, class X>
 __global__ void __launch_bounds__(BLOCK) k_kr_init(Seg sg) {
     __shared__ double sh[2 * BLOCK];
@@ -817,7 +887,7 @@ void ppe_free(sph_ctx* c) {
     if (!P) return;
     void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->ecols, P->gbase, P->slotmap, P->vals64, P->vals32,
                     P->diag, P->fdiag, P->rhs, P->x, P->pnew, P->r, P->rhat, P->p, P->phat, P->v, P->s, P->shat, P->t, P->fr, P->frhat, P->fp,
-                    P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->partA, P->partB};
+                    P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->partA, P->partB, P->ltail, P->lrows, P->tail_off, P->tail_cols, P->tail_vals, P->lsum};
     for (void* q : ptrs) dfree(c, q);
     delete P;
     c->ppe = nullptr;
@@ -888,11 +958,42 @@ static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
 
 // Entry count of every row and the SELL slice offsets for the current row
 // numbering.
-static int ppe_count_and_slices(sph_ctx* c, PPEState* P) {
+static int ppe_count_and_slices(sph_ctx* c, PPEState* P, bool tails) {
     const int nr = P->nr;
     hipLaunchKernelGGL(k_ppe_struct<false>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
-                       (const int*)P->rowof, c->sview(), c->nlist(), P->cnt, (const long long*)nullptr, (int*)nullptr);
+                       (const int*)P->rowof, c->sview(), c->nlist(), P->cnt, (const long long*)nullptr, (int*)nullptr,
+                       (const int*)nullptr, (const long long*)nullptr, (int*)nullptr);
     SPH_TRY(launch_check(c, "ppe_count"));
+    if (tails) {
+        // rows past LONG_ROW entries keep the rest of their entries in a tail
+        int *flag = nullptr, *scan = nullptr, nlong = 0;
+        SPH_TRY(dalloc(c, &flag, nr));
+        SPH_TRY(dalloc(c, &scan, nr));
+        hipLaunchKernelGGL(k_long_flag, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->cnt, flag);
+        SPH_TRY(launch_check(c, "long_flag"));
+        SPH_TRY(exclusive_scan_i32(c, flag, scan, nr, &nlong));
+        if (nlong > 0) {
+            long long* tlen = nullptr;
+            P->nlong = nlong;
+            SPH_TRY(dalloc(c, &P->ltail, nr));
+            SPH_TRY(dalloc(c, &P->lrows, nlong));
+            SPH_TRY(dalloc(c, &P->tail_off, (size_t)nlong + 1));
+            SPH_TRY(dalloc(c, &P->lsum, nlong));
+            SPH_TRY(dalloc(c, &tlen, nlong));
+            hipLaunchKernelGGL(k_long_index, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, P->cnt, (const int*)flag,
+                               (const int*)scan, P->lrows, P->ltail, tlen);
+            SPH_TRY(launch_check(c, "long_index"));
+            SPH_TRY(exclusive_scan_i64(c, tlen, P->tail_off, nlong, &P->tail_total));
+            SPH_HIP(c, hipMemcpyAsync(P->tail_off + nlong, &P->tail_total, sizeof(long long), hipMemcpyHostToDevice,
+                                      c->stream));
+            SPH_HIP(c, hipStreamSynchronize(c->stream));
+            dfree(c, tlen);
+            SPH_TRY(dalloc(c, &P->tail_cols, (size_t)P->tail_total));
+            SPH_TRY(dalloc(c, &P->tail_vals, (size_t)P->tail_total));
+        }
+        dfree(c, flag);
+        dfree(c, scan);
+    }
     long long* w = nullptr;
     SPH_TRY(dalloc(c, &w, P->nslices + 1));
     hipLaunchKernelGGL(k_row_slice_width, dim3(grid1(P->nslices)), dim3(BLOCK), 0, c->stream, nr, P->nslices,
@@ -915,7 +1016,7 @@ static int ppe_nnz(sph_ctx* c, PPEState* P) {
     SPH_TRY(exclusive_scan_i64(c, wide, tmp, nr, &total));
     dfree(c, wide);
     dfree(c, tmp);
-    P->nnz = total;
+    P->nnz = total + P->tail_total;
     return 0;
 }
 
@@ -955,7 +1056,9 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
     P->nslices = (nr + SLICE - 1) / SLICE;
     SPH_TRY(dalloc(c, &P->cnt, nr));
     SPH_TRY(dalloc(c, &P->sl_off, P->nslices + 1));
-    SPH_TRY(ppe_count_and_slices(c, P));
+    // MIXED keeps the SELL slices at most LONG_ROW wide (FP64 keeps each row
+    // whole: its products add the entries in the reference's order)
+    SPH_TRY(ppe_count_and_slices(c, P, mixed));
     SPH_TRY(dalloc(c, &P->cols, P->nnz_slots + 1));
     hipLaunchKernelGGL(k_zero<int>, dim3(grid1(P->nnz_slots + 1)), dim3(BLOCK), 0, c->stream, P->nnz_slots + 1, P->cols);
     if (lattice) {
@@ -980,7 +1083,8 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
         P->ordering = SPH_ORDER_LATTICE;
     } else {
         hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
-                           (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols);
+                           (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols,
+                           (const int*)P->ltail, (const long long*)P->tail_off, P->tail_cols);
         SPH_TRY(launch_check(c, "ppe_cols"));
     }
     P->nb = (nr + BLOCK - 1) / BLOCK;
@@ -1041,6 +1145,8 @@ PPEView ppe_view(const sph_ctx* c) {
     v.fr = P->fr; v.frhat = P->frhat; v.fp = P->fp; v.fphat = P->fphat; v.fv = P->fv; v.fs = P->fs;
     v.fshat = P->fshat; v.ft = P->ft; v.fx = P->fx; v.frhs = P->frhs;
     v.partA = P->partA; v.partB = P->partB;
+    v.ltail = P->ltail; v.lrows = P->lrows; v.tail_off = P->tail_off; v.tail_cols = P->tail_cols;
+    v.tail_vals = P->tail_vals; v.nlong = P->nlong; v.lsum = P->lsum;
     return v;
 }
 
@@ -1084,14 +1190,31 @@ static int ctl_upload(Eng& e) {
         SPH_TRY(prof_end(e));                                                                      \
     } while (0)
 
+// the long-row sums for the product that follows (a case without long
+// rows has no waves in the launch)
+#define RLONG(V_, X_, WHICH)                                                                      \
+    do {                                                                                          \
+        Shape sh_;                                                                                \
+        SPH_TRY(shape(e, 2, BLOCK, &sh_));                                                        \
+        if (sh_.grid > 0) {                                                                       \
+            SPH_TRY(prof_begin(e, "kr_long", 0.0));                                               \
+            hipLaunchKernelGGL((k_kr_long<V_, X_, WHICH>), dim3(sh_.grid), dim3(BLOCK), 0, e.st,  \
+                               Seg{e.d_desc, sh_.tab});                                           \
+            SPH_TRY(launch_check(&e, "kr_long"));                                                 \
+            SPH_TRY(prof_end(e));                                                                 \
+        }                                                                                         \
+    } while (0)
+
 // one BiCGSTAB iteration of every unfinished case
 template <class V, class X>
 static int kr_iteration(Eng& e, double spmv_bytes) {
     RL("kr_p", 0, k_kr_p<X>);
+    RLONG(V, X, 1);
     RL("spmv", spmv_bytes, (k_kr_spmv<V, X, false>));
     KF(FIN_ALPHA);
     RL("kr_s", 0, k_kr_s<X>);
     KF(FIN_S);
+    RLONG(V, X, 2);
     RL("spmv", spmv_bytes, (k_kr_spmv<V, X, true>));
     KF(FIN_OMEGA);
     RL("kr_xr", 0, k_kr_xr<X>);
@@ -1146,6 +1269,7 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters) {
                       (double)P->nr * (4.0 + diag_b + 3.0 * sizeof(X) + (i16 ? 4.0 : 0.0));
         maxit = std::max(maxit, (int)e.h_ctl[k * CTL_COUNT + CTL_MAXIT]);
     }
+    RLONG(V, X, 0);
     RL("kr_init", spmv_bytes, (k_kr_init<V, X>));
     KF(FIN_INIT);
     RL("kr_zerox", 0, k_kr_zerox<X>);
@@ -1210,6 +1334,7 @@ int ppe_solve(Eng& e) {
         c->rep.ordering = c->ppe ? c->ppe->ordering : SPH_ORDER_CELL;
         c->rep.slots = c->ppe ? c->ppe->nnz_slots : 0;
         c->rep.index16 = c->ppe && c->ppe->ecols ? 1 : 0;
+        c->rep.tail_rows = c->ppe ? c->ppe->nlong : 0;
         if (c->rep.unknowns == 0) c->rep.converged = 1;
         rows_total += c->rep.unknowns;
         asm_bytes += (double)c->rep.unknowns * (15.0 * 8) + (double)c->rep.nnz * (4.0 + (mixed ? 4.0 : 8.0));
@@ -1275,6 +1400,7 @@ int ppe_solve(Eng& e) {
             SPH_TRY(ctl_wait(e));
             for (int k = 0; k < nc; k++) e.h_ctl[k * CTL_COUNT + CTL_SKIP] = active[k] ? 0.0 : 1.0;
             SPH_TRY(ctl_upload(e));
+            RLONG(float, float, 3);
             RL("outer_res", res_bytes, k_outer_res);
             KF(FIN_NORM);
             SPH_TRY(read_scalars(e));

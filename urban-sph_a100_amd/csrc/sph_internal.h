@@ -94,7 +94,19 @@ struct PPEView {
     double *r, *rhat, *p, *phat, *v, *s, *shat, *t;
     float *fr, *frhat, *fp, *fphat, *fv, *fs, *fshat, *ft, *fx, *frhs;
     double *partA, *partB;
+    // MIXED: a row of more than LONG_ROW entries (a coarse row beside
+    // refined particles) keeps its first LONG_ROW entries in the SELL slices
+    // and the rest contiguously in a tail; one wave per such row sums its
+    // tail just before each product, which adds that sum
+    const int* ltail;             // row -> long-row index, or -1 (null: no long rows)
+    const int* lrows;             // long-row index -> row
+    const long long* tail_off;    // long-row index -> first tail entry (nlong + 1)
+    const int* tail_cols;
+    float* tail_vals;
+    int nlong;
+    double* lsum;                 // long-row index -> tail sum for the next product
 };
+constexpr int LONG_ROW = 256;
 
 // Everything a pass needs to know about one case.  The engine keeps one per
 // case of a step in device memory; kernels find theirs through the segment
