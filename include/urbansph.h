@@ -278,7 +278,8 @@ int sph_set_engine_options(sph_ctx* c, int use_graphs, int autotune);
  * per size class (floor(log2(particles in the step)), then the size of each
  * tunable pass: grad_corr, prep, boundary3d, neumann, advection, freeslip,
  * pressure_force, ibm_force, ibm_spread, ppe_assemble; 0 while that class is
- * still being timed).  Returns the number of rows written (<= max_rows). */
+ * still being timed; neumann always runs one 64-thread wave per boundary
+ * particle and reads 64).  Returns the number of rows written (<= max_rows). */
 #define SPH_TUNE_PASSES 10
 int sph_tuning(const sph_ctx* c, int* out, int max_rows);
 

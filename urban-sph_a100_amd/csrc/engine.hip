@@ -79,6 +79,7 @@ static Eng::Tune& tune_for(Eng& e, int cls) {
         if (t.first == cls) return t.second;
     Eng::Tune t{};
     for (int k = 0; k < TK_COUNT; k++) t.block[k] = BLOCK;
+    t.block[TK_NEUMANN] = 64;   // one wave per boundary particle, not tuned
     e.tune.emplace_back(cls, t);
     return e.tune.back().second;
 }
@@ -129,6 +130,7 @@ static int tune_fold(Eng& e) {
         if (t.phase >= TUNE_CANDIDATES) continue;
         if (++t.phase < TUNE_CANDIDATES) continue;
         for (int k = 0; k < TK_COUNT; k++) {
+            if (k == TK_NEUMANN) continue;
             int best = -1;
             double best_ms = 0.0;
             for (int q = 0; q < TUNE_CANDIDATES; q++) {
@@ -147,8 +149,10 @@ int shape(Eng& e, int kind, int block, Shape* out) {
     const int nc = (int)e.cs.size();
     auto units = [&](int k) -> long long {
         const sph_ctx* c = e.cs[k];
-        // kind 0: particles; 1: PPE rows; 2: one 64-lane wave per long PPE row
-        return kind == 0 ? c->n : kind == 1 ? ppe_view(c).nr : 64LL * ppe_view(c).nlong;
+        // kind 0: particles; 1: PPE rows; 2: one 64-lane wave per long PPE row;
+        // 3: one 64-lane wave per boundary particle of the Neumann pass
+        return kind == 0 ? c->n : kind == 1 ? ppe_view(c).nr : kind == 2 ? 64LL * ppe_view(c).nlong
+                                                                           : 64LL * c->nm_count;
     };
     out->block = block;
     if (nc == 1) {
@@ -279,6 +283,8 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.gc_fresh = !c->prm.ale_positions && c->gc_gen == c->generation;
     // the PPE values depend on the same inputs and on the row numbering
     d.vals_fresh = !c->prm.ale_positions && c->vals_gen == c->generation;
+    d.nm_list = c->nm_list;
+    d.nm_count = c->nm_count;
     d.dt = dt;
     d.time = time;
     d.kc = KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15};

@@ -411,6 +411,36 @@ static int build_neighbor_list(sph_ctx* c) {
     return 0;
 }
 
+// The particles the Neumann pass updates (every p_type but fluid), in the
+// sorted order of the search.
+__global__ void __launch_bounds__(BLOCK) k_nm_flag(int n, const int* p_type, int* flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = p_type[i] != 1 ? 1 : 0;
+}
+__global__ void __launch_bounds__(BLOCK) k_nm_fill(int n, const int* flag, const int* scan, int* list) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) list[scan[i]] = i;
+}
+static int neumann_list(sph_ctx* c) {
+    const int n = c->n;
+    int *flag = nullptr, *scan = nullptr, cnt = 0;
+    SPH_TRY(dalloc(c, &flag, n));
+    SPH_TRY(dalloc(c, &scan, n));
+    hipLaunchKernelGGL(k_nm_flag, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->pi[SPH_I_P_TYPE], flag);
+    SPH_TRY(launch_check(c, "nm_flag"));
+    SPH_TRY(exclusive_scan_i32(c, flag, scan, n, &cnt));
+    dfree(c, c->nm_list);
+    c->nm_list = nullptr;
+    SPH_TRY(dalloc(c, &c->nm_list, cnt > 0 ? cnt : 1));
+    hipLaunchKernelGGL(k_nm_fill, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)flag, (const int*)scan,
+                       c->nm_list);
+    SPH_TRY(launch_check(c, "nm_fill"));
+    c->nm_count = cnt;
+    dfree(c, flag);
+    dfree(c, scan);
+    return 0;
+}
+
 // The neighbour list alone, on the cells and order of the last search.
 int nnps_list(sph_ctx* c) { return build_neighbor_list(c); }
 
@@ -452,6 +482,7 @@ int nnps_run(sph_ctx* c, bool build_list) {
     SPH_TRY(launch_check(c, "cell_start"));
     SPH_TRY(prof_end(c));
     SPH_TRY(permute_state(c));
+    SPH_TRY(neumann_list(c));
     if (build_list) SPH_TRY(build_neighbor_list(c));
     c->searched = true;
     c->generation++;   // state arrays were swapped for their sorted copies

@@ -401,22 +401,26 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_most_deposit(Seg sg) {
 // KERNEL_Neumann_boundary3D, functions_BC.cuh:290-446.  Pressures are read
 // from a snapshot (pold) taken before the launch.
 template <class R>
-__global__ void __launch_bounds__(MAX_BLOCK) k_neumann(Seg sg, int which, int only_count0) {
-    SEG_PARTICLE(sg);
+__global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_count0) {
+    // one 64-lane wave per boundary particle of the case's compact list
+    int first_;
+    const int cid_ = seg_case(sg, first_);
+    const CaseDesc& cd = sg.cd[cid_];
+    const int u = first_ >> 6, lane = threadIdx.x;
+    if (u >= cd.nm_count) return;
     if (only_count0 && cd.count != 0) return;   // the extra pre-advection call, ISPH_Calc.cuh:88-90
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const PView A = which ? cd.P : cd.S;
     const Grid g = cd.g;
     const double* pold = cd.snap[3];
-    if (A.p_type[i] == 1) return;
+    const int i = cd.nm_list[u];
     const sph_domain& d = g.d;
     R hi = (R)(2.0 * A.h[i]);
     R tmp_A = calc_tmpA<R>(kc, hi);
     R search_range = (R)kc.kappa * hi;
     double xi = A.x[i], yi = A.y[i], zi = A.z[i];
-    double grad_px = 0, grad_py = 0, grad_pz = 0, tpres = 0, tflt_s = 0, pres_g = 0;
-    double tux = 0, tuy = 0, tuz = 0;
+    double pres_g = 0;
     int bt = A.buffer_type[i];
     double xg = (bt == 1 || bt == -1) ? 2.0 * pr.L2 - xi : xi;
     double yg = yi, zg = zi;
@@ -425,50 +429,117 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_neumann(Seg sg, int which, int on
     const bool inlet = (bt == 1 || bt == -1);
     // The window's cells of one x-row are consecutive in cell order, so
     // their particles form one range of the sorted order, visited in the
-    // same order as cell by cell.
+    // same order as cell by cell.  Lanes test consecutive candidates; the
+    // last match in that order wins, as in the serial scan.
     for (int z = -1; z <= 1 && inlet; z++)
         for (int y = -1; y <= 1; y++) {
             int2 rg;
             if (!row_range(g, c.i - 1, c.i + 1, c.j + y, c.k + z, rg)) continue;
-            for (int j = rg.x; j < rg.y; j++) {
-                R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
-                R dist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
-                if (dist < (R)pr.neumann_ghost_r) pres_g = pold[j];
+            for (int b = rg.x; b < rg.y; b += 64) {
+                const int j = b + lane;
+                bool hit = false;
+                double pj = 0.0;
+                if (j < rg.y) {
+                    R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+                    R dist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+                    hit = dist < (R)pr.neumann_ghost_r;
+                    pj = pold[j];
+                }
+                const unsigned long long m = __ballot(hit);
+                if (m) pres_g = __shfl(pj, 63 - __clzll(m), 64);
             }
         }
-    int nc = A.ncell[i];
-    for (int z = -nc; z <= nc; z++)
-        for (int y = -nc; y <= nc; y++) {
-            int2 rg;
-            if (!row_range(g, c.i - nc, c.i + nc, c.j + y, c.k + z, rg)) continue;
-            {
-                for (int j = rg.x; j < rg.y; j++) {
-                    // only fluid, non-buffer neighbours carry weight (the
-                    // reference multiplies the others by 0; skipping those
-                    // exact zeros leaves every sum bit for bit the same)
-                    if (A.p_type[j] != 1 || A.buffer_type[j] != 0) continue;
-                    R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
-                    R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
-                    if (!(tdist < search_range)) continue;
-                    R mj = (R)A.m[j], rhoj = (R)A.rho[j];
-                    double presj = pold[j];
-                    R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
+    // Each lane forms the terms of one candidate exactly as the serial
+    // scan does; lanes 0..7 then add term k of every contributing candidate
+    // in candidate order, so every sum is bit for bit the serial one.
+    __shared__ double term[8][64];
+    __shared__ int use[64];
+    double acc = 0.0;   // lane k < 8 holds sum k
+    const int nt = inlet ? 8 : 5;
+    // one chunk of candidates v0 .. v0 + 63 of the window's flattened order:
+    // lane l holds candidate v0 + l (particle j, or j < 0 for none)
+    auto chunk = [&](int j, int count) {
+        int ok = 0;
+        // only fluid, non-buffer neighbours carry weight (the reference
+        // multiplies the others by 0; skipping those exact zeros leaves every
+        // sum bit for bit the same)
+        if (j >= 0 && A.p_type[j] == 1 && A.buffer_type[j] == 0) {
+            R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+            R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+            if (tdist < search_range) {
+                ok = 1;
+                R mj = (R)A.m[j], rhoj = (R)A.rho[j];
+                double presj = pold[j];
+                R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
+                term[0][lane] = (double)(mj / rhoj * (R)presj * twij);
+                term[1][lane] = (double)(mj / rhoj * (R)A.ux[j] * twij);
+                term[2][lane] = (double)(mj / rhoj * (R)A.uy[j] * twij);
+                term[3][lane] = (double)(mj / rhoj * (R)A.uz[j] * twij);
+                term[4][lane] = (double)(mj / rhoj * twij);
+                if (inlet) {
                     R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
                     R tdwx = dx / tdist * tdwij, tdwy = dy / tdist * tdwij, tdwz = dz / tdist * tdwij;
-                    tpres += (double)(mj / rhoj * (R)presj * twij);
-                    tux += (double)(mj / rhoj * (R)A.ux[j] * twij);
-                    tuy += (double)(mj / rhoj * (R)A.uy[j] * twij);
-                    tuz += (double)(mj / rhoj * (R)A.uz[j] * twij);
-                    tflt_s += (double)(mj / rhoj * twij);
-                    if (inlet) {
-                        R dp = (R)(presj - pres_g);
-                        grad_px += (double)(mj / rhoj * dp * tdwx);
-                        grad_py += (double)(mj / rhoj * dp * tdwy);
-                        grad_pz += (double)(mj / rhoj * dp * tdwz);
-                    }
+                    R dp = (R)(presj - pres_g);
+                    term[5][lane] = (double)(mj / rhoj * dp * tdwx);
+                    term[6][lane] = (double)(mj / rhoj * dp * tdwy);
+                    term[7][lane] = (double)(mj / rhoj * dp * tdwz);
                 }
             }
         }
+        use[lane] = ok;
+        __syncthreads();   // one wave per block: a wave barrier
+        if (lane < nt)
+            for (int q = 0; q < count; q++)
+                if (use[q]) acc += term[lane][q];
+        __syncthreads();
+    };
+    const int nc = A.ncell[i];
+    const int side = 2 * nc + 1, nrows = side * side;
+    if (nrows <= 64) {
+        // Lane k holds x-row k of the window (z-major, then y): its particle
+        // range and the exclusive prefix of the lengths, so the window's
+        // candidates are numbered v = 0 .. total-1 in scan order.
+        int rs = 0, rl = 0;
+        if (lane < nrows) {
+            int2 rg;
+            if (row_range(g, c.i - nc, c.i + nc, c.j + lane % side - nc, c.k + lane / side - nc, rg)) {
+                rs = rg.x;
+                rl = rg.y - rg.x;
+            }
+        }
+        int incl = rl;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        const int excl = incl - rl, total = __shfl(incl, 63, 64);
+        for (int v0 = 0; v0 < total; v0 += 64) {
+            const int v = v0 + lane, vv = min(v, total - 1);
+            // the row holding vv: the last row whose prefix is <= vv (never
+            // an empty row); every lane takes part in every shuffle
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = lo + step;
+                const int pe = __shfl(excl, min(cand, 63), 64);
+                if (cand < nrows && pe <= vv) lo = cand;
+            }
+            const int start = __shfl(rs, lo, 64), pre = __shfl(excl, lo, 64);
+            chunk(v < total ? start + (vv - pre) : -1, min(64, total - v0));
+        }
+    } else {
+        for (int z = -nc; z <= nc; z++)
+            for (int y = -nc; y <= nc; y++) {
+                int2 rg;
+                if (!row_range(g, c.i - nc, c.i + nc, c.j + y, c.k + z, rg)) continue;
+                for (int b = rg.x; b < rg.y; b += 64) chunk(b + lane < rg.y ? b + lane : -1, min(64, rg.y - b));
+            }
+    }
+    const double tpres = __shfl(acc, 0, 64), tux = __shfl(acc, 1, 64), tuy = __shfl(acc, 2, 64);
+    const double tuz = __shfl(acc, 3, 64), tflt_s = __shfl(acc, 4, 64);
+    const double grad_px = __shfl(acc, 5, 64), grad_py = __shfl(acc, 6, 64), grad_pz = __shfl(acc, 7, 64);
+    if (lane != 0) return;
     if (tflt_s < 1e-6) {
         A.pres[i] = 0.0;
     } else {
@@ -1029,8 +1100,20 @@ static bool mixed_of(const Eng& e) { return e.cs[0]->prm.precision == SPH_MIXED;
 // which: 0 = step working copy (SP1), 1 = persistent state (P1)
 static int run_neumann(Eng& e, int which, int only_count0) {
     SEGL("snapshot", -1, k_snapshot, which);
-    if (mixed_of(e)) SEGL("neumann", TK_NEUMANN, k_neumann<float>, which, only_count0);
-    else SEGL("neumann", TK_NEUMANN, k_neumann<double>, which, only_count0);
+    // one 64-thread block (one wave) per boundary particle of each case
+    Shape sh;
+    SPH_TRY(shape(e, 3, 64, &sh));
+    if (sh.grid > 0) {
+        SPH_TRY(prof_begin(e, "neumann", 0.0));
+        if (mixed_of(e))
+            hipLaunchKernelGGL(k_neumann<float>, dim3(sh.grid), dim3(64), 0, e.st, Seg{e.d_desc, sh.tab}, which,
+                               only_count0);
+        else
+            hipLaunchKernelGGL(k_neumann<double>, dim3(sh.grid), dim3(64), 0, e.st, Seg{e.d_desc, sh.tab}, which,
+                               only_count0);
+        SPH_TRY(launch_check(&e, "neumann"));
+        SPH_TRY(prof_end(e));
+    }
     return 0;
 }
 static int run_outlet(Eng& e, int which) {

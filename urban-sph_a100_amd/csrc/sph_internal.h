@@ -115,6 +115,8 @@ struct CaseDesc {
     int n, count;
     int gc_fresh;        // the correction matrices inv_cm hold this search's values (fixed positions)
     int vals_fresh;      // the PPE values and diagonal hold this search's values (fixed positions)
+    const int* nm_list;  // particles the Neumann pass updates (p_type != 1), in sorted order
+    int nm_count;
     double dt, time;
     KConst kc;
     sph_params pr;
@@ -249,6 +251,8 @@ struct sph_ctx {
     int* keys_tmp = nullptr; int* perm_tmp = nullptr;
     int* gstr = nullptr;  int* gend = nullptr;
     int* cstart = nullptr;
+    int* nm_list = nullptr;   // non-fluid particles of the search (Neumann pass)
+    int nm_count = 0;
     int* sort_hist = nullptr; int sort_hist_n = 0;
     long long* nl_slice_off = nullptr; int* nl_cnt = nullptr; unsigned* nl_ent = nullptr;
     long long nl_total = 0;
