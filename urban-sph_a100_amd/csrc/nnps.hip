@@ -263,73 +263,110 @@ __global__ void __launch_bounds__(BLOCK) k_cell_start(int ncs, const int* keys, 
 // Neighbour list: the union of the dcell-query window and the binning
 // window of particle i (both +-ncell), scanned in increasing cell index so
 // entries come out in ascending j, each tagged with the flags of flags.h.
+// One 64-lane wave per particle (particle = global thread index / 64).  The
+// window's x-rows are numbered z-major then y (lane k describes row k of a
+// batch of 64), their particle ranges are flattened into one candidate
+// order v, and each round of 64 candidates is compacted in order with a
+// ballot, so the list is the one a serial scan in cell order produces.
 template <bool FILL>
-__device__ __forceinline__ int nl_scan(int i, const Grid& g, const double* x, const double* y, const double* z,
-                                       const double* h, const int* ncell, double kappa, unsigned* out) {
+__device__ __forceinline__ void nl_wave(int i, const Grid& g, const double* x, const double* y, const double* z,
+                                        const double* h, const int* ncell, double kappa, int* cnt_out,
+                                        unsigned* out) {
+    const int lane = threadIdx.x & 63;
     const sph_domain& d = g.d;
-    double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
-    int nc = ncell[i];
-    Cell q = query_cell(d, xi, yi, zi);
-    Cell b = bin_cell(d, xi, yi, zi);
-    int jlo = min(q.j, b.j) - nc, jhi = max(q.j, b.j) + nc;
-    int klo = min(q.k, b.k) - nc, khi = max(q.k, b.k) + nc;
+    const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
+    const int nc = ncell[i];
+    const Cell q = query_cell(d, xi, yi, zi);
+    const Cell b = bin_cell(d, xi, yi, zi);
+    const int jlo = max(min(q.j, b.j) - nc, 0), jhi = min(max(q.j, b.j) + nc, d.NJ - 1);
+    const int klo = max(min(q.k, b.k) - nc, 0), khi = min(max(q.k, b.k) + nc, d.NK - 1);
+    const int nj = max(jhi - jlo + 1, 0), nk = max(khi - klo + 1, 0), nrows = nj * nk;
     const double khi_r = kappa * hi, khalf = kappa * (0.5 * hi);
     int cnt = 0;
-    for (int ck = max(klo, 0); ck <= min(khi, d.NK - 1); ck++)
-        for (int cj = max(jlo, 0); cj <= min(jhi, d.NJ - 1); cj++) {
-            // the cells of this x-row that either window holds are
-            // consecutive, so their particles are one range of the order
+    for (int r0 = 0; r0 < nrows; r0 += 64) {
+        // lane k: row r0 + k of the union window
+        int rs = 0, rl = 0, base = 0, flags = 0;
+        const int rk = r0 + lane;
+        if (rk < nrows) {
+            const int ck = klo + rk / nj, cj = jlo + rk % nj;
             const bool rowD = abs(cj - q.j) <= nc && abs(ck - q.k) <= nc;
             const bool rowB = abs(cj - b.j) <= nc && abs(ck - b.k) <= nc;
-            if (!rowD && !rowB) continue;
             int i0 = INT_MAX, i1 = INT_MIN;
             if (rowD) { i0 = min(i0, q.i - nc); i1 = max(i1, q.i + nc); }
             if (rowB) { i0 = min(i0, b.i - nc); i1 = max(i1, b.i + nc); }
             i0 = max(i0, 0);
             i1 = min(i1, d.NI - 1);
-            if (i0 > i1) continue;
-            const int base = d.NI * cj + d.NI * d.NJ * ck;
-            const int s = g.cstart[base + i0], e = g.cstart[base + i1 + 1];
-            for (int j = s; j < e; j++) {
-                const int ci = g.keys[j] - base;
-                const bool inD = rowD && abs(ci - q.i) <= nc;
-                const bool inB = rowB && abs(ci - b.i) <= nc;
-                if (!inD && !inB) continue;
-                double r = dist3<double>(xi, yi, zi, x[j], y[j], z[j]);
-                double kmax = kappa * fmax(hi, h[j]);
-                double tp = r + 1e-20, tm = r - 1e-20;
-                unsigned f = 0;
-                if (inD) {
-                    if (tp < kmax) f |= F_DP;
-                    if (tm < kmax) f |= F_DM;
-                    if (tp < khi_r) f |= F_HP;
-                    if (tm < khi_r) f |= F_HM;
-                }
-                if (inB) {
-                    if (r > 0 && r < khi_r) f |= F_B;
-                    if (tp < khalf) f |= F_BH;
-                }
-                if (f) {
-                    if (FILL) out[(size_t)cnt * SLICE] = (unsigned)j | f;
-                    cnt++;
-                }
+            base = d.NI * cj + d.NI * d.NJ * ck;
+            flags = (rowD ? 1 : 0) | (rowB ? 2 : 0);
+            if ((rowD || rowB) && i0 <= i1) {
+                rs = g.cstart[base + i0];
+                rl = g.cstart[base + i1 + 1] - rs;
             }
         }
-    return cnt;
+        int incl = rl;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        const int excl = incl - rl, total = __shfl(incl, 63, 64);
+        const int nr_b = min(64, nrows - r0);
+        for (int v0 = 0; v0 < total; v0 += 64) {
+            const int v = v0 + lane, vv = min(v, total - 1);
+            int lo = 0;   // the row holding vv: the last row whose prefix is <= vv
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = lo + step;
+                const int pe = __shfl(excl, min(cand, 63), 64);
+                if (cand < nr_b && pe <= vv) lo = cand;
+            }
+            const int start = __shfl(rs, lo, 64), pre = __shfl(excl, lo, 64);
+            const int rbase = __shfl(base, lo, 64), rflags = __shfl(flags, lo, 64);
+            unsigned f = 0;
+            int j = -1;
+            if (v < total) {
+                j = start + (vv - pre);
+                const int ci = g.keys[j] - rbase;
+                const bool inD = (rflags & 1) && abs(ci - q.i) <= nc;
+                const bool inB = (rflags & 2) && abs(ci - b.i) <= nc;
+                if (inD || inB) {
+                    double r = dist3<double>(xi, yi, zi, x[j], y[j], z[j]);
+                    double kmax = kappa * fmax(hi, h[j]);
+                    double tp = r + 1e-20, tm = r - 1e-20;
+                    if (inD) {
+                        if (tp < kmax) f |= F_DP;
+                        if (tm < kmax) f |= F_DM;
+                        if (tp < khi_r) f |= F_HP;
+                        if (tm < khi_r) f |= F_HM;
+                    }
+                    if (inB) {
+                        if (r > 0 && r < khi_r) f |= F_B;
+                        if (tp < khalf) f |= F_BH;
+                    }
+                }
+            }
+            const unsigned long long m = __ballot(f != 0);
+            if (FILL && f) {
+                const int below = __popcll(m & ((1ULL << lane) - 1ULL));
+                out[(size_t)(cnt + below) * SLICE] = (unsigned)j | f;
+            }
+            cnt += __popcll(m);
+        }
+    }
+    if (!FILL && lane == 0) *cnt_out = cnt;
 }
 __global__ void __launch_bounds__(BLOCK) k_nl_count(int n, Grid g, const double* x, const double* y, const double* z,
                                                     const double* h, const int* ncell, double kappa, int* cnt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    cnt[i] = nl_scan<false>(i, g, x, y, z, h, ncell, kappa, nullptr);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (i >= n) return;   // whole waves
+    nl_wave<false>(i, g, x, y, z, h, ncell, kappa, cnt + i, nullptr);
 }
 __global__ void __launch_bounds__(BLOCK) k_nl_fill(int n, Grid g, const double* x, const double* y, const double* z,
                                                    const double* h, const int* ncell, double kappa,
                                                    const long long* slice_off, unsigned* ent) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    unsigned* out = ent + slice_off[i >> 6] + (i & 63);
-    nl_scan<true>(i, g, x, y, z, h, ncell, kappa, out);
+    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (i >= n) return;   // whole waves
+    nl_wave<true>(i, g, x, y, z, h, ncell, kappa, nullptr, ent + slice_off[i >> 6] + (i & 63));
 }
 __global__ void __launch_bounds__(BLOCK) k_slice_width(int n, int nslices, const int* cnt, long long* w) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -391,7 +428,9 @@ static int build_neighbor_list(sph_ctx* c) {
     Grid g = c->grid();
     const double *x = c->pf[SPH_F_X], *y = c->pf[SPH_F_Y], *z = c->pf[SPH_F_Z], *h = c->pf[SPH_F_H];
     SPH_TRY(prof_begin(c, "nl_count", 0));
-    hipLaunchKernelGGL(k_nl_count, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
+    // one wave per particle
+    const unsigned nlgrid = (unsigned)(((long long)n * 64 + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_nl_count, dim3(nlgrid), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
                        (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, c->nl_cnt);
     SPH_TRY(launch_check(c, "nl_count"));
     SPH_TRY(prof_end(c));
@@ -404,7 +443,7 @@ static int build_neighbor_list(sph_ctx* c) {
     c->nl_total = total;
     SPH_TRY(dalloc(c, &c->nl_ent, (size_t)total + 1));
     SPH_TRY(prof_begin(c, "nl_fill", 0));
-    hipLaunchKernelGGL(k_nl_fill, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
+    hipLaunchKernelGGL(k_nl_fill, dim3(nlgrid), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
                        (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, (const long long*)c->nl_slice_off, c->nl_ent);
     SPH_TRY(launch_check(c, "nl_fill"));
     SPH_TRY(prof_end(c));
