@@ -6,8 +6,11 @@ Headline workload: the urban street canyon at 2M particles -- domain
 0.2 m spacing) carried by IBM markers, log-law inflow (u* = 0.3 m/s,
 z0 = 0.001 m), MOST ground layer with Obukhov length L = +50 m, Smagorinsky
 LES (Cs = 0.12), dt = 5e-5 s, BiCGSTAB PPE at relative tolerance 1e-6.  The
-same steps run in FP64 and in MIXED precision; ``value`` is the MIXED
-throughput.
+same steps run in FP64 (the reference's Jacobi-preconditioned BiCGSTAB)
+and in MIXED precision with the multigrid-preconditioned inner solve;
+``value`` is the MIXED throughput.  A third leg runs MIXED with the
+reference's Jacobi preconditioner, which separates what the precision buys
+from what the preconditioner buys.
 
 particle-steps/s = n * K / wall time of K timed steps (after W warm-up steps,
 the first of which runs the neighbour search and the next four of which time
@@ -42,8 +45,9 @@ def log(*a):
 GRAPHS = True   # --no-graphs: launch every BiCGSTAB kernel directly (for counter runs)
 
 
-def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, dt, tol):
+def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, dt, tol, precond=0):
     prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=precision)
+    prm.update(ppe_precond=precond)
     case = urbansph.Case(prm, len(P))
     if not GRAPHS:
         case.options(graphs=False, autotune=True)
@@ -82,6 +86,7 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
     rep = case.ppe_report()
     out = dict(value=len(P) * steps / wall, ms_per_step=wall / steps * 1e3, lat=lat, iters=iters,
                prof=prof, prof_steps=prof_steps, rel_res=rep.rel_residual, refinements=rep.refinements,
+               precond=rep.precond, mg_levels=rep.mg_levels, mg_coarsest=rep.mg_coarsest,
                warmup_s=t_setup, dev_bytes=case.device_bytes(), pres=case.field("PRES"), ux=case.field("UX"),
                first=first, tuning=case.tuning())
     case.close()
@@ -182,6 +187,7 @@ def run_apr(cases, abi, urbansph, steps, warmup, dt, tol, r1=0.05, r2=0.0125):
                                most=P["MOST_buffer"])
     cap = len(P) + 8 * int((t_deck == 1).sum()) + 72 * int((t_deck == 2).sum()) + 1024
     prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED)
+    prm.update(ppe_precond=abi.PRECOND_MULTIGRID)   # the solve reports Jacobi where rows have tails
     c = urbansph.Case(prm, cap)
     c.upload(P, None, nrm)
     c.step(0, dt)
@@ -222,6 +228,7 @@ def run_apr(cases, abi, urbansph, steps, warmup, dt, tol, r1=0.05, r2=0.0125):
                coarse=len(P), refine_s=t_apr, iters=iters, rel_res=rep.rel_residual, converged=rep.converged,
                ppe_rows=rep.unknowns, ppe_nnz=rep.nnz, ppe_slots=rep.slots, ppe_ordering=rep.ordering,
                ppe_index16=rep.index16, ppe_tail_rows=rep.tail_rows,
+               ppe_precond="multigrid" if rep.precond == abi.PRECOND_MULTIGRID else "jacobi",
                kernel_ms=dict(sorted(((k, round(v[1], 2)) for k, v in prof.items()), key=lambda kv: -kv[1])[:12]))
     log(f"[bench] apr: {out['value'] / 1e6:.2f} M particle-steps/s, {out['ms_per_step']:.1f} ms/step, iters {iters}")
     log(f"[bench] apr: kernel ms of one step {out['kernel_ms']}")
@@ -236,7 +243,7 @@ def run_ale(cases, abi, urbansph, P, nrm, geom, steps, warmup, dt, tol):
     P = P.copy()
     P["eli"] = np.where(P["p_type"] == 1, 1.0, 0.0).astype(np.float32)
     prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED)
-    prm.update(ale_positions=1)
+    prm.update(ale_positions=1, ppe_precond=abi.PRECOND_MULTIGRID)   # hierarchy rebuilt every step
     c = urbansph.Case(prm, len(P))
     c.upload(P, None, nrm)
     count = 0
@@ -263,7 +270,7 @@ def run_ale(cases, abi, urbansph, P, nrm, geom, steps, warmup, dt, tol):
     prof = c.profile_report()
     c.profile(False)
     out = dict(value=len(P) * steps / wall, ms_per_step=wall / steps * 1e3, iters=iters, rel_res=rep.rel_residual,
-               converged=rep.converged,
+               converged=rep.converged, ppe_precond="multigrid" if rep.precond == abi.PRECOND_MULTIGRID else "jacobi",
                kernel_ms=dict(sorted(((k, round(v[1], 2)) for k, v in prof.items()), key=lambda kv: -kv[1])[:14]))
     log(f"[bench] ale: {out['value'] / 1e6:.2f} M particle-steps/s, {out['ms_per_step']:.1f} ms/step, iters {iters}")
     log(f"[bench] ale: kernel ms of one step {out['kernel_ms']}")
@@ -358,7 +365,7 @@ def main():
     ap.add_argument("--no-apr", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--only", choices=["fp64", "mixed", "mixed_jacobi"], default=None)
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a 1 GiB device-to-device copy kernel (for scaling HBM counters)")
     args = ap.parse_args()
@@ -401,17 +408,21 @@ def main():
         f"markers {geom['n_marker']}) built in {time.perf_counter() - t:.1f}s")
 
     res = {}
-    for name, prec in (("fp64", abi.FP64), ("mixed", abi.MIXED)):
+    legs = (("fp64", abi.FP64, abi.PRECOND_JACOBI), ("mixed", abi.MIXED, abi.PRECOND_MULTIGRID),
+            ("mixed_jacobi", abi.MIXED, abi.PRECOND_JACOBI))
+    for name, prec, pc in legs:
         if args.only and args.only != name:
             continue
         log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps}")
         res[name] = run_precision(cases, abi, urbansph, P, nrm, geom, prec, args.steps, args.warmup, args.dt,
-                                  args.tol)
+                                  args.tol, precond=pc)
         r = res[name]
         log(f"[bench] {name}: {r['value'] / 1e6:.2f} M particle-steps/s, {r['ms_per_step']:.1f} ms/step, "
-            f"PPE iters {r['iters']}, refinements {r['refinements']}, rel res {r['rel_res']:.2e}")
+            f"PPE iters {r['iters']}, refinements {r['refinements']}, rel res {r['rel_res']:.2e}, "
+            f"preconditioner {'multigrid' if r['precond'] == abi.PRECOND_MULTIGRID else 'jacobi'} "
+            f"(levels {r['mg_levels']}, coarsest {r['mg_coarsest']})")
 
-    main_name = "mixed" if "mixed" in res else "fp64"
+    main_name = "mixed" if "mixed" in res else ("mixed_jacobi" if "mixed_jacobi" in res else "fp64")
     m = res[main_name]
     # dominant kernel: the SpMV of the PPE solve
     sp = m["prof"].get("spmv")
@@ -421,7 +432,7 @@ def main():
         avg_s = ms / launches / 1e3
         achieved = byts / launches / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(main_name),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("fp64" if main_name == "fp64" else "mixed"),
                 "kernel": "spmv", "avg_us": round(avg_s * 1e6, 2),
                 "alg_bytes_per_launch": round(byts / launches)}
     breakdown = {k: round(v[1] / m["prof_steps"], 3) for k, v in sorted(m["prof"].items(), key=lambda kv: -kv[1][1])}
@@ -437,11 +448,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair sums + f32 inner Krylov, f64 accumulation/dots/refinement" if main_name == "mixed" else "f64",
+        "dtype": "f32 pair sums + f32 inner Krylov, f64 accumulation/dots/refinement" if main_name != "fp64" else "f64",
         "data": "synthetic (generated street-canyon deck, no dataset)",
         "config": {"workload": f"urban street canyon 2.0x1.0x0.5 m, dx={args.dx} m, 4x2 IBM buildings, "
                                f"MOST L=+50 m, SSM Cs=0.12, dt={args.dt} s, BiCGSTAB tol {args.tol:g}",
-                   "particles": n, "precision": main_name},
+                   "particles": n, "precision": "fp64" if main_name == "fp64" else "mixed",
+                   "ppe_precond": "multigrid" if m["precond"] == abi.PRECOND_MULTIGRID else "jacobi",
+                   "mg_levels": m["mg_levels"], "mg_coarsest": m["mg_coarsest"]},
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                             "p99": round(float(np.percentile(lat, 99)), 3),
                             "what": "per-step latency of the resident 2M case"},
@@ -453,6 +466,13 @@ def main():
     if "fp64" in res:
         line["fp64"] = {"value": round(res["fp64"]["value"], 1), "ms_per_step": round(res["fp64"]["ms_per_step"], 3),
                         "ppe_iters_per_step": res["fp64"]["iters"]}
+        if "mixed_jacobi" in res:
+            line["mixed_jacobi"] = {
+                "value": round(res["mixed_jacobi"]["value"], 1),
+                "ms_per_step": round(res["mixed_jacobi"]["ms_per_step"], 3),
+                "ppe_iters_per_step": res["mixed_jacobi"]["iters"],
+                "over_fp64": round(res["mixed_jacobi"]["value"] / res["fp64"]["value"], 3),
+                "what": "MIXED with the reference's Jacobi preconditioner: the precision's share of the speed-up"}
         if "mixed" in res:
             line["mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
 
@@ -492,17 +512,17 @@ def main():
             "value": round(ar["value"], 1), "unit": "particle-steps/s (live particles)",
             "ms_per_step": round(ar["ms_per_step"], 3), "steps": 3, "warmup": 2,
             "refine_seconds": round(ar["refine_s"], 2), "ppe_iters_per_step": ar["iters"],
-            "converged": ar["converged"], "kernel_ms_one_step": ar["kernel_ms"],
+            "converged": ar["converged"], "kernel_ms_one_step": ar["kernel_ms"], "ppe_precond": ar["ppe_precond"],
             "ppe": {"rows": ar["ppe_rows"], "nnz": ar["ppe_nnz"], "sell_slots": ar["ppe_slots"],
                     "ordering": ["cell", "lattice"][ar["ppe_ordering"]], "index16": ar["ppe_index16"], "tail_rows": ar["ppe_tail_rows"]}}
     if not args.no_ale:
         al = run_ale(cases, abi, urbansph, P, nrm, geom, 3, 2, args.dt, args.tol)
         line["ale"] = {
-            "workload": "the same 2M canyon in Lagrangian/ALE mode: fluid particles move (eli = 1), neighbour search "
-                        "and PPE structure rebuilt every step, mixed precision",
+            "workload": "the same 2M canyon in Lagrangian/ALE mode: fluid particles move (eli = 1), neighbour search, "
+                        "PPE structure and multigrid hierarchy rebuilt every step, mixed precision",
             "particles": n, "value": round(al["value"], 1), "unit": "particle-steps/s",
             "ms_per_step": round(al["ms_per_step"], 3), "steps": 3, "warmup": 2, "ppe_iters_per_step": al["iters"],
-            "converged": al["converged"], "kernel_ms_one_step": al["kernel_ms"]}
+            "converged": al["converged"], "kernel_ms_one_step": al["kernel_ms"], "ppe_precond": al["ppe_precond"]}
     if not args.no_serve:
         sv = run_serving(cases, abi, urbansph, args.dt, args.tol, n_req=args.serve_requests)
         line["latency_ms"] = {"p50": round(sv["p50_ms"], 1), "p99": round(sv["p99_ms"], 1),

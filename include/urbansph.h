@@ -167,7 +167,27 @@ typedef struct sph_params {
                                    neighbour search runs every step
                                    (Lagrangian / ALE); 0 (default): positions
                                    fixed, search at count == 0 only          */
+    int    ppe_precond;         /* MIXED: preconditioner of the inner float
+                                   BiCGSTAB.  SPH_PRECOND_JACOBI (default):
+                                   the reference's diagonal scaling.
+                                   SPH_PRECOND_MULTIGRID: one V-cycle of an
+                                   aggregation multigrid (2x2x2 lattice blocks
+                                   per level, Galerkin coarse operators, damped
+                                   Jacobi smoothing, dense inverse of the
+                                   coarsest level), built once per set of PPE
+                                   values; the outer FP64 refinement and the
+                                   tolerance are unchanged.  Used in steps of
+                                   a single case whose PPE has more than 1024
+                                   unknowns and no row of more than 256
+                                   entries; elsewhere (batches of several
+                                   cases, refined cases with long rows) and
+                                   after a multigrid inner solve that stalls,
+                                   the solve uses SPH_PRECOND_JACOBI.  The
+                                   report says which ran.  FP64 ignores it. */
 } sph_params;
+
+/* Preconditioners of the MIXED inner solve (ppe_precond). */
+enum { SPH_PRECOND_JACOBI = 0, SPH_PRECOND_MULTIGRID = 1 };
 
 /* ---------------------------------------------------------------------------
  * Errors
@@ -234,6 +254,11 @@ typedef struct sph_ppe_report {
     int    tail_rows;           /* MIXED: rows of more than 256 entries, whose
                                    entries past the 256th are stored
                                    contiguously and summed by a wave each     */
+    int    precond;             /* preconditioner of the last inner solve:
+                                   SPH_PRECOND_*                              */
+    int    mg_levels;           /* multigrid levels incl. the PPE itself and
+                                   the coarsest (0 without multigrid)         */
+    int    mg_coarsest;         /* unknowns of the coarsest level             */
 } sph_ppe_report;
 /* PPE row numbering: the particles' cell order or lattice (z, y, x) order
  * (see ppe_reorder). */

@@ -1516,6 +1516,7 @@ void oracle_params_default(sph_params* p) {
     p->ppe_solver = SPH_PPE_REF_JACOBI; p->precision = SPH_FP64; p->ppe_tol = 1e-8; p->ppe_maxit = 5000;
     p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1;
     p->ale_positions = 0;
+    p->ppe_precond = SPH_PRECOND_JACOBI;   // the library's multigrid option; the oracle solves with Jacobi
 }
 
 void* oracle_create(const sph_params* prm, int n) {

@@ -1,0 +1,161 @@
+"""Multigrid preconditioner of the MIXED inner solve (ppe_precond =
+SPH_PRECOND_MULTIGRID), through the C ABI.
+
+The preconditioner changes the Krylov path, not the system or its stopping
+rule: at PPE tolerance 1e-6 the result must be as close to the converged
+FP64 oracle as the FP64 solve at the same tolerance is, driven to a tight
+tolerance it must stay within 1e-5 of it, it must take fewer iterations
+than the Jacobi form, and it must be bit-reproducible (hierarchy build,
+graphs or direct launches).  Cases it does not cover (PPE of at most 1024
+unknowns, batches of several cases) run the Jacobi form, and the report
+says so.
+"""
+import numpy as np
+import pytest
+
+import abi
+import cases
+import oracle
+import urbansph
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm(a - b)
+    s = np.linalg.norm(b)
+    return d / s if s > 0 else d
+
+
+def _case(which):
+    if which == "box":
+        P, nrm = cases.box(nf=12, walls=3)
+        return P, nrm, lambda tol: cases.box_params(urbansph.default_params(), tol=tol), 1e-4
+    P, nrm, geom = cases.small_canyon(dx=0.01)
+    return P, nrm, lambda tol: cases.canyon_params(geom, urbansph.default_params(), tol=tol), 5e-5
+
+
+def _oracle(P, nrm, prm, dt):
+    o = oracle.Oracle(prm, len(P))
+    o.upload(P, None, nrm)
+    o.step(0, dt)
+    out = {f: o.field(f) for f in ("PRES", "UX", "UY", "UZ")}
+    o.close()
+    return out
+
+
+def _gpu(P, nrm, prm, dt, steps=1, precond=abi.PRECOND_MULTIGRID, graphs=True):
+    prm.update(precision=abi.MIXED, ppe_precond=precond)
+    print(f"[mg] n={len(P)} precond={precond} steps={steps} graphs={graphs}", flush=True)
+    g = urbansph.Case(prm, len(P))
+    g.options(graphs=graphs, autotune=True)
+    g.upload(P, None, nrm)
+    reps = []
+    for c in range(steps):
+        g.step(c, dt)
+        g.sync()
+        r = g.ppe_report()
+        reps.append(dict(iterations=r.iterations, refinements=r.refinements, converged=r.converged,
+                         rel_residual=r.rel_residual, precond=r.precond, mg_levels=r.mg_levels,
+                         mg_coarsest=r.mg_coarsest, unknowns=r.unknowns))
+    return g, reps
+
+
+@pytest.mark.parametrize("which", ["box", "canyon"])
+def test_multigrid_same_tolerance_as_fp64(which):
+    P, nrm, mk, dt = _case(which)
+    conv = _oracle(P, nrm, mk(1e-12), dt)
+    at_tol = _oracle(P, nrm, mk(1e-6), dt)
+    g, reps = _gpu(P, nrm, mk(1e-6), dt)
+    _, jac = _gpu(P, nrm, mk(1e-6), dt, precond=abi.PRECOND_JACOBI)
+    rep = reps[0]
+    print(f"[mg] {which}: {rep}  jacobi iterations {jac[0]['iterations']}", flush=True)
+    assert rep["precond"] == abi.PRECOND_MULTIGRID and rep["mg_levels"] >= 2
+    assert 0 < rep["mg_coarsest"] <= 1024
+    assert rep["converged"] == 1 and rep["rel_residual"] <= 1e-6
+    assert jac[0]["precond"] == abi.PRECOND_JACOBI and jac[0]["mg_levels"] == 0
+    assert rep["iterations"] < jac[0]["iterations"]
+    for f in conv:
+        e_mg = rel_l2(g.field(f), conv[f])
+        e_fp64 = rel_l2(at_tol[f], conv[f])
+        print(f"[mg] {which} {f}: multigrid@1e-6 {e_mg:.2e}  fp64@1e-6 {e_fp64:.2e}", flush=True)
+        assert e_mg <= 2.0 * e_fp64 + 1e-6, f
+
+
+@pytest.mark.parametrize("which", ["box", "canyon"])
+def test_multigrid_precision_floor(which):
+    """Driven to a tight tolerance, the multigrid-preconditioned MIXED
+    solve stays within 1e-5 relative L2 of the converged FP64 oracle."""
+    P, nrm, mk, dt = _case(which)
+    conv = _oracle(P, nrm, mk(1e-12), dt)
+    g, reps = _gpu(P, nrm, mk(1e-9), dt)
+    assert reps[0]["precond"] == abi.PRECOND_MULTIGRID and reps[0]["converged"] == 1
+    for f in conv:
+        e = rel_l2(g.field(f), conv[f])
+        print(f"[mg-floor] {which} {f}: {e:.2e} ({reps[0]})", flush=True)
+        assert e <= 1e-5, f
+
+
+def test_multigrid_steps_reproducible_graphs_and_direct():
+    """Three steps (the hierarchy built at the first, reused after): every
+    step converges with the multigrid, and two fresh runs, one with HIP
+    graphs and one launching each kernel directly, agree bit for bit."""
+    P, nrm, mk, dt = _case("canyon")
+    a, ra = _gpu(P, nrm, mk(1e-6), dt, steps=3, graphs=True)
+    b, rb = _gpu(P, nrm, mk(1e-6), dt, steps=3, graphs=False)
+    for r in ra + rb:
+        assert r["precond"] == abi.PRECOND_MULTIGRID and r["converged"] == 1, r
+    assert [r["iterations"] for r in ra] == [r["iterations"] for r in rb]
+    for f in ("PRES", "UX", "UY", "UZ"):
+        np.testing.assert_array_equal(a.field(f), b.field(f), err_msg=f)
+
+
+def test_multigrid_falls_back_to_jacobi():
+    """A PPE of at most 1024 unknowns, and the cases of a batch, run the
+    Jacobi form even when the multigrid is asked for."""
+    P, nrm = cases.box(nf=6, walls=3)
+    prm = cases.box_params(urbansph.default_params(), tol=1e-6)
+    _, reps = _gpu(P, nrm, prm, 1e-4)
+    assert reps[0]["unknowns"] <= 1024
+    assert reps[0]["precond"] == abi.PRECOND_JACOBI and reps[0]["converged"] == 1
+
+    P1, n1, mk, dt = _case("canyon")
+    P2, n2 = cases.box(nf=12, walls=3)
+    p1 = mk(1e-6)
+    p2 = cases.box_params(urbansph.default_params(), tol=1e-6)
+    cs = []
+    for P, nrm, prm in ((P1, n1, p1), (P2, n2, p2)):
+        prm.update(precision=abi.MIXED, ppe_precond=abi.PRECOND_MULTIGRID)
+        c = urbansph.Case(prm, len(P))
+        c.upload(P, None, nrm)
+        cs.append(c)
+    b = urbansph.Batch()
+    b.step(cs, [0, 0], [dt, dt])
+    b.sync()
+    for c in cs:
+        r = c.ppe_report()
+        assert r.precond == abi.PRECOND_JACOBI and r.mg_levels == 0 and r.converged == 1
+
+
+def test_multigrid_profiled_build_and_ale():
+    """With the per-kernel profile on, a step that builds the hierarchy
+    (every step in Lagrangian/ALE mode) records each set-up launch and the
+    V-cycle passes, and the step completes."""
+    P, nrm, mk, dt = _case("canyon")
+    P = P.copy()
+    P["eli"] = np.where(P["p_type"] == 1, 1.0, 0.0).astype(np.float32)
+    prm = mk(1e-6)
+    prm.update(precision=abi.MIXED, ppe_precond=abi.PRECOND_MULTIGRID, ale_positions=1)
+    print(f"[mg] profiled ALE n={len(P)}", flush=True)
+    g = urbansph.Case(prm, len(P))
+    g.upload(P, None, nrm)
+    g.profile(True)
+    for c in range(2):
+        g.step(c, dt)
+        g.sync()
+        r = g.ppe_report()
+        assert r.precond == abi.PRECOND_MULTIGRID and r.converged == 1
+    prof = g.profile_report()
+    g.profile(False)
+    for k in ("mg_rap_count", "mg_rap_fill", "mg_gauss_jordan", "mg_res0", "mg_post0", "spmv"):
+        assert k in prof and prof[k][0] > 0, k

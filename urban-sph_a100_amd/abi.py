@@ -88,7 +88,7 @@ class Params(C.Structure):
            ("ppe_tol", C.c_double), ("ppe_maxit", C.c_int),
            ("ppe_inner_tol", C.c_double), ("ppe_max_refine", C.c_int), ("ppe_index16", C.c_int),
            ("ppe_reorder", C.c_int), ("apr_h_change", C.c_int),
-           ("ale_positions", C.c_int)]
+           ("ale_positions", C.c_int), ("ppe_precond", C.c_int)]
     )
 
     def update(self, **kw):
@@ -112,10 +112,12 @@ class PPEReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refinements", C.c_int),
                 ("unknowns", C.c_int), ("nnz", C.c_longlong),
                 ("rel_residual", C.c_double), ("converged", C.c_int),
-                ("ordering", C.c_int), ("slots", C.c_longlong), ("index16", C.c_int), ("tail_rows", C.c_int)]
+                ("ordering", C.c_int), ("slots", C.c_longlong), ("index16", C.c_int), ("tail_rows", C.c_int),
+                ("precond", C.c_int), ("mg_levels", C.c_int), ("mg_coarsest", C.c_int)]
 
 
 ORDER_CELL, ORDER_LATTICE = 0, 1
+PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
 
 
 class KernelTime(C.Structure):

@@ -57,19 +57,27 @@ int prof_end(sph_ctx* c) {
     SPH_HIP(c, hipEventRecord(c->profp->pending.back().b, c->stream));
     return 0;
 }
+// Every pending pair is timed and destroyed, also after a failure (the
+// first failure is returned), so no event outlives the list.
 int prof_collect(Eng& e) {
+    hipError_t first = hipSuccess;
     for (auto& p : e.prof.pending) {
         float ms = 0.f;
-        SPH_HIP(&e, hipEventSynchronize(p.b));
-        SPH_HIP(&e, hipEventElapsedTime(&ms, p.a, p.b));
-        auto& r = e.prof.recs[p.rec];
-        r.launches++;
-        r.ms += ms;
-        r.bytes += p.bytes;
+        hipError_t er = hipEventSynchronize(p.b);
+        if (er == hipSuccess) er = hipEventElapsedTime(&ms, p.a, p.b);
+        if (er == hipSuccess) {
+            auto& r = e.prof.recs[p.rec];
+            r.launches++;
+            r.ms += ms;
+            r.bytes += p.bytes;
+        } else if (first == hipSuccess) {
+            first = er;
+        }
         (void)hipEventDestroy(p.a);
         (void)hipEventDestroy(p.b);
     }
     e.prof.pending.clear();
+    SPH_HIP(&e, first);
     return 0;
 }
 

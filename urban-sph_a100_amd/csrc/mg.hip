@@ -1,0 +1,674 @@
+// Aggregation multigrid preconditioner of the MIXED inner BiCGSTAB.
+//
+// Hierarchy (built once per set of PPE values, on the device):
+//   level 0      the PPE operator itself (sliced ELLPACK, float values);
+//   level l + 1  one unknown per 2x2x2 block of level-l lattice positions
+//                (unsmoothed aggregation), with the Galerkin operator
+//                A_{l+1} = P^T A_l P, P the 0/1 aggregate indicator;
+//   coarsest     the first level of at most MG_COARSE_MAX unknowns, inverted
+//                densely (Gauss-Jordan in double, stored as float).
+// Level-0 lattice positions come from the particle positions at the PPE's
+// lattice spacing; a level's block coordinates are the previous level's
+// halved.  Every coarse row is formed by one wave that sorts its columns in
+// LDS and adds each column's terms in a fixed order (members in row order,
+// diagonal first, then entries in storage order), so the hierarchy is
+// bit-reproducible.
+//
+// Application (one V-cycle, z = M^-1 b): at every level but the coarsest
+//     x = w b / diag;  r = b - A x;  b_next = P^T r
+// down to the dense solve, then on the way up
+//     x += s P z_next;  z = x + w (b - A x) / diag
+// with w = MG_OMEGA (damped Jacobi) and s = MG_SCALE (the over-correction
+// that unsmoothed aggregation needs).  All products accumulate in double.
+// Every kernel returns at once when the solve's done flag is set, so a
+// captured iteration costs nothing after convergence.
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "ppe_format.h"
+#include "sph_internal.h"
+
+namespace sph {
+
+constexpr int MG_MAX_LEVELS = 12;
+static_assert(MG_COARSE_MAX <= 1024, "the coarsest level is inverted densely");
+constexpr int MG_ENT_CAP = 2048;    // terms of one coarse row (members' entries + diagonals)
+constexpr int MG_QMAX = 8;          // distinct columns of one coarse row: at most 64 * MG_QMAX
+constexpr double MG_OMEGA = 0.9;
+constexpr double MG_SCALE = 1.5;
+
+struct MGLevel {
+    int n = 0, nslices = 0;
+    long long slots = 0;
+    long long nx = 0, ny = 0, nz = 0;   // extents of the block coordinates
+    int *cx = nullptr, *cy = nullptr, *cz = nullptr;
+    // operator of a coarse level (level 0 is the PPE)
+    long long* sl_off = nullptr;
+    int* cnt = nullptr;
+    int* cols = nullptr;
+    float* vals = nullptr;   // -A_IJ, so that y = diag x - sum vals x[cols] as for the PPE
+    float* diag = nullptr;
+    // link to the next level
+    int* agg = nullptr;      // row -> row of the next level
+    int* mstart = nullptr;   // next-level row -> first member in mem (n_next + 1 entries)
+    int* mem = nullptr;      // this level's rows grouped by aggregate, ascending within each
+    // V-cycle vectors
+    float *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
+};
+struct MGState {
+    int nlev = 0;            // levels, the PPE and the coarsest included
+    MGLevel L[MG_MAX_LEVELS];
+    float* ainv = nullptr;   // dense inverse of the coarsest operator, row-major
+    long long fine_nnz = 0;
+};
+
+int mg_levels(const MGState* mg) { return mg ? mg->nlev : 0; }
+int mg_coarsest(const MGState* mg) { return mg && mg->nlev > 0 ? mg->L[mg->nlev - 1].n : 0; }
+
+// ------------------------------------------------------------------------
+// operators as row sources for the Galerkin product
+struct FineOp {
+    PPEView pp;
+    __device__ int cnt(int m) const { return pp.cnt[m]; }
+    __device__ int col(int m, int k) const {
+        const long long e = prow(pp.sl_off, m) + pslot(k);
+        return pp.ecols ? ecol(pp.gbase + (m & ~63), pp.ecols[e]) : pp.cols[e];
+    }
+    __device__ float val(int m, int k) const { return pp.vals32[prow(pp.sl_off, m) + pslot(k)]; }
+    __device__ float dg(int m) const { return pp.fdiag[m]; }
+};
+struct CoarseOp {
+    const long long* sl_off;
+    const int* cnt_;
+    const int* cols;
+    const float* vals;
+    const float* diag;
+    __device__ int cnt(int m) const { return cnt_[m]; }
+    __device__ int col(int m, int k) const { return cols[prow(sl_off, m) + pslot(k)]; }
+    __device__ float val(int m, int k) const { return vals[prow(sl_off, m) + pslot(k)]; }
+    __device__ float dg(int m) const { return diag[m]; }
+};
+static CoarseOp coarse_op(const MGLevel& L) { return CoarseOp{L.sl_off, L.cnt, L.cols, L.vals, L.diag}; }
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// exclusive prefix sum over the 64 lanes
+__device__ __forceinline__ int wave_excl(int v) {
+    const int lane = threadIdx.x & 63;
+    int s = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(s, off, 64);
+        if (lane >= off) s += u;
+    }
+    return s - v;
+}
+
+// ------------------------------------------------------------------------
+// hierarchy construction
+// lattice position of each PPE row's particle, clamped to the extents (a
+// particle outside them joins the boundary blocks)
+__device__ __forceinline__ int lattice_index(double v, double v0, double q, long long n) {
+    const double k = floor((v - v0) / q + 0.5);
+    return (int)fmin(fmax(k, 0.0), (double)(n - 1));
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_coord0(int nr, const int* rows, PView S, double x0, double y0, double z0,
+                                                     double q, long long nx, long long ny, long long nz, int* cx,
+                                                     int* cy, int* cz) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const int i = rows[r];
+    cx[r] = lattice_index(S.x[i], x0, q, nx);
+    cy[r] = lattice_index(S.y[i], y0, q, ny);
+    cz[r] = lattice_index(S.z[i], z0, q, nz);
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_key(int n, const int* cx, const int* cy, const int* cz, long long nx1,
+                                                  long long ny1, int* key, int* val) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    key[r] = (int)(((long long)(cz[r] >> 1) * ny1 + (cy[r] >> 1)) * nx1 + (cx[r] >> 1));
+    val[r] = r;
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_flag(int n, const int* key, int* flag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) flag[t] = (t == 0 || key[t] != key[t - 1]) ? 1 : 0;
+}
+// aggregate of every row, member lists, block coordinates of the aggregates
+__global__ void __launch_bounds__(BLOCK) k_mg_link(int n, const int* key, const int* val, const int* flag,
+                                                   const int* scan, long long nx1, long long ny1, int* agg, int* mem,
+                                                   int* mstart, int* cx1, int* cy1, int* cz1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int I = scan[t] + flag[t] - 1;
+    agg[val[t]] = I;
+    mem[t] = val[t];
+    if (flag[t]) {
+        mstart[I] = t;
+        const long long k = key[t];
+        cx1[I] = (int)(k % nx1);
+        cy1[I] = (int)((k / nx1) % ny1);
+        cz1[I] = (int)(k / (nx1 * ny1));
+    }
+    if (t == n - 1) mstart[I + 1] = n;
+}
+
+// One coarse row per 64-thread block (one wave): its terms are the
+// members' diagonals and entries mapped to aggregates; the distinct
+// aggregate columns are found by a bitonic sort in LDS, and each column's
+// terms are added by its owner lane (column slot mod 64) in a fixed order.
+// COUNT pass: the row's off-diagonal count.  FILL pass: columns ascending,
+// values -sum, the diagonal.  err: 1 = too many terms, 2 = too many columns.
+template <bool FILL, class OP>
+__global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg, const int* mstart, const int* mem,
+                                                  int* cnt1, const long long* sl_off1, int* cols1, float* vals1,
+                                                  float* diag1, int* err) {
+    __shared__ int sk[MG_ENT_CAP];
+    __shared__ int st_slot[SLICE];
+    __shared__ int st_col[SLICE];
+    __shared__ double st_val[SLICE];
+    const int I = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (I >= n1) return;
+    const int t0 = mstart[I], t1 = mstart[I + 1];
+    // terms of the row: one diagonal per member plus its entries
+    int nterm = 0;
+    for (int t = t0 + lane; t < t1; t += SLICE) nterm += op.cnt(mem[t]) + 1;
+    nterm = wave_sum(nterm);
+    if (nterm > MG_ENT_CAP) {
+        if (lane == 0) { err[0] = 1; if (!FILL) cnt1[I] = 0; }
+        return;
+    }
+    int np2 = 1;
+    while (np2 < nterm) np2 <<= 1;
+    // gather the aggregate columns (the member's own aggregate for its diagonal)
+    int pos = 0;
+    for (int t = t0; t < t1; t++) {
+        const int m = mem[t];
+        const int c = op.cnt(m);
+        for (int k = lane - 1; k < c; k += SLICE) sk[pos + k + 1] = k < 0 ? I : agg[op.col(m, k)];
+        pos += c + 1;
+    }
+    for (int p = nterm + lane; p < np2; p += SLICE) sk[p] = INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= np2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < np2; i += SLICE) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int a = sk[i], b = sk[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) { sk[i] = b; sk[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // distinct columns, compacted in place (each lane owns a run of the sorted list)
+    const int per = (nterm + SLICE - 1) / SLICE;
+    const int p0 = min(nterm, lane * per), p1 = min(nterm, p0 + per);
+    int own = 0;
+    for (int p = p0; p < p1; p++) own += (p == 0 || sk[p] != sk[p - 1]) ? 1 : 0;
+    const int at = wave_excl(own);
+    const int total = wave_sum(own);
+    int keep[MG_ENT_CAP / SLICE];
+    int nk = 0;
+    for (int p = p0; p < p1; p++)
+        if (p == 0 || sk[p] != sk[p - 1]) keep[nk++] = sk[p];
+    __syncthreads();
+    for (int q = 0; q < nk; q++) sk[at + q] = keep[q];
+    __syncthreads();
+    if (total > SLICE * MG_QMAX) {
+        if (lane == 0) { err[0] = 2; if (!FILL) cnt1[I] = 0; }
+        return;
+    }
+    if (!FILL) {
+        if (lane == 0) cnt1[I] = total - 1;
+        return;
+    }
+    auto slot_of = [&](int J) {
+        int lo = 0, hi = total - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sk[mid] < J) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const int sI = slot_of(I);
+    double acc[MG_QMAX];
+    int jc[MG_QMAX];
+#pragma unroll
+    for (int q = 0; q < MG_QMAX; q++) { acc[q] = 0.0; jc[q] = -1; }
+    for (int t = t0; t < t1; t++) {
+        const int m = mem[t];
+        const int c = op.cnt(m);
+        for (int kb = -1; kb < c; kb += SLICE) {
+            const int k = kb + lane;
+            const int nv = min(SLICE, c - kb);
+            if (lane < nv) {
+                int J;
+                double v;
+                if (k < 0) { J = I; v = (double)op.dg(m); }
+                else { J = agg[op.col(m, k)]; v = -(double)op.val(m, k); }
+                st_slot[lane] = J == I ? sI : slot_of(J);
+                st_col[lane] = J;
+                st_val[lane] = v;
+            }
+            __syncthreads();
+            for (int u = 0; u < nv; u++) {
+                const int s = st_slot[u];
+                if ((s & (SLICE - 1)) == lane) {
+                    const int qs = s >> 6;
+#pragma unroll
+                    for (int q = 0; q < MG_QMAX; q++)
+                        if (q == qs) { acc[q] += st_val[u]; jc[q] = st_col[u]; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const long long base = prow(sl_off1, I);
+#pragma unroll
+    for (int q = 0; q < MG_QMAX; q++) {
+        const int s = q * SLICE + lane;
+        if (s >= total) continue;
+        if (s == sI) {
+            diag1[I] = (float)acc[q];
+        } else {
+            const int o = s - (s > sI ? 1 : 0);
+            cols1[base + pslot(o)] = jc[q];
+            vals1[base + pslot(o)] = (float)(-acc[q]);
+        }
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_width(int n, int nslices, const int* cnt, long long* w) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslices) return;
+    int m = 0;
+    for (int r = s * SLICE; r < min(n, (s + 1) * SLICE); r++) m = max(m, cnt[r]);
+    w[s] = (long long)m * SLICE;
+}
+
+// coarsest level: dense copy, Gauss-Jordan inversion without pivoting
+// (the Galerkin operators of the diagonally dominant PPE keep nonzero
+// pivots; a zero or non-finite pivot sets err and the preconditioner is
+// not used), float copy of the inverse
+__global__ void __launch_bounds__(BLOCK) k_mg_dense_zero(long long n, double* a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_dense_fill(CoarseOp op, int n, double* a) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    a[(long long)r * n + r] = (double)op.dg(r);
+    const int c = op.cnt(r);
+    for (int k = 0; k < c; k++) a[(long long)r * n + op.col(r, k)] = -(double)op.val(r, k);
+}
+__global__ void __launch_bounds__(BLOCK) k_gj_pivot(int n, const double* a, int k, double* rowk, double* colk,
+                                                    int* err) {
+    const double piv = a[(long long)k * n + k];
+    const bool bad = !(fabs(piv) > 0.0) || !(fabs(piv) <= DBL_MAX);
+    if (bad) {
+        if (threadIdx.x == 0) err[0] = 3;
+        return;
+    }
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        rowk[j] = (j == k ? 1.0 : a[(long long)k * n + j]) / piv;
+        colk[j] = a[(long long)j * n + k];
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_gj_elim(int n, double* a, int k, const double* rowk, const double* colk) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * n) return;
+    const int i = (int)(idx / n), j = (int)(idx % n);
+    if (i == k) a[idx] = rowk[j];
+    else a[idx] = (j == k ? 0.0 : a[idx]) - colk[i] * rowk[j];
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_tofloat(long long n, const double* a, float* b) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+// ------------------------------------------------------------------------
+// V-cycle kernels
+#define MG_GATE(done) if (*(done) != 0.0) return
+
+__device__ __forceinline__ double coarse_row(const CoarseOp& o, int r, const float* x) {
+    const long long base = prow(o.sl_off, r);
+    const int m = o.cnt_[r];
+    double s = 0.0;
+    for (int k = 0; k < m; k++) s += (double)o.vals[base + pslot(k)] * (double)x[o.cols[base + pslot(k)]];
+    return (double)o.diag[r] * (double)x[r] - s;
+}
+
+// x = w b / diag
+__global__ void __launch_bounds__(BLOCK) k_mg_pre(int n, const double* done, const float* b, const float* diag,
+                                                  float* x) {
+    MG_GATE(done);
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) x[r] = (float)(MG_OMEGA * (double)b[r] / (double)diag[r]);
+}
+// r = b - A0 x (level 0: the PPE)
+__global__ void __launch_bounds__(BLOCK) k_mg_res0(PPEView pp, const double* done, const float* b, const float* x,
+                                                   float* r) {
+    __shared__ int wbase[BLOCK];
+    MG_GATE(done);
+    const int first = blockIdx.x * BLOCK;
+    load_group_bases(pp, first, wbase);
+    const int row = first + (int)threadIdx.x;
+    if (row < pp.nr) r[row] = (float)((double)b[row] - spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, x, wbase));
+}
+// out = x + w (b - A0 x) / diag
+__global__ void __launch_bounds__(BLOCK) k_mg_post0(PPEView pp, const double* done, const float* b, const float* x,
+                                                    float* out) {
+    __shared__ int wbase[BLOCK];
+    MG_GATE(done);
+    const int first = blockIdx.x * BLOCK;
+    load_group_bases(pp, first, wbase);
+    const int row = first + (int)threadIdx.x;
+    if (row < pp.nr) {
+        const double ax = spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, x, wbase);
+        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] - ax) / (double)pp.fdiag[row]);
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_resc(CoarseOp op, int n, const double* done, const float* b,
+                                                   const float* x, float* r) {
+    MG_GATE(done);
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < n) r[row] = (float)((double)b[row] - coarse_row(op, row, x));
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_postc(CoarseOp op, int n, const double* done, const float* b,
+                                                    const float* x, float* out) {
+    MG_GATE(done);
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row < n) {
+        const double ax = coarse_row(op, row, x);
+        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] - ax) / (double)op.diag[row]);
+    }
+}
+// b_next[I] = sum of r over the members of I, in member order
+__global__ void __launch_bounds__(BLOCK) k_mg_restrict(int n1, const double* done, const int* mstart, const int* mem,
+                                                       const float* r, float* b1) {
+    MG_GATE(done);
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n1) return;
+    double s = 0.0;
+    for (int t = mstart[I]; t < mstart[I + 1]; t++) s += (double)r[mem[t]];
+    b1[I] = (float)s;
+}
+// x += s z_next[agg]
+__global__ void __launch_bounds__(BLOCK) k_mg_prolong(int n, const double* done, const int* agg, const float* z1,
+                                                      float* x) {
+    MG_GATE(done);
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) x[r] = (float)((double)x[r] + MG_SCALE * (double)z1[agg[r]]);
+}
+// x = Ainv b, one wave per row, lanes over the columns, fixed shuffle tree
+__global__ void __launch_bounds__(BLOCK) k_mg_dense(int n, const double* done, const float* ainv, const float* b,
+                                                    float* x) {
+    MG_GATE(done);
+    const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float* a = ainv + (long long)row * n;
+    double s = 0.0;
+    for (int j = lane; j < n; j += SLICE) s += (double)a[j] * (double)b[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) x[row] = (float)s;
+}
+
+// ------------------------------------------------------------------------
+// host side
+void mg_free(sph_ctx* c, MGState* mg) {
+    if (!mg) return;
+    for (int l = 0; l < MG_MAX_LEVELS; l++) {
+        MGLevel& L = mg->L[l];
+        void* ptrs[] = {L.cx, L.cy, L.cz, L.sl_off, L.cnt, L.cols, L.vals, L.diag, L.agg, L.mstart, L.mem, L.b, L.x, L.y, L.r};
+        for (void* q : ptrs) dfree(c, q);
+    }
+    dfree(c, mg->ainv);
+    delete mg;
+}
+
+// set-up launches, each its own profile record (records do not nest)
+#define MGK(name, grid, block, kern, ...)                                                  \
+    do {                                                                                   \
+        SPH_TRY(prof_begin(c, name, 0.0));                                                 \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, c->stream, ##__VA_ARGS__);    \
+        SPH_TRY(launch_check(c, name));                                                    \
+        SPH_TRY(prof_end(c));                                                              \
+    } while (0)
+
+// Galerkin operator of level l + 1 from level l (src), whose link arrays
+// are set.  Returns 0 with *ok = 0 when a row is out of the kernel's bounds.
+template <class OP>
+static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0, int* err, int* ok) {
+    const int n1 = L1.n;
+    SPH_TRY(dalloc(c, &L1.cnt, n1));
+    MGK("mg_rap_count", n1, SLICE, (k_mg_rap<false, OP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
+        (const int*)L0.mem, L1.cnt, (const long long*)nullptr, (int*)nullptr, (float*)nullptr, (float*)nullptr, err);
+    int h_err = 0;
+    SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    if (h_err) { *ok = 0; return 0; }
+    L1.nslices = (n1 + SLICE - 1) / SLICE;
+    long long* w = nullptr;
+    SPH_TRY(dalloc(c, &w, (size_t)L1.nslices + 1));
+    SPH_TRY(dalloc(c, &L1.sl_off, (size_t)L1.nslices + 1));
+    MGK("mg_width", grid1(L1.nslices), BLOCK, k_mg_width, n1, L1.nslices, (const int*)L1.cnt, w);
+    SPH_TRY(exclusive_scan_i64(c, w, L1.sl_off, L1.nslices, &L1.slots));
+    dfree(c, w);
+    SPH_TRY(dalloc(c, &L1.cols, (size_t)std::max(1LL, L1.slots)));
+    SPH_TRY(dalloc(c, &L1.vals, (size_t)std::max(1LL, L1.slots)));
+    SPH_TRY(dalloc(c, &L1.diag, n1));
+    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<true, OP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
+        (const int*)L0.mem, L1.cnt, (const long long*)L1.sl_off, L1.cols, L1.vals, L1.diag, err);
+    SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    *ok = h_err == 0;
+    return 0;
+}
+
+// Aggregates of level l (its coordinates set) -> level l + 1 rows,
+// coordinates and link arrays.
+static int mg_aggregate(sph_ctx* c, MGLevel& L0, MGLevel& L1) {
+    const int n = L0.n;
+    L1.nx = (L0.nx >> 1) + 1;
+    L1.ny = (L0.ny >> 1) + 1;
+    L1.nz = (L0.nz >> 1) + 1;
+    int *key = nullptr, *val = nullptr, *ktmp = nullptr, *vtmp = nullptr, *flag = nullptr, *scan = nullptr;
+    SPH_TRY(dalloc(c, &key, n));
+    SPH_TRY(dalloc(c, &val, n));
+    SPH_TRY(dalloc(c, &ktmp, n));
+    SPH_TRY(dalloc(c, &vtmp, n));
+    MGK("mg_key", grid1(n), BLOCK, k_mg_key, n, (const int*)L0.cx, (const int*)L0.cy, (const int*)L0.cz, L1.nx, L1.ny,
+        key, val);
+    SPH_TRY(radix_sort_pairs(c, &key, &val, &ktmp, &vtmp, n, (int)(L1.nx * L1.ny * L1.nz - 1)));
+    dfree(c, ktmp);
+    dfree(c, vtmp);
+    SPH_TRY(dalloc(c, &flag, n));
+    SPH_TRY(dalloc(c, &scan, n));
+    MGK("mg_flag", grid1(n), BLOCK, k_mg_flag, n, (const int*)key, flag);
+    int n1 = 0;
+    SPH_TRY(exclusive_scan_i32(c, flag, scan, n, &n1));
+    L1.n = n1;
+    SPH_TRY(dalloc(c, &L0.agg, n));
+    SPH_TRY(dalloc(c, &L0.mem, n));
+    SPH_TRY(dalloc(c, &L0.mstart, (size_t)n1 + 1));
+    SPH_TRY(dalloc(c, &L1.cx, n1));
+    SPH_TRY(dalloc(c, &L1.cy, n1));
+    SPH_TRY(dalloc(c, &L1.cz, n1));
+    MGK("mg_link", grid1(n), BLOCK, k_mg_link, n, (const int*)key, (const int*)val, (const int*)flag,
+        (const int*)scan, L1.nx, L1.ny, L0.agg, L0.mem, L0.mstart, L1.cx, L1.cy, L1.cz);
+    dfree(c, key);
+    dfree(c, val);
+    dfree(c, flag);
+    dfree(c, scan);
+    return 0;
+}
+
+static int mg_vectors(sph_ctx* c, MGLevel& L, bool fine, bool coarsest) {
+    if (!fine) SPH_TRY(dalloc(c, &L.b, L.n));
+    SPH_TRY(dalloc(c, &L.x, L.n));
+    if (!coarsest) SPH_TRY(dalloc(c, &L.r, L.n));
+    if (!fine && !coarsest) SPH_TRY(dalloc(c, &L.y, L.n));
+    return 0;
+}
+
+int mg_build(sph_ctx* c, const PPEView& pp, double q, MGState** out, int* ok) {
+    *out = nullptr;
+    *ok = 0;
+    if (pp.nr <= MG_COARSE_MAX || pp.ltail || !pp.vals32 || !pp.fdiag || !(q > 0)) return 0;
+    const sph_domain& d = c->dom;
+    const double ex = ((double)d.x_max - d.x_min) / q + 3.0, ey = ((double)d.y_max - d.y_min) / q + 3.0,
+                 ez = ((double)d.z_max - d.z_min) / q + 3.0;
+    // level-1 keys are 32-bit
+    if ((ex / 2 + 1) * (ey / 2 + 1) * (ez / 2 + 1) > 2.0e9) return 0;
+    MGState* mg = new MGState();
+    int* err = nullptr;
+    int rc = 0;
+    auto fail_out = [&](int code) {
+        dfree(c, err);
+        mg_free(c, mg);
+        return code;
+    };
+#define MGTRY(x)                          \
+    do {                                  \
+        int rc_ = (x);                    \
+        if (rc_) return fail_out(rc_);    \
+    } while (0)
+    MGTRY(dalloc(c, &err, 1));
+    {
+        MGLevel& L0 = mg->L[0];
+        L0.n = pp.nr;
+        L0.nx = (long long)ex;
+        L0.ny = (long long)ey;
+        L0.nz = (long long)ez;
+        MGTRY(dalloc(c, &L0.cx, L0.n));
+        MGTRY(dalloc(c, &L0.cy, L0.n));
+        MGTRY(dalloc(c, &L0.cz, L0.n));
+        const double x0 = (double)d.x_min - q, y0 = (double)d.y_min - q, z0 = (double)d.z_min - q;
+        hipLaunchKernelGGL(k_mg_coord0, dim3(grid1(L0.n)), dim3(BLOCK), 0, c->stream, L0.n, pp.rows, c->sview(), x0,
+                           y0, z0, q, L0.nx, L0.ny, L0.nz, L0.cx, L0.cy, L0.cz);
+        MGTRY(launch_check(c, "mg_coord"));
+        mg->fine_nnz = 0;
+    }
+    int nlev = 1;
+    bool good = true;
+    while (mg->L[nlev - 1].n > MG_COARSE_MAX) {
+        if (nlev == MG_MAX_LEVELS) { good = false; break; }
+        MGLevel& L0 = mg->L[nlev - 1];
+        MGLevel& L1 = mg->L[nlev];
+        SPH_HIP(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        MGTRY(mg_aggregate(c, L0, L1));
+        if (L1.n * 10LL > L0.n * 9LL) { nlev++; good = false; break; }   // coarsening stalled
+        int lok = 0;
+        if (nlev == 1) MGTRY(mg_galerkin(c, FineOp{pp}, L1, L0, err, &lok));
+        else MGTRY(mg_galerkin(c, coarse_op(L0), L1, L0, err, &lok));
+        nlev++;
+        if (!lok) { good = false; break; }
+    }
+    mg->nlev = nlev;
+    if (!good || nlev < 2) return fail_out(0);
+    (void)rc;
+    // dense inverse of the coarsest level
+    {
+        MGLevel& LC = mg->L[nlev - 1];
+        const int n = LC.n;
+        const long long nn = (long long)n * n;
+        double *a = nullptr, *rowk = nullptr, *colk = nullptr;
+        MGTRY(dalloc(c, &a, (size_t)nn));
+        MGTRY(dalloc(c, &rowk, n));
+        MGTRY(dalloc(c, &colk, n));
+        SPH_HIP(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_mg_dense_zero, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, nn, a);
+        MGTRY(launch_check(c, "mg_dense_zero"));
+        hipLaunchKernelGGL(k_mg_dense_fill, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, coarse_op(LC), n, a);
+        MGTRY(launch_check(c, "mg_dense_fill"));
+        MGTRY(prof_begin(c, "mg_gauss_jordan", 0.0));
+        for (int k = 0; k < n; k++) {
+            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(BLOCK), 0, c->stream, n, (const double*)a, k, rowk, colk, err);
+            hipLaunchKernelGGL(k_gj_elim, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, n, a, k, (const double*)rowk,
+                               (const double*)colk);
+        }
+        MGTRY(launch_check(c, "mg_gauss_jordan"));
+        MGTRY(prof_end(c));
+        MGTRY(dalloc(c, &mg->ainv, (size_t)nn));
+        hipLaunchKernelGGL(k_mg_tofloat, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, nn, (const double*)a, mg->ainv);
+        MGTRY(launch_check(c, "mg_tofloat"));
+        int h_err = 0;
+        SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(c, a);
+        dfree(c, rowk);
+        dfree(c, colk);
+        if (h_err) return fail_out(0);
+    }
+    for (int l = 0; l < nlev; l++) MGTRY(mg_vectors(c, mg->L[l], l == 0, l == nlev - 1));
+    // coordinates are needed only while building
+    for (int l = 0; l < nlev; l++) {
+        MGLevel& L = mg->L[l];
+        dfree(c, L.cx); dfree(c, L.cy); dfree(c, L.cz);
+        L.cx = L.cy = L.cz = nullptr;
+    }
+    dfree(c, err);
+#undef MGTRY
+    *out = mg;
+    *ok = 1;
+    return 0;
+}
+
+// z = M^-1 b on the case's stream (e.st), skipped on the device once *done
+// is set.  spmv_bytes: the compulsory bytes of one product with the PPE (for
+// the profile of the two level-0 passes).
+int mg_apply(Eng& e, const MGState* mg, const PPEView& pp, const double* done, const float* b, float* z,
+             double spmv_bytes) {
+    const int nl = mg->nlev;
+    auto res = [&](int l) -> const float* { return l == nl - 1 ? mg->L[l].x : mg->L[l].y; };
+#define MGE(name, bytes, grid, kern, ...)                                                     \
+    do {                                                                                      \
+        SPH_TRY(prof_begin(e, name, (double)(bytes)));                                        \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, e.st, ##__VA_ARGS__);            \
+        SPH_TRY(launch_check(&e, name));                                                      \
+        SPH_TRY(prof_end(e));                                                                 \
+    } while (0)
+    const MGLevel& F = mg->L[0];
+    const int nr = F.n;
+    // down
+    MGE("mg_pre", 12.0 * nr, grid1(nr), k_mg_pre, nr, done, b, (const float*)pp.fdiag, F.x);
+    MGE("mg_res0", spmv_bytes, grid1(nr), k_mg_res0, pp, done, b, (const float*)F.x, F.r);
+    MGE("mg_restrict", 12.0 * nr, grid1(mg->L[1].n), k_mg_restrict, mg->L[1].n, done, (const int*)F.mstart,
+        (const int*)F.mem, (const float*)F.r, mg->L[1].b);
+    for (int l = 1; l < nl - 1; l++) {
+        const MGLevel& L = mg->L[l];
+        const MGLevel& N = mg->L[l + 1];
+        const CoarseOp op = coarse_op(L);
+        MGE("mg_pre", 12.0 * L.n, grid1(L.n), k_mg_pre, L.n, done, (const float*)L.b, (const float*)L.diag, L.x);
+        MGE("mg_res", 0, grid1(L.n), k_mg_resc, op, L.n, done, (const float*)L.b, (const float*)L.x, L.r);
+        MGE("mg_restrict", 12.0 * L.n, grid1(N.n), k_mg_restrict, N.n, done, (const int*)L.mstart, (const int*)L.mem,
+            (const float*)L.r, N.b);
+    }
+    const MGLevel& C = mg->L[nl - 1];
+    MGE("mg_dense", 4.0 * C.n * C.n, grid1((long long)C.n * SLICE), k_mg_dense, C.n, done, (const float*)mg->ainv,
+        (const float*)C.b, C.x);
+    // up
+    for (int l = nl - 2; l >= 1; l--) {
+        const MGLevel& L = mg->L[l];
+        const CoarseOp op = coarse_op(L);
+        MGE("mg_prolong", 16.0 * L.n, grid1(L.n), k_mg_prolong, L.n, done, (const int*)L.agg, res(l + 1), L.x);
+        MGE("mg_post", 0, grid1(L.n), k_mg_postc, op, L.n, done, (const float*)L.b, (const float*)L.x, L.y);
+    }
+    MGE("mg_prolong", 16.0 * nr, grid1(nr), k_mg_prolong, nr, done, (const int*)F.agg, res(1), F.x);
+    MGE("mg_post0", spmv_bytes, grid1(nr), k_mg_post0, pp, done, b, (const float*)F.x, z);
+#undef MGE
+    return 0;
+}
+
+}  // namespace sph

@@ -42,6 +42,7 @@
 #include "flags.h"
 #include "sph_internal.h"
 #include "sph_math.h"
+#include "ppe_format.h"
 
 namespace sph {
 
@@ -49,21 +50,6 @@ enum Scal {
     SC_RHO, SC_RHO_OLD, SC_ALPHA, SC_OMEGA, SC_BETA, SC_BNORM, SC_RNORM, SC_SNORM, SC_TOL,
     SC_DONE, SC_CONV, SC_HALF, SC_IT, SC_ITERS, SC_MAXIT, SC_ZEROX, SC_COUNT = SCAL_COUNT
 };
-
-// 16-bit column entries: 10 offset bits, up to 64 window groups per slice
-constexpr int EC_OFF_BITS = 10;
-constexpr int EC_OFF_MASK = (1 << EC_OFF_BITS) - 1;
-constexpr int EC_MAX_GROUPS = 64;
-constexpr int EC_MAX_WORDS = (65536 * 256 >> 10) >> 5;   // window bitmap words for 16.7M unknowns
-
-// Matrix layout: sliced ELLPACK over 64-row slices, one wave64 per slice,
-// entries column-major within a slice: entry k of row r lives at
-//     sl_off[r / 64] + k * 64 + r % 64,
-// so the wave's loads of one slot are one contiguous block.
-__host__ __device__ __forceinline__ long long prow(const long long* sl_off, int r) {
-    return sl_off[r >> 6] + (r & 63);
-}
-__host__ __device__ __forceinline__ long long pslot(int k) { return (long long)k * SLICE; }
 
 struct PPEState {
     int nr = 0;                 // unknowns
@@ -102,6 +88,10 @@ struct PPEState {
     int nlong = 0;
     long long tail_total = 0;
     double* lsum = nullptr;
+    // MIXED multigrid preconditioner of the current values: 0 not built,
+    // 1 built (mg), -1 not usable for this matrix (Jacobi)
+    MGState* mg = nullptr;
+    int mg_state = 0;
 };
 
 // ------------------------------------------------------------------------
@@ -524,87 +514,6 @@ __global__ void __launch_bounds__(BLOCK) k_ppe_scatter(Seg sg, int jacobi) {
 #define GATE(cd) if ((cd).sc[SC_DONE] != 0.0) return
 #define SKIPPED(cd) ((cd).ctl[CTL_SKIP] != 0.0)
 
-// Off-diagonal part of one row, sum_k vals[k] * x[col[k]], accumulated in
-// double.  Matrix entries stream through once: nontemporal loads keep the
-// caches for the x gathers; four independent loads in flight per lane.
-// Off-diagonal part of one row, sum_k vals[k] * x[col[k]], accumulated in
-// double in entry order.  Matrix entries stream through once: nontemporal
-// loads keep the caches for the x gathers; four independent loads in flight
-// per lane.
-template <class V, class X>
-__device__ __forceinline__ double offdiag_i32(const int* cp, const V* vp, int m, const X* x) {
-    double s = 0.0;
-    int k = 0;
-    for (; k + 4 <= m; k += 4) {
-        const int c0 = __builtin_nontemporal_load(cp + pslot(k + 0));
-        const int c1 = __builtin_nontemporal_load(cp + pslot(k + 1));
-        const int c2 = __builtin_nontemporal_load(cp + pslot(k + 2));
-        const int c3 = __builtin_nontemporal_load(cp + pslot(k + 3));
-        const V v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
-        const V v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
-        const V v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
-        const V v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
-        const X x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-        s += (double)v0 * (double)x0;
-        s += (double)v1 * (double)x1;
-        s += (double)v2 * (double)x2;
-        s += (double)v3 * (double)x3;
-    }
-    for (; k < m; k++) s += (double)__builtin_nontemporal_load(vp + pslot(k)) * (double)x[__builtin_nontemporal_load(cp + pslot(k))];
-    return s;
-}
-// The same over 16-bit columns: entry e = (group << EC_OFF_BITS) | offset,
-// column = wb[group] + offset, wb = the slice's group bases (in LDS).
-__device__ __forceinline__ int ecol(const int* wb, int e) { return wb[e >> EC_OFF_BITS] + (e & EC_OFF_MASK); }
-template <class V, class X>
-__device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const V* vp, int m, const int* wb,
-                                              const X* x) {
-    double s = 0.0;
-    int k = 0;
-    for (; k + 4 <= m; k += 4) {
-        const int e0 = __builtin_nontemporal_load(ep + pslot(k + 0));
-        const int e1 = __builtin_nontemporal_load(ep + pslot(k + 1));
-        const int e2 = __builtin_nontemporal_load(ep + pslot(k + 2));
-        const int e3 = __builtin_nontemporal_load(ep + pslot(k + 3));
-        const V v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
-        const V v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
-        const V v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
-        const V v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
-        const X x0 = x[ecol(wb, e0)], x1 = x[ecol(wb, e1)], x2 = x[ecol(wb, e2)], x3 = x[ecol(wb, e3)];
-        s += (double)v0 * (double)x0;
-        s += (double)v1 * (double)x1;
-        s += (double)v2 * (double)x2;
-        s += (double)v3 * (double)x3;
-    }
-    for (; k < m; k++)
-        s += (double)__builtin_nontemporal_load(vp + pslot(k)) * (double)x[ecol(wb, __builtin_nontemporal_load(ep + pslot(k)))];
-    return s;
-}
-
-// Group bases of the block's slices into LDS (every thread of the block
-// takes part; call before the row guard).
-__device__ __forceinline__ void load_group_bases(const PPEView& pp, int first, int* wbase) {
-    if (pp.ecols) wbase[threadIdx.x] = pp.gbase[first + (int)threadIdx.x];
-    __syncthreads();
-}
-
-// y_r = diag_r x_r - offdiag_r(x); dg = the diagonal of this operator
-template <class V, class X, class DG>
-__device__ __forceinline__ double spmv_row(const PPEView& pp, int r, const V* vals, const DG* dg, const X* x,
-                                           const int* wbase) {
-    const long long base = prow(pp.sl_off, r);
-    const int m = pp.cnt[r];
-    double s;
-    if (pp.ecols) s = offdiag_u16<V, X>(pp.ecols + base, vals + base, m, wbase + (threadIdx.x & ~63), x);
-    else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
-    if (pp.ltail) {
-        const int u = pp.ltail[r];
-        if (u >= 0) s += pp.lsum[u];
-    }
-    return (double)dg[r] * (double)x[r] - s;
-}
-
-// r = rhs - A x ; rhat = r ; partials rhs.rhs (A) and r.r (B)
 // Tail sums of the long rows, one 64-lane wave per row: lane l takes tail
 // entries l, l + 64, ... (contiguous, so every load is coalesced) and a
 // fixed shuffle tree adds the lanes.  WHICH picks the vector the next
@@ -635,6 +544,7 @@ __global__ void __launch_bounds__(BLOCK) k_kr_long(Seg sg) {
     if (lane == 0) pp.lsum[u] = s;
 }
 
+// r = rhs - A x ; rhat = r ; partials rhs.rhs (A) and r.r (B)
 template <class V, class X>
 __global__ void __launch_bounds__(BLOCK) k_kr_init(Seg sg) {
     __shared__ double sh[2 * BLOCK];
@@ -889,6 +799,7 @@ void ppe_free(sph_ctx* c) {
                     P->diag, P->fdiag, P->rhs, P->x, P->pnew, P->r, P->rhat, P->p, P->phat, P->v, P->s, P->shat, P->t, P->fr, P->frhat, P->fp,
                     P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->partA, P->partB, P->ltail, P->lrows, P->tail_off, P->tail_cols, P->tail_vals, P->lsum};
     for (void* q : ptrs) dfree(c, q);
+    mg_free(c, P->mg);
     delete P;
     c->ppe = nullptr;
     c->generation++;
@@ -897,16 +808,21 @@ void ppe_free(sph_ctx* c) {
 // Renumbers the rows by lattice position (see k_lattice_key).  The lattice
 // spacing is taken as h / 1.5 of the first row's particle; keys are
 // rounded, so jitter below half a spacing leaves the order intact.
-static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
-    const int nr = P->nr;
-    PView S = c->sview();
+static int lattice_spacing(sph_ctx* c, const PPEState* P, double* q) {
     int r0 = 0;
     double h0 = 0.0;
     SPH_HIP(c, hipMemcpyAsync(&r0, P->rows, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
-    SPH_HIP(c, hipMemcpyAsync(&h0, S.h + r0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipMemcpyAsync(&h0, c->sview().h + r0, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
-    const double q = h0 / 1.5;
+    *q = h0 / 1.5;
+    return 0;
+}
+static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
+    const int nr = P->nr;
+    PView S = c->sview();
+    double q = 0.0;
+    SPH_TRY(lattice_spacing(c, P, &q));
     const sph_domain& d = c->dom;
     const double ex = ((double)d.x_max - d.x_min) / q + 3.0, ey = ((double)d.y_max - d.y_min) / q + 3.0,
                  ez = ((double)d.z_max - d.z_min) / q + 3.0;
@@ -1206,32 +1122,70 @@ static int ctl_upload(Eng& e) {
     } while (0)
 
 // one BiCGSTAB iteration of every unfinished case
+// true while some case of the step still iterates (half: or ends with the
+// half-step update of x), from the host copy of the scalars
+static bool any_live(const Eng& e, bool half) {
+    for (size_t k = 0; k < e.cs.size(); k++) {
+        if (e.h_ctl[k * CTL_COUNT + CTL_SKIP] != 0.0) continue;
+        const double* sc = e.h_sc + k * SC_COUNT;
+        if (sc[SC_DONE] == 0.0 || (half && sc[SC_HALF] == 1.0)) return true;
+    }
+    return false;
+}
+
+// One BiCGSTAB iteration of every unfinished case.  mg: the multigrid
+// preconditioner of the single case of the step (float vectors), which
+// replaces the diagonal scaling of p-hat and s-hat.  With the profile on
+// (launches not captured) the host reads the scalars after every finalize
+// and leaves out the launches whose device gate would return at once, so
+// per-launch times count only launches that do work; the results are the
+// same.
 template <class V, class X>
-static int kr_iteration(Eng& e, double spmv_bytes) {
-    RL("kr_p", 0, k_kr_p<X>);
-    RLONG(V, X, 1);
-    RL("spmv", spmv_bytes, (k_kr_spmv<V, X, false>));
-    KF(FIN_ALPHA);
-    RL("kr_s", 0, k_kr_s<X>);
-    KF(FIN_S);
-    RLONG(V, X, 2);
-    RL("spmv", spmv_bytes, (k_kr_spmv<V, X, true>));
-    KF(FIN_OMEGA);
-    RL("kr_xr", 0, k_kr_xr<X>);
-    KF(FIN_R);
+static int kr_iteration(Eng& e, double spmv_bytes, const MGState* mg) {
+    const bool pf = e.prof.on;
+    auto live = [&](bool half) { return !pf || any_live(e, half); };
+#define KFP(op)                                   \
+    do {                                          \
+        KF(op);                                   \
+        if (pf) SPH_TRY(read_scalars(e));         \
+    } while (0)
+    if (live(false)) {
+        RL("kr_p", 0, k_kr_p<X>);
+        if (mg) {
+            const PPEView pv = ppe_view(e.cs[0]);
+            SPH_TRY(mg_apply(e, mg, pv, e.d_sc + SC_DONE, pv.fp, pv.fphat, spmv_bytes));
+        }
+        RLONG(V, X, 1);
+        RL("spmv", spmv_bytes, (k_kr_spmv<V, X, false>));
+    }
+    KFP(FIN_ALPHA);
+    if (live(false)) RL("kr_s", 0, k_kr_s<X>);
+    KFP(FIN_S);
+    if (live(false)) {
+        if (mg) {
+            const PPEView pv = ppe_view(e.cs[0]);
+            SPH_TRY(mg_apply(e, mg, pv, e.d_sc + SC_DONE, pv.fs, pv.fshat, spmv_bytes));
+        }
+        RLONG(V, X, 2);
+        RL("spmv", spmv_bytes, (k_kr_spmv<V, X, true>));
+    }
+    KFP(FIN_OMEGA);
+    if (live(true)) RL("kr_xr", 0, k_kr_xr<X>);
+    KFP(FIN_R);
+#undef KFP
     return 0;
 }
 
 // The iteration as a HIP graph, captured once per composition of the step
 // (its launches read everything else from the device descriptors).
 template <class V, class X>
-static int kr_graph(Eng& e, int gi, double spmv_bytes, hipGraphExec_t* out) {
+static int kr_graph(Eng& e, int gi, double spmv_bytes, const MGState* mg, hipGraphExec_t* out) {
     if (!e.iter_graph[gi]) {
         Shape sh;
         SPH_TRY(shape(e, 1, BLOCK, &sh));   // segment table exists before the capture
         hipGraph_t g = nullptr;
         SPH_HIP(&e, hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
-        int rc = kr_iteration<V, X>(e, spmv_bytes);
+        int rc = kr_iteration<V, X>(e, spmv_bytes, mg);
         hipError_t er = hipStreamEndCapture(e.st, &g);
         if (rc) {
             if (er == hipSuccess && g) (void)hipGraphDestroy(g);
@@ -1252,7 +1206,7 @@ static int kr_graph(Eng& e, int gi, double spmv_bytes, hipGraphExec_t* out) {
 // with the tolerances and iteration caps already in the controls.
 // iters[k] receives case k's iteration count.
 template <class V, class X>
-static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters) {
+static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, const MGState* mg = nullptr) {
     const int nc = (int)e.cs.size();
     double spmv_bytes = 0.0;
     int maxit = 0;
@@ -1291,10 +1245,10 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters) {
         batch = std::min(batch, maxit - launched);
         if (graphs) {
             hipGraphExec_t g = nullptr;
-            SPH_TRY((kr_graph<V, X>(e, gi, spmv_bytes, &g)));
+            SPH_TRY((kr_graph<V, X>(e, gi, spmv_bytes, mg, &g)));
             for (int b = 0; b < batch; b++) SPH_HIP(&e, hipGraphLaunch(g, e.st));
         } else {
-            for (int b = 0; b < batch; b++) SPH_TRY((kr_iteration<V, X>(e, spmv_bytes)));
+            for (int b = 0; b < batch; b++) SPH_TRY((kr_iteration<V, X>(e, spmv_bytes, mg)));
         }
         launched += batch;
         SPH_TRY(read_scalars(e));
@@ -1335,6 +1289,7 @@ int ppe_solve(Eng& e) {
         c->rep.slots = c->ppe ? c->ppe->nnz_slots : 0;
         c->rep.index16 = c->ppe && c->ppe->ecols ? 1 : 0;
         c->rep.tail_rows = c->ppe ? c->ppe->nlong : 0;
+        c->rep.precond = SPH_PRECOND_JACOBI;
         if (c->rep.unknowns == 0) c->rep.converged = 1;
         rows_total += c->rep.unknowns;
         asm_bytes += (double)c->rep.unknowns * (15.0 * 8) + (double)c->rep.nnz * (4.0 + (mixed ? 4.0 : 8.0));
@@ -1386,6 +1341,32 @@ int ppe_solve(Eng& e) {
             c->rep.rel_residual = rel_residual(sc[SC_RNORM], sc[SC_BNORM]);
         }
     } else {
+        // the multigrid preconditioner, for a single case that asks for it;
+        // its hierarchy is built once per set of values (with fixed
+        // particles, at the first step of a search)
+        const MGState* mg = nullptr;
+        if (nc == 1 && e.cs[0]->prm.ppe_precond == SPH_PRECOND_MULTIGRID && e.cs[0]->ppe) {
+            sph_ctx* c = e.cs[0];
+            PPEState* P = c->ppe;
+            const bool fresh = !c->prm.ale_positions && c->vals_gen == c->generation;
+            if (!fresh || P->mg_state == 0) {
+                mg_free(c, P->mg);
+                P->mg = nullptr;
+                P->mg_state = -1;
+                if (e.iter_graph[2]) {   // its launches hold the old hierarchy
+                    (void)hipGraphExecDestroy(e.iter_graph[2]);
+                    e.iter_graph[2] = nullptr;
+                }
+                if (P->nr > MG_COARSE_MAX && P->nlong == 0) {
+                    double q = 0.0;
+                    int ok = 0;
+                    SPH_TRY(lattice_spacing(c, P, &q));
+                    SPH_TRY(mg_build(c, ppe_view(c), q, &P->mg, &ok));
+                    P->mg_state = ok ? 1 : -1;
+                }
+            }
+            if (P->mg_state == 1) mg = P->mg;
+        }
         // outer FP64 iterative refinement around float inner solves; each
         // case refines until its own tolerance is met
         std::vector<char> active(nc);
@@ -1429,9 +1410,23 @@ int ppe_solve(Eng& e) {
             SPH_TRY(ctl_upload(e));
             if (!any) break;
             RL("zero_fx", 0, k_zero_fx);
-            SPH_TRY((bicgstab<float, float>(e, 1, outer, iters)));
+            SPH_TRY((bicgstab<float, float>(e, mg ? 2 : 1, outer, iters, mg)));
             for (int k = 0; k < nc; k++)
                 if (active[k]) { e.cs[k]->rep.iterations += iters[k]; refine[k]++; }
+            if (mg) {
+                sph_ctx* c = e.cs[0];
+                c->rep.precond = SPH_PRECOND_MULTIGRID;
+                c->rep.mg_levels = mg_levels(mg);
+                c->rep.mg_coarsest = mg_coarsest(mg);
+                // an inner solve that stalled (breakdown or the iteration
+                // cap) with finite residuals: Jacobi from here on, for the
+                // rest of this search
+                const double* sc = e.h_sc;
+                if (sc[SC_CONV] == 0.0 && std::isfinite(sc[SC_RNORM])) {
+                    c->ppe->mg_state = -1;
+                    mg = nullptr;
+                }
+            }
             RL("add_corr", 0, k_add_correction);
         }
     }

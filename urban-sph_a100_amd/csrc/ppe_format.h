@@ -1,0 +1,100 @@
+// Sliced-ELLPACK storage of the PPE operator and its row product, shared
+// by the Krylov kernels and the multigrid preconditioner.
+#pragma once
+#include "sph_internal.h"
+
+namespace sph {
+
+// 16-bit column entries: 10 offset bits, up to 64 window groups per slice
+constexpr int EC_OFF_BITS = 10;
+constexpr int EC_OFF_MASK = (1 << EC_OFF_BITS) - 1;
+constexpr int EC_MAX_GROUPS = 64;
+constexpr int EC_MAX_WORDS = (65536 * 256 >> 10) >> 5;   // window bitmap words for 16.7M unknowns
+
+// Matrix layout: sliced ELLPACK over 64-row slices, one wave64 per slice,
+// entries column-major within a slice: entry k of row r lives at
+//     sl_off[r / 64] + k * 64 + r % 64,
+// so the wave's loads of one slot are one contiguous block.
+__host__ __device__ __forceinline__ long long prow(const long long* sl_off, int r) {
+    return sl_off[r >> 6] + (r & 63);
+}
+__host__ __device__ __forceinline__ long long pslot(int k) { return (long long)k * SLICE; }
+
+// Off-diagonal part of one row, sum_k vals[k] * x[col[k]], accumulated in
+// double in entry order.  Matrix entries stream through once: nontemporal
+// loads keep the caches for the x gathers; four independent loads in flight
+// per lane.
+template <class V, class X>
+__device__ __forceinline__ double offdiag_i32(const int* cp, const V* vp, int m, const X* x) {
+    double s = 0.0;
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+        const int c0 = __builtin_nontemporal_load(cp + pslot(k + 0));
+        const int c1 = __builtin_nontemporal_load(cp + pslot(k + 1));
+        const int c2 = __builtin_nontemporal_load(cp + pslot(k + 2));
+        const int c3 = __builtin_nontemporal_load(cp + pslot(k + 3));
+        const V v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
+        const V v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
+        const V v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
+        const V v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
+        const X x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        s += (double)v0 * (double)x0;
+        s += (double)v1 * (double)x1;
+        s += (double)v2 * (double)x2;
+        s += (double)v3 * (double)x3;
+    }
+    for (; k < m; k++) s += (double)__builtin_nontemporal_load(vp + pslot(k)) * (double)x[__builtin_nontemporal_load(cp + pslot(k))];
+    return s;
+}
+// The same over 16-bit columns: entry e = (group << EC_OFF_BITS) | offset,
+// column = wb[group] + offset, wb = the slice's group bases (in LDS).
+__device__ __forceinline__ int ecol(const int* wb, int e) { return wb[e >> EC_OFF_BITS] + (e & EC_OFF_MASK); }
+template <class V, class X>
+__device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const V* vp, int m, const int* wb,
+                                              const X* x) {
+    double s = 0.0;
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+        const int e0 = __builtin_nontemporal_load(ep + pslot(k + 0));
+        const int e1 = __builtin_nontemporal_load(ep + pslot(k + 1));
+        const int e2 = __builtin_nontemporal_load(ep + pslot(k + 2));
+        const int e3 = __builtin_nontemporal_load(ep + pslot(k + 3));
+        const V v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
+        const V v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
+        const V v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
+        const V v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
+        const X x0 = x[ecol(wb, e0)], x1 = x[ecol(wb, e1)], x2 = x[ecol(wb, e2)], x3 = x[ecol(wb, e3)];
+        s += (double)v0 * (double)x0;
+        s += (double)v1 * (double)x1;
+        s += (double)v2 * (double)x2;
+        s += (double)v3 * (double)x3;
+    }
+    for (; k < m; k++)
+        s += (double)__builtin_nontemporal_load(vp + pslot(k)) * (double)x[ecol(wb, __builtin_nontemporal_load(ep + pslot(k)))];
+    return s;
+}
+
+// Group bases of the block's slices into LDS (every thread of the block
+// takes part; call before the row guard).
+__device__ __forceinline__ void load_group_bases(const PPEView& pp, int first, int* wbase) {
+    if (pp.ecols) wbase[threadIdx.x] = pp.gbase[first + (int)threadIdx.x];
+    __syncthreads();
+}
+
+// y_r = diag_r x_r - offdiag_r(x); dg = the diagonal of this operator
+template <class V, class X, class DG>
+__device__ __forceinline__ double spmv_row(const PPEView& pp, int r, const V* vals, const DG* dg, const X* x,
+                                           const int* wbase) {
+    const long long base = prow(pp.sl_off, r);
+    const int m = pp.cnt[r];
+    double s;
+    if (pp.ecols) s = offdiag_u16<V, X>(pp.ecols + base, vals + base, m, wbase + (threadIdx.x & ~63), x);
+    else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
+    if (pp.ltail) {
+        const int u = pp.ltail[r];
+        if (u >= 0) s += pp.lsum[u];
+    }
+    return (double)dg[r] * (double)x[r] - s;
+}
+
+}  // namespace sph

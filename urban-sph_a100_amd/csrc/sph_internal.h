@@ -207,11 +207,11 @@ struct Eng {
     int tune_class = 0;
     // graph of one BiCGSTAB iteration per vector precision, valid for comp
     bool use_graphs = true;
-    hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t iter_graph[3] = {nullptr, nullptr, nullptr};   // FP64, float, float with multigrid
     // Krylov iterations of the last solves (per vector precision and
     // refinement pass): the host polls for convergence only near them
     static constexpr int PRED_SLOTS = 4;
-    int pred_iters[2][PRED_SLOTS] = {};
+    int pred_iters[3][PRED_SLOTS] = {};
 };
 
 }  // namespace sph
@@ -345,6 +345,18 @@ int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax);
 // ppe.hip
 int ppe_build(sph_ctx* c);
 PPEView ppe_view(const sph_ctx* c);
+// aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
+struct MGState;
+constexpr int MG_COARSE_MAX = 1024;   // unknowns of the densely inverted coarsest level
+// Builds the hierarchy of the PPE in pp (float values) with lattice spacing
+// q; *ok = 0 (and *out = null) when the case does not suit it.
+int mg_build(sph_ctx* c, const PPEView& pp, double q, MGState** out, int* ok);
+// z = one V-cycle applied to b, on e.st; skipped once *done != 0
+int mg_apply(Eng& e, const MGState* mg, const PPEView& pp, const double* done, const float* b, float* z,
+             double spmv_bytes);
+void mg_free(sph_ctx* c, MGState* mg);
+int mg_levels(const MGState* mg);
+int mg_coarsest(const MGState* mg);
 int ppe_solve(Eng& e);   // every case of the step; 0, or SPH_E_NOCONV when a case did not converge
 void ppe_free(sph_ctx* c);
 
