@@ -308,6 +308,7 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     for (int f = 0; f < 5; f++) d.snap[f] = c->snap[f];
     d.ppe = ppe_view(c);
     for (int f = 0; f < 3; f++) d.xnew[f] = c->xnew[f];
+    d.rec = c->rec;
     d.sc = e.d_sc + (size_t)k * SCAL_COUNT;
     d.ctl = e.d_ctl + (size_t)k * CTL_COUNT;
 }
@@ -319,6 +320,7 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
         sph_ctx* c = cases[k];
         if (c->prm.ale_positions && !c->xnew[0])
             for (int f = 0; f < 3; f++) SPH_TRY(dalloc(c, &c->xnew[f], c->cap));
+        if (!c->rec) SPH_TRY(dalloc(c, &c->rec, (size_t)c->cap * REC_DOUBLES));
         if (count[k] == 0 || c->prm.ale_positions) {   // moving particles: a search every step
             SPH_TRY(set_ncell_run(c));
             SPH_TRY(nnps_run(c));

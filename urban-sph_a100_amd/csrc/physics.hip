@@ -158,6 +158,26 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_grad_corr(Seg sg) {
 
 // ------------------------------------------------------------------------
 // KERNEL_clc_prep3D, functions_PREP.cuh:287-488
+// Neighbour records of every particle from the working state (see
+// rec_pos): all pairs, or only those that hold velocities and pressures, or
+// pressures.
+template <int WHICH>
+__global__ void __launch_bounds__(BLOCK) k_pack_records(Seg sg) {
+    SEG_PARTICLE(sg);
+    const PView S = cd.S;
+    double2* r = (double2*)(cd.rec + (size_t)i * REC_DOUBLES);
+    r[2] = make_double2(S.rho[i], S.pres[i] - S.pres0[i]);
+    if (WHICH == REC_PQ) return;
+    r[3] = make_double2(S.ux[i], S.uy[i]);
+    r[4] = make_double2(S.uz[i], S.h[i]);
+    if (WHICH == REC_VEL_PQ) return;
+    const long long t = (long long)(unsigned)S.p_type[i] | ((long long)S.i_type[i] << 32);
+    r[0] = make_double2(S.x[i], S.y[i]);
+    r[1] = make_double2(S.z[i], S.m[i]);
+    r[5] = make_double2(S.vis_t[i], __longlong_as_double(t));
+    r[6] = make_double2(S.temp[i], S.concn[i]);
+}
+
 template <class R>
 __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
     SEG_PARTICLE(sg);
@@ -587,20 +607,25 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
-        int ptypej = S.p_type[j];
+        double hj_d, vis_tj, pqj_, xj, yj, zj, mj_d, uxj, uyj, uzj, rhoj_d;
+        int ptypej, itj_;
+        rec_aux(cd.rec, j, vis_tj, ptypej, itj_);
         if (ptypej >= 1000) continue;
-        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        rec_pos(cd.rec, j, xj, yj, zj, mj_d, rhoj_d, pqj_);
+        rec_vel(cd.rec, j, uxj, uyj, uzj, hj_d);
+        R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
         R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
         R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
         R tdwx = tdwij * dx / tdist;
         R tdwy = tdwij * dy / tdist;
         R tdwz = tdwij * dz / tdist;
-        R mj = (R)S.m[j], rhoj = (R)S.rho[j];
-        double hj_d = S.h[j];
-        R dux = (R)(uxi - S.ux[j]), duy = (R)(uyi - S.uy[j]), duz = (R)(uzi - S.uz[j]);   // u_i - u_j
+        R mj = (R)mj_d, rhoj = (R)rhoj_d;
+        R dux = (R)(uxi - uxj), duy = (R)(uyi - uyj), duz = (R)(uzi - uzj);   // u_i - u_j
+        double tempj = 0.0, concnj_d = 0.0;
+        if (pr.con_solve || pr.concn_solve || pr.boussinesq_solve) rec_scalar(cd.rec, j, tempj, concnj_d);
         if (pr.fv_solve == SPH_FV_CLEARY) {
-            R visj = (R)(viscosity2(pr, ptypej) + S.vis_t[j] * S.rho[j]);
+            R visj = (R)(viscosity2(pr, ptypej) + vis_tj * rhoj_d);
             R C_v = dx * tdwx + dy * tdwy + dz * tdwz;
             C_v *= (visi * visj) / (visi + visj);
             C_v *= (R)4 * (mj / (rhoi * rhoj));
@@ -609,7 +634,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             tmpx += (double)(C_v * dux); tmpy += (double)(C_v * duy); tmpz += (double)(C_v * duz);
         } else if (pr.fv_solve == SPH_FV_MONAGHAN) {
             R uij_xij = dux * dx + duy * dy + duz * dz;
-            R visj = (R)(viscosity2(pr, ptypej) + S.vis_t[j] * S.rho[j]);
+            R visj = (R)(viscosity2(pr, ptypej) + vis_tj * rhoj_d);
             R phi_ij = uij_xij;
             phi_ij /= (tdist * tdist + (R)0.01 * hi * hi);
             R P_ij = (R)(-2.0) * (R)(3 + 2.0) * phi_ij * mj / rhoj;
@@ -627,9 +652,9 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             eulery += (double)((Rux * vy * tdwx + Ruy * vy * tdwy + Ruz * vy * tdwz) * mj / rhoj * ome);
             eulerz += (double)((Rux * vz * tdwx + Ruy * vz * tdwy + Ruz * vz * tdwz) * mj / rhoj * ome);
             if (pr.con_solve)
-                eulert -= (double)((R)2.0 * mj / (rhoi + rhoj) * (R)(S.temp[j] - tempi) * (dux * tdwx + duy * tdwy + duz * tdwz) * ome);
+                eulert -= (double)((R)2.0 * mj / (rhoi + rhoj) * (R)(tempj - tempi) * (dux * tdwx + duy * tdwy + duz * tdwz) * ome);
             if (pr.concn_solve) {
-                R dc = (R)(S.concn[j] - (double)concni);
+                R dc = (R)(concnj_d - (double)concni);
                 tmp_Rd -= (double)((Rux * dc * tdwx + Ruy * dc * tdwy + Ruz * dc * tdwz) * mj / rhoj * ome);
             }
         }
@@ -637,12 +662,12 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             R kcj = (R)(pr.solid_k * (ptypej == 0 || ptypej == 1000) + pr.air_k * (ptypej == 1 || ptypej == -1));
             R rhoj_temp = (ptypej == 1 || ptypej == -1) ? rhoj : (R)pr.solid_rho;
             R mj_temp = mj * rhoj_temp / rhoj;
-            R sum_con_H = (R)4.0 * mj_temp * kcj * kci * (R)(tempi - S.temp[j]) * tdwij;
+            R sum_con_H = (R)4.0 * mj_temp * kcj * kci * (R)(tempi - tempj) * tdwij;
             sum_con_H /= (tdist + (R)1e-10) * rhoi * rhoj_temp * (kci + kcj);
             tmp_Rc += (double)sum_con_H;
         }
         if (pr.concn_solve) {
-            R concnj = (R)S.concn[j];
+            R concnj = (R)concnj_d;
             R diffj = (R)pr.diff_coef;
             R tmprd = mi * (diffi * rhoi + diffj * rhoj);
             tmprd *= (dx * tdwx + dy * tdwy + dz * tdwz);
@@ -651,7 +676,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             tmp_Rd += (double)tmprd1;
         }
         if (pr.boussinesq_solve) {
-            tmp_temp += (double)((mj / rhoj) * (R)S.temp[j] * twij);
+            tmp_temp += (double)((mj / rhoj) * (R)tempj * twij);
             tmp_flt += (double)((mj / rhoj) * twij);
         }
         if (pr.artificial_viscosity) {
@@ -872,16 +897,19 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
         if (!(e & F_DM)) continue;
         int j = e & NB_IDX;
         if (S.p_type[j] >= 1000) continue;
-        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
+        double hj, pqj, xj, yj, zj, mj_d, rhoj_d;
+        rec_pos(cd.rec, j, xj, yj, zj, mj_d, rhoj_d, pqj);
+        hj = rec_at(cd.rec, j)[4].y;
+        R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
-        R hij = (R)(0.5 * (hi_d + S.h[j]));
+        R hij = (R)(0.5 * (hi_d + hj));
         R tmp_Aij = calc_tmpA<R>(kc, hij);
         R tdwij = kernel_dw<R>(kc, tmp_Aij, hij, tdist);
         R tdwx = tdwij * dx / tdist;
         R tdwy = tdwij * dy / tdist;
         R tdwz = tdwij * dz / tdist;
-        R mj = (R)S.m[j], rhoj = (R)S.rho[j];
-        R pj = (R)(S.pres[j] - S.pres0[j]);
+        R mj = (R)mj_d, rhoj = (R)rhoj_d;
+        R pj = (R)pqj;
         if (pr.fp_solve) {
             const R Pb = (R)0.0;
             R C_p = (R)(-2.0) * mj / (rhoi * rhoj) * ((pi + Pb) * rhoj + (pj + Pb) * rhoi) / (rhoi + rhoj);
@@ -1134,6 +1162,15 @@ static int run_freeslip(Eng& e, int which) {
 
 // Everything of SOPHIA_single_ISPH before the pressure solve
 // (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
+// Neighbour records of every case of the step, from its working state as
+// it is now; run before each pass that gathers from them.
+int pack_records(Eng& e, int which) {
+    if (which == REC_PQ) SEGL("pack_records", -1, k_pack_records<REC_PQ>);
+    else if (which == REC_VEL_PQ) SEGL("pack_records", -1, k_pack_records<REC_VEL_PQ>);
+    else SEGL("pack_records", -1, k_pack_records<REC_ALL>);
+    return 0;
+}
+
 int step_physics_front(Eng& e) {
     const bool mixed = mixed_of(e);
     bool any_count0 = false;
@@ -1152,6 +1189,7 @@ int step_physics_front(Eng& e) {
     SEGL("most_self", -1, k_most_self);
     SEGL("most_deposit", -1, k_most_deposit);
     if (any_count0) SPH_TRY(run_neumann(e, 0, 1));
+    SPH_TRY(pack_records(e, REC_ALL));
     if (mixed) SEGL("advection", TK_ADVECTION, k_advection<float>);
     else SEGL("advection", TK_ADVECTION, k_advection<double>);
     SEGL("projection", -1, k_projection);
@@ -1165,6 +1203,7 @@ int step_physics_back(Eng& e) {
     const bool mixed = mixed_of(e);
     SPH_TRY(run_neumann(e, 0, 0));
     SPH_TRY(run_outlet(e, 0));
+    SPH_TRY(pack_records(e, REC_PQ));   // pressures after the solve and the boundary passes
     if (mixed) SEGL("pressure_force", TK_PRESSURE, k_pressure_force<float>);
     else SEGL("pressure_force", TK_PRESSURE, k_pressure_force<double>);
     if (mixed) SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<float>);

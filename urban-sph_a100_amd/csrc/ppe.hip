@@ -452,9 +452,15 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         unsigned e = nrow[(size_t)k * SLICE];
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
-        if (S.p_type[j] >= 1000 || S.i_type[j] == 4) continue;
-        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
-        R rhoj = (R)S.rho[j], mj = (R)S.m[j];
+        double vtj_;
+        int ptj, itj;
+        rec_aux(cd.rec, j, vtj_, ptj, itj);
+        if (ptj >= 1000 || itj == 4) continue;
+        double xj, yj, zj, mj_d, rhoj_d, qj, uxj, uyj, uzj, hj_;
+        rec_pos(cd.rec, j, xj, yj, zj, mj_d, rhoj_d, qj);
+        rec_vel(cd.rec, j, uxj, uyj, uzj, hj_);
+        R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
+        R rhoj = (R)rhoj_d, mj = (R)mj_d;
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
         R tdwij = kernel_dwp<R>(kc, tmp_A, hi, tdist);
         R tdwx = tdwij * dx / tdist;
@@ -464,13 +470,12 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         R tdwyc = gc12 * tdwx + gc22 * tdwy + gc23 * tdwz;
         R tdwzc = gc31 * tdwx + gc23 * tdwy + gc33 * tdwz;
         R hm = (R)2.0 * rhoi * rhoj / (rhoi + rhoj);
-        bix += (double)(mj / rhoj * (R)(S.ux[j] - uxi) * tdwxc * hm);
-        biy += (double)(mj / rhoj * (R)(S.uy[j] - uyi) * tdwyc * hm);
-        biz += (double)(mj / rhoj * (R)(S.uz[j] - uzi) * tdwzc * hm);
+        bix += (double)(mj / rhoj * (R)(uxj - uxi) * tdwxc * hm);
+        biy += (double)(mj / rhoj * (R)(uyj - uyi) * tdwyc * hm);
+        biz += (double)(mj / rhoj * (R)(uzj - uzi) * tdwzc * hm);
         R cc = (R)2.0 * mj / rhoj * (dx * tdwxc + dy * tdwyc + dz * tdwzc) / tdist / tdist;
         double ccd = (double)cc;
         Aij += ccd;
-        double qj = S.pres[j] - S.pres0[j];
         if (jacobi) AijPij += ccd * qj;
         if (j != i) {
             if (rowof[j] >= 0) {
@@ -1295,6 +1300,7 @@ int ppe_solve(Eng& e) {
         asm_bytes += (double)c->rep.unknowns * (15.0 * 8) + (double)c->rep.nnz * (4.0 + (mixed ? 4.0 : 8.0));
     }
     if (rows_total == 0) return 0;
+    SPH_TRY(pack_records(e, REC_VEL_PQ));   // velocities after the predictor and its boundary passes
     {
         const int blk = tuned_block(e, TK_ASSEMBLE);
         Shape sh;

@@ -131,6 +131,7 @@ struct CaseDesc {
     double* sc;          // Krylov scalars, computed on the device
     const double* ctl;   // solve controls written by the host: tol, maxit, skip
     double* xnew[3];     // ALE positions of the corrector (null unless ale_positions)
+    double* rec;         // neighbour records (REC_DOUBLES per particle), refreshed before each gathering pass
 };
 enum Ctl { CTL_TOL, CTL_MAXIT, CTL_SKIP, CTL_COUNT = 4 };
 constexpr int SCAL_COUNT = 32;   // Krylov scalars per case
@@ -170,6 +171,45 @@ static_assert(TK_COUNT == SPH_TUNE_PASSES, "tunable passes of the public tuning 
 constexpr int TUNE_CANDIDATES = 4;           // 64, 128, 256, 512 threads
 constexpr int TUNE_BLOCKS[TUNE_CANDIDATES] = {64, 128, 256, 512};
 constexpr int MAX_BLOCK = 512;
+
+// Neighbour record: the fields a gathering pass reads of its neighbour j,
+// one 128-byte line per particle, so a pair costs one line from L2 instead
+// of one line per field array (advection, PPE assembly and pressure force;
+// the strain pass reads fewer fields and is faster from the field arrays).  Pairs of doubles: 0 (x, y), 1 (z, m),
+// 2 (rho, pres - pres0), 3 (ux, uy), 4 (uz, h), 5 (vis_t, p_type | i_type << 32),
+// 6 (temp, concn).  The values are copies of the working state taken just
+// before the pass (k_pack_records), so every pass computes with exactly the
+// values it read from the field arrays before.
+constexpr int REC_DOUBLES = 16;
+__device__ __forceinline__ const double2* rec_at(const double* rec, int j) {
+    return (const double2*)(rec + (size_t)j * REC_DOUBLES);
+}
+// x y z m rho pq
+__device__ __forceinline__ void rec_pos(const double* rec, int j, double& x, double& y, double& z, double& m,
+                                        double& rho, double& pq) {
+    const double2* r = rec_at(rec, j);
+    const double2 a = r[0], b = r[1], c = r[2];
+    x = a.x; y = a.y; z = b.x; m = b.y; rho = c.x; pq = c.y;
+}
+// ux uy uz h
+__device__ __forceinline__ void rec_vel(const double* rec, int j, double& ux, double& uy, double& uz, double& h) {
+    const double2* r = rec_at(rec, j);
+    const double2 a = r[3], b = r[4];
+    ux = a.x; uy = a.y; uz = b.x; h = b.y;
+}
+// vis_t p_type i_type
+__device__ __forceinline__ void rec_aux(const double* rec, int j, double& vis_t, int& p_type, int& i_type) {
+    const double2 a = rec_at(rec, j)[5];
+    vis_t = a.x;
+    const long long t = __double_as_longlong(a.y);
+    p_type = (int)(unsigned)(t & 0xffffffffLL);
+    i_type = (int)(t >> 32);
+}
+// temp concn
+__device__ __forceinline__ void rec_scalar(const double* rec, int j, double& temp, double& concn) {
+    const double2 a = rec_at(rec, j)[6];
+    temp = a.x; concn = a.y;
+}
 
 struct PPEState;   // ppe.hip
 
@@ -261,6 +301,7 @@ struct sph_ctx {
     float* stat_buf = nullptr;           // stats reduction scratch
     double* apr_target = nullptr;        // APR target level per particle (the reference's M_num)
     double* xnew[3] = {};                // ALE: corrector positions, committed at the end of the step
+    double* rec = nullptr;               // neighbour records of the gathering passes (cap * REC_DOUBLES)
 
     sph::PPEState* ppe = nullptr;
     sph_ppe_report rep{};
@@ -345,6 +386,10 @@ int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax);
 // ppe.hip
 int ppe_build(sph_ctx* c);
 PPEView ppe_view(const sph_ctx* c);
+// neighbour records from the working state (physics.hip): REC_ALL, or only
+// the pairs that hold velocities and pressures (REC_VEL_PQ) or pressures (REC_PQ)
+enum { REC_ALL = 0, REC_VEL_PQ = 1, REC_PQ = 2 };
+int pack_records(Eng& e, int which);
 // aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
 struct MGState;
 constexpr int MG_COARSE_MAX = 1024;   // unknowns of the densely inverted coarsest level
