@@ -62,10 +62,15 @@ struct MGState {
     int nlev = 0;            // levels, the PPE and the coarsest included
     MGLevel L[MG_MAX_LEVELS];
     float* ainv = nullptr;   // dense inverse of the coarsest operator, row-major
-    long long fine_nnz = 0;
+    // level-0 operator of the V-cycle (16-bit PPE columns only): per stored
+    // entry one word, the 16-bit column in the low half and the entry over
+    // its row's diagonal as a half float in the high half, in the PPE's
+    // slot layout; null: the V-cycle reads the PPE's float values
+    unsigned* ev = nullptr;
 };
 
 int mg_levels(const MGState* mg) { return mg ? mg->nlev : 0; }
+bool mg_compact(const MGState* mg) { return mg && mg->ev; }
 int mg_coarsest(const MGState* mg) { return mg && mg->nlev > 0 ? mg->L[mg->nlev - 1].n : 0; }
 
 // ------------------------------------------------------------------------
@@ -334,6 +339,19 @@ __global__ void __launch_bounds__(BLOCK) k_mg_tofloat(long long n, const double*
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
+// the V-cycle's compact level-0 operator: column word | half(value / diag) << 16
+__global__ void __launch_bounds__(BLOCK) k_mg_pack16(PPEView pp, unsigned* ev) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= pp.nr) return;
+    const long long base = prow(pp.sl_off, r);
+    const float d = pp.fdiag[r];
+    const int m = pp.cnt[r];
+    for (int k = 0; k < m; k++) {
+        const long long e = base + pslot(k);
+        const _Float16 h = (_Float16)(pp.vals32[e] / d);
+        ev[e] = (unsigned)pp.ecols[e] | ((unsigned)__builtin_bit_cast(unsigned short, h) << 16);
+    }
+}
 
 // ------------------------------------------------------------------------
 // V-cycle kernels
@@ -347,6 +365,32 @@ __device__ __forceinline__ double coarse_row(const CoarseOp& o, int r, const flo
     return (double)o.diag[r] * (double)x[r] - s;
 }
 
+// sum_k (a_rk / d_r) x[col_k] over the compact level-0 operator, in entry
+// order, accumulated in double; four independent loads in flight per lane
+__device__ __forceinline__ double mg_row16(const PPEView& pp, const unsigned* ev, int r, const float* x,
+                                           const int* wb) {
+    const unsigned* p = ev + prow(pp.sl_off, r);
+    const int m = pp.cnt[r];
+    auto term = [&](unsigned u) {
+        const _Float16 h = __builtin_bit_cast(_Float16, (unsigned short)(u >> 16));
+        return (double)(float)h * (double)x[ecol(wb, (int)(u & 0xffffu))];
+    };
+    double s = 0.0;
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+        const unsigned u0 = __builtin_nontemporal_load(p + pslot(k + 0));
+        const unsigned u1 = __builtin_nontemporal_load(p + pslot(k + 1));
+        const unsigned u2 = __builtin_nontemporal_load(p + pslot(k + 2));
+        const unsigned u3 = __builtin_nontemporal_load(p + pslot(k + 3));
+        s += term(u0);
+        s += term(u1);
+        s += term(u2);
+        s += term(u3);
+    }
+    for (; k < m; k++) s += term(__builtin_nontemporal_load(p + pslot(k)));
+    return s;
+}
+
 // x = w b / diag
 __global__ void __launch_bounds__(BLOCK) k_mg_pre(int n, const double* done, const float* b, const float* diag,
                                                   float* x) {
@@ -354,27 +398,38 @@ __global__ void __launch_bounds__(BLOCK) k_mg_pre(int n, const double* done, con
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < n) x[r] = (float)(MG_OMEGA * (double)b[r] / (double)diag[r]);
 }
-// r = b - A0 x (level 0: the PPE)
-__global__ void __launch_bounds__(BLOCK) k_mg_res0(PPEView pp, const double* done, const float* b, const float* x,
-                                                   float* r) {
+// r = b - A0 x (level 0: the PPE, or its compact copy ev)
+__global__ void __launch_bounds__(BLOCK) k_mg_res0(PPEView pp, const unsigned* ev, const double* done, const float* b,
+                                                   const float* x, float* r) {
     __shared__ int wbase[BLOCK];
     MG_GATE(done);
     const int first = blockIdx.x * BLOCK;
     load_group_bases(pp, first, wbase);
     const int row = first + (int)threadIdx.x;
-    if (row < pp.nr) r[row] = (float)((double)b[row] - spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, x, wbase));
+    if (row >= pp.nr) return;
+    if (ev) {
+        const double s = mg_row16(pp, ev, row, x, wbase + (threadIdx.x & ~63));
+        r[row] = (float)((double)b[row] - (double)pp.fdiag[row] * ((double)x[row] - s));
+    } else {
+        r[row] = (float)((double)b[row] - spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, x, wbase));
+    }
 }
 // out = x + w (b - A0 x) / diag
-__global__ void __launch_bounds__(BLOCK) k_mg_post0(PPEView pp, const double* done, const float* b, const float* x,
-                                                    float* out) {
+__global__ void __launch_bounds__(BLOCK) k_mg_post0(PPEView pp, const unsigned* ev, const double* done,
+                                                    const float* b, const float* x, float* out) {
     __shared__ int wbase[BLOCK];
     MG_GATE(done);
     const int first = blockIdx.x * BLOCK;
     load_group_bases(pp, first, wbase);
     const int row = first + (int)threadIdx.x;
-    if (row < pp.nr) {
+    if (row >= pp.nr) return;
+    const double d = (double)pp.fdiag[row];
+    if (ev) {
+        const double s = mg_row16(pp, ev, row, x, wbase + (threadIdx.x & ~63));
+        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] / d - (double)x[row] + s));
+    } else {
         const double ax = spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, x, wbase);
-        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] - ax) / (double)pp.fdiag[row]);
+        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] - ax) / d);
     }
 }
 __global__ void __launch_bounds__(BLOCK) k_mg_resc(CoarseOp op, int n, const double* done, const float* b,
@@ -434,6 +489,7 @@ void mg_free(sph_ctx* c, MGState* mg) {
         for (void* q : ptrs) dfree(c, q);
     }
     dfree(c, mg->ainv);
+    dfree(c, mg->ev);
     delete mg;
 }
 
@@ -522,7 +578,7 @@ static int mg_vectors(sph_ctx* c, MGLevel& L, bool fine, bool coarsest) {
     return 0;
 }
 
-int mg_build(sph_ctx* c, const PPEView& pp, double q, MGState** out, int* ok) {
+int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok) {
     *out = nullptr;
     *ok = 0;
     if (pp.nr <= MG_COARSE_MAX || pp.ltail || !pp.vals32 || !pp.fdiag || !(q > 0)) return 0;
@@ -558,7 +614,6 @@ int mg_build(sph_ctx* c, const PPEView& pp, double q, MGState** out, int* ok) {
         hipLaunchKernelGGL(k_mg_coord0, dim3(grid1(L0.n)), dim3(BLOCK), 0, c->stream, L0.n, pp.rows, c->sview(), x0,
                            y0, z0, q, L0.nx, L0.ny, L0.nz, L0.cx, L0.cy, L0.cz);
         MGTRY(launch_check(c, "mg_coord"));
-        mg->fine_nnz = 0;
     }
     int nlev = 1;
     bool good = true;
@@ -611,6 +666,11 @@ int mg_build(sph_ctx* c, const PPEView& pp, double q, MGState** out, int* ok) {
         dfree(c, colk);
         if (h_err) return fail_out(0);
     }
+    if (pp.ecols) {
+        MGTRY(dalloc(c, &mg->ev, (size_t)std::max(1LL, slots)));
+        hipLaunchKernelGGL(k_mg_pack16, dim3(grid1(pp.nr)), dim3(BLOCK), 0, c->stream, pp, mg->ev);
+        MGTRY(launch_check(c, "mg_pack16"));
+    }
     for (int l = 0; l < nlev; l++) MGTRY(mg_vectors(c, mg->L[l], l == 0, l == nlev - 1));
     // coordinates are needed only while building
     for (int l = 0; l < nlev; l++) {
@@ -643,7 +703,8 @@ int mg_apply(Eng& e, const MGState* mg, const PPEView& pp, const double* done, c
     const int nr = F.n;
     // down
     MGE("mg_pre", 12.0 * nr, grid1(nr), k_mg_pre, nr, done, b, (const float*)pp.fdiag, F.x);
-    MGE("mg_res0", spmv_bytes, grid1(nr), k_mg_res0, pp, done, b, (const float*)F.x, F.r);
+    const double b0 = mg->ev ? spmv_bytes * (2.0 / 3.0) : spmv_bytes;   // 4 instead of 6 bytes per entry
+    MGE("mg_res0", b0, grid1(nr), k_mg_res0, pp, (const unsigned*)mg->ev, done, b, (const float*)F.x, F.r);
     MGE("mg_restrict", 12.0 * nr, grid1(mg->L[1].n), k_mg_restrict, mg->L[1].n, done, (const int*)F.mstart,
         (const int*)F.mem, (const float*)F.r, mg->L[1].b);
     for (int l = 1; l < nl - 1; l++) {
@@ -666,7 +727,7 @@ int mg_apply(Eng& e, const MGState* mg, const PPEView& pp, const double* done, c
         MGE("mg_post", 0, grid1(L.n), k_mg_postc, op, L.n, done, (const float*)L.b, (const float*)L.x, L.y);
     }
     MGE("mg_prolong", 16.0 * nr, grid1(nr), k_mg_prolong, nr, done, (const int*)F.agg, res(1), F.x);
-    MGE("mg_post0", spmv_bytes, grid1(nr), k_mg_post0, pp, done, b, (const float*)F.x, z);
+    MGE("mg_post0", b0, grid1(nr), k_mg_post0, pp, (const unsigned*)mg->ev, done, b, (const float*)F.x, z);
 #undef MGE
     return 0;
 }

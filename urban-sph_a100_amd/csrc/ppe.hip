@@ -1367,7 +1367,7 @@ int ppe_solve(Eng& e) {
                     double q = 0.0;
                     int ok = 0;
                     SPH_TRY(lattice_spacing(c, P, &q));
-                    SPH_TRY(mg_build(c, ppe_view(c), q, &P->mg, &ok));
+                    SPH_TRY(mg_build(c, ppe_view(c), P->nnz_slots, q, &P->mg, &ok));
                     P->mg_state = ok ? 1 : -1;
                 }
             }

@@ -393,9 +393,11 @@ int pack_records(Eng& e, int which);
 // aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
 struct MGState;
 constexpr int MG_COARSE_MAX = 1024;   // unknowns of the densely inverted coarsest level
-// Builds the hierarchy of the PPE in pp (float values) with lattice spacing
-// q; *ok = 0 (and *out = null) when the case does not suit it.
-int mg_build(sph_ctx* c, const PPEView& pp, double q, MGState** out, int* ok);
+// Builds the hierarchy of the PPE in pp (float values, `slots` stored SELL
+// slots) with lattice spacing q; *ok = 0 (and *out = null) when the case
+// does not suit it.
+int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok);
+bool mg_compact(const MGState* mg);   // the V-cycle's level 0 reads the half-precision copy
 // z = one V-cycle applied to b, on e.st; skipped once *done != 0
 int mg_apply(Eng& e, const MGState* mg, const PPEView& pp, const double* done, const float* b, float* z,
              double spmv_bytes);
