@@ -98,6 +98,11 @@ struct CoarseOp {
 };
 static CoarseOp coarse_op(const MGLevel& L) { return CoarseOp{L.sl_off, L.cnt, L.cols, L.vals, L.diag}; }
 
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
 __device__ __forceinline__ int wave_sum(int v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -164,24 +169,27 @@ __global__ void __launch_bounds__(BLOCK) k_mg_link(int n, const int* key, const 
 }
 
 // One coarse row per 64-thread block (one wave): its terms are the
-// members' diagonals and entries mapped to aggregates; the distinct
-// aggregate columns are found by a bitonic sort in LDS, and each column's
-// terms are added by its owner lane (column slot mod 64) in a fixed order.
-// COUNT pass: the row's off-diagonal count.  FILL pass: columns ascending,
-// values -sum, the diagonal.  err: 1 = too many terms, 2 = too many columns.
+// members' diagonals and entries mapped to aggregates, numbered in order
+// (members ascending, each member's diagonal first, then its entries in
+// storage order).  A bitonic sort in LDS of 32-bit keys (column - the row's
+// first column) << 11 | term number groups each column's terms in term
+// order, so each column's sum is formed in that fixed order by one lane:
+// the hierarchy is bit-reproducible.  The terms are float values, kept as
+// floats in LDS and added in double.  COUNT pass: the row's off-diagonal
+// count.  FILL pass: columns ascending, values -sum, the diagonal.
+// err: 1 = too many terms, 2 = too many columns, 3 = columns too far apart.
 template <bool FILL, class OP>
 __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg, const int* mstart, const int* mem,
                                                   int* cnt1, const long long* sl_off1, int* cols1, float* vals1,
                                                   float* diag1, int* err) {
-    __shared__ int sk[MG_ENT_CAP];
-    __shared__ int st_slot[SLICE];
-    __shared__ int st_col[SLICE];
-    __shared__ double st_val[SLICE];
+    static_assert(MG_ENT_CAP <= (1 << 11), "term numbers take 11 key bits");
+    __shared__ unsigned sk[MG_ENT_CAP];           // (column - jmin) << 11 | term number
+    __shared__ float sv[FILL ? MG_ENT_CAP : 1];   // term values by term number
+    __shared__ int seg[SLICE * MG_QMAX + 1];      // first sorted position of each distinct column
     const int I = blockIdx.x;
     const int lane = threadIdx.x;
     if (I >= n1) return;
     const int t0 = mstart[I], t1 = mstart[I + 1];
-    // terms of the row: one diagonal per member plus its entries
     int nterm = 0;
     for (int t = t0 + lane; t < t1; t += SLICE) nterm += op.cnt(mem[t]) + 1;
     nterm = wave_sum(nterm);
@@ -191,22 +199,36 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     }
     int np2 = 1;
     while (np2 < nterm) np2 <<= 1;
-    // gather the aggregate columns (the member's own aggregate for its diagonal)
-    int pos = 0;
+    int pos = 0, jmin = I, jmax = I;
     for (int t = t0; t < t1; t++) {
         const int m = mem[t];
         const int c = op.cnt(m);
-        for (int k = lane - 1; k < c; k += SLICE) sk[pos + k + 1] = k < 0 ? I : agg[op.col(m, k)];
+        for (int k = lane - 1; k < c; k += SLICE) {
+            const int q = pos + k + 1;
+            const int J = k < 0 ? I : agg[op.col(m, k)];
+            sk[q] = (unsigned)J;
+            jmin = min(jmin, J);
+            jmax = max(jmax, J);
+            if (FILL) sv[q] = k < 0 ? op.dg(m) : -op.val(m, k);
+        }
         pos += c + 1;
     }
-    for (int p = nterm + lane; p < np2; p += SLICE) sk[p] = INT_MAX;
+    jmin = -wave_max(-jmin);
+    jmax = wave_max(jmax);
+    if (jmax - jmin >= (1 << 21) - 1) {
+        if (lane == 0) { err[0] = 3; if (!FILL) cnt1[I] = 0; }
+        return;
+    }
+    __syncthreads();
+    for (int p = lane; p < np2; p += SLICE)
+        sk[p] = p < nterm ? ((sk[p] - (unsigned)jmin) << 11) | (unsigned)p : 0xffffffffu;
     __syncthreads();
     for (int k = 2; k <= np2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = lane; i < np2; i += SLICE) {
                 const int ixj = i ^ j;
                 if (ixj > i) {
-                    const int a = sk[i], b = sk[ixj];
+                    const unsigned a = sk[i], b = sk[ixj];
                     const bool up = (i & k) == 0;
                     if ((a > b) == up) { sk[i] = b; sk[ixj] = a; }
                 }
@@ -214,20 +236,14 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
             __syncthreads();
         }
     }
-    // distinct columns, compacted in place (each lane owns a run of the sorted list)
+    // distinct columns: each lane counts the group starts in its run
+    auto col_at = [&](int p) { return (int)(sk[p] >> 11) + jmin; };
     const int per = (nterm + SLICE - 1) / SLICE;
     const int p0 = min(nterm, lane * per), p1 = min(nterm, p0 + per);
     int own = 0;
-    for (int p = p0; p < p1; p++) own += (p == 0 || sk[p] != sk[p - 1]) ? 1 : 0;
+    for (int p = p0; p < p1; p++) own += (p == 0 || col_at(p) != col_at(p - 1)) ? 1 : 0;
     const int at = wave_excl(own);
     const int total = wave_sum(own);
-    int keep[MG_ENT_CAP / SLICE];
-    int nk = 0;
-    for (int p = p0; p < p1; p++)
-        if (p == 0 || sk[p] != sk[p - 1]) keep[nk++] = sk[p];
-    __syncthreads();
-    for (int q = 0; q < nk; q++) sk[at + q] = keep[q];
-    __syncthreads();
     if (total > SLICE * MG_QMAX) {
         if (lane == 0) { err[0] = 2; if (!FILL) cnt1[I] = 0; }
         return;
@@ -236,59 +252,28 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
         if (lane == 0) cnt1[I] = total - 1;
         return;
     }
-    auto slot_of = [&](int J) {
-        int lo = 0, hi = total - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (sk[mid] < J) lo = mid + 1;
-            else hi = mid;
-        }
-        return lo;
-    };
-    const int sI = slot_of(I);
-    double acc[MG_QMAX];
-    int jc[MG_QMAX];
-#pragma unroll
-    for (int q = 0; q < MG_QMAX; q++) { acc[q] = 0.0; jc[q] = -1; }
-    for (int t = t0; t < t1; t++) {
-        const int m = mem[t];
-        const int c = op.cnt(m);
-        for (int kb = -1; kb < c; kb += SLICE) {
-            const int k = kb + lane;
-            const int nv = min(SLICE, c - kb);
-            if (lane < nv) {
-                int J;
-                double v;
-                if (k < 0) { J = I; v = (double)op.dg(m); }
-                else { J = agg[op.col(m, k)]; v = -(double)op.val(m, k); }
-                st_slot[lane] = J == I ? sI : slot_of(J);
-                st_col[lane] = J;
-                st_val[lane] = v;
-            }
-            __syncthreads();
-            for (int u = 0; u < nv; u++) {
-                const int s = st_slot[u];
-                if ((s & (SLICE - 1)) == lane) {
-                    const int qs = s >> 6;
-#pragma unroll
-                    for (int q = 0; q < MG_QMAX; q++)
-                        if (q == qs) { acc[q] += st_val[u]; jc[q] = st_col[u]; }
-                }
-            }
-            __syncthreads();
-        }
+    int w = at;
+    for (int p = p0; p < p1; p++)
+        if (p == 0 || col_at(p) != col_at(p - 1)) seg[w++] = p;
+    if (lane == 0) seg[total] = nterm;
+    __syncthreads();
+    int lo = 0, hi = total - 1;   // the diagonal's slot
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col_at(seg[mid]) < I) lo = mid + 1;
+        else hi = mid;
     }
+    const int sI = lo;
     const long long base = prow(sl_off1, I);
-#pragma unroll
-    for (int q = 0; q < MG_QMAX; q++) {
-        const int s = q * SLICE + lane;
-        if (s >= total) continue;
+    for (int s = lane; s < total; s += SLICE) {
+        double acc = 0.0;
+        for (int p = seg[s]; p < seg[s + 1]; p++) acc += (double)sv[sk[p] & 0x7ffu];
         if (s == sI) {
-            diag1[I] = (float)acc[q];
+            diag1[I] = (float)acc;
         } else {
             const int o = s - (s > sI ? 1 : 0);
-            cols1[base + pslot(o)] = jc[q];
-            vals1[base + pslot(o)] = (float)(-acc[q]);
+            cols1[base + pslot(o)] = col_at(seg[s]);
+            vals1[base + pslot(o)] = (float)(-acc);
         }
     }
 }
