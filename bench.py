@@ -128,6 +128,7 @@ def run_batched(cases, abi, urbansph, steps, warmup, dt, tol, n_cases=64):
     cs = []
     for P, nrm, geom, info in ens:
         prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED)
+        prm.update(ppe_precond=abi.PRECOND_MULTIGRID)
         c = urbansph.Case(prm, len(P))
         c.upload(P, None, nrm)
         cs.append(c)
@@ -287,6 +288,7 @@ def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=24, rate=4.0)
     for r in reqs:
         P, nrm, geom = cases.sized_canyon(r["target"], r["u_scale"], r["bh"])
         prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED, L=r["L"])
+        prm.update(ppe_precond=abi.PRECOND_MULTIGRID)
         decks.append((P, nrm, prm))
     log(f"[bench] serving: {n_req} requests at {rate}/s, sizes {min(len(d[0]) for d in decks)}.."
         f"{max(len(d[0]) for d in decks)}, steps {min(r['steps'] for r in reqs)}..{max(r['steps'] for r in reqs)}")
@@ -495,7 +497,8 @@ def main():
         bt = run_batched(cases, abi, urbansph, args.batch_steps, 5, args.dt, args.tol)
         line["batched"] = {
             "workload": "64 canyon cases, particle counts log-uniform in [2e4, 3e5] (seed 7), inflow scale U(0.5, 2), "
-                        "building height U(0.05, 0.2), dx=0.008 m, mixed precision, one launch sequence per step",
+                        "building height U(0.05, 0.2), dx=0.008 m, mixed precision with the multigrid preconditioner, "
+                        "one launch sequence per step",
             "particles": bt["particles"], "value": round(bt["value"], 1), "unit": "particle-steps/s",
             "ms_per_step": round(bt["ms_per_step"], 3), "steps": args.batch_steps, "warmup": 5,
             "sequential_value": round(bt["sequential_value"], 1),
@@ -529,7 +532,7 @@ def main():
                               "what": "serving: completion - arrival per canyon request"}
         line["serving"] = {
             "workload": f"{sv['requests']} canyon requests, Poisson arrivals {sv['rate_per_s']}/s, counts log-uniform "
-                        "in [1e4, 2e5], 10..40 steps, L in {-50, +50, inf} m, mixed precision, 240 GB budget "
+                        "in [1e4, 2e5], 10..40 steps, L in {-50, +50, inf} m, mixed precision with the multigrid preconditioner, 240 GB budget "
                         "(sizes and step counts scaled down 10x from the 2e6-particle / 500-step request mix "
                         "so that the offered load stays below one GPU's capacity)",
             "value": round(sv["value"], 1), "unit": "particle-steps/s", "p50_ms": round(sv["p50_ms"], 1),

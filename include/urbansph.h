@@ -176,14 +176,15 @@ typedef struct sph_params {
                                    Jacobi smoothing, dense inverse of the
                                    coarsest level), built once per set of PPE
                                    values; the outer FP64 refinement and the
-                                   tolerance are unchanged.  Used in steps of
-                                   a single case whose PPE has more than 1024
-                                   unknowns and no row of more than 256
-                                   entries; elsewhere (batches of several
-                                   cases, refined cases with long rows) and
-                                   after a multigrid inner solve that stalls,
-                                   the solve uses SPH_PRECOND_JACOBI.  The
-                                   report says which ran.  FP64 ignores it. */
+                                   tolerance are unchanged.  Used for a case
+                                   whose PPE has more than 1024 unknowns and
+                                   no row of more than 256 entries, alone or
+                                   in a batch (each case of a batch takes its
+                                   own); elsewhere (small or refined cases
+                                   with long rows) and after a multigrid
+                                   inner solve that stalls, the solve uses
+                                   SPH_PRECOND_JACOBI.  The report says which
+                                   ran.  FP64 ignores it.                     */
 } sph_params;
 
 /* Preconditioners of the MIXED inner solve (ppe_precond). */

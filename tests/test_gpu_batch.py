@@ -84,6 +84,63 @@ def test_batch_equals_alone(precision):
     b.close()
 
 
+def _specs_mg():
+    """The MIXED cases with the multigrid preconditioner asked for: the
+    7^3 box (343 unknowns) runs Jacobi, the others multigrid."""
+    out = _specs(abi.MIXED)
+    for _, _, prm, _ in out:
+        prm.update(ppe_precond=abi.PRECOND_MULTIGRID)
+    return out
+
+
+def test_batch_equals_alone_multigrid():
+    """Multigrid and Jacobi cases in one batch: every case's V-cycles run
+    in launches shared with the others, and each case ends bitwise where it
+    ends alone."""
+    specs = _specs_mg()
+    steps = 3
+    print(f"[batch-mg] sizes={[len(s[0]) for s in specs]}", flush=True)
+    alone = [_alone(s, steps) for s in specs]
+    b = urbansph.Batch()
+    cs = [_make(s) for s in specs]
+    for st in range(steps):
+        b.step(cs, [st] * len(cs), [s[3] for s in specs])
+    b.sync()
+    pcs = [c.ppe_report().precond for c in cs]
+    print(f"[batch-mg] preconditioners {pcs}", flush=True)
+    assert pcs[0] == abi.PRECOND_JACOBI and all(p == abi.PRECOND_MULTIGRID for p in pcs[1:])
+    for k, c in enumerate(cs):
+        _assert_same(_state(c), alone[k], f"case {k} (n={len(specs[k][0])})")
+        c.close()
+    b.close()
+
+
+def test_batch_staggered_membership_multigrid():
+    """As test_batch_staggered_membership with the multigrid: the V-cycle
+    tables follow the batch's membership."""
+    specs = _specs_mg()
+    steps = [4, 3, 2, 4]
+    join = [0, 1, 2, 2]
+    alone = [_alone(s, steps[k]) for k, s in enumerate(specs)]
+    b = urbansph.Batch()
+    cs = [_make(s) for s in specs]
+    done = [0] * len(specs)
+    for t in range(8):
+        members = [k for k in range(len(specs)) if join[k] <= t and done[k] < steps[k]]
+        if not members:
+            break
+        print(f"[batch-mg] tick {t}: members {members} counts {[done[k] for k in members]}", flush=True)
+        b.step([cs[k] for k in members], [done[k] for k in members], [specs[k][3] for k in members])
+        for k in members:
+            done[k] += 1
+    b.sync()
+    for k, c in enumerate(cs):
+        assert done[k] == steps[k]
+        _assert_same(_state(c), alone[k], f"case {k}")
+        c.close()
+    b.close()
+
+
 def test_batch_staggered_membership():
     """Cases join and leave the batch at different step counters (the
     serving pattern); each still matches its solo run bitwise."""

@@ -6,9 +6,9 @@ rule: at PPE tolerance 1e-6 the result must be as close to the converged
 FP64 oracle as the FP64 solve at the same tolerance is, driven to a tight
 tolerance it must stay within 1e-5 of it, it must take fewer iterations
 than the Jacobi form, and it must be bit-reproducible (hierarchy build,
-graphs or direct launches).  Cases it does not cover (PPE of at most 1024
-unknowns, batches of several cases) run the Jacobi form, and the report
-says so.
+graphs or direct launches).  A PPE of at most 1024 unknowns runs the
+Jacobi form, and the report says so; batches mixing both forms are covered
+in test_gpu_batch.py.
 """
 import numpy as np
 import pytest
@@ -111,8 +111,9 @@ def test_multigrid_steps_reproducible_graphs_and_direct():
 
 
 def test_multigrid_falls_back_to_jacobi():
-    """A PPE of at most 1024 unknowns, and the cases of a batch, run the
-    Jacobi form even when the multigrid is asked for."""
+    """A PPE of at most 1024 unknowns runs the Jacobi form even when the
+    multigrid is asked for; in a batch each case takes its own
+    preconditioner."""
     P, nrm = cases.box(nf=6, walls=3)
     prm = cases.box_params(urbansph.default_params(), tol=1e-6)
     _, reps = _gpu(P, nrm, prm, 1e-4)
@@ -120,7 +121,7 @@ def test_multigrid_falls_back_to_jacobi():
     assert reps[0]["precond"] == abi.PRECOND_JACOBI and reps[0]["converged"] == 1
 
     P1, n1, mk, dt = _case("canyon")
-    P2, n2 = cases.box(nf=12, walls=3)
+    P2, n2 = cases.box(nf=6, walls=3)
     p1 = mk(1e-6)
     p2 = cases.box_params(urbansph.default_params(), tol=1e-6)
     cs = []
@@ -132,9 +133,9 @@ def test_multigrid_falls_back_to_jacobi():
     b = urbansph.Batch()
     b.step(cs, [0, 0], [dt, dt])
     b.sync()
-    for c in cs:
-        r = c.ppe_report()
-        assert r.precond == abi.PRECOND_JACOBI and r.mg_levels == 0 and r.converged == 1
+    r0, r1 = cs[0].ppe_report(), cs[1].ppe_report()
+    assert r0.precond == abi.PRECOND_MULTIGRID and r0.mg_levels >= 2 and r0.converged == 1
+    assert r1.precond == abi.PRECOND_JACOBI and r1.mg_levels == 0 and r1.converged == 1
 
 
 def test_multigrid_profiled_build_and_ale():

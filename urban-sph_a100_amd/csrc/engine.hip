@@ -235,6 +235,10 @@ void eng_free(Eng& e) {
     for (auto& t : e.tabs)
         if (t.d) (void)hipFree(t.d);
     e.tabs.clear();
+    if (e.mg_lv) (void)hipFree(e.mg_lv);
+    if (e.mg_tab) (void)hipFree(e.mg_tab);
+    e.mg_lv = nullptr; e.mg_tab = nullptr; e.mg_lv_cap = e.mg_tab_cap = 0;
+    e.mg_sig.clear();
     if (e.d_desc) (void)hipFree(e.d_desc);
     if (e.d_sc) (void)hipFree(e.d_sc);
     if (e.d_ctl) (void)hipFree(e.d_ctl);

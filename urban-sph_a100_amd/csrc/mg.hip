@@ -20,12 +20,16 @@
 //     x += s P z_next;  z = x + w (b - A x) / diag
 // with w = MG_OMEGA (damped Jacobi) and s = MG_SCALE (the over-correction
 // that unsmoothed aggregation needs).  All products accumulate in double.
-// Every kernel returns at once when the solve's done flag is set, so a
-// captured iteration costs nothing after convergence.
+// One launch per pass covers the multigrid cases of a step (block tables of
+// each level's rows, as the engine's segment tables do for particles and
+// rows); every block returns at once when its case's solve is done, so a
+// captured iteration costs nothing after convergence, and a case's V-cycle
+// does not depend on the other cases of the step.
 #include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "ppe_format.h"
@@ -33,7 +37,6 @@
 
 namespace sph {
 
-constexpr int MG_MAX_LEVELS = 12;
 static_assert(MG_COARSE_MAX <= 1024, "the coarsest level is inverted densely");
 constexpr int MG_ENT_CAP = 2048;    // terms of one coarse row (members' entries + diagonals)
 constexpr int MG_QMAX = 8;          // distinct columns of one coarse row: at most 64 * MG_QMAX
@@ -59,6 +62,7 @@ struct MGLevel {
     float *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
 };
 struct MGState {
+    long long id = 0;        // distinct for every hierarchy built (table signatures)
     int nlev = 0;            // levels, the PPE and the coarsest included
     MGLevel L[MG_MAX_LEVELS];
     float* ainv = nullptr;   // dense inverse of the coarsest operator, row-major
@@ -340,7 +344,23 @@ __global__ void __launch_bounds__(BLOCK) k_mg_pack16(PPEView pp, unsigned* ev) {
 
 // ------------------------------------------------------------------------
 // V-cycle kernels
-#define MG_GATE(done) if (*(done) != 0.0) return
+// A case's level as the V-cycle kernels see it (level 0: the PPE itself,
+// read through the case's descriptor).
+struct MGLv {
+    int n, lc;                    // rows (0: no such level); the case's coarsest level
+    const long long* sl_off;      // operator of a coarse level
+    const int* cnt;
+    const int* cols;
+    const float* vals;
+    const float* diag;
+    const int* agg;               // link to the next level
+    const int* mstart;
+    const int* mem;
+    float *b, *x, *y, *r;
+    const float* ainv;            // coarsest level: dense inverse
+    const unsigned* ev;           // level 0: compact operator, or null
+};
+__device__ __forceinline__ CoarseOp lv_op(const MGLv& L) { return CoarseOp{L.sl_off, L.cnt, L.cols, L.vals, L.diag}; }
 
 __device__ __forceinline__ double coarse_row(const CoarseOp& o, int r, const float* x) {
     const long long base = prow(o.sl_off, r);
@@ -376,92 +396,109 @@ __device__ __forceinline__ double mg_row16(const PPEView& pp, const unsigned* ev
     return s;
 }
 
+// block -> (case, first row of the level); the block returns when the
+// case's solve is done
+#define MG_HEAD(tab, lvl)                                      \
+    const int2 te_ = (tab)[blockIdx.x];                        \
+    const CaseDesc& cd = cds[te_.x];                           \
+    if (cd.sc[sc_done] != 0.0) return;                         \
+    const MGLv& L = lv[te_.x * MG_MAX_LEVELS + (lvl)];         \
+    const int row = te_.y + (int)threadIdx.x;
+
 // x = w b / diag
-__global__ void __launch_bounds__(BLOCK) k_mg_pre(int n, const double* done, const float* b, const float* diag,
-                                                  float* x) {
-    MG_GATE(done);
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) x[r] = (float)(MG_OMEGA * (double)b[r] / (double)diag[r]);
+__global__ void __launch_bounds__(BLOCK) k_mg_pre(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                  int which, int sc_done) {
+    MG_HEAD(tab, l);
+    if (row >= L.n) return;
+    const float* b = l == 0 ? (which ? cd.ppe.fs : cd.ppe.fp) : L.b;
+    const float* d = l == 0 ? cd.ppe.fdiag : L.diag;
+    L.x[row] = (float)(MG_OMEGA * (double)b[row] / (double)d[row]);
 }
-// r = b - A0 x (level 0: the PPE, or its compact copy ev)
-__global__ void __launch_bounds__(BLOCK) k_mg_res0(PPEView pp, const unsigned* ev, const double* done, const float* b,
-                                                   const float* x, float* r) {
+// r = b - A x
+__global__ void __launch_bounds__(BLOCK) k_mg_res(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                  int which, int sc_done) {
     __shared__ int wbase[BLOCK];
-    MG_GATE(done);
-    const int first = blockIdx.x * BLOCK;
-    load_group_bases(pp, first, wbase);
-    const int row = first + (int)threadIdx.x;
-    if (row >= pp.nr) return;
-    if (ev) {
-        const double s = mg_row16(pp, ev, row, x, wbase + (threadIdx.x & ~63));
-        r[row] = (float)((double)b[row] - (double)pp.fdiag[row] * ((double)x[row] - s));
+    MG_HEAD(tab, l);
+    if (l == 0) {
+        const PPEView& pp = cd.ppe;
+        load_group_bases(pp, te_.y, wbase);
+        if (row >= L.n) return;
+        const float* b = which ? pp.fs : pp.fp;
+        if (L.ev) {
+            const double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
+            L.r[row] = (float)((double)b[row] - (double)pp.fdiag[row] * ((double)L.x[row] - s));
+        } else {
+            L.r[row] = (float)((double)b[row] - spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, L.x, wbase));
+        }
     } else {
-        r[row] = (float)((double)b[row] - spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, x, wbase));
+        if (row >= L.n) return;
+        L.r[row] = (float)((double)L.b[row] - coarse_row(lv_op(L), row, L.x));
     }
 }
-// out = x + w (b - A0 x) / diag
-__global__ void __launch_bounds__(BLOCK) k_mg_post0(PPEView pp, const unsigned* ev, const double* done,
-                                                    const float* b, const float* x, float* out) {
+// z = x + w (b - A x) / diag  (level 0: into p-hat or s-hat)
+__global__ void __launch_bounds__(BLOCK) k_mg_post(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                   int which, int sc_done) {
     __shared__ int wbase[BLOCK];
-    MG_GATE(done);
-    const int first = blockIdx.x * BLOCK;
-    load_group_bases(pp, first, wbase);
-    const int row = first + (int)threadIdx.x;
-    if (row >= pp.nr) return;
-    const double d = (double)pp.fdiag[row];
-    if (ev) {
-        const double s = mg_row16(pp, ev, row, x, wbase + (threadIdx.x & ~63));
-        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] / d - (double)x[row] + s));
+    MG_HEAD(tab, l);
+    if (l == 0) {
+        const PPEView& pp = cd.ppe;
+        load_group_bases(pp, te_.y, wbase);
+        if (row >= L.n) return;
+        const float* b = which ? pp.fs : pp.fp;
+        float* out = which ? pp.fshat : pp.fphat;
+        const double d = (double)pp.fdiag[row];
+        if (L.ev) {
+            const double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
+            out[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)b[row] / d - (double)L.x[row] + s));
+        } else {
+            const double ax = spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, L.x, wbase);
+            out[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)b[row] - ax) / d);
+        }
     } else {
-        const double ax = spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, x, wbase);
-        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] - ax) / d);
+        if (row >= L.n) return;
+        const double ax = coarse_row(lv_op(L), row, L.x);
+        L.y[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)L.b[row] - ax) / (double)L.diag[row]);
     }
 }
-__global__ void __launch_bounds__(BLOCK) k_mg_resc(CoarseOp op, int n, const double* done, const float* b,
-                                                   const float* x, float* r) {
-    MG_GATE(done);
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row < n) r[row] = (float)((double)b[row] - coarse_row(op, row, x));
-}
-__global__ void __launch_bounds__(BLOCK) k_mg_postc(CoarseOp op, int n, const double* done, const float* b,
-                                                    const float* x, float* out) {
-    MG_GATE(done);
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row < n) {
-        const double ax = coarse_row(op, row, x);
-        out[row] = (float)((double)x[row] + MG_OMEGA * ((double)b[row] - ax) / (double)op.diag[row]);
-    }
-}
-// b_next[I] = sum of r over the members of I, in member order
-__global__ void __launch_bounds__(BLOCK) k_mg_restrict(int n1, const double* done, const int* mstart, const int* mem,
-                                                       const float* r, float* b1) {
-    MG_GATE(done);
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= n1) return;
+// b of level l + 1: the sum of r over each aggregate's members, in member
+// order (tab: rows of level l + 1)
+__global__ void __launch_bounds__(BLOCK) k_mg_restrict(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                       int sc_done) {
+    MG_HEAD(tab, l);
+    const MGLv& N = lv[te_.x * MG_MAX_LEVELS + l + 1];
+    if (row >= N.n) return;
     double s = 0.0;
-    for (int t = mstart[I]; t < mstart[I + 1]; t++) s += (double)r[mem[t]];
-    b1[I] = (float)s;
+    for (int t = L.mstart[row]; t < L.mstart[row + 1]; t++) s += (double)L.r[L.mem[t]];
+    N.b[row] = (float)s;
 }
 // x += s z_next[agg]
-__global__ void __launch_bounds__(BLOCK) k_mg_prolong(int n, const double* done, const int* agg, const float* z1,
-                                                      float* x) {
-    MG_GATE(done);
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n) x[r] = (float)((double)x[r] + MG_SCALE * (double)z1[agg[r]]);
+__global__ void __launch_bounds__(BLOCK) k_mg_prolong(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                      int sc_done) {
+    MG_HEAD(tab, l);
+    if (row >= L.n) return;
+    const MGLv& N = lv[te_.x * MG_MAX_LEVELS + l + 1];
+    const float* z = l + 1 == L.lc ? N.x : N.y;
+    L.x[row] = (float)((double)L.x[row] + MG_SCALE * (double)z[L.agg[row]]);
 }
-// x = Ainv b, one wave per row, lanes over the columns, fixed shuffle tree
-__global__ void __launch_bounds__(BLOCK) k_mg_dense(int n, const double* done, const float* ainv, const float* b,
-                                                    float* x) {
-    MG_GATE(done);
-    const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+// coarsest level: x = Ainv b, one wave per row, lanes over the columns,
+// fixed shuffle tree (tab: four rows per block)
+__global__ void __launch_bounds__(BLOCK) k_mg_dense(const CaseDesc* cds, const MGLv* lv, const int2* tab,
+                                                    int sc_done) {
+    const int2 te_ = tab[blockIdx.x];
+    const CaseDesc& cd = cds[te_.x];
+    if (cd.sc[sc_done] != 0.0) return;
+    const int lc = lv[te_.x * MG_MAX_LEVELS].lc;
+    const MGLv& C = lv[te_.x * MG_MAX_LEVELS + lc];
+    const int row = te_.y + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (row >= n) return;
-    const float* a = ainv + (long long)row * n;
+    if (row >= C.n) return;
+    const int n = C.n;
+    const float* a = C.ainv + (long long)row * n;
     double s = 0.0;
-    for (int j = lane; j < n; j += SLICE) s += (double)a[j] * (double)b[j];
+    for (int j = lane; j < n; j += SLICE) s += (double)a[j] * (double)C.b[j];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) x[row] = (float)s;
+    if (lane == 0) C.x[row] = (float)s;
 }
 
 // ------------------------------------------------------------------------
@@ -572,7 +609,9 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
                  ez = ((double)d.z_max - d.z_min) / q + 3.0;
     // level-1 keys are 32-bit
     if ((ex / 2 + 1) * (ey / 2 + 1) * (ez / 2 + 1) > 2.0e9) return 0;
+    static long long next_id = 0;
     MGState* mg = new MGState();
+    mg->id = ++next_id;
     int* err = nullptr;
     int rc = 0;
     auto fail_out = [&](int code) {
@@ -670,49 +709,106 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
     return 0;
 }
 
-// z = M^-1 b on the case's stream (e.st), skipped on the device once *done
-// is set.  spmv_bytes: the compulsory bytes of one product with the PPE (for
-// the profile of the two level-0 passes).
-int mg_apply(Eng& e, const MGState* mg, const PPEView& pp, const double* done, const float* b, float* z,
-             double spmv_bytes) {
-    const int nl = mg->nlev;
-    auto res = [&](int l) -> const float* { return l == nl - 1 ? mg->L[l].x : mg->L[l].y; };
-#define MGE(name, bytes, grid, kern, ...)                                                     \
-    do {                                                                                      \
-        SPH_TRY(prof_begin(e, name, (double)(bytes)));                                        \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, e.st, ##__VA_ARGS__);            \
-        SPH_TRY(launch_check(&e, name));                                                      \
-        SPH_TRY(prof_end(e));                                                                 \
+// Descriptors and block tables of the step's multigrid cases.
+int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool* changed) {
+    (void)sc_done;
+    const int nc = (int)mgs.size();
+    std::vector<long long> sig;
+    for (const MGState* mg : mgs) sig.push_back(mg ? mg->id : 0);
+    *changed = sig != e.mg_sig;
+    if (!*changed) return 0;
+    std::vector<MGLv> lv((size_t)nc * MG_MAX_LEVELS);
+    std::memset(lv.data(), 0, sizeof(MGLv) * lv.size());
+    Eng::MGPlan P;
+    std::vector<int2> tab;
+    for (int k = 0; k < nc; k++) {
+        const MGState* mg = mgs[k];
+        if (!mg) continue;
+        P.lmax = std::max(P.lmax, mg->nlev);
+        for (int l = 0; l < mg->nlev; l++) {
+            const MGLevel& S = mg->L[l];
+            MGLv& D = lv[(size_t)k * MG_MAX_LEVELS + l];
+            D.n = S.n;
+            D.lc = mg->nlev - 1;
+            D.sl_off = S.sl_off; D.cnt = S.cnt; D.cols = S.cols; D.vals = S.vals; D.diag = S.diag;
+            D.agg = S.agg; D.mstart = S.mstart; D.mem = S.mem;
+            D.b = S.b; D.x = S.x; D.y = S.y; D.r = S.r;
+            D.ainv = l == mg->nlev - 1 ? mg->ainv : nullptr;
+            D.ev = l == 0 ? mg->ev : nullptr;
+        }
+    }
+    auto rows_table = [&](int l, bool below, int* off, int* cnt) {
+        *off = (int)tab.size();
+        for (int k = 0; k < nc; k++) {
+            const MGState* mg = mgs[k];
+            if (!mg) continue;
+            const int lc = mg->nlev - 1;
+            if (below ? !(l < lc) : !(l <= lc)) continue;
+            for (int f = 0; f < mg->L[l].n; f += BLOCK) tab.push_back(make_int2(k, f));
+        }
+        *cnt = (int)tab.size() - *off;
+    };
+    for (int l = 0; l < P.lmax; l++) {
+        rows_table(l, true, &P.down_off[l], &P.down_cnt[l]);
+        rows_table(l, false, &P.lvl_off[l], &P.lvl_cnt[l]);
+    }
+    P.dense_off = (int)tab.size();
+    for (int k = 0; k < nc; k++) {
+        const MGState* mg = mgs[k];
+        if (!mg) continue;
+        for (int f = 0; f < mg->L[mg->nlev - 1].n; f += BLOCK / SLICE) tab.push_back(make_int2(k, f));
+    }
+    P.dense_cnt = (int)tab.size() - P.dense_off;
+    // launches in flight may read the old tables
+    SPH_HIP(&e, hipStreamSynchronize(e.st));
+    const size_t lv_bytes = sizeof(MGLv) * lv.size(), tab_bytes = sizeof(int2) * std::max<size_t>(1, tab.size());
+    if (e.mg_lv_cap < lv_bytes) {
+        if (e.mg_lv) (void)hipFree(e.mg_lv);
+        e.mg_lv = nullptr; e.mg_lv_cap = 0;
+        SPH_HIP(&e, hipMalloc(&e.mg_lv, lv_bytes));
+        e.mg_lv_cap = lv_bytes;
+    }
+    if (e.mg_tab_cap < tab_bytes) {
+        if (e.mg_tab) (void)hipFree(e.mg_tab);
+        e.mg_tab = nullptr; e.mg_tab_cap = 0;
+        SPH_HIP(&e, hipMalloc((void**)&e.mg_tab, tab_bytes));
+        e.mg_tab_cap = tab_bytes;
+    }
+    SPH_HIP(&e, hipMemcpy(e.mg_lv, lv.data(), lv_bytes, hipMemcpyHostToDevice));
+    if (!tab.empty()) SPH_HIP(&e, hipMemcpy(e.mg_tab, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
+    P.spmv_bytes0 = e.mg_plan.spmv_bytes0;
+    e.mg_plan = P;
+    e.mg_sig = sig;
+    return 0;
+}
+
+// One V-cycle for every multigrid case of the step (see mg_prepare).
+int mg_apply(Eng& e, int which, int sc_done) {
+    const Eng::MGPlan& P = e.mg_plan;
+    const CaseDesc* cds = e.d_desc;
+    const MGLv* lv = (const MGLv*)e.mg_lv;
+    const int2* T = e.mg_tab;
+#define MGE(name, bytes, tab_off, cnt, kern, ...)                                                    \
+    do {                                                                                             \
+        if ((cnt) > 0) {                                                                             \
+            SPH_TRY(prof_begin(e, name, (double)(bytes)));                                           \
+            hipLaunchKernelGGL(kern, dim3(cnt), dim3(BLOCK), 0, e.st, cds, lv, T + (tab_off), ##__VA_ARGS__); \
+            SPH_TRY(launch_check(&e, name));                                                         \
+            SPH_TRY(prof_end(e));                                                                    \
+        }                                                                                            \
     } while (0)
-    const MGLevel& F = mg->L[0];
-    const int nr = F.n;
-    // down
-    MGE("mg_pre", 12.0 * nr, grid1(nr), k_mg_pre, nr, done, b, (const float*)pp.fdiag, F.x);
-    const double b0 = mg->ev ? spmv_bytes * (2.0 / 3.0) : spmv_bytes;   // 4 instead of 6 bytes per entry
-    MGE("mg_res0", b0, grid1(nr), k_mg_res0, pp, (const unsigned*)mg->ev, done, b, (const float*)F.x, F.r);
-    MGE("mg_restrict", 12.0 * nr, grid1(mg->L[1].n), k_mg_restrict, mg->L[1].n, done, (const int*)F.mstart,
-        (const int*)F.mem, (const float*)F.r, mg->L[1].b);
-    for (int l = 1; l < nl - 1; l++) {
-        const MGLevel& L = mg->L[l];
-        const MGLevel& N = mg->L[l + 1];
-        const CoarseOp op = coarse_op(L);
-        MGE("mg_pre", 12.0 * L.n, grid1(L.n), k_mg_pre, L.n, done, (const float*)L.b, (const float*)L.diag, L.x);
-        MGE("mg_res", 0, grid1(L.n), k_mg_resc, op, L.n, done, (const float*)L.b, (const float*)L.x, L.r);
-        MGE("mg_restrict", 12.0 * L.n, grid1(N.n), k_mg_restrict, N.n, done, (const int*)L.mstart, (const int*)L.mem,
-            (const float*)L.r, N.b);
+    for (int l = 0; l + 1 < P.lmax; l++) {
+        MGE("mg_pre", 0, P.down_off[l], P.down_cnt[l], k_mg_pre, l, which, sc_done);
+        MGE(l == 0 ? "mg_res0" : "mg_res", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_res, l,
+            which, sc_done);
+        MGE("mg_restrict", 0, P.lvl_off[l + 1], P.lvl_cnt[l + 1], k_mg_restrict, l, sc_done);
     }
-    const MGLevel& C = mg->L[nl - 1];
-    MGE("mg_dense", 4.0 * C.n * C.n, grid1((long long)C.n * SLICE), k_mg_dense, C.n, done, (const float*)mg->ainv,
-        (const float*)C.b, C.x);
-    // up
-    for (int l = nl - 2; l >= 1; l--) {
-        const MGLevel& L = mg->L[l];
-        const CoarseOp op = coarse_op(L);
-        MGE("mg_prolong", 16.0 * L.n, grid1(L.n), k_mg_prolong, L.n, done, (const int*)L.agg, res(l + 1), L.x);
-        MGE("mg_post", 0, grid1(L.n), k_mg_postc, op, L.n, done, (const float*)L.b, (const float*)L.x, L.y);
+    MGE("mg_dense", 0, P.dense_off, P.dense_cnt, k_mg_dense, sc_done);
+    for (int l = P.lmax - 2; l >= 0; l--) {
+        MGE("mg_prolong", 0, P.down_off[l], P.down_cnt[l], k_mg_prolong, l, sc_done);
+        MGE(l == 0 ? "mg_post0" : "mg_post", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_post,
+            l, which, sc_done);
     }
-    MGE("mg_prolong", 16.0 * nr, grid1(nr), k_mg_prolong, nr, done, (const int*)F.agg, res(1), F.x);
-    MGE("mg_post0", b0, grid1(nr), k_mg_post0, pp, (const unsigned*)mg->ev, done, b, (const float*)F.x, z);
 #undef MGE
     return 0;
 }

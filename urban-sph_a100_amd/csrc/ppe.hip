@@ -1139,14 +1139,14 @@ static bool any_live(const Eng& e, bool half) {
 }
 
 // One BiCGSTAB iteration of every unfinished case.  mg: the multigrid
-// preconditioner of the single case of the step (float vectors), which
-// replaces the diagonal scaling of p-hat and s-hat.  With the profile on
+// cases of the step (float vectors, tables from mg_prepare) replace the
+// diagonal scaling of p-hat and s-hat by one V-cycle.  With the profile on
 // (launches not captured) the host reads the scalars after every finalize
 // and leaves out the launches whose device gate would return at once, so
 // per-launch times count only launches that do work; the results are the
 // same.
 template <class V, class X>
-static int kr_iteration(Eng& e, double spmv_bytes, const MGState* mg) {
+static int kr_iteration(Eng& e, double spmv_bytes, bool mg) {
     const bool pf = e.prof.on;
     auto live = [&](bool half) { return !pf || any_live(e, half); };
 #define KFP(op)                                   \
@@ -1156,10 +1156,7 @@ static int kr_iteration(Eng& e, double spmv_bytes, const MGState* mg) {
     } while (0)
     if (live(false)) {
         RL("kr_p", 0, k_kr_p<X>);
-        if (mg) {
-            const PPEView pv = ppe_view(e.cs[0]);
-            SPH_TRY(mg_apply(e, mg, pv, e.d_sc + SC_DONE, pv.fp, pv.fphat, spmv_bytes));
-        }
+        if (mg) SPH_TRY(mg_apply(e, 0, SC_DONE));
         RLONG(V, X, 1);
         RL("spmv", spmv_bytes, (k_kr_spmv<V, X, false>));
     }
@@ -1167,10 +1164,7 @@ static int kr_iteration(Eng& e, double spmv_bytes, const MGState* mg) {
     if (live(false)) RL("kr_s", 0, k_kr_s<X>);
     KFP(FIN_S);
     if (live(false)) {
-        if (mg) {
-            const PPEView pv = ppe_view(e.cs[0]);
-            SPH_TRY(mg_apply(e, mg, pv, e.d_sc + SC_DONE, pv.fs, pv.fshat, spmv_bytes));
-        }
+        if (mg) SPH_TRY(mg_apply(e, 1, SC_DONE));
         RLONG(V, X, 2);
         RL("spmv", spmv_bytes, (k_kr_spmv<V, X, true>));
     }
@@ -1184,7 +1178,7 @@ static int kr_iteration(Eng& e, double spmv_bytes, const MGState* mg) {
 // The iteration as a HIP graph, captured once per composition of the step
 // (its launches read everything else from the device descriptors).
 template <class V, class X>
-static int kr_graph(Eng& e, int gi, double spmv_bytes, const MGState* mg, hipGraphExec_t* out) {
+static int kr_graph(Eng& e, int gi, double spmv_bytes, bool mg, hipGraphExec_t* out) {
     if (!e.iter_graph[gi]) {
         Shape sh;
         SPH_TRY(shape(e, 1, BLOCK, &sh));   // segment table exists before the capture
@@ -1211,7 +1205,7 @@ static int kr_graph(Eng& e, int gi, double spmv_bytes, const MGState* mg, hipGra
 // with the tolerances and iteration caps already in the controls.
 // iters[k] receives case k's iteration count.
 template <class V, class X>
-static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, const MGState* mg = nullptr) {
+static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, bool mg = false) {
     const int nc = (int)e.cs.size();
     double spmv_bytes = 0.0;
     int maxit = 0;
@@ -1228,6 +1222,8 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, cons
                       (double)P->nr * (4.0 + diag_b + 3.0 * sizeof(X) + (i16 ? 4.0 : 0.0));
         maxit = std::max(maxit, (int)e.h_ctl[k * CTL_COUNT + CTL_MAXIT]);
     }
+    // the V-cycle's level-0 passes read 4 bytes per entry (compact copy)
+    if (mg) e.mg_plan.spmv_bytes0 = spmv_bytes * (2.0 / 3.0);
     RLONG(V, X, 0);
     RL("kr_init", spmv_bytes, (k_kr_init<V, X>));
     KF(FIN_INIT);
@@ -1347,22 +1343,20 @@ int ppe_solve(Eng& e) {
             c->rep.rel_residual = rel_residual(sc[SC_RNORM], sc[SC_BNORM]);
         }
     } else {
-        // the multigrid preconditioner, for a single case that asks for it;
-        // its hierarchy is built once per set of values (with fixed
-        // particles, at the first step of a search)
-        const MGState* mg = nullptr;
-        if (nc == 1 && e.cs[0]->prm.ppe_precond == SPH_PRECOND_MULTIGRID && e.cs[0]->ppe) {
-            sph_ctx* c = e.cs[0];
+        // the multigrid preconditioner of every case that asks for it; a
+        // hierarchy is built once per set of values (with fixed particles, at
+        // the first step of a search)
+        std::vector<const MGState*> mgs(nc, nullptr);
+        bool any_mg = false;
+        for (int k = 0; k < nc; k++) {
+            sph_ctx* c = e.cs[k];
             PPEState* P = c->ppe;
+            if (c->prm.ppe_precond != SPH_PRECOND_MULTIGRID || !P) continue;
             const bool fresh = !c->prm.ale_positions && c->vals_gen == c->generation;
             if (!fresh || P->mg_state == 0) {
                 mg_free(c, P->mg);
                 P->mg = nullptr;
                 P->mg_state = -1;
-                if (e.iter_graph[2]) {   // its launches hold the old hierarchy
-                    (void)hipGraphExecDestroy(e.iter_graph[2]);
-                    e.iter_graph[2] = nullptr;
-                }
                 if (P->nr > MG_COARSE_MAX && P->nlong == 0) {
                     double q = 0.0;
                     int ok = 0;
@@ -1371,8 +1365,18 @@ int ppe_solve(Eng& e) {
                     P->mg_state = ok ? 1 : -1;
                 }
             }
-            if (P->mg_state == 1) mg = P->mg;
+            if (P->mg_state == 1) { mgs[k] = P->mg; any_mg = true; }
         }
+        auto mg_tables = [&]() -> int {
+            bool changed = false;
+            SPH_TRY(mg_prepare(e, mgs, SC_DONE, &changed));
+            if (changed && e.iter_graph[2]) {   // its launches hold the old tables
+                (void)hipGraphExecDestroy(e.iter_graph[2]);
+                e.iter_graph[2] = nullptr;
+            }
+            return 0;
+        };
+        if (any_mg) SPH_TRY(mg_tables());
         // outer FP64 iterative refinement around float inner solves; each
         // case refines until its own tolerance is met
         std::vector<char> active(nc);
@@ -1416,22 +1420,31 @@ int ppe_solve(Eng& e) {
             SPH_TRY(ctl_upload(e));
             if (!any) break;
             RL("zero_fx", 0, k_zero_fx);
-            SPH_TRY((bicgstab<float, float>(e, mg ? 2 : 1, outer, iters, mg)));
-            for (int k = 0; k < nc; k++)
-                if (active[k]) { e.cs[k]->rep.iterations += iters[k]; refine[k]++; }
-            if (mg) {
-                sph_ctx* c = e.cs[0];
+            SPH_TRY((bicgstab<float, float>(e, any_mg ? 2 : 1, outer, iters, any_mg)));
+            bool stalled = false;
+            for (int k = 0; k < nc; k++) {
+                if (!active[k]) continue;
+                sph_ctx* c = e.cs[k];
+                c->rep.iterations += iters[k];
+                refine[k]++;
+                if (!mgs[k]) continue;
                 c->rep.precond = SPH_PRECOND_MULTIGRID;
-                c->rep.mg_levels = mg_levels(mg);
-                c->rep.mg_coarsest = mg_coarsest(mg);
+                c->rep.mg_levels = mg_levels(mgs[k]);
+                c->rep.mg_coarsest = mg_coarsest(mgs[k]);
                 // an inner solve that stalled (breakdown or the iteration
                 // cap) with finite residuals: Jacobi from here on, for the
                 // rest of this search
-                const double* sc = e.h_sc;
+                const double* sc = e.h_sc + k * SC_COUNT;
                 if (sc[SC_CONV] == 0.0 && std::isfinite(sc[SC_RNORM])) {
                     c->ppe->mg_state = -1;
-                    mg = nullptr;
+                    mgs[k] = nullptr;
+                    stalled = true;
                 }
+            }
+            if (stalled) {
+                any_mg = false;
+                for (const MGState* m : mgs) any_mg |= m != nullptr;
+                if (any_mg) SPH_TRY(mg_tables());
             }
             RL("add_corr", 0, k_add_correction);
         }

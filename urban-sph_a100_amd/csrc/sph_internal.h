@@ -252,6 +252,20 @@ struct Eng {
     // refinement pass): the host polls for convergence only near them
     static constexpr int PRED_SLOTS = 4;
     int pred_iters[3][PRED_SLOTS] = {};
+    // multigrid V-cycles of the step's cases (mg.hip): per case and level a
+    // descriptor, and block tables of the level rows (mg_plan)
+    void* mg_lv = nullptr;            // device MGLv[case * MG_MAX_LEVELS + level]
+    size_t mg_lv_cap = 0;
+    int2* mg_tab = nullptr;           // device block tables (case, first row)
+    size_t mg_tab_cap = 0;
+    std::vector<long long> mg_sig;    // hierarchies the tables describe
+    struct MGPlan {
+        int lmax = 0;                 // most levels of any case
+        int down_off[16], down_cnt[16];   // rows of level l, cases with a level below l
+        int lvl_off[16], lvl_cnt[16];     // rows of level l, cases that have level l
+        int dense_off = 0, dense_cnt = 0; // coarsest rows, four per block
+        double spmv_bytes0 = 0.0;         // level-0 bytes of one pass (profile)
+    } mg_plan;
 };
 
 }  // namespace sph
@@ -393,14 +407,20 @@ int pack_records(Eng& e, int which);
 // aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
 struct MGState;
 constexpr int MG_COARSE_MAX = 1024;   // unknowns of the densely inverted coarsest level
+constexpr int MG_MAX_LEVELS = 12;
 // Builds the hierarchy of the PPE in pp (float values, `slots` stored SELL
 // slots) with lattice spacing q; *ok = 0 (and *out = null) when the case
 // does not suit it.
 int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok);
 bool mg_compact(const MGState* mg);   // the V-cycle's level 0 reads the half-precision copy
-// z = one V-cycle applied to b, on e.st; skipped once *done != 0
-int mg_apply(Eng& e, const MGState* mg, const PPEView& pp, const double* done, const float* b, float* z,
-             double spmv_bytes);
+// Tables and descriptors of the V-cycles of the step's cases (mgs[k]: case
+// k's hierarchy or null for a case that uses Jacobi); *changed when they
+// differ from the previous call's.
+int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool* changed);
+// One V-cycle for every multigrid case of the step: which = 0 p -> p-hat,
+// 1 s -> s-hat (float vectors); a case's launches return at once when its
+// done flag (scalar sc_done) is set.
+int mg_apply(Eng& e, int which, int sc_done);
 void mg_free(sph_ctx* c, MGState* mg);
 int mg_levels(const MGState* mg);
 int mg_coarsest(const MGState* mg);
