@@ -297,6 +297,8 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.vals_fresh = !c->prm.ale_positions && c->vals_gen == c->generation;
     d.nm_list = c->nm_list;
     d.nm_count = c->nm_count;
+    d.nmp_off = c->nmp_off;
+    d.nmp_ent = c->nmp_ent;
     d.dt = dt;
     d.time = time;
     d.kc = KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15};
@@ -332,6 +334,10 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
         } else if (!c->ppe) {
             SPH_TRY(ppe_build(c));
         }
+        // the Neumann pass's pairs depend on positions, types and the search
+        const int prec = c->prm.precision;
+        if (c->nmp_gen != c->generation || c->nmp_for != c->nm_list || c->nmp_prec != prec || c->prm.ale_positions)
+            SPH_TRY(neumann_pairs(c));
     }
     // composition of the step: tables and graphs stay valid while it holds
     std::vector<long long> sig;

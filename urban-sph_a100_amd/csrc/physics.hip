@@ -420,6 +420,94 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_most_deposit(Seg sg) {
 // ------------------------------------------------------------------------
 // KERNEL_Neumann_boundary3D, functions_BC.cuh:290-446.  Pressures are read
 // from a snapshot (pold) taken before the launch.
+// The window of the Neumann scan around cell c (ncell nc), in scan order
+// (z-major rows of x-cells, the particles of a row in sorted order), as
+// chunks of 64 candidates: f(j, count) with lane l holding candidate l of
+// the chunk (j < 0: none) and count the chunk's candidates.  Every lane
+// makes the same calls.
+template <class F>
+__device__ __forceinline__ void neumann_window(const Grid& g, const Cell& c, int nc, int lane, F&& f) {
+    const int side = 2 * nc + 1, nrows = side * side;
+    if (nrows <= 64) {
+        // Lane k holds x-row k of the window (z-major, then y): its particle
+        // range and the exclusive prefix of the lengths, so the window's
+        // candidates are numbered v = 0 .. total-1 in scan order.
+        int rs = 0, rl = 0;
+        if (lane < nrows) {
+            int2 rg;
+            if (row_range(g, c.i - nc, c.i + nc, c.j + lane % side - nc, c.k + lane / side - nc, rg)) {
+                rs = rg.x;
+                rl = rg.y - rg.x;
+            }
+        }
+        int incl = rl;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        const int excl = incl - rl, total = __shfl(incl, 63, 64);
+        for (int v0 = 0; v0 < total; v0 += 64) {
+            const int v = v0 + lane, vv = min(v, total - 1);
+            // the row holding vv: the last row whose prefix is <= vv (never
+            // an empty row); every lane takes part in every shuffle
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = lo + step;
+                const int pe = __shfl(excl, min(cand, 63), 64);
+                if (cand < nrows && pe <= vv) lo = cand;
+            }
+            const int start = __shfl(rs, lo, 64), pre = __shfl(excl, lo, 64);
+            f(v < total ? start + (vv - pre) : -1, min(64, total - v0));
+        }
+    } else {
+        for (int z = -nc; z <= nc; z++)
+            for (int y = -nc; y <= nc; y++) {
+                int2 rg;
+                if (!row_range(g, c.i - nc, c.i + nc, c.j + y, c.k + z, rg)) continue;
+                for (int b = rg.x; b < rg.y; b += 64) f(b + lane < rg.y ? b + lane : -1, min(64, rg.y - b));
+            }
+    }
+}
+
+// Neumann pairs of one boundary particle per 64-thread block: the
+// candidates of its window that carry weight in the Neumann sums (fluid,
+// non-buffer, inside the 2h search range, with the pass's own precision R
+// for the distance), in scan order.  COUNT: their number; FILL: the list.
+template <class R, bool FILL>
+__global__ void __launch_bounds__(64) k_nmp_build(PView A, Grid g, KConst kc, double L2, const int* nm_list,
+                                                  int nm_count, int* cnt, const long long* off, int* ent) {
+    const int u = blockIdx.x, lane = threadIdx.x;
+    if (u >= nm_count) return;
+    const int i = nm_list[u];
+    const R hi = (R)(2.0 * A.h[i]);
+    const R search_range = (R)kc.kappa * hi;
+    const double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    const int bt = A.buffer_type[i];
+    const double xg = (bt == 1 || bt == -1) ? 2.0 * L2 - xi : xi;
+    const Cell c = query_cell(g.d, xg, yi, zi);
+    const long long at = FILL ? off[u] : 0;
+    int n = 0;
+    neumann_window(g, c, A.ncell[i], lane, [&](int j, int count) {
+        (void)count;
+        bool hit = false;
+        if (j >= 0 && A.p_type[j] == 1 && A.buffer_type[j] == 0) {
+            R dx = (R)(xg - A.x[j]), dy = (R)(yi - A.y[j]), dz = (R)(zi - A.z[j]);
+            R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+            hit = tdist < search_range;
+        }
+        const unsigned long long m = __ballot(hit);
+        if (FILL && hit) ent[at + n + __popcll(m & ((1ULL << lane) - 1ULL))] = j;
+        n += __popcll(m);
+    });
+    if (!FILL && lane == 0) cnt[u] = n;
+}
+__global__ void __launch_bounds__(BLOCK) k_nmp_widen(int n, const int* a, long long* b) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) b[r] = a[r];
+}
+
 template <class R>
 __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_count0) {
     // one 64-lane wave per boundary particle of the case's compact list
@@ -476,14 +564,15 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
     __shared__ int use[64];
     double acc = 0.0;   // lane k < 8 holds sum k
     const int nt = inlet ? 8 : 5;
+    const bool listed = cd.nmp_off != nullptr;
     // one chunk of candidates v0 .. v0 + 63 of the window's flattened order:
     // lane l holds candidate v0 + l (particle j, or j < 0 for none)
     auto chunk = [&](int j, int count) {
         int ok = 0;
         // only fluid, non-buffer neighbours carry weight (the reference
         // multiplies the others by 0; skipping those exact zeros leaves every
-        // sum bit for bit the same)
-        if (j >= 0 && A.p_type[j] == 1 && A.buffer_type[j] == 0) {
+        // sum bit for bit the same); the pair list holds only those
+        if (j >= 0 && (listed || (A.p_type[j] == 1 && A.buffer_type[j] == 0))) {
             R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
             R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
             if (tdist < search_range) {
@@ -513,48 +602,15 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
                 if (use[q]) acc += term[lane][q];
         __syncthreads();
     };
-    const int nc = A.ncell[i];
-    const int side = 2 * nc + 1, nrows = side * side;
-    if (nrows <= 64) {
-        // Lane k holds x-row k of the window (z-major, then y): its particle
-        // range and the exclusive prefix of the lengths, so the window's
-        // candidates are numbered v = 0 .. total-1 in scan order.
-        int rs = 0, rl = 0;
-        if (lane < nrows) {
-            int2 rg;
-            if (row_range(g, c.i - nc, c.i + nc, c.j + lane % side - nc, c.k + lane / side - nc, rg)) {
-                rs = rg.x;
-                rl = rg.y - rg.x;
-            }
-        }
-        int incl = rl;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int o = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += o;
-        }
-        const int excl = incl - rl, total = __shfl(incl, 63, 64);
-        for (int v0 = 0; v0 < total; v0 += 64) {
-            const int v = v0 + lane, vv = min(v, total - 1);
-            // the row holding vv: the last row whose prefix is <= vv (never
-            // an empty row); every lane takes part in every shuffle
-            int lo = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int cand = lo + step;
-                const int pe = __shfl(excl, min(cand, 63), 64);
-                if (cand < nrows && pe <= vv) lo = cand;
-            }
-            const int start = __shfl(rs, lo, 64), pre = __shfl(excl, lo, 64);
-            chunk(v < total ? start + (vv - pre) : -1, min(64, total - v0));
+    if (cd.nmp_off) {
+        // the pairs of this search (the window's weighted candidates, scan order)
+        const long long e0 = cd.nmp_off[u], e1 = cd.nmp_off[u + 1];
+        for (long long b = e0; b < e1; b += 64) {
+            const long long e = b + lane;
+            chunk(e < e1 ? cd.nmp_ent[e] : -1, (int)min((long long)64, e1 - b));
         }
     } else {
-        for (int z = -nc; z <= nc; z++)
-            for (int y = -nc; y <= nc; y++) {
-                int2 rg;
-                if (!row_range(g, c.i - nc, c.i + nc, c.j + y, c.k + z, rg)) continue;
-                for (int b = rg.x; b < rg.y; b += 64) chunk(b + lane < rg.y ? b + lane : -1, min(64, rg.y - b));
-            }
+        neumann_window(g, c, A.ncell[i], lane, chunk);
     }
     const double tpres = __shfl(acc, 0, 64), tux = __shfl(acc, 1, 64), tuy = __shfl(acc, 2, 64);
     const double tuz = __shfl(acc, 3, 64), tflt_s = __shfl(acc, 4, 64);
@@ -1126,6 +1182,54 @@ __global__ void k_stats_final(const unsigned* in, float* out) {
 static bool mixed_of(const Eng& e) { return e.cs[0]->prm.precision == SPH_MIXED; }
 
 // which: 0 = step working copy (SP1), 1 = persistent state (P1)
+// Neumann pairs of the case's boundary particles for its current search,
+// in the precision of its steps.
+int neumann_pairs(sph_ctx* c) {
+    dfree(c, c->nmp_off);
+    dfree(c, c->nmp_ent);
+    c->nmp_off = nullptr;
+    c->nmp_ent = nullptr;
+    const int m = c->nm_count;
+    if (m > 0 && c->nm_list) {
+        const bool mixed = c->prm.precision == SPH_MIXED;
+        const KConst kc{c->prm.kernel_type, c->prm.kappa, c->c_pi15};
+        int* cnt = nullptr;
+        long long* wide = nullptr;
+        long long total = 0;
+        SPH_TRY(dalloc(c, &cnt, m));
+        SPH_TRY(dalloc(c, &wide, m));
+        SPH_TRY(dalloc(c, &c->nmp_off, (size_t)m + 1));
+        if (mixed)
+            hipLaunchKernelGGL((k_nmp_build<float, false>), dim3(m), dim3(64), 0, c->stream, c->pview(), c->grid(), kc,
+                               c->prm.L2, (const int*)c->nm_list, m, cnt, (const long long*)nullptr, (int*)nullptr);
+        else
+            hipLaunchKernelGGL((k_nmp_build<double, false>), dim3(m), dim3(64), 0, c->stream, c->pview(), c->grid(),
+                               kc, c->prm.L2, (const int*)c->nm_list, m, cnt, (const long long*)nullptr, (int*)nullptr);
+        SPH_TRY(launch_check(c, "neumann_pairs_count"));
+        hipLaunchKernelGGL(k_nmp_widen, dim3(grid1(m)), dim3(BLOCK), 0, c->stream, m, (const int*)cnt, wide);
+        SPH_TRY(launch_check(c, "neumann_pairs_widen"));
+        SPH_TRY(exclusive_scan_i64(c, wide, c->nmp_off, m, &total));
+        SPH_HIP(c, hipMemcpyAsync(c->nmp_off + m, &total, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+        SPH_TRY(dalloc(c, &c->nmp_ent, (size_t)std::max(1LL, total)));
+        if (mixed)
+            hipLaunchKernelGGL((k_nmp_build<float, true>), dim3(m), dim3(64), 0, c->stream, c->pview(), c->grid(), kc,
+                               c->prm.L2, (const int*)c->nm_list, m, (int*)nullptr, (const long long*)c->nmp_off,
+                               c->nmp_ent);
+        else
+            hipLaunchKernelGGL((k_nmp_build<double, true>), dim3(m), dim3(64), 0, c->stream, c->pview(), c->grid(),
+                               kc, c->prm.L2, (const int*)c->nm_list, m, (int*)nullptr, (const long long*)c->nmp_off,
+                               c->nmp_ent);
+        SPH_TRY(launch_check(c, "neumann_pairs_fill"));
+        SPH_HIP(c, hipStreamSynchronize(c->stream));   // total lives on this frame
+        dfree(c, cnt);
+        dfree(c, wide);
+    }
+    c->nmp_gen = c->generation;
+    c->nmp_for = c->nm_list;
+    c->nmp_prec = c->prm.precision;
+    return 0;
+}
+
 static int run_neumann(Eng& e, int which, int only_count0) {
     SEGL("snapshot", -1, k_snapshot, which);
     // one 64-thread block (one wave) per boundary particle of each case

@@ -117,6 +117,8 @@ struct CaseDesc {
     int vals_fresh;      // the PPE values and diagonal hold this search's values (fixed positions)
     const int* nm_list;  // particles the Neumann pass updates (p_type != 1), in sorted order
     int nm_count;
+    const long long* nmp_off;   // per nm_list particle: its first Neumann pair (nm_count + 1)
+    const int* nmp_ent;         // Neumann pairs: the fluid neighbours inside the 2h search range, scan order
     double dt, time;
     KConst kc;
     sph_params pr;
@@ -307,6 +309,11 @@ struct sph_ctx {
     int* cstart = nullptr;
     int* nm_list = nullptr;   // non-fluid particles of the search (Neumann pass)
     int nm_count = 0;
+    long long* nmp_off = nullptr;   // Neumann pairs of nm_list (built by neumann_pairs)
+    int* nmp_ent = nullptr;
+    long long nmp_gen = -1;         // generation, list and precision they were built for
+    const int* nmp_for = nullptr;
+    int nmp_prec = -1;
     int* sort_hist = nullptr; int sort_hist_n = 0;
     long long* nl_slice_off = nullptr; int* nl_cnt = nullptr; unsigned* nl_ent = nullptr;
     long long nl_total = 0;
@@ -404,6 +411,8 @@ PPEView ppe_view(const sph_ctx* c);
 // the pairs that hold velocities and pressures (REC_VEL_PQ) or pressures (REC_PQ)
 enum { REC_ALL = 0, REC_VEL_PQ = 1, REC_PQ = 2 };
 int pack_records(Eng& e, int which);
+// Neumann pairs of a case for its current search and precision (physics.hip)
+int neumann_pairs(sph_ctx* c);
 // aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
 struct MGState;
 constexpr int MG_COARSE_MAX = 1024;   // unknowns of the densely inverted coarsest level
