@@ -26,6 +26,7 @@
 // captured iteration costs nothing after convergence, and a case's V-cycle
 // does not depend on the other cases of the step.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -609,11 +610,10 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
                  ez = ((double)d.z_max - d.z_min) / q + 3.0;
     // level-1 keys are 32-bit
     if ((ex / 2 + 1) * (ey / 2 + 1) * (ez / 2 + 1) > 2.0e9) return 0;
-    static long long next_id = 0;
+    static std::atomic<long long> next_id{0};
     MGState* mg = new MGState();
     mg->id = ++next_id;
     int* err = nullptr;
-    int rc = 0;
     auto fail_out = [&](int code) {
         dfree(c, err);
         mg_free(c, mg);
@@ -624,6 +624,10 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         int rc_ = (x);                    \
         if (rc_) return fail_out(rc_);    \
     } while (0)
+    // a HIP error of the build: reported, and the partial hierarchy freed
+    auto hipc = [&](hipError_t er, const char* what) {
+        return er == hipSuccess ? 0 : fail(c, SPH_E_HIP, std::string(what) + ": " + hipGetErrorString(er));
+    };
     MGTRY(dalloc(c, &err, 1));
     {
         MGLevel& L0 = mg->L[0];
@@ -645,7 +649,7 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         if (nlev == MG_MAX_LEVELS) { good = false; break; }
         MGLevel& L0 = mg->L[nlev - 1];
         MGLevel& L1 = mg->L[nlev];
-        SPH_HIP(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        MGTRY(hipc(hipMemsetAsync(err, 0, sizeof(int), c->stream), "mg: clear flag"));
         MGTRY(mg_aggregate(c, L0, L1));
         if (L1.n * 10LL > L0.n * 9LL) { nlev++; good = false; break; }   // coarsening stalled
         int lok = 0;
@@ -656,7 +660,6 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
     }
     mg->nlev = nlev;
     if (!good || nlev < 2) return fail_out(0);
-    (void)rc;
     // dense inverse of the coarsest level
     {
         MGLevel& LC = mg->L[nlev - 1];
@@ -666,7 +669,7 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         MGTRY(dalloc(c, &a, (size_t)nn));
         MGTRY(dalloc(c, &rowk, n));
         MGTRY(dalloc(c, &colk, n));
-        SPH_HIP(c, hipMemsetAsync(err, 0, sizeof(int), c->stream));
+        MGTRY(hipc(hipMemsetAsync(err, 0, sizeof(int), c->stream), "mg: coarsest level"));
         hipLaunchKernelGGL(k_mg_dense_zero, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, nn, a);
         MGTRY(launch_check(c, "mg_dense_zero"));
         hipLaunchKernelGGL(k_mg_dense_fill, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, coarse_op(LC), n, a);
@@ -683,8 +686,8 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         hipLaunchKernelGGL(k_mg_tofloat, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, nn, (const double*)a, mg->ainv);
         MGTRY(launch_check(c, "mg_tofloat"));
         int h_err = 0;
-        SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        SPH_HIP(c, hipStreamSynchronize(c->stream));
+        MGTRY(hipc(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "mg: coarsest level"));
+        MGTRY(hipc(hipStreamSynchronize(c->stream), "mg: coarsest level"));
         dfree(c, a);
         dfree(c, rowk);
         dfree(c, colk);
