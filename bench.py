@@ -350,6 +350,48 @@ def cpu_baseline(cases, abi, dt, tol, max_seconds=30.0):
                        f"FP64 BiCGSTAB tol {tol:g}, median of {len(times)} steps after 1 warm-up")
 
 
+def run_box_cfg0(cases, abi, urbansph, oracle_too=True, gpu_steps=20):
+    """The reference's CPU-runnable case: the 32^3 FP64 box (dx = 0.01 m,
+    jitter 0.05 dx, Wendland C2, h = 1.5 dx, 3 wall layers, dt = 1e-4 s,
+    PPE tolerance 1e-8), FP64 on the GPU and the oracle on 1 host thread and
+    on all of them (median of 3 steps after the searching first step)."""
+    P, nrm = cases.box(nf=32, walls=3)
+    prm = cases.box_params(urbansph.default_params(), tol=1e-8)
+    dt = 1e-4
+    g = urbansph.Case(prm, len(P))
+    g.upload(P, None, nrm)
+    for c in range(3):
+        g.step(c, dt)
+    g.sync()
+    t0 = time.perf_counter()
+    for c in range(3, 3 + gpu_steps):
+        g.step(c, dt)
+    g.sync()
+    gpu = len(P) * gpu_steps / (time.perf_counter() - t0)
+    g.close()
+    out = {"workload": "32^3 FP64 box, dx=0.01 m, jitter 0.05 dx, Wendland C2, 3 wall layers, dt=1e-4 s, "
+                       "BiCGSTAB tol 1e-8", "particles": len(P), "gpu_fp64": round(gpu, 1),
+           "unit": "particle-steps/s"}
+    if oracle_too:
+        import oracle
+        oprm = cases.box_params(oracle.default_params(), tol=1e-8)
+        every = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        for label, threads in (("cpu_all_threads", every), ("cpu_1_thread", 1)):
+            used = oracle.set_threads(threads)
+            o = oracle.Oracle(oprm, len(P))
+            o.upload(P, None, nrm)
+            o.step(0, dt)
+            times = []
+            for c in range(1, 4):
+                a = time.perf_counter()
+                o.step(c, dt)
+                times.append(time.perf_counter() - a)
+            o.close()
+            out[label] = {"value": round(len(P) / float(np.median(times)), 1), "threads": used}
+        oracle.set_threads(every)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -543,6 +585,12 @@ def main():
         line["cpu_baseline"] = cpu_baseline(cases, abi, args.dt, args.tol)
         log(f"[bench] cpu baseline {line['cpu_baseline']['value'] / 1e3:.1f} k particle-steps/s "
             f"on {line['cpu_baseline']['cores']} threads")
+        log("[bench] 32^3 box (the reference's CPU case): GPU FP64 and the oracle on 1 and all threads ...")
+        line["box_cfg0"] = run_box_cfg0(cases, abi, urbansph)
+        b0 = line["box_cfg0"]
+        log(f"[bench] box: GPU FP64 {b0['gpu_fp64'] / 1e6:.2f} M, oracle {b0['cpu_all_threads']['value'] / 1e3:.1f} k "
+            f"({b0['cpu_all_threads']['threads']} threads), {b0['cpu_1_thread']['value'] / 1e3:.1f} k (1 thread) "
+            f"particle-steps/s")
     print(json.dumps(line), flush=True)
 
 
