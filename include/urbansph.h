@@ -260,7 +260,26 @@ typedef struct sph_ppe_report {
     int    mg_levels;           /* multigrid levels incl. the PPE itself and
                                    the coarsest (0 without multigrid)         */
     int    mg_coarsest;         /* unknowns of the coarsest level             */
+    int    mg_status;           /* SPH_MG_*: why a multigrid that was asked
+                                   for is not in use (SPH_MG_OK otherwise)   */
 } sph_ppe_report;
+
+/* Why the multigrid preconditioner was not used (sph_ppe_report.mg_status). */
+enum {
+    SPH_MG_OK = 0,
+    SPH_MG_UNSUITED = 1,        /* at most 1024 unknowns, long rows with 32-bit
+                                   columns, or a lattice beyond 32-bit keys  */
+    SPH_MG_TERMS = 2,           /* a coarse row gathers more than 8192 terms
+                                   or 64 members                              */
+    SPH_MG_COLUMNS = 3,         /* a coarse row has more than 2048 columns    */
+    SPH_MG_SPAN = 4,            /* a coarse row's columns lie too far apart   */
+    SPH_MG_PIVOT = 5,           /* zero or non-finite pivot of the coarsest
+                                   level                                      */
+    SPH_MG_STALLED = 6,         /* a level kept more than 90 % of the unknowns
+                                   of the one before                          */
+    SPH_MG_LEVELS = 7,          /* more than 12 levels                        */
+    SPH_MG_STALLED_SOLVE = 8    /* an inner solve stalled with it             */
+};
 /* PPE row numbering: the particles' cell order or lattice (z, y, x) order
  * (see ppe_reorder). */
 enum { SPH_ORDER_CELL = 0, SPH_ORDER_LATTICE = 1 };

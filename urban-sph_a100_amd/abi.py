@@ -113,11 +113,14 @@ class PPEReport(C.Structure):
                 ("unknowns", C.c_int), ("nnz", C.c_longlong),
                 ("rel_residual", C.c_double), ("converged", C.c_int),
                 ("ordering", C.c_int), ("slots", C.c_longlong), ("index16", C.c_int), ("tail_rows", C.c_int),
-                ("precond", C.c_int), ("mg_levels", C.c_int), ("mg_coarsest", C.c_int)]
+                ("precond", C.c_int), ("mg_levels", C.c_int), ("mg_coarsest", C.c_int),
+                ("mg_status", C.c_int)]
 
 
 ORDER_CELL, ORDER_LATTICE = 0, 1
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
+(MG_OK, MG_UNSUITED, MG_TERMS, MG_COLUMNS, MG_SPAN, MG_PIVOT, MG_STALLED, MG_LEVELS,
+ MG_STALLED_SOLVE) = range(9)
 
 
 class KernelTime(C.Structure):

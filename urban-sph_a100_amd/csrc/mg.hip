@@ -39,8 +39,13 @@
 namespace sph {
 
 static_assert(MG_COARSE_MAX <= 1024, "the coarsest level is inverted densely");
-constexpr int MG_ENT_CAP = 2048;    // terms of one coarse row (members' entries + diagonals)
-constexpr int MG_QMAX = 8;          // distinct columns of one coarse row: at most 64 * MG_QMAX
+// Galerkin rows: the common kernel takes a row of up to 2048 terms (members'
+// entries + diagonals) and 512 distinct columns; rows beyond that (coarse
+// particles beside refined ones) go to a second kernel with room for 8192
+// terms, 64 members and 2048 columns.
+constexpr int MG_ENT_CAP = 2048;
+constexpr int MG_BIG_CAP = 8192;
+constexpr int MG_BIG_MEMBERS = 64;
 constexpr double MG_OMEGA = 0.9;
 constexpr double MG_SCALE = 1.5;
 
@@ -72,6 +77,7 @@ struct MGState {
     // its row's diagonal as a half float in the high half, in the PPE's
     // slot layout; null: the V-cycle reads the PPE's float values
     unsigned* ev = nullptr;
+    double* lsum = nullptr;  // level 0: tail sums of the long rows for the next pass
 };
 
 int mg_levels(const MGState* mg) { return mg ? mg->nlev : 0; }
@@ -80,14 +86,24 @@ int mg_coarsest(const MGState* mg) { return mg && mg->nlev > 0 ? mg->L[mg->nlev 
 
 // ------------------------------------------------------------------------
 // operators as row sources for the Galerkin product
+// (a row of more than LONG_ROW entries continues in its tail)
 struct FineOp {
     PPEView pp;
-    __device__ int cnt(int m) const { return pp.cnt[m]; }
+    __device__ int tail_len(int m) const {
+        if (!pp.ltail) return 0;
+        const int u = pp.ltail[m];
+        return u < 0 ? 0 : (int)(pp.tail_off[u + 1] - pp.tail_off[u]);
+    }
+    __device__ int cnt(int m) const { return pp.cnt[m] + tail_len(m); }
     __device__ int col(int m, int k) const {
+        if (k >= LONG_ROW && pp.ltail) return pp.tail_cols[pp.tail_off[pp.ltail[m]] + (k - LONG_ROW)];
         const long long e = prow(pp.sl_off, m) + pslot(k);
         return pp.ecols ? ecol(pp.gbase + (m & ~63), pp.ecols[e]) : pp.cols[e];
     }
-    __device__ float val(int m, int k) const { return pp.vals32[prow(pp.sl_off, m) + pslot(k)]; }
+    __device__ float val(int m, int k) const {
+        if (k >= LONG_ROW && pp.ltail) return pp.tail_vals[pp.tail_off[pp.ltail[m]] + (k - LONG_ROW)];
+        return pp.vals32[prow(pp.sl_off, m) + pslot(k)];
+    }
     __device__ float dg(int m) const { return pp.fdiag[m]; }
 };
 struct CoarseOp {
@@ -177,20 +193,26 @@ __global__ void __launch_bounds__(BLOCK) k_mg_link(int n, const int* key, const 
 // members' diagonals and entries mapped to aggregates, numbered in order
 // (members ascending, each member's diagonal first, then its entries in
 // storage order).  A bitonic sort in LDS of 32-bit keys (column - the row's
-// first column) << 11 | term number groups each column's terms in term
+// first column) << TB | term number groups each column's terms in term
 // order, so each column's sum is formed in that fixed order by one lane:
-// the hierarchy is bit-reproducible.  The terms are float values, kept as
-// floats in LDS and added in double.  COUNT pass: the row's off-diagonal
-// count.  FILL pass: columns ascending, values -sum, the diagonal.
-// err: 1 = too many terms, 2 = too many columns, 3 = columns too far apart.
-template <bool FILL, class OP>
+// the hierarchy is bit-reproducible.  Terms are float values added in
+// double.  CAP = MG_ENT_CAP: rows of at most CAP terms, values staged in
+// LDS; CAP = MG_BIG_CAP: only the larger rows, values read back from the
+// operator.  COUNT pass: the row's off-diagonal count.  FILL pass: columns
+// ascending, values -sum, the diagonal.  err: SPH_MG_TERMS (too many terms
+// or members), SPH_MG_COLUMNS, SPH_MG_SPAN (columns too far apart).
+template <bool FILL, class OP, int CAP>
 __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg, const int* mstart, const int* mem,
                                                   int* cnt1, const long long* sl_off1, int* cols1, float* vals1,
                                                   float* diag1, int* err) {
-    static_assert(MG_ENT_CAP <= (1 << 11), "term numbers take 11 key bits");
-    __shared__ unsigned sk[MG_ENT_CAP];           // (column - jmin) << 11 | term number
-    __shared__ float sv[FILL ? MG_ENT_CAP : 1];   // term values by term number
-    __shared__ int seg[SLICE * MG_QMAX + 1];      // first sorted position of each distinct column
+    constexpr bool BIG = CAP > MG_ENT_CAP;
+    constexpr int TB = BIG ? 13 : 11;               // term-number bits
+    constexpr int QMAX = BIG ? 32 : 8;              // distinct columns: at most 64 * QMAX
+    static_assert(CAP <= (1 << TB), "term numbers fit their key bits");
+    __shared__ unsigned sk[CAP];                    // (column - jmin) << TB | term number
+    __shared__ float sv[(FILL && !BIG) ? CAP : 1];  // term values by term number
+    __shared__ int seg[SLICE * QMAX + 1];           // first sorted position of each distinct column
+    __shared__ int moff[BIG ? MG_BIG_MEMBERS + 1 : 1];   // first term of each member
     const int I = blockIdx.x;
     const int lane = threadIdx.x;
     if (I >= n1) return;
@@ -198,8 +220,9 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     int nterm = 0;
     for (int t = t0 + lane; t < t1; t += SLICE) nterm += op.cnt(mem[t]) + 1;
     nterm = wave_sum(nterm);
-    if (nterm > MG_ENT_CAP) {
-        if (lane == 0) { err[0] = 1; if (!FILL) cnt1[I] = 0; }
+    if (BIG ? nterm <= MG_ENT_CAP : nterm > MG_ENT_CAP) return;   // the other kernel's row
+    if (nterm > CAP || (BIG && t1 - t0 > MG_BIG_MEMBERS)) {
+        if (lane == 0) { err[0] = SPH_MG_TERMS; if (!FILL) cnt1[I] = 0; }
         return;
     }
     int np2 = 1;
@@ -208,25 +231,27 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     for (int t = t0; t < t1; t++) {
         const int m = mem[t];
         const int c = op.cnt(m);
+        if (BIG && lane == 0) moff[t - t0] = pos;
         for (int k = lane - 1; k < c; k += SLICE) {
             const int q = pos + k + 1;
             const int J = k < 0 ? I : agg[op.col(m, k)];
             sk[q] = (unsigned)J;
             jmin = min(jmin, J);
             jmax = max(jmax, J);
-            if (FILL) sv[q] = k < 0 ? op.dg(m) : -op.val(m, k);
+            if (FILL && !BIG) sv[q] = k < 0 ? op.dg(m) : -op.val(m, k);
         }
         pos += c + 1;
     }
+    if (BIG && lane == 0) moff[t1 - t0] = pos;
     jmin = -wave_max(-jmin);
     jmax = wave_max(jmax);
-    if (jmax - jmin >= (1 << 21) - 1) {
-        if (lane == 0) { err[0] = 3; if (!FILL) cnt1[I] = 0; }
+    if (jmax - jmin >= (1 << (32 - TB)) - 1) {
+        if (lane == 0) { err[0] = SPH_MG_SPAN; if (!FILL) cnt1[I] = 0; }
         return;
     }
     __syncthreads();
     for (int p = lane; p < np2; p += SLICE)
-        sk[p] = p < nterm ? ((sk[p] - (unsigned)jmin) << 11) | (unsigned)p : 0xffffffffu;
+        sk[p] = p < nterm ? ((sk[p] - (unsigned)jmin) << TB) | (unsigned)p : 0xffffffffu;
     __syncthreads();
     for (int k = 2; k <= np2; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
@@ -242,15 +267,15 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
         }
     }
     // distinct columns: each lane counts the group starts in its run
-    auto col_at = [&](int p) { return (int)(sk[p] >> 11) + jmin; };
+    auto col_at = [&](int p) { return (int)(sk[p] >> TB) + jmin; };
     const int per = (nterm + SLICE - 1) / SLICE;
     const int p0 = min(nterm, lane * per), p1 = min(nterm, p0 + per);
     int own = 0;
     for (int p = p0; p < p1; p++) own += (p == 0 || col_at(p) != col_at(p - 1)) ? 1 : 0;
     const int at = wave_excl(own);
     const int total = wave_sum(own);
-    if (total > SLICE * MG_QMAX) {
-        if (lane == 0) { err[0] = 2; if (!FILL) cnt1[I] = 0; }
+    if (total > SLICE * QMAX) {
+        if (lane == 0) { err[0] = SPH_MG_COLUMNS; if (!FILL) cnt1[I] = 0; }
         return;
     }
     if (!FILL) {
@@ -262,6 +287,14 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
         if (p == 0 || col_at(p) != col_at(p - 1)) seg[w++] = p;
     if (lane == 0) seg[total] = nterm;
     __syncthreads();
+    // a term's value: from LDS, or (large rows) from its member's entry
+    auto term_val = [&](int q) -> float {
+        if (!BIG) return sv[q];
+        int mi = 0;
+        while (mi + 1 < t1 - t0 && moff[mi + 1] <= q) mi++;
+        const int m = mem[t0 + mi], k = q - moff[mi] - 1;
+        return k < 0 ? op.dg(m) : -op.val(m, k);
+    };
     int lo = 0, hi = total - 1;   // the diagonal's slot
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -272,7 +305,7 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     const long long base = prow(sl_off1, I);
     for (int s = lane; s < total; s += SLICE) {
         double acc = 0.0;
-        for (int p = seg[s]; p < seg[s + 1]; p++) acc += (double)sv[sk[p] & 0x7ffu];
+        for (int p = seg[s]; p < seg[s + 1]; p++) acc += (double)term_val((int)(sk[p] & ((1u << TB) - 1u)));
         if (s == sI) {
             diag1[I] = (float)acc;
         } else {
@@ -310,7 +343,7 @@ __global__ void __launch_bounds__(BLOCK) k_gj_pivot(int n, const double* a, int 
     const double piv = a[(long long)k * n + k];
     const bool bad = !(fabs(piv) > 0.0) || !(fabs(piv) <= DBL_MAX);
     if (bad) {
-        if (threadIdx.x == 0) err[0] = 3;
+        if (threadIdx.x == 0) err[0] = SPH_MG_PIVOT;
         return;
     }
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -360,6 +393,7 @@ struct MGLv {
     float *b, *x, *y, *r;
     const float* ainv;            // coarsest level: dense inverse
     const unsigned* ev;           // level 0: compact operator, or null
+    double* lsum;                 // level 0: tail sums of the long rows
 };
 __device__ __forceinline__ CoarseOp lv_op(const MGLv& L) { return CoarseOp{L.sl_off, L.cnt, L.cols, L.vals, L.diag}; }
 
@@ -426,8 +460,10 @@ __global__ void __launch_bounds__(BLOCK) k_mg_res(const CaseDesc* cds, const MGL
         if (row >= L.n) return;
         const float* b = which ? pp.fs : pp.fp;
         if (L.ev) {
-            const double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
-            L.r[row] = (float)((double)b[row] - (double)pp.fdiag[row] * ((double)L.x[row] - s));
+            const double d = (double)pp.fdiag[row];
+            double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
+            if (pp.ltail && pp.ltail[row] >= 0) s += L.lsum[pp.ltail[row]] / d;
+            L.r[row] = (float)((double)b[row] - d * ((double)L.x[row] - s));
         } else {
             L.r[row] = (float)((double)b[row] - spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, L.x, wbase));
         }
@@ -449,7 +485,8 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(const CaseDesc* cds, const MG
         float* out = which ? pp.fshat : pp.fphat;
         const double d = (double)pp.fdiag[row];
         if (L.ev) {
-            const double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
+            double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
+            if (pp.ltail && pp.ltail[row] >= 0) s += L.lsum[pp.ltail[row]] / d;
             out[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)b[row] / d - (double)L.x[row] + s));
         } else {
             const double ax = spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, L.x, wbase);
@@ -480,6 +517,26 @@ __global__ void __launch_bounds__(BLOCK) k_mg_prolong(const CaseDesc* cds, const
     const MGLv& N = lv[te_.x * MG_MAX_LEVELS + l + 1];
     const float* z = l + 1 == L.lc ? N.x : N.y;
     L.x[row] = (float)((double)L.x[row] + MG_SCALE * (double)z[L.agg[row]]);
+}
+// level 0: the tail sums of the long rows over the current x, one wave per
+// long row, lanes over its contiguous tail, fixed shuffle tree (tab: four
+// long rows per block)
+__global__ void __launch_bounds__(BLOCK) k_mg_long(const CaseDesc* cds, const MGLv* lv, const int2* tab,
+                                                   int sc_done) {
+    const int2 te_ = tab[blockIdx.x];
+    const CaseDesc& cd = cds[te_.x];
+    if (cd.sc[sc_done] != 0.0) return;
+    const MGLv& L = lv[te_.x * MG_MAX_LEVELS];
+    const PPEView& pp = cd.ppe;
+    const int u = te_.y + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (u >= pp.nlong) return;
+    double s = 0.0;
+    for (long long e = pp.tail_off[u] + lane; e < pp.tail_off[u + 1]; e += SLICE)
+        s += (double)pp.tail_vals[e] * (double)L.x[pp.tail_cols[e]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) L.lsum[u] = s;
 }
 // coarsest level: x = Ainv b, one wave per row, lanes over the columns,
 // fixed shuffle tree (tab: four rows per block)
@@ -513,6 +570,7 @@ void mg_free(sph_ctx* c, MGState* mg) {
     }
     dfree(c, mg->ainv);
     dfree(c, mg->ev);
+    dfree(c, mg->lsum);
     delete mg;
 }
 
@@ -528,15 +586,19 @@ void mg_free(sph_ctx* c, MGState* mg) {
 // Galerkin operator of level l + 1 from level l (src), whose link arrays
 // are set.  Returns 0 with *ok = 0 when a row is out of the kernel's bounds.
 template <class OP>
-static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0, int* err, int* ok) {
+static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0, int* err, int* ok, int* why) {
     const int n1 = L1.n;
     SPH_TRY(dalloc(c, &L1.cnt, n1));
-    MGK("mg_rap_count", n1, SLICE, (k_mg_rap<false, OP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
-        (const int*)L0.mem, L1.cnt, (const long long*)nullptr, (int*)nullptr, (float*)nullptr, (float*)nullptr, err);
+    MGK("mg_rap_count", n1, SLICE, (k_mg_rap<false, OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg,
+        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)nullptr, (int*)nullptr, (float*)nullptr,
+        (float*)nullptr, err);
+    MGK("mg_rap_count", n1, SLICE, (k_mg_rap<false, OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg,
+        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)nullptr, (int*)nullptr, (float*)nullptr,
+        (float*)nullptr, err);
     int h_err = 0;
     SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
-    if (h_err) { *ok = 0; return 0; }
+    if (h_err) { *ok = 0; *why = h_err; return 0; }
     L1.nslices = (n1 + SLICE - 1) / SLICE;
     long long* w = nullptr;
     SPH_TRY(dalloc(c, &w, (size_t)L1.nslices + 1));
@@ -547,11 +609,14 @@ static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0
     SPH_TRY(dalloc(c, &L1.cols, (size_t)std::max(1LL, L1.slots)));
     SPH_TRY(dalloc(c, &L1.vals, (size_t)std::max(1LL, L1.slots)));
     SPH_TRY(dalloc(c, &L1.diag, n1));
-    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<true, OP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
-        (const int*)L0.mem, L1.cnt, (const long long*)L1.sl_off, L1.cols, L1.vals, L1.diag, err);
+    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<true, OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg,
+        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)L1.sl_off, L1.cols, L1.vals, L1.diag, err);
+    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<true, OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg,
+        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)L1.sl_off, L1.cols, L1.vals, L1.diag, err);
     SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
     *ok = h_err == 0;
+    *why = h_err;
     return 0;
 }
 
@@ -601,10 +666,12 @@ static int mg_vectors(sph_ctx* c, MGLevel& L, bool fine, bool coarsest) {
     return 0;
 }
 
-int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok) {
+int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok, int* why) {
     *out = nullptr;
     *ok = 0;
-    if (pp.nr <= MG_COARSE_MAX || pp.ltail || !pp.vals32 || !pp.fdiag || !(q > 0)) return 0;
+    *why = SPH_MG_UNSUITED;
+    // long rows need the compact level-0 copy (its passes add their own tail sums)
+    if (pp.nr <= MG_COARSE_MAX || (pp.ltail && !pp.ecols) || !pp.vals32 || !pp.fdiag || !(q > 0)) return 0;
     const sph_domain& d = c->dom;
     const double ex = ((double)d.x_max - d.x_min) / q + 3.0, ey = ((double)d.y_max - d.y_min) / q + 3.0,
                  ez = ((double)d.z_max - d.z_min) / q + 3.0;
@@ -646,15 +713,17 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
     int nlev = 1;
     bool good = true;
     while (mg->L[nlev - 1].n > MG_COARSE_MAX) {
-        if (nlev == MG_MAX_LEVELS) { good = false; break; }
+        if (nlev == MG_MAX_LEVELS) { good = false; *why = SPH_MG_LEVELS; break; }
         MGLevel& L0 = mg->L[nlev - 1];
         MGLevel& L1 = mg->L[nlev];
         MGTRY(hipc(hipMemsetAsync(err, 0, sizeof(int), c->stream), "mg: clear flag"));
         MGTRY(mg_aggregate(c, L0, L1));
-        if (L1.n * 10LL > L0.n * 9LL) { nlev++; good = false; break; }   // coarsening stalled
+        if (L1.n * 10LL > L0.n * 9LL) { nlev++; good = false; *why = SPH_MG_STALLED; break; }
         int lok = 0;
-        if (nlev == 1) MGTRY(mg_galerkin(c, FineOp{pp}, L1, L0, err, &lok));
-        else MGTRY(mg_galerkin(c, coarse_op(L0), L1, L0, err, &lok));
+        int code = 0;
+        if (nlev == 1) MGTRY(mg_galerkin(c, FineOp{pp}, L1, L0, err, &lok, &code));
+        else MGTRY(mg_galerkin(c, coarse_op(L0), L1, L0, err, &lok, &code));
+        if (!lok) *why = code;   // SPH_MG_TERMS, SPH_MG_COLUMNS or SPH_MG_SPAN
         nlev++;
         if (!lok) { good = false; break; }
     }
@@ -691,8 +760,9 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         dfree(c, a);
         dfree(c, rowk);
         dfree(c, colk);
-        if (h_err) return fail_out(0);
+        if (h_err) { *why = SPH_MG_PIVOT; return fail_out(0); }
     }
+    if (pp.nlong > 0) MGTRY(dalloc(c, &mg->lsum, pp.nlong));
     if (pp.ecols) {
         MGTRY(dalloc(c, &mg->ev, (size_t)std::max(1LL, slots)));
         hipLaunchKernelGGL(k_mg_pack16, dim3(grid1(pp.nr)), dim3(BLOCK), 0, c->stream, pp, mg->ev);
@@ -709,6 +779,7 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
 #undef MGTRY
     *out = mg;
     *ok = 1;
+    *why = SPH_MG_OK;
     return 0;
 }
 
@@ -738,6 +809,7 @@ int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool
             D.b = S.b; D.x = S.x; D.y = S.y; D.r = S.r;
             D.ainv = l == mg->nlev - 1 ? mg->ainv : nullptr;
             D.ev = l == 0 ? mg->ev : nullptr;
+            D.lsum = l == 0 ? mg->lsum : nullptr;
         }
     }
     auto rows_table = [&](int l, bool below, int* off, int* cnt) {
@@ -762,6 +834,14 @@ int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool
         for (int f = 0; f < mg->L[mg->nlev - 1].n; f += BLOCK / SLICE) tab.push_back(make_int2(k, f));
     }
     P.dense_cnt = (int)tab.size() - P.dense_off;
+    P.long_off = (int)tab.size();
+    for (int k = 0; k < nc; k++) {
+        const MGState* mg = mgs[k];
+        if (!mg || !mg->lsum) continue;
+        const int nlong = e.cs[k]->ppe ? ppe_nlong(e.cs[k]) : 0;
+        for (int f = 0; f < nlong; f += BLOCK / SLICE) tab.push_back(make_int2(k, f));
+    }
+    P.long_cnt = (int)tab.size() - P.long_off;
     // launches in flight may read the old tables
     SPH_HIP(&e, hipStreamSynchronize(e.st));
     const size_t lv_bytes = sizeof(MGLv) * lv.size(), tab_bytes = sizeof(int2) * std::max<size_t>(1, tab.size());
@@ -802,6 +882,7 @@ int mg_apply(Eng& e, int which, int sc_done) {
     } while (0)
     for (int l = 0; l + 1 < P.lmax; l++) {
         MGE("mg_pre", 0, P.down_off[l], P.down_cnt[l], k_mg_pre, l, which, sc_done);
+        if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
         MGE(l == 0 ? "mg_res0" : "mg_res", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_res, l,
             which, sc_done);
         MGE("mg_restrict", 0, P.lvl_off[l + 1], P.lvl_cnt[l + 1], k_mg_restrict, l, sc_done);
@@ -809,6 +890,7 @@ int mg_apply(Eng& e, int which, int sc_done) {
     MGE("mg_dense", 0, P.dense_off, P.dense_cnt, k_mg_dense, sc_done);
     for (int l = P.lmax - 2; l >= 0; l--) {
         MGE("mg_prolong", 0, P.down_off[l], P.down_cnt[l], k_mg_prolong, l, sc_done);
+        if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
         MGE(l == 0 ? "mg_post0" : "mg_post", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_post,
             l, which, sc_done);
     }

@@ -92,6 +92,7 @@ struct PPEState {
     // 1 built (mg), -1 not usable for this matrix (Jacobi)
     MGState* mg = nullptr;
     int mg_state = 0;
+    int mg_why = SPH_MG_OK;     // SPH_MG_*: why the last build gave no hierarchy
 };
 
 // ------------------------------------------------------------------------
@@ -823,6 +824,37 @@ static int lattice_spacing(sph_ctx* c, const PPEState* P, double* q) {
     *q = h0 / 1.5;
     return 0;
 }
+// The multigrid's lattice spacing: h / 1.5 of the finest row, so that
+// refined particles aggregate among themselves first and coarse ones join
+// once the blocks reach their spacing.
+__global__ void __launch_bounds__(BLOCK) k_min_h(int nr, const int* rows, const double* h, double* out) {
+    __shared__ double sh[BLOCK];
+    const int r = blockIdx.x * BLOCK + threadIdx.x;
+    sh[threadIdx.x] = r < nr ? h[rows[r]] : DBL_MAX;
+    __syncthreads();
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+static int finest_spacing(sph_ctx* c, const PPEState* P, double* q) {
+    const int nb = (int)grid1(P->nr);
+    double* part = nullptr;
+    SPH_TRY(dalloc(c, &part, nb));
+    hipLaunchKernelGGL(k_min_h, dim3(nb), dim3(BLOCK), 0, c->stream, P->nr, (const int*)P->rows,
+                       (const double*)c->sview().h, part);
+    SPH_TRY(launch_check(c, "min_h"));
+    std::vector<double> h(nb);
+    SPH_HIP(c, hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c, part);
+    double m = DBL_MAX;
+    for (double v : h) m = std::min(m, v);
+    *q = m / 1.5;
+    return 0;
+}
+
 static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
     const int nr = P->nr;
     PView S = c->sview();
@@ -1053,6 +1085,8 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
     }
     return 0;
 }
+
+int ppe_nlong(const sph_ctx* c) { return c->ppe ? c->ppe->nlong : 0; }
 
 PPEView ppe_view(const sph_ctx* c) {
     PPEView v{};
@@ -1357,15 +1391,17 @@ int ppe_solve(Eng& e) {
                 mg_free(c, P->mg);
                 P->mg = nullptr;
                 P->mg_state = -1;
-                if (P->nr > MG_COARSE_MAX && P->nlong == 0) {
+                P->mg_why = SPH_MG_UNSUITED;
+                if (P->nr > MG_COARSE_MAX) {
                     double q = 0.0;
                     int ok = 0;
-                    SPH_TRY(lattice_spacing(c, P, &q));
-                    SPH_TRY(mg_build(c, ppe_view(c), P->nnz_slots, q, &P->mg, &ok));
+                    SPH_TRY(finest_spacing(c, P, &q));
+                    SPH_TRY(mg_build(c, ppe_view(c), P->nnz_slots, q, &P->mg, &ok, &P->mg_why));
                     P->mg_state = ok ? 1 : -1;
                 }
             }
             if (P->mg_state == 1) { mgs[k] = P->mg; any_mg = true; }
+            c->rep.mg_status = P->mg_state == -1 ? P->mg_why : SPH_MG_OK;
         }
         auto mg_tables = [&]() -> int {
             bool changed = false;
@@ -1437,6 +1473,8 @@ int ppe_solve(Eng& e) {
                 const double* sc = e.h_sc + k * SC_COUNT;
                 if (sc[SC_CONV] == 0.0 && std::isfinite(sc[SC_RNORM])) {
                     c->ppe->mg_state = -1;
+                    c->ppe->mg_why = SPH_MG_STALLED_SOLVE;
+                    c->rep.mg_status = SPH_MG_STALLED_SOLVE;
                     mgs[k] = nullptr;
                     stalled = true;
                 }

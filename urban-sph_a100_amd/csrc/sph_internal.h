@@ -266,6 +266,7 @@ struct Eng {
         int down_off[16], down_cnt[16];   // rows of level l, cases with a level below l
         int lvl_off[16], lvl_cnt[16];     // rows of level l, cases that have level l
         int dense_off = 0, dense_cnt = 0; // coarsest rows, four per block
+        int long_off = 0, long_cnt = 0;   // level-0 long rows, four per block
         double spmv_bytes0 = 0.0;         // level-0 bytes of one pass (profile)
     } mg_plan;
 };
@@ -407,6 +408,7 @@ int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax);
 // ppe.hip
 int ppe_build(sph_ctx* c);
 PPEView ppe_view(const sph_ctx* c);
+int ppe_nlong(const sph_ctx* c);   // rows with tails of the case's PPE
 // neighbour records from the working state (physics.hip): REC_ALL, or only
 // the pairs that hold velocities and pressures (REC_VEL_PQ) or pressures (REC_PQ)
 enum { REC_ALL = 0, REC_VEL_PQ = 1, REC_PQ = 2 };
@@ -420,7 +422,7 @@ constexpr int MG_MAX_LEVELS = 12;
 // Builds the hierarchy of the PPE in pp (float values, `slots` stored SELL
 // slots) with lattice spacing q; *ok = 0 (and *out = null) when the case
 // does not suit it.
-int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok);
+int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok, int* why);
 bool mg_compact(const MGState* mg);   // the V-cycle's level 0 reads the half-precision copy
 // Tables and descriptors of the V-cycles of the step's cases (mgs[k]: case
 // k's hierarchy or null for a case that uses Jacobi); *changed when they
