@@ -81,7 +81,6 @@ struct MGState {
 };
 
 int mg_levels(const MGState* mg) { return mg ? mg->nlev : 0; }
-bool mg_compact(const MGState* mg) { return mg && mg->ev; }
 int mg_coarsest(const MGState* mg) { return mg && mg->nlev > 0 ? mg->L[mg->nlev - 1].n : 0; }
 
 // ------------------------------------------------------------------------

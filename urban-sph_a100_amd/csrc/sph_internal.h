@@ -423,7 +423,6 @@ constexpr int MG_MAX_LEVELS = 12;
 // slots) with lattice spacing q; *ok = 0 (and *out = null) when the case
 // does not suit it.
 int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState** out, int* ok, int* why);
-bool mg_compact(const MGState* mg);   // the V-cycle's level 0 reads the half-precision copy
 // Tables and descriptors of the V-cycles of the step's cases (mgs[k]: case
 // k's hierarchy or null for a case that uses Jacobi); *changed when they
 // differ from the previous call's.
