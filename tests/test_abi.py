@@ -87,3 +87,25 @@ def test_aux_struct_layouts_match_c(tmp_path, struct, cname):
     assert out[0] == C.sizeof(S)
     for m, off in zip(members, out[1:]):
         assert getattr(S, m).offset == off, m
+
+
+def test_enum_values_match_c(tmp_path):
+    """The Python constants of the solver options and report codes equal the
+    header's enumerators."""
+    pairs = [("PRECOND_JACOBI", "SPH_PRECOND_JACOBI"), ("PRECOND_MULTIGRID", "SPH_PRECOND_MULTIGRID"),
+             ("ORDER_CELL", "SPH_ORDER_CELL"), ("ORDER_LATTICE", "SPH_ORDER_LATTICE"),
+             ("FP64", "SPH_FP64"), ("MIXED", "SPH_MIXED"),
+             ("MG_OK", "SPH_MG_OK"), ("MG_UNSUITED", "SPH_MG_UNSUITED"), ("MG_TERMS", "SPH_MG_TERMS"),
+             ("MG_COLUMNS", "SPH_MG_COLUMNS"), ("MG_SPAN", "SPH_MG_SPAN"), ("MG_PIVOT", "SPH_MG_PIVOT"),
+             ("MG_STALLED", "SPH_MG_STALLED"), ("MG_LEVELS", "SPH_MG_LEVELS"),
+             ("MG_STALLED_SOLVE", "SPH_MG_STALLED_SOLVE")]
+    lines = ['#include <stdio.h>', '#include "urbansph.h"', 'int main(void){']
+    lines += [f'printf("%d\\n", (int){c});' for _, c in pairs]
+    lines.append("return 0;}")
+    src = tmp_path / "enums.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "enums"
+    import subprocess
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    out = [int(v) for v in subprocess.check_output([str(exe)]).decode().split()]
+    assert out == [getattr(abi, p) for p, _ in pairs]

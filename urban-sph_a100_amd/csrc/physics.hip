@@ -1264,8 +1264,6 @@ static int run_freeslip(Eng& e, int which) {
     return 0;
 }
 
-// Everything of SOPHIA_single_ISPH before the pressure solve
-// (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
 // Neighbour records of every case of the step, from its working state as
 // it is now; run before each pass that gathers from them.
 int pack_records(Eng& e, int which) {
@@ -1275,6 +1273,8 @@ int pack_records(Eng& e, int which) {
     return 0;
 }
 
+// Everything of SOPHIA_single_ISPH before the pressure solve
+// (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
 int step_physics_front(Eng& e) {
     const bool mixed = mixed_of(e);
     bool any_count0 = false;
