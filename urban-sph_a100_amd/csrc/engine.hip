@@ -153,6 +153,12 @@ static int tune_fold(Eng& e) {
 }
 
 // ---------------------------------------------------------------- shapes
+// the cell key of the first particle of every block of `block` particles
+__global__ void k_block_keys(const int* keys, long long nb, int block, int* out) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) out[b] = keys[b * block];
+}
+
 int shape(Eng& e, int kind, int block, Shape* out) {
     const int nc = (int)e.cs.size();
     auto units = [&](int k) -> long long {
@@ -163,7 +169,7 @@ int shape(Eng& e, int kind, int block, Shape* out) {
                                                                            : 64LL * c->nm_count;
     };
     out->block = block;
-    if (nc == 1) {
+    if (nc == 1 && kind != 0) {
         out->grid = (unsigned)((units(0) + block - 1) / block);
         out->tab = nullptr;
         return 0;
@@ -177,9 +183,59 @@ int shape(Eng& e, int kind, int block, Shape* out) {
         return 0;
     }
     std::vector<int2> h;
-    for (int k = 0; k < nc; k++) {
-        const long long u = units(k);
-        for (long long f = 0; f < u; f += block) h.push_back(make_int2(k, (int)f));
+    if (kind == 0) {
+        // Particle blocks of each case in Morton order of the cell of their
+        // first particle: the blocks one XCD runs at a time (a contiguous
+        // share of the table, see xcd_block) then cover a compact 3-D region,
+        // whose neighbours its L2 holds better than a row-by-row sweep's.
+        for (int k = 0; k < nc; k++) {
+            const sph_ctx* c = e.cs[k];
+            const long long u = units(k);
+            // the cell key of each block's first particle
+            const size_t nb = (size_t)((u + block - 1) / block);
+            std::vector<int> key(c->keys && u > 0 ? nb : 0);
+            if (!key.empty()) {
+                int* dk = nullptr;
+                SPH_TRY(dalloc(e.cs[k], &dk, nb));   // the context's block cache: no device-wide wait
+                hipLaunchKernelGGL(k_block_keys, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, e.st,
+                                   (const int*)c->keys, (long long)nb, block, dk);
+                hipError_t er = hipMemcpyAsync(key.data(), dk, sizeof(int) * nb, hipMemcpyDeviceToHost, e.st);
+                if (er == hipSuccess) er = hipStreamSynchronize(e.st);
+                dfree(e.cs[k], dk);
+                SPH_HIP(&e, er);
+            }
+            const sph_domain& d = c->dom;
+            auto spread = [](unsigned v) {
+                unsigned long long x = v & 0x1fffff;
+                x = (x | x << 32) & 0x1f00000000ffffULL;
+                x = (x | x << 16) & 0x1f0000ff0000ffULL;
+                x = (x | x << 8) & 0x100f00f00f00f00fULL;
+                x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+                x = (x | x << 2) & 0x1249249249249249ULL;
+                return x;
+            };
+            std::vector<std::pair<unsigned long long, int>> blk;
+            for (long long f = 0; f < u; f += block) {
+                unsigned long long m = 0;
+                if (!key.empty() && d.NI > 0 && d.NJ > 0) {
+                    const long long cell = key[(size_t)(f / block)];
+                    const unsigned ci = (unsigned)(cell % d.NI), cj = (unsigned)((cell / d.NI) % d.NJ),
+                                   ck = (unsigned)(cell / ((long long)d.NI * d.NJ));
+                    m = spread(ci) | spread(cj) << 1 | spread(ck) << 2;
+                }
+                blk.push_back({m, (int)f});
+            }
+            std::stable_sort(blk.begin(), blk.end(),
+                             [](const std::pair<unsigned long long, int>& a, const std::pair<unsigned long long, int>& b) {
+                                 return a.first < b.first;
+                             });
+            for (auto& b : blk) h.push_back(make_int2(k, b.second));
+        }
+    } else {
+        for (int k = 0; k < nc; k++) {
+            const long long u = units(k);
+            for (long long f = 0; f < u; f += block) h.push_back(make_int2(k, (int)f));
+        }
     }
     if (!t) {
         e.tabs.push_back(Eng::Tab{});
