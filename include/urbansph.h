@@ -271,7 +271,8 @@ enum {
                                    columns, or a lattice beyond 32-bit keys  */
     SPH_MG_TERMS = 2,           /* a coarse row gathers more than 8192 terms
                                    or 64 members                              */
-    SPH_MG_COLUMNS = 3,         /* a coarse row has more than 2048 columns    */
+    SPH_MG_COLUMNS = 3,         /* reserved: a coarse row's columns are no
+                                   longer bounded apart from its terms       */
     SPH_MG_SPAN = 4,            /* a coarse row's columns lie too far apart   */
     SPH_MG_PIVOT = 5,           /* zero or non-finite pivot of the coarsest
                                    level                                      */

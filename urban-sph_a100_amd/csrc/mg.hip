@@ -198,19 +198,19 @@ __global__ void __launch_bounds__(BLOCK) k_mg_link(int n, const int* key, const 
 // double.  CAP = MG_ENT_CAP: rows of at most CAP terms, values staged in
 // LDS; CAP = MG_BIG_CAP: only the larger rows, values read back from the
 // operator.  COUNT pass: the row's off-diagonal count.  FILL pass: columns
-// ascending, values -sum, the diagonal.  err: SPH_MG_TERMS (too many terms
-// or members), SPH_MG_COLUMNS, SPH_MG_SPAN (columns too far apart).
+// ascending, values -sum, the diagonal.  Each lane takes the columns whose
+// first sorted term falls in its run of sorted positions and sums each of
+// them to its end.  err: SPH_MG_TERMS (too many terms or members),
+// SPH_MG_SPAN (columns too far apart).
 template <bool FILL, class OP, int CAP>
 __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg, const int* mstart, const int* mem,
                                                   int* cnt1, const long long* sl_off1, int* cols1, float* vals1,
                                                   float* diag1, int* err) {
     constexpr bool BIG = CAP > MG_ENT_CAP;
     constexpr int TB = BIG ? 13 : 11;               // term-number bits
-    constexpr int QMAX = BIG ? 32 : 8;              // distinct columns: at most 64 * QMAX
     static_assert(CAP <= (1 << TB), "term numbers fit their key bits");
     __shared__ unsigned sk[CAP];                    // (column - jmin) << TB | term number
     __shared__ float sv[(FILL && !BIG) ? CAP : 1];  // term values by term number
-    __shared__ int seg[SLICE * QMAX + 1];           // first sorted position of each distinct column
     __shared__ int moff[BIG ? MG_BIG_MEMBERS + 1 : 1];   // first term of each member
     const int I = blockIdx.x;
     const int lane = threadIdx.x;
@@ -265,27 +265,22 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
             __syncthreads();
         }
     }
-    // distinct columns: each lane counts the group starts in its run
+    // distinct columns: each lane counts the group starts in its run, and
+    // those below the diagonal's column (the diagonal's slot)
     auto col_at = [&](int p) { return (int)(sk[p] >> TB) + jmin; };
+    auto starts = [&](int p) { return p == 0 || col_at(p) != col_at(p - 1); };
     const int per = (nterm + SLICE - 1) / SLICE;
     const int p0 = min(nterm, lane * per), p1 = min(nterm, p0 + per);
-    int own = 0;
-    for (int p = p0; p < p1; p++) own += (p == 0 || col_at(p) != col_at(p - 1)) ? 1 : 0;
+    int own = 0, below = 0;
+    for (int p = p0; p < p1; p++)
+        if (starts(p)) { own++; below += col_at(p) < I ? 1 : 0; }
     const int at = wave_excl(own);
     const int total = wave_sum(own);
-    if (total > SLICE * QMAX) {
-        if (lane == 0) { err[0] = SPH_MG_COLUMNS; if (!FILL) cnt1[I] = 0; }
-        return;
-    }
+    const int sI = wave_sum(below);
     if (!FILL) {
         if (lane == 0) cnt1[I] = total - 1;
         return;
     }
-    int w = at;
-    for (int p = p0; p < p1; p++)
-        if (p == 0 || col_at(p) != col_at(p - 1)) seg[w++] = p;
-    if (lane == 0) seg[total] = nterm;
-    __syncthreads();
     // a term's value: from LDS, or (large rows) from its member's entry
     auto term_val = [&](int q) -> float {
         if (!BIG) return sv[q];
@@ -294,24 +289,25 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
         const int m = mem[t0 + mi], k = q - moff[mi] - 1;
         return k < 0 ? op.dg(m) : -op.val(m, k);
     };
-    int lo = 0, hi = total - 1;   // the diagonal's slot
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (col_at(seg[mid]) < I) lo = mid + 1;
-        else hi = mid;
-    }
-    const int sI = lo;
     const long long base = prow(sl_off1, I);
-    for (int s = lane; s < total; s += SLICE) {
+    int s = at;
+    for (int p = p0; p < p1; p++) {
+        if (!starts(p)) continue;
+        const int J = col_at(p);
         double acc = 0.0;
-        for (int p = seg[s]; p < seg[s + 1]; p++) acc += (double)term_val((int)(sk[p] & ((1u << TB) - 1u)));
+        int e = p;
+        do {
+            acc += (double)term_val((int)(sk[e] & ((1u << TB) - 1u)));
+            e++;
+        } while (e < nterm && col_at(e) == J);
         if (s == sI) {
             diag1[I] = (float)acc;
         } else {
             const int o = s - (s > sI ? 1 : 0);
-            cols1[base + pslot(o)] = col_at(seg[s]);
+            cols1[base + pslot(o)] = J;
             vals1[base + pslot(o)] = (float)(-acc);
         }
+        s++;
     }
 }
 __global__ void __launch_bounds__(BLOCK) k_mg_width(int n, int nslices, const int* cnt, long long* w) {
@@ -722,7 +718,7 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         int code = 0;
         if (nlev == 1) MGTRY(mg_galerkin(c, FineOp{pp}, L1, L0, err, &lok, &code));
         else MGTRY(mg_galerkin(c, coarse_op(L0), L1, L0, err, &lok, &code));
-        if (!lok) *why = code;   // SPH_MG_TERMS, SPH_MG_COLUMNS or SPH_MG_SPAN
+        if (!lok) *why = code;   // SPH_MG_TERMS or SPH_MG_SPAN
         nlev++;
         if (!lok) { good = false; break; }
     }
