@@ -52,8 +52,10 @@ constexpr double MG_SCALE = 1.5;
 struct MGLevel {
     int n = 0, nslices = 0;
     long long slots = 0;
-    long long nx = 0, ny = 0, nz = 0;   // extents of the block coordinates
-    int *cx = nullptr, *cy = nullptr, *cz = nullptr;
+    long long nx = 0, ny = 0, nz = 0;   // level 0: extents of the lattice coordinates
+    // while building: each row's block coordinates on the lattice of
+    // spacing 2^ex q, and ex
+    int *cx = nullptr, *cy = nullptr, *cz = nullptr, *ex = nullptr;
     // operator of a coarse level (level 0 is the PPE)
     long long* sl_off = nullptr;
     int* cnt = nullptr;
@@ -142,6 +144,17 @@ __device__ __forceinline__ int wave_excl(int v) {
 
 // ------------------------------------------------------------------------
 // hierarchy construction
+//
+// Aggregates are 2x2x2 blocks of a lattice.  q is the finest particles'
+// spacing; a particle of smoothing length h = 2^r h_min starts on the
+// lattice of spacing 2^r q (its own spacing), so every resolution zone of a
+// refined case coarsens by about 8 per level.  A row on the lattice of
+// spacing 2^e q goes to the block of spacing 2^(e+1) q holding it; blocks of
+// different spacings are different aggregates.  A block whose rows gather
+// more than MG_BIG_CAP terms (or more than MG_BIG_MEMBERS rows) is split
+// into runs of its rows, in ascending row order, that do not; the runs keep
+// the block's coordinates and meet again in the next level's block.
+
 // lattice position of each PPE row's particle, clamped to the extents (a
 // particle outside them joins the boundary blocks)
 __device__ __forceinline__ int lattice_index(double v, double v0, double q, long long n) {
@@ -150,40 +163,77 @@ __device__ __forceinline__ int lattice_index(double v, double v0, double q, long
 }
 __global__ void __launch_bounds__(BLOCK) k_mg_coord0(int nr, const int* rows, PView S, double x0, double y0, double z0,
                                                      double q, long long nx, long long ny, long long nz, int* cx,
-                                                     int* cy, int* cz) {
+                                                     int* cy, int* cz, int* ex, int* rmax) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
     const int i = rows[r];
-    cx[r] = lattice_index(S.x[i], x0, q, nx);
-    cy[r] = lattice_index(S.y[i], y0, q, ny);
-    cz[r] = lattice_index(S.z[i], z0, q, nz);
+    // h_min = 1.5 q; refinement halves h, so h / h_min is a power of two
+    const int e = (int)fmin(fmax(rint(log2(S.h[i] / (1.5 * q))), 0.0), 20.0);
+    cx[r] = lattice_index(S.x[i], x0, q, nx) >> e;
+    cy[r] = lattice_index(S.y[i], y0, q, ny) >> e;
+    cz[r] = lattice_index(S.z[i], z0, q, nz) >> e;
+    ex[r] = e;
+    if (e > 0) atomicMax(rmax, e);
 }
-__global__ void __launch_bounds__(BLOCK) k_mg_key(int n, const int* cx, const int* cy, const int* cz, long long nx1,
-                                                  long long ny1, int* key, int* val) {
+// extent of the lattice of spacing 2^e q
+__device__ __host__ __forceinline__ long long lattice_extent(long long n0, int e) {
+    return (n0 + (1LL << e) - 1) >> e;
+}
+// first key of the blocks of spacing 2^e q (e >= 1)
+__device__ __host__ __forceinline__ long long lattice_key_base(long long nx0, long long ny0, long long nz0, int e) {
+    long long b = 0;
+    for (int k = 1; k < e; k++) b += lattice_extent(nx0, k) * lattice_extent(ny0, k) * lattice_extent(nz0, k);
+    return b;
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_key(int n, const int* cx, const int* cy, const int* cz, const int* ex,
+                                                  long long nx0, long long ny0, long long nz0, int* key, int* val) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    key[r] = (int)(((long long)(cz[r] >> 1) * ny1 + (cy[r] >> 1)) * nx1 + (cx[r] >> 1));
+    const int e = ex[r] + 1;
+    const long long nx1 = lattice_extent(nx0, e), ny1 = lattice_extent(ny0, e);
+    key[r] = (int)(lattice_key_base(nx0, ny0, nz0, e) + ((long long)(cz[r] >> 1) * ny1 + (cy[r] >> 1)) * nx1 +
+                   (cx[r] >> 1));
     val[r] = r;
 }
 __global__ void __launch_bounds__(BLOCK) k_mg_flag(int n, const int* key, int* flag) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) flag[t] = (t == 0 || key[t] != key[t - 1]) ? 1 : 0;
 }
+// the thread of a block's first sorted row walks the block and starts a new
+// aggregate where the next row would pass the bounds
+template <class OP>
+__global__ void __launch_bounds__(BLOCK) k_mg_split(OP op, int n, const int* key, const int* val, int* flag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n || !flag[t]) return;
+    int terms = 0, members = 0;
+    for (int u = t; u < n && (u == t || key[u] == key[t]); u++) {
+        const int w = op.cnt(val[u]) + 1;
+        if (members > 0 && (terms + w > MG_BIG_CAP || members == MG_BIG_MEMBERS)) {
+            flag[u] = 1;
+            terms = members = 0;
+        }
+        terms += w;
+        members++;
+    }
+}
 // aggregate of every row, member lists, block coordinates of the aggregates
+// (those of any member: all share the block)
 __global__ void __launch_bounds__(BLOCK) k_mg_link(int n, const int* key, const int* val, const int* flag,
-                                                   const int* scan, long long nx1, long long ny1, int* agg, int* mem,
-                                                   int* mstart, int* cx1, int* cy1, int* cz1) {
+                                                   const int* scan, const int* cx, const int* cy, const int* cz,
+                                                   const int* ex, int* agg, int* mem, int* mstart, int* cx1, int* cy1,
+                                                   int* cz1, int* ex1) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int I = scan[t] + flag[t] - 1;
-    agg[val[t]] = I;
-    mem[t] = val[t];
+    const int m = val[t];
+    agg[m] = I;
+    mem[t] = m;
     if (flag[t]) {
         mstart[I] = t;
-        const long long k = key[t];
-        cx1[I] = (int)(k % nx1);
-        cy1[I] = (int)((k / nx1) % ny1);
-        cz1[I] = (int)(k / (nx1 * ny1));
+        cx1[I] = cx[m] >> 1;
+        cy1[I] = cy[m] >> 1;
+        cz1[I] = cz[m] >> 1;
+        ex1[I] = ex[m] + 1;
     }
     if (t == n - 1) mstart[I + 1] = n;
 }
@@ -560,7 +610,7 @@ void mg_free(sph_ctx* c, MGState* mg) {
     if (!mg) return;
     for (int l = 0; l < MG_MAX_LEVELS; l++) {
         MGLevel& L = mg->L[l];
-        void* ptrs[] = {L.cx, L.cy, L.cz, L.sl_off, L.cnt, L.cols, L.vals, L.diag, L.agg, L.mstart, L.mem, L.b, L.x, L.y, L.r};
+        void* ptrs[] = {L.cx, L.cy, L.cz, L.ex, L.sl_off, L.cnt, L.cols, L.vals, L.diag, L.agg, L.mstart, L.mem, L.b, L.x, L.y, L.r};
         for (void* q : ptrs) dfree(c, q);
     }
     dfree(c, mg->ainv);
@@ -615,26 +665,32 @@ static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0
     return 0;
 }
 
+// Keys of the aggregation of level l (rows on lattices of exponent at most
+// emax): the blocks of spacings 2 q .. 2^(emax+1) q.
+static long long mg_key_count(const MGLevel& F, int emax) {
+    return lattice_key_base(F.nx, F.ny, F.nz, emax + 2);
+}
+
 // Aggregates of level l (its coordinates set) -> level l + 1 rows,
-// coordinates and link arrays.
-static int mg_aggregate(sph_ctx* c, MGLevel& L0, MGLevel& L1) {
+// coordinates and link arrays.  F: level 0 (lattice extents); emax: the
+// largest lattice exponent of level l's rows.
+template <class OP>
+static int mg_aggregate(sph_ctx* c, OP op, const MGLevel& F, int emax, MGLevel& L0, MGLevel& L1) {
     const int n = L0.n;
-    L1.nx = (L0.nx >> 1) + 1;
-    L1.ny = (L0.ny >> 1) + 1;
-    L1.nz = (L0.nz >> 1) + 1;
     int *key = nullptr, *val = nullptr, *ktmp = nullptr, *vtmp = nullptr, *flag = nullptr, *scan = nullptr;
     SPH_TRY(dalloc(c, &key, n));
     SPH_TRY(dalloc(c, &val, n));
     SPH_TRY(dalloc(c, &ktmp, n));
     SPH_TRY(dalloc(c, &vtmp, n));
-    MGK("mg_key", grid1(n), BLOCK, k_mg_key, n, (const int*)L0.cx, (const int*)L0.cy, (const int*)L0.cz, L1.nx, L1.ny,
-        key, val);
-    SPH_TRY(radix_sort_pairs(c, &key, &val, &ktmp, &vtmp, n, (int)(L1.nx * L1.ny * L1.nz - 1)));
+    MGK("mg_key", grid1(n), BLOCK, k_mg_key, n, (const int*)L0.cx, (const int*)L0.cy, (const int*)L0.cz,
+        (const int*)L0.ex, F.nx, F.ny, F.nz, key, val);
+    SPH_TRY(radix_sort_pairs(c, &key, &val, &ktmp, &vtmp, n, (int)(mg_key_count(F, emax) - 1)));
     dfree(c, ktmp);
     dfree(c, vtmp);
     SPH_TRY(dalloc(c, &flag, n));
     SPH_TRY(dalloc(c, &scan, n));
     MGK("mg_flag", grid1(n), BLOCK, k_mg_flag, n, (const int*)key, flag);
+    MGK("mg_split", grid1(n), BLOCK, k_mg_split<OP>, op, n, (const int*)key, (const int*)val, flag);
     int n1 = 0;
     SPH_TRY(exclusive_scan_i32(c, flag, scan, n, &n1));
     L1.n = n1;
@@ -644,8 +700,10 @@ static int mg_aggregate(sph_ctx* c, MGLevel& L0, MGLevel& L1) {
     SPH_TRY(dalloc(c, &L1.cx, n1));
     SPH_TRY(dalloc(c, &L1.cy, n1));
     SPH_TRY(dalloc(c, &L1.cz, n1));
+    SPH_TRY(dalloc(c, &L1.ex, n1));
     MGK("mg_link", grid1(n), BLOCK, k_mg_link, n, (const int*)key, (const int*)val, (const int*)flag,
-        (const int*)scan, L1.nx, L1.ny, L0.agg, L0.mem, L0.mstart, L1.cx, L1.cy, L1.cz);
+        (const int*)scan, (const int*)L0.cx, (const int*)L0.cy, (const int*)L0.cz, (const int*)L0.ex, L0.agg, L0.mem,
+        L0.mstart, L1.cx, L1.cy, L1.cz, L1.ex);
     dfree(c, key);
     dfree(c, val);
     dfree(c, flag);
@@ -691,6 +749,7 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         return er == hipSuccess ? 0 : fail(c, SPH_E_HIP, std::string(what) + ": " + hipGetErrorString(er));
     };
     MGTRY(dalloc(c, &err, 1));
+    int rmax = 0;   // the largest lattice exponent of the particles
     {
         MGLevel& L0 = mg->L[0];
         L0.n = pp.nr;
@@ -700,10 +759,16 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         MGTRY(dalloc(c, &L0.cx, L0.n));
         MGTRY(dalloc(c, &L0.cy, L0.n));
         MGTRY(dalloc(c, &L0.cz, L0.n));
+        MGTRY(dalloc(c, &L0.ex, L0.n));
+        MGTRY(hipc(hipMemsetAsync(err, 0, sizeof(int), c->stream), "mg: lattice"));
         const double x0 = (double)d.x_min - q, y0 = (double)d.y_min - q, z0 = (double)d.z_min - q;
         hipLaunchKernelGGL(k_mg_coord0, dim3(grid1(L0.n)), dim3(BLOCK), 0, c->stream, L0.n, pp.rows, c->sview(), x0,
-                           y0, z0, q, L0.nx, L0.ny, L0.nz, L0.cx, L0.cy, L0.cz);
+                           y0, z0, q, L0.nx, L0.ny, L0.nz, L0.cx, L0.cy, L0.cz, L0.ex, err);
         MGTRY(launch_check(c, "mg_coord"));
+        MGTRY(hipc(hipMemcpyAsync(&rmax, err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "mg: lattice"));
+        MGTRY(hipc(hipStreamSynchronize(c->stream), "mg: lattice"));
+        // every level's keys are 32-bit
+        if (mg_key_count(L0, rmax + MG_MAX_LEVELS) > 2000000000LL) return fail_out(0);
     }
     int nlev = 1;
     bool good = true;
@@ -712,7 +777,9 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         MGLevel& L0 = mg->L[nlev - 1];
         MGLevel& L1 = mg->L[nlev];
         MGTRY(hipc(hipMemsetAsync(err, 0, sizeof(int), c->stream), "mg: clear flag"));
-        MGTRY(mg_aggregate(c, L0, L1));
+        // level l's rows lie on lattices of exponent at most rmax + l
+        if (nlev == 1) MGTRY(mg_aggregate(c, FineOp{pp}, mg->L[0], rmax, L0, L1));
+        else MGTRY(mg_aggregate(c, coarse_op(L0), mg->L[0], rmax + nlev - 1, L0, L1));
         if (L1.n * 10LL > L0.n * 9LL) { nlev++; good = false; *why = SPH_MG_STALLED; break; }
         int lok = 0;
         int code = 0;
@@ -767,8 +834,8 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
     // coordinates are needed only while building
     for (int l = 0; l < nlev; l++) {
         MGLevel& L = mg->L[l];
-        dfree(c, L.cx); dfree(c, L.cy); dfree(c, L.cz);
-        L.cx = L.cy = L.cz = nullptr;
+        dfree(c, L.cx); dfree(c, L.cy); dfree(c, L.cz); dfree(c, L.ex);
+        L.cx = L.cy = L.cz = L.ex = nullptr;
     }
     dfree(c, err);
 #undef MGTRY
