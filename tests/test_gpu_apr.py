@@ -104,8 +104,8 @@ def test_apr_mixed_runs():
 
 
 def test_apr_multigrid_runs():
-    """The multigrid preconditioner on a refined case: aggregation at the
-    finest particles' spacing, long rows (tails) in the level-0 passes and
+    """The multigrid preconditioner on a refined case: aggregation on each
+    particle's own lattice, long rows (tails) in the level-0 passes and
     in the Galerkin products; every solve converges with it, in fewer
     iterations than the Jacobi form, and the result stays as close to a
     tightly converged FP64 solve as the PPE tolerance allows."""
