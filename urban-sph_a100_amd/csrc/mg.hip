@@ -383,25 +383,29 @@ __global__ void __launch_bounds__(BLOCK) k_mg_dense_fill(CoarseOp op, int n, dou
     const int c = op.cnt(r);
     for (int k = 0; k < c; k++) a[(long long)r * n + op.col(r, k)] = -(double)op.val(r, k);
 }
-__global__ void __launch_bounds__(BLOCK) k_gj_pivot(int n, const double* a, int k, double* rowk, double* colk,
-                                                    int* err) {
-    const double piv = a[(long long)k * n + k];
-    const bool bad = !(fabs(piv) > 0.0) || !(fabs(piv) <= DBL_MAX);
-    if (bad) {
-        if (threadIdx.x == 0) err[0] = SPH_MG_PIVOT;
-        return;
-    }
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        rowk[j] = (j == k ? 1.0 : a[(long long)k * n + j]) / piv;
-        colk[j] = a[(long long)j * n + k];
-    }
+// One launch per pivot k.  rowk / colk hold row k and column k of the
+// matrix as the previous step left it; the step writes row k + 1 and
+// column k + 1 as it leaves them into rown / coln for the next launch (the
+// buffers alternate).  Pivot row: a[k][j] / piv (1 / piv at j = k); other
+// rows: a[i][j] - a[i][k] a[k][j] / piv (0 - ... at j = k).
+__global__ void __launch_bounds__(BLOCK) k_gj_first(int n, const double* a, double* row0, double* col0) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    row0[j] = a[j];
+    col0[j] = a[(long long)j * n];
 }
-__global__ void __launch_bounds__(BLOCK) k_gj_elim(int n, double* a, int k, const double* rowk, const double* colk) {
+__global__ void __launch_bounds__(BLOCK) k_gj_step(int n, double* a, int k, const double* rowk, const double* colk,
+                                                   double* rown, double* coln, int* err) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)n * n) return;
     const int i = (int)(idx / n), j = (int)(idx % n);
-    if (i == k) a[idx] = rowk[j];
-    else a[idx] = (j == k ? 0.0 : a[idx]) - colk[i] * rowk[j];
+    const double piv = rowk[k];
+    if (idx == 0 && (!(fabs(piv) > 0.0) || !(fabs(piv) <= DBL_MAX))) err[0] = SPH_MG_PIVOT;
+    const double rj = (j == k ? 1.0 : rowk[j]) / piv;
+    const double v = i == k ? rj : (j == k ? 0.0 : a[idx]) - colk[i] * rj;
+    a[idx] = v;
+    if (i == k + 1) rown[j] = v;
+    if (j == k + 1) coln[i] = v;
 }
 __global__ void __launch_bounds__(BLOCK) k_mg_tofloat(long long n, const double* a, float* b) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -798,18 +802,19 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         const long long nn = (long long)n * n;
         double *a = nullptr, *rowk = nullptr, *colk = nullptr;
         MGTRY(dalloc(c, &a, (size_t)nn));
-        MGTRY(dalloc(c, &rowk, n));
-        MGTRY(dalloc(c, &colk, n));
+        MGTRY(dalloc(c, &rowk, 2 * n));
+        MGTRY(dalloc(c, &colk, 2 * n));
         MGTRY(hipc(hipMemsetAsync(err, 0, sizeof(int), c->stream), "mg: coarsest level"));
         hipLaunchKernelGGL(k_mg_dense_zero, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, nn, a);
         MGTRY(launch_check(c, "mg_dense_zero"));
         hipLaunchKernelGGL(k_mg_dense_fill, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, coarse_op(LC), n, a);
         MGTRY(launch_check(c, "mg_dense_fill"));
         MGTRY(prof_begin(c, "mg_gauss_jordan", 0.0));
+        hipLaunchKernelGGL(k_gj_first, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const double*)a, rowk, colk);
         for (int k = 0; k < n; k++) {
-            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(BLOCK), 0, c->stream, n, (const double*)a, k, rowk, colk, err);
-            hipLaunchKernelGGL(k_gj_elim, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, n, a, k, (const double*)rowk,
-                               (const double*)colk);
+            const int cur = (k & 1) * n, nxt = n - cur;
+            hipLaunchKernelGGL(k_gj_step, dim3(grid1(nn)), dim3(BLOCK), 0, c->stream, n, a, k,
+                               (const double*)rowk + cur, (const double*)colk + cur, rowk + nxt, colk + nxt, err);
         }
         MGTRY(launch_check(c, "mg_gauss_jordan"));
         MGTRY(prof_end(c));
