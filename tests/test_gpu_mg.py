@@ -158,5 +158,5 @@ def test_multigrid_profiled_build_and_ale():
         assert r.precond == abi.PRECOND_MULTIGRID and r.converged == 1
     prof = g.profile_report()
     g.profile(False)
-    for k in ("mg_rap_count", "mg_rap_fill", "mg_gauss_jordan", "mg_res0", "mg_post0", "spmv"):
+    for k in ("mg_rap_terms", "mg_rap_fill", "mg_rap_pack", "mg_gauss_jordan", "mg_res0", "mg_post0", "spmv"):
         assert k in prof and prof[k][0] > 0, k

@@ -247,20 +247,21 @@ __global__ void __launch_bounds__(BLOCK) k_mg_link(int n, const int* key, const 
 // the hierarchy is bit-reproducible.  Terms are float values added in
 // double.  CAP = MG_ENT_CAP: rows of at most CAP terms, values staged in
 // LDS; CAP = MG_BIG_CAP: only the larger rows, values read back from the
-// operator.  COUNT pass: the row's off-diagonal count.  FILL pass: columns
-// ascending, values -sum, the diagonal.  Each lane takes the columns whose
-// first sorted term falls in its run of sorted positions and sums each of
-// them to its end.  err: SPH_MG_TERMS (too many terms or members),
-// SPH_MG_SPAN (columns too far apart).
-template <bool FILL, class OP, int CAP>
+// operator.  Output: the off-diagonal count, the columns ascending and
+// their values -sum from row_off[I] on (room for the row's terms), the
+// diagonal.  Each lane takes the columns whose first sorted term falls in
+// its run of sorted positions and sums each of them to its end.  err:
+// SPH_MG_TERMS (too many terms or members), SPH_MG_SPAN (columns too far
+// apart).
+template <class OP, int CAP>
 __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg, const int* mstart, const int* mem,
-                                                  int* cnt1, const long long* sl_off1, int* cols1, float* vals1,
+                                                  int* cnt1, const long long* row_off, int* cols1, float* vals1,
                                                   float* diag1, int* err) {
     constexpr bool BIG = CAP > MG_ENT_CAP;
     constexpr int TB = BIG ? 13 : 11;               // term-number bits
     static_assert(CAP <= (1 << TB), "term numbers fit their key bits");
     __shared__ unsigned sk[CAP];                    // (column - jmin) << TB | term number
-    __shared__ float sv[(FILL && !BIG) ? CAP : 1];  // term values by term number
+    __shared__ float sv[BIG ? 1 : CAP];             // term values by term number
     __shared__ int moff[BIG ? MG_BIG_MEMBERS + 1 : 1];   // first term of each member
     const int I = blockIdx.x;
     const int lane = threadIdx.x;
@@ -271,7 +272,7 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     nterm = wave_sum(nterm);
     if (BIG ? nterm <= MG_ENT_CAP : nterm > MG_ENT_CAP) return;   // the other kernel's row
     if (nterm > CAP || (BIG && t1 - t0 > MG_BIG_MEMBERS)) {
-        if (lane == 0) { err[0] = SPH_MG_TERMS; if (!FILL) cnt1[I] = 0; }
+        if (lane == 0) err[0] = SPH_MG_TERMS;
         return;
     }
     int np2 = 1;
@@ -287,7 +288,7 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
             sk[q] = (unsigned)J;
             jmin = min(jmin, J);
             jmax = max(jmax, J);
-            if (FILL && !BIG) sv[q] = k < 0 ? op.dg(m) : -op.val(m, k);
+            if (!BIG) sv[q] = k < 0 ? op.dg(m) : -op.val(m, k);
         }
         pos += c + 1;
     }
@@ -295,7 +296,7 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     jmin = -wave_max(-jmin);
     jmax = wave_max(jmax);
     if (jmax - jmin >= (1 << (32 - TB)) - 1) {
-        if (lane == 0) { err[0] = SPH_MG_SPAN; if (!FILL) cnt1[I] = 0; }
+        if (lane == 0) err[0] = SPH_MG_SPAN;
         return;
     }
     __syncthreads();
@@ -327,10 +328,7 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     const int at = wave_excl(own);
     const int total = wave_sum(own);
     const int sI = wave_sum(below);
-    if (!FILL) {
-        if (lane == 0) cnt1[I] = total - 1;
-        return;
-    }
+    if (lane == 0) cnt1[I] = total - 1;
     // a term's value: from LDS, or (large rows) from its member's entry
     auto term_val = [&](int q) -> float {
         if (!BIG) return sv[q];
@@ -339,7 +337,7 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
         const int m = mem[t0 + mi], k = q - moff[mi] - 1;
         return k < 0 ? op.dg(m) : -op.val(m, k);
     };
-    const long long base = prow(sl_off1, I);
+    const long long base = row_off[I];
     int s = at;
     for (int p = p0; p < p1; p++) {
         if (!starts(p)) continue;
@@ -354,10 +352,30 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
             diag1[I] = (float)acc;
         } else {
             const int o = s - (s > sI ? 1 : 0);
-            cols1[base + pslot(o)] = J;
-            vals1[base + pslot(o)] = (float)(-acc);
+            cols1[base + o] = J;
+            vals1[base + o] = (float)(-acc);
         }
         s++;
+    }
+}
+// room for each coarse row's columns: its terms (members' entries + 1)
+template <class OP>
+__global__ void __launch_bounds__(BLOCK) k_mg_terms(OP op, int n1, const int* mstart, const int* mem, long long* t) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n1) return;
+    long long s = 0;
+    for (int q = mstart[I]; q < mstart[I + 1]; q++) s += op.cnt(mem[q]) + 1;
+    t[I] = s;
+}
+// rows packed by row_off -> the sliced layout
+__global__ void __launch_bounds__(BLOCK) k_mg_pack(int n1, const long long* row_off, const int* cnt, const int* pc,
+                                                   const float* pv, const long long* sl_off, int* cols, float* vals) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n1) return;
+    const long long src = row_off[r], dst = prow(sl_off, r);
+    for (int k = 0; k < cnt[r]; k++) {
+        cols[dst + pslot(k)] = pc[src + k];
+        vals[dst + pslot(k)] = pv[src + k];
     }
 }
 __global__ void __launch_bounds__(BLOCK) k_mg_width(int n, int nslices, const int* cnt, long long* w) {
@@ -637,17 +655,35 @@ void mg_free(sph_ctx* c, MGState* mg) {
 template <class OP>
 static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0, int* err, int* ok, int* why) {
     const int n1 = L1.n;
+    // one pass: each row's columns into room for its terms, then packed
+    long long *row_off = nullptr, *tcount = nullptr;
+    long long room = 0;
+    SPH_TRY(dalloc(c, &tcount, (size_t)n1 + 1));
+    SPH_TRY(dalloc(c, &row_off, (size_t)n1 + 1));
+    MGK("mg_rap_terms", grid1(n1), BLOCK, k_mg_terms<OP>, src, n1, (const int*)L0.mstart, (const int*)L0.mem, tcount);
+    SPH_TRY(exclusive_scan_i64(c, tcount, row_off, n1, &room));
+    dfree(c, tcount);
+    int* pc = nullptr;
+    float* pv = nullptr;
+    SPH_TRY(dalloc(c, &pc, (size_t)std::max(1LL, room)));
+    SPH_TRY(dalloc(c, &pv, (size_t)std::max(1LL, room)));
     SPH_TRY(dalloc(c, &L1.cnt, n1));
-    MGK("mg_rap_count", n1, SLICE, (k_mg_rap<false, OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg,
-        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)nullptr, (int*)nullptr, (float*)nullptr,
-        (float*)nullptr, err);
-    MGK("mg_rap_count", n1, SLICE, (k_mg_rap<false, OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg,
-        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)nullptr, (int*)nullptr, (float*)nullptr,
-        (float*)nullptr, err);
+    SPH_TRY(dalloc(c, &L1.diag, n1));
+    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
+        (const int*)L0.mem, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
+    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
+        (const int*)L0.mem, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
     int h_err = 0;
     SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
-    if (h_err) { *ok = 0; *why = h_err; return 0; }
+    if (h_err) {
+        dfree(c, row_off);
+        dfree(c, pc);
+        dfree(c, pv);
+        *ok = 0;
+        *why = h_err;
+        return 0;
+    }
     L1.nslices = (n1 + SLICE - 1) / SLICE;
     long long* w = nullptr;
     SPH_TRY(dalloc(c, &w, (size_t)L1.nslices + 1));
@@ -657,15 +693,13 @@ static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0
     dfree(c, w);
     SPH_TRY(dalloc(c, &L1.cols, (size_t)std::max(1LL, L1.slots)));
     SPH_TRY(dalloc(c, &L1.vals, (size_t)std::max(1LL, L1.slots)));
-    SPH_TRY(dalloc(c, &L1.diag, n1));
-    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<true, OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg,
-        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)L1.sl_off, L1.cols, L1.vals, L1.diag, err);
-    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<true, OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg,
-        (const int*)L0.mstart, (const int*)L0.mem, L1.cnt, (const long long*)L1.sl_off, L1.cols, L1.vals, L1.diag, err);
-    SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    SPH_HIP(c, hipStreamSynchronize(c->stream));
-    *ok = h_err == 0;
-    *why = h_err;
+    MGK("mg_rap_pack", grid1(n1), BLOCK, k_mg_pack, n1, (const long long*)row_off, (const int*)L1.cnt,
+        (const int*)pc, (const float*)pv, (const long long*)L1.sl_off, L1.cols, L1.vals);
+    dfree(c, row_off);
+    dfree(c, pc);
+    dfree(c, pv);
+    *ok = 1;
+    *why = 0;
     return 0;
 }
 
