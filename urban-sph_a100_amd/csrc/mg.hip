@@ -255,8 +255,8 @@ __global__ void __launch_bounds__(BLOCK) k_mg_link(int n, const int* key, const 
 // apart).
 template <class OP, int CAP>
 __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg, const int* mstart, const int* mem,
-                                                  int* cnt1, const long long* row_off, int* cols1, float* vals1,
-                                                  float* diag1, int* err) {
+                                                  const int* redo, int* cnt1, const long long* row_off, int* cols1,
+                                                  float* vals1, float* diag1, int* err) {
     constexpr bool BIG = CAP > MG_ENT_CAP;
     constexpr int TB = BIG ? 13 : 11;               // term-number bits
     static_assert(CAP <= (1 << TB), "term numbers fit their key bits");
@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
     __shared__ int moff[BIG ? MG_BIG_MEMBERS + 1 : 1];   // first term of each member
     const int I = blockIdx.x;
     const int lane = threadIdx.x;
-    if (I >= n1) return;
+    if (I >= n1 || !redo[I]) return;
     const int t0 = mstart[I], t1 = mstart[I + 1];
     int nterm = 0;
     for (int t = t0 + lane; t < t1; t += SLICE) nterm += op.cnt(mem[t]) + 1;
@@ -358,6 +358,115 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
         s++;
     }
 }
+// The common form of a coarse row: its terms go into an LDS hash table of
+// MG_HASH columns, each summed in 64-bit fixed point (the term times
+// `scale`, a power of two that keeps every row's sums below 2^62), so the
+// sums do not depend on the order the lanes add them in and the hierarchy is
+// bit-reproducible without sorting the terms; then the distinct columns are
+// sorted.  A row of more than MG_HASH / 2 distinct columns (the table at most
+// half full, so probes stay short; the count does not depend on the order
+// of insertion) is flagged in redo, and counted in nredo, for the sorting
+// kernels above.
+constexpr int MG_HASH = 512;
+template <class OP>
+__global__ void __launch_bounds__(SLICE) k_mg_rap_hash(OP op, int n1, const int* agg, const int* mstart,
+                                                       const int* mem, double scale, int* redo, int* nredo,
+                                                       int* cnt1, const long long* row_off, int* cols1, float* vals1,
+                                                       float* diag1, int* err) {
+    __shared__ int hk[MG_HASH];                    // column, or -1
+    __shared__ long long hs[MG_HASH];              // its fixed-point sum
+    __shared__ unsigned long long key[MG_HASH / 2];   // column << 32 | table slot, sorted
+    __shared__ int full, ndist;
+    const int I = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (I >= n1) return;
+    const int t0 = mstart[I], t1 = mstart[I + 1];
+    int nterm = 0;
+    for (int t = t0 + lane; t < t1; t += SLICE) nterm += op.cnt(mem[t]) + 1;
+    nterm = wave_sum(nterm);
+    if (nterm > MG_BIG_CAP) {   // the sums' bound
+        if (lane == 0) err[0] = SPH_MG_TERMS;
+        return;
+    }
+    for (int h = lane; h < MG_HASH; h += SLICE) { hk[h] = -1; hs[h] = 0; }
+    if (lane == 0) { full = 0; ndist = 0; }
+    __syncthreads();
+    for (int t = t0; t < t1 && !full; t++) {
+        const int m = mem[t];
+        const int c = op.cnt(m);
+        for (int k = lane - 1; k < c && !full; k += SLICE) {
+            const int J = k < 0 ? I : agg[op.col(m, k)];
+            const double v = k < 0 ? (double)op.dg(m) : -(double)op.val(m, k);
+            const long long f = llrint(v * scale);
+            int h = (int)(((unsigned)J * 2654435761u) >> 23) & (MG_HASH - 1);
+            int probe = 0;
+            for (; probe < MG_HASH; probe++) {
+                const int old = atomicCAS(&hk[h], -1, J);
+                if (old == -1 && atomicAdd(&ndist, 1) >= MG_HASH / 2) full = 1;
+                if (old == -1 || old == J) break;
+                h = (h + 1) & (MG_HASH - 1);
+            }
+            if (probe < MG_HASH) atomicAdd((unsigned long long*)&hs[h], (unsigned long long)f);
+            else full = 1;
+        }
+    }
+    __syncthreads();
+    if (full) {
+        if (lane == 0) { redo[I] = 1; atomicAdd(nredo, 1); }
+        return;
+    }
+    // distinct columns, ascending
+    int own = 0;
+    for (int h = lane; h < MG_HASH; h += SLICE) own += hk[h] >= 0 ? 1 : 0;
+    int w = wave_excl(own);
+    const int total = wave_sum(own);
+    for (int h = lane; h < MG_HASH; h += SLICE)
+        if (hk[h] >= 0) key[w++] = ((unsigned long long)(unsigned)hk[h] << 32) | (unsigned)h;
+    int np2 = 1;
+    while (np2 < total) np2 <<= 1;
+    for (int p = total + lane; p < np2; p += SLICE) key[p] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= np2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = lane; i < np2; i += SLICE) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long a = key[i], b = key[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int below = 0;
+    for (int p = lane; p < total; p += SLICE) below += (int)(key[p] >> 32) < I ? 1 : 0;
+    const int sI = wave_sum(below);
+    if (lane == 0) { cnt1[I] = total - 1; redo[I] = 0; }
+    const long long base = row_off[I];
+    for (int p = lane; p < total; p += SLICE) {
+        const int J = (int)(key[p] >> 32);
+        const double v = (double)hs[(int)(key[p] & 0xffffffffu)] / scale;
+        if (p == sI) {
+            diag1[I] = (float)v;
+        } else {
+            const int o = p - (p > sI ? 1 : 0);
+            cols1[base + o] = J;
+            vals1[base + o] = (float)(-v);
+        }
+    }
+}
+// largest |entry| of an operator (the diagonal included), as float bits
+template <class OP>
+__global__ void __launch_bounds__(BLOCK) k_mg_maxabs(OP op, int n, unsigned* out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    float m = fabsf(op.dg(r));
+    const int c = op.cnt(r);
+    for (int k = 0; k < c; k++) m = fmaxf(m, fabsf(op.val(r, k)));
+    if (m == m) atomicMax(out, __float_as_uint(m));
+}
+
 // room for each coarse row's columns: its terms (members' entries + 1)
 template <class OP>
 __global__ void __launch_bounds__(BLOCK) k_mg_terms(OP op, int n1, const int* mstart, const int* mem, long long* t) {
@@ -669,10 +778,33 @@ static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0
     SPH_TRY(dalloc(c, &pv, (size_t)std::max(1LL, room)));
     SPH_TRY(dalloc(c, &L1.cnt, n1));
     SPH_TRY(dalloc(c, &L1.diag, n1));
-    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
-        (const int*)L0.mem, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
-    MGK("mg_rap_fill", n1, SLICE, (k_mg_rap<OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
-        (const int*)L0.mem, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
+    // fixed-point scale of the hashed rows: 2^S with |entry| * 2^S <= 2^49,
+    // so that MG_BIG_CAP (2^13) terms sum below 2^62
+    int* redo = nullptr;
+    int* aux = nullptr;   // [0]: largest |entry| (float bits), [1]: rows left to the sorting kernels
+    SPH_TRY(dalloc(c, &redo, n1));
+    SPH_TRY(dalloc(c, &aux, 2));
+    SPH_HIP(c, hipMemsetAsync(aux, 0, 2 * sizeof(int), c->stream));
+    MGK("mg_rap_scale", grid1(L0.n), BLOCK, k_mg_maxabs<OP>, src, L0.n, (unsigned*)aux);
+    int h_aux[2] = {0, 0};
+    SPH_HIP(c, hipMemcpyAsync(h_aux, aux, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    float amax = 0.0f;
+    std::memcpy(&amax, &h_aux[0], sizeof(float));
+    if (!(amax > 0.0f) || !std::isfinite(amax)) amax = 1.0f;
+    const double scale = std::ldexp(1.0, 49 - (std::ilogb((double)amax) + 1));
+    MGK("mg_rap_fill", n1, SLICE, k_mg_rap_hash<OP>, src, n1, (const int*)L0.agg, (const int*)L0.mstart,
+        (const int*)L0.mem, scale, redo, aux + 1, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
+    SPH_HIP(c, hipMemcpyAsync(h_aux + 1, aux + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    if (h_aux[1] > 0) {
+        MGK("mg_rap_sort", n1, SLICE, (k_mg_rap<OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
+            (const int*)L0.mem, (const int*)redo, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
+        MGK("mg_rap_sort", n1, SLICE, (k_mg_rap<OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
+            (const int*)L0.mem, (const int*)redo, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
+    }
+    dfree(c, redo);
+    dfree(c, aux);
     int h_err = 0;
     SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
