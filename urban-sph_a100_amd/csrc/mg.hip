@@ -7,12 +7,13 @@
 //                A_{l+1} = P^T A_l P, P the 0/1 aggregate indicator;
 //   coarsest     the first level of at most MG_COARSE_MAX unknowns, inverted
 //                densely (Gauss-Jordan in double, stored as float).
-// Level-0 lattice positions come from the particle positions at the PPE's
-// lattice spacing; a level's block coordinates are the previous level's
-// halved.  Every coarse row is formed by one wave that sorts its columns in
-// LDS and adds each column's terms in a fixed order (members in row order,
-// diagonal first, then entries in storage order), so the hierarchy is
-// bit-reproducible.
+// Level-0 lattice positions come from the particle positions, each on the
+// lattice of its own spacing; a level's block coordinates are the previous
+// level's halved.  Every coarse row is formed by one wave that sums its
+// columns' terms in 64-bit fixed point in an LDS hash table (integer sums
+// do not depend on the order of the lanes) or, for a row of many distinct
+// columns, sorts its terms in LDS and adds each column's terms in a fixed
+// order; either way the hierarchy is bit-reproducible.
 //
 // Application (one V-cycle, z = M^-1 b): at every level but the coarsest
 //     x = w b / diag;  r = b - A x;  b_next = P^T r
