@@ -269,9 +269,8 @@ __global__ void __launch_bounds__(BLOCK) k_cell_start(int ncs, const int* keys, 
 // order v, and each round of 64 candidates is compacted in order with a
 // ballot, so the list is the one a serial scan in cell order produces.
 template <bool FILL>
-__device__ __forceinline__ void nl_wave(int i, const Grid& g, const double* x, const double* y, const double* z,
-                                        const double* h, const int* ncell, double kappa, int* cnt_out,
-                                        unsigned* out) {
+__device__ __forceinline__ int nl_wave(int i, const Grid& g, const double* x, const double* y, const double* z,
+                                       const double* h, const int* ncell, double kappa, unsigned* out, int cap) {
     const int lane = threadIdx.x & 63;
     const sph_domain& d = g.d;
     const double xi = x[i], yi = y[i], zi = z[i], hi = h[i];
@@ -348,25 +347,42 @@ __device__ __forceinline__ void nl_wave(int i, const Grid& g, const double* x, c
             const unsigned long long m = __ballot(f != 0);
             if (FILL && f) {
                 const int below = __popcll(m & ((1ULL << lane) - 1ULL));
-                out[(size_t)(cnt + below) * SLICE] = (unsigned)j | f;
+                if (cnt + below < cap) out[(size_t)(cnt + below) * SLICE] = (unsigned)j | f;
             }
             cnt += __popcll(m);
         }
     }
-    if (!FILL && lane == 0) *cnt_out = cnt;
+    return cnt;
 }
 __global__ void __launch_bounds__(BLOCK) k_nl_count(int n, Grid g, const double* x, const double* y, const double* z,
                                                     const double* h, const int* ncell, double kappa, int* cnt) {
     const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (i >= n) return;   // whole waves
-    nl_wave<false>(i, g, x, y, z, h, ncell, kappa, cnt + i, nullptr);
+    const int m = nl_wave<false>(i, g, x, y, z, h, ncell, kappa, nullptr, 0);
+    if ((threadIdx.x & 63) == 0) cnt[i] = m;
 }
+// fill pass; with cnt (a fill without a count pass) into the room of each
+// slice, writing the counts and over = 1 when a list does not fit
 __global__ void __launch_bounds__(BLOCK) k_nl_fill(int n, Grid g, const double* x, const double* y, const double* z,
                                                    const double* h, const int* ncell, double kappa,
-                                                   const long long* slice_off, unsigned* ent) {
+                                                   const long long* slice_off, long long total, unsigned* ent, int* cnt,
+                                                   int* over) {
     const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (i >= n) return;   // whole waves
-    nl_wave<true>(i, g, x, y, z, h, ncell, kappa, nullptr, ent + slice_off[i >> 6] + (i & 63));
+    const int s = i >> 6;
+    const long long end = s + 1 < (n + SLICE - 1) / SLICE ? slice_off[s + 1] : total;
+    const int cap = cnt ? (int)((end - slice_off[s]) / SLICE) : INT_MAX;
+    const int m = nl_wave<true>(i, g, x, y, z, h, ncell, kappa, ent + slice_off[s] + (i & 63), cap);
+    if (cnt && (threadIdx.x & 63) == 0) {
+        cnt[i] = m;
+        if (m > cap) over[0] = 1;
+    }
+}
+// room of each slice for a fill without a count pass
+__global__ void __launch_bounds__(BLOCK) k_slice_spare(int nslices, const long long* prev, long long spare,
+                                                       long long* w) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslices) w[s] = prev[s] + spare * SLICE;
 }
 __global__ void __launch_bounds__(BLOCK) k_slice_width(int n, int nslices, const int* cnt, long long* w) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -415,6 +431,13 @@ static int permute_state(sph_ctx* c) {
     return 0;
 }
 
+// The list of a search: one wave per particle.  When the previous search
+// had as many slices (the particles keep their cell order, and move little
+// between searches), the fill runs at once into each slice's previous
+// width plus NL_SPARE entries; if a list does not fit, the search counts
+// first and fills slices of exact widths.  The entries are the same either
+// way.
+constexpr int NL_SPARE = 8;
 static int build_neighbor_list(sph_ctx* c) {
     const int n = c->n;
     int nslices = (n + SLICE - 1) / SLICE;
@@ -423,13 +446,44 @@ static int build_neighbor_list(sph_ctx* c) {
     dfree(c, c->nl_ent); c->nl_ent = nullptr;
     SPH_TRY(dalloc(c, &c->nl_cnt, n));
     SPH_TRY(dalloc(c, &c->nl_slice_off, nslices + 1));
-    long long* w = nullptr;
-    SPH_TRY(dalloc(c, &w, nslices + 1));
     Grid g = c->grid();
     const double *x = c->pf[SPH_F_X], *y = c->pf[SPH_F_Y], *z = c->pf[SPH_F_Z], *h = c->pf[SPH_F_H];
-    SPH_TRY(prof_begin(c, "nl_count", 0));
-    // one wave per particle
     const unsigned nlgrid = (unsigned)(((long long)n * 64 + BLOCK - 1) / BLOCK);
+    long long* w = nullptr;
+    SPH_TRY(dalloc(c, &w, nslices + 1));
+    long long total = 0;
+    if (c->nl_wprev && c->nl_wprev_n == nslices) {
+        int* over = nullptr;
+        SPH_TRY(dalloc(c, &over, 1));
+        SPH_HIP(c, hipMemsetAsync(over, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_slice_spare, dim3(grid1(nslices)), dim3(BLOCK), 0, c->stream, nslices,
+                           (const long long*)c->nl_wprev, (long long)NL_SPARE, w);
+        SPH_TRY(launch_check(c, "slice_width"));
+        SPH_TRY(exclusive_scan_i64(c, w, c->nl_slice_off, nslices, &total));
+        SPH_TRY(dalloc(c, &c->nl_ent, (size_t)total + 1));
+        SPH_TRY(prof_begin(c, "nl_fill", 0));
+        hipLaunchKernelGGL(k_nl_fill, dim3(nlgrid), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
+                           (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, (const long long*)c->nl_slice_off, total,
+                           c->nl_ent, c->nl_cnt, over);
+        SPH_TRY(launch_check(c, "nl_fill"));
+        SPH_TRY(prof_end(c));
+        int h_over = 0;
+        SPH_HIP(c, hipMemcpyAsync(&h_over, over, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        SPH_HIP(c, hipStreamSynchronize(c->stream));
+        dfree(c, over);
+        if (!h_over) {
+            // exact widths of this search: the next one's guess
+            hipLaunchKernelGGL(k_slice_width, dim3(grid1(nslices)), dim3(BLOCK), 0, c->stream, n, nslices,
+                               (const int*)c->nl_cnt, c->nl_wprev);
+            SPH_TRY(launch_check(c, "slice_width"));
+            c->nl_total = total;
+            dfree(c, w);
+            return 0;
+        }
+        dfree(c, c->nl_ent);
+        c->nl_ent = nullptr;
+    }
+    SPH_TRY(prof_begin(c, "nl_count", 0));
     hipLaunchKernelGGL(k_nl_count, dim3(nlgrid), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
                        (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, c->nl_cnt);
     SPH_TRY(launch_check(c, "nl_count"));
@@ -437,14 +491,21 @@ static int build_neighbor_list(sph_ctx* c) {
     hipLaunchKernelGGL(k_slice_width, dim3(grid1(nslices)), dim3(BLOCK), 0, c->stream, n, nslices,
                        (const int*)c->nl_cnt, w);
     SPH_TRY(launch_check(c, "slice_width"));
-    long long total = 0;
+    if (c->nl_wprev_n != nslices) {
+        dfree(c, c->nl_wprev);
+        c->nl_wprev = nullptr;
+        SPH_TRY(dalloc(c, &c->nl_wprev, nslices + 1));
+        c->nl_wprev_n = nslices;
+    }
+    SPH_HIP(c, hipMemcpyAsync(c->nl_wprev, w, sizeof(long long) * nslices, hipMemcpyDeviceToDevice, c->stream));
     SPH_TRY(exclusive_scan_i64(c, w, c->nl_slice_off, nslices, &total));
     dfree(c, w);
     c->nl_total = total;
     SPH_TRY(dalloc(c, &c->nl_ent, (size_t)total + 1));
     SPH_TRY(prof_begin(c, "nl_fill", 0));
     hipLaunchKernelGGL(k_nl_fill, dim3(nlgrid), dim3(BLOCK), 0, c->stream, n, g, x, y, z, h,
-                       (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, (const long long*)c->nl_slice_off, c->nl_ent);
+                       (const int*)c->pi[SPH_I_NCELL], c->prm.kappa, (const long long*)c->nl_slice_off, total,
+                       c->nl_ent, (int*)nullptr, (int*)nullptr);
     SPH_TRY(launch_check(c, "nl_fill"));
     SPH_TRY(prof_end(c));
     return 0;

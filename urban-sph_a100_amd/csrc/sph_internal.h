@@ -318,6 +318,8 @@ struct sph_ctx {
     int* sort_hist = nullptr; int sort_hist_n = 0;
     long long* nl_slice_off = nullptr; int* nl_cnt = nullptr; unsigned* nl_ent = nullptr;
     long long nl_total = 0;
+    long long* nl_wprev = nullptr;   // slice widths of the last search (room for the next)
+    int nl_wprev_n = 0;              // its slices
     int* scan_tmp = nullptr; size_t scan_tmp_n = 0;
 
     float* stat_buf = nullptr;           // stats reduction scratch
