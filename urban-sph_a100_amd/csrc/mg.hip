@@ -69,7 +69,13 @@ struct MGLevel {
     int* mem = nullptr;      // this level's rows grouped by aggregate, ascending within each
     // V-cycle vectors
     float *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
+    // coarse level: rows of more than MG_CLONG entries, whose products a
+    // wave per row forms before each pass (lidx: row -> list index or -1)
+    int nlong = 0;
+    int *lrows = nullptr, *lidx = nullptr;
+    double* lsum = nullptr;
 };
+constexpr int MG_CLONG = 128;
 struct MGState {
     long long id = 0;        // distinct for every hierarchy built (table signatures)
     int nlev = 0;            // levels, the PPE and the coarsest included
@@ -559,6 +565,9 @@ __global__ void __launch_bounds__(BLOCK) k_mg_pack16(PPEView pp, unsigned* ev) {
 // read through the case's descriptor).
 struct MGLv {
     int n, lc;                    // rows (0: no such level); the case's coarsest level
+    int nlong;                    // coarse level: its long rows (lrows, lidx, lsum)
+    const int* lrows;
+    const int* lidx;
     const long long* sl_off;      // operator of a coarse level
     const int* cnt;
     const int* cols;
@@ -570,7 +579,7 @@ struct MGLv {
     float *b, *x, *y, *r;
     const float* ainv;            // coarsest level: dense inverse
     const unsigned* ev;           // level 0: compact operator, or null
-    double* lsum;                 // level 0: tail sums of the long rows
+    double* lsum;                 // level 0: tail sums of the long rows; coarse: long rows' products
 };
 __device__ __forceinline__ CoarseOp lv_op(const MGLv& L) { return CoarseOp{L.sl_off, L.cnt, L.cols, L.vals, L.diag}; }
 
@@ -606,6 +615,14 @@ __device__ __forceinline__ double mg_row16(const PPEView& pp, const unsigned* ev
     }
     for (; k < m; k++) s += term(__builtin_nontemporal_load(p + pslot(k)));
     return s;
+}
+
+// (A x)_row of a coarse level over its x; a long row's product comes from
+// k_mg_clong
+__device__ __forceinline__ double coarse_ax(const MGLv& L, int row) {
+    const int u = L.lidx ? L.lidx[row] : -1;
+    if (u >= 0) return (double)L.diag[row] * (double)L.x[row] - L.lsum[u];
+    return coarse_row(lv_op(L), row, L.x);
 }
 
 // block -> (case, first row of the level); the block returns when the
@@ -646,7 +663,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_res(const CaseDesc* cds, const MGL
         }
     } else {
         if (row >= L.n) return;
-        L.r[row] = (float)((double)L.b[row] - coarse_row(lv_op(L), row, L.x));
+        L.r[row] = (float)((double)L.b[row] - coarse_ax(L, row));
     }
 }
 // z = x + w (b - A x) / diag  (level 0: into p-hat or s-hat)
@@ -671,7 +688,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(const CaseDesc* cds, const MG
         }
     } else {
         if (row >= L.n) return;
-        const double ax = coarse_row(lv_op(L), row, L.x);
+        const double ax = coarse_ax(L, row);
         L.y[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)L.b[row] - ax) / (double)L.diag[row]);
     }
 }
@@ -715,6 +732,38 @@ __global__ void __launch_bounds__(BLOCK) k_mg_long(const CaseDesc* cds, const MG
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (lane == 0) L.lsum[u] = s;
 }
+// coarse level l: the off-diagonal products of its long rows over x, one
+// wave per row, lanes over its entries, fixed shuffle tree (tab: four long
+// rows per block)
+__global__ void __launch_bounds__(BLOCK) k_mg_clong(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                    int sc_done) {
+    const int2 te_ = tab[blockIdx.x];
+    const CaseDesc& cd = cds[te_.x];
+    if (cd.sc[sc_done] != 0.0) return;
+    const MGLv& L = lv[te_.x * MG_MAX_LEVELS + l];
+    const int u = te_.y + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (u >= L.nlong) return;
+    const int row = L.lrows[u];
+    const long long base = prow(L.sl_off, row);
+    double s = 0.0;
+    for (int k = lane; k < L.cnt[row]; k += SLICE)
+        s += (double)L.vals[base + pslot(k)] * (double)L.x[L.cols[base + pslot(k)]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) L.lsum[u] = s;
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_longflag(int n, const int* cnt, int* flag) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) flag[r] = cnt[r] > MG_CLONG ? 1 : 0;
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_longlist(int n, const int* flag, const int* scan, int* lrows,
+                                                       int* lidx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    lidx[r] = flag[r] ? scan[r] : -1;
+    if (flag[r]) lrows[scan[r]] = r;
+}
 // coarsest level: x = Ainv b, one wave per row, lanes over the columns,
 // fixed shuffle tree (tab: four rows per block)
 __global__ void __launch_bounds__(BLOCK) k_mg_dense(const CaseDesc* cds, const MGLv* lv, const int2* tab,
@@ -742,7 +791,8 @@ void mg_free(sph_ctx* c, MGState* mg) {
     if (!mg) return;
     for (int l = 0; l < MG_MAX_LEVELS; l++) {
         MGLevel& L = mg->L[l];
-        void* ptrs[] = {L.cx, L.cy, L.cz, L.ex, L.sl_off, L.cnt, L.cols, L.vals, L.diag, L.agg, L.mstart, L.mem, L.b, L.x, L.y, L.r};
+        void* ptrs[] = {L.cx, L.cy, L.cz, L.ex, L.sl_off, L.cnt, L.cols, L.vals, L.diag, L.agg, L.mstart, L.mem, L.b, L.x, L.y, L.r,
+                        L.lrows, L.lidx, L.lsum};
         for (void* q : ptrs) dfree(c, q);
     }
     dfree(c, mg->ainv);
@@ -1003,6 +1053,27 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         MGTRY(launch_check(c, "mg_pack16"));
     }
     for (int l = 0; l < nlev; l++) MGTRY(mg_vectors(c, mg->L[l], l == 0, l == nlev - 1));
+    for (int l = 1; l + 1 < nlev; l++) {
+        MGLevel& L = mg->L[l];
+        int *flag = nullptr, *scan = nullptr;
+        MGTRY(dalloc(c, &flag, L.n));
+        MGTRY(dalloc(c, &scan, L.n));
+        hipLaunchKernelGGL(k_mg_longflag, dim3(grid1(L.n)), dim3(BLOCK), 0, c->stream, L.n, (const int*)L.cnt, flag);
+        MGTRY(launch_check(c, "mg_longflag"));
+        int nl = 0;
+        MGTRY(exclusive_scan_i32(c, flag, scan, L.n, &nl));
+        if (nl > 0) {
+            L.nlong = nl;
+            MGTRY(dalloc(c, &L.lrows, nl));
+            MGTRY(dalloc(c, &L.lidx, L.n));
+            MGTRY(dalloc(c, &L.lsum, nl));
+            hipLaunchKernelGGL(k_mg_longlist, dim3(grid1(L.n)), dim3(BLOCK), 0, c->stream, L.n, (const int*)flag,
+                               (const int*)scan, L.lrows, L.lidx);
+            MGTRY(launch_check(c, "mg_longlist"));
+        }
+        dfree(c, flag);
+        dfree(c, scan);
+    }
     // coordinates are needed only while building
     for (int l = 0; l < nlev; l++) {
         MGLevel& L = mg->L[l];
@@ -1043,7 +1114,10 @@ int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool
             D.b = S.b; D.x = S.x; D.y = S.y; D.r = S.r;
             D.ainv = l == mg->nlev - 1 ? mg->ainv : nullptr;
             D.ev = l == 0 ? mg->ev : nullptr;
-            D.lsum = l == 0 ? mg->lsum : nullptr;
+            D.lsum = l == 0 ? mg->lsum : S.lsum;
+            D.nlong = l == 0 ? 0 : S.nlong;
+            D.lrows = S.lrows;
+            D.lidx = S.lidx;
         }
     }
     auto rows_table = [&](int l, bool below, int* off, int* cnt) {
@@ -1076,6 +1150,15 @@ int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool
         for (int f = 0; f < nlong; f += BLOCK / SLICE) tab.push_back(make_int2(k, f));
     }
     P.long_cnt = (int)tab.size() - P.long_off;
+    for (int l = 0; l < P.lmax; l++) {
+        P.clong_off[l] = (int)tab.size();
+        for (int k = 0; k < nc; k++) {
+            const MGState* mg = mgs[k];
+            if (!mg || l == 0 || l + 1 >= mg->nlev) continue;
+            for (int f = 0; f < mg->L[l].nlong; f += BLOCK / SLICE) tab.push_back(make_int2(k, f));
+        }
+        P.clong_cnt[l] = (int)tab.size() - P.clong_off[l];
+    }
     // launches in flight may read the old tables
     SPH_HIP(&e, hipStreamSynchronize(e.st));
     const size_t lv_bytes = sizeof(MGLv) * lv.size(), tab_bytes = sizeof(int2) * std::max<size_t>(1, tab.size());
@@ -1117,6 +1200,7 @@ int mg_apply(Eng& e, int which, int sc_done) {
     for (int l = 0; l + 1 < P.lmax; l++) {
         MGE("mg_pre", 0, P.down_off[l], P.down_cnt[l], k_mg_pre, l, which, sc_done);
         if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
+        else MGE("mg_clong", 0, P.clong_off[l], P.clong_cnt[l], k_mg_clong, l, sc_done);
         MGE(l == 0 ? "mg_res0" : "mg_res", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_res, l,
             which, sc_done);
         MGE("mg_restrict", 0, P.lvl_off[l + 1], P.lvl_cnt[l + 1], k_mg_restrict, l, sc_done);
@@ -1125,6 +1209,7 @@ int mg_apply(Eng& e, int which, int sc_done) {
     for (int l = P.lmax - 2; l >= 0; l--) {
         MGE("mg_prolong", 0, P.down_off[l], P.down_cnt[l], k_mg_prolong, l, sc_done);
         if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
+        else MGE("mg_clong", 0, P.clong_off[l], P.clong_cnt[l], k_mg_clong, l, sc_done);
         MGE(l == 0 ? "mg_post0" : "mg_post", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_post,
             l, which, sc_done);
     }

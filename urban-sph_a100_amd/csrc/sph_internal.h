@@ -267,6 +267,7 @@ struct Eng {
         int lvl_off[16], lvl_cnt[16];     // rows of level l, cases that have level l
         int dense_off = 0, dense_cnt = 0; // coarsest rows, four per block
         int long_off = 0, long_cnt = 0;   // level-0 long rows, four per block
+        int clong_off[16], clong_cnt[16]; // coarse level l's long rows, four per block
         double spmv_bytes0 = 0.0;         // level-0 bytes of one pass (profile)
     } mg_plan;
 };
