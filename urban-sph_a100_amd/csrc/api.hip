@@ -277,6 +277,8 @@ int sph_upload(sph_ctx* c, const sph_part1* P1, const sph_part2* P2, const float
             return fail(c, SPH_E_ARG, "non-finite position");
     }
     c->n = n;
+    dfree(c, c->porder);   // the pass order of the last search; the next search rebuilds it
+    c->porder = nullptr;
     // find_minmax (skips the last particle), function_init.cuh:678-708; cell
     // sizing ISPH.cuh:89-97, functions_NNPS.cuh:534-546.  float arithmetic.
     float mnx = P1[0].x, mxx = P1[0].x, mny = P1[0].y, mxy = P1[0].y, mnz = P1[0].z, mxz = P1[0].z;
@@ -594,7 +596,7 @@ void sph_destroy(sph_ctx* c) {
     for (auto p : c->s2) dfree(c, p);
     for (auto p : c->q) dfree(c, p);
     for (auto p : c->snap) dfree(c, p);
-    void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->cstart, c->nm_list, c->nl_slice_off, c->nl_wprev, c->nl_cnt,
+    void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->cstart, c->nm_list, c->nl_slice_off, c->nl_wprev, c->porder, c->nl_cnt,
                     c->nl_ent, c->stat_buf, c->apr_target, c->xnew[0], c->xnew[1], c->xnew[2], c->rec, c->nmp_off, c->nmp_ent};
     for (auto p : rest) dfree(c, p);
     dcache_release(c);

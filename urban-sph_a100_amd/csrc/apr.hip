@@ -135,6 +135,8 @@ int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out) {
                            (const int*)rank, (const int*)pick, c->n, F, c->prm.apr_h_change);
         if ((rc = launch_check(c, "apr_split"))) break;
         c->n += 8 * nsplit;
+        dfree(c, c->porder);   // the pass order covers the old count; the next list rebuilds it
+        c->porder = nullptr;
         ppe_free(c);
         if ((rc = set_ncell_run(c))) break;
         // the next pass needs only the cells; the neighbour list is built

@@ -366,6 +366,7 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.Q = c->qview();
     d.g = c->grid();
     d.nl = c->nlist();
+    d.porder = c->porder;
     d.keys = c->keys;
     for (int f = 0; f < 5; f++) d.snap[f] = c->snap[f];
     d.ppe = ppe_view(c);

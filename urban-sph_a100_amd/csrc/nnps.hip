@@ -431,6 +431,45 @@ static int permute_state(sph_ctx* c) {
     return 0;
 }
 
+// Order of the record-gathering particle passes (thread -> particle):
+// particles with lists of at most NL_HEAVY entries first, in their order,
+// then the longer ones (coarse particles beside refined zones), so that
+// those do not hold waves of short lists for their length.  Null when no
+// list is that long.  A pass computes each particle on its own, so the
+// order changes no result.
+constexpr int NL_HEAVY = 512;
+__global__ void __launch_bounds__(BLOCK) k_heavy_flag(int n, const int* cnt, int* flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = cnt[i] > NL_HEAVY ? 1 : 0;
+}
+__global__ void __launch_bounds__(BLOCK) k_porder(int n, int nlight, const int* flag, const int* scan, int* order) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    order[flag[i] ? nlight + scan[i] : i - scan[i]] = i;
+}
+static int build_porder(sph_ctx* c) {
+    const int n = c->n;
+    dfree(c, c->porder);
+    c->porder = nullptr;
+    if (n == 0) return 0;
+    int *flag = nullptr, *scan = nullptr;
+    SPH_TRY(dalloc(c, &flag, n));
+    SPH_TRY(dalloc(c, &scan, n));
+    hipLaunchKernelGGL(k_heavy_flag, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)c->nl_cnt, flag);
+    SPH_TRY(launch_check(c, "porder"));
+    int nheavy = 0;
+    SPH_TRY(exclusive_scan_i32(c, flag, scan, n, &nheavy));
+    if (nheavy > 0) {
+        SPH_TRY(dalloc(c, &c->porder, n));
+        hipLaunchKernelGGL(k_porder, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, n - nheavy, (const int*)flag,
+                           (const int*)scan, c->porder);
+        SPH_TRY(launch_check(c, "porder"));
+    }
+    dfree(c, flag);
+    dfree(c, scan);
+    return 0;
+}
+
 // The list of a search: one wave per particle.  When the previous search
 // had as many slices (the particles keep their cell order, and move little
 // between searches), the fill runs at once into each slice's previous
@@ -478,7 +517,7 @@ static int build_neighbor_list(sph_ctx* c) {
             SPH_TRY(launch_check(c, "slice_width"));
             c->nl_total = total;
             dfree(c, w);
-            return 0;
+            return build_porder(c);
         }
         dfree(c, c->nl_ent);
         c->nl_ent = nullptr;
@@ -508,7 +547,7 @@ static int build_neighbor_list(sph_ctx* c) {
                        c->nl_ent, (int*)nullptr, (int*)nullptr);
     SPH_TRY(launch_check(c, "nl_fill"));
     SPH_TRY(prof_end(c));
-    return 0;
+    return build_porder(c);
 }
 
 // The particles the Neumann pass updates (every p_type but fluid), in the

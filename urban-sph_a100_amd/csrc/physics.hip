@@ -27,6 +27,15 @@ namespace sph {
     const CaseDesc& cd = (sg).cd[cid_];           \
     const int i = first_ + (int)threadIdx.x;      \
     if (i >= cd.n) return;
+// The same through the case's pass order when it has one (long neighbour
+// lists last): for the record-gathering passes, where it measured faster.
+#define SEG_PARTICLE_ORDERED(sg)                          \
+    int first_;                                           \
+    const int cid_ = seg_case((sg), first_);              \
+    const CaseDesc& cd = (sg).cd[cid_];                   \
+    const int t_ = first_ + (int)threadIdx.x;             \
+    if (t_ >= cd.n) return;                               \
+    const int i = cd.porder ? cd.porder[t_] : t_;
 
 // Particles of the cells (i0..i1, cj, ck) that lie inside the grid, as one
 // range [rg.x, rg.y) of the sorted order; false when the row is outside the
@@ -629,7 +638,7 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
 // KERNEL_advection_force3D, functions_PPE.cuh:191-480
 template <class R>
 __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
-    SEG_PARTICLE(sg);
+    SEG_PARTICLE_ORDERED(sg);
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const PView S = cd.S;
@@ -932,7 +941,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_outlet(Seg sg, int which) {
 // KERNEL_pressureforce3D, functions_PPE.cuh:482-647
 template <class R>
 __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
-    SEG_PARTICLE(sg);
+    SEG_PARTICLE_ORDERED(sg);
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const PView S = cd.S;

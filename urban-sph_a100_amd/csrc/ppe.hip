@@ -425,7 +425,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     SEG_ROW(sg);
     const PPEView pp = cd.ppe;
     if (row >= cd.n) return;
-    const int r = pp.rowof[row];
+    const int i = cd.porder ? cd.porder[row] : row;   // this thread's particle
+    const int r = pp.rowof[i];
     if (r < 0) return;
     const KConst kc = cd.kc;
     const PView S = cd.S;
@@ -433,7 +434,6 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     const NList nl = cd.nl;
     const double dt = cd.dt;
     const int* rowof = pp.rowof;
-    const int i = row;
     R hi = (R)(1.0 * S.h[i]);
     R tmp_A = calc_tmpA<R>(kc, hi);
     double xi = S.x[i], yi = S.y[i], zi = S.z[i];

@@ -115,6 +115,7 @@ struct CaseDesc {
     int n, count;
     int gc_fresh;        // the correction matrices inv_cm hold this search's values (fixed positions)
     int vals_fresh;      // the PPE values and diagonal hold this search's values (fixed positions)
+    const int* porder;   // particle passes: thread -> particle (long neighbour lists last), or null
     const int* nm_list;  // particles the Neumann pass updates (p_type != 1), in sorted order
     int nm_count;
     const long long* nmp_off;   // per nm_list particle: its first Neumann pair (nm_count + 1)
@@ -320,6 +321,7 @@ struct sph_ctx {
     long long* nl_slice_off = nullptr; int* nl_cnt = nullptr; unsigned* nl_ent = nullptr;
     long long nl_total = 0;
     long long* nl_wprev = nullptr;   // slice widths of the last search (room for the next)
+    int* porder = nullptr;           // order of the particle passes (CaseDesc::porder), or null
     int nl_wprev_n = 0;              // its slices
     int* scan_tmp = nullptr; size_t scan_tmp_n = 0;
 
