@@ -65,9 +65,10 @@ def test_apr_canyon_fp64(passes):
         assert e <= 1e-10, f"{f}: {e:.3e}"
 
 
-def _refined(prm, steps=3):
+def _refined(prm, steps=3, graphs=True):
     P, nrm, geom = cases.small_canyon(dx=0.02)
     g = urbansph.Case(prm, 40 * len(P))
+    g.options(graphs=graphs, autotune=True)
     g.upload(P, None, nrm)
     g.step(0, 5e-5)
     n = g.apr_refine(_targets(g, geom["buildings"], 0.05, 0.03), max_passes=40)
@@ -170,3 +171,20 @@ def test_refined_mixed_case_keeps_cell_order():
     assert out[0][0] == abi.ORDER_CELL and out[0][1] == 1 and out[1][1] == 0
     assert out[0][2] == out[1][2]
     np.testing.assert_array_equal(out[0][3], out[1][3])
+
+
+def test_apr_multigrid_reproducible_graphs_and_direct():
+    """The refined case on the multigrid (hashed and sorted Galerkin rows,
+    wave-per-row products of long coarse rows, long neighbour lists last in
+    the record-gathering passes): two fresh runs, one with HIP graphs and one
+    launching each kernel directly, agree bit for bit."""
+    _, _, geom = cases.small_canyon(dx=0.02)
+    prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-6, precision=abi.MIXED)
+    prm.update(ppe_precond=abi.PRECOND_MULTIGRID)
+    print("[apr mg repro] graphs / direct", flush=True)
+    a, _, ra, _, _ = _refined(prm, graphs=True)
+    b, _, rb, _, _ = _refined(prm, graphs=False)
+    assert ra.precond == abi.PRECOND_MULTIGRID and rb.precond == abi.PRECOND_MULTIGRID
+    assert ra.tail_rows > 0 and ra.iterations == rb.iterations
+    for f in a:
+        np.testing.assert_array_equal(a[f], b[f], err_msg=f)
