@@ -56,6 +56,12 @@ def _load():
     lib.oracle_phi_m.restype = C.c_double
     lib.oracle_phi_m.argtypes = [C.c_double, C.c_double, C.c_double]
     lib.oracle_set_threads.argtypes = [C.c_int]
+    lib.oracle_search.argtypes = [vp]
+    lib.oracle_run_pass.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_double]
+    lib.oracle_get_work_f64.argtypes = [vp, C.c_int, vp, C.c_int]
+    lib.oracle_set_work_f64.argtypes = [vp, C.c_int, vp, C.c_int]
+    lib.oracle_set_work_i32.argtypes = [vp, C.c_int, vp, C.c_int]
+    lib.oracle_set_mutant.argtypes = [C.c_int]
     return lib
 
 
@@ -70,6 +76,16 @@ def default_params():
 
 def set_threads(n):
     return lib.oracle_set_threads(int(n))
+
+
+# Passes of oracle_run_pass (the oracle's OP_* order).
+PASSES = ("grad_corr", "prep", "boundary", "most", "neumann", "advection", "projection", "periodic",
+          "freeslip", "ppe", "outlet", "pressure_force", "ibm_force", "ibm_spread")
+
+
+def set_mutant(k):
+    """Plants mistake k in the oracle (0 = none); returns the previous one."""
+    return lib.oracle_set_mutant(int(k))
 
 
 def canon_sum(v):
@@ -99,6 +115,37 @@ class Oracle:
         if time is None:
             time = count * dt
         return lib.oracle_step(self.h, int(count), float(dt), float(time))
+
+    def search(self):
+        """The count == 0 neighbour search alone (then single passes)."""
+        rc = lib.oracle_search(self.h)
+        if rc:
+            raise RuntimeError(f"oracle_search rc={rc}")
+
+    def run_pass(self, name, count=0, dt=1e-4, time=None):
+        if time is None:
+            time = count * dt
+        return lib.oracle_run_pass(self.h, PASSES.index(name), int(count), float(dt), float(time))
+
+    def work(self, name):
+        """A field of the working state (SP1 / SP2 / P3), sorted order."""
+        out = np.empty(self.n, np.float64)
+        rc = lib.oracle_get_work_f64(self.h, abi.F[name], abi.ptr(out), self.n)
+        if rc:
+            raise RuntimeError(rc)
+        return out
+
+    def set_work(self, name, values):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), (self.n,)))
+        rc = lib.oracle_set_work_f64(self.h, abi.F[name], abi.ptr(v), self.n)
+        if rc:
+            raise RuntimeError(rc)
+
+    def set_work_i(self, name, values):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.int32), (self.n,)))
+        rc = lib.oracle_set_work_i32(self.h, abi.I[name], abi.ptr(v), self.n)
+        if rc:
+            raise RuntimeError(rc)
 
     def apr_refine(self, target=None, max_passes=1):
         """Refinement passes; target: per-particle target level in the

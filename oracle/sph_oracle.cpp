@@ -30,16 +30,23 @@
  * std::stable_sort against a numpy stable argsort; gradient correction
  * = identity on a lattice and exact for linear fields; PPE operator row
  * sums, b = 0 for uniform u*, Laplacian of x^2; BiCGSTAB against scipy's
- * sparse LU; Shepard reproduction of constants; hand-built predictor and
- * corrector cases; committed golden fixtures; APR split geometry and mass
- * conservation; ALE trapezoid positions.
+ * sparse LU; hand-built predictor and corrector cases; committed golden
+ * fixtures; APR split geometry and mass conservation; ALE trapezoid
+ * positions.  Each remaining pass is run on its own (oracle_run_pass) and
+ * pinned in tests/test_oracle_pins.py: pressure force and the Cleary and
+ * Euler terms by lattice moments of the kernel for linear and quadratic
+ * fields; advection (Cleary, Monaghan, artificial viscosity, Euler),
+ * pressure force, IBM interpolation and spreading, the periodic and outlet
+ * copies and the Neumann pass with its inlet ghost by all-pairs numpy sums
+ * under the window rule; the MOST Obukhov length by brentq on the
+ * Businger-Dyer relation; the LES Shepard sum, kernel length, Smagorinsky
+ * viscosity and the Deardorff k-equation by lattice closed forms.  The
+ * planted mistakes below (g_mutant) show that each of those pins fails on a
+ * one-line slip.
  *
- * Parity unpinned (no closed form, and the reference ships no vectors):
- * the MOST Newton branch (functions_MOST.cuh:214-249), the DDF k-equation
- * (functions_PREP.cuh:462-478), the IBM target-velocity table values
- * (functions_IBM.cuh:289-306) and the ghost "last match" selection of the
- * Neumann, periodic and outlet copies (functions_BC.cuh:345-369,732-744,
- * 1122-1129).  These are checked only as transcriptions.
+ * Only the IBM target-velocity table values (functions_IBM.cuh:289-306) are
+ * data with nothing to check them against beyond their transcription; the
+ * interpolation over them is pinned.
  */
 #include "../include/urbansph.h"
 
@@ -62,6 +69,14 @@ const double MOST_TOL = 1e-5;               // Parameters.cuh:112
 const int MOST_MAXIT = 200;                 // Parameters.cuh:111
 const double BOUNDARY_VELOCITY = 1.0;       // Parameters.cuh:98
 const int MOVING = 9;                       // Parameters.cuh:17
+
+// Planted mistakes for the mutation tests (tests/test_oracle_pins.py): 0 =
+// none.  Each other value changes one coefficient, operand or selection rule
+// of one pass, the kind of slip a transcription makes, so that the tests can
+// show that the pin of that pass detects it.  Set only by
+// oracle_set_mutant(); every product run has 0.
+int g_mutant = 0;
+inline bool MUT(int k) { return g_mutant == k; }
 
 struct P1d {
     int i_type, buffer_type, MOST_buffer, p_type;
@@ -301,7 +316,7 @@ double psi_m(double zeta) {
     return -5.0 * zeta;
 }
 double psi_h(double zeta) {
-    if (zeta < 0.0) return 2.0 * log(0.5 * (1.0 + sqrt(1.0 - 16.0 * zeta)));
+    if (zeta < 0.0) return 2.0 * log(0.5 * (1.0 + sqrt(1.0 - (MUT(25) ? 9.0 : 16.0) * zeta)));
     return -5.0 * zeta;
 }
 double Phi_m(double L, double z, double z0) { return log(z / z0) - psi_m(z / L) + psi_m(z0 / L); }
@@ -490,7 +505,7 @@ void prep(Oracle& o, int count, double dt) {
                 double tdwx = tmp_dwij * (xi - xj) / tdist;
                 double tdwy = tmp_dwij * (yi - yj) / tdist;
                 double tdwz = tmp_dwij * (zi - zj) / tdist;
-                if (filt) tmp_flt += mj / rhoj * tmp_wij;
+                if (filt) tmp_flt += MUT(30) ? tmp_wij : mj / rhoj * tmp_wij;
                 if (les) {
                     tmp_sigma += tdist * tdist * tmp_wij * mj / rhoj;
                     if (ptypej == 1) {
@@ -511,9 +526,9 @@ void prep(Oracle& o, int count, double dt) {
             tmp_SR = 2 * (st11 * st11 + st22 * st22 + st33 * st33 + 2 * st12 * st12 + 2 * st31 * st31 + 2 * st23 * st23);
             tmp_SR = sqrt(tmp_SR);
             S2[i].SR = tmp_SR;
-            tmp_sigma = 0.3333 * tmp_sigma;
+            tmp_sigma = (MUT(28) ? 0.5 : 0.3333) * tmp_sigma;
             tmp_sigma = sqrt(tmp_sigma);
-            double tmp_cl = tmp_sigma > hi ? tmp_sigma : hi;
+            double tmp_cl = MUT(29) ? fmin(tmp_sigma, hi) : (tmp_sigma > hi ? tmp_sigma : hi);
             if (pr.turbulence_model == SPH_SSM) {
                 tvis_t = pr.Cs_sgs * pr.Cs_sgs * tmp_cl * tmp_cl * tmp_SR;
                 if (S[i].buffer_type != 1) S[i].vis_t = tvis_t;
@@ -523,8 +538,10 @@ void prep(Oracle& o, int count, double dt) {
                 double eps = pr.Ce_sgs * ki * sqrt(ki) / tmp_cl;
                 double dk = p + b - eps + 0.0;
                 ki += dk * dt;
-                ki = ki > 1e-4 ? ki : 1e-4;
-                tvis_t = pr.Ck_sgs * tmp_cl * sqrt(ki);
+                const double kfloor = MUT(33) ? 0.0 : 1e-4;
+                ki = ki > kfloor ? ki : kfloor;
+                if (MUT(31)) eps = pr.Ce_sgs * ki * sqrt(ki) / tmp_cl;
+                tvis_t = MUT(32) ? pr.Ck_sgs * tmp_cl * ki : pr.Ck_sgs * tmp_cl * sqrt(ki);
                 if (S[i].buffer_type != 1) {
                     S[i].vis_t = tvis_t; S[i].k_turb = ki; S[i].e_turb = eps;
                 }
@@ -598,14 +615,15 @@ void most(Oracle& o, int count, double dt) {
             if (fabs(Ri_b) > 1e-3) {
                 OB_length = zi / Ri_b;
                 OB_length = fmax(-500.0, fmin(OB_length, 500.0));
+                const double zh = MUT(26) ? z0 : z0h;
                 for (int iter = 0; iter < MOST_MAXIT; iter++) {
                     double ol_m = OB_length, ol_l = 0.999 * ol_m, ol_u = 1.001 * ol_m;
                     double pm = Phi_m(ol_m, zi, z0);
-                    double fL = Ri_b - (zi / ol_m) * Phi_h(ol_m, zi, z0h) / (pm * pm);
-                    double dfL = ((-(zi / ol_u) * Phi_h(ol_u, zi, z0h) / (Phi_m(ol_u, zi, z0) * Phi_m(ol_u, zi, z0))) -
-                                  (-(zi / ol_l) * Phi_h(ol_l, zi, z0h) / (Phi_m(ol_l, zi, z0) * Phi_m(ol_l, zi, z0)))) /
+                    double fL = Ri_b - (zi / ol_m) * Phi_h(ol_m, zi, zh) / (pm * pm);
+                    double dfL = ((-(zi / ol_u) * Phi_h(ol_u, zi, zh) / (Phi_m(ol_u, zi, z0) * Phi_m(ol_u, zi, z0))) -
+                                  (-(zi / ol_l) * Phi_h(ol_l, zi, zh) / (Phi_m(ol_l, zi, z0) * Phi_m(ol_l, zi, z0)))) /
                                  (ol_u - ol_l);
-                    OB_length -= 1.0 * fL / dfL;
+                    OB_length -= (MUT(27) ? -1.0 : 1.0) * fL / dfL;
                     if (fabs(OB_length) > 1e-6) {
                         if (fabs((OB_length - ol_m) / OB_length) < MOST_TOL) break;
                         else if (fabs(OB_length - ol_m) < MOST_TOL) break;
@@ -649,19 +667,20 @@ void neumann(Oracle& o, std::vector<P1d>& A) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < o.n; i++) {
         if (A[i].p_type == 1) continue;
-        double search_coeff = 2.0;
+        double search_coeff = MUT(24) ? 1.0 : 2.0;
         double hi = search_coeff * A[i].h;
         double tmp_A = calc_tmpA(pr.kernel_type, pr.kappa, o.c_pi15, hi);
         double search_range = pr.kappa * hi;
         double xi = A[i].x, yi = A[i].y, zi = A[i].z;
         double grad_px = 0, grad_py = 0, grad_pz = 0, tpres = 0, tflt_s = 0, pres_g = 0;
         double tux = 0, tuy = 0, tuz = 0;
-        double xg = (A[i].buffer_type == 1 || A[i].buffer_type == -1) ? 2.0 * pr.L2 - xi : xi;
+        double xg = (A[i].buffer_type == 1 || A[i].buffer_type == -1) ? (MUT(21) ? xi - 2.0 * pr.L2 : 2.0 * pr.L2 - xi) : xi;
         double yg = yi, zg = zi;
         Cell c = query_cell(o.d, xg, yg, zg);
+        bool got_g = false;
         window(o, c, 1, [&](int j) {
             double dist = dist3(xg, yg, zg, A[j].x, A[j].y, A[j].z) - 1e-20;
-            if (dist < pr.neumann_ghost_r) pres_g = pold[j];
+            if (dist < pr.neumann_ghost_r && !(MUT(22) && got_g)) { pres_g = pold[j]; got_g = true; }
         });
         window(o, c, A[i].ncell, [&](int j) {
             double xj = A[j].x, yj = A[j].y, zj = A[j].z;
@@ -694,7 +713,7 @@ void neumann(Oracle& o, std::vector<P1d>& A) {
                 A[i].ux = tux / tflt_s; A[i].uy = tuy / tflt_s; A[i].uz = tuz / tflt_s;
             }
             if (A[i].buffer_type == 1 || A[i].buffer_type == -1)
-                A[i].pres = pres_g + (xi - xg) * grad_px + (yi - yg) * grad_py + (zi - zg) * grad_pz;
+                A[i].pres = pres_g + (MUT(23) ? xg - xi : xi - xg) * grad_px + (yi - yg) * grad_py + (zi - zg) * grad_pz;
         }
     }
 }
@@ -741,10 +760,10 @@ void advection(Oracle& o) {
                     double uxj = S[j].ux, uyj = S[j].uy, uzj = S[j].uz;
                     double mj = S[j].m, tempj = S[j].temp, rhoj = S[j].rho;
                     if (pr.fv_solve == SPH_FV_CLEARY) {
-                        double visj = viscosity2(pr, ptypej) + S[j].vis_t * rhoj;
+                        double visj = viscosity2(pr, ptypej) + (MUT(6) ? S[i].vis_t : S[j].vis_t) * rhoj;
                         double C_v = (xi - xj) * tdwx + (yi - yj) * tdwy + (zi - zj) * tdwz;
-                        C_v *= (visi * visj) / (visi + visj);
-                        C_v *= 4 * (mj / (rhoi * rhoj));
+                        C_v *= MUT(5) ? 0.5 * (visi + visj) : (visi * visj) / (visi + visj);
+                        C_v *= (MUT(4) ? 2 : 4) * (mj / (rhoi * rhoj));
                         C_v /= tdist;
                         C_v /= tdist;
                         tmpx += C_v * (uxi - uxj); tmpy += C_v * (uyi - uyj); tmpz += C_v * (uzi - uzj);
@@ -762,9 +781,10 @@ void advection(Oracle& o) {
                         tmpx += -(P_ij)*tdwx; tmpy += -(P_ij)*tdwy; tmpz += -(P_ij)*tdwz;
                     }
                     if (eli < 1.0) {
-                        eulerx += (uxi * (uxj - uxi) * tdwx + uyi * (uxj - uxi) * tdwy + uzi * (uxj - uxi) * tdwz) * mj / rhoj * (1 - eli);
-                        eulery += (uxi * (uyj - uyi) * tdwx + uyi * (uyj - uyi) * tdwy + uzi * (uyj - uyi) * tdwz) * mj / rhoj * (1 - eli);
-                        eulerz += (uxi * (uzj - uzi) * tdwx + uyi * (uzj - uzi) * tdwy + uzi * (uzj - uzi) * tdwz) * mj / rhoj * (1 - eli);
+                        const double cx = MUT(10) ? uxj : uxi, cy = MUT(10) ? uyj : uyi, cz = MUT(10) ? uzj : uzi;
+                        eulerx += (cx * (uxj - uxi) * tdwx + cy * (uxj - uxi) * tdwy + cz * (uxj - uxi) * tdwz) * mj / rhoj * (1 - eli);
+                        eulery += (cx * (uyj - uyi) * tdwx + cy * (uyj - uyi) * tdwy + cz * (uyj - uyi) * tdwz) * mj / rhoj * (1 - eli);
+                        eulerz += (cx * (uzj - uzi) * tdwx + cy * (uzj - uzi) * tdwy + cz * (uzj - uzi) * tdwz) * mj / rhoj * (1 - eli);
                         if (pr.con_solve)
                             eulert -= 2.0 * mj / (rhoi + rhoj) * (tempj - tempi) * ((uxi - uxj) * tdwx + (uyi - uyj) * tdwy + (uzi - uzj) * tdwz) * (1 - eli);
                         if (pr.concn_solve) {
@@ -796,12 +816,12 @@ void advection(Oracle& o) {
                     if (pr.artificial_viscosity) {
                         double uij_xij = (uxi - uxj) * (xi - xj) + (uyi - uyj) * (yi - yj) + (uzi - uzj) * (zi - zj);
                         if (uij_xij < 0) {
-                            double h_ij = (hi + hj) * 0.5;
+                            double h_ij = MUT(7) ? hi : (hi + hj) * 0.5;
                             double phi_ij = h_ij * uij_xij;
-                            phi_ij /= (tdist * tdist + 0.01 * h_ij * h_ij);
+                            phi_ij /= (tdist * tdist + (MUT(8) ? 0.1 : 0.01) * h_ij * h_ij);
                             double P_ij = mi * phi_ij * (-Alpha * soundspeed + Beta * phi_ij);
                             P_ij /= (rhoi + rhoj);
-                            P_ij *= 0.5;
+                            if (!MUT(9)) P_ij *= 0.5;
                             tmpx += -(P_ij)*tdwx; tmpy += -(P_ij)*tdwy; tmpz += -(P_ij)*tdwz;
                         }
                     }
@@ -812,9 +832,9 @@ void advection(Oracle& o) {
         if ((pr.boussinesq_solve) && (ptypei > 0) && (ptypei < 1000))
             tmpz = betai * pr.gravity * (tmp_temp / tmp_flt - pr.tempb);
         P3d& q = o.P3[i];
-        q.ftotalx = tmpx - eulerx;
-        q.ftotaly = tmpy - eulery;
-        q.ftotalz = tmpz - eulerz;
+        q.ftotalx = MUT(11) ? tmpx + eulerx : tmpx - eulerx;
+        q.ftotaly = MUT(11) ? tmpy + eulery : tmpy - eulery;
+        q.ftotalz = MUT(11) ? tmpz + eulerz : tmpz - eulerz;
         if (pr.con_solve) q.dtemp += tmp_Rc / cpi - eulert;
         q.ftotal = sqrt(tmpx * tmpx + tmpy * tmpy + tmpz * tmpz);
         if (pr.concn_solve) q.dconcn = tmp_Rd;
@@ -877,7 +897,7 @@ void periodic(Oracle& o, std::vector<P1d>& A) {
         if (old[i].p_type >= 1000) continue;
         double xi = old[i].x, yi = old[i].y, zi = old[i].z;
         double xg, yg, zg;
-        if (old[i].buffer_type == 3) { xg = xi; yg = yi + pr.L_right - pr.L_left; zg = zi; }
+        if (old[i].buffer_type == 3) { xg = xi; yg = MUT(17) ? yi - pr.L_right + pr.L_left : yi + pr.L_right - pr.L_left; zg = zi; }
         else if (old[i].buffer_type == 4) { xg = xi; yg = yi - pr.L_right + pr.L_left; zg = zi; }
         else continue;
         int nc = old[i].ncell;
@@ -895,7 +915,7 @@ void periodic(Oracle& o, std::vector<P1d>& A) {
                     if (o.gstr[k] != -1) {
                         for (int j = o.gstr[k]; j < o.gend[k]; j++) {
                             double tdist = dist3(xg, yg, zg, old[j].x, old[j].y, old[j].z) - 1e-20;
-                            if (tdist < pr.copy_tol) {
+                            if (tdist < pr.copy_tol && !(MUT(18) && flag)) {
                                 tux = old[j].ux; tuy = old[j].uy; tuz = old[j].uz;
                                 tpres = old[j].pres; ttemp = old[j].temp;
                                 flag = 1;
@@ -951,11 +971,13 @@ void outlet(Oracle& o, std::vector<P1d>& A) {
         Cell c = query_cell(o.d, xi, yi, zi);
         int nc = 2 * old[i].ncell;
         double tux = 0, tuy = 0, tuz = 0, tpres = 0, ttemp = 0;
-        double xg = pr.L3 - pr.outlet_copy_dx;
+        double xg = MUT(19) ? pr.L3 + pr.outlet_copy_dx : pr.L3 - pr.outlet_copy_dx;
+        bool got = false;
         window(o, c, nc, [&](int j) {
             double tdist = dist3(xg, yi, zi, old[j].x, old[j].y, old[j].z) - 1e-20;
             if (old[j].p_type < 1000) {
-                if (tdist < pr.copy_tol) {
+                if (tdist < pr.copy_tol && !(MUT(20) && got)) {
+                    got = true;
                     tux = old[j].ux; tuy = old[j].uy; tuz = old[j].uz;
                     tpres = old[j].pres; ttemp = old[j].temp;
                 }
@@ -1175,7 +1197,7 @@ void pressure_force(Oracle& o) {
             double search_range = pr.kappa * fmax(hi, hj);
             if (S[j].p_type < 1000) {
                 if (tdist < search_range) {
-                    double hij = 0.5 * (hi + hj);
+                    double hij = MUT(2) ? fmax(hi, hj) : 0.5 * (hi + hj);
                     double tmp_Aij = calc_tmpA(pr.kernel_type, pr.kappa, o.c_pi15, hij);
                     double tdwij = kernel_dw(pr.kernel_type, pr.kappa, tmp_Aij, hij, tdist);
                     double tdwx = tdwij * (xi - xj) / tdist;
@@ -1184,7 +1206,8 @@ void pressure_force(Oracle& o) {
                     double mj = S[j].m, rhoj = S[j].rho, pj = S[j].pres - S[j].pres0;
                     if (pr.fp_solve) {
                         const double Pb = 0.0;
-                        double C_p = -2.0 * mj / (rhoi * rhoj) * ((pi + Pb) * rhoj + (pj + Pb) * rhoi) / (rhoi + rhoj);
+                        double C_p = (MUT(1) ? -1.0 : -2.0) * mj / (rhoi * rhoj) *
+                                     (MUT(3) ? (pi + Pb) * rhoi + (pj + Pb) * rhoj : (pi + Pb) * rhoj + (pj + Pb) * rhoi) / (rhoi + rhoj);
                         tmpx += C_p * tdwx; tmpy += C_p * tdwy; tmpz += C_p * tdwz;
                     }
                 }
@@ -1202,7 +1225,7 @@ void ibm_force(Oracle& o, double dt) {
     for (int i = 0; i < o.n; i++) {
         if (S[i].p_type < 1000) continue;
         double xi = S[i].x, yi = S[i].y, zi = S[i].z;
-        double tmp_h = 0.5 * S[i].h;
+        double tmp_h = (MUT(12) ? 1.0 : 0.5) * S[i].h;
         double tmp_A = calc_tmpA(pr.kernel_type, pr.kappa, o.c_pi15, tmp_h);
         double search_range = pr.kappa * tmp_h;
         Cell c = bin_cell(o.d, xi, yi, zi);
@@ -1229,12 +1252,13 @@ void ibm_force(Oracle& o, double dt) {
         nx = nx / nmag; ny = ny / nmag; nz = nz / nmag;
         double un = ufx * nx + ufy * ny + ufz * nz;
         double utx = ufx - un * nx, uty = ufy - un * ny, utz = ufz - un * nz;
-        double Ubtx = pr.ibm_ut_factor * interp_ux_caseH(zi, 1.0);
-        double Ubty = 0.0, Ubtz = 0.0;
+        double Ubtx = MUT(13) ? 0.0 : pr.ibm_ut_factor * interp_ux_caseH(zi, 1.0);
+        double Ubty = MUT(13) ? pr.ibm_ut_factor * interp_ux_caseH(zi, 1.0) : 0.0, Ubtz = 0.0;
         double beta_t = 1.0;
-        double corrx = (-un) * nx + beta_t * (Ubtx - utx);
-        double corry = (-un) * ny + beta_t * (Ubty - uty);
-        double corrz = (-un) * nz + beta_t * (Ubtz - utz);
+        const double mun = MUT(14) ? un : -un;
+        double corrx = (mun) * nx + beta_t * (Ubtx - utx);
+        double corry = (mun) * ny + beta_t * (Ubty - uty);
+        double corrz = (mun) * nz + beta_t * (Ubtz - utz);
         S[i].fbx = pr.rho_air * corrx / dt;
         S[i].fby = pr.rho_air * corry / dt;
         S[i].fbz = pr.rho_air * corrz / dt;
@@ -1250,7 +1274,7 @@ void ibm_spread(Oracle& o) {
         if (S[i].p_type != 1) continue;
         if (S[i].buffer_type == 2) continue;
         double xi = S[i].x, yi = S[i].y, zi = S[i].z;
-        double tmp_h = 0.5 * S[i].h;
+        double tmp_h = (MUT(16) ? 1.0 : 0.5) * S[i].h;
         double tmp_A = calc_tmpA(pr.kernel_type, pr.kappa, o.c_pi15, tmp_h);
         double search_range = pr.kappa * tmp_h;
         Cell c = bin_cell(o.d, xi, yi, zi);
@@ -1274,9 +1298,9 @@ void ibm_spread(Oracle& o) {
             S[i].fbx = 0.0; S[i].fby = 0.0; S[i].fbz = 0.0; S[i].concentration = 0.0;
             continue;
         }
-        S[i].fbx = fb_x / (filt + 1e-10);
-        S[i].fby = fb_y / (filt + 1e-10);
-        S[i].fbz = fb_z / (filt + 1e-10);
+        S[i].fbx = MUT(15) ? fb_x : fb_x / (filt + 1e-10);
+        S[i].fby = MUT(15) ? fb_y : fb_y / (filt + 1e-10);
+        S[i].fbz = MUT(15) ? fb_z : fb_z / (filt + 1e-10);
         S[i].concentration = filt;
     }
 }
@@ -1488,6 +1512,47 @@ double field_value(const Oracle& o, int f, int i) {
     return NAN;
 }
 
+// Working-state view for the per-pass entry points: SP1 / SP2 / P3.
+double* work_ref(Oracle& o, int f, int i) {
+    P1d& p = o.SP1[i];
+    P2d& p2 = o.SP2[i];
+    P3d& q = o.P3[i];
+    switch (f) {
+    case SPH_F_X: return &p.x; case SPH_F_Y: return &p.y; case SPH_F_Z: return &p.z;
+    case SPH_F_UX: return &p.ux; case SPH_F_UY: return &p.uy; case SPH_F_UZ: return &p.uz;
+    case SPH_F_M: return &p.m; case SPH_F_H: return &p.h; case SPH_F_TEMP: return &p.temp;
+    case SPH_F_PRES: return &p.pres; case SPH_F_PRES0: return &p.pres0; case SPH_F_RHO: return &p.rho;
+    case SPH_F_FLT_S: return &p.flt_s; case SPH_F_CONCN: return &p.concn;
+    case SPH_F_CONCENTRATION: return &p.concentration; case SPH_F_K_TURB: return &p.k_turb;
+    case SPH_F_E_TURB: return &p.e_turb; case SPH_F_VIS_T: return &p.vis_t;
+    case SPH_F_FBX: return &p.fbx; case SPH_F_FBY: return &p.fby; case SPH_F_FBZ: return &p.fbz;
+    case SPH_F_FMX: return &p.fmx; case SPH_F_FMY: return &p.fmy; case SPH_F_HMZ: return &p.hmz;
+    case SPH_F_FLOAT1: return &p.float1; case SPH_F_FLOAT2: return &p.float2; case SPH_F_FLOAT3: return &p.float3;
+    case SPH_F_VOL: return &p.vol; case SPH_F_ENTHALPY: return &p.enthalpy;
+    case SPH_F_RHO_REF: return &p2.rho_ref; case SPH_F_SR: return &p2.SR;
+    case SPH_F_X0: return &p2.x0; case SPH_F_Y0: return &p2.y0; case SPH_F_Z0: return &p2.z0;
+    case SPH_F_UX0: return &p2.ux0; case SPH_F_UY0: return &p2.uy0; case SPH_F_UZ0: return &p2.uz0;
+    case SPH_F_TEMP0: return &p2.temp0; case SPH_F_CONCN0: return &p2.concn0;
+    case SPH_F_FTOTALX: return &q.ftotalx; case SPH_F_FTOTALY: return &q.ftotaly;
+    case SPH_F_FTOTALZ: return &q.ftotalz; case SPH_F_FTOTAL: return &q.ftotal;
+    case SPH_F_FPX: return &q.fpx; case SPH_F_FPY: return &q.fpy; case SPH_F_FPZ: return &q.fpz;
+    case SPH_F_DTEMP: return &q.dtemp; case SPH_F_DCONCN: return &q.dconcn;
+    case SPH_F_INV_CM_XX: return &q.inv_cm_xx; case SPH_F_INV_CM_YY: return &q.inv_cm_yy;
+    case SPH_F_INV_CM_ZZ: return &q.inv_cm_zz; case SPH_F_INV_CM_XY: return &q.inv_cm_xy;
+    case SPH_F_INV_CM_YZ: return &q.inv_cm_yz; case SPH_F_INV_CM_ZX: return &q.inv_cm_zx;
+    case SPH_F_NX: return &o.nx[i]; case SPH_F_NY: return &o.ny[i]; case SPH_F_NZ: return &o.nz[i];
+    }
+    return nullptr;
+}
+
+// Passes of the step that oracle_run_pass runs one at a time on the
+// working state (the step's SP1 / SP2 / P3), for the per-pass pins.
+enum {
+    OP_GRAD_CORR = 0, OP_PREP, OP_BOUNDARY, OP_MOST, OP_NEUMANN, OP_ADVECTION, OP_PROJECTION,
+    OP_PERIODIC, OP_FREESLIP, OP_PPE, OP_OUTLET, OP_PRESSURE_FORCE, OP_IBM_FORCE, OP_IBM_SPREAD,
+    OP_COUNT
+};
+
 }  // namespace
 
 // ------------------------------------------------------------ C API ------
@@ -1571,6 +1636,78 @@ int oracle_step(void* h, int count, double dt, double time) {
     Oracle& o = *(Oracle*)h;
     if (!o.uploaded) return SPH_E_STATE;
     return step(o, count, dt, time);
+}
+
+// Test hooks: the neighbour search alone (set_ncell, hash, sort, cell
+// bounds, reorder; P3 cleared), then single passes on the working state,
+// whose fields can be written and read in the sorted order.
+int oracle_search(void* h) {
+    Oracle& o = *(Oracle*)h;
+    if (!o.uploaded) return SPH_E_STATE;
+    set_ncell(o);
+    nnps(o);
+    o.P3.assign(o.n, P3d{});
+    return 0;
+}
+
+int oracle_run_pass(void* h, int pass, int count, double dt, double time) {
+    Oracle& o = *(Oracle*)h;
+    if (!o.searched) return SPH_E_STATE;
+    switch (pass) {
+    case OP_GRAD_CORR: gradient_correction(o); break;
+    case OP_PREP: prep(o, count, dt); break;
+    case OP_BOUNDARY: boundary3D(o); break;
+    case OP_MOST: most(o, count, dt); break;
+    case OP_NEUMANN: neumann(o, o.SP1); break;
+    case OP_ADVECTION: advection(o); break;
+    case OP_PROJECTION: projection(o, dt, time); break;
+    case OP_PERIODIC: periodic(o, o.SP1); break;
+    case OP_FREESLIP: freeslip(o, o.SP1); break;
+    case OP_PPE: return ppe(o, dt);
+    case OP_OUTLET: outlet(o, o.SP1); break;
+    case OP_PRESSURE_FORCE: pressure_force(o); break;
+    case OP_IBM_FORCE: ibm_force(o, dt); break;
+    case OP_IBM_SPREAD: ibm_spread(o); break;
+    default: return SPH_E_ARG;
+    }
+    return 0;
+}
+
+int oracle_get_work_f64(void* h, int field, double* out, int n) {
+    Oracle& o = *(Oracle*)h;
+    if (!o.searched || n != o.n || field < 0 || field >= SPH_F_COUNT_F64) return SPH_E_ARG;
+    for (int i = 0; i < n; i++) out[i] = *work_ref(o, field, i);
+    return 0;
+}
+
+int oracle_set_work_f64(void* h, int field, const double* in, int n) {
+    Oracle& o = *(Oracle*)h;
+    if (!o.searched || n != o.n || field < 0 || field >= SPH_F_COUNT_F64) return SPH_E_ARG;
+    for (int i = 0; i < n; i++) *work_ref(o, field, i) = in[i];
+    return 0;
+}
+
+int oracle_set_work_i32(void* h, int field, const int* in, int n) {
+    Oracle& o = *(Oracle*)h;
+    if (!o.searched || n != o.n) return SPH_E_ARG;
+    for (int i = 0; i < n; i++) {
+        P1d& p = o.SP1[i];
+        switch (field) {
+        case SPH_I_I_TYPE: p.i_type = in[i]; break;
+        case SPH_I_BUFFER_TYPE: p.buffer_type = in[i]; break;
+        case SPH_I_MOST_BUFFER: p.MOST_buffer = in[i]; break;
+        case SPH_I_P_TYPE: p.p_type = in[i]; break;
+        default: return SPH_E_ARG;
+        }
+    }
+    return 0;
+}
+
+// Planted mistake for the mutation tests (0 = none); returns the previous.
+int oracle_set_mutant(int k) {
+    int prev = g_mutant;
+    g_mutant = k;
+    return prev;
 }
 
 int oracle_get_field_f64(void* h, int field, double* out, int n) {
