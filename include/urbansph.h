@@ -358,6 +358,15 @@ int sph_ppe_info(const sph_ctx* c, sph_ppe_report* r);
  * 3x DeviceReduce::Max, functions_OUTPUT.cuh:19-33, ISPH_Calc.cuh:504-514). */
 int sph_stats(sph_ctx* c, float* umax, float* drho_pct_max, float* fmax);
 
+/* sph_stats without any device allocation, memset or atomic: the same three
+ * maxima, bit for bit.  No counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Its working memory is one
+ * library-owned array of about 768 KB per GPU, held until the process ends
+ * and shared by every call on that GPU: calls that overlap on one GPU race.
+ * Cases of more than 16,777,216 particles get SPH_E_ARG (use sph_stats). */
+int sph_stats_noalloc(sph_ctx* c, float* umax, float* drho_pct_max, float* fmax);
+
 /* Blocks until all enqueued work of the context is done; reports deferred
  * errors (non-convergence, non-finite state). */
 int sph_synchronize(sph_ctx* c);

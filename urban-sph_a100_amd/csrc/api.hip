@@ -239,6 +239,7 @@ int sph_create(const sph_params* prm, int n_part2, sph_ctx** out) {
     sph_ctx* c = new sph_ctx();
     c->prm = *prm;
     c->cap = n_part2;
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return SPH_E_HIP; }
     c->c_pi15 = std::pow(3.141592653, 1.5);
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SPH_E_HIP; }
     c->stream = c->own_stream;
@@ -480,6 +481,12 @@ int sph_stats(sph_ctx* c, float* umax, float* drho_pct_max, float* fmax_) {
     if (!c || !umax || !drho_pct_max || !fmax_) return SPH_E_ARG;
     if (!c->searched) return fail(c, SPH_E_STATE, "no step yet");
     return stats_run(c, umax, drho_pct_max, fmax_);
+}
+
+int sph_stats_noalloc(sph_ctx* c, float* umax, float* drho_pct_max, float* fmax_) {
+    if (!c || !umax || !drho_pct_max || !fmax_) return SPH_E_ARG;
+    if (!c->searched) return fail(c, SPH_E_STATE, "no step yet");
+    return stats_run_noalloc(c, umax, drho_pct_max, fmax_);
 }
 
 static int profile_enable(Eng& e, int enable) {

@@ -13,6 +13,9 @@
 // always accumulated in double.  Differences of positions and velocities are
 // taken in double and then rounded to R, so the float path loses no
 // significance to cancellation.
+#include <mutex>
+#include <string>
+
 #include "flags.h"
 #include "sph_internal.h"
 #include "sph_math.h"
@@ -1344,6 +1347,102 @@ int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax_) {
     SPH_TRY(launch_check(c, "stats_final"));
     float h[3];
     SPH_HIP(c, hipMemcpyAsync(h, c->stat_buf + 4, 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    *umax = h[0]; *drho = h[1]; *fmax_ = h[2];
+    return 0;
+}
+
+// ---- statistics without allocation (sph_stats_noalloc) -------------------
+// Per-workgroup maxima go to a file-scope device array instead of the
+// context's buffer, and a one-workgroup fold reduces them, so the call needs
+// no device allocation, no memset and no atomics.  The array is this
+// library's own on each GPU: every call on a GPU shares it, so calls that
+// overlap on one GPU race.  Particles up to NA_STATS_MAX_BLOCKS * BLOCK.
+constexpr int NA_STATS_MAX_BLOCKS = 1 << 16;
+constexpr int NA_STATS_OUT = 3 * NA_STATS_MAX_BLOCKS;   // first word of the 3 float results
+constexpr long long NA_STATS_MAX_N = (long long)NA_STATS_MAX_BLOCKS * BLOCK;
+static __device__ __align__(256) unsigned scratch[NA_STATS_OUT + 64];
+
+// The maxima of k_stats for this workgroup's particles, stored at part[3b..3b+2].
+__global__ void __launch_bounds__(BLOCK) k_stats_part(int n, PView P, P2View S2, QView Q, unsigned* part) {
+    __shared__ unsigned s[3][BLOCK];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float a = 0, b = 0, c = 0;
+    if (i < n && !(P.p_type[i] == 0 || P.i_type[i] >= 2)) {
+        a = (float)sqrt(P.ux[i] * P.ux[i] + P.uy[i] * P.uy[i] + P.uz[i] * P.uz[i]);
+        b = (float)((P.rho[i] - S2.rho_ref[i]) / S2.rho_ref[i] * 100.0);
+        c = (float)Q.ftotal[i];
+    }
+    s[0][threadIdx.x] = i < n ? f2ord(a) : 0u;
+    s[1][threadIdx.x] = i < n ? f2ord(b) : 0u;
+    s[2][threadIdx.x] = i < n ? f2ord(c) : 0u;
+    __syncthreads();
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 3; q++) s[q][threadIdx.x] = max(s[q][threadIdx.x], s[q][threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+// One workgroup: the maxima over the nb partials (starting from 0, as the
+// zeroed accumulator of k_stats does), as floats into out[0..2].
+__global__ void __launch_bounds__(BLOCK) k_stats_fold(int nb, const unsigned* part, float* out) {
+    __shared__ unsigned s[3][BLOCK];
+    unsigned a = 0u, b = 0u, c = 0u;
+    for (int k = threadIdx.x; k < nb; k += BLOCK) {
+        a = max(a, part[3 * k]);
+        b = max(b, part[3 * k + 1]);
+        c = max(c, part[3 * k + 2]);
+    }
+    s[0][threadIdx.x] = a; s[1][threadIdx.x] = b; s[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 3; q++) s[q][threadIdx.x] = max(s[q][threadIdx.x], s[q][threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[threadIdx.x] = ord2f(s[threadIdx.x][0]);
+}
+
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (the host-side name of the array is no device address).
+static std::mutex na_mu;
+static void* na_addr[64] = {};
+
+static int na_scratch(sph_ctx* c, unsigned** out) {
+    const int dev = c->device;
+    if (dev < 0 || dev >= 64) return fail(c, SPH_E_ARG, "sph_stats_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (!na_addr[dev]) {
+        int cur = -1;
+        SPH_HIP(c, hipGetDevice(&cur));
+        if (cur != dev) SPH_HIP(c, hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t er = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) SPH_HIP(c, hipSetDevice(cur));
+        if (er != hipSuccess || !p)
+            return fail(c, SPH_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(er));
+        na_addr[dev] = p;
+    }
+    *out = (unsigned*)na_addr[dev];
+    return 0;
+}
+
+int stats_run_noalloc(sph_ctx* c, float* umax, float* drho, float* fmax_) {
+    const int n = c->n;
+    if ((long long)n > NA_STATS_MAX_N)
+        return fail(c, SPH_E_ARG, "sph_stats_noalloc: " + std::to_string(n) + " particles exceed its bound of " +
+                                      std::to_string(NA_STATS_MAX_N) + " (use sph_stats)");
+    unsigned* sc = nullptr;
+    SPH_TRY(na_scratch(c, &sc));
+    const unsigned nb = grid1(n);
+    hipLaunchKernelGGL(k_stats_part, dim3(nb), dim3(BLOCK), 0, c->stream, n, c->pview(), c->s2view(), c->qview(), sc);
+    SPH_TRY(launch_check(c, "stats_part"));
+    float* res = (float*)(sc + NA_STATS_OUT);
+    hipLaunchKernelGGL(k_stats_fold, dim3(1), dim3(BLOCK), 0, c->stream, (int)nb, (const unsigned*)sc, res);
+    SPH_TRY(launch_check(c, "stats_fold"));
+    float h[3];
+    SPH_HIP(c, hipMemcpyAsync(h, res, 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
     *umax = h[0]; *drho = h[1]; *fmax_ = h[2];
     return 0;

@@ -278,6 +278,7 @@ struct Eng {
 struct sph_ctx {
     sph_params prm{};
     int cap = 0, n = 0;
+    int device = 0;                     // HIP device current at sph_create
     bool uploaded = false, searched = false;
     sph_domain dom{};
     double c_pi15 = 0.0;
@@ -409,6 +410,7 @@ int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out);
 int step_physics_front(Eng& e);
 int step_physics_back(Eng& e);
 int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax);
+int stats_run_noalloc(sph_ctx* c, float* umax, float* drho, float* fmax);
 
 // ppe.hip
 int ppe_build(sph_ctx* c);

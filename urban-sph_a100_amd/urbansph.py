@@ -37,6 +37,7 @@ def lib():
     L.sph_domain_info.argtypes = [vp, C.POINTER(abi.Domain)]
     L.sph_ppe_info.argtypes = [vp, C.POINTER(abi.PPEReport)]
     L.sph_stats.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.sph_stats_noalloc.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
     L.sph_synchronize.argtypes = [vp]
     L.sph_profile_enable.argtypes = [vp, C.c_int]
     L.sph_profile_report.argtypes = [vp, C.POINTER(abi.KernelTime), C.c_int]
@@ -88,7 +89,7 @@ def lib():
 EXPORTED = [
     "sph_params_default", "sph_create", "sph_upload", "sph_step", "sph_download",
     "sph_get_field_f64", "sph_get_field_i32", "sph_nnps_debug", "sph_neighbors",
-    "sph_domain_info", "sph_ppe_info", "sph_stats", "sph_synchronize",
+    "sph_domain_info", "sph_ppe_info", "sph_stats", "sph_stats_noalloc", "sph_synchronize",
     "sph_profile_enable", "sph_profile_report", "sph_device_bytes",
     "sph_last_error", "sph_destroy", "sph_set_engine_options", "sph_tuning", "sph_apr_refine",
     "sph_batch_create", "sph_batch_step", "sph_batch_synchronize", "sph_batch_set_options",
@@ -268,6 +269,11 @@ class Case:
     def stats(self):
         a, b, c = C.c_float(), C.c_float(), C.c_float()
         self._check(self.L.sph_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "stats")
+        return a.value, b.value, c.value
+
+    def stats_noalloc(self):
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        self._check(self.L.sph_stats_noalloc(self.h, C.byref(a), C.byref(b), C.byref(c)), "stats_noalloc")
         return a.value, b.value, c.value
 
     def profile(self, on=True):
