@@ -185,7 +185,30 @@ typedef struct sph_params {
                                    inner solve that stalls, the solve uses
                                    SPH_PRECOND_JACOBI.  The report says which
                                    ran.  FP64 ignores it.                     */
+    int    ib_normals;          /* where the IB markers' (p_type >= 1000)
+                                   normals come from, SPH_IBN_*.
+                                   SPH_IBN_REFERENCE (default): every
+                                   count == 0 step sets them from the face
+                                   planes below (KERNEL_clc_IB_normal_vector3D,
+                                   functions_PREP.cuh:923-1015), and steps with
+                                   count > 0 run with zero normals, as the
+                                   reference's per-step part3 reset leaves them
+                                   (ISPH_Calc.cuh:61).  SPH_IBN_PLANES: the
+                                   same plane normals, kept at later steps.
+                                   SPH_IBN_UPLOADED: the normals passed to
+                                   sph_upload (zero where none), kept.         */
+    double ib_face_xlo, ib_face_xhi, ib_face_ylo, ib_face_yhi, ib_face_zlo, ib_face_zhi;
+                                /* marker face planes of the plane rule: a
+                                   marker within ib_face_tol of x = xlo gets
+                                   n_x = -1, of x = xhi n_x = +1 (y, z alike),
+                                   and the sum is normalised in float.
+                                   Defaults 0.764, 0.836, -0.036, 0.036, 0,
+                                   0.152 (functions_PREP.cuh:958-963)         */
+    double ib_face_tol;         /* 1e-5 m                                     */
 } sph_params;
+
+/* Sources of the IB marker normals (ib_normals). */
+enum { SPH_IBN_REFERENCE = 0, SPH_IBN_PLANES = 1, SPH_IBN_UPLOADED = 2 };
 
 /* Preconditioners of the MIXED inner solve (ppe_precond). */
 enum { SPH_PRECOND_JACOBI = 0, SPH_PRECOND_MULTIGRID = 1 };

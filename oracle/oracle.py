@@ -80,7 +80,7 @@ def set_threads(n):
 
 # Passes of oracle_run_pass (the oracle's OP_* order).
 PASSES = ("grad_corr", "prep", "boundary", "most", "neumann", "advection", "projection", "periodic",
-          "freeslip", "ppe", "outlet", "pressure_force", "ibm_force", "ibm_spread")
+          "freeslip", "ppe", "outlet", "pressure_force", "ibm_force", "ibm_spread", "ib_normals")
 
 
 def set_mutant(k):

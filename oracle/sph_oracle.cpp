@@ -416,6 +416,40 @@ void nnps(Oracle& o) {
 }
 
 // KERNEL_clc_gradient_correction_3D, functions_KNL.cuh:216-324
+// KERNEL_clc_IB_normal_vector3D, functions_PREP.cuh:923-1015, run at count ==
+// 0 (ISPH_Calc.cuh:70-74): a marker (p_type >= 1000) within ib_face_tol of a
+// face plane gets -1 / +1 on that axis (the lower plane first, so the upper
+// one wins a tie), and the vector is normalised in float.  At count > 0 the
+// reference runs with the zero normals its part3 reset leaves
+// (ISPH_Calc.cuh:61).  Modes: SPH_IBN_REFERENCE as the reference,
+// SPH_IBN_PLANES keeps the count == 0 normals, SPH_IBN_UPLOADED keeps the
+// caller's.
+void ib_normals(Oracle& o, int count) {
+    const sph_params& pr = o.prm;
+    if (pr.ib_normals == SPH_IBN_UPLOADED) return;
+    if (count > 0 && pr.ib_normals == SPH_IBN_PLANES) return;
+    const auto& S = o.SP1;
+    for (int i = 0; i < o.n; i++) {
+        if (S[i].p_type < 1000) continue;
+        float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        if (count == 0) {
+            const double xi = S[i].x, yi = S[i].y, zi = S[i].z, tol = pr.ib_face_tol;
+            if (fabs(xi - pr.ib_face_xlo) < tol) nx = -1.0f;
+            if (fabs(xi - pr.ib_face_xhi) < tol) nx = MUT(34) ? -1.0f : 1.0f;
+            if (fabs(yi - pr.ib_face_ylo) < tol) ny = -1.0f;
+            if (fabs(yi - pr.ib_face_yhi) < tol) ny = 1.0f;
+            if (fabs(zi - pr.ib_face_zlo) < tol) nz = -1.0f;
+            if (fabs(zi - pr.ib_face_zhi) < tol) nz = 1.0f;
+        }
+        const float mag = MUT(35) ? std::fabs(nx) + std::fabs(ny) + std::fabs(nz) : sqrtf(nx * nx + ny * ny + nz * nz);
+        if ((double)mag > 1e-9) {
+            o.nx[i] = (double)(nx / mag); o.ny[i] = (double)(ny / mag); o.nz[i] = (double)(nz / mag);
+        } else {
+            o.nx[i] = 0.0; o.ny[i] = 0.0; o.nz[i] = 0.0;
+        }
+    }
+}
+
 void gradient_correction(Oracle& o) {
     const sph_params& pr = o.prm;
     auto& S = o.SP1;
@@ -1454,6 +1488,7 @@ int step(Oracle& o, int count, double dt, double time) {
         o.SP1 = o.P1; o.SP2 = o.P2;
     }
     o.P3.assign(o.n, P3d{});
+    ib_normals(o, count);
     gradient_correction(o);
     prep(o, count, dt);
     if (pr.noslip_bc == 1) boundary3D(o);
@@ -1549,7 +1584,7 @@ double* work_ref(Oracle& o, int f, int i) {
 // working state (the step's SP1 / SP2 / P3), for the per-pass pins.
 enum {
     OP_GRAD_CORR = 0, OP_PREP, OP_BOUNDARY, OP_MOST, OP_NEUMANN, OP_ADVECTION, OP_PROJECTION,
-    OP_PERIODIC, OP_FREESLIP, OP_PPE, OP_OUTLET, OP_PRESSURE_FORCE, OP_IBM_FORCE, OP_IBM_SPREAD,
+    OP_PERIODIC, OP_FREESLIP, OP_PPE, OP_OUTLET, OP_PRESSURE_FORCE, OP_IBM_FORCE, OP_IBM_SPREAD, OP_IB_NORMALS,
     OP_COUNT
 };
 
@@ -1582,6 +1617,10 @@ void oracle_params_default(sph_params* p) {
     p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1;
     p->ale_positions = 0;
     p->ppe_precond = SPH_PRECOND_JACOBI;   // the library's multigrid option; the oracle solves with Jacobi
+    // IB normals: the face planes of functions_PREP.cuh:958-963
+    p->ib_normals = SPH_IBN_REFERENCE;
+    p->ib_face_xlo = 0.764; p->ib_face_xhi = 0.836; p->ib_face_ylo = -0.036; p->ib_face_yhi = 0.036;
+    p->ib_face_zlo = 0.0; p->ib_face_zhi = 0.152; p->ib_face_tol = 1e-5;
 }
 
 void* oracle_create(const sph_params* prm, int n) {
@@ -1668,6 +1707,7 @@ int oracle_run_pass(void* h, int pass, int count, double dt, double time) {
     case OP_PRESSURE_FORCE: pressure_force(o); break;
     case OP_IBM_FORCE: ibm_force(o, dt); break;
     case OP_IBM_SPREAD: ibm_spread(o); break;
+    case OP_IB_NORMALS: ib_normals(o, count); break;
     default: return SPH_E_ARG;
     }
     return 0;

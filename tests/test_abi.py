@@ -98,7 +98,8 @@ def test_enum_values_match_c(tmp_path):
              ("MG_OK", "SPH_MG_OK"), ("MG_UNSUITED", "SPH_MG_UNSUITED"), ("MG_TERMS", "SPH_MG_TERMS"),
              ("MG_COLUMNS", "SPH_MG_COLUMNS"), ("MG_SPAN", "SPH_MG_SPAN"), ("MG_PIVOT", "SPH_MG_PIVOT"),
              ("MG_STALLED", "SPH_MG_STALLED"), ("MG_LEVELS", "SPH_MG_LEVELS"),
-             ("MG_STALLED_SOLVE", "SPH_MG_STALLED_SOLVE")]
+             ("MG_STALLED_SOLVE", "SPH_MG_STALLED_SOLVE"), ("IBN_REFERENCE", "SPH_IBN_REFERENCE"),
+             ("IBN_PLANES", "SPH_IBN_PLANES"), ("IBN_UPLOADED", "SPH_IBN_UPLOADED")]
     lines = ['#include <stdio.h>', '#include "urbansph.h"', 'int main(void){']
     lines += [f'printf("%d\\n", (int){c});' for _, c in pairs]
     lines.append("return 0;}")

@@ -22,6 +22,10 @@ def _load(path):
     blob = z["params"].tobytes()
     prm = oracle.default_params()
     C.memmove(C.addressof(prm), blob, min(len(blob), C.sizeof(prm)))
+    if len(blob) <= abi.Params.ib_normals.offset:
+        # written before the marker-normal source was a parameter: the
+        # fixture's normals were the uploaded ones
+        prm.ib_normals = abi.IBN_UPLOADED
     nrm = z["normals"] if "normals" in z.files else None
     return z, prm, nrm
 

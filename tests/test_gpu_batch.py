@@ -12,6 +12,7 @@ import pytest
 import abi
 import cases
 import urbansph
+from test_gpu_parity import check_mixed
 
 pytestmark = pytest.mark.gpu
 
@@ -278,12 +279,15 @@ def test_lattice_order_matches_cell_order():
     prm0.update(ppe_reorder=0)
     ref_prm = type(prm).from_buffer_copy(prm)
     ref_prm.update(precision=abi.FP64, ppe_tol=1e-11)
+    tol_prm = type(prm).from_buffer_copy(prm)
+    tol_prm.update(precision=abi.FP64)
     ref, _ = _alone((P, nrm, ref_prm, dt), 3)
+    at_tol, _ = _alone((P, nrm, tol_prm, dt), 3)
     a, ra = _alone((P, nrm, prm0, dt), 3)
     b, rb = _alone(spec, 3)
     assert ra[2] == 1 and rb[2] == 1
-    for f in ("UX", "UY", "UZ", "PRES"):
-        ea = np.linalg.norm(a[f] - ref[f]) / np.linalg.norm(ref[f])
-        eb = np.linalg.norm(b[f] - ref[f]) / np.linalg.norm(ref[f])
-        print(f"[lattice] {f}: lattice {eb:.2e}  cell {ea:.2e}", flush=True)
-        assert eb <= 2.0 * ea + 1e-6, f
+    for name, got, p in (("cell", a, prm0), ("lattice", b, prm)):
+        tight = type(p).from_buffer_copy(p)
+        tight.update(ppe_tol=1e-10)
+        t, _ = _alone((P, nrm, tight, dt), 3)
+        check_mixed(f"lattice {name}", got, ref, at_tol, t)

@@ -17,6 +17,7 @@ import abi
 import cases
 import oracle
 import urbansph
+from test_gpu_parity import check_mixed
 
 pytestmark = pytest.mark.gpu
 
@@ -75,11 +76,8 @@ def test_multigrid_same_tolerance_as_fp64(which):
     assert rep["converged"] == 1 and rep["rel_residual"] <= 1e-6
     assert jac[0]["precond"] == abi.PRECOND_JACOBI and jac[0]["mg_levels"] == 0
     assert rep["iterations"] < jac[0]["iterations"]
-    for f in conv:
-        e_mg = rel_l2(g.field(f), conv[f])
-        e_fp64 = rel_l2(at_tol[f], conv[f])
-        print(f"[mg] {which} {f}: multigrid@1e-6 {e_mg:.2e}  fp64@1e-6 {e_fp64:.2e}", flush=True)
-        assert e_mg <= 2.0 * e_fp64 + 1e-6, f
+    gt, _ = _gpu(P, nrm, mk(1e-10), dt)
+    check_mixed(f"mg {which}", {f: g.field(f) for f in conv}, conv, at_tol, {f: gt.field(f) for f in conv})
 
 
 @pytest.mark.parametrize("which", ["box", "canyon"])

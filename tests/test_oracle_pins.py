@@ -717,7 +717,43 @@ def pin_ddf():
             assert k0 + dt * (nu0 * SR * SR - eps) < 1e-4     # the floor case
 
 
+# ------------------------------------------------------- IB normals --
+def _face_deck(mode):
+    P, nrm, geom, faces = cases.face_marker_canyon()
+    prm = cases.canyon_params(geom, oracle.default_params(), tol=1e-8)
+    prm.update(ib_normals=mode, **faces)
+    return P, nrm, geom, faces, prm
+
+
+def pin_ib_normals():
+    """Marker normals of the face-plane rule (functions_PREP.cuh:923-1015):
+    on a face the outward unit normal of that face, on an edge the outward
+    bisector, on a corner the outward diagonal, each of unit length to float
+    precision; zero off the planes and on every non-marker."""
+    P, nrm, geom, faces, prm = _face_deck(abi.IBN_REFERENCE)
+    o = _searched(P, prm, nrm * 0.0)
+    assert o.run_pass("ib_normals", count=0) == 0
+    S = _state(o, P)
+    pos, mk = S["pos"], S["P_TYPE"] >= 1000
+    n = np.stack([o.work("NX"), o.work("NY"), o.work("NZ")], 1)
+    assert np.all(n[~mk] == 0.0)
+    lo = np.array([faces["ib_face_xlo"], faces["ib_face_ylo"], faces["ib_face_zlo"]])
+    hi = np.array([faces["ib_face_xhi"], faces["ib_face_yhi"], faces["ib_face_zhi"]])
+    on_lo = np.abs(pos - lo) < 1e-5
+    on_hi = np.abs(pos - hi) < 1e-5
+    out = on_hi.astype(float) - on_lo.astype(float)        # outward direction, per axis
+    k = np.abs(out).sum(1)
+    assert (k[mk] == 1).sum() > 40 and (k[mk] == 2).sum() >= 10 and (k[mk] == 3).sum() == 4
+    assert (k[mk] == 0).sum() == 1
+    for m in (1, 2, 3):
+        sel = mk & (k == m)
+        _close(n[sel], out[sel] / math.sqrt(m), 1e-7, f"normal on {m} planes")
+        _close(np.linalg.norm(n[sel], axis=1), np.ones(sel.sum()), 1e-7, f"unit length on {m} planes")
+    assert np.all(n[mk & (k == 0)] == 0.0)
+
+
 PINS = {f.__name__: f for f in (
+    pin_ib_normals,
     pin_pressure_force_lattice, pin_pressure_force_pairs, pin_cleary_lattice, pin_euler_lattice,
     pin_advection_pairs_cleary, pin_advection_pairs_monaghan, pin_ibm_force, pin_ibm_spread, pin_periodic,
     pin_outlet, pin_neumann, pin_most_newton, pin_prep_lattice, pin_ddf)}
@@ -757,6 +793,8 @@ MUTANTS = {
     31: ("pin_ddf",),                                                 # eps from the updated k
     32: ("pin_ddf",),                                                 # nu_t without sqrt
     33: ("pin_ddf",),                                                 # k floor 1e-4 -> 0
+    34: ("pin_ib_normals",),                                          # upper x face points inward
+    35: ("pin_ib_normals",),                                          # normalised by the 1-norm
 }
 
 
@@ -774,3 +812,30 @@ def test_planted_mistake_is_caught(mutant, name):
             PINS[name]()
     finally:
         oracle.set_mutant(prev)
+
+
+@pytest.mark.parametrize("mode", [abi.IBN_REFERENCE, abi.IBN_PLANES, abi.IBN_UPLOADED])
+def test_ib_normal_modes_over_steps(mode):
+    """Which normals each mode leaves after the count == 0 step and after a
+    later one: the reference's rule sets them at count == 0 and its part3
+    reset zeroes them at later steps (ISPH_Calc.cuh:61,70-74); the plane mode
+    keeps them; the uploaded mode keeps the caller's."""
+    P, nrm, geom, faces, prm = _face_deck(mode)
+    up = nrm.copy()
+    up[P["p_type"] == 1000] = (0.0, 0.6, 0.8)      # differs from every plane normal
+    o = oracle.Oracle(prm, len(P))
+    o.upload(P, None, up)
+    plane = None
+    for count in range(2):
+        assert o.step(count, 5e-5) == 0
+        n = np.stack([o.field("NX"), o.field("NY"), o.field("NZ")], 1)
+        mk = o.ifield("P_TYPE") >= 1000
+        if mode == abi.IBN_UPLOADED:
+            assert np.allclose(n[mk], np.array([0.0, 0.6, 0.8], np.float32).astype(np.float64), rtol=0, atol=0)
+        elif count == 0:
+            plane = n
+            assert (np.abs(n[mk]).sum(1) > 0).sum() == mk.sum() - 1
+        elif mode == abi.IBN_REFERENCE:
+            assert np.all(n == 0.0)
+        else:
+            assert np.array_equal(n, plane)

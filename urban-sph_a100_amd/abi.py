@@ -88,7 +88,10 @@ class Params(C.Structure):
            ("ppe_tol", C.c_double), ("ppe_maxit", C.c_int),
            ("ppe_inner_tol", C.c_double), ("ppe_max_refine", C.c_int), ("ppe_index16", C.c_int),
            ("ppe_reorder", C.c_int), ("apr_h_change", C.c_int),
-           ("ale_positions", C.c_int), ("ppe_precond", C.c_int)]
+           ("ale_positions", C.c_int), ("ppe_precond", C.c_int), ("ib_normals", C.c_int)]
+        + [(n, C.c_double) for n in (
+            "ib_face_xlo", "ib_face_xhi", "ib_face_ylo", "ib_face_yhi", "ib_face_zlo", "ib_face_zhi",
+            "ib_face_tol")]
     )
 
     def update(self, **kw):
@@ -119,6 +122,7 @@ class PPEReport(C.Structure):
 
 ORDER_CELL, ORDER_LATTICE = 0, 1
 PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1
+IBN_REFERENCE, IBN_PLANES, IBN_UPLOADED = 0, 1, 2
 (MG_OK, MG_UNSUITED, MG_TERMS, MG_COLUMNS, MG_SPAN, MG_PIVOT, MG_STALLED, MG_LEVELS,
  MG_STALLED_SOLVE) = range(9)
 

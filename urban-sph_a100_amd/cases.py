@@ -100,7 +100,8 @@ def box_params(P=None, dt=1e-4, tol=1e-8, precision=abi.FP64):
     prm.update(kernel_type=abi.WENDLAND2, kappa=2.0, fv_solve=abi.FV_CLEARY,
                turbulence_model=abi.LAMINAR, fg_solve=0, time_type=abi.PRE_COR,
                noslip_bc=1, mu_fluid=1.0e-3, u_limit=1000.0,
-               ppe_solver=abi.PPE_BICGSTAB, ppe_tol=tol, precision=precision)
+               ppe_solver=abi.PPE_BICGSTAB, ppe_tol=tol, precision=precision,
+               ib_normals=abi.IBN_UPLOADED)   # outward normals of every building, from the generator
     return prm
 
 
@@ -233,6 +234,39 @@ def small_canyon(dx=0.01, **kw):
                 bsize=(0.05, 0.05, 0.08))
     args.update(kw)
     return canyon(**args)
+
+
+def face_marker_canyon(dx=0.02):
+    """small_canyon with markers added on the building's vertical edges, roof
+    edges and roof corners and one marker off every face plane (at mid-height
+    inside the building), for the face-plane rule of the marker normals.
+    Returns (part1, generator normals, geometry, faces) with faces the
+    ib_face_* parameters of the building (no marker lies on its z = 0 plane)."""
+    P, nrm, geom = small_canyon(dx=dx)
+    xc, yc, bx, by, bh = geom["buildings"][0]
+    xlo, xhi, ylo, yhi = xc - 0.5 * bx, xc + 0.5 * bx, yc - 0.5 * by, yc + 0.5 * by
+    extra = []
+    for zz in (0.25 * bh, 0.5 * bh):              # vertical edges
+        for xx in (xlo, xhi):
+            for yy in (ylo, yhi):
+                extra.append((xx, yy, zz))
+    for xx in (xlo, xhi):                         # roof edges along y, and corners
+        for yy in (yc, ylo, yhi):
+            extra.append((xx, yy, bh))
+    for yy in (ylo, yhi):                         # roof edges along x
+        extra.append((xc, yy, bh))
+    extra.append((xc, yc, 0.5 * bh))              # on no face plane
+    extra = np.array(extra)
+    add = np.zeros(len(extra), P.dtype)
+    proto = P[np.flatnonzero(P["p_type"] == 1000)[0]]
+    for name in P.dtype.names:
+        add[name] = proto[name]
+    add["x"], add["y"], add["z"] = extra[:, 0], extra[:, 1], extra[:, 2]
+    P2 = np.concatenate([P, add])
+    nrm2 = np.concatenate([nrm, np.zeros((len(extra), 3), np.float32)])
+    faces = dict(ib_face_xlo=xlo, ib_face_xhi=xhi, ib_face_ylo=ylo, ib_face_yhi=yhi,
+                 ib_face_zlo=0.0, ib_face_zhi=bh)
+    return P2, nrm2, geom, faces
 
 
 def apr_targets(x, y, z, p_type, i_type, buildings, r1, r2, most=None):

@@ -205,6 +205,10 @@ void sph_params_default(sph_params* p) {
     p->ppe_solver = SPH_PPE_REF_JACOBI; p->precision = SPH_FP64; p->ppe_tol = 1e-8; p->ppe_maxit = 5000;
     p->ppe_inner_tol = 1e-5; p->ppe_max_refine = 20; p->ppe_index16 = 1; p->ppe_reorder = 1; p->apr_h_change = 1; p->ale_positions = 0;
     p->ppe_precond = SPH_PRECOND_JACOBI;
+    // IB normals: the face planes of functions_PREP.cuh:958-963
+    p->ib_normals = SPH_IBN_REFERENCE;
+    p->ib_face_xlo = 0.764; p->ib_face_xhi = 0.836; p->ib_face_ylo = -0.036; p->ib_face_yhi = 0.036;
+    p->ib_face_zlo = 0.0; p->ib_face_zhi = 0.152; p->ib_face_tol = 1e-5;
 }
 
 static int validate(const sph_params* p, std::string& why) {
@@ -224,6 +228,7 @@ static int validate(const sph_params* p, std::string& why) {
     if (p->precision == SPH_MIXED && (!(p->ppe_inner_tol > 0) || p->ppe_max_refine < 0)) { why = "ppe_inner_tol"; return SPH_E_ARG; }
     if (p->most_mode != SPH_MOST_REFERENCE && p->most_mode != SPH_MOST_PRESCRIBED_L) { why = "most_mode"; return SPH_E_ARG; }
     if (p->ppe_precond != SPH_PRECOND_JACOBI && p->ppe_precond != SPH_PRECOND_MULTIGRID) { why = "ppe_precond"; return SPH_E_ARG; }
+    if (p->ib_normals < SPH_IBN_REFERENCE || p->ib_normals > SPH_IBN_UPLOADED) { why = "ib_normals"; return SPH_E_ARG; }
     return 0;
 }
 

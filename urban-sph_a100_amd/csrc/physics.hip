@@ -1287,11 +1287,51 @@ int pack_records(Eng& e, int which) {
 
 // Everything of SOPHIA_single_ISPH before the pressure solve
 // (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
+// KERNEL_clc_IB_normal_vector3D, functions_PREP.cuh:923-1015 (count == 0,
+// ISPH_Calc.cuh:70-74): markers within ib_face_tol of a face plane get -1 / +1
+// on that axis (the upper plane wins a tie, as it is tested last), normalised
+// in float with IEEE round-to-nearest square root and division.  In
+// SPH_IBN_REFERENCE mode a step with count > 0 writes zero normals, the
+// values the reference's per-step part3 reset leaves (ISPH_Calc.cuh:61).
+__global__ void __launch_bounds__(BLOCK) k_ib_normals(Seg sg) {
+    SEG_PARTICLE(sg);
+    const sph_params& pr = cd.pr;
+    if (pr.ib_normals == SPH_IBN_UPLOADED) return;
+    if (cd.count > 0 && pr.ib_normals == SPH_IBN_PLANES) return;
+    const PView S = cd.S;
+    if (S.p_type[i] < 1000) return;
+    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    if (cd.count == 0) {
+        const double xi = S.x[i], yi = S.y[i], zi = S.z[i], tol = pr.ib_face_tol;
+        if (fabs(xi - pr.ib_face_xlo) < tol) nx = -1.0f;
+        if (fabs(xi - pr.ib_face_xhi) < tol) nx = 1.0f;
+        if (fabs(yi - pr.ib_face_ylo) < tol) ny = -1.0f;
+        if (fabs(yi - pr.ib_face_yhi) < tol) ny = 1.0f;
+        if (fabs(zi - pr.ib_face_zlo) < tol) nz = -1.0f;
+        if (fabs(zi - pr.ib_face_zhi) < tol) nz = 1.0f;
+    }
+    const float mag = __fsqrt_rn(nx * nx + ny * ny + nz * nz);
+    const QView Q = cd.Q;
+    if ((double)mag > 1e-9) {
+        Q.nx[i] = (double)__fdiv_rn(nx, mag);
+        Q.ny[i] = (double)__fdiv_rn(ny, mag);
+        Q.nz[i] = (double)__fdiv_rn(nz, mag);
+    } else {
+        Q.nx[i] = 0.0; Q.ny[i] = 0.0; Q.nz[i] = 0.0;
+    }
+}
+
 int step_physics_front(Eng& e) {
     const bool mixed = mixed_of(e);
     bool any_count0 = false;
     for (size_t k = 0; k < e.cs.size(); k++) any_count0 |= e.h_desc[k].count == 0;
     SEGL("step_begin", -1, k_step_begin);
+    bool any_nrm = false;   // a case whose marker normals this step sets
+    for (size_t k = 0; k < e.cs.size(); k++) {
+        const int m = e.cs[k]->prm.ib_normals;
+        any_nrm |= m == SPH_IBN_REFERENCE || (m == SPH_IBN_PLANES && e.h_desc[k].count == 0);
+    }
+    if (any_nrm) SEGL("ib_normals", -1, k_ib_normals);
     bool any_gc = false;
     for (size_t k = 0; k < e.cs.size(); k++) any_gc |= !e.h_desc[k].gc_fresh;
     if (any_gc) {
