@@ -60,7 +60,7 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
         case.step(count, dt)
         case.sync()
         if first is None:   # the first step from the deck: the per-step precision check
-            first = (case.field("PRES"), case.field("UX"))
+            first = {f: case.field(f) for f in ("PRES", "UX", "UY", "UZ")}
         count += 1
     t_setup = time.perf_counter() - t_setup
     lat = []
@@ -93,15 +93,35 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
     return out
 
 
-def run_first_step(cases, abi, urbansph, P, nrm, geom, dt, tol):
-    """Pressure and x-velocity after the first FP64 step from the deck."""
-    prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.FP64)
+def run_first_step(cases, abi, urbansph, P, nrm, geom, dt, tol, precision=None):
+    """Pressure and velocity after the first step from the deck (FP64 unless
+    another precision is given; MIXED runs the multigrid)."""
+    precision = abi.FP64 if precision is None else precision
+    prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=precision)
+    if precision == abi.MIXED:
+        prm.update(ppe_precond=abi.PRECOND_MULTIGRID)
     case = urbansph.Case(prm, len(P))
     case.upload(P, None, nrm)
     case.step(0, dt)
     case.sync()
-    out = (case.field("PRES"), case.field("UX"))
+    out = {f: case.field(f) for f in ("PRES", "UX", "UY", "UZ")}
     case.close()
+    return out
+
+
+def precision_report(first_fp64, first_mixed, first_conv, floor_mixed):
+    """Relative L2 distances after the first step from the deck: MIXED and
+    FP64 at the configured PPE tolerance against FP64 converged to 1e-4 of
+    it, and the float floor (MIXED at that tight tolerance), for the pressure
+    and the velocity vector."""
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    def vec(d):
+        return np.concatenate([d["UX"], d["UY"], d["UZ"]])
+    out = {}
+    for name, d in (("mixed", first_mixed), ("fp64", first_fp64), ("mixed_float_floor", floor_mixed)):
+        out[name] = {"pres": rel(d["PRES"], first_conv["PRES"]), "u": rel(vec(d), vec(first_conv))}
     return out
 
 
@@ -279,11 +299,12 @@ def run_ale(cases, abi, urbansph, P, nrm, geom, steps, warmup, dt, tol):
     return out
 
 
-def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=24, rate=4.0):
+def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=16, rate=4.0, lo=1e4, hi=2e6, steps_lo=50,
+                steps_hi=500):
     """A seeded Poisson stream of canyon requests through the server: wall
     clock, requests submitted at their arrival times, one co-batched step per
     tick.  Latency = completion - arrival."""
-    reqs = cases.serving_requests(n_req=n_req, rate=rate)
+    reqs = cases.serving_requests(n_req=n_req, rate=rate, lo=lo, hi=hi, steps_lo=steps_lo, steps_hi=steps_hi)
     decks = []
     for r in reqs:
         P, nrm, geom = cases.sized_canyon(r["target"], r["u_scale"], r["bh"])
@@ -314,7 +335,10 @@ def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=24, rate=4.0)
     wait = np.array([i.t_admit - i.t_arrival for i in infos])
     st = srv.status()
     span = max(i.t_finish for i in infos) - (t0 + reqs[0]["t_arrival"])
+    offered = sum(len(d[0]) * r["steps"] for d, r in zip(decks, reqs)) / max(reqs[-1]["t_arrival"], 1e-9)
     out = dict(requests=n_req, rate_per_s=rate, failed=sum(i.state != abi.REQ_DONE for i in infos),
+               offered_particle_steps_per_s=offered, span_s=span, lo=lo, hi=hi, steps_lo=steps_lo, steps_hi=steps_hi,
+               latencies_s=sorted(round(float(x), 3) for x in lat),
                p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
                admit_wait_p99_ms=float(np.percentile(wait, 99) * 1e3),
                value=st.particle_steps / span, ticks=st.ticks, max_running=max_running,
@@ -325,20 +349,42 @@ def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=24, rate=4.0)
     return out
 
 
-def cpu_baseline(cases, abi, dt, tol, max_seconds=30.0):
-    """The FP64 oracle (OpenMP) on a bounded sample of the same workload: a
-    0.4 x 0.32 x 0.5 m slab of the canyon at the same dx with one building."""
+def host_cpu():
+    """Model name and physical core count of this host (lscpu), and the
+    CPUs this process may run on."""
+    import subprocess
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict(l.split(":", 1) for l in txt.splitlines() if ":" in l)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        cps, sk = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sk and cps.isdigit() and sk.isdigit():
+            info["physical_cores"] = int(cps) * int(sk)
+        info["logical_cpus"] = int(kv["CPU(s)"]) if kv.get("CPU(s)", "").isdigit() else None
+    except (OSError, subprocess.SubprocessError):
+        pass
+    info["cpus_available_to_process"] = len(os.sched_getaffinity(0))
+    return info
+
+
+def cpu_baseline(cases, abi, P, nrm, geom, dt, tol, max_seconds=20.0):
+    """The FP64 oracle (OpenMP, one thread per available CPU) on the same
+    2.13 M-particle canyon: the searching first step untimed, then steps
+    while the bound allows (at least one)."""
     import oracle
     threads = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
-    P, nrm, geom = cases.canyon(dx=0.008, Lx=0.4, Ly=0.32, Lz=0.5, buildings=[(0.2, 0.16)])
     prm = cases.canyon_params(geom, oracle.default_params(), dt=dt, tol=tol)
     o = oracle.Oracle(prm, len(P))
     o.upload(P, None, nrm)
+    a = time.perf_counter()
     o.step(0, dt)                       # neighbour search + first solve (not timed)
+    t_first = time.perf_counter() - a
     times = []
     count = 1
     t_all = time.perf_counter()
-    while count <= 5 and time.perf_counter() - t_all < max_seconds:
+    while count == 1 or (count <= 3 and time.perf_counter() - t_all + times[-1] < max_seconds):
         a = time.perf_counter()
         o.step(count, dt)
         times.append(time.perf_counter() - a)
@@ -346,8 +392,9 @@ def cpu_baseline(cases, abi, dt, tol, max_seconds=30.0):
     o.close()
     med = float(np.median(times))
     return dict(value=len(P) / med, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"canyon slab 0.4x0.32x0.5 m, dx=0.008, one building, n={len(P)}, "
-                       f"FP64 BiCGSTAB tol {tol:g}, median of {len(times)} steps after 1 warm-up")
+                host=host_cpu(), seconds_per_step=[round(t, 2) for t in times], first_step_s=round(t_first, 2),
+                sample=f"the same urban canyon, n={len(P)}, FP64 BiCGSTAB (Jacobi) tol {tol:g}, "
+                       f"median of {len(times)} step(s) after the searching first step, {threads} OpenMP threads")
 
 
 def run_box_cfg0(cases, abi, urbansph, oracle_too=True, gpu_steps=20):
@@ -408,8 +455,8 @@ def main():
                     help="headline legs without HIP graphs (each Krylov kernel launched directly)")
     ap.add_argument("--no-apr", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=5)
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--only", choices=["fp64", "mixed", "mixed_jacobi"], default=None)
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--only", choices=["fp64", "mixed", "mixed_jacobi", "fp64_multigrid"], default=None)
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a 1 GiB device-to-device copy kernel (for scaling HBM counters)")
     args = ap.parse_args()
@@ -453,7 +500,7 @@ def main():
 
     res = {}
     legs = (("fp64", abi.FP64, abi.PRECOND_JACOBI), ("mixed", abi.MIXED, abi.PRECOND_MULTIGRID),
-            ("mixed_jacobi", abi.MIXED, abi.PRECOND_JACOBI))
+            ("mixed_jacobi", abi.MIXED, abi.PRECOND_JACOBI), ("fp64_multigrid", abi.FP64, abi.PRECOND_MULTIGRID))
     for name, prec, pc in legs:
         if args.only and args.only != name:
             continue
@@ -517,24 +564,47 @@ def main():
                 "ppe_iters_per_step": res["mixed_jacobi"]["iters"],
                 "over_fp64": round(res["mixed_jacobi"]["value"] / res["fp64"]["value"], 3),
                 "what": "MIXED with the reference's Jacobi preconditioner: the precision's share of the speed-up"}
+        if "fp64_multigrid" in res:
+            fm = res["fp64_multigrid"]
+            line["fp64_multigrid"] = {
+                "value": round(fm["value"], 1), "ms_per_step": round(fm["ms_per_step"], 3),
+                "ppe_iters_per_step": fm["iters"], "mg_levels": fm["mg_levels"],
+                "ppe_precond": "multigrid" if fm["precond"] == abi.PRECOND_MULTIGRID else "jacobi",
+                "what": "FP64 BiCGSTAB with the same multigrid preconditioner as the headline MIXED leg"}
+            if "mixed" in res:
+                line["fp64_multigrid"]["mixed_over_this"] = round(res["mixed"]["value"] / fm["value"], 3)
+                line["fp64_multigrid"]["note"] = ("mixed_over_this: the precision's share of the speed-up at the same "
+                                                  "preconditioner; fp64_multigrid over fp64: the preconditioner's share "
+                                                  "in FP64")
+                line["fp64_multigrid"]["over_fp64"] = round(fm["value"] / res["fp64"]["value"], 3)
         if "mixed" in res:
             line["mixed_over_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
 
             def rel(a, b):
                 return float(np.linalg.norm(a - b) / np.linalg.norm(b))
-            # the solver-tolerance floor: FP64 at this tolerance against FP64
-            # at a 1000x tighter one, one step from the deck
-            tight = run_first_step(cases, abi, urbansph, P, nrm, geom, args.dt, args.tol * 1e-3)
-            f64, mx = res["fp64"]["first"], res["mixed"]["first"]
+            # against FP64 converged to 1e-4 of the tolerance, one step from
+            # the deck; the float floor is MIXED at that tight tolerance
+            conv = run_first_step(cases, abi, urbansph, P, nrm, geom, args.dt, args.tol * 1e-4)
+            floor = run_first_step(cases, abi, urbansph, P, nrm, geom, args.dt, args.tol * 1e-4, precision=abi.MIXED)
+            pr = precision_report(res["fp64"]["first"], res["mixed"]["first"], conv, floor)
+            bound = (pr["mixed"]["pres"] <= 2 * pr["fp64"]["pres"] + 2 * pr["mixed_float_floor"]["pres"]
+                     and pr["mixed"]["u"] <= 2 * pr["fp64"]["u"] + 2 * pr["mixed_float_floor"]["u"]
+                     and pr["mixed_float_floor"]["pres"] <= 5e-6 and pr["mixed_float_floor"]["u"] <= 5e-6)
             line["mixed_vs_fp64_rel_l2"] = {
-                "one_step": {"pres": rel(mx[0], f64[0]), "ux": rel(mx[1], f64[1])},
-                "fp64_tolerance_floor_one_step": {"pres": rel(f64[0], tight[0]), "ux": rel(f64[1], tight[1])},
+                "vs_converged_fp64_one_step": pr,
+                "bound_met": bool(bound),
+                "mixed_within_1e-5_of_converged": bool(max(pr["mixed"].values()) <= 1e-5),
+                "mixed_vs_fp64_same_tol_one_step": {
+                    "pres": rel(res["mixed"]["first"]["PRES"], res["fp64"]["first"]["PRES"]),
+                    "ux": rel(res["mixed"]["first"]["UX"], res["fp64"]["first"]["UX"])},
                 "after_run": {"pres": rel(res["mixed"]["pres"], res["fp64"]["pres"]),
                               "ux": rel(res["mixed"]["ux"], res["fp64"]["ux"])},
-                "note": "one_step: the first step from the same deck in each precision (PPE tol as configured); "
-                        "fp64_tolerance_floor_one_step: FP64 at that tol vs FP64 at tol/1000, same step; "
-                        "after_run: the two runs after warmup+steps+2 steps, where solver-tolerance "
-                        "differences have been carried through the dynamics"}
+                "note": f"vs_converged_fp64_one_step: the first step from the deck, MIXED (multigrid) and FP64 (Jacobi) "
+                        f"at PPE tol {args.tol:g} and MIXED at {args.tol * 1e-4:g} (the float floor), each against "
+                        f"FP64 at {args.tol * 1e-4:g}; bound: MIXED <= 2 x FP64 at the same tol + 2 x float floor, "
+                        f"float floor <= 5e-6 (pressure and velocity vector); after_run: the two timed runs after "
+                        f"warmup+steps+2 steps, where tolerance differences have been carried through the dynamics"}
+            log(f"[bench] precision vs converged FP64: {pr}")
     if not args.no_batch:
         bt = run_batched(cases, abi, urbansph, args.batch_steps, 5, args.dt, args.tol)
         line["batched"] = {
@@ -575,15 +645,19 @@ def main():
                               "what": "serving: completion - arrival per canyon request"}
         line["serving"] = {
             "workload": f"{sv['requests']} canyon requests, Poisson arrivals {sv['rate_per_s']}/s, counts log-uniform "
-                        "in [1e4, 2e5], 10..40 steps, L in {-50, +50, inf} m, mixed precision with the multigrid preconditioner, 240 GB budget "
-                        "(sizes and step counts scaled down 10x from the 2e6-particle / 500-step request mix "
-                        "so that the offered load stays below one GPU's capacity)",
+                        f"in [{sv['lo']:g}, {sv['hi']:g}], {sv['steps_lo']}..{sv['steps_hi']} steps, "
+                        "L in {-50, +50, inf} m, mixed precision with the multigrid preconditioner, 240 GB budget, "
+                        "state resident, steps of concurrent requests co-batched",
             "value": round(sv["value"], 1), "unit": "particle-steps/s", "p50_ms": round(sv["p50_ms"], 1),
             "p99_ms": round(sv["p99_ms"], 1), "admit_wait_p99_ms": round(sv["admit_wait_p99_ms"], 1),
-            "ticks": sv["ticks"], "max_running": sv["max_running"], "failed": sv["failed"]}
+            "offered_particle_steps_per_s": round(sv["offered_particle_steps_per_s"], 1),
+            "span_s": round(sv["span_s"], 2), "latencies_s": sv["latencies_s"],
+            "ticks": sv["ticks"], "max_running": sv["max_running"], "failed": sv["failed"],
+            "note": "the offered load (particle-steps per second of arrivals) exceeds one GPU's throughput, so "
+                    "latency includes the time requests share the device with those admitted before them"}
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
-        line["cpu_baseline"] = cpu_baseline(cases, abi, args.dt, args.tol)
+        line["cpu_baseline"] = cpu_baseline(cases, abi, P, nrm, geom, args.dt, args.tol)
         log(f"[bench] cpu baseline {line['cpu_baseline']['value'] / 1e3:.1f} k particle-steps/s "
             f"on {line['cpu_baseline']['cores']} threads")
         log("[bench] 32^3 box (the reference's CPU case): GPU FP64 and the oracle on 1 and all threads ...")

@@ -184,7 +184,11 @@ typedef struct sph_params {
                                    with long rows) and after a multigrid
                                    inner solve that stalls, the solve uses
                                    SPH_PRECOND_JACOBI.  The report says which
-                                   ran.  FP64 ignores it.                     */
+                                   ran.  FP64 (BiCGSTAB) uses the same float
+                                   V-cycle, built from a float copy of its
+                                   operator, on float copies of p and s; its
+                                   Krylov vectors, operator and tolerance stay
+                                   double.                                    */
     int    ib_normals;          /* where the IB markers' (p_type >= 1000)
                                    normals come from, SPH_IBN_*.
                                    SPH_IBN_REFERENCE (default): every
@@ -210,7 +214,7 @@ typedef struct sph_params {
 /* Sources of the IB marker normals (ib_normals). */
 enum { SPH_IBN_REFERENCE = 0, SPH_IBN_PLANES = 1, SPH_IBN_UPLOADED = 2 };
 
-/* Preconditioners of the MIXED inner solve (ppe_precond). */
+/* Preconditioners of the BiCGSTAB solve (ppe_precond). */
 enum { SPH_PRECOND_JACOBI = 0, SPH_PRECOND_MULTIGRID = 1 };
 
 /* ---------------------------------------------------------------------------

@@ -17,7 +17,7 @@ import abi
 import cases
 import oracle
 import urbansph
-from test_gpu_parity import check_mixed
+from test_gpu_parity import check_mixed, mixed_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -158,3 +158,96 @@ def test_multigrid_profiled_build_and_ale():
     g.profile(False)
     for k in ("mg_rap_terms", "mg_rap_fill", "mg_rap_pack", "mg_gauss_jordan", "mg_res0", "mg_post0", "spmv"):
         assert k in prof and prof[k][0] > 0, k
+
+
+def _gpu64(P, nrm, prm, dt, steps=1, graphs=True):
+    prm.update(precision=abi.FP64, ppe_precond=abi.PRECOND_MULTIGRID)
+    g = urbansph.Case(prm, len(P))
+    g.options(graphs=graphs, autotune=True)
+    g.upload(P, None, nrm)
+    for c in range(steps):
+        g.step(c, dt)
+    g.sync()
+    r = g.ppe_report()
+    out = {f: g.field(f) for f in ("PRES", "UX", "UY", "UZ")}
+    g.close()
+    return out, r
+
+
+@pytest.mark.parametrize("which", ["box", "canyon"])
+def test_fp64_multigrid(which):
+    """FP64 BiCGSTAB preconditioned by the float V-cycle: the same system in
+    double, so at PPE tolerance tol it is no farther from the converged
+    oracle than the reference's Jacobi solve at tol (2x, and 1e-9 for the
+    rounding of double sums), and driven to 1e-10 it matches the converged
+    oracle to 1e-8; in fewer iterations than Jacobi."""
+    P, nrm, mk, dt = _case(which)
+    conv = _oracle(P, nrm, mk(1e-12), dt)
+    at_tol = _oracle(P, nrm, mk(1e-6), dt)
+    got, r = _gpu64(P, nrm, mk(1e-6), dt)
+    jac = urbansph.Case(mk(1e-6), len(P))
+    jac.upload(P, None, nrm)
+    jac.step(0, dt)
+    jac.sync()
+    rj = jac.ppe_report()
+    jac.close()
+    print(f"[fp64-mg] {which}: iterations {r.iterations} (jacobi {rj.iterations}), levels {r.mg_levels}, "
+          f"coarsest {r.mg_coarsest}, rel res {r.rel_residual:.2e}", flush=True)
+    assert r.precond == abi.PRECOND_MULTIGRID and r.mg_levels >= 2 and r.converged == 1
+    assert r.rel_residual <= 1e-6 and r.iterations < rj.iterations
+    e, e64 = mixed_errors(got, conv, at_tol)
+    for f in e:
+        print(f"[fp64-mg] {which} {f}: multigrid {e[f]:.2e}  jacobi {e64[f]:.2e}", flush=True)
+        assert e[f] <= 2.0 * e64[f] + 1e-9, f
+    tight, rt = _gpu64(P, nrm, mk(1e-10), dt)
+    assert rt.converged == 1 and rt.precond == abi.PRECOND_MULTIGRID
+    et, _ = mixed_errors(tight, conv, at_tol)
+    print(f"[fp64-mg] {which} at 1e-10: {et}", flush=True)
+    assert max(et.values()) <= 1e-8
+
+
+def test_fp64_multigrid_batch_equals_alone_and_graphs():
+    """Two FP64 multigrid cases and a Jacobi one in one batch end bitwise
+    where they end alone; graphs on and off give the same bits."""
+    specs = []
+    for k, nf in enumerate((7, 12, 14)):
+        P, nrm = cases.box(nf=nf, walls=3, seed_pos=30 + k, seed_vel=40 + k)
+        prm = cases.box_params(urbansph.default_params(), tol=1e-8)
+        prm.update(precision=abi.FP64, ppe_precond=abi.PRECOND_MULTIGRID, ppe_solver=abi.PPE_BICGSTAB)
+        specs.append((P, nrm, prm, 1e-4))
+
+    def alone(spec, graphs=True):
+        P, nrm, prm, dt = spec
+        g = urbansph.Case(prm, len(P))
+        g.options(graphs=graphs, autotune=True)
+        g.upload(P, None, nrm)
+        for c in range(3):
+            g.step(c, dt)
+        g.sync()
+        r = g.ppe_report()
+        out = {f: g.field(f) for f in ("PRES", "UX", "UY", "UZ")}, (r.iterations, r.precond)
+        g.close()
+        return out
+
+    solo = [alone(s) for s in specs]
+    assert [s[1][1] for s in solo] == [abi.PRECOND_JACOBI, abi.PRECOND_MULTIGRID, abi.PRECOND_MULTIGRID]
+    nog = alone(specs[2], graphs=False)
+    assert nog[1] == solo[2][1]
+    for f in nog[0]:
+        np.testing.assert_array_equal(nog[0][f], solo[2][0][f], err_msg=f"graphs {f}")
+    b = urbansph.Batch()
+    cs = []
+    for P, nrm, prm, dt in specs:
+        c = urbansph.Case(prm, len(P))
+        c.upload(P, None, nrm)
+        cs.append(c)
+    for st in range(3):
+        b.step(cs, [st] * 3, [1e-4] * 3)
+    b.sync()
+    for k, c in enumerate(cs):
+        r = c.ppe_report()
+        assert (r.iterations, r.precond) == solo[k][1], k
+        for f in solo[k][0]:
+            np.testing.assert_array_equal(c.field(f), solo[k][0][f], err_msg=f"case {k} {f}")
+        c.close()
+    b.close()

@@ -1285,8 +1285,6 @@ int pack_records(Eng& e, int which) {
     return 0;
 }
 
-// Everything of SOPHIA_single_ISPH before the pressure solve
-// (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
 // KERNEL_clc_IB_normal_vector3D, functions_PREP.cuh:923-1015 (count == 0,
 // ISPH_Calc.cuh:70-74): markers within ib_face_tol of a face plane get -1 / +1
 // on that axis (the upper plane wins a tie, as it is tested last), normalised
@@ -1321,6 +1319,8 @@ __global__ void __launch_bounds__(BLOCK) k_ib_normals(Seg sg) {
     }
 }
 
+// Everything of SOPHIA_single_ISPH before the pressure solve
+// (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
 int step_physics_front(Eng& e) {
     const bool mixed = mixed_of(e);
     bool any_count0 = false;

@@ -596,7 +596,22 @@ __global__ void __launch_bounds__(BLOCK) k_kr_p(Seg sg) {
     if (sc[SC_IT] == 1.0) pv = K.r[row];
     else pv = (X)((double)K.r[row] + sc[SC_BETA] * ((double)K.p[row] - sc[SC_OMEGA] * (double)K.v[row]));
     K.p[row] = pv;
-    K.phat[row] = (X)((double)pv / (double)kdiag(pp, (X*)nullptr)[row]);
+    if (sizeof(X) == 8 && cd.ctl[CTL_MG] != 0.0) pp.fp[row] = (float)pv;   // the V-cycle's input
+    else K.phat[row] = (X)((double)pv / (double)kdiag(pp, (X*)nullptr)[row]);
+}
+
+// FP64 solve with the multigrid: p-hat (which = 0) or s-hat (1) from the
+// float V-cycle's output
+template <class X>
+__global__ void __launch_bounds__(BLOCK) k_kr_widen(Seg sg, int which) {
+    SEG_ROW(sg);
+    GATE(cd);
+    if (cd.ctl[CTL_MG] == 0.0) return;
+    const PPEView pp = cd.ppe;
+    if (row >= pp.nr) return;
+    const KSet<X> K = kset(pp, (X*)nullptr);
+    if (which) K.shat[row] = (X)pp.fshat[row];
+    else K.phat[row] = (X)pp.fphat[row];
 }
 
 // TWO = false: v = A phat, partial rhat.v (A)
@@ -646,7 +661,8 @@ __global__ void __launch_bounds__(BLOCK) k_kr_s(Seg sg) {
     if (row < pp.nr) {
         X sv = (X)((double)K.r[row] - cd.sc[SC_ALPHA] * (double)K.v[row]);
         K.s[row] = sv;
-        K.shat[row] = (X)((double)sv / (double)kdiag(pp, (X*)nullptr)[row]);
+        if (sizeof(X) == 8 && cd.ctl[CTL_MG] != 0.0) pp.fs[row] = (float)sv;
+        else K.shat[row] = (X)((double)sv / (double)kdiag(pp, (X*)nullptr)[row]);
         a = (double)sv * (double)sv;
     }
     double ta = block_canon(a, sh);
@@ -838,6 +854,12 @@ __global__ void __launch_bounds__(BLOCK) k_min_h(int nr, const int* rows, const 
     }
     if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
+// the float copy of an FP64 operator's stored values (multigrid set-up)
+__global__ void __launch_bounds__(BLOCK) k_vals_narrow(long long n, const double* a, float* b) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
 static int finest_spacing(sph_ctx* c, const PPEState* P, double* q) {
     const int nb = (int)grid1(P->nr);
     double* part = nullptr;
@@ -1064,10 +1086,14 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
     }
     SPH_TRY(ppe_nnz(c, P));
     if (P->ecols) { dfree(c, P->cols); P->cols = nullptr; }   // the 16-bit copy replaces them
-    if (mixed) SPH_TRY(dalloc(c, &P->vals32, P->nnz_slots + 1));
-    else SPH_TRY(dalloc(c, &P->vals64, P->nnz_slots + 1));
+    // FP64 with the multigrid: a float copy of the operator builds the
+    // hierarchy, and float copies of p and s feed the V-cycle
+    const bool fp64_mg = !mixed && c->prm.ppe_precond == SPH_PRECOND_MULTIGRID &&
+                         c->prm.ppe_solver == SPH_PPE_BICGSTAB && nr > MG_COARSE_MAX;
+    if (mixed || fp64_mg) SPH_TRY(dalloc(c, &P->vals32, P->nnz_slots + 1));
+    if (!mixed) SPH_TRY(dalloc(c, &P->vals64, P->nnz_slots + 1));
     SPH_TRY(dalloc(c, &P->diag, nr));
-    if (mixed) SPH_TRY(dalloc(c, &P->fdiag, nr));
+    if (mixed || fp64_mg) SPH_TRY(dalloc(c, &P->fdiag, nr));
     SPH_TRY(dalloc(c, &P->rhs, nr));
     SPH_TRY(dalloc(c, &P->x, nr));
     SPH_TRY(dalloc(c, &P->pnew, nr));
@@ -1081,6 +1107,10 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
         } else {
             double** dv[] = {&P->r, &P->rhat, &P->p, &P->phat, &P->v, &P->s, &P->shat, &P->t};
             for (auto q : dv) SPH_TRY(dalloc(c, q, nr));
+            if (fp64_mg) {
+                float** fv[] = {&P->fp, &P->fphat, &P->fs, &P->fshat};
+                for (auto q : fv) SPH_TRY(dalloc(c, q, nr));
+            }
         }
     }
     return 0;
@@ -1188,9 +1218,11 @@ static int kr_iteration(Eng& e, double spmv_bytes, bool mg) {
         KF(op);                                   \
         if (pf) SPH_TRY(read_scalars(e));         \
     } while (0)
+    constexpr bool wide = sizeof(X) == 8;   // FP64 vectors: the V-cycle works on float copies
     if (live(false)) {
         RL("kr_p", 0, k_kr_p<X>);
         if (mg) SPH_TRY(mg_apply(e, 0, SC_DONE));
+        if (mg && wide) RL("kr_widen", 0, k_kr_widen<X>, 0);
         RLONG(V, X, 1);
         RL("spmv", spmv_bytes, (k_kr_spmv<V, X, false>));
     }
@@ -1199,6 +1231,7 @@ static int kr_iteration(Eng& e, double spmv_bytes, bool mg) {
     KFP(FIN_S);
     if (live(false)) {
         if (mg) SPH_TRY(mg_apply(e, 1, SC_DONE));
+        if (mg && wide) RL("kr_widen", 0, k_kr_widen<X>, 1);
         RLONG(V, X, 2);
         RL("spmv", spmv_bytes, (k_kr_spmv<V, X, true>));
     }
@@ -1358,61 +1391,97 @@ int ppe_solve(Eng& e) {
         return 0;
     }
     std::vector<int> iters;
+    // the multigrid preconditioner of every case that asks for it; a
+    // hierarchy is built once per set of values (with fixed particles, at
+    // the first step of a search).  FP64 builds it from a float copy of its
+    // operator.
+    std::vector<const MGState*> mgs(nc, nullptr);
+    bool any_mg = false;
+    for (int k = 0; k < nc; k++) {
+        sph_ctx* c = e.cs[k];
+        PPEState* P = c->ppe;
+        if (c->prm.ppe_precond != SPH_PRECOND_MULTIGRID || !P) continue;
+        const bool fresh = !c->prm.ale_positions && c->vals_gen == c->generation;
+        if (!fresh || P->mg_state == 0) {
+            mg_free(c, P->mg);
+            P->mg = nullptr;
+            P->mg_state = -1;
+            P->mg_why = SPH_MG_UNSUITED;
+            if (P->nr > MG_COARSE_MAX && (mixed || P->fp)) {
+                if (!mixed) {
+                    hipLaunchKernelGGL(k_vals_narrow, dim3(grid1(P->nnz_slots + 1)), dim3(BLOCK), 0, e.st,
+                                       P->nnz_slots + 1, (const double*)P->vals64, P->vals32);
+                    SPH_TRY(launch_check(&e, "vals_narrow"));
+                }
+                double q = 0.0;
+                int ok = 0;
+                SPH_TRY(finest_spacing(c, P, &q));
+                SPH_TRY(mg_build(c, ppe_view(c), P->nnz_slots, q, &P->mg, &ok, &P->mg_why));
+                P->mg_state = ok ? 1 : -1;
+            }
+        }
+        if (P->mg_state == 1) { mgs[k] = P->mg; any_mg = true; }
+        c->rep.mg_status = P->mg_state == -1 ? P->mg_why : SPH_MG_OK;
+    }
+    auto mg_tables = [&]() -> int {
+        bool changed = false;
+        SPH_TRY(mg_prepare(e, mgs, SC_DONE, &changed));
+        for (int gi : {2, 3})
+            if (changed && e.iter_graph[gi]) {   // its launches hold the old tables
+                (void)hipGraphExecDestroy(e.iter_graph[gi]);
+                e.iter_graph[gi] = nullptr;
+            }
+        return 0;
+    };
+    if (any_mg) SPH_TRY(mg_tables());
     if (!mixed) {
-        SPH_TRY(ctl_wait(e));
-        for (int k = 0; k < nc; k++) {
-            double* ct = e.h_ctl + k * CTL_COUNT;
-            ct[CTL_TOL] = e.cs[k]->prm.ppe_tol;
-            ct[CTL_MAXIT] = e.cs[k]->prm.ppe_maxit;
-            ct[CTL_SKIP] = nr_of(e.cs[k]) == 0 ? 1.0 : 0.0;
-        }
-        SPH_TRY(ctl_upload(e));
-        SPH_TRY((bicgstab<double, double>(e, 0, 0, iters)));
-        for (int k = 0; k < nc; k++) {
-            sph_ctx* c = e.cs[k];
-            if (c->rep.unknowns == 0) continue;
-            const double* sc = e.h_sc + k * SC_COUNT;
-            c->rep.iterations = iters[k];
-            c->rep.converged = sc[SC_CONV] != 0.0;
-            c->rep.rel_residual = rel_residual(sc[SC_RNORM], sc[SC_BNORM]);
-        }
-    } else {
-        // the multigrid preconditioner of every case that asks for it; a
-        // hierarchy is built once per set of values (with fixed particles, at
-        // the first step of a search)
-        std::vector<const MGState*> mgs(nc, nullptr);
-        bool any_mg = false;
-        for (int k = 0; k < nc; k++) {
-            sph_ctx* c = e.cs[k];
-            PPEState* P = c->ppe;
-            if (c->prm.ppe_precond != SPH_PRECOND_MULTIGRID || !P) continue;
-            const bool fresh = !c->prm.ale_positions && c->vals_gen == c->generation;
-            if (!fresh || P->mg_state == 0) {
-                mg_free(c, P->mg);
-                P->mg = nullptr;
-                P->mg_state = -1;
-                P->mg_why = SPH_MG_UNSUITED;
-                if (P->nr > MG_COARSE_MAX) {
-                    double q = 0.0;
-                    int ok = 0;
-                    SPH_TRY(finest_spacing(c, P, &q));
-                    SPH_TRY(mg_build(c, ppe_view(c), P->nnz_slots, q, &P->mg, &ok, &P->mg_why));
-                    P->mg_state = ok ? 1 : -1;
+        // one solve, and for a multigrid case whose solve stalled with
+        // finite residuals a second one with the diagonal scaling from where
+        // it stopped
+        std::vector<char> todo(nc);
+        for (int k = 0; k < nc; k++) todo[k] = nr_of(e.cs[k]) > 0;
+        for (int pass = 0; pass < 2; pass++) {
+            SPH_TRY(ctl_wait(e));
+            bool any = false;
+            for (int k = 0; k < nc; k++) {
+                double* ct = e.h_ctl + k * CTL_COUNT;
+                ct[CTL_TOL] = e.cs[k]->prm.ppe_tol;
+                ct[CTL_MAXIT] = e.cs[k]->prm.ppe_maxit;
+                ct[CTL_SKIP] = todo[k] ? 0.0 : 1.0;
+                ct[CTL_MG] = mgs[k] ? 1.0 : 0.0;
+                any |= todo[k] != 0;
+            }
+            SPH_TRY(ctl_upload(e));
+            if (!any) break;
+            SPH_TRY((bicgstab<double, double>(e, any_mg ? 3 : 0, pass, iters, any_mg)));
+            bool stalled = false;
+            for (int k = 0; k < nc; k++) {
+                if (!todo[k]) continue;
+                sph_ctx* c = e.cs[k];
+                const double* sc = e.h_sc + k * SC_COUNT;
+                c->rep.iterations += iters[k];
+                c->rep.converged = sc[SC_CONV] != 0.0;
+                c->rep.rel_residual = rel_residual(sc[SC_RNORM], sc[SC_BNORM]);
+                todo[k] = 0;
+                if (!mgs[k]) continue;
+                c->rep.precond = SPH_PRECOND_MULTIGRID;
+                c->rep.mg_levels = mg_levels(mgs[k]);
+                c->rep.mg_coarsest = mg_coarsest(mgs[k]);
+                if (sc[SC_CONV] == 0.0 && std::isfinite(sc[SC_RNORM])) {
+                    c->ppe->mg_state = -1;
+                    c->ppe->mg_why = SPH_MG_STALLED_SOLVE;
+                    c->rep.mg_status = SPH_MG_STALLED_SOLVE;
+                    mgs[k] = nullptr;
+                    todo[k] = 1;
+                    stalled = true;
                 }
             }
-            if (P->mg_state == 1) { mgs[k] = P->mg; any_mg = true; }
-            c->rep.mg_status = P->mg_state == -1 ? P->mg_why : SPH_MG_OK;
+            if (!stalled) break;
+            any_mg = false;
+            for (const MGState* m : mgs) any_mg |= m != nullptr;
+            if (any_mg) SPH_TRY(mg_tables());
         }
-        auto mg_tables = [&]() -> int {
-            bool changed = false;
-            SPH_TRY(mg_prepare(e, mgs, SC_DONE, &changed));
-            if (changed && e.iter_graph[2]) {   // its launches hold the old tables
-                (void)hipGraphExecDestroy(e.iter_graph[2]);
-                e.iter_graph[2] = nullptr;
-            }
-            return 0;
-        };
-        if (any_mg) SPH_TRY(mg_tables());
+    } else {
         // outer FP64 iterative refinement around float inner solves; each
         // case refines until its own tolerance is met
         std::vector<char> active(nc);
@@ -1425,7 +1494,10 @@ int ppe_solve(Eng& e) {
             for (int k = 0; k < nc; k++) any |= active[k] != 0;
             if (!any) break;
             SPH_TRY(ctl_wait(e));
-            for (int k = 0; k < nc; k++) e.h_ctl[k * CTL_COUNT + CTL_SKIP] = active[k] ? 0.0 : 1.0;
+            for (int k = 0; k < nc; k++) {
+                e.h_ctl[k * CTL_COUNT + CTL_SKIP] = active[k] ? 0.0 : 1.0;
+                e.h_ctl[k * CTL_COUNT + CTL_MG] = 0.0;   // float vectors: the V-cycle reads them directly
+            }
             SPH_TRY(ctl_upload(e));
             RLONG(float, float, 3);
             RL("outer_res", res_bytes, k_outer_res);

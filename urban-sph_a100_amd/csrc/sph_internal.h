@@ -136,7 +136,9 @@ struct CaseDesc {
     double* xnew[3];     // ALE positions of the corrector (null unless ale_positions)
     double* rec;         // neighbour records (REC_DOUBLES per particle), refreshed before each gathering pass
 };
-enum Ctl { CTL_TOL, CTL_MAXIT, CTL_SKIP, CTL_COUNT = 4 };
+// CTL_MG: an FP64 solve whose p-hat and s-hat come from the float V-cycle
+// (the double p and s are staged as float for it and widened after)
+enum Ctl { CTL_TOL, CTL_MAXIT, CTL_SKIP, CTL_MG, CTL_COUNT = 4 };
 constexpr int SCAL_COUNT = 32;   // Krylov scalars per case
 
 // A segmented launch: block b works on case tab[b].x from index tab[b].y on
@@ -250,11 +252,13 @@ struct Eng {
     int tune_class = 0;
     // graph of one BiCGSTAB iteration per vector precision, valid for comp
     bool use_graphs = true;
-    hipGraphExec_t iter_graph[3] = {nullptr, nullptr, nullptr};   // FP64, float, float with multigrid
+    // FP64, float, float with multigrid, FP64 with multigrid
+    static constexpr int NGRAPH = 4;
+    hipGraphExec_t iter_graph[NGRAPH] = {nullptr, nullptr, nullptr, nullptr};
     // Krylov iterations of the last solves (per vector precision and
     // refinement pass): the host polls for convergence only near them
     static constexpr int PRED_SLOTS = 4;
-    int pred_iters[3][PRED_SLOTS] = {};
+    int pred_iters[NGRAPH][PRED_SLOTS] = {};
     // multigrid V-cycles of the step's cases (mg.hip): per case and level a
     // descriptor, and block tables of the level rows (mg_plan)
     void* mg_lv = nullptr;            // device MGLv[case * MG_MAX_LEVELS + level]
