@@ -138,6 +138,36 @@ def measured_traffic(precision):
     return int(m.group(1)) if m else None
 
 
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+GATHER_PASSES = ("ppe_assemble", "advection", "prep", "pressure_force", "neumann")
+
+
+def interaction_rooflines(prof):
+    """The gather passes against their bounds: FLOP per launch and memory-
+    fabric bytes per launch (FETCH_SIZE x2 + WRITE_SIZE) from the committed
+    rocprofv3 summary, divided by this run's average launch time (HIP events
+    on the engine's stream)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "interaction_canyon2m.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    out = {}
+    for k in GATHER_PASSES:
+        m = re.search(r"interaction_%s: fetch_bytes (\d+) write_bytes (\d+) flop (\d+)" % k, text)
+        if not m or k not in prof:
+            continue
+        launches, ms, _ = prof[k]
+        avg_s = ms / launches / 1e3
+        fab, flop = int(m.group(1)) + int(m.group(2)), int(m.group(3))
+        out[k] = {"avg_us": round(avg_s * 1e6, 1), "flop_per_launch": flop,
+                  "tflops": round(flop / avg_s / 1e12, 3),
+                  "frac_fp32_vector_peak": round(flop / avg_s / 1e12 / FP32_VECTOR_PEAK_TFLOPS, 4),
+                  "fabric_bytes_per_launch": fab, "fabric_GBps": round(fab / avg_s / 1e9, 1),
+                  "frac_hbm_peak": round(fab / avg_s / 1e9 / HBM_PEAK_GBS, 3)}
+    return out or None
+
+
 def run_batched(cases, abi, urbansph, steps, warmup, dt, tol, n_cases=64):
     """The 64-case ensemble (counts log-uniform in [2e4, 3e5], inflow scale
     U(0.5, 2), building height U(0.05, 0.2), seed 7) stepped as one batch,
@@ -476,8 +506,20 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         sec = e0.elapsed_time(e1) / 1e3 / 10
+        # read-only rate: a sum over 4 GiB (reads dominate, one float written)
+        big = torch.ones(1 << 30, dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            big.sum()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(10):
+            big.sum()
+        e1.record()
+        torch.cuda.synchronize()
+        rsec = e0.elapsed_time(e1) / 1e3 / 10
         print(json.dumps({"copy_calib_bytes_read": a.numel() * 4, "copy_calib_bytes_written": a.numel() * 4,
-                          "copy_GBps_read_plus_write": round(2 * a.numel() * 4 / sec / 1e9, 1)}))
+                          "copy_GBps_read_plus_write": round(2 * a.numel() * 4 / sec / 1e9, 1),
+                          "read_only_GBps": round(big.numel() * 4 / rsec / 1e9, 1)}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
@@ -553,6 +595,7 @@ def main():
         "ppe_iters_per_step": m["iters"],
         "kernel_ms_per_step": breakdown,
         "roofline": roof,
+        "interaction_rooflines": interaction_rooflines(m["prof"]),
     }
     if "fp64" in res:
         line["fp64"] = {"value": round(res["fp64"]["value"], 1), "ms_per_step": round(res["fp64"]["ms_per_step"], 3),
