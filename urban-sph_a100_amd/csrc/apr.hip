@@ -135,6 +135,7 @@ int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out) {
                            (const int*)rank, (const int*)pick, c->n, F, c->prm.apr_h_change);
         if ((rc = launch_check(c, "apr_split"))) break;
         c->n += 8 * nsplit;
+        c->pos_f32 = false;   // children sit at computed offsets, not at uploaded float positions
         dfree(c, c->porder);   // the pass order covers the old count; the next list rebuilds it
         c->porder = nullptr;
         ppe_free(c);

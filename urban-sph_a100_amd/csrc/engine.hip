@@ -101,7 +101,7 @@ int tune_begin(Eng& e, int kid, int block) {
     e.tune_open = false;
     if (!e.autotune) return 0;
     Eng::Tune& t = tune_for(e, e.tune_class);
-    if (t.phase >= TUNE_CANDIDATES || TUNE_BLOCKS[t.phase] != block) return 0;
+    if (e.tune_skip || t.phase >= TUNE_CANDIDATES || TUNE_BLOCKS[t.phase] != block) return 0;
     Eng::TuneEv ev{e.tune_class, kid, t.phase, nullptr, nullptr};
     SPH_HIP(&e, hipEventCreate(&ev.a));
     SPH_HIP(&e, hipEventCreate(&ev.b));
@@ -372,6 +372,9 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.ppe = ppe_view(c);
     for (int f = 0; f < 3; f++) d.xnew[f] = c->xnew[f];
     d.rec = c->rec;
+    const bool compact = c->prm.precision == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions;
+    d.crec = compact ? c->crec : nullptr;
+    d.crec_adv = compact ? c->crec_adv : nullptr;
     d.sc = e.d_sc + (size_t)k * SCAL_COUNT;
     d.ctl = e.d_ctl + (size_t)k * CTL_COUNT;
 }
@@ -384,6 +387,9 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
         if (c->prm.ale_positions && !c->xnew[0])
             for (int f = 0; f < 3; f++) SPH_TRY(dalloc(c, &c->xnew[f], c->cap));
         if (!c->rec) SPH_TRY(dalloc(c, &c->rec, (size_t)c->cap * REC_DOUBLES));
+        if (!c->crec && c->prm.precision == SPH_MIXED) SPH_TRY(dalloc(c, &c->crec, (size_t)c->cap * CREC_FLOATS));
+        if (!c->crec_adv && c->prm.precision == SPH_MIXED)
+            SPH_TRY(dalloc(c, &c->crec_adv, (size_t)c->cap * CREC_ADV_FLOATS));
         if (count[k] == 0 || c->prm.ale_positions) {   // moving particles: a search every step
             SPH_TRY(set_ncell_run(c));
             SPH_TRY(nnps_run(c));
@@ -417,6 +423,11 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
     int cls = 0;
     while ((1LL << (cls + 1)) <= total) cls++;
     e.tune_class = cls;
+    // the first step of a search writes the gradient correction and the PPE
+    // values and is several times the later steps' work: the candidates are
+    // timed on the steps after it
+    e.tune_skip = false;
+    for (int k = 0; k < nc; k++) e.tune_skip |= count[k] == 0;
     // descriptors
     SPH_HIP(&e, hipEventSynchronize(e.up_ev));
     for (int k = 0; k < nc; k++) fill_desc(e, k, cases[k], count[k], dt[k], time[k]);

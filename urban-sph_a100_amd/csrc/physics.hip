@@ -177,6 +177,25 @@ template <int WHICH>
 __global__ void __launch_bounds__(BLOCK) k_pack_records(Seg sg) {
     SEG_PARTICLE(sg);
     const PView S = cd.S;
+    if (cd.crec) {   // the gathering passes of this case read the compact records
+        if (WHICH == REC_PQ) {
+            float4* c = (float4*)(cd.crec + (size_t)i * CREC_FLOATS);
+            c[0] = make_float4((float)S.x[i], (float)S.y[i], (float)S.z[i], (float)S.m[i]);
+            c[1] = make_float4((float)S.rho[i], (float)(S.pres[i] - S.pres0[i]), (float)S.h[i],
+                               __int_as_float(S.p_type[i]));
+            return;
+        }
+        float4* c = (float4*)(cd.crec_adv + (size_t)i * CREC_ADV_FLOATS);
+        double2* d = (double2*)(c + 2);
+        d[0] = make_double2(S.ux[i], S.uy[i]);
+        d[1] = make_double2(S.uz[i], S.pres[i] - S.pres0[i]);
+        if (WHICH == REC_VEL_PQ) return;
+        const int t = (S.p_type[i] & 0xffff) | (S.i_type[i] << 16);
+        c[0] = make_float4((float)S.x[i], (float)S.y[i], (float)S.z[i], (float)S.m[i]);
+        c[1] = make_float4((float)S.rho[i], (float)S.h[i],
+                           (float)(viscosity2(cd.pr, S.p_type[i]) + S.vis_t[i] * S.rho[i]), __int_as_float(t));
+        return;
+    }
     double2* r = (double2*)(cd.rec + (size_t)i * REC_DOUBLES);
     r[2] = make_double2(S.rho[i], S.pres[i] - S.pres0[i]);
     if (WHICH == REC_PQ) return;
@@ -670,17 +689,34 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
     double eulerx = 0, eulery = 0, eulerz = 0, eulert = 0, tmp_temp = 0, tmp_flt = 0;
     const R Alpha = (R)pr.av_alpha, Beta = (R)pr.av_beta, soundspeed = (R)pr.av_c;
     const R ome = (R)(1 - eli);
+    const float* const crec = sizeof(R) == 4 ? cd.crec_adv : nullptr;
+    const bool compact = crec != nullptr;
     NLRow row = nl_row(nl, i);
     for (int k = 0; k < row.cnt; k++) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
-        double hj_d, vis_tj, pqj_, xj, yj, zj, mj_d, uxj, uyj, uzj, rhoj_d;
-        int ptypej, itj_;
-        rec_aux(cd.rec, j, vis_tj, ptypej, itj_);
-        if (ptypej >= 1000) continue;
-        rec_pos(cd.rec, j, xj, yj, zj, mj_d, rhoj_d, pqj_);
-        rec_vel(cd.rec, j, uxj, uyj, uzj, hj_d);
+        double hj_d, visj_d, xj, yj, zj, mj_d, uxj, uyj, uzj, rhoj_d;
+        int ptypej;
+        if (compact) {
+            const float* C = crec + (size_t)j * CREC_ADV_FLOATS;
+            const rec_f4 b = ld_f4(C + 4);
+            ptypej = crec_ptype(b.w);
+            if (ptypej >= 1000) continue;
+            const rec_f4 a = ld_f4(C);
+            const rec_d2 u = ld_d2(C + 8), w = ld_d2(C + 12);
+            xj = a.x; yj = a.y; zj = a.z; mj_d = a.w;
+            rhoj_d = b.x; hj_d = b.y; visj_d = b.z;
+            uxj = u.x; uyj = u.y; uzj = w.x;
+        } else {
+            double vis_tj, pqj_;
+            int itj_;
+            rec_aux(cd.rec, j, vis_tj, ptypej, itj_);
+            if (ptypej >= 1000) continue;
+            rec_pos(cd.rec, j, xj, yj, zj, mj_d, rhoj_d, pqj_);
+            rec_vel(cd.rec, j, uxj, uyj, uzj, hj_d);
+            visj_d = viscosity2(pr, ptypej) + vis_tj * rhoj_d;
+        }
         R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
         R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
@@ -691,9 +727,12 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
         R mj = (R)mj_d, rhoj = (R)rhoj_d;
         R dux = (R)(uxi - uxj), duy = (R)(uyi - uyj), duz = (R)(uzi - uzj);   // u_i - u_j
         double tempj = 0.0, concnj_d = 0.0;
-        if (pr.con_solve || pr.concn_solve || pr.boussinesq_solve) rec_scalar(cd.rec, j, tempj, concnj_d);
+        if (pr.con_solve || pr.concn_solve || pr.boussinesq_solve) {
+            if (compact) { tempj = S.temp[j]; concnj_d = S.concn[j]; }
+            else rec_scalar(cd.rec, j, tempj, concnj_d);
+        }
         if (pr.fv_solve == SPH_FV_CLEARY) {
-            R visj = (R)(viscosity2(pr, ptypej) + vis_tj * rhoj_d);
+            R visj = (R)visj_d;
             R C_v = dx * tdwx + dy * tdwy + dz * tdwz;
             C_v *= (visi * visj) / (visi + visj);
             C_v *= (R)4 * (mj / (rhoi * rhoj));
@@ -702,7 +741,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             tmpx += (double)(C_v * dux); tmpy += (double)(C_v * duy); tmpz += (double)(C_v * duz);
         } else if (pr.fv_solve == SPH_FV_MONAGHAN) {
             R uij_xij = dux * dx + duy * dy + duz * dz;
-            R visj = (R)(viscosity2(pr, ptypej) + vis_tj * rhoj_d);
+            R visj = (R)visj_d;
             R phi_ij = uij_xij;
             phi_ij /= (tdist * tdist + (R)0.01 * hi * hi);
             R P_ij = (R)(-2.0) * (R)(3 + 2.0) * phi_ij * mj / rhoj;
@@ -960,6 +999,34 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
     R pi = (R)pi_d, rhoi = (R)S.rho[i];
     double tmpx = 0, tmpy = 0, tmpz = 0;
     NLRow row = nl_row(nl, i);
+    if (sizeof(R) == 4 && cd.crec) {
+        // the same sums from the compact float records
+        const float* C = cd.crec;
+        for (int k = 0; k < row.cnt; k++) {
+            unsigned e = NB_AT(row, k);
+            if (!(e & F_DM)) continue;
+            int j = e & NB_IDX;
+            const rec_f4 b = ld_f4(C + (size_t)j * CREC_FLOATS + 4);
+            if (__float_as_int(b.w) >= 1000) continue;
+            const rec_f4 a = ld_f4(C + (size_t)j * CREC_FLOATS);
+            R dx = (R)(xi - (double)a.x), dy = (R)(yi - (double)a.y), dz = (R)(zi - (double)a.z);
+            R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+            R hij = (R)(0.5 * (hi_d + (double)b.z));
+            R tmp_Aij = calc_tmpA<R>(kc, hij);
+            R tdwij = kernel_dw<R>(kc, tmp_Aij, hij, tdist);
+            R tdwx = tdwij * dx / tdist;
+            R tdwy = tdwij * dy / tdist;
+            R tdwz = tdwij * dz / tdist;
+            R mj = (R)a.w, rhoj = (R)b.x, pj = (R)b.y;
+            if (pr.fp_solve) {
+                const R Pb = (R)0.0;
+                R C_p = (R)(-2.0) * mj / (rhoi * rhoj) * ((pi + Pb) * rhoj + (pj + Pb) * rhoi) / (rhoi + rhoj);
+                tmpx += (double)(C_p * tdwx); tmpy += (double)(C_p * tdwy); tmpz += (double)(C_p * tdwz);
+            }
+        }
+        Q.fpx[i] = tmpx; Q.fpy[i] = tmpy; Q.fpz[i] = tmpz;
+        return;
+    }
     for (int k = 0; k < row.cnt; k++) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_DM)) continue;
@@ -967,7 +1034,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
         if (S.p_type[j] >= 1000) continue;
         double hj, pqj, xj, yj, zj, mj_d, rhoj_d;
         rec_pos(cd.rec, j, xj, yj, zj, mj_d, rhoj_d, pqj);
-        hj = rec_at(cd.rec, j)[4].y;
+        hj = ld_d2(rec_at(cd.rec, j, 4)).y;
         R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
         R hij = (R)(0.5 * (hi_d + hj));

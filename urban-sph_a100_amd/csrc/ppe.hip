@@ -449,17 +449,30 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     V* out = (jacobi || cd.vals_fresh) ? nullptr : wvals_of(pp, (V*)nullptr) + prow(pp.sl_off, r);
     const unsigned char* smap = pp.slotmap ? pp.slotmap + prow(pp.sl_off, r) : nullptr;
     int slot = 0;
+    // read once: the stores of the loop could alias the descriptor
+    const float* const crec = sizeof(R) == 4 ? cd.crec_adv : nullptr;
+    const double* const rec = cd.rec;
     for (int k = 0; k < m; k++) {
         unsigned e = nrow[(size_t)k * SLICE];
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
-        double vtj_;
-        int ptj, itj;
-        rec_aux(cd.rec, j, vtj_, ptj, itj);
-        if (ptj >= 1000 || itj == 4) continue;
-        double xj, yj, zj, mj_d, rhoj_d, qj, uxj, uyj, uzj, hj_;
-        rec_pos(cd.rec, j, xj, yj, zj, mj_d, rhoj_d, qj);
-        rec_vel(cd.rec, j, uxj, uyj, uzj, hj_);
+        double xj, yj, zj, mj_d, rhoj_d, qj, uxj, uyj, uzj;
+        if (crec) {   // the compact record (positions float-exact)
+            const float* C = crec + (size_t)j * CREC_ADV_FLOATS;
+            const rec_f4 b = ld_f4(C + 4);
+            if (crec_ptype(b.w) >= 1000 || crec_itype(b.w) == 4) continue;
+            const rec_f4 a = ld_f4(C);
+            const rec_d2 u = ld_d2(C + 8), w = ld_d2(C + 12);
+            xj = a.x; yj = a.y; zj = a.z; mj_d = a.w; rhoj_d = b.x;
+            uxj = u.x; uyj = u.y; uzj = w.x; qj = w.y;
+        } else {
+            double vtj_, hj_;
+            int ptj, itj;
+            rec_aux(rec, j, vtj_, ptj, itj);
+            if (ptj >= 1000 || itj == 4) continue;
+            rec_pos(rec, j, xj, yj, zj, mj_d, rhoj_d, qj);
+            rec_vel(rec, j, uxj, uyj, uzj, hj_);
+        }
         R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
         R rhoj = (R)rhoj_d, mj = (R)mj_d;
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;

@@ -135,6 +135,9 @@ struct CaseDesc {
     const double* ctl;   // solve controls written by the host: tol, maxit, skip
     double* xnew[3];     // ALE positions of the corrector (null unless ale_positions)
     double* rec;         // neighbour records (REC_DOUBLES per particle), refreshed before each gathering pass
+    float* crec;         // MIXED with positions still the uploaded floats: compact records (CREC_FLOATS per
+                         // particle) of the pressure force, or null
+    float* crec_adv;     // the same case's compact records of advection and assembly (CREC_ADV_FLOATS), or null
 };
 // CTL_MG: an FP64 solve whose p-hat and s-hat come from the float V-cycle
 // (the double p and s are staged as float for it and widened after)
@@ -186,25 +189,34 @@ constexpr int MAX_BLOCK = 512;
 // before the pass (k_pack_records), so every pass computes with exactly the
 // values it read from the field arrays before.
 constexpr int REC_DOUBLES = 16;
-__device__ __forceinline__ const double2* rec_at(const double* rec, int j) {
-    return (const double2*)(rec + (size_t)j * REC_DOUBLES);
+// 16-byte loads of record pieces: explicit vector types in the global
+// address space, so each piece is one global_load_dwordx4 (a double2 read
+// through a generic pointer becomes two 8-byte flat loads)
+typedef double rec_d2 __attribute__((ext_vector_type(2)));
+typedef float rec_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ rec_d2 ld_d2(const void* p) {
+    return *(const __attribute__((address_space(1))) rec_d2*)p;
+}
+__device__ __forceinline__ rec_f4 ld_f4(const void* p) {
+    return *(const __attribute__((address_space(1))) rec_f4*)p;
+}
+__device__ __forceinline__ const double* rec_at(const double* rec, int j, int piece) {
+    return rec + (size_t)j * REC_DOUBLES + 2 * piece;
 }
 // x y z m rho pq
 __device__ __forceinline__ void rec_pos(const double* rec, int j, double& x, double& y, double& z, double& m,
                                         double& rho, double& pq) {
-    const double2* r = rec_at(rec, j);
-    const double2 a = r[0], b = r[1], c = r[2];
+    const rec_d2 a = ld_d2(rec_at(rec, j, 0)), b = ld_d2(rec_at(rec, j, 1)), c = ld_d2(rec_at(rec, j, 2));
     x = a.x; y = a.y; z = b.x; m = b.y; rho = c.x; pq = c.y;
 }
 // ux uy uz h
 __device__ __forceinline__ void rec_vel(const double* rec, int j, double& ux, double& uy, double& uz, double& h) {
-    const double2* r = rec_at(rec, j);
-    const double2 a = r[3], b = r[4];
+    const rec_d2 a = ld_d2(rec_at(rec, j, 3)), b = ld_d2(rec_at(rec, j, 4));
     ux = a.x; uy = a.y; uz = b.x; h = b.y;
 }
 // vis_t p_type i_type
 __device__ __forceinline__ void rec_aux(const double* rec, int j, double& vis_t, int& p_type, int& i_type) {
-    const double2 a = rec_at(rec, j)[5];
+    const rec_d2 a = ld_d2(rec_at(rec, j, 5));
     vis_t = a.x;
     const long long t = __double_as_longlong(a.y);
     p_type = (int)(unsigned)(t & 0xffffffffLL);
@@ -212,9 +224,27 @@ __device__ __forceinline__ void rec_aux(const double* rec, int j, double& vis_t,
 }
 // temp concn
 __device__ __forceinline__ void rec_scalar(const double* rec, int j, double& temp, double& concn) {
-    const double2 a = rec_at(rec, j)[6];
+    const rec_d2 a = ld_d2(rec_at(rec, j, 6));
     temp = a.x; concn = a.y;
 }
+
+// Compact neighbour records of the MIXED gathering passes, used while the
+// positions are the uploaded float values: the float pair math then reads
+// exactly the values it reads from the double state ((float)(x_i - x_j) of
+// two float-exact doubles is the rounded exact difference either way; m, rho
+// and p - p0 enter the pressure force rounded to float; h is float-exact; the
+// neighbour's viscosity is stored as the float the kernels round it to;
+// velocities and the assembly's p - p0 stay double), so the results are bit
+// for bit those of the 128-byte double records.
+// Pressure force, 32 bytes: float4 (x, y, z, m), float4 (rho, pres - pres0, h,
+// p_type as bits).
+constexpr int CREC_FLOATS = 8;
+// Advection and PPE assembly, 64 bytes: float4 (x, y, z, m), float4 (rho, h,
+// (float)(viscosity2 + vis_t rho), p_type | i_type << 16 as bits), double2
+// (ux, uy), double2 (uz, pres - pres0).
+constexpr int CREC_ADV_FLOATS = 16;
+__device__ __forceinline__ int crec_ptype(float w) { return (int)(short)(__float_as_int(w) & 0xffff); }
+__device__ __forceinline__ int crec_itype(float w) { return __float_as_int(w) >> 16; }
 
 struct PPEState;   // ppe.hip
 
@@ -249,6 +279,7 @@ struct Eng {
     struct TuneEv { int cls, kid, cand; hipEvent_t a, b; };
     std::vector<TuneEv> tune_ev;
     bool tune_open = false;
+    bool tune_skip = false;   // this step starts a search for some case: not representative, not timed
     int tune_class = 0;
     // graph of one BiCGSTAB iteration per vector precision, valid for comp
     bool use_graphs = true;
@@ -334,6 +365,9 @@ struct sph_ctx {
     double* apr_target = nullptr;        // APR target level per particle (the reference's M_num)
     double* xnew[3] = {};                // ALE: corrector positions, committed at the end of the step
     double* rec = nullptr;               // neighbour records of the gathering passes (cap * REC_DOUBLES)
+    float* crec = nullptr;               // compact pressure-force records (cap * CREC_FLOATS), MIXED only
+    float* crec_adv = nullptr;           // compact advection / assembly records (cap * CREC_ADV_FLOATS), MIXED only
+    bool pos_f32 = true;                 // positions are the uploaded float values (no refinement, no motion)
 
     sph::PPEState* ppe = nullptr;
     sph_ppe_report rep{};
