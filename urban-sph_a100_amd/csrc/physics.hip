@@ -191,11 +191,12 @@ __global__ void __launch_bounds__(BLOCK) k_pack_records(Seg sg) {
         d[1] = make_double2(S.uz[i], S.pres[i] - S.pres0[i]);
         if (WHICH == REC_VEL_PQ) return;
         const int t = (S.p_type[i] & 0xffff) | (S.i_type[i] << 16);
-        c[0] = make_float4((float)S.x[i], (float)S.y[i], (float)S.z[i], (float)S.m[i]);
+        if (WHICH == REC_PRE) c[0] = make_float4((float)S.x[i], (float)S.y[i], (float)S.z[i], (float)S.m[i]);
         c[1] = make_float4((float)S.rho[i], (float)S.h[i],
                            (float)(viscosity2(cd.pr, S.p_type[i]) + S.vis_t[i] * S.rho[i]), __int_as_float(t));
         return;
     }
+    if (WHICH == REC_PRE) return;
     double2* r = (double2*)(cd.rec + (size_t)i * REC_DOUBLES);
     r[2] = make_double2(S.rho[i], S.pres[i] - S.pres0[i]);
     if (WHICH == REC_PQ) return;
@@ -236,13 +237,25 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
     double st11 = 0, st12 = 0, st22 = 0, st23 = 0, st31 = 0, st33 = 0;
     const bool filt = (count % pr.freq_filt) == 0;
     const bool les = (pr.fv_solve != 0) && (pr.turbulence_model != SPH_LAMINAR);
+    const float* const crec = sizeof(R) == 4 ? cd.crec_adv : nullptr;
     NLRow row = nl_row(nl, i);
     for (int k = 0; k < row.cnt; k++) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_HP)) continue;
         int j = e & NB_IDX;
-        R dx = (R)(xi - S.x[j]), dy = (R)(yi - S.y[j]), dz = (R)(zi - S.z[j]);
-        R mj = (R)S.m[j], rhoj = (R)S.rho[j];
+        // the neighbour's position, m, rho and type from the compact record
+        // (the same values as the field arrays hold) or the field arrays
+        double xj, yj, zj, mj_d, rhoj_d;
+        int ptj;
+        const float* C = crec ? crec + (size_t)j * CREC_ADV_FLOATS : nullptr;
+        if (C) {
+            const rec_f4 a = ld_f4(C), b = ld_f4(C + 4);
+            xj = a.x; yj = a.y; zj = a.z; mj_d = a.w; rhoj_d = b.x; ptj = crec_ptype(b.w);
+        } else {
+            xj = S.x[j]; yj = S.y[j]; zj = S.z[j]; mj_d = S.m[j]; rhoj_d = S.rho[j]; ptj = S.p_type[j];
+        }
+        R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
+        R mj = (R)mj_d, rhoj = (R)rhoj_d;
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) + (R)1e-20;
         R tmp_wij = kernel_w<R>(kc, tmp_A, hi, tdist);
         R tmp_dwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
@@ -252,11 +265,18 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
         if (filt) tmp_flt += (double)(mj / rhoj * tmp_wij);
         if (les) {
             tmp_sigma += (double)(tdist * tdist * tmp_wij * mj / rhoj);
-            if (S.p_type[j] == 1) {
+            if (ptj == 1) {
                 R tdwxc = gc11 * tdwx + gc12 * tdwy + gc31 * tdwz;
                 R tdwyc = gc12 * tdwx + gc22 * tdwy + gc23 * tdwz;
                 R tdwzc = gc31 * tdwx + gc23 * tdwy + gc33 * tdwz;
-                R dux = (R)(S.ux[j] - uxi), duy = (R)(S.uy[j] - uyi), duz = (R)(S.uz[j] - uzi);
+                double uxj, uyj, uzj;
+                if (C) {
+                    const rec_d2 u = ld_d2(C + 8), w = ld_d2(C + 12);
+                    uxj = u.x; uyj = u.y; uzj = w.x;
+                } else {
+                    uxj = S.ux[j]; uyj = S.uy[j]; uzj = S.uz[j];
+                }
+                R dux = (R)(uxj - uxi), duy = (R)(uyj - uyi), duz = (R)(uzj - uzi);
                 st11 += (double)(mj / rhoj * dux * tdwxc);
                 st12 += (double)((R)0.5 * mj / rhoj * (dux * tdwyc + duy * tdwxc));
                 st22 += (double)(mj / rhoj * duy * tdwyc);
@@ -588,13 +608,56 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
                 if (m) pres_g = __shfl(pj, 63 - __clzll(m), 64);
             }
         }
-    // Each lane forms the terms of one candidate exactly as the serial
+    const int nt = inlet ? 8 : 5;
+    if (sizeof(R) == 4 && cd.nmp_off) {
+        // MIXED: each lane adds the terms of its share of the listed pairs
+        // (pairs lane, lane + 64, ... in list order), then a fixed butterfly
+        // adds the 64 lane sums; lane 0's total is used, so the sums are the
+        // same on every call, alone or in a batch
+        double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const long long e0 = cd.nmp_off[u], e1 = cd.nmp_off[u + 1];
+        for (long long e = e0 + lane; e < e1; e += 64) {
+            const int j = cd.nmp_ent[e];
+            R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+            R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
+            if (!(tdist < search_range)) continue;
+            R mj = (R)A.m[j], rhoj = (R)A.rho[j];
+            double presj = pold[j];
+            R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
+            s[0] += (double)(mj / rhoj * (R)presj * twij);
+            s[1] += (double)(mj / rhoj * (R)A.ux[j] * twij);
+            s[2] += (double)(mj / rhoj * (R)A.uy[j] * twij);
+            s[3] += (double)(mj / rhoj * (R)A.uz[j] * twij);
+            s[4] += (double)(mj / rhoj * twij);
+            if (inlet) {
+                R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
+                R tdwx = dx / tdist * tdwij, tdwy = dy / tdist * tdwij, tdwz = dz / tdist * tdwij;
+                R dp = (R)(presj - pres_g);
+                s[5] += (double)(mj / rhoj * dp * tdwx);
+                s[6] += (double)(mj / rhoj * dp * tdwy);
+                s[7] += (double)(mj / rhoj * dp * tdwz);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            for (int q = 0; q < 8; q++)
+                if (q < nt) s[q] += __shfl_xor(s[q], off, 64);
+        if (lane != 0) return;
+        const double tflt = s[4];
+        if (tflt < 1e-6) {
+            A.pres[i] = 0.0;
+        } else {
+            A.pres[i] = s[0] / tflt;
+            if (A.p_type[i] == -1) { A.ux[i] = s[1] / tflt; A.uy[i] = s[2] / tflt; A.uz[i] = s[3] / tflt; }
+            if (inlet) A.pres[i] = pres_g + (xi - xg) * s[5] + (yi - yg) * s[6] + (zi - zg) * s[7];
+        }
+        return;
+    }
+    // FP64: each lane forms the terms of one candidate exactly as the serial
     // scan does; lanes 0..7 then add term k of every contributing candidate
     // in candidate order, so every sum is bit for bit the serial one.
     __shared__ double term[8][64];
     __shared__ int use[64];
     double acc = 0.0;   // lane k < 8 holds sum k
-    const int nt = inlet ? 8 : 5;
     const bool listed = cd.nmp_off != nullptr;
     // one chunk of candidates v0 .. v0 + 63 of the window's flattened order:
     // lane l holds candidate v0 + l (particle j, or j < 0 for none)
@@ -1346,6 +1409,12 @@ static int run_freeslip(Eng& e, int which) {
 // Neighbour records of every case of the step, from its working state as
 // it is now; run before each pass that gathers from them.
 int pack_records(Eng& e, int which) {
+    if (which == REC_PRE) {
+        bool any = false;
+        for (size_t k = 0; k < e.cs.size(); k++) any |= e.h_desc[k].crec_adv != nullptr;
+        if (any) SEGL("pack_records", -1, k_pack_records<REC_PRE>);
+        return 0;
+    }
     if (which == REC_PQ) SEGL("pack_records", -1, k_pack_records<REC_PQ>);
     else if (which == REC_VEL_PQ) SEGL("pack_records", -1, k_pack_records<REC_VEL_PQ>);
     else SEGL("pack_records", -1, k_pack_records<REC_ALL>);
@@ -1405,6 +1474,7 @@ int step_physics_front(Eng& e) {
         if (mixed) SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<float>);
         else SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<double>);
     }
+    SPH_TRY(pack_records(e, REC_PRE));   // compact records for the strain pass
     if (mixed) SEGL("prep", TK_PREP, k_prep<float>);
     else SEGL("prep", TK_PREP, k_prep<double>);
     if (mixed) SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<float>);

@@ -456,7 +456,9 @@ PPEView ppe_view(const sph_ctx* c);
 int ppe_nlong(const sph_ctx* c);   // rows with tails of the case's PPE
 // neighbour records from the working state (physics.hip): REC_ALL, or only
 // the pairs that hold velocities and pressures (REC_VEL_PQ) or pressures (REC_PQ)
-enum { REC_ALL = 0, REC_VEL_PQ = 1, REC_PQ = 2 };
+// REC_PRE: before the strain pass, the compact records only (their viscosity
+// slot is refreshed by REC_ALL after that pass has computed vis_t)
+enum { REC_ALL = 0, REC_VEL_PQ = 1, REC_PQ = 2, REC_PRE = 3 };
 int pack_records(Eng& e, int which);
 // Neumann pairs of a case for its current search and precision (physics.hip)
 int neumann_pairs(sph_ctx* c);
