@@ -110,3 +110,14 @@ def test_enum_values_match_c(tmp_path):
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     out = [int(v) for v in subprocess.check_output([str(exe)]).decode().split()]
     assert out == [getattr(abi, p) for p, _ in pairs]
+
+
+def test_case_presets_normal_sources():
+    """The canyon generator supplies every building's outward normals, so its
+    parameters keep the uploaded normals; the library default is the
+    reference's face-plane rule."""
+    import cases
+    _, _, geom = cases.small_canyon(dx=0.02)
+    assert cases.canyon_params(geom, urbansph.default_params()).ib_normals == abi.IBN_UPLOADED
+    assert urbansph.default_params().ib_normals == abi.IBN_REFERENCE
+    assert cases.box_params(urbansph.default_params()).ib_normals == abi.IBN_REFERENCE

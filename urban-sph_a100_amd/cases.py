@@ -100,8 +100,7 @@ def box_params(P=None, dt=1e-4, tol=1e-8, precision=abi.FP64):
     prm.update(kernel_type=abi.WENDLAND2, kappa=2.0, fv_solve=abi.FV_CLEARY,
                turbulence_model=abi.LAMINAR, fg_solve=0, time_type=abi.PRE_COR,
                noslip_bc=1, mu_fluid=1.0e-3, u_limit=1000.0,
-               ppe_solver=abi.PPE_BICGSTAB, ppe_tol=tol, precision=precision,
-               ib_normals=abi.IBN_UPLOADED)   # outward normals of every building, from the generator
+               ppe_solver=abi.PPE_BICGSTAB, ppe_tol=tol, precision=precision)
     return prm
 
 
@@ -224,7 +223,8 @@ def canyon_params(geom, P=None, dt=5e-5, tol=1e-6, precision=abi.FP64, Cs=0.12,
                most_z0h=0.1 * z0, L2=geom["L2"], L3=geom["L3"],
                L_left=geom["L_left"], L_right=geom["L_right"],
                outlet_copy_dx=geom["outlet_copy_dx"],
-               ppe_solver=abi.PPE_BICGSTAB, ppe_tol=tol, precision=precision)
+               ppe_solver=abi.PPE_BICGSTAB, ppe_tol=tol, precision=precision,
+               ib_normals=abi.IBN_UPLOADED)   # outward normals of every building, from the generator
     return prm
 
 
