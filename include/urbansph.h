@@ -209,6 +209,11 @@ typedef struct sph_params {
                                    Defaults 0.764, 0.836, -0.036, 0.036, 0,
                                    0.152 (functions_PREP.cuh:958-963)         */
     double ib_face_tol;         /* 1e-5 m                                     */
+    int    mg_sort_all;         /* check of the multigrid set-up: 1 forms
+                                   every coarse row with the sorting Galerkin
+                                   kernels (term order, double sums) instead
+                                   of the fixed-point hash; 0 (default) only
+                                   rows of more than 256 distinct columns     */
 } sph_params;
 
 /* Sources of the IB marker normals (ib_normals). */
@@ -289,6 +294,14 @@ typedef struct sph_ppe_report {
     int    mg_coarsest;         /* unknowns of the coarsest level             */
     int    mg_status;           /* SPH_MG_*: why a multigrid that was asked
                                    for is not in use (SPH_MG_OK otherwise)   */
+    long long mg_sorted_rows;   /* coarse rows the sorting Galerkin kernels
+                                   formed (rows of more than 256 distinct
+                                   columns, or all with mg_sort_all)         */
+    int    mg_long_rows;        /* coarse rows of more than 128 entries,
+                                   summed by a wave each in every pass       */
+    int    heavy_lists;         /* particles of the last search with more than
+                                   512 neighbour-list entries (the record
+                                   passes run them last)                     */
 } sph_ppe_report;
 
 /* Why the multigrid preconditioner was not used (sph_ppe_report.mg_status). */

@@ -207,7 +207,35 @@ def test_apr_multigrid_reproducible_graphs_and_direct():
     print("[apr mg repro] graphs / direct", flush=True)
     a, _, ra, _, _ = _refined(prm, graphs=True)
     b, _, rb, _, _ = _refined(prm, graphs=False)
+    print(f"[apr mg repro] sorted rows {ra.mg_sorted_rows}, long coarse rows {ra.mg_long_rows}, "
+          f"heavy lists {ra.heavy_lists}, tails {ra.tail_rows}", flush=True)
     assert ra.precond == abi.PRECOND_MULTIGRID and rb.precond == abi.PRECOND_MULTIGRID
-    assert ra.tail_rows > 0 and ra.iterations == rb.iterations
+    assert ra.iterations == rb.iterations
+    # the paths the docstring names all ran
+    assert ra.tail_rows > 0 and ra.mg_sorted_rows > 0 and ra.mg_long_rows > 0 and ra.heavy_lists > 0
     for f in a:
         np.testing.assert_array_equal(a[f], b[f], err_msg=f)
+
+
+def test_sorting_galerkin_kernels_agree_with_the_hash():
+    """Every coarse row of the refined case formed by the sorting kernels
+    (double sums in term order) instead of the fixed-point hash: the coarse
+    operators agree to float rounding, so the solve takes the same number of
+    iterations (one more at most) and ends within the tolerance of the
+    hashed hierarchy's result."""
+    _, _, geom = cases.small_canyon(dx=0.02)
+    prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-6, precision=abi.MIXED)
+    prm.update(ppe_precond=abi.PRECOND_MULTIGRID)
+    srt = type(prm).from_buffer_copy(prm)
+    srt.update(mg_sort_all=1)
+    a, live, ra, _, _ = _refined(prm)
+    b, _, rb, _, _ = _refined(srt)
+    print(f"[apr sort-all] iterations hash {ra.iterations} sort {rb.iterations}; sorted rows {ra.mg_sorted_rows} / "
+          f"{rb.mg_sorted_rows}", flush=True)
+    assert rb.precond == abi.PRECOND_MULTIGRID and rb.converged == 1
+    assert rb.mg_sorted_rows > ra.mg_sorted_rows and rb.mg_levels == ra.mg_levels
+    assert abs(rb.iterations - ra.iterations) <= 1
+    for f in a:
+        e = rel_l2(b[f][live], a[f][live])
+        print(f"[apr sort-all] {f}: {e:.2e}", flush=True)
+        assert e <= 1e-5, f

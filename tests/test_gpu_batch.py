@@ -291,3 +291,28 @@ def test_lattice_order_matches_cell_order():
         tight.update(ppe_tol=1e-10)
         t, _ = _alone((P, nrm, tight, dt), 3)
         check_mixed(f"lattice {name}", got, ref, at_tol, t)
+
+
+@pytest.mark.parametrize("precision,precond", [(abi.MIXED, abi.PRECOND_MULTIGRID), (abi.FP64, abi.PRECOND_MULTIGRID),
+                                               (abi.MIXED, abi.PRECOND_JACOBI), (abi.FP64, abi.PRECOND_JACOBI)])
+def test_admission_estimate_bounds_device_memory(precision, precond):
+    """The server admits by sph_estimate_bytes before a request has run; after
+    two steps the context holds no more than that (live and cached blocks,
+    the multigrid set-up included), and the estimate is not loose by more
+    than 3x."""
+    P, nrm, geom = cases.sized_canyon(1.2e5)
+    prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-6, precision=precision)
+    prm.update(ppe_precond=precond)
+    est = urbansph.estimate_bytes(prm, len(P))
+    c = urbansph.Case(prm, len(P))
+    c.upload(P, None, nrm)
+    for s in range(2):
+        c.step(s, 5e-5)
+    c.sync()
+    held = c.device_bytes()
+    pc = c.ppe_report().precond
+    c.close()
+    print(f"[estimate] precision {precision} precond {pc}: held {held / 1e6:.1f} MB, estimate {est / 1e6:.1f} MB",
+          flush=True)
+    assert pc == precond
+    assert held <= est and est <= 3 * held

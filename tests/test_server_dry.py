@@ -32,6 +32,10 @@ def test_estimate_scales_with_particles():
     assert abs((b - a) - (a - (16 << 20))) <= 1
     m = _params(abi.MIXED)
     assert urbansph.estimate_bytes(m, 10_000) < a   # float matrix and Krylov vectors
+    # the multigrid preconditioner adds its hierarchy and set-up terms
+    mg = _params(abi.MIXED)
+    mg.update(ppe_precond=abi.PRECOND_MULTIGRID)
+    assert urbansph.estimate_bytes(mg, 10_000) > urbansph.estimate_bytes(m, 10_000) + 10_000 * 1000
 
 
 def test_budget_admission_fifo_and_continuous_batching():

@@ -92,6 +92,7 @@ class Params(C.Structure):
         + [(n, C.c_double) for n in (
             "ib_face_xlo", "ib_face_xhi", "ib_face_ylo", "ib_face_yhi", "ib_face_zlo", "ib_face_zhi",
             "ib_face_tol")]
+        + [("mg_sort_all", C.c_int)]
     )
 
     def update(self, **kw):
@@ -117,7 +118,8 @@ class PPEReport(C.Structure):
                 ("rel_residual", C.c_double), ("converged", C.c_int),
                 ("ordering", C.c_int), ("slots", C.c_longlong), ("index16", C.c_int), ("tail_rows", C.c_int),
                 ("precond", C.c_int), ("mg_levels", C.c_int), ("mg_coarsest", C.c_int),
-                ("mg_status", C.c_int)]
+                ("mg_status", C.c_int), ("mg_sorted_rows", C.c_longlong), ("mg_long_rows", C.c_int),
+                ("heavy_lists", C.c_int)]
 
 
 ORDER_CELL, ORDER_LATTICE = 0, 1

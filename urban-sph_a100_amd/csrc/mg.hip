@@ -87,10 +87,18 @@ struct MGState {
     // slot layout; null: the V-cycle reads the PPE's float values
     unsigned* ev = nullptr;
     double* lsum = nullptr;  // level 0: tail sums of the long rows for the next pass
+    long long sorted_rows = 0;   // coarse rows the sorting Galerkin kernels formed (all levels)
 };
 
 int mg_levels(const MGState* mg) { return mg ? mg->nlev : 0; }
 int mg_coarsest(const MGState* mg) { return mg && mg->nlev > 0 ? mg->L[mg->nlev - 1].n : 0; }
+long long mg_sorted_rows(const MGState* mg) { return mg ? mg->sorted_rows : 0; }
+int mg_long_rows(const MGState* mg) {
+    int k = 0;
+    if (mg)
+        for (int l = 1; l < mg->nlev; l++) k += mg->L[l].nlong;
+    return k;
+}
 
 // ------------------------------------------------------------------------
 // operators as row sources for the Galerkin product
@@ -379,7 +387,7 @@ template <class OP>
 __global__ void __launch_bounds__(SLICE) k_mg_rap_hash(OP op, int n1, const int* agg, const int* mstart,
                                                        const int* mem, double scale, int* redo, int* nredo,
                                                        int* cnt1, const long long* row_off, int* cols1, float* vals1,
-                                                       float* diag1, int* err) {
+                                                       float* diag1, int* err, int force) {
     __shared__ int hk[MG_HASH];                    // column, or -1
     __shared__ long long hs[MG_HASH];              // its fixed-point sum
     __shared__ unsigned long long key[MG_HASH / 2];   // column << 32 | table slot, sorted
@@ -393,6 +401,10 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap_hash(OP op, int n1, const int*
     nterm = wave_sum(nterm);
     if (nterm > MG_BIG_CAP) {   // the sums' bound
         if (lane == 0) err[0] = SPH_MG_TERMS;
+        return;
+    }
+    if (force) {   // every row to the sorting kernels (their check)
+        if (lane == 0) { redo[I] = 1; atomicAdd(nredo, 1); }
         return;
     }
     for (int h = lane; h < MG_HASH; h += SLICE) { hk[h] = -1; hs[h] = 0; }
@@ -812,29 +824,45 @@ void mg_free(sph_ctx* c, MGState* mg) {
 
 // Galerkin operator of level l + 1 from level l (src), whose link arrays
 // are set.  Returns 0 with *ok = 0 when a row is out of the kernel's bounds.
+// Device buffers of one set-up call, freed on every way out of it.
+struct ScopedBufs {
+    sph_ctx* c;
+    std::vector<void*> p;
+    template <class T>
+    int get(T** out, size_t count) {
+        SPH_TRY(dalloc(c, out, count));
+        p.push_back(*out);
+        return 0;
+    }
+    ~ScopedBufs() {
+        for (void* q : p) dfree(c, q);
+    }
+};
+
 template <class OP>
-static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0, int* err, int* ok, int* why) {
+static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0, int* err, int* ok, int* why,
+                       int force_sort, long long* nsorted) {
     const int n1 = L1.n;
+    ScopedBufs tmp{c, {}};
     // one pass: each row's columns into room for its terms, then packed
     long long *row_off = nullptr, *tcount = nullptr;
     long long room = 0;
-    SPH_TRY(dalloc(c, &tcount, (size_t)n1 + 1));
-    SPH_TRY(dalloc(c, &row_off, (size_t)n1 + 1));
+    SPH_TRY(tmp.get(&tcount, (size_t)n1 + 1));
+    SPH_TRY(tmp.get(&row_off, (size_t)n1 + 1));
     MGK("mg_rap_terms", grid1(n1), BLOCK, k_mg_terms<OP>, src, n1, (const int*)L0.mstart, (const int*)L0.mem, tcount);
     SPH_TRY(exclusive_scan_i64(c, tcount, row_off, n1, &room));
-    dfree(c, tcount);
     int* pc = nullptr;
     float* pv = nullptr;
-    SPH_TRY(dalloc(c, &pc, (size_t)std::max(1LL, room)));
-    SPH_TRY(dalloc(c, &pv, (size_t)std::max(1LL, room)));
+    SPH_TRY(tmp.get(&pc, (size_t)std::max(1LL, room)));
+    SPH_TRY(tmp.get(&pv, (size_t)std::max(1LL, room)));
     SPH_TRY(dalloc(c, &L1.cnt, n1));
     SPH_TRY(dalloc(c, &L1.diag, n1));
     // fixed-point scale of the hashed rows: 2^S with |entry| * 2^S <= 2^49,
     // so that MG_BIG_CAP (2^13) terms sum below 2^62
     int* redo = nullptr;
     int* aux = nullptr;   // [0]: largest |entry| (float bits), [1]: rows left to the sorting kernels
-    SPH_TRY(dalloc(c, &redo, n1));
-    SPH_TRY(dalloc(c, &aux, 2));
+    SPH_TRY(tmp.get(&redo, n1));
+    SPH_TRY(tmp.get(&aux, 2));
     SPH_HIP(c, hipMemsetAsync(aux, 0, 2 * sizeof(int), c->stream));
     MGK("mg_rap_scale", grid1(L0.n), BLOCK, k_mg_maxabs<OP>, src, L0.n, (unsigned*)aux);
     int h_aux[2] = {0, 0};
@@ -845,42 +873,34 @@ static int mg_galerkin(sph_ctx* c, const OP& src, MGLevel& L1, const MGLevel& L0
     if (!(amax > 0.0f) || !std::isfinite(amax)) amax = 1.0f;
     const double scale = std::ldexp(1.0, 49 - (std::ilogb((double)amax) + 1));
     MGK("mg_rap_fill", n1, SLICE, k_mg_rap_hash<OP>, src, n1, (const int*)L0.agg, (const int*)L0.mstart,
-        (const int*)L0.mem, scale, redo, aux + 1, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
+        (const int*)L0.mem, scale, redo, aux + 1, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err, force_sort);
     SPH_HIP(c, hipMemcpyAsync(h_aux + 1, aux + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
+    *nsorted += h_aux[1];
     if (h_aux[1] > 0) {
         MGK("mg_rap_sort", n1, SLICE, (k_mg_rap<OP, MG_ENT_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
             (const int*)L0.mem, (const int*)redo, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
         MGK("mg_rap_sort", n1, SLICE, (k_mg_rap<OP, MG_BIG_CAP>), src, n1, (const int*)L0.agg, (const int*)L0.mstart,
             (const int*)L0.mem, (const int*)redo, L1.cnt, (const long long*)row_off, pc, pv, L1.diag, err);
     }
-    dfree(c, redo);
-    dfree(c, aux);
     int h_err = 0;
     SPH_HIP(c, hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SPH_HIP(c, hipStreamSynchronize(c->stream));
     if (h_err) {
-        dfree(c, row_off);
-        dfree(c, pc);
-        dfree(c, pv);
         *ok = 0;
         *why = h_err;
         return 0;
     }
     L1.nslices = (n1 + SLICE - 1) / SLICE;
     long long* w = nullptr;
-    SPH_TRY(dalloc(c, &w, (size_t)L1.nslices + 1));
+    SPH_TRY(tmp.get(&w, (size_t)L1.nslices + 1));
     SPH_TRY(dalloc(c, &L1.sl_off, (size_t)L1.nslices + 1));
     MGK("mg_width", grid1(L1.nslices), BLOCK, k_mg_width, n1, L1.nslices, (const int*)L1.cnt, w);
     SPH_TRY(exclusive_scan_i64(c, w, L1.sl_off, L1.nslices, &L1.slots));
-    dfree(c, w);
     SPH_TRY(dalloc(c, &L1.cols, (size_t)std::max(1LL, L1.slots)));
     SPH_TRY(dalloc(c, &L1.vals, (size_t)std::max(1LL, L1.slots)));
     MGK("mg_rap_pack", grid1(n1), BLOCK, k_mg_pack, n1, (const long long*)row_off, (const int*)L1.cnt,
         (const int*)pc, (const float*)pv, (const long long*)L1.sl_off, L1.cols, L1.vals);
-    dfree(c, row_off);
-    dfree(c, pc);
-    dfree(c, pv);
     *ok = 1;
     *why = 0;
     return 0;
@@ -1004,8 +1024,9 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         if (L1.n * 10LL > L0.n * 9LL) { nlev++; good = false; *why = SPH_MG_STALLED; break; }
         int lok = 0;
         int code = 0;
-        if (nlev == 1) MGTRY(mg_galerkin(c, FineOp{pp}, L1, L0, err, &lok, &code));
-        else MGTRY(mg_galerkin(c, coarse_op(L0), L1, L0, err, &lok, &code));
+        if (nlev == 1)
+            MGTRY(mg_galerkin(c, FineOp{pp}, L1, L0, err, &lok, &code, c->prm.mg_sort_all, &mg->sorted_rows));
+        else MGTRY(mg_galerkin(c, coarse_op(L0), L1, L0, err, &lok, &code, c->prm.mg_sort_all, &mg->sorted_rows));
         if (!lok) *why = code;   // SPH_MG_TERMS or SPH_MG_SPAN
         nlev++;
         if (!lok) { good = false; break; }

@@ -451,6 +451,7 @@ static int build_porder(sph_ctx* c) {
     const int n = c->n;
     dfree(c, c->porder);
     c->porder = nullptr;
+    c->n_heavy = 0;
     if (n == 0) return 0;
     int *flag = nullptr, *scan = nullptr;
     SPH_TRY(dalloc(c, &flag, n));
@@ -459,6 +460,7 @@ static int build_porder(sph_ctx* c) {
     SPH_TRY(launch_check(c, "porder"));
     int nheavy = 0;
     SPH_TRY(exclusive_scan_i32(c, flag, scan, n, &nheavy));
+    c->n_heavy = nheavy;
     if (nheavy > 0) {
         SPH_TRY(dalloc(c, &c->porder, n));
         hipLaunchKernelGGL(k_porder, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, n - nheavy, (const int*)flag,

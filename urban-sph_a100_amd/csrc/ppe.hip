@@ -1370,6 +1370,7 @@ int ppe_solve(Eng& e) {
         c->rep.slots = c->ppe ? c->ppe->nnz_slots : 0;
         c->rep.index16 = c->ppe && c->ppe->ecols ? 1 : 0;
         c->rep.tail_rows = c->ppe ? c->ppe->nlong : 0;
+        c->rep.heavy_lists = c->n_heavy;
         c->rep.precond = SPH_PRECOND_JACOBI;
         if (c->rep.unknowns == 0) c->rep.converged = 1;
         rows_total += c->rep.unknowns;
@@ -1433,7 +1434,12 @@ int ppe_solve(Eng& e) {
                 P->mg_state = ok ? 1 : -1;
             }
         }
-        if (P->mg_state == 1) { mgs[k] = P->mg; any_mg = true; }
+        if (P->mg_state == 1) {
+            mgs[k] = P->mg;
+            any_mg = true;
+            c->rep.mg_sorted_rows = mg_sorted_rows(P->mg);
+            c->rep.mg_long_rows = mg_long_rows(P->mg);
+        }
         c->rep.mg_status = P->mg_state == -1 ? P->mg_why : SPH_MG_OK;
     }
     auto mg_tables = [&]() -> int {

@@ -155,10 +155,24 @@ long long sph_estimate_bytes(const sph_params* prm, int n) {
     // column index + value, plus the Krylov vectors.  Upper bounds for the
     // 1.5h / kappa=2 lattices the workloads use; the real figure replaces
     // the estimate after the first step.
-    const double val = prm->precision == SPH_MIXED ? 4.0 : 8.0;
-    const double krylov = prm->precision == SPH_MIXED ? 10 * 4.0 : 8 * 8.0;
+    const bool mixed = prm->precision == SPH_MIXED;
+    const double val = mixed ? 4.0 : 8.0;
+    const double krylov = mixed ? 10 * 4.0 : 8 * 8.0;
     double per = 96 * 8.0 + 6 * 4.0 + 16.0 + 160 * 4.0 + 128 * (4.0 + val) + 6 * 8.0 + krylov;
-    return (long long)(per * n) + (16LL << 20);
+    // neighbour records of the gathering passes: 128 bytes, plus the compact
+    // 32- and 64-byte records in MIXED
+    per += 128.0 + (mixed ? 96.0 : 0.0);
+    long long fixed = 16LL << 20;
+    if (prm->ppe_precond == SPH_PRECOND_MULTIGRID && prm->ppe_solver == SPH_PPE_BICGSTAB) {
+        // the level-0 compact copy (4 bytes a slot), the coarse levels (about
+        // a seventh of the fine entries at 8 bytes), the Galerkin terms of the
+        // set-up (8 bytes a fine entry, kept by the block cache), aggregation
+        // keys and links; FP64 adds the float copy of its operator and four
+        // float vectors; the dense coarsest inverse and its double build
+        per += 128 * 4.0 + 160.0 + 128 * 8.0 + 64.0 + (mixed ? 0.0 : 128 * 4.0 + 4 * 4.0 + 4.0);
+        fixed += 12LL << 20;
+    }
+    return (long long)(per * n) + fixed;
 }
 
 int sph_server_create(long long hbm_budget_bytes, int max_cases, double dry_run_rate, sph_server** out) {

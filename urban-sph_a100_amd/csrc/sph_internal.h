@@ -358,6 +358,7 @@ struct sph_ctx {
     long long nl_total = 0;
     long long* nl_wprev = nullptr;   // slice widths of the last search (room for the next)
     int* porder = nullptr;           // order of the particle passes (CaseDesc::porder), or null
+    int n_heavy = 0;                 // particles of that search with lists of more than 512 entries
     int nl_wprev_n = 0;              // its slices
     int* scan_tmp = nullptr; size_t scan_tmp_n = 0;
 
@@ -481,6 +482,8 @@ int mg_apply(Eng& e, int which, int sc_done);
 void mg_free(sph_ctx* c, MGState* mg);
 int mg_levels(const MGState* mg);
 int mg_coarsest(const MGState* mg);
+long long mg_sorted_rows(const MGState* mg);   // coarse rows formed by the sorting Galerkin kernels
+int mg_long_rows(const MGState* mg);           // coarse rows of more than 128 entries (a wave each)
 int ppe_solve(Eng& e);   // every case of the step; 0, or SPH_E_NOCONV when a case did not converge
 void ppe_free(sph_ctx* c);
 
