@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(BLOCK) k_pack_records(Seg sg) {
         d[0] = make_double2(S.ux[i], S.uy[i]);
         d[1] = make_double2(S.uz[i], S.pres[i] - S.pres0[i]);
         if (WHICH == REC_VEL_PQ) return;
-        const int t = (S.p_type[i] & 0xffff) | (S.i_type[i] << 16);
+        const int t = (S.p_type[i] & 0xffff) | (S.i_type[i] << 16) |
+                      (cd.ppe.rowof && cd.ppe.rowof[i] >= 0 ? CREC_ROW_BIT : 0);
         if (WHICH == REC_PRE) c[0] = make_float4((float)S.x[i], (float)S.y[i], (float)S.z[i], (float)S.m[i]);
         c[1] = make_float4((float)S.rho[i], (float)S.h[i],
                            (float)(viscosity2(cd.pr, S.p_type[i]) + S.vis_t[i] * S.rho[i]), __int_as_float(t));
@@ -721,11 +722,17 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
 
 // ------------------------------------------------------------------------
 // KERNEL_advection_force3D, functions_PPE.cuh:191-480
-template <class R>
-__global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
+// SCAL: some case of the launch transports temperature or concentration or
+// uses the Boussinesq buoyancy; without them (the canyon) those terms and
+// their registers are compiled out (130 -> fewer VGPRs, more waves per SIMD)
+template <class R, bool SCAL>
+__global__ void __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(SCAL ? 1 : 5)))
+k_advection(Seg sg) {
     SEG_PARTICLE_ORDERED(sg);
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
+    const bool con_solve = SCAL && pr.con_solve, concn_solve = SCAL && pr.concn_solve;
+    const bool boussinesq_solve = SCAL && pr.boussinesq_solve;
     const PView S = cd.S;
     const QView Q = cd.Q;
     const NList nl = cd.nl;
@@ -735,17 +742,17 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
     if (ptypei <= 0) return;
     double xi = S.x[i], yi = S.y[i], zi = S.z[i];
     double uxi = S.ux[i], uyi = S.uy[i], uzi = S.uz[i];
-    double tempi = S.temp[i];
+    double tempi = SCAL ? S.temp[i] : 0.0;
     R hi = (R)S.h[i], mi = (R)S.m[i], rhoi = (R)S.rho[i];
     double hi_d = S.h[i];
     double eli = S.eli[i];
     R tmp_A = calc_tmpA<R>(kc, hi);
     R kci = 0, cpi = 1, concni = 0, diffi = 0;
-    if (pr.con_solve) {
+    if (con_solve) {
         kci = (R)(pr.solid_k * (ptypei == 0 || ptypei == 1000) + pr.air_k * (ptypei == 1 || ptypei == -1));
         cpi = (R)(pr.solid_cp * (ptypei == 0 || ptypei == 1000) + pr.air_cp * (ptypei == 1 || ptypei == -1));
     }
-    if (pr.concn_solve) { concni = (R)S.concn[i]; diffi = (R)pr.diff_coef; }
+    if (concn_solve) { concni = (R)S.concn[i]; diffi = (R)pr.diff_coef; }
     R visi = (R)(viscosity2(pr, ptypei) + S.vis_t[i] * S.rho[i]);
     double betai = 1.0 / pr.T_ini;
     double tmpx = 0, tmpy = 0, tmpz = 0, tmp_Rc = 0, tmp_Rd = 0;
@@ -790,7 +797,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
         R mj = (R)mj_d, rhoj = (R)rhoj_d;
         R dux = (R)(uxi - uxj), duy = (R)(uyi - uyj), duz = (R)(uzi - uzj);   // u_i - u_j
         double tempj = 0.0, concnj_d = 0.0;
-        if (pr.con_solve || pr.concn_solve || pr.boussinesq_solve) {
+        if (con_solve || concn_solve || boussinesq_solve) {
             if (compact) { tempj = S.temp[j]; concnj_d = S.concn[j]; }
             else rec_scalar(cd.rec, j, tempj, concnj_d);
         }
@@ -821,14 +828,14 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             eulerx += (double)((Rux * vx * tdwx + Ruy * vx * tdwy + Ruz * vx * tdwz) * mj / rhoj * ome);
             eulery += (double)((Rux * vy * tdwx + Ruy * vy * tdwy + Ruz * vy * tdwz) * mj / rhoj * ome);
             eulerz += (double)((Rux * vz * tdwx + Ruy * vz * tdwy + Ruz * vz * tdwz) * mj / rhoj * ome);
-            if (pr.con_solve)
+            if (con_solve)
                 eulert -= (double)((R)2.0 * mj / (rhoi + rhoj) * (R)(tempj - tempi) * (dux * tdwx + duy * tdwy + duz * tdwz) * ome);
-            if (pr.concn_solve) {
+            if (concn_solve) {
                 R dc = (R)(concnj_d - (double)concni);
                 tmp_Rd -= (double)((Rux * dc * tdwx + Ruy * dc * tdwy + Ruz * dc * tdwz) * mj / rhoj * ome);
             }
         }
-        if (pr.con_solve) {
+        if (con_solve) {
             R kcj = (R)(pr.solid_k * (ptypej == 0 || ptypej == 1000) + pr.air_k * (ptypej == 1 || ptypej == -1));
             R rhoj_temp = (ptypej == 1 || ptypej == -1) ? rhoj : (R)pr.solid_rho;
             R mj_temp = mj * rhoj_temp / rhoj;
@@ -836,7 +843,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             sum_con_H /= (tdist + (R)1e-10) * rhoi * rhoj_temp * (kci + kcj);
             tmp_Rc += (double)sum_con_H;
         }
-        if (pr.concn_solve) {
+        if (concn_solve) {
             R concnj = (R)concnj_d;
             R diffj = (R)pr.diff_coef;
             R tmprd = mi * (diffi * rhoi + diffj * rhoj);
@@ -845,7 +852,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
             R tmprd1 = tmprd * (concni - concnj) * (R)(ptypei == ptypej);
             tmp_Rd += (double)tmprd1;
         }
-        if (pr.boussinesq_solve) {
+        if (boussinesq_solve) {
             tmp_temp += (double)((mj / rhoj) * (R)tempj * twij);
             tmp_flt += (double)((mj / rhoj) * twij);
         }
@@ -863,14 +870,14 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_advection(Seg sg) {
         }
     }
     if (pr.fg_solve) tmpz += -pr.gravity;
-    if ((pr.boussinesq_solve) && (ptypei > 0) && (ptypei < 1000))
+    if ((boussinesq_solve) && (ptypei > 0) && (ptypei < 1000))
         tmpz = betai * pr.gravity * (tmp_temp / tmp_flt - pr.tempb);
     Q.ftotalx[i] = tmpx - eulerx;
     Q.ftotaly[i] = tmpy - eulery;
     Q.ftotalz[i] = tmpz - eulerz;
-    if (pr.con_solve) Q.dtemp[i] += tmp_Rc / (double)cpi - eulert;
+    if (con_solve) Q.dtemp[i] += tmp_Rc / (double)cpi - eulert;
     Q.ftotal[i] = sqrt(tmpx * tmpx + tmpy * tmpy + tmpz * tmpz);
-    if (pr.concn_solve) Q.dconcn[i] = tmp_Rd;
+    if (concn_solve) Q.dconcn[i] = tmp_Rd;
 }
 
 // ------------------------------------------------------------------------
@@ -1483,8 +1490,15 @@ int step_physics_front(Eng& e) {
     SEGL("most_deposit", -1, k_most_deposit);
     if (any_count0) SPH_TRY(run_neumann(e, 0, 1));
     SPH_TRY(pack_records(e, REC_ALL));
-    if (mixed) SEGL("advection", TK_ADVECTION, k_advection<float>);
-    else SEGL("advection", TK_ADVECTION, k_advection<double>);
+    bool scal = false;   // scalar transport or buoyancy in some case of the step
+    for (sph_ctx* c : e.cs) scal |= c->prm.con_solve || c->prm.concn_solve || c->prm.boussinesq_solve;
+    if (mixed) {
+        if (scal) SEGL("advection", TK_ADVECTION, (k_advection<float, true>));
+        else SEGL("advection", TK_ADVECTION, (k_advection<float, false>));
+    } else {
+        if (scal) SEGL("advection", TK_ADVECTION, (k_advection<double, true>));
+        else SEGL("advection", TK_ADVECTION, (k_advection<double, false>));
+    }
     SEGL("projection", -1, k_projection);
     SPH_TRY(run_periodic(e, 0));
     SPH_TRY(run_freeslip(e, 0));

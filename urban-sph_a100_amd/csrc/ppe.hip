@@ -457,10 +457,12 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
         double xj, yj, zj, mj_d, rhoj_d, qj, uxj, uyj, uzj;
+        bool jrow;   // j is an unknown of the PPE
         if (crec) {   // the compact record (positions float-exact)
             const float* C = crec + (size_t)j * CREC_ADV_FLOATS;
             const rec_f4 b = ld_f4(C + 4);
             if (crec_ptype(b.w) >= 1000 || crec_itype(b.w) == 4) continue;
+            jrow = crec_has_row(b.w);
             const rec_f4 a = ld_f4(C);
             const rec_d2 u = ld_d2(C + 8), w = ld_d2(C + 12);
             xj = a.x; yj = a.y; zj = a.z; mj_d = a.w; rhoj_d = b.x;
@@ -472,6 +474,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
             if (ptj >= 1000 || itj == 4) continue;
             rec_pos(rec, j, xj, yj, zj, mj_d, rhoj_d, qj);
             rec_vel(rec, j, uxj, uyj, uzj, hj_);
+            jrow = rowof[j] >= 0;
         }
         R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
         R rhoj = (R)rhoj_d, mj = (R)mj_d;
@@ -492,7 +495,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         Aij += ccd;
         if (jacobi) AijPij += ccd * qj;
         if (j != i) {
-            if (rowof[j] >= 0) {
+            if (jrow) {
                 if (out) {
                     if (slot < LONG_ROW || !pp.ltail) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
                     else pp.tail_vals[pp.tail_off[pp.ltail[r]] + (slot - LONG_ROW)] = (float)cc;

@@ -243,8 +243,12 @@ constexpr int CREC_FLOATS = 8;
 // (float)(viscosity2 + vis_t rho), p_type | i_type << 16 as bits), double2
 // (ux, uy), double2 (uz, pres - pres0).
 constexpr int CREC_ADV_FLOATS = 16;
+// type word: p_type in the low 16 bits, i_type in bits 16..29, bit 30 set
+// when the particle is an unknown of the case's PPE (it has a row)
+constexpr int CREC_ROW_BIT = 1 << 30;
 __device__ __forceinline__ int crec_ptype(float w) { return (int)(short)(__float_as_int(w) & 0xffff); }
-__device__ __forceinline__ int crec_itype(float w) { return __float_as_int(w) >> 16; }
+__device__ __forceinline__ int crec_itype(float w) { return (__float_as_int(w) >> 16) & 0x3fff; }
+__device__ __forceinline__ bool crec_has_row(float w) { return (__float_as_int(w) & CREC_ROW_BIT) != 0; }
 
 struct PPEState;   // ppe.hip
 
