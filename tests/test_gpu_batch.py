@@ -170,8 +170,11 @@ def test_batch_staggered_membership():
 
 @pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
 def test_graphs_and_autotune_change_nothing(precision):
+    """Steps 1-4 time the autotuner's candidates; from step 5 on the
+    physics before and after the solve runs directly once, is captured at
+    the next step and replayed after that: ten steps cover every stage."""
     spec = _specs(precision)[3]
-    steps = 6   # past the four tuning steps
+    steps = 10
     ref = _alone(spec, steps, graphs=False, autotune=False)
     _assert_same(_alone(spec, steps, graphs=True, autotune=False), ref, "graphs")
     _assert_same(_alone(spec, steps, graphs=False, autotune=True), ref, "autotune")

@@ -97,6 +97,9 @@ int tuned_block(Eng& e, int kid) {
     if (t.phase < TUNE_CANDIDATES) return TUNE_BLOCKS[t.phase];
     return t.block[kid];
 }
+bool tune_pending(Eng& e) {
+    return e.autotune && tune_for(e, e.tune_class).phase < TUNE_CANDIDATES;
+}
 int tune_begin(Eng& e, int kid, int block) {
     e.tune_open = false;
     if (!e.autotune) return 0;
@@ -280,6 +283,48 @@ int eng_init(Eng& e, hipStream_t st) {
 static void drop_graphs(Eng& e) {
     for (auto& g : e.iter_graph)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto& g : e.step_graph) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = Eng::StepGraph{};
+    }
+}
+
+// One half of the step's physics: replayed from its graph when this
+// composition already ran the same variant twice, else run directly (the
+// first time) or captured and launched (the second).  Graphs are off while
+// the profile is on or the autotuner still times candidates.
+static int run_half(Eng& e, int which, long long key, int (*fn)(Eng&)) {
+    if (!e.use_graphs || e.prof.on || tune_pending(e)) return fn(e);
+    Eng::StepGraph& g = e.step_graph[which];
+    if (g.exec && g.key == key) {
+        SPH_HIP(&e, hipGraphLaunch(g.exec, e.st));
+        return 0;
+    }
+    if (g.seen != key) {
+        g.seen = key;
+        return fn(e);
+    }
+    if (g.exec) {   // another variant's graph: nothing in flight may still run it
+        SPH_HIP(&e, hipStreamSynchronize(e.st));
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    SPH_HIP(&e, hipStreamBeginCapture(e.st, hipStreamCaptureModeThreadLocal));
+    const int rc = fn(e);
+    hipError_t er = hipStreamEndCapture(e.st, &gr);
+    if (rc) {
+        if (er == hipSuccess && gr) (void)hipGraphDestroy(gr);
+        return rc;
+    }
+    SPH_HIP(&e, er);
+    er = hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (er != hipSuccess) g.exec = nullptr;
+    SPH_HIP(&e, er);
+    g.key = key;
+    SPH_HIP(&e, hipGraphLaunch(g.exec, e.st));
+    return 0;
 }
 
 void eng_free(Eng& e) {
@@ -436,14 +481,14 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
     Shape sh;
     SPH_TRY(shape(e, 0, BLOCK, &sh));
     SPH_TRY(shape(e, 1, BLOCK, &sh));
-    SPH_TRY(step_physics_front(e));
+    SPH_TRY(run_half(e, 0, step_physics_front_key(e), step_physics_front));
     for (int k = 0; k < nc; k++) cases[k]->gc_gen = cases[k]->generation;
     const int prc = ppe_solve(e);
     for (int k = 0; k < nc; k++) cases[k]->vals_gen = cases[k]->generation;
     // a case whose solve did not converge, or met NaN/Inf, still completes
     // the step with the others; its error is reported at the end
     if (prc != 0 && prc != SPH_E_NOCONV && prc != SPH_E_NONFINITE) return prc;
-    SPH_TRY(step_physics_back(e));
+    SPH_TRY(run_half(e, 1, step_physics_back_key(e), step_physics_back));
     return prc;
 }
 

@@ -1462,6 +1462,27 @@ __global__ void __launch_bounds__(BLOCK) k_ib_normals(Seg sg) {
     }
 }
 
+// The host-side choices of step_physics_front, one bit each.
+long long step_physics_front_key(const Eng& e) {
+    long long key = mixed_of(e) ? 1 : 0;
+    bool count0 = false, gc = false, nrm = false, scal = false, compact = false;
+    for (size_t k = 0; k < e.cs.size(); k++) {
+        const sph_ctx* c = e.cs[k];
+        count0 |= e.h_desc[k].count == 0;
+        gc |= !e.h_desc[k].gc_fresh;
+        const int m = c->prm.ib_normals;
+        nrm |= m == SPH_IBN_REFERENCE || (m == SPH_IBN_PLANES && e.h_desc[k].count == 0);
+        scal |= c->prm.con_solve || c->prm.concn_solve || c->prm.boussinesq_solve;
+        compact |= e.h_desc[k].crec_adv != nullptr;
+    }
+    return key | (count0 ? 2 : 0) | (gc ? 4 : 0) | (nrm ? 8 : 0) | (scal ? 16 : 0) | (compact ? 32 : 0);
+}
+long long step_physics_back_key(const Eng& e) {
+    bool ale = false;
+    for (const sph_ctx* c : e.cs) ale |= c->prm.ale_positions != 0;
+    return (mixed_of(e) ? 1 : 0) | (ale ? 2 : 0);
+}
+
 // Everything of SOPHIA_single_ISPH before the pressure solve
 // (ISPH_Calc.cuh:20-132).  The count==0 neighbour searches have already run.
 int step_physics_front(Eng& e) {

@@ -294,6 +294,12 @@ struct Eng {
     // refinement pass): the host polls for convergence only near them
     static constexpr int PRED_SLOTS = 4;
     int pred_iters[NGRAPH][PRED_SLOTS] = {};
+    // the physics before (0) and after (1) the pressure solve as graphs, for
+    // the composition of the step and the variant of its launch sequence
+    // (key): the first step of a variant runs directly (which builds every
+    // table it uses), the next one is captured, later ones replay it
+    struct StepGraph { long long key = -1, seen = -1; hipGraphExec_t exec = nullptr; };
+    StepGraph step_graph[2];
     // multigrid V-cycles of the step's cases (mg.hip): per case and level a
     // descriptor, and block tables of the level rows (mg_plan)
     void* mg_lv = nullptr;            // device MGLv[case * MG_MAX_LEVELS + level]
@@ -452,6 +458,10 @@ int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out);
 // physics.hip
 int step_physics_front(Eng& e);
 int step_physics_back(Eng& e);
+// the choices of those launch sequences that vary from step to step, as keys
+long long step_physics_front_key(const Eng& e);
+long long step_physics_back_key(const Eng& e);
+bool tune_pending(Eng& e);   // the autotuner still times candidates for this step's size class
 int stats_run(sph_ctx* c, float* umax, float* drho, float* fmax);
 int stats_run_noalloc(sph_ctx* c, float* umax, float* drho, float* fmax);
 
