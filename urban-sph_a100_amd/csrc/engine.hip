@@ -162,6 +162,83 @@ __global__ void k_block_keys(const int* keys, long long nb, int block, int* out)
     if (b < nb) out[b] = keys[b * block];
 }
 
+// Morton code (10 bits an axis) of the cell of each block's first particle,
+// and the block's first particle
+__device__ __forceinline__ unsigned spread10(unsigned v) {
+    v &= 0x3ff;
+    v = (v | v << 16) & 0x030000ffu;
+    v = (v | v << 8) & 0x0300f00fu;
+    v = (v | v << 4) & 0x030c30c3u;
+    v = (v | v << 2) & 0x09249249u;
+    return v;
+}
+__global__ void __launch_bounds__(256) k_block_morton(const int* keys, int nb, int block, int NI, int NJ, int* code,
+                                                     int* first) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const long long cell = keys[(size_t)b * block];
+    const unsigned ci = (unsigned)(cell % NI), cj = (unsigned)((cell / NI) % NJ),
+                   ck = (unsigned)(cell / ((long long)NI * NJ));
+    code[b] = (int)(spread10(ci) | spread10(cj) << 1 | spread10(ck) << 2);
+    first[b] = b * block;
+}
+__global__ void __launch_bounds__(256) k_block_table(const int* first, int nb, int k, int2* out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) out[b] = make_int2(k, first[b]);
+}
+
+// The particle-block table of shape() built on the device: per case, a stable
+// radix sort of the blocks by the Morton code of their first cell (the order
+// the host's stable sort gives), written at the case's offset.
+template <class U>
+static int shape_device(Eng& e, int block, long long total, const U& units, Eng::Tab* t, Shape* out) {
+    const int nc = (int)e.cs.size();
+    if (!t) {
+        e.tabs.push_back(Eng::Tab{});
+        t = &e.tabs.back();
+        t->kind = 0;
+        t->block = block;
+    }
+    if ((size_t)total > t->cap) {
+        if (t->d) (void)hipFree(t->d);
+        t->d = nullptr;
+        t->cap = std::max<size_t>((size_t)total, 2 * t->cap);
+        if (hipMalloc(&t->d, sizeof(int2) * t->cap) != hipSuccess) {
+            t->d = nullptr;
+            t->cap = 0;
+            (void)hipGetLastError();
+            return fail(&e, SPH_E_NOMEM, "segment table allocation failed");
+        }
+    }
+    long long off = 0;
+    for (int k = 0; k < nc; k++) {
+        sph_ctx* c = e.cs[k];
+        const int nb = (int)((units(k) + block - 1) / block);
+        if (nb == 0) continue;
+        int *code = nullptr, *first = nullptr, *ct = nullptr, *ft = nullptr;
+        SPH_TRY(dalloc(c, &code, nb));
+        SPH_TRY(dalloc(c, &first, nb));
+        SPH_TRY(dalloc(c, &ct, nb));
+        SPH_TRY(dalloc(c, &ft, nb));
+        hipLaunchKernelGGL(k_block_morton, dim3(grid1(nb)), dim3(256), 0, e.st, (const int*)c->keys, nb, block,
+                           c->dom.NI, c->dom.NJ, code, first);
+        SPH_TRY(launch_check(&e, "block_morton"));
+        SPH_TRY(radix_sort_pairs(c, &code, &first, &ct, &ft, nb, (1 << 30) - 1));
+        hipLaunchKernelGGL(k_block_table, dim3(grid1(nb)), dim3(256), 0, e.st, (const int*)first, nb, k, t->d + off);
+        SPH_TRY(launch_check(&e, "block_table"));
+        dfree(c, code);
+        dfree(c, first);
+        dfree(c, ct);
+        dfree(c, ft);
+        off += nb;
+    }
+    t->nblk = (unsigned)total;
+    t->valid = true;
+    out->grid = t->nblk;
+    out->tab = t->d;
+    return 0;
+}
+
 int shape(Eng& e, int kind, int block, Shape* out) {
     const int nc = (int)e.cs.size();
     auto units = [&](int k) -> long long {
@@ -184,6 +261,19 @@ int shape(Eng& e, int kind, int block, Shape* out) {
         out->grid = t->nblk;
         out->tab = t->d;
         return 0;
+    }
+    if (kind == 0) {
+        // cases whose cell grid fits 10 bits an axis sort their blocks on the
+        // device (no copy to the host and back)
+        bool dev = true;
+        long long total = 0;
+        for (int k = 0; k < nc; k++) {
+            const sph_ctx* c = e.cs[k];
+            const sph_domain& d = c->dom;
+            dev &= c->keys != nullptr && d.NI > 0 && d.NJ > 0 && d.NI <= 1024 && d.NJ <= 1024 && d.NK < 1024;
+            total += (units(k) + block - 1) / block;
+        }
+        if (dev) return shape_device(e, block, total, units, t, out);
     }
     std::vector<int2> h;
     if (kind == 0) {

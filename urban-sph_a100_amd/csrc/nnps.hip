@@ -158,13 +158,14 @@ static int exclusive_scan(sph_ctx* c, const T* in, T* out, long long n, T* total
     hipLaunchKernelGGL(k_scan_block<T>, dim3((unsigned)nb), dim3(BLOCK), 0, c->stream, in, out, n, sums);
     SPH_TRY(launch_check(c, "scan_block"));
     T total = 0;
+    // the total goes to the host (a wait for the stream) only when asked for
     if (nb > 1) {
         SPH_TRY(dalloc(c, &sums_scan, nb + 1));
-        SPH_TRY(exclusive_scan<T>(c, sums, sums_scan, nb, &total));
+        SPH_TRY(exclusive_scan<T>(c, sums, sums_scan, nb, total_out ? &total : nullptr));
         hipLaunchKernelGGL(k_scan_add<T>, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, out, n, (const T*)sums_scan);
         SPH_TRY(launch_check(c, "scan_add"));
         dfree(c, sums_scan);
-    } else {
+    } else if (total_out) {
         SPH_HIP(c, hipMemcpyAsync(&total, sums, sizeof(T), hipMemcpyDeviceToHost, c->stream));
         SPH_HIP(c, hipStreamSynchronize(c->stream));
     }
