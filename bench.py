@@ -43,11 +43,14 @@ def log(*a):
 
 
 GRAPHS = True   # --no-graphs: launch every BiCGSTAB kernel directly (for counter runs)
+INNER_TOL = None   # --inner-tol: floor of the MIXED inner solves' relative tolerance (library default if None)
 
 
 def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, dt, tol, precond=0):
     prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=precision)
     prm.update(ppe_precond=precond)
+    if INNER_TOL is not None:
+        prm.update(ppe_inner_tol=INNER_TOL)
     case = urbansph.Case(prm, len(P))
     if not GRAPHS:
         case.options(graphs=False, autotune=True)
@@ -488,6 +491,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--only", choices=["fp64", "mixed", "mixed_jacobi", "fp64_multigrid"], default=None)
+    ap.add_argument("--inner-tol", type=float, default=None,
+                    help="floor of the MIXED inner solves' relative tolerance (sph_params.ppe_inner_tol)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a 1 GiB device-to-device copy kernel (for scaling HBM counters)")
     args = ap.parse_args()
@@ -527,8 +532,9 @@ def main():
               f"(multi-GPU halo exchange is design-only)", file=sys.stderr)
         sys.exit(2)
     args.warmup = max(1, args.warmup)
-    global GRAPHS
+    global GRAPHS, INNER_TOL
     GRAPHS = not args.no_graphs
+    INNER_TOL = args.inner_tol
 
     import abi
     import cases

@@ -510,6 +510,7 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     const bool compact = c->prm.precision == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions;
     d.crec = compact ? c->crec : nullptr;
     d.crec_adv = compact ? c->crec_adv : nullptr;
+    d.ibm_near = c->ibm_near_gen == c->generation ? c->ibm_near : nullptr;
     d.sc = e.d_sc + (size_t)k * SCAL_COUNT;
     d.ctl = e.d_ctl + (size_t)k * CTL_COUNT;
 }
@@ -536,6 +537,7 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
         const int prec = c->prm.precision;
         if (c->nmp_gen != c->generation || c->nmp_for != c->nm_list || c->nmp_prec != prec || c->prm.ale_positions)
             SPH_TRY(neumann_pairs(c));
+        SPH_TRY(ibm_near_build(c));   // the spreading targets of this search
     }
     // composition of the step: tables and graphs stay valid while it holds
     std::vector<long long> sig;

@@ -1182,6 +1182,10 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ibm_spread(Seg sg) {
     const NList nl = cd.nl;
     if (S.p_type[i] != 1) return;
     if (S.buffer_type[i] == 2) return;
+    if (cd.ibm_near && !cd.ibm_near[i]) {   // no marker in the list: what the sums below give
+        S.fbx[i] = 0.0; S.fby[i] = 0.0; S.fbz[i] = 0.0; S.concentration[i] = 0.0;
+        return;
+    }
     double xi = S.x[i], yi = S.y[i], zi = S.z[i];
     R tmp_h = (R)(0.5 * S.h[i]);
     R tmp_A = calc_tmpA<R>(kc, tmp_h);
@@ -1333,6 +1337,32 @@ static bool mixed_of(const Eng& e) { return e.cs[0]->prm.precision == SPH_MIXED;
 // which: 0 = step working copy (SP1), 1 = persistent state (P1)
 // Neumann pairs of the case's boundary particles for its current search,
 // in the precision of its steps.
+// 1 for a particle whose spreading list (the F_BH entries) holds an IB
+// marker (p_type 1000): a superset of the particles the spreading sums
+// reach, whatever the types of i and the markers' i_type become in the step
+__global__ void __launch_bounds__(BLOCK) k_ibm_near(int n, PView S, NList nl, unsigned char* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned char hit = 0;
+    NLRow row = nl_row(nl, i);
+    for (int k = 0; k < row.cnt && !hit; k++) {
+        const unsigned e = NB_AT(row, k);
+        if (e & F_BH) hit = S.p_type[e & NB_IDX] == 1000;
+    }
+    out[i] = hit;
+}
+int ibm_near_build(sph_ctx* c) {
+    if (c->ibm_near_gen == c->generation && c->ibm_near) return 0;
+    if (!c->ibm_near) SPH_TRY(dalloc(c, &c->ibm_near, (size_t)c->cap));
+    if (c->n > 0) {
+        hipLaunchKernelGGL(k_ibm_near, dim3(grid1(c->n)), dim3(BLOCK), 0, c->stream, c->n, c->sview(), c->nlist(),
+                           c->ibm_near);
+        SPH_TRY(launch_check(c, "ibm_near"));
+    }
+    c->ibm_near_gen = c->generation;
+    return 0;
+}
+
 int neumann_pairs(sph_ctx* c) {
     dfree(c, c->nmp_off);
     dfree(c, c->nmp_ent);

@@ -138,6 +138,7 @@ struct CaseDesc {
     float* crec;         // MIXED with positions still the uploaded floats: compact records (CREC_FLOATS per
                          // particle) of the pressure force, or null
     float* crec_adv;     // the same case's compact records of advection and assembly (CREC_ADV_FLOATS), or null
+    const unsigned char* ibm_near;   // per particle: 1 if an IB marker is in its spreading list (this search)
 };
 // CTL_MG: an FP64 solve whose p-hat and s-hat come from the float V-cycle
 // (the double p and s are staged as float for it and widened after)
@@ -369,6 +370,8 @@ struct sph_ctx {
     long long* nl_wprev = nullptr;   // slice widths of the last search (room for the next)
     int* porder = nullptr;           // order of the particle passes (CaseDesc::porder), or null
     int n_heavy = 0;                 // particles of that search with lists of more than 512 entries
+    unsigned char* ibm_near = nullptr;   // per particle of the search: an IB marker in its spreading list
+    long long ibm_near_gen = -1;         // generation the flags were built for
     int nl_wprev_n = 0;              // its slices
     int* scan_tmp = nullptr; size_t scan_tmp_n = 0;
 
@@ -477,6 +480,8 @@ enum { REC_ALL = 0, REC_VEL_PQ = 1, REC_PQ = 2, REC_PRE = 3 };
 int pack_records(Eng& e, int which);
 // Neumann pairs of a case for its current search and precision (physics.hip)
 int neumann_pairs(sph_ctx* c);
+// the IBM spreading targets of a case's current search (physics.hip)
+int ibm_near_build(sph_ctx* c);
 // aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
 struct MGState;
 constexpr int MG_COARSE_MAX = 1024;   // unknowns of the densely inverted coarsest level
