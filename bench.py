@@ -44,6 +44,7 @@ def log(*a):
 
 GRAPHS = True   # --no-graphs: launch every BiCGSTAB kernel directly (for counter runs)
 INNER_TOL = None   # --inner-tol: floor of the MIXED inner solves' relative tolerance (library default if None)
+PARAM_SET = {}     # --set name=value: further sph_params of the headline legs (library defaults otherwise)
 
 
 def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, dt, tol, precond=0):
@@ -51,6 +52,7 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
     prm.update(ppe_precond=precond)
     if INNER_TOL is not None:
         prm.update(ppe_inner_tol=INNER_TOL)
+    prm.update(**PARAM_SET)
     case = urbansph.Case(prm, len(P))
     if not GRAPHS:
         case.options(graphs=False, autotune=True)
@@ -493,6 +495,8 @@ def main():
     ap.add_argument("--only", choices=["fp64", "mixed", "mixed_jacobi", "fp64_multigrid"], default=None)
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="floor of the MIXED inner solves' relative tolerance (sph_params.ppe_inner_tol)")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="an sph_params member of the headline legs (int or float value), repeatable")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a 1 GiB device-to-device copy kernel (for scaling HBM counters)")
     args = ap.parse_args()
@@ -535,6 +539,12 @@ def main():
     global GRAPHS, INNER_TOL
     GRAPHS = not args.no_graphs
     INNER_TOL = args.inner_tol
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        try:
+            PARAM_SET[k] = int(v, 0)
+        except ValueError:
+            PARAM_SET[k] = float(v)
 
     import abi
     import cases

@@ -214,6 +214,24 @@ typedef struct sph_params {
                                    kernels (term order, double sums) instead
                                    of the fixed-point hash; 0 (default) only
                                    rows of more than 256 distinct columns     */
+    int    tile_cells;          /* MIXED passes over float-exact positions
+                                   (prep, advection, pressure force, PPE
+                                   assembly): blocks of cells whose particles
+                                   one workgroup runs with the neighbours'
+                                   records staged in LDS, x | y << 8 | z << 16
+                                   cells (1..16 each); 0 gathers each record
+                                   from memory.  Results are the same bitwise.
+                                   Default 0: on the MI355X the per-particle
+                                   gathers of the compact records run faster
+                                   (the staged halo limits residency to one or
+                                   two workgroups a CU, and its slot reads
+                                   conflict in the LDS banks)                 */
+    int    tile_particles;      /* particles of one workgroup of those passes
+                                   (64, 128, 192 or 256; a block of more is
+                                   split evenly; 0 means 128).  Default 128   */
+    int    tile_halo_max;       /* most distinct neighbours a workgroup stages
+                                   (1..2048); a workgroup with more gathers
+                                   from memory; 0 means 1536).  Default 1536  */
 } sph_params;
 
 /* Sources of the IB marker normals (ib_normals). */
@@ -302,6 +320,12 @@ typedef struct sph_ppe_report {
     int    heavy_lists;         /* particles of the last search with more than
                                    512 neighbour-list entries (the record
                                    passes run them last)                     */
+    int    tile_chunks;         /* workgroups of the LDS-staged passes (0: the
+                                   step gathered per particle)               */
+    int    tile_raw_chunks;     /* of those, workgroups whose halo exceeded
+                                   tile_halo_max (memory gathers)            */
+    int    tile_halo_largest;   /* largest staged halo (records)             */
+    long long tile_halo_total;  /* staged records of all workgroups          */
 } sph_ppe_report;
 
 /* Why the multigrid preconditioner was not used (sph_ppe_report.mg_status). */

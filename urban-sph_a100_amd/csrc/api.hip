@@ -209,6 +209,8 @@ void sph_params_default(sph_params* p) {
     p->ib_normals = SPH_IBN_REFERENCE;
     p->ib_face_xlo = 0.764; p->ib_face_xhi = 0.836; p->ib_face_ylo = -0.036; p->ib_face_yhi = 0.036;
     p->ib_face_zlo = 0.0; p->ib_face_zhi = 0.152; p->ib_face_tol = 1e-5;
+    // LDS-staged record passes: off (measured slower than the per-particle gathers)
+    p->tile_cells = 0; p->tile_particles = 128; p->tile_halo_max = 1536;
 }
 
 static int validate(const sph_params* p, std::string& why) {
@@ -229,6 +231,17 @@ static int validate(const sph_params* p, std::string& why) {
     if (p->most_mode != SPH_MOST_REFERENCE && p->most_mode != SPH_MOST_PRESCRIBED_L) { why = "most_mode"; return SPH_E_ARG; }
     if (p->ppe_precond != SPH_PRECOND_JACOBI && p->ppe_precond != SPH_PRECOND_MULTIGRID) { why = "ppe_precond"; return SPH_E_ARG; }
     if (p->ib_normals < SPH_IBN_REFERENCE || p->ib_normals > SPH_IBN_UPLOADED) { why = "ib_normals"; return SPH_E_ARG; }
+    if (p->tile_cells != 0) {
+        for (int a = 0; a < 3; a++) {
+            const int v = (p->tile_cells >> (8 * a)) & 0xff;
+            if (v < 1 || v > 16) { why = "tile_cells: 1..16 cells an axis"; return SPH_E_ARG; }
+        }
+        if (p->tile_cells >> 24) { why = "tile_cells"; return SPH_E_ARG; }
+    }
+    if (p->tile_particles != 0 && (p->tile_particles < 64 || p->tile_particles > 256 || p->tile_particles % 64)) {
+        why = "tile_particles: 64, 128, 192 or 256"; return SPH_E_ARG;
+    }
+    if (p->tile_halo_max < 0 || p->tile_halo_max > 2048) { why = "tile_halo_max: 1..2048"; return SPH_E_ARG; }
     return 0;
 }
 
@@ -609,6 +622,7 @@ void sph_destroy(sph_ctx* c) {
     for (auto p : c->s2) dfree(c, p);
     for (auto p : c->q) dfree(c, p);
     for (auto p : c->snap) dfree(c, p);
+    tile_release(c);
     void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->cstart, c->nm_list, c->nl_slice_off, c->nl_wprev, c->porder, c->nl_cnt,
                     c->nl_ent, c->stat_buf, c->apr_target, c->xnew[0], c->xnew[1], c->xnew[2], c->rec, c->crec, c->crec_adv, c->ibm_near, c->nmp_off, c->nmp_ent};
     for (auto p : rest) dfree(c, p);

@@ -244,9 +244,10 @@ int shape(Eng& e, int kind, int block, Shape* out) {
     auto units = [&](int k) -> long long {
         const sph_ctx* c = e.cs[k];
         // kind 0: particles; 1: PPE rows; 2: one 64-lane wave per long PPE row;
-        // 3: one 64-lane wave per boundary particle of the Neumann pass
+        // 3: one 64-lane wave per boundary particle of the Neumann pass;
+        // 4: the lanes of the cell-tile chunks (a block per chunk)
         return kind == 0 ? c->n : kind == 1 ? ppe_view(c).nr : kind == 2 ? 64LL * ppe_view(c).nlong
-                                                                           : 64LL * c->nm_count;
+             : kind == 3 ? 64LL * c->nm_count : (long long)c->tl_nchunk * c->tl_lanes;
     };
     out->block = block;
     if (nc == 1 && kind != 0) {
@@ -355,6 +356,23 @@ int shape(Eng& e, int kind, int block, Shape* out) {
     t->valid = true;
     out->grid = t->nblk;
     out->tab = t->d;
+    return 0;
+}
+
+// dynamic LDS of a cell-tile launch: the largest staged halo of the step's
+// cases, recf floats a record
+size_t tile_lds_bytes(const Eng& e, int recf) {
+    int h = 0;
+    for (const sph_ctx* c : e.cs) h = std::max(h, c->tl_hbig);
+    return (size_t)h * recf * sizeof(float);
+}
+// launches of more than 64 KB of dynamic LDS are allowed per kernel
+int tile_lds_allow(Eng& e, const void* kern, size_t bytes) {
+    if (bytes <= 65536) return 0;
+    auto it = e.lds_allowed.find(kern);
+    if (it != e.lds_allowed.end() && it->second >= bytes) return 0;
+    SPH_HIP(&e, hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    e.lds_allowed[kern] = bytes;
     return 0;
 }
 
@@ -511,6 +529,9 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.crec = compact ? c->crec : nullptr;
     d.crec_adv = compact ? c->crec_adv : nullptr;
     d.ibm_near = c->ibm_near_gen == c->generation ? c->ibm_near : nullptr;
+    d.tile = TileView{};
+    if (compact && c->tl_gen == c->generation && c->tl_part && c->tl_nchunk > 0)
+        d.tile = TileView{c->tl_part, c->tl_chunk, c->tl_halo, c->tl_self, NList{c->tl_slice_off, c->tl_cnt, c->tl_ent}};
     d.sc = e.d_sc + (size_t)k * SCAL_COUNT;
     d.ctl = e.d_ctl + (size_t)k * CTL_COUNT;
 }
@@ -538,6 +559,7 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
         if (c->nmp_gen != c->generation || c->nmp_for != c->nm_list || c->nmp_prec != prec || c->prm.ale_positions)
             SPH_TRY(neumann_pairs(c));
         SPH_TRY(ibm_near_build(c));   // the spreading targets of this search
+        if (prec == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions) SPH_TRY(tile_build(c));
     }
     // composition of the step: tables and graphs stay valid while it holds
     std::vector<long long> sig;
@@ -570,9 +592,17 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
     for (int k = 0; k < nc; k++) fill_desc(e, k, cases[k], count[k], dt[k], time[k]);
     SPH_HIP(&e, hipMemcpyAsync(e.d_desc, e.h_desc, sizeof(CaseDesc) * nc, hipMemcpyHostToDevice, e.st));
     SPH_HIP(&e, hipEventRecord(e.up_ev, e.st));
+    // cell tiles: every case of the step has them, with one chunk width
+    e.tile_lanes = 0;
+    for (int k = 0; k < nc; k++) {
+        const int w = e.h_desc[k].tile.part ? cases[k]->tl_lanes : 0;
+        if (k == 0) e.tile_lanes = w;
+        else if (w != e.tile_lanes) e.tile_lanes = 0;
+    }
     Shape sh;
     SPH_TRY(shape(e, 0, BLOCK, &sh));
     SPH_TRY(shape(e, 1, BLOCK, &sh));
+    if (e.tile_lanes) SPH_TRY(shape(e, 4, e.tile_lanes, &sh));
     SPH_TRY(run_half(e, 0, step_physics_front_key(e), step_physics_front));
     for (int k = 0; k < nc; k++) cases[k]->gc_gen = cases[k]->generation;
     const int prc = ppe_solve(e);

@@ -40,6 +40,7 @@ namespace sph {
     if (t_ >= cd.n) return;                               \
     const int i = cd.porder ? cd.porder[t_] : t_;
 
+
 // Particles of the cells (i0..i1, cj, ck) that lie inside the grid, as one
 // range [rg.x, rg.y) of the sorted order; false when the row is outside the
 // grid or the clipped x-range is empty.
@@ -66,11 +67,6 @@ __device__ __forceinline__ void ball_bins(const sph_domain& d, double px, double
               bin_axis((float)(pz + r + m), d.z_min, d.z_max, d.NK)};
 }
 
-struct NLRow { const unsigned* p; int cnt; };
-__device__ __forceinline__ NLRow nl_row(const NList& nl, int i) {
-    return {nl.ent + nl.slice_off[i >> 6] + (i & 63), nl.cnt[i]};
-}
-#define NB_AT(row, k) ((row).p[(size_t)(k) * SLICE])
 
 // ------------------------------------------------------------------------
 // Step start: SP1 <- P1 and SP2 <- P2 for the mutable fields, P3 <- 0
@@ -211,15 +207,13 @@ __global__ void __launch_bounds__(BLOCK) k_pack_records(Seg sg) {
     r[6] = make_double2(S.temp[i], S.concn[i]);
 }
 
-template <class R>
-__global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
-    SEG_PARTICLE(sg);
+template <class R, class Src>
+__device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src) {
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const PView S = cd.S;
     const P2View S2 = cd.S2;
     const QView Q = cd.Q;
-    const NList nl = cd.nl;
     const int count = cd.count;
     const double dt = cd.dt;
     if (S.i_type[i] > 2) return;
@@ -238,8 +232,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
     double st11 = 0, st12 = 0, st22 = 0, st23 = 0, st31 = 0, st33 = 0;
     const bool filt = (count % pr.freq_filt) == 0;
     const bool les = (pr.fv_solve != 0) && (pr.turbulence_model != SPH_LAMINAR);
-    const float* const crec = sizeof(R) == 4 ? cd.crec_adv : nullptr;
-    NLRow row = nl_row(nl, i);
+    const bool compact = Src::lds || (sizeof(R) == 4 && src.ok());
+    const NLRow row = nl_row(nl, li);
     for (int k = 0; k < row.cnt; k++) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_HP)) continue;
@@ -248,11 +242,12 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
         // (the same values as the field arrays hold) or the field arrays
         double xj, yj, zj, mj_d, rhoj_d;
         int ptj;
-        const float* C = crec ? crec + (size_t)j * CREC_ADV_FLOATS : nullptr;
-        if (C) {
-            const rec_f4 a = ld_f4(C), b = ld_f4(C + 4);
+        const float* C = compact ? src.at(j) : nullptr;
+        if (compact) {
+            const rec_f4 a = Src::f4(C), b = Src::f4(C + 4);
             xj = a.x; yj = a.y; zj = a.z; mj_d = a.w; rhoj_d = b.x; ptj = crec_ptype(b.w);
         } else {
+            if constexpr (Src::lds) continue;   // staged records are compact
             xj = S.x[j]; yj = S.y[j]; zj = S.z[j]; mj_d = S.m[j]; rhoj_d = S.rho[j]; ptj = S.p_type[j];
         }
         R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
@@ -271,8 +266,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
                 R tdwyc = gc12 * tdwx + gc22 * tdwy + gc23 * tdwz;
                 R tdwzc = gc31 * tdwx + gc23 * tdwy + gc33 * tdwz;
                 double uxj, uyj, uzj;
-                if (C) {
-                    const rec_d2 u = ld_d2(C + 8), w = ld_d2(C + 12);
+                if (compact) {
+                    const rec_d2 u = Src::d2(C + 8), w = Src::d2(C + 12);
                     uxj = u.x; uyj = u.y; uzj = w.x;
                 } else {
                     uxj = S.ux[j]; uyj = S.uy[j]; uzj = S.uz[j];
@@ -310,6 +305,18 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
     }
     if (count == 0) S2.rho_ref[i] = S.rho[i];
     if (filt) S.flt_s[i] = tmp_flt;
+}
+template <class R>
+__global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
+    SEG_PARTICLE(sg);
+    prep_body<R>(cd, i, cd.nl, i, GSrc<CREC_ADV_FLOATS>{sizeof(R) == 4 ? &cd.crec_adv : nullptr});
+}
+__global__ void __launch_bounds__(TILE_MAX) k_prep_tile(Seg sg) {
+    TILE_PARTICLE(sg, cd.crec_adv, CREC_ADV_FLOATS);
+    if (chunk_.y >= 0)
+        prep_body<float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+                                             cd.tile.self[lane_]});
+    else prep_body<float>(cd, i, cd.tile.nl, lane_, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv});
 }
 
 // ------------------------------------------------------------------------
@@ -725,17 +732,14 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
 // SCAL: some case of the launch transports temperature or concentration or
 // uses the Boussinesq buoyancy; without them (the canyon) those terms and
 // their registers are compiled out (130 -> fewer VGPRs, more waves per SIMD)
-template <class R, bool SCAL>
-__global__ void __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(SCAL ? 1 : 5)))
-k_advection(Seg sg) {
-    SEG_PARTICLE_ORDERED(sg);
+template <class R, bool SCAL, class Src>
+__device__ __forceinline__ void advection_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src) {
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const bool con_solve = SCAL && pr.con_solve, concn_solve = SCAL && pr.concn_solve;
     const bool boussinesq_solve = SCAL && pr.boussinesq_solve;
     const PView S = cd.S;
     const QView Q = cd.Q;
-    const NList nl = cd.nl;
     if (S.i_type[i] > 2) return;
     int ptypei = S.p_type[i];
     if (ptypei == 1000) return;
@@ -759,9 +763,8 @@ k_advection(Seg sg) {
     double eulerx = 0, eulery = 0, eulerz = 0, eulert = 0, tmp_temp = 0, tmp_flt = 0;
     const R Alpha = (R)pr.av_alpha, Beta = (R)pr.av_beta, soundspeed = (R)pr.av_c;
     const R ome = (R)(1 - eli);
-    const float* const crec = sizeof(R) == 4 ? cd.crec_adv : nullptr;
-    const bool compact = crec != nullptr;
-    NLRow row = nl_row(nl, i);
+    const bool compact = Src::lds || (sizeof(R) == 4 && src.ok());
+    const NLRow row = nl_row(nl, li);
     for (int k = 0; k < row.cnt; k++) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_DP)) continue;
@@ -769,16 +772,17 @@ k_advection(Seg sg) {
         double hj_d, visj_d, xj, yj, zj, mj_d, uxj, uyj, uzj, rhoj_d;
         int ptypej;
         if (compact) {
-            const float* C = crec + (size_t)j * CREC_ADV_FLOATS;
-            const rec_f4 b = ld_f4(C + 4);
+            const float* C = src.at(j);
+            const rec_f4 b = Src::f4(C + 4);
             ptypej = crec_ptype(b.w);
             if (ptypej >= 1000) continue;
-            const rec_f4 a = ld_f4(C);
-            const rec_d2 u = ld_d2(C + 8), w = ld_d2(C + 12);
+            const rec_f4 a = Src::f4(C);
+            const rec_d2 u = Src::d2(C + 8), w = Src::d2(C + 12);
             xj = a.x; yj = a.y; zj = a.z; mj_d = a.w;
             rhoj_d = b.x; hj_d = b.y; visj_d = b.z;
             uxj = u.x; uyj = u.y; uzj = w.x;
         } else {
+            if constexpr (Src::lds) continue;   // staged records are compact
             double vis_tj, pqj_;
             int itj_;
             rec_aux(cd.rec, j, vis_tj, ptypej, itj_);
@@ -798,7 +802,7 @@ k_advection(Seg sg) {
         R dux = (R)(uxi - uxj), duy = (R)(uyi - uyj), duz = (R)(uzi - uzj);   // u_i - u_j
         double tempj = 0.0, concnj_d = 0.0;
         if (con_solve || concn_solve || boussinesq_solve) {
-            if (compact) { tempj = S.temp[j]; concnj_d = S.concn[j]; }
+            if (compact) { tempj = S.temp[src.gidx(j)]; concnj_d = S.concn[src.gidx(j)]; }
             else rec_scalar(cd.rec, j, tempj, concnj_d);
         }
         if (pr.fv_solve == SPH_FV_CLEARY) {
@@ -878,6 +882,20 @@ k_advection(Seg sg) {
     if (con_solve) Q.dtemp[i] += tmp_Rc / (double)cpi - eulert;
     Q.ftotal[i] = sqrt(tmpx * tmpx + tmpy * tmpy + tmpz * tmpz);
     if (concn_solve) Q.dconcn[i] = tmp_Rd;
+}
+template <class R, bool SCAL>
+__global__ void __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(SCAL ? 1 : 4)))
+k_advection(Seg sg) {
+    SEG_PARTICLE_ORDERED(sg);
+    advection_body<R, SCAL>(cd, i, cd.nl, i, GSrc<CREC_ADV_FLOATS>{sizeof(R) == 4 ? &cd.crec_adv : nullptr});
+}
+template <bool SCAL>
+__global__ void __launch_bounds__(TILE_MAX) k_advection_tile(Seg sg) {
+    TILE_PARTICLE(sg, cd.crec_adv, CREC_ADV_FLOATS);
+    if (chunk_.y >= 0)
+        advection_body<float, SCAL>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+                                                         cd.tile.self[lane_]});
+    else advection_body<float, SCAL>(cd, i, cd.tile.nl, lane_, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv});
 }
 
 // ------------------------------------------------------------------------
@@ -1050,15 +1068,15 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_outlet(Seg sg, int which) {
 }
 
 // ------------------------------------------------------------------------
-// KERNEL_pressureforce3D, functions_PPE.cuh:482-647
-template <class R>
-__global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
-    SEG_PARTICLE_ORDERED(sg);
+// KERNEL_pressureforce3D, functions_PPE.cuh:482-647.  row: i's neighbour
+// entries; src: where their records are (compact records in MIXED).
+template <class R, class Src>
+__device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, const NList& nl, int li,
+                                                    const Src src) {
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const PView S = cd.S;
     const QView Q = cd.Q;
-    const NList nl = cd.nl;
     if (S.i_type[i] > 2) return;
     if (S.p_type[i] >= 1000) return;
     if (S.p_type[i] <= 0) return;
@@ -1068,17 +1086,16 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
     double pi_d = S.pres[i] - S.pres0[i];
     R pi = (R)pi_d, rhoi = (R)S.rho[i];
     double tmpx = 0, tmpy = 0, tmpz = 0;
-    NLRow row = nl_row(nl, i);
-    if (sizeof(R) == 4 && cd.crec) {
+    const NLRow row = nl_row(nl, li);
+    if (sizeof(R) == 4 && src.ok()) {
         // the same sums from the compact float records
-        const float* C = cd.crec;
         for (int k = 0; k < row.cnt; k++) {
             unsigned e = NB_AT(row, k);
             if (!(e & F_DM)) continue;
-            int j = e & NB_IDX;
-            const rec_f4 b = ld_f4(C + (size_t)j * CREC_FLOATS + 4);
+            const float* C = src.at(e & NB_IDX);
+            const rec_f4 b = Src::f4(C + 4);
             if (__float_as_int(b.w) >= 1000) continue;
-            const rec_f4 a = ld_f4(C + (size_t)j * CREC_FLOATS);
+            const rec_f4 a = Src::f4(C);
             R dx = (R)(xi - (double)a.x), dy = (R)(yi - (double)a.y), dz = (R)(zi - (double)a.z);
             R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
             R hij = (R)(0.5 * (hi_d + (double)b.z));
@@ -1097,6 +1114,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
         Q.fpx[i] = tmpx; Q.fpy[i] = tmpy; Q.fpz[i] = tmpz;
         return;
     }
+    if constexpr (Src::lds) return;   // staged records are compact
     for (int k = 0; k < row.cnt; k++) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_DM)) continue;
@@ -1122,6 +1140,18 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
         }
     }
     Q.fpx[i] = tmpx; Q.fpy[i] = tmpy; Q.fpz[i] = tmpz;
+}
+template <class R>
+__global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
+    SEG_PARTICLE_ORDERED(sg);
+    pressure_force_body<R>(cd, i, cd.nl, i, GSrc<CREC_FLOATS>{sizeof(R) == 4 ? &cd.crec : nullptr});
+}
+__global__ void __launch_bounds__(TILE_MAX) k_pressure_force_tile(Seg sg) {
+    TILE_PARTICLE(sg, cd.crec, CREC_FLOATS);
+    if (chunk_.y >= 0)
+        pressure_force_body<float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+                                                      cd.tile.self[lane_]});
+    else pressure_force_body<float>(cd, i, cd.tile.nl, lane_, GSrc<CREC_FLOATS>{&cd.crec});
 }
 
 // ------------------------------------------------------------------------
@@ -1317,6 +1347,21 @@ __global__ void k_stats_final(const unsigned* in, float* out) {
 
 // launch over the particles of every case; tunable passes (kid >= 0) run at
 // the workgroup size the autotuner picked for this batch size
+// the cell-tile variant of a record pass: a workgroup per chunk, the halo
+// records (recf floats each) in dynamic LDS
+#define TILEL(name, kern, recf)                                                                      \
+    do {                                                                                            \
+        Shape sh_;                                                                                  \
+        SPH_TRY(shape(e, 4, e.tile_lanes, &sh_));                                                   \
+        if (sh_.grid > 0) {                                                                         \
+            const size_t lds_ = tile_lds_bytes(e, (recf));                                          \
+            SPH_TRY(tile_lds_allow(e, (const void*)(kern), lds_));                                  \
+            SPH_TRY(prof_begin(e, name, 0.0));                                                      \
+            hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(e.tile_lanes), lds_, e.st, Seg{e.d_desc, sh_.tab}); \
+            SPH_TRY(launch_check(&e, name));                                                        \
+            SPH_TRY(prof_end(e));                                                                   \
+        }                                                                                           \
+    } while (0)
 #define SEGL(name, kid, kern, ...)                                                                   \
     do {                                                                                            \
         const int blk_ = (kid) >= 0 ? tuned_block(e, (kid)) : BLOCK;                                \
@@ -1505,12 +1550,13 @@ long long step_physics_front_key(const Eng& e) {
         scal |= c->prm.con_solve || c->prm.concn_solve || c->prm.boussinesq_solve;
         compact |= e.h_desc[k].crec_adv != nullptr;
     }
-    return key | (count0 ? 2 : 0) | (gc ? 4 : 0) | (nrm ? 8 : 0) | (scal ? 16 : 0) | (compact ? 32 : 0);
+    return key | (count0 ? 2 : 0) | (gc ? 4 : 0) | (nrm ? 8 : 0) | (scal ? 16 : 0) | (compact ? 32 : 0) |
+           (e.tile_lanes ? 64 : 0);
 }
 long long step_physics_back_key(const Eng& e) {
     bool ale = false;
     for (const sph_ctx* c : e.cs) ale |= c->prm.ale_positions != 0;
-    return (mixed_of(e) ? 1 : 0) | (ale ? 2 : 0);
+    return (mixed_of(e) ? 1 : 0) | (ale ? 2 : 0) | (e.tile_lanes ? 4 : 0);
 }
 
 // Everything of SOPHIA_single_ISPH before the pressure solve
@@ -1533,7 +1579,8 @@ int step_physics_front(Eng& e) {
         else SEGL("grad_corr", TK_GRAD_CORR, k_grad_corr<double>);
     }
     SPH_TRY(pack_records(e, REC_PRE));   // compact records for the strain pass
-    if (mixed) SEGL("prep", TK_PREP, k_prep<float>);
+    if (mixed && e.tile_lanes) TILEL("prep", k_prep_tile, CREC_ADV_FLOATS);
+    else if (mixed) SEGL("prep", TK_PREP, k_prep<float>);
     else SEGL("prep", TK_PREP, k_prep<double>);
     if (mixed) SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<float>);
     else SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<double>);
@@ -1544,7 +1591,10 @@ int step_physics_front(Eng& e) {
     bool scal = false;   // scalar transport or buoyancy in some case of the step
     for (sph_ctx* c : e.cs) scal |= c->prm.con_solve || c->prm.concn_solve || c->prm.boussinesq_solve;
     if (mixed) {
-        if (scal) SEGL("advection", TK_ADVECTION, (k_advection<float, true>));
+        if (e.tile_lanes) {
+            if (scal) TILEL("advection", k_advection_tile<true>, CREC_ADV_FLOATS);
+            else TILEL("advection", k_advection_tile<false>, CREC_ADV_FLOATS);
+        } else if (scal) SEGL("advection", TK_ADVECTION, (k_advection<float, true>));
         else SEGL("advection", TK_ADVECTION, (k_advection<float, false>));
     } else {
         if (scal) SEGL("advection", TK_ADVECTION, (k_advection<double, true>));
@@ -1562,7 +1612,8 @@ int step_physics_back(Eng& e) {
     SPH_TRY(run_neumann(e, 0, 0));
     SPH_TRY(run_outlet(e, 0));
     SPH_TRY(pack_records(e, REC_PQ));   // pressures after the solve and the boundary passes
-    if (mixed) SEGL("pressure_force", TK_PRESSURE, k_pressure_force<float>);
+    if (mixed && e.tile_lanes) TILEL("pressure_force", k_pressure_force_tile, CREC_FLOATS);
+    else if (mixed) SEGL("pressure_force", TK_PRESSURE, k_pressure_force<float>);
     else SEGL("pressure_force", TK_PRESSURE, k_pressure_force<double>);
     if (mixed) SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<float>);
     else SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<double>);

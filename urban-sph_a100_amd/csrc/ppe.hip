@@ -417,21 +417,15 @@ __device__ __forceinline__ KSet<float> kset(const PPEView& v, float*) {
 
 // ------------------------------------------------------------------------
 // per-step assembly, functions_PPE.cuh:2-160
-template <class R, class V>
-// Launched over particles (not rows): a particle's neighbour list is read
-// in particle order, coalesced across the wave, whatever order the rows
-// are numbered in.
-__global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) {
-    SEG_ROW(sg);
+template <class R, class V, class Src>
+__device__ __forceinline__ void assemble_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src,
+                                              int jacobi) {
     const PPEView pp = cd.ppe;
-    if (row >= cd.n) return;
-    const int i = cd.porder ? cd.porder[row] : row;   // this thread's particle
     const int r = pp.rowof[i];
     if (r < 0) return;
     const KConst kc = cd.kc;
     const PView S = cd.S;
     const QView Q = cd.Q;
-    const NList nl = cd.nl;
     const double dt = cd.dt;
     const int* rowof = pp.rowof;
     R hi = (R)(1.0 * S.h[i]);
@@ -442,32 +436,33 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
     R gc11 = (R)Q.inv_xx[i], gc12 = (R)Q.inv_xy[i], gc22 = (R)Q.inv_yy[i];
     R gc23 = (R)Q.inv_yz[i], gc31 = (R)Q.inv_zx[i], gc33 = (R)Q.inv_zz[i];
     double bix = 0, biy = 0, biz = 0, Aij = 0, AijPij = 0, kn = 0;
-    const unsigned* nrow = nl.ent + nl.slice_off[i >> 6] + (i & 63);
-    int m = nl.cnt[i];
+    const NLRow nr = nl_row(nl, li);
+    const int m = nr.cnt;
     // with fixed particles the values of a search are written by its first
     // step only; later steps compute the right-hand side
     V* out = (jacobi || cd.vals_fresh) ? nullptr : wvals_of(pp, (V*)nullptr) + prow(pp.sl_off, r);
     const unsigned char* smap = pp.slotmap ? pp.slotmap + prow(pp.sl_off, r) : nullptr;
     int slot = 0;
     // read once: the stores of the loop could alias the descriptor
-    const float* const crec = sizeof(R) == 4 ? cd.crec_adv : nullptr;
+    const bool compact = Src::lds || (sizeof(R) == 4 && src.ok());
     const double* const rec = cd.rec;
     for (int k = 0; k < m; k++) {
-        unsigned e = nrow[(size_t)k * SLICE];
+        unsigned e = NB_AT(nr, k);
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
         double xj, yj, zj, mj_d, rhoj_d, qj, uxj, uyj, uzj;
         bool jrow;   // j is an unknown of the PPE
-        if (crec) {   // the compact record (positions float-exact)
-            const float* C = crec + (size_t)j * CREC_ADV_FLOATS;
-            const rec_f4 b = ld_f4(C + 4);
+        if (compact) {   // the compact record (positions float-exact)
+            const float* C = src.at(j);
+            const rec_f4 b = Src::f4(C + 4);
             if (crec_ptype(b.w) >= 1000 || crec_itype(b.w) == 4) continue;
             jrow = crec_has_row(b.w);
-            const rec_f4 a = ld_f4(C);
-            const rec_d2 u = ld_d2(C + 8), w = ld_d2(C + 12);
+            const rec_f4 a = Src::f4(C);
+            const rec_d2 u = Src::d2(C + 8), w = Src::d2(C + 12);
             xj = a.x; yj = a.y; zj = a.z; mj_d = a.w; rhoj_d = b.x;
             uxj = u.x; uyj = u.y; uzj = w.x; qj = w.y;
         } else {
+            if constexpr (Src::lds) continue;   // staged records are compact
             double vtj_, hj_;
             int ptj, itj;
             rec_aux(rec, j, vtj_, ptj, itj);
@@ -494,7 +489,7 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         double ccd = (double)cc;
         Aij += ccd;
         if (jacobi) AijPij += ccd * qj;
-        if (j != i) {
+        if (!src.self(j, i)) {
             if (jrow) {
                 if (out) {
                     if (slot < LONG_ROW || !pp.ltail) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
@@ -518,6 +513,20 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
         pp.rhs[r] = bi + kn;
         pp.x[r] = q0;
     }
+}
+template <class R, class V>
+__global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) {
+    SEG_ROW(sg);
+    if (row >= cd.n) return;
+    const int i = cd.porder ? cd.porder[row] : row;   // this thread's particle
+    assemble_body<R, V>(cd, i, cd.nl, i, GSrc<CREC_ADV_FLOATS>{sizeof(R) == 4 ? &cd.crec_adv : nullptr}, jacobi);
+}
+__global__ void __launch_bounds__(TILE_MAX) k_ppe_assemble_tile(Seg sg, int jacobi) {
+    TILE_PARTICLE(sg, cd.crec_adv, CREC_ADV_FLOATS);
+    if (chunk_.y >= 0)
+        assemble_body<float, float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+                                                        cd.tile.self[lane_]}, jacobi);
+    else assemble_body<float, float>(cd, i, cd.tile.nl, lane_, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv}, jacobi);
 }
 
 // pressure <- Jacobi sweep output, or p0 + solved increment
@@ -1374,6 +1383,12 @@ int ppe_solve(Eng& e) {
         c->rep.index16 = c->ppe && c->ppe->ecols ? 1 : 0;
         c->rep.tail_rows = c->ppe ? c->ppe->nlong : 0;
         c->rep.heavy_lists = c->n_heavy;
+        if (e.tile_lanes) {
+            c->rep.tile_chunks = c->tl_nchunk;
+            c->rep.tile_raw_chunks = c->tl_raw;
+            c->rep.tile_halo_largest = c->tl_hbig;
+            c->rep.tile_halo_total = c->tl_halo_total;
+        }
         c->rep.precond = SPH_PRECOND_JACOBI;
         if (c->rep.unknowns == 0) c->rep.converged = 1;
         rows_total += c->rep.unknowns;
@@ -1381,7 +1396,17 @@ int ppe_solve(Eng& e) {
     }
     if (rows_total == 0) return 0;
     SPH_TRY(pack_records(e, REC_VEL_PQ));   // velocities after the predictor and its boundary passes
-    {
+    if (mixed && e.tile_lanes) {   // cell tiles: a workgroup per chunk
+        Shape sh;
+        SPH_TRY(shape(e, 4, e.tile_lanes, &sh));
+        const size_t lds = tile_lds_bytes(e, CREC_ADV_FLOATS);
+        SPH_TRY(tile_lds_allow(e, (const void*)k_ppe_assemble_tile, lds));
+        SPH_TRY(prof_begin(e, "ppe_assemble", asm_bytes));
+        hipLaunchKernelGGL(k_ppe_assemble_tile, dim3(sh.grid), dim3(e.tile_lanes), lds, e.st, Seg{e.d_desc, sh.tab},
+                           jacobi ? 1 : 0);
+        SPH_TRY(launch_check(&e, "ppe_assemble"));
+        SPH_TRY(prof_end(e));
+    } else {
         const int blk = tuned_block(e, TK_ASSEMBLE);
         Shape sh;
         SPH_TRY(shape(e, 0, blk, &sh));   // over particles

@@ -51,6 +51,23 @@ struct NList {
     const int* cnt;               // entries per particle
     const unsigned* ent;          // j | selection flags (flags.h)
 };
+// one particle's (or tile lane's) entries: entry k at p[k * SLICE]
+struct NLRow { const unsigned* p; int cnt; };
+__device__ __forceinline__ NLRow nl_row(const NList& nl, long long i) {
+    return {nl.ent + nl.slice_off[i >> 6] + (i & 63), nl.cnt[i]};
+}
+#define NB_AT(row, k) ((row).p[(size_t)(k) * SLICE])
+
+// Cell tiles of the record-gathering passes (tiles.hip): chunks of at most
+// tile_particles particles of one block of cells, each with its halo.
+constexpr int TILE_MAX = 256;   // most lanes of a tile chunk (the tile passes' workgroup size)
+struct TileView {
+    const int* part;     // lane -> particle (chunk c: lanes c * cap ..), -1 on padding
+    const int2* chunk;   // per chunk: first halo entry, halo size (-1: entries hold particle indices)
+    const int* halo;     // the chunks' halo particles, ascending within a chunk
+    const int* self;     // per lane: the halo slot of its own particle, or -1
+    NList nl;            // the lanes' neighbour lists: halo slot | selection flags
+};
 
 // Mutable P1 fields: the per-step working copy holds its own arrays for
 // these; the others alias the persistent state.
@@ -139,6 +156,7 @@ struct CaseDesc {
                          // particle) of the pressure force, or null
     float* crec_adv;     // the same case's compact records of advection and assembly (CREC_ADV_FLOATS), or null
     const unsigned char* ibm_near;   // per particle: 1 if an IB marker is in its spreading list (this search)
+    TileView tile;       // the cell tiles of this search (tile.part null: the passes gather per particle)
 };
 // CTL_MG: an FP64 solve whose p-hat and s-hat come from the float V-cycle
 // (the double p and s are staged as float for it and widened after)
@@ -170,6 +188,7 @@ __device__ __forceinline__ int seg_case(const Seg& s, int& first) {
     first = t.y;
     return t.x;
 }
+
 
 // launch tuning: the neighbour passes whose workgroup size the autotuner picks
 enum TuneKernel {
@@ -247,6 +266,78 @@ constexpr int CREC_ADV_FLOATS = 16;
 // type word: p_type in the low 16 bits, i_type in bits 16..29, bit 30 set
 // when the particle is an unknown of the case's PPE (it has a row)
 constexpr int CREC_ROW_BIT = 1 << 30;
+
+// Where a gathering pass reads its neighbour records: in memory, indexed by
+// particle (base null: the FP64 field arrays), or staged in LDS by a tile,
+// indexed by halo slot.
+template <int STRIDE>   // floats per record
+struct GSrc {
+    static constexpr bool lds = false;
+    float* const* ref;   // the descriptor's record pointer (read where the loop starts), or null
+    __device__ bool ok() const { return ref && *ref; }
+    __device__ const float* at(int j) const { return *ref + (size_t)j * STRIDE; }
+    __device__ static rec_f4 f4(const float* p) { return ld_f4(p); }
+    __device__ static rec_d2 d2(const float* p) { return ld_d2(p); }
+    __device__ int gidx(int j) const { return j; }           // particle of an entry
+    __device__ bool self(int j, int i) const { return j == i; }
+};
+template <int STRIDE>
+struct LSrc {
+    static constexpr bool lds = true;
+    const float* base;   // LDS
+    const int* halo;     // the chunk's halo: slot -> particle
+    int self_slot;       // the slot of this thread's particle, or -1
+    __device__ bool ok() const { return true; }
+    __device__ const float* at(int j) const { return base + j * STRIDE; }
+    __device__ static rec_f4 f4(const float* p) { return *(const __attribute__((address_space(3))) rec_f4*)p; }
+    __device__ static rec_d2 d2(const float* p) { return *(const __attribute__((address_space(3))) rec_d2*)p; }
+    __device__ int gidx(int j) const { return halo[j]; }
+    __device__ bool self(int j, int) const { return j == self_slot; }
+};
+
+// Cell-tile passes (tiles.hip): a workgroup per chunk, blockDim = the chunk
+// width.  Stages the chunk's halo records (RECF floats each, from RECS) in
+// LDS, then maps the thread to its lane (a row of cd.tile.nl) and the
+// lane's particle; padding lanes return after the barrier.
+// A thread copies whole records (Q float4 pieces), TILE_U of them a round,
+// all loads of a round issued before its stores.
+constexpr int TILE_U = 4;
+template <int Q>
+__device__ __forceinline__ void tile_stage_q(float4* lds, const float* recs, const int* halo, int cnt) {
+    const int B = blockDim.x;
+    for (int r0 = threadIdx.x; r0 < cnt; r0 += TILE_U * B) {
+        int idx[TILE_U];
+#pragma unroll
+        for (int u = 0; u < TILE_U; u++) idx[u] = r0 + u * B < cnt ? halo[r0 + u * B] : -1;
+        rec_f4 v[TILE_U][Q];
+#pragma unroll
+        for (int u = 0; u < TILE_U; u++)
+            if (idx[u] >= 0)
+#pragma unroll
+                for (int q = 0; q < Q; q++) v[u][q] = ld_f4(recs + ((size_t)idx[u] * Q + q) * 4);
+#pragma unroll
+        for (int u = 0; u < TILE_U; u++)
+            if (idx[u] >= 0)
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+                    lds[(r0 + u * B) * Q + q] = make_float4(v[u][q].x, v[u][q].y, v[u][q].z, v[u][q].w);
+    }
+}
+__device__ __forceinline__ void tile_stage(float4* lds, const float* recs, const int* halo, int cnt, int recf) {
+    if (recf == 8) tile_stage_q<2>(lds, recs, halo, cnt);
+    else tile_stage_q<4>(lds, recs, halo, cnt);
+}
+#define TILE_PARTICLE(sg, RECS, RECF)                                                            \
+    extern __shared__ float4 tile_lds_[];                                                       \
+    int first_;                                                                                 \
+    const int cid_ = seg_case((sg), first_);                                                    \
+    const CaseDesc& cd = (sg).cd[cid_];                                                         \
+    const int2 chunk_ = cd.tile.chunk[first_ / (int)blockDim.x];                                \
+    if (chunk_.y > 0) tile_stage(tile_lds_, (RECS), cd.tile.halo + chunk_.x, chunk_.y, (RECF)); \
+    __syncthreads();                                                                            \
+    const int lane_ = first_ + (int)threadIdx.x;                                                \
+    const int i = cd.tile.part[lane_];                                                          \
+    if (i < 0) return;
 __device__ __forceinline__ int crec_ptype(float w) { return (int)(short)(__float_as_int(w) & 0xffff); }
 __device__ __forceinline__ int crec_itype(float w) { return (__float_as_int(w) >> 16) & 0x3fff; }
 __device__ __forceinline__ bool crec_has_row(float w) { return (__float_as_int(w) & CREC_ROW_BIT) != 0; }
@@ -301,6 +392,8 @@ struct Eng {
     // table it uses), the next one is captured, later ones replay it
     struct StepGraph { long long key = -1, seen = -1; hipGraphExec_t exec = nullptr; };
     StepGraph step_graph[2];
+    int tile_lanes = 0;   // this step's record passes run on cell tiles of this width (0: per particle)
+    std::unordered_map<const void*, size_t> lds_allowed;   // kernels allowed more than 64 KB of dynamic LDS
     // multigrid V-cycles of the step's cases (mg.hip): per case and level a
     // descriptor, and block tables of the level rows (mg_plan)
     void* mg_lv = nullptr;            // device MGLv[case * MG_MAX_LEVELS + level]
@@ -372,6 +465,15 @@ struct sph_ctx {
     int n_heavy = 0;                 // particles of that search with lists of more than 512 entries
     unsigned char* ibm_near = nullptr;   // per particle of the search: an IB marker in its spreading list
     long long ibm_near_gen = -1;         // generation the flags were built for
+    // cell tiles of the record passes (tiles.hip), built for tl_gen with the
+    // tile parameters tl_code / tl_cap / tl_hcap
+    int* tl_part = nullptr; int2* tl_chunk = nullptr; int* tl_halo = nullptr; int* tl_self = nullptr;
+    long long* tl_slice_off = nullptr; int* tl_cnt = nullptr; unsigned* tl_ent = nullptr;
+    int tl_nchunk = 0, tl_hbig = 0, tl_raw = 0;   // chunks, largest staged halo, chunks above the cap
+    long long tl_halo_total = 0;
+    int tl_code = 0, tl_cap = 0, tl_hcap = 0;     // the parameters they were built with
+    int tl_lanes = 0;                             // lanes (threads) of a chunk
+    long long tl_gen = -1;
     int nl_wprev_n = 0;              // its slices
     int* scan_tmp = nullptr; size_t scan_tmp_n = 0;
 
@@ -454,6 +556,8 @@ int shape(Eng& e, int kind, int block, Shape* out);
 int tuned_block(Eng& e, int kid);
 int tune_begin(Eng& e, int kid, int block);
 int tune_end(Eng& e);
+size_t tile_lds_bytes(const Eng& e, int recf);
+int tile_lds_allow(Eng& e, const void* kern, size_t bytes);
 
 // apr.hip
 int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out);
@@ -482,6 +586,9 @@ int pack_records(Eng& e, int which);
 int neumann_pairs(sph_ctx* c);
 // the IBM spreading targets of a case's current search (physics.hip)
 int ibm_near_build(sph_ctx* c);
+// tiles.hip: the cell tiles of a case's current search
+int tile_build(sph_ctx* c);
+void tile_release(sph_ctx* c);
 // aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
 struct MGState;
 constexpr int MG_COARSE_MAX = 1024;   // unknowns of the densely inverted coarsest level
