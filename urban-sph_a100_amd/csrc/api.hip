@@ -624,7 +624,7 @@ void sph_destroy(sph_ctx* c) {
     for (auto p : c->snap) dfree(c, p);
     tile_release(c);
     void* rest[] = {c->keys, c->perm, c->keys_tmp, c->perm_tmp, c->gstr, c->gend, c->cstart, c->nm_list, c->nl_slice_off, c->nl_wprev, c->porder, c->nl_cnt,
-                    c->nl_ent, c->stat_buf, c->apr_target, c->xnew[0], c->xnew[1], c->xnew[2], c->rec, c->crec, c->crec_adv, c->ibm_near, c->nmp_off, c->nmp_ent};
+                    c->nl_ent, c->stat_buf, c->apr_target, c->xnew[0], c->xnew[1], c->xnew[2], c->rec, c->crec, c->crec_adv, c->ibm_near, c->nmp_off, c->nmp_ent, c->nmp_cent, c->nm_set, c->nm_rec};
     for (auto p : rest) dfree(c, p);
     dcache_release(c);
     for (auto& b : c->live) (void)hipFree(b.first);   // none expected: every buffer is freed above

@@ -508,6 +508,11 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.nm_count = c->nm_count;
     d.nmp_off = c->nmp_off;
     d.nmp_ent = c->nmp_ent;
+    const bool nmrec = c->prm.precision == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions && c->nm_rec;
+    d.nmp_cent = nmrec ? c->nmp_cent : nullptr;
+    d.nm_set = nmrec ? c->nm_set : nullptr;
+    d.nm_nset = nmrec ? c->nm_nset : 0;
+    d.nm_rec = nmrec ? c->nm_rec : nullptr;
     d.dt = dt;
     d.time = time;
     d.kc = KConst{c->prm.kernel_type, c->prm.kappa, c->c_pi15};

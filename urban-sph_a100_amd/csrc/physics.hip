@@ -104,11 +104,18 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_commit_positions(Seg sg) {
 }
 
 // Snapshot of the fields a copy/mirror pass reads while it writes them.
-__global__ void __launch_bounds__(MAX_BLOCK) k_snapshot(Seg sg, int which) {
+__global__ void __launch_bounds__(MAX_BLOCK) k_snapshot(Seg sg, int which, int nm_pack) {
     SEG_PARTICLE(sg);
     const PView A = which ? cd.P : cd.S;
     cd.snap[0][i] = A.ux[i]; cd.snap[1][i] = A.uy[i]; cd.snap[2][i] = A.uz[i];
     cd.snap[3][i] = A.pres[i]; cd.snap[4][i] = A.temp[i];
+    if (nm_pack && cd.nm_rec && i < cd.nm_nset) {   // the Neumann pass's records, from the same state
+        const int j = cd.nm_set[i];
+        float4* r = (float4*)(cd.nm_rec + (size_t)i * NMREC_FLOATS);
+        r[0] = make_float4((float)A.x[j], (float)A.y[j], (float)A.z[j], (float)A.m[j]);
+        r[1] = make_float4((float)A.rho[j], (float)A.ux[j], (float)A.uy[j], (float)A.uz[j]);
+        *(double2*)(r + 2) = make_double2(A.pres[j], 0.0);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -624,18 +631,29 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
         // same on every call, alone or in a batch
         double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         const long long e0 = cd.nmp_off[u], e1 = cd.nmp_off[u + 1];
+        const float* const nrec = cd.nm_rec;   // the same values, packed (float-exact positions)
         for (long long e = e0 + lane; e < e1; e += 64) {
-            const int j = cd.nmp_ent[e];
-            R dx = (R)(xg - A.x[j]), dy = (R)(yg - A.y[j]), dz = (R)(zg - A.z[j]);
+            double xj, yj, zj, presj;
+            R mj, rhoj, uxj, uyj, uzj;
+            if (nrec) {
+                const float* C = nrec + (size_t)cd.nmp_cent[e] * NMREC_FLOATS;
+                const rec_f4 a = ld_f4(C), b = ld_f4(C + 4);
+                xj = a.x; yj = a.y; zj = a.z; mj = (R)a.w; rhoj = (R)b.x; uxj = (R)b.y; uyj = (R)b.z; uzj = (R)b.w;
+                presj = ld_d2(C + 8).x;
+            } else {
+                const int j = cd.nmp_ent[e];
+                xj = A.x[j]; yj = A.y[j]; zj = A.z[j]; mj = (R)A.m[j]; rhoj = (R)A.rho[j];
+                uxj = (R)A.ux[j]; uyj = (R)A.uy[j]; uzj = (R)A.uz[j];
+                presj = pold[j];
+            }
+            R dx = (R)(xg - xj), dy = (R)(yg - yj), dz = (R)(zg - zj);
             R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
             if (!(tdist < search_range)) continue;
-            R mj = (R)A.m[j], rhoj = (R)A.rho[j];
-            double presj = pold[j];
             R twij = kernel_w<R>(kc, tmp_A, hi, tdist);
             s[0] += (double)(mj / rhoj * (R)presj * twij);
-            s[1] += (double)(mj / rhoj * (R)A.ux[j] * twij);
-            s[2] += (double)(mj / rhoj * (R)A.uy[j] * twij);
-            s[3] += (double)(mj / rhoj * (R)A.uz[j] * twij);
+            s[1] += (double)(mj / rhoj * uxj * twij);
+            s[2] += (double)(mj / rhoj * uyj * twij);
+            s[3] += (double)(mj / rhoj * uzj * twij);
             s[4] += (double)(mj / rhoj * twij);
             if (inlet) {
                 R tdwij = kernel_dw<R>(kc, tmp_A, hi, tdist);
@@ -1408,11 +1426,60 @@ int ibm_near_build(sph_ctx* c) {
     return 0;
 }
 
+// The distinct particles of the Neumann pairs (a thin layer of fluid along
+// walls, lid and buildings) and the pairs as indices into that set.
+__global__ void __launch_bounds__(BLOCK) k_zero_i32(int n, int* a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+__global__ void __launch_bounds__(BLOCK) k_nms_mark(long long m, const int* ent, int* mark) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < m) mark[ent[e]] = 1;
+}
+__global__ void __launch_bounds__(BLOCK) k_nms_list(int n, const int* mark, const int* idx, int* set) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && mark[i]) set[idx[i]] = i;
+}
+__global__ void __launch_bounds__(BLOCK) k_nms_remap(long long m, const int* ent, const int* idx, int* cent) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < m) cent[e] = idx[ent[e]];
+}
+static int neumann_set(sph_ctx* c, long long total) {
+    const int n = c->n;
+    int *mark = nullptr, *idx = nullptr;
+    SPH_TRY(dalloc(c, &mark, n));
+    SPH_TRY(dalloc(c, &idx, n));
+    hipLaunchKernelGGL(k_zero_i32, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, mark);
+    hipLaunchKernelGGL(k_nms_mark, dim3(grid1(total)), dim3(BLOCK), 0, c->stream, total, (const int*)c->nmp_ent, mark);
+    SPH_TRY(launch_check(c, "neumann_set_mark"));
+    int nset = 0;
+    SPH_TRY(exclusive_scan_i32(c, mark, idx, n, &nset));
+    SPH_TRY(dalloc(c, &c->nm_set, (size_t)std::max(1, nset)));
+    SPH_TRY(dalloc(c, &c->nmp_cent, (size_t)total));
+    SPH_TRY(dalloc(c, &c->nm_rec, (size_t)std::max(1, nset) * NMREC_FLOATS));
+    hipLaunchKernelGGL(k_nms_list, dim3(grid1(n)), dim3(BLOCK), 0, c->stream, n, (const int*)mark, (const int*)idx,
+                       c->nm_set);
+    hipLaunchKernelGGL(k_nms_remap, dim3(grid1(total)), dim3(BLOCK), 0, c->stream, total, (const int*)c->nmp_ent,
+                       (const int*)idx, c->nmp_cent);
+    SPH_TRY(launch_check(c, "neumann_set"));
+    dfree(c, mark);
+    dfree(c, idx);
+    c->nm_nset = nset;
+    return 0;
+}
+
 int neumann_pairs(sph_ctx* c) {
     dfree(c, c->nmp_off);
     dfree(c, c->nmp_ent);
+    dfree(c, c->nmp_cent);
+    dfree(c, c->nm_set);
+    dfree(c, c->nm_rec);
     c->nmp_off = nullptr;
     c->nmp_ent = nullptr;
+    c->nmp_cent = nullptr;
+    c->nm_set = nullptr;
+    c->nm_rec = nullptr;
+    c->nm_nset = 0;
     const int m = c->nm_count;
     if (m > 0 && c->nm_list) {
         const bool mixed = c->prm.precision == SPH_MIXED;
@@ -1447,6 +1514,7 @@ int neumann_pairs(sph_ctx* c) {
         SPH_HIP(c, hipStreamSynchronize(c->stream));   // total lives on this frame
         dfree(c, cnt);
         dfree(c, wide);
+        if (mixed && c->pos_f32 && !c->prm.ale_positions && total > 0) SPH_TRY(neumann_set(c, total));
     }
     c->nmp_gen = c->generation;
     c->nmp_for = c->nm_list;
@@ -1455,7 +1523,7 @@ int neumann_pairs(sph_ctx* c) {
 }
 
 static int run_neumann(Eng& e, int which, int only_count0) {
-    SEGL("snapshot", -1, k_snapshot, which);
+    SEGL("snapshot", -1, k_snapshot, which, 1);
     // one 64-thread block (one wave) per boundary particle of each case
     Shape sh;
     SPH_TRY(shape(e, 3, 64, &sh));
@@ -1473,12 +1541,12 @@ static int run_neumann(Eng& e, int which, int only_count0) {
     return 0;
 }
 static int run_outlet(Eng& e, int which) {
-    SEGL("snapshot", -1, k_snapshot, which);
+    SEGL("snapshot", -1, k_snapshot, which, 0);
     SEGL("outlet", -1, k_outlet, which);
     return 0;
 }
 static int run_periodic(Eng& e, int which) {
-    SEGL("snapshot", -1, k_snapshot, which);
+    SEGL("snapshot", -1, k_snapshot, which, 0);
     SEGL("periodic", -1, k_periodic, which);
     return 0;
 }

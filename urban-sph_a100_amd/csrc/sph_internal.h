@@ -137,6 +137,14 @@ struct CaseDesc {
     int nm_count;
     const long long* nmp_off;   // per nm_list particle: its first Neumann pair (nm_count + 1)
     const int* nmp_ent;         // Neumann pairs: the fluid neighbours inside the 2h search range, scan order
+    // MIXED with float-exact positions: the distinct particles of those pairs
+    // (nm_set, ascending), the pairs as indices into it, and a record of each
+    // (NMREC_FLOATS: x y z m | rho ux uy uz | pres as a double, 0) packed by
+    // the snapshot before each Neumann pass; nm_rec null otherwise
+    const int* nmp_cent;
+    const int* nm_set;
+    int nm_nset;
+    float* nm_rec;
     double dt, time;
     KConst kc;
     sph_params pr;
@@ -266,6 +274,9 @@ constexpr int CREC_ADV_FLOATS = 16;
 // type word: p_type in the low 16 bits, i_type in bits 16..29, bit 30 set
 // when the particle is an unknown of the case's PPE (it has a row)
 constexpr int CREC_ROW_BIT = 1 << 30;
+// Neumann-pass record (MIXED, float-exact positions): x y z m, rho ux uy uz as
+// floats, then the pressure snapshot as a double and a pad double
+constexpr int NMREC_FLOATS = 12;
 
 // Where a gathering pass reads its neighbour records: in memory, indexed by
 // particle (base null: the FP64 field arrays), or staged in LDS by a tile,
@@ -454,6 +465,10 @@ struct sph_ctx {
     int nm_count = 0;
     long long* nmp_off = nullptr;   // Neumann pairs of nm_list (built by neumann_pairs)
     int* nmp_ent = nullptr;
+    int* nmp_cent = nullptr;        // MIXED, float-exact positions: pairs as indices into nm_set
+    int* nm_set = nullptr;          // the distinct particles of the pairs, ascending
+    int nm_nset = 0;
+    float* nm_rec = nullptr;        // their records (NMREC_FLOATS each)
     long long nmp_gen = -1;         // generation, list and precision they were built for
     const int* nmp_for = nullptr;
     int nmp_prec = -1;
