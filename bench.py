@@ -170,6 +170,11 @@ def interaction_rooflines(prof):
                   "frac_fp32_vector_peak": round(flop / avg_s / 1e12 / FP32_VECTOR_PEAK_TFLOPS, 4),
                   "fabric_bytes_per_launch": fab, "fabric_GBps": round(fab / avg_s / 1e9, 1),
                   "frac_hbm_peak": round(fab / avg_s / 1e9 / HBM_PEAK_GBS, 3)}
+        v = re.search(r"interaction_%s: .* valu_instructions (\d+)" % k, text)
+        if v:
+            # wave64 VALU instructions x 4 cycles over 1024 SIMDs at the 2.4 GHz peak clock
+            out[k]["valu_wave_instructions"] = int(v.group(1))
+            out[k]["valu_issue_share"] = round(int(v.group(1)) * 4 / (1024 * 2.4e9 * avg_s), 3)
     return out or None
 
 
