@@ -296,16 +296,18 @@ def test_lattice_order_matches_cell_order():
         check_mixed(f"lattice {name}", got, ref, at_tol, t)
 
 
-@pytest.mark.parametrize("precision,precond", [(abi.MIXED, abi.PRECOND_MULTIGRID), (abi.FP64, abi.PRECOND_MULTIGRID),
-                                               (abi.MIXED, abi.PRECOND_JACOBI), (abi.FP64, abi.PRECOND_JACOBI)])
-def test_admission_estimate_bounds_device_memory(precision, precond):
+@pytest.mark.parametrize("precision,precond,tiles", [(abi.MIXED, abi.PRECOND_MULTIGRID, 0),
+                                                     (abi.FP64, abi.PRECOND_MULTIGRID, 0),
+                                                     (abi.MIXED, abi.PRECOND_JACOBI, 0), (abi.FP64, abi.PRECOND_JACOBI, 0),
+                                                     (abi.MIXED, abi.PRECOND_MULTIGRID, 4 | 4 << 8 | 2 << 16)])
+def test_admission_estimate_bounds_device_memory(precision, precond, tiles):
     """The server admits by sph_estimate_bytes before a request has run; after
     two steps the context holds no more than that (live and cached blocks,
-    the multigrid set-up included), and the estimate is not loose by more
-    than 3x."""
+    the multigrid set-up and the cell tiles included), and the estimate is
+    not loose by more than 3x."""
     P, nrm, geom = cases.sized_canyon(1.2e5)
     prm = cases.canyon_params(geom, urbansph.default_params(), tol=1e-6, precision=precision)
-    prm.update(ppe_precond=precond)
+    prm.update(ppe_precond=precond, tile_cells=tiles)
     est = urbansph.estimate_bytes(prm, len(P))
     c = urbansph.Case(prm, len(P))
     c.upload(P, None, nrm)
@@ -315,7 +317,8 @@ def test_admission_estimate_bounds_device_memory(precision, precond):
     held = c.device_bytes()
     pc = c.ppe_report().precond
     c.close()
-    print(f"[estimate] precision {precision} precond {pc}: held {held / 1e6:.1f} MB, estimate {est / 1e6:.1f} MB",
+    print(f"[estimate] precision {precision} precond {pc} tiles {tiles:#x}: held {held / 1e6:.1f} MB, "
+          f"estimate {est / 1e6:.1f} MB",
           flush=True)
     assert pc == precond
     assert held <= est and est <= 3 * held

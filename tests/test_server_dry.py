@@ -36,6 +36,13 @@ def test_estimate_scales_with_particles():
     mg = _params(abi.MIXED)
     mg.update(ppe_precond=abi.PRECOND_MULTIGRID)
     assert urbansph.estimate_bytes(mg, 10_000) > urbansph.estimate_bytes(m, 10_000) + 10_000 * 1000
+    # the cell tiles add their copy of the lists (MIXED only)
+    t = _params(abi.MIXED)
+    t.update(tile_cells=4 | 4 << 8 | 2 << 16)
+    assert urbansph.estimate_bytes(t, 10_000) >= urbansph.estimate_bytes(m, 10_000) + 10_000 * 600
+    f = _params()
+    f.update(tile_cells=4 | 4 << 8 | 2 << 16)
+    assert urbansph.estimate_bytes(f, 10_000) == a
 
 
 def test_budget_admission_fifo_and_continuous_batching():

@@ -162,6 +162,12 @@ long long sph_estimate_bytes(const sph_params* prm, int n) {
     // neighbour records of the gathering passes: 128 bytes, plus the compact
     // 32- and 64-byte records in MIXED
     per += 128.0 + (mixed ? 96.0 : 0.0);
+    // Neumann pairs of the boundary particles (a few hundred each, a few per
+    // cent of the particles): about 32 four-byte pairs per particle, kept
+    // twice in MIXED (particle and set indices) with the set's 48-byte records
+    per += 32 * 4.0 + (mixed ? 32 * 4.0 + 8.0 : 0.0);
+    // cell tiles, when on: a copy of the list, lanes, halos
+    if (mixed && prm->tile_cells != 0) per += 160 * 4.0 + 3 * 4.0 + 16 * 4.0;
     long long fixed = 16LL << 20;
     if (prm->ppe_precond == SPH_PRECOND_MULTIGRID && prm->ppe_solver == SPH_PPE_BICGSTAB) {
         // the level-0 compact copy (4 bytes a slot), the coarse levels (about
