@@ -1104,6 +1104,9 @@ __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, c
     double pi_d = S.pres[i] - S.pres0[i];
     R pi = (R)pi_d, rhoi = (R)S.rho[i];
     double tmpx = 0, tmpy = 0, tmpz = 0;
+    // the pair's kernel normalisation and derivative factor depend only on
+    // h_ij: kept while consecutive pairs share it
+    R h_c = (R)-1, A_c = (R)0, f_c = (R)0;
     const NLRow row = nl_row(nl, li);
     if (sizeof(R) == 4 && src.ok()) {
         // the same sums from the compact float records
@@ -1117,8 +1120,8 @@ __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, c
             R dx = (R)(xi - (double)a.x), dy = (R)(yi - (double)a.y), dz = (R)(zi - (double)a.z);
             R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
             R hij = (R)(0.5 * (hi_d + (double)b.z));
-            R tmp_Aij = calc_tmpA<R>(kc, hij);
-            R tdwij = kernel_dw<R>(kc, tmp_Aij, hij, tdist);
+            if (hij != h_c) { h_c = hij; A_c = calc_tmpA<R>(kc, hij); f_c = dw_factor<R>(kc, hij); }
+            R tdwij = kernel_dw_f<R>(kc, A_c, hij, f_c, tdist);
             R tdwx = tdwij * dx / tdist;
             R tdwy = tdwij * dy / tdist;
             R tdwz = tdwij * dz / tdist;
@@ -1144,8 +1147,8 @@ __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, c
         R dx = (R)(xi - xj), dy = (R)(yi - yj), dz = (R)(zi - zj);
         R tdist = sqrt(dx * dx + dy * dy + dz * dz) - (R)1e-20;
         R hij = (R)(0.5 * (hi_d + hj));
-        R tmp_Aij = calc_tmpA<R>(kc, hij);
-        R tdwij = kernel_dw<R>(kc, tmp_Aij, hij, tdist);
+        if (hij != h_c) { h_c = hij; A_c = calc_tmpA<R>(kc, hij); f_c = dw_factor<R>(kc, hij); }
+        R tdwij = kernel_dw_f<R>(kc, A_c, hij, f_c, tdist);
         R tdwx = tdwij * dx / tdist;
         R tdwy = tdwij * dy / tdist;
         R tdwz = tdwij * dz / tdist;

@@ -61,33 +61,47 @@ __device__ __forceinline__ R kernel_w(const KConst& k, R tA, R tH, R rr) {
     return (R)0;
 }
 
-// calc_kernel_dwij, functions_KNL.cuh:73-103 (3-D)
+// calc_kernel_dwij, functions_KNL.cuh:73-103 (3-D).  The leading factor
+// 1/h (Gaussian, quintic, quartic) or 1/(kappa h) (Wendland) depends on h
+// only: dw_factor computes it, kernel_dw_f takes it, so that a caller whose
+// pairs share h evaluates it once (the same value in the same position of
+// the product).
 template <class R>
-__device__ __forceinline__ R kernel_dw(const KConst& k, R tA, R tH, R rr) {
+__device__ __forceinline__ R dw_factor(const KConst& k, R tH) {
+    const R one = (R)1;
+    return (k.type == SPH_GAUSSIAN || k.type == SPH_QUINTIC || k.type == SPH_QUARTIC) ? one / tH
+                                                                                       : one / ((R)k.kappa * tH);
+}
+template <class R>
+__device__ __forceinline__ R kernel_dw_f(const KConst& k, R tA, R tH, R f, R rr) {
     const R kap = (R)k.kappa;
     R tR = rr / tH / kap;
     const R one = (R)1;
     switch (k.type) {
-    case SPH_GAUSSIAN: { R a = (R)2.0 * tR; return (one / tH) * tA * (R)(-2.0) * (R)2.0 * tR * exp(-(a * a)); }
+    case SPH_GAUSSIAN: { R a = (R)2.0 * tR; return f * tA * (R)(-2.0) * (R)2.0 * tR * exp(-(a * a)); }
     case SPH_QUINTIC: {
         R a = (R)3.0 - (R)2.0 * tR, b = (R)2.0 - (R)2.0 * tR, c = (R)1.0 - (R)2.0 * tR;
-        if (tR < (R)0.5) return (one / tH) * tA * ((R)(-5.0) * p4(a) + (R)30.0 * p4(b) - (R)75.0 * p4(c));
-        else if ((R)0.5 <= tR && tR < one) return (one / tH) * tA * ((R)(-5.0) * p4(a) + (R)30.0 * p4(b));
-        else if (one <= tR && tR < (R)1.5) return (one / tH) * tA * ((R)(-5.0) * p4(a));
+        if (tR < (R)0.5) return f * tA * ((R)(-5.0) * p4(a) + (R)30.0 * p4(b) - (R)75.0 * p4(c));
+        else if ((R)0.5 <= tR && tR < one) return f * tA * ((R)(-5.0) * p4(a) + (R)30.0 * p4(b));
+        else if (one <= tR && tR < (R)1.5) return f * tA * ((R)(-5.0) * p4(a));
         return (R)0;
     }
     case SPH_QUARTIC:
-        return (R)(tR < one) * (one / tH) * tA * ((R)(-9.0 / 2.0) * (R)2 * tR + (R)(19.0 / 3.0) * (R)3.0 * (tR * tR) - (R)(5.0 / 2.0) * (R)4.0 * (tR * tR * tR));
+        return (R)(tR < one) * f * tA * ((R)(-9.0 / 2.0) * (R)2 * tR + (R)(19.0 / 3.0) * (R)3.0 * (tR * tR) - (R)(5.0 / 2.0) * (R)4.0 * (tR * tR * tR));
     case SPH_WENDLAND2:
-        return (R)(tR < one) * (one / (kap * tH)) * tA * ((R)(-20) * tR * (one - tR) * (one - tR) * (one - tR));
+        return (R)(tR < one) * f * tA * ((R)(-20) * tR * (one - tR) * (one - tR) * (one - tR));
     case SPH_WENDLAND4:
-        return (R)(tR < one) * (one / (kap * tH)) * tA *
+        return (R)(tR < one) * f * tA *
                ((R)(-18.666666666666668) * tR * (one - tR) * (one - tR) * (one - tR) * (one - tR) * (one - tR) * ((R)5 * tR + one));
     case SPH_WENDLAND6:
-        return (R)(tR < one) * (one / (kap * tH)) * tA *
+        return (R)(tR < one) * f * tA *
                ((R)(-22) * tR * (one - tR) * (one - tR) * (one - tR) * (one - tR) * (one - tR) * (one - tR) * (one - tR) * ((R)16 * tR * tR + (R)7 * tR + one));
     }
     return (R)0;
+}
+template <class R>
+__device__ __forceinline__ R kernel_dw(const KConst& k, R tA, R tH, R rr) {
+    return kernel_dw_f(k, tA, tH, dw_factor(k, tH), rr);
 }
 
 // calc_kernel_dwij_pressure, functions_KNL.cuh:355-369: Wendland-6 shape,
