@@ -251,7 +251,7 @@ __device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList
         int ptj;
         const float* C = compact ? src.at(j) : nullptr;
         if (compact) {
-            const rec_f4 a = Src::f4(C), b = Src::f4(C + 4);
+            const rec_f4 a = Src::f4(C), b = Src::f4(src.piece(C, 1));
             xj = a.x; yj = a.y; zj = a.z; mj_d = a.w; rhoj_d = b.x; ptj = crec_ptype(b.w);
         } else {
             if constexpr (Src::lds) continue;   // staged records are compact
@@ -274,7 +274,7 @@ __device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList
                 R tdwzc = gc31 * tdwx + gc23 * tdwy + gc33 * tdwz;
                 double uxj, uyj, uzj;
                 if (compact) {
-                    const rec_d2 u = Src::d2(C + 8), w = Src::d2(C + 12);
+                    const rec_d2 u = Src::d2(src.piece(C, 2)), w = Src::d2(src.piece(C, 3));
                     uxj = u.x; uyj = u.y; uzj = w.x;
                 } else {
                     uxj = S.ux[j]; uyj = S.uy[j]; uzj = S.uz[j];
@@ -321,7 +321,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
 __global__ void __launch_bounds__(TILE_MAX) k_prep_tile(Seg sg) {
     TILE_PARTICLE(sg, cd.crec_adv, CREC_ADV_FLOATS);
     if (chunk_.y >= 0)
-        prep_body<float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+        prep_body<float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, chunk_.y,
+                                                     cd.tile.halo + chunk_.x,
                                              cd.tile.self[lane_]});
     else prep_body<float>(cd, i, cd.tile.nl, lane_, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv});
 }
@@ -791,11 +792,11 @@ __device__ __forceinline__ void advection_body(const CaseDesc& cd, int i, const 
         int ptypej;
         if (compact) {
             const float* C = src.at(j);
-            const rec_f4 b = Src::f4(C + 4);
+            const rec_f4 b = Src::f4(src.piece(C, 1));
             ptypej = crec_ptype(b.w);
             if (ptypej >= 1000) continue;
             const rec_f4 a = Src::f4(C);
-            const rec_d2 u = Src::d2(C + 8), w = Src::d2(C + 12);
+            const rec_d2 u = Src::d2(src.piece(C, 2)), w = Src::d2(src.piece(C, 3));
             xj = a.x; yj = a.y; zj = a.z; mj_d = a.w;
             rhoj_d = b.x; hj_d = b.y; visj_d = b.z;
             uxj = u.x; uyj = u.y; uzj = w.x;
@@ -911,7 +912,8 @@ template <bool SCAL>
 __global__ void __launch_bounds__(TILE_MAX) k_advection_tile(Seg sg) {
     TILE_PARTICLE(sg, cd.crec_adv, CREC_ADV_FLOATS);
     if (chunk_.y >= 0)
-        advection_body<float, SCAL>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+        advection_body<float, SCAL>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, chunk_.y,
+                                                     cd.tile.halo + chunk_.x,
                                                          cd.tile.self[lane_]});
     else advection_body<float, SCAL>(cd, i, cd.tile.nl, lane_, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv});
 }
@@ -1114,7 +1116,7 @@ __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, c
             unsigned e = NB_AT(row, k);
             if (!(e & F_DM)) continue;
             const float* C = src.at(e & NB_IDX);
-            const rec_f4 b = Src::f4(C + 4);
+            const rec_f4 b = Src::f4(src.piece(C, 1));
             if (__float_as_int(b.w) >= 1000) continue;
             const rec_f4 a = Src::f4(C);
             R dx = (R)(xi - (double)a.x), dy = (R)(yi - (double)a.y), dz = (R)(zi - (double)a.z);
@@ -1170,7 +1172,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
 __global__ void __launch_bounds__(TILE_MAX) k_pressure_force_tile(Seg sg) {
     TILE_PARTICLE(sg, cd.crec, CREC_FLOATS);
     if (chunk_.y >= 0)
-        pressure_force_body<float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+        pressure_force_body<float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_FLOATS>{(const float*)tile_lds_, chunk_.y,
+                                                     cd.tile.halo + chunk_.x,
                                                       cd.tile.self[lane_]});
     else pressure_force_body<float>(cd, i, cd.tile.nl, lane_, GSrc<CREC_FLOATS>{&cd.crec});
 }

@@ -454,11 +454,11 @@ __device__ __forceinline__ void assemble_body(const CaseDesc& cd, int i, const N
         bool jrow;   // j is an unknown of the PPE
         if (compact) {   // the compact record (positions float-exact)
             const float* C = src.at(j);
-            const rec_f4 b = Src::f4(C + 4);
+            const rec_f4 b = Src::f4(src.piece(C, 1));
             if (crec_ptype(b.w) >= 1000 || crec_itype(b.w) == 4) continue;
             jrow = crec_has_row(b.w);
             const rec_f4 a = Src::f4(C);
-            const rec_d2 u = Src::d2(C + 8), w = Src::d2(C + 12);
+            const rec_d2 u = Src::d2(src.piece(C, 2)), w = Src::d2(src.piece(C, 3));
             xj = a.x; yj = a.y; zj = a.z; mj_d = a.w; rhoj_d = b.x;
             uxj = u.x; uyj = u.y; uzj = w.x; qj = w.y;
         } else {
@@ -524,7 +524,8 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ppe_assemble(Seg sg, int jacobi) 
 __global__ void __launch_bounds__(TILE_MAX) k_ppe_assemble_tile(Seg sg, int jacobi) {
     TILE_PARTICLE(sg, cd.crec_adv, CREC_ADV_FLOATS);
     if (chunk_.y >= 0)
-        assemble_body<float, float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, cd.tile.halo + chunk_.x,
+        assemble_body<float, float>(cd, i, cd.tile.nl, lane_, LSrc<CREC_ADV_FLOATS>{(const float*)tile_lds_, chunk_.y,
+                                                     cd.tile.halo + chunk_.x,
                                                         cd.tile.self[lane_]}, jacobi);
     else assemble_body<float, float>(cd, i, cd.tile.nl, lane_, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv}, jacobi);
 }

@@ -287,19 +287,25 @@ struct GSrc {
     float* const* ref;   // the descriptor's record pointer (read where the loop starts), or null
     __device__ bool ok() const { return ref && *ref; }
     __device__ const float* at(int j) const { return *ref + (size_t)j * STRIDE; }
+    __device__ const float* piece(const float* rec, int q) const { return rec + 4 * q; }   // 16-byte piece q
     __device__ static rec_f4 f4(const float* p) { return ld_f4(p); }
     __device__ static rec_d2 d2(const float* p) { return ld_d2(p); }
     __device__ int gidx(int j) const { return j; }           // particle of an entry
     __device__ bool self(int j, int i) const { return j == i; }
 };
+// The staged halo is piece-major: the first 16-byte piece of every record,
+// then the second, ...: lanes reading one piece of random slots then spread
+// over all LDS bank quads (whole records would use a quarter or half of them).
 template <int STRIDE>
 struct LSrc {
     static constexpr bool lds = true;
     const float* base;   // LDS
+    int nrec;            // records staged (the stride between pieces, in 16-byte units)
     const int* halo;     // the chunk's halo: slot -> particle
     int self_slot;       // the slot of this thread's particle, or -1
     __device__ bool ok() const { return true; }
-    __device__ const float* at(int j) const { return base + j * STRIDE; }
+    __device__ const float* at(int j) const { return base + 4 * j; }
+    __device__ const float* piece(const float* rec, int q) const { return rec + 4 * q * nrec; }
     __device__ static rec_f4 f4(const float* p) { return *(const __attribute__((address_space(3))) rec_f4*)p; }
     __device__ static rec_d2 d2(const float* p) { return *(const __attribute__((address_space(3))) rec_d2*)p; }
     __device__ int gidx(int j) const { return halo[j]; }
@@ -311,7 +317,8 @@ struct LSrc {
 // LDS, then maps the thread to its lane (a row of cd.tile.nl) and the
 // lane's particle; padding lanes return after the barrier.
 // A thread copies whole records (Q float4 pieces), TILE_U of them a round,
-// all loads of a round issued before its stores.
+// all loads of a round issued before its stores; piece q of record r goes
+// to float4 q * cnt + r.
 constexpr int TILE_U = 4;
 template <int Q>
 __device__ __forceinline__ void tile_stage_q(float4* lds, const float* recs, const int* halo, int cnt) {
@@ -331,7 +338,7 @@ __device__ __forceinline__ void tile_stage_q(float4* lds, const float* recs, con
             if (idx[u] >= 0)
 #pragma unroll
                 for (int q = 0; q < Q; q++)
-                    lds[(r0 + u * B) * Q + q] = make_float4(v[u][q].x, v[u][q].y, v[u][q].z, v[u][q].w);
+                    lds[q * cnt + r0 + u * B] = make_float4(v[u][q].x, v[u][q].y, v[u][q].z, v[u][q].w);
     }
 }
 __device__ __forceinline__ void tile_stage(float4* lds, const float* recs, const int* halo, int cnt, int recf) {
