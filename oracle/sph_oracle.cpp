@@ -1458,9 +1458,11 @@ int apr_refine(Oracle& o, const double* target, int max_passes, int* n_out) {
             for (int q = 0; q < 8; q++) {
                 const int c = n0 + 8 * k + q;
                 P1d ch = par;
-                ch.x = q < 4 ? par.x + a : par.x - a;
-                ch.y = (q & 2) ? par.y - a : par.y + a;
-                ch.z = (q & 1) ? par.z - a : par.z + a;
+                // positions rounded to float: the reference stores them in
+                // its float records (functions_APR.cuh:544-590)
+                ch.x = (double)(float)(q < 4 ? par.x + a : par.x - a);
+                ch.y = (double)(float)((q & 2) ? par.y - a : par.y + a);
+                ch.z = (double)(float)((q & 1) ? par.z - a : par.z + a);
                 ch.m = 0.125 * par.m;
                 ch.vol = 0.125 * par.vol;
                 if (o.prm.apr_h_change) ch.h = 0.5 * par.h;

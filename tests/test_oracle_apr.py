@@ -66,7 +66,9 @@ def test_one_pass_splits_the_first_eligible_particle_of_each_cell():
     # mass and volume are conserved exactly in the live particles
     assert np.isclose(np.sum(s1["M"][live]), np.sum(s0["M"]), rtol=1e-13, atol=0)
     assert np.isclose(np.sum(s1["VOL"][live]), np.sum(s0["VOL"]), rtol=1e-13, atol=0)
-    # every parent has 8 children: m/8, h/2, at +-cbrt(V)/4 around it
+    # every parent has 8 children: m/8, h/2, at +-cbrt(V)/4 around it, the
+    # positions rounded to float as the reference's float records hold them
+    # (functions_APR.cuh:544-590)
     for d in dummies:
         px, py, pz = s1["X"][d], s1["Y"][d], s1["Z"][d]
         a = 0.25 * np.cbrt(s1["M"][d] / s1["RHO"][d])
@@ -74,9 +76,13 @@ def test_one_pass_splits_the_first_eligible_particle_of_each_cell():
         kids = np.nonzero(live & (dist < 3.01 * a) & np.isclose(s1["M"], s1["M"][d] / 8, rtol=1e-15))[0]
         assert len(kids) == 8
         offs = np.stack([s1["X"][kids] - px, s1["Y"][kids] - py, s1["Z"][kids] - pz], 1)
-        np.testing.assert_allclose(np.abs(offs), a, rtol=1e-14)
         assert len({tuple(np.sign(v)) for v in offs}) == 8       # the 8 octants
-        np.testing.assert_allclose(offs.sum(0), 0.0, atol=1e-15)
+        for k, (sx, sy, sz) in zip(kids, np.sign(offs)):
+            assert s1["X"][k] == float(np.float32(px + sx * a))
+            assert s1["Y"][k] == float(np.float32(py + sy * a))
+            assert s1["Z"][k] == float(np.float32(pz + sz * a))
+        ulp = np.spacing(np.float32(max(abs(px), abs(py), abs(pz)) + a))
+        np.testing.assert_allclose(np.abs(offs), a, atol=float(ulp))
         np.testing.assert_array_equal(s1["H"][kids], 0.5 * s1["H"][d])
 
 

@@ -8,8 +8,9 @@
 // A particle's refinement level is log8(m_ref / m).  In every occupied cell
 // the first particle (in cell order) whose target level exceeds its level
 // by at least 0.75 is split into 8 children at +-a along each axis,
-// a = cbrt(m / rho) / 4, with m/8, vol/8 and h/2 (when apr_h_change), every
-// other field copied.  Children of the k-th splitting cell (ascending cell
+// a = cbrt(m / rho) / 4, rounded to float positions as the reference's
+// float records hold them, with m/8, vol/8 and h/2 (when apr_h_change),
+// every other field copied.  Children of the k-th splitting cell (ascending cell
 // index) go to n + 8k..n + 8k + 7 in the reference's order; the parent
 // becomes a dummy (i_type 3).  One split per cell per pass; the neighbour
 // search runs again after each pass.
@@ -85,10 +86,13 @@ __global__ void __launch_bounds__(BLOCK) k_apr_split(int ncells, const int* flag
         for (int f = 0; f < F.nf; f++) F.f[f][c] = F.f[f][j];
         for (int f = 0; f < SPH_I_COUNT; f++) F.iv[f][c] = F.iv[f][j];
         // child order of functions_APR.cuh:440-478: x +,+,+,+,-,-,-,-;
-        // y +,+,-,-,+,+,-,-; z +,-,+,-,+,-,+,-
-        F.x[c] = k < 4 ? xm + a : xm - a;
-        F.y[c] = (k & 2) ? ym - a : ym + a;
-        F.z[c] = (k & 1) ? zm - a : zm + a;
+        // y +,+,-,-,+,+,-,-; z +,-,+,-,+,-,+,-.  The reference stores the
+        // children's positions in its float records (functions_APR.cuh:
+        // 544-590): rounded to float here too, so a refined case's positions
+        // stay float values
+        F.x[c] = (double)(float)(k < 4 ? xm + a : xm - a);
+        F.y[c] = (double)(float)((k & 2) ? ym - a : ym + a);
+        F.z[c] = (double)(float)((k & 1) ? zm - a : zm + a);
         F.m[c] = 0.125 * F.m[j];
         F.vol[c] = 0.125 * F.vol[j];
         if (h_change) F.h[c] = 0.5 * F.h[j];
@@ -134,8 +138,7 @@ int apr_refine(sph_ctx* c, const double* target, int max_passes, int* n_out) {
         hipLaunchKernelGGL(k_apr_split, dim3(grid1(ncells)), dim3(BLOCK), 0, c->stream, ncells, (const int*)flag,
                            (const int*)rank, (const int*)pick, c->n, F, c->prm.apr_h_change);
         if ((rc = launch_check(c, "apr_split"))) break;
-        c->n += 8 * nsplit;
-        c->pos_f32 = false;   // children sit at computed offsets, not at uploaded float positions
+        c->n += 8 * nsplit;   // the children's positions are float values like their parents'
         dfree(c, c->porder);   // the pass order covers the old count; the next list rebuilds it
         c->porder = nullptr;
         ppe_free(c);
