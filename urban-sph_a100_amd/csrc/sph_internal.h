@@ -505,7 +505,7 @@ struct sph_ctx {
     double* rec = nullptr;               // neighbour records of the gathering passes (cap * REC_DOUBLES)
     float* crec = nullptr;               // compact pressure-force records (cap * CREC_FLOATS), MIXED only
     float* crec_adv = nullptr;           // compact advection / assembly records (cap * CREC_ADV_FLOATS), MIXED only
-    bool pos_f32 = true;                 // positions are the uploaded float values (no refinement, no motion)
+    bool pos_f32 = true;                 // positions are float values (uploaded or refined; no motion)
 
     sph::PPEState* ppe = nullptr;
     sph_ppe_report rep{};
