@@ -160,7 +160,7 @@ struct CaseDesc {
     const double* ctl;   // solve controls written by the host: tol, maxit, skip
     double* xnew[3];     // ALE positions of the corrector (null unless ale_positions)
     double* rec;         // neighbour records (REC_DOUBLES per particle), refreshed before each gathering pass
-    float* crec;         // MIXED with positions still the uploaded floats: compact records (CREC_FLOATS per
+    float* crec;         // MIXED with positions still float values: compact records (CREC_FLOATS per
                          // particle) of the pressure force, or null
     float* crec_adv;     // the same case's compact records of advection and assembly (CREC_ADV_FLOATS), or null
     const unsigned char* ibm_near;   // per particle: 1 if an IB marker is in its spreading list (this search)
@@ -257,7 +257,7 @@ __device__ __forceinline__ void rec_scalar(const double* rec, int j, double& tem
 }
 
 // Compact neighbour records of the MIXED gathering passes, used while the
-// positions are the uploaded float values: the float pair math then reads
+// positions are float values: the float pair math then reads
 // exactly the values it reads from the double state ((float)(x_i - x_j) of
 // two float-exact doubles is the rounded exact difference either way; m, rho
 // and p - p0 enter the pressure force rounded to float; h is float-exact; the
