@@ -370,6 +370,54 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_boundary3d(Seg sg) {
     }
 }
 
+// The same in MIXED, a 64-lane wave per boundary particle (walls are a few
+// per cent of the case): lane sums over entries lane, lane + 64, ..., then
+// a fixed butterfly; only the particles the serial kernel writes (p_type 0
+// and 9) are visited.
+__global__ void __launch_bounds__(64) k_boundary3d_wave(Seg sg) {
+    int first_;
+    const int cid_ = seg_case(sg, first_);
+    const CaseDesc& cd = sg.cd[cid_];
+    const int u = first_ >> 6, lane = threadIdx.x;
+    if (u >= cd.nm_count) return;
+    const sph_params pr = cd.pr;
+    if (pr.noslip_bc != 1) return;   // ISPH_Calc.cuh:83
+    const PView S = cd.S;
+    const int i = cd.nm_list[u];
+    if (S.i_type[i] > 2) return;
+    const int p_type_i = S.p_type[i];
+    if (p_type_i != 0 && p_type_i != 9) return;
+    const KConst kc = cd.kc;
+    const NList nl = cd.nl;
+    float hi = (float)S.h[i];
+    float tmp_A = calc_tmpA<float>(kc, hi);
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    const NLRow row = nl_row(nl, i);
+    for (int k = lane; k < row.cnt; k += 64) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_HM)) continue;
+        int j = e & NB_IDX;
+        const int p_type_j = S.p_type[j];
+        if (!((p_type_j > 0) && (p_type_j != 9))) continue;
+        float dx = (float)(xi - S.x[j]), dy = (float)(yi - S.y[j]), dz = (float)(zi - S.z[j]);
+        float tdist = sqrt(dx * dx + dy * dy + dz * dz) - 1e-20f;
+        float twij = kernel_w<float>(kc, tmp_A, hi, tdist);
+        t[0] += (double)((float)S.ux[j] * twij);
+        t[1] += (double)((float)S.uy[j] * twij);
+        t[2] += (double)((float)S.uz[j] * twij);
+        t[3] += (double)twij;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 4; q++) t[q] += __shfl_xor(t[q], off, 64);
+    if (lane != 0) return;
+    const double flt = 1.0e-10 + t[3];
+    const double uxi = S.ux[i], uyi = S.uy[i], uzi = S.uz[i];
+    S.ux[i] = 2 * uxi * (p_type_i == 9) - t[0] / flt;
+    S.uy[i] = 2 * uyi * (p_type_i == 9) - t[1] / flt;
+    S.uz[i] = 2 * uzi * (p_type_i == 9) - t[2] / flt;
+}
+
 // ------------------------------------------------------------------------
 // KERNEL_MOST_boundary3D, functions_MOST.cuh:175-322.  Part 1: the
 // surface-layer fluxes of each MOST particle; the neighbour heat deposit is
@@ -1046,6 +1094,52 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_freeslip(Seg sg, int which) {
     }
 }
 
+// The same sums in MIXED with a 64-lane wave per boundary particle (the
+// lid's particles are a few per cent of the case, one thread each left
+// the device nearly idle): lane l adds entries l, l + 64, ... of the list,
+// then a fixed butterfly adds the lane sums, so a case gives the same bits
+// alone and in a batch.
+__global__ void __launch_bounds__(64) k_freeslip_wave(Seg sg, int which) {
+    int first_;
+    const int cid_ = seg_case(sg, first_);
+    const CaseDesc& cd = sg.cd[cid_];
+    const int u = first_ >> 6, lane = threadIdx.x;
+    if (u >= cd.nm_count) return;
+    const PView A = which ? cd.P : cd.S;
+    const int i = cd.nm_list[u];
+    if (A.p_type[i] != -1) return;
+    const KConst kc = cd.kc;
+    const NList nl = cd.nl;
+    float hi = (float)A.h[i];
+    float tmp_A = calc_tmpA<float>(kc, hi);
+    double xg = A.x[i], yg = A.y[i], zg = A.z[i];
+    double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const NLRow row = nl_row(nl, i);
+    for (int k = lane; k < row.cnt; k += 64) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_HM)) continue;
+        int j = e & NB_IDX;
+        if (A.p_type[j] != 1) continue;
+        float dx = (float)(xg - A.x[j]), dy = (float)(yg - A.y[j]), dz = (float)(zg - A.z[j]);
+        float tdist = sqrt(dx * dx + dy * dy + dz * dz) - 1e-20f;
+        float mj = (float)A.m[j], rhoj = (float)A.rho[j];
+        float twij = kernel_w<float>(kc, tmp_A, hi, tdist);
+        t[0] += (double)((float)A.ux[j] * (mj / rhoj) * twij);
+        t[1] += (double)((float)A.uy[j] * (mj / rhoj) * twij);
+        t[2] += (double)((float)A.uz[j] * (mj / rhoj) * twij);
+        t[3] += (double)((float)A.pres[j] * (mj / rhoj) * twij);
+        t[4] += (double)((float)A.temp[j] * (mj / rhoj) * twij);
+        t[5] += (double)((mj / rhoj) * twij);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 6; q++) t[q] += __shfl_xor(t[q], off, 64);
+    if (lane != 0) return;
+    if (t[5] > 1e-5) {
+        A.ux[i] = t[0] / t[5]; A.uy[i] = t[1] / t[5]; A.uz[i] = t[2] / t[5];
+        A.pres[i] = t[3] / t[5]; A.temp[i] = t[4] / t[5];
+    }
+}
+
 // ------------------------------------------------------------------------
 // KERNEL_donothing_buffer3D, functions_BC.cuh:673-761 (snapshot reads)
 __global__ void __launch_bounds__(MAX_BLOCK) k_outlet(Seg sg, int which) {
@@ -1227,6 +1321,64 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ibm_force(Seg sg) {
     S.fbz[i] = pr.rho_air * corrz / dt;
 }
 
+// The marker force in MIXED, a 64-lane wave per marker (markers are in the
+// boundary list): lane sums over entries lane, lane + 64, ..., then a fixed
+// butterfly; lane 0 forms the force from the sums as the serial kernel does.
+__global__ void __launch_bounds__(64) k_ibm_force_wave(Seg sg) {
+    int first_;
+    const int cid_ = seg_case(sg, first_);
+    const CaseDesc& cd = sg.cd[cid_];
+    const int u = first_ >> 6, lane = threadIdx.x;
+    if (u >= cd.nm_count) return;
+    const PView S = cd.S;
+    const int i = cd.nm_list[u];
+    if (S.p_type[i] < 1000) return;
+    const KConst kc = cd.kc;
+    const sph_params pr = cd.pr;
+    const QView Q = cd.Q;
+    const NList nl = cd.nl;
+    const double dt = cd.dt;
+    double xi = S.x[i], yi = S.y[i], zi = S.z[i];
+    float tmp_h = (float)(0.5 * S.h[i]);
+    float tmp_A = calc_tmpA<float>(kc, tmp_h);
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    const NLRow row = nl_row(nl, i);
+    for (int k = lane; k < row.cnt; k += 64) {
+        unsigned e = NB_AT(row, k);
+        if (!(e & F_BH)) continue;
+        int j = e & NB_IDX;
+        if (S.i_type[j] == 4 || S.p_type[j] != 1) continue;
+        float dx = (float)(xi - S.x[j]), dy = (float)(yi - S.y[j]), dz = (float)(zi - S.z[j]);
+        float tdist = sqrt(dx * dx + dy * dy + dz * dz) + 1e-20f;
+        float twij = kernel_w<float>(kc, tmp_A, tmp_h, tdist);
+        float tmprho = (float)S.m[j] / (float)S.rho[j];
+        t[0] += (double)(((float)S.ux[j]) * tmprho * twij);
+        t[1] += (double)(((float)S.uy[j]) * tmprho * twij);
+        t[2] += (double)(((float)S.uz[j]) * tmprho * twij);
+        t[3] += (double)(tmprho * twij);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 4; q++) t[q] += __shfl_xor(t[q], off, 64);
+    if (lane != 0) return;
+    const double filt = t[3];
+    S.concentration[i] = filt;
+    double ufx = t[0] / (filt + 1e-10), ufy = t[1] / (filt + 1e-10), ufz = t[2] / (filt + 1e-10);
+    double nx = Q.nx[i], ny = Q.ny[i], nz = Q.nz[i];
+    double nmag = sqrt(nx * nx + ny * ny + nz * nz) + 1e-20;
+    nx = nx / nmag; ny = ny / nmag; nz = nz / nmag;
+    double un = ufx * nx + ufy * ny + ufz * nz;
+    double utx = ufx - un * nx, uty = ufy - un * ny, utz = ufz - un * nz;
+    double Ubtx = pr.ibm_ut_factor * interp_ux_caseH(zi, 1.0);
+    double Ubty = 0.0, Ubtz = 0.0;
+    double beta_t = 1.0;
+    double corrx = (-un) * nx + beta_t * (Ubtx - utx);
+    double corry = (-un) * ny + beta_t * (Ubty - uty);
+    double corrz = (-un) * nz + beta_t * (Ubtz - utz);
+    S.fbx[i] = pr.rho_air * corrx / dt;
+    S.fby[i] = pr.rho_air * corry / dt;
+    S.fbz[i] = pr.rho_air * corrz / dt;
+}
+
 // IBM_spreading_interpolation3D, functions_IBM.cuh:492-613
 template <class R>
 __global__ void __launch_bounds__(MAX_BLOCK) k_ibm_spread(Seg sg) {
@@ -1371,6 +1523,18 @@ __global__ void k_stats_final(const unsigned* in, float* out) {
 
 // launch over the particles of every case; tunable passes (kid >= 0) run at
 // the workgroup size the autotuner picked for this batch size
+// a boundary pass with a 64-lane wave per boundary particle of each case
+#define WAVEL(name, kern, ...)                                                                        \
+    do {                                                                                            \
+        Shape sh_;                                                                                  \
+        SPH_TRY(shape(e, 3, 64, &sh_));                                                             \
+        if (sh_.grid > 0) {                                                                         \
+            SPH_TRY(prof_begin(e, name, 0.0));                                                      \
+            hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(64), 0, e.st, Seg{e.d_desc, sh_.tab}, ##__VA_ARGS__); \
+            SPH_TRY(launch_check(&e, name));                                                        \
+            SPH_TRY(prof_end(e));                                                                   \
+        }                                                                                           \
+    } while (0)
 // the cell-tile variant of a record pass: a workgroup per chunk, the halo
 // records (recf floats each) in dynamic LDS
 #define TILEL(name, kern, recf)                                                                      \
@@ -1557,7 +1721,7 @@ static int run_periodic(Eng& e, int which) {
     return 0;
 }
 static int run_freeslip(Eng& e, int which) {
-    if (mixed_of(e)) SEGL("freeslip", TK_FREESLIP, k_freeslip<float>, which);
+    if (mixed_of(e)) WAVEL("freeslip", k_freeslip_wave, which);   // the lid's particles among the boundary ones
     else SEGL("freeslip", TK_FREESLIP, k_freeslip<double>, which);
     return 0;
 }
@@ -1656,7 +1820,7 @@ int step_physics_front(Eng& e) {
     if (mixed && e.tile_lanes) TILEL("prep", k_prep_tile, CREC_ADV_FLOATS);
     else if (mixed) SEGL("prep", TK_PREP, k_prep<float>);
     else SEGL("prep", TK_PREP, k_prep<double>);
-    if (mixed) SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<float>);
+    if (mixed) WAVEL("boundary3d", k_boundary3d_wave);
     else SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<double>);
     SEGL("most_self", -1, k_most_self);
     SEGL("most_deposit", -1, k_most_deposit);
@@ -1689,7 +1853,7 @@ int step_physics_back(Eng& e) {
     if (mixed && e.tile_lanes) TILEL("pressure_force", k_pressure_force_tile, CREC_FLOATS);
     else if (mixed) SEGL("pressure_force", TK_PRESSURE, k_pressure_force<float>);
     else SEGL("pressure_force", TK_PRESSURE, k_pressure_force<double>);
-    if (mixed) SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<float>);
+    if (mixed) WAVEL("ibm_force", k_ibm_force_wave);
     else SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<double>);
     if (mixed) SEGL("ibm_spread", TK_IBM_SPREAD, k_ibm_spread<float>);
     else SEGL("ibm_spread", TK_IBM_SPREAD, k_ibm_spread<double>);
