@@ -305,7 +305,10 @@ def run_ale(cases, abi, urbansph, P, nrm, geom, steps, warmup, dt, tol):
     P["eli"] = np.where(P["p_type"] == 1, 1.0, 0.0).astype(np.float32)
     prm = cases.canyon_params(geom, urbansph.default_params(), dt=dt, tol=tol, precision=abi.MIXED)
     prm.update(ale_positions=1, ppe_precond=abi.PRECOND_MULTIGRID)   # hierarchy rebuilt every step
+    prm.update(**PARAM_SET)
     c = urbansph.Case(prm, len(P))
+    if not GRAPHS:
+        c.options(graphs=False, autotune=True)
     c.upload(P, None, nrm)
     count = 0
     for k in range(warmup):
