@@ -232,6 +232,11 @@ typedef struct sph_params {
     int    tile_halo_max;       /* most distinct neighbours a workgroup stages
                                    (1..2048); a workgroup with more gathers
                                    from memory; 0 means 1536).  Default 1536  */
+    int    compact_records;     /* MIXED with float positions: 1 (default)
+                                   gathers 32/64-byte float records of the
+                                   neighbours; 0 the 128-byte double records
+                                   and field arrays.  The same results
+                                   bitwise; 0 is for checking that           */
 } sph_params;
 
 /* Sources of the IB marker normals (ib_normals). */

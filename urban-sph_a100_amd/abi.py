@@ -93,7 +93,7 @@ class Params(C.Structure):
             "ib_face_xlo", "ib_face_xhi", "ib_face_ylo", "ib_face_yhi", "ib_face_zlo", "ib_face_zhi",
             "ib_face_tol")]
         + [("mg_sort_all", C.c_int), ("tile_cells", C.c_int), ("tile_particles", C.c_int),
-           ("tile_halo_max", C.c_int)]
+           ("tile_halo_max", C.c_int), ("compact_records", C.c_int)]
     )
 
     def update(self, **kw):

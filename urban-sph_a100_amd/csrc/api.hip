@@ -210,7 +210,7 @@ void sph_params_default(sph_params* p) {
     p->ib_face_xlo = 0.764; p->ib_face_xhi = 0.836; p->ib_face_ylo = -0.036; p->ib_face_yhi = 0.036;
     p->ib_face_zlo = 0.0; p->ib_face_zhi = 0.152; p->ib_face_tol = 1e-5;
     // LDS-staged record passes: off (measured slower than the per-particle gathers)
-    p->tile_cells = 0; p->tile_particles = 128; p->tile_halo_max = 1536;
+    p->tile_cells = 0; p->tile_particles = 128; p->tile_halo_max = 1536; p->compact_records = 1;
 }
 
 static int validate(const sph_params* p, std::string& why) {

@@ -508,7 +508,8 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.nm_count = c->nm_count;
     d.nmp_off = c->nmp_off;
     d.nmp_ent = c->nmp_ent;
-    const bool nmrec = c->prm.precision == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions && c->nm_rec;
+    const bool nmrec = c->prm.precision == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions && c->prm.compact_records &&
+                       c->nm_rec;
     d.nmp_cent = nmrec ? c->nmp_cent : nullptr;
     d.nm_set = nmrec ? c->nm_set : nullptr;
     d.nm_nset = nmrec ? c->nm_nset : 0;
@@ -530,7 +531,7 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.ppe = ppe_view(c);
     for (int f = 0; f < 3; f++) d.xnew[f] = c->xnew[f];
     d.rec = c->rec;
-    const bool compact = c->prm.precision == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions;
+    const bool compact = c->prm.precision == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions && c->prm.compact_records;
     d.crec = compact ? c->crec : nullptr;
     d.crec_adv = compact ? c->crec_adv : nullptr;
     d.ibm_near = c->ibm_near_gen == c->generation ? c->ibm_near : nullptr;
@@ -564,7 +565,7 @@ static int step_body(Eng& e, sph_ctx* const* cases, int nc, const int* count, co
         if (c->nmp_gen != c->generation || c->nmp_for != c->nm_list || c->nmp_prec != prec || c->prm.ale_positions)
             SPH_TRY(neumann_pairs(c));
         SPH_TRY(ibm_near_build(c));   // the spreading targets of this search
-        if (prec == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions) SPH_TRY(tile_build(c));
+        if (prec == SPH_MIXED && c->pos_f32 && !c->prm.ale_positions && c->prm.compact_records) SPH_TRY(tile_build(c));
     }
     // composition of the step: tables and graphs stay valid while it holds
     std::vector<long long> sig;
