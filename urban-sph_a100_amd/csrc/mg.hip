@@ -646,6 +646,15 @@ __device__ __forceinline__ double coarse_ax(const MGLv& L, int row) {
     const MGLv& L = lv[te_.x * MG_MAX_LEVELS + (lvl)];         \
     const int row = te_.y + (int)threadIdx.x;
 
+// Prolongation of a level's correction z to the level above, done by the
+// thread that forms z[a]: x += s z[a] over the members of aggregate a (each
+// member belongs to one aggregate, so no two threads write one x)
+__device__ __forceinline__ void mg_prolong_members(const MGLv& F, int a, float z) {
+    for (int t = F.mstart[a]; t < F.mstart[a + 1]; t++) {
+        const int m = F.mem[t];
+        F.x[m] = (float)((double)F.x[m] + MG_SCALE * (double)z);
+    }
+}
 // x = w b / diag
 __global__ void __launch_bounds__(BLOCK) k_mg_pre(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
                                                   int which, int sc_done) {
@@ -701,7 +710,9 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(const CaseDesc* cds, const MG
     } else {
         if (row >= L.n) return;
         const double ax = coarse_ax(L, row);
-        L.y[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)L.b[row] - ax) / (double)L.diag[row]);
+        const float y = (float)((double)L.x[row] + MG_OMEGA * ((double)L.b[row] - ax) / (double)L.diag[row]);
+        L.y[row] = y;
+        mg_prolong_members(lv[te_.x * MG_MAX_LEVELS + l - 1], row, y);   // into level l - 1
     }
 }
 // b of level l + 1: the sum of r over each aggregate's members, in member
@@ -713,16 +724,10 @@ __global__ void __launch_bounds__(BLOCK) k_mg_restrict(const CaseDesc* cds, cons
     if (row >= N.n) return;
     double s = 0.0;
     for (int t = L.mstart[row]; t < L.mstart[row + 1]; t++) s += (double)L.r[L.mem[t]];
-    N.b[row] = (float)s;
-}
-// x += s z_next[agg]
-__global__ void __launch_bounds__(BLOCK) k_mg_prolong(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
-                                                      int sc_done) {
-    MG_HEAD(tab, l);
-    if (row >= L.n) return;
-    const MGLv& N = lv[te_.x * MG_MAX_LEVELS + l + 1];
-    const float* z = l + 1 == L.lc ? N.x : N.y;
-    L.x[row] = (float)((double)L.x[row] + MG_SCALE * (double)z[L.agg[row]]);
+    const float b = (float)s;
+    N.b[row] = b;
+    // the next level's pre-smoothing, x = w b / diag, when it has a level below
+    if (l + 1 < L.lc) N.x[row] = (float)(MG_OMEGA * (double)b / (double)N.diag[row]);
 }
 // level 0: the tail sums of the long rows over the current x, one wave per
 // long row, lanes over its contiguous tail, fixed shuffle tree (tab: four
@@ -794,7 +799,10 @@ __global__ void __launch_bounds__(BLOCK) k_mg_dense(const CaseDesc* cds, const M
     for (int j = lane; j < n; j += SLICE) s += (double)a[j] * (double)C.b[j];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) C.x[row] = (float)s;
+    if (lane != 0) return;
+    const float x = (float)s;
+    C.x[row] = x;
+    if (lc > 0) mg_prolong_members(lv[te_.x * MG_MAX_LEVELS + lc - 1], row, x);   // into the level above
 }
 
 // ------------------------------------------------------------------------
@@ -1218,8 +1226,12 @@ int mg_apply(Eng& e, int which, int sc_done) {
             SPH_TRY(prof_end(e));                                                                    \
         }                                                                                            \
     } while (0)
+    // The pointwise passes ride on the row passes: a level's pre-smoothing
+    // on the restriction that forms its b (levels below the finest), the
+    // prolongation of a level's correction on the pass that forms it (the
+    // coarse post-smoothing, or the dense solve)
     for (int l = 0; l + 1 < P.lmax; l++) {
-        MGE("mg_pre", 0, P.down_off[l], P.down_cnt[l], k_mg_pre, l, which, sc_done);
+        if (l == 0) MGE("mg_pre", 0, P.down_off[l], P.down_cnt[l], k_mg_pre, l, which, sc_done);
         if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
         else MGE("mg_clong", 0, P.clong_off[l], P.clong_cnt[l], k_mg_clong, l, sc_done);
         MGE(l == 0 ? "mg_res0" : "mg_res", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_res, l,
@@ -1228,7 +1240,6 @@ int mg_apply(Eng& e, int which, int sc_done) {
     }
     MGE("mg_dense", 0, P.dense_off, P.dense_cnt, k_mg_dense, sc_done);
     for (int l = P.lmax - 2; l >= 0; l--) {
-        MGE("mg_prolong", 0, P.down_off[l], P.down_cnt[l], k_mg_prolong, l, sc_done);
         if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
         else MGE("mg_clong", 0, P.clong_off[l], P.clong_cnt[l], k_mg_clong, l, sc_done);
         MGE(l == 0 ? "mg_post0" : "mg_post", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_post,
