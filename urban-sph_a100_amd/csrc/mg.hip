@@ -47,8 +47,6 @@ static_assert(MG_COARSE_MAX <= 1024, "the coarsest level is inverted densely");
 constexpr int MG_ENT_CAP = 2048;
 constexpr int MG_BIG_CAP = 8192;
 constexpr int MG_BIG_MEMBERS = 64;
-constexpr double MG_OMEGA = 0.9;
-constexpr double MG_SCALE = 1.5;
 
 struct MGLevel {
     int n = 0, nslices = 0;
@@ -67,8 +65,10 @@ struct MGLevel {
     int* agg = nullptr;      // row -> row of the next level
     int* mstart = nullptr;   // next-level row -> first member in mem (n_next + 1 entries)
     int* mem = nullptr;      // this level's rows grouped by aggregate, ascending within each
-    // V-cycle vectors
+    // V-cycle vectors (level 0: x is the PPE's mgx0, which the Krylov
+    // kernels fill with the pre-smoothing; not owned here)
     float *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
+    bool x_shared = false;
     // coarse level: rows of more than MG_CLONG entries, whose products a
     // wave per row forms before each pass (lidx: row -> list index or -1)
     int nlong = 0;
@@ -655,15 +655,6 @@ __device__ __forceinline__ void mg_prolong_members(const MGLv& F, int a, float z
         F.x[m] = (float)((double)F.x[m] + MG_SCALE * (double)z);
     }
 }
-// x = w b / diag
-__global__ void __launch_bounds__(BLOCK) k_mg_pre(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
-                                                  int which, int sc_done) {
-    MG_HEAD(tab, l);
-    if (row >= L.n) return;
-    const float* b = l == 0 ? (which ? cd.ppe.fs : cd.ppe.fp) : L.b;
-    const float* d = l == 0 ? cd.ppe.fdiag : L.diag;
-    L.x[row] = (float)(MG_OMEGA * (double)b[row] / (double)d[row]);
-}
 // r = b - A x
 __global__ void __launch_bounds__(BLOCK) k_mg_res(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
                                                   int which, int sc_done) {
@@ -811,6 +802,7 @@ void mg_free(sph_ctx* c, MGState* mg) {
     if (!mg) return;
     for (int l = 0; l < MG_MAX_LEVELS; l++) {
         MGLevel& L = mg->L[l];
+        if (L.x_shared) L.x = nullptr;   // the PPE's
         void* ptrs[] = {L.cx, L.cy, L.cz, L.ex, L.sl_off, L.cnt, L.cols, L.vals, L.diag, L.agg, L.mstart, L.mem, L.b, L.x, L.y, L.r,
                         L.lrows, L.lidx, L.lsum};
         for (void* q : ptrs) dfree(c, q);
@@ -960,9 +952,14 @@ static int mg_aggregate(sph_ctx* c, OP op, const MGLevel& F, int emax, MGLevel& 
     return 0;
 }
 
-static int mg_vectors(sph_ctx* c, MGLevel& L, bool fine, bool coarsest) {
+static int mg_vectors(sph_ctx* c, MGLevel& L, bool fine, bool coarsest, float* fine_x) {
     if (!fine) SPH_TRY(dalloc(c, &L.b, L.n));
-    SPH_TRY(dalloc(c, &L.x, L.n));
+    if (fine) {
+        L.x = fine_x;
+        L.x_shared = true;
+    } else {
+        SPH_TRY(dalloc(c, &L.x, L.n));
+    }
     if (!coarsest) SPH_TRY(dalloc(c, &L.r, L.n));
     if (!fine && !coarsest) SPH_TRY(dalloc(c, &L.y, L.n));
     return 0;
@@ -972,8 +969,10 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
     *out = nullptr;
     *ok = 0;
     *why = SPH_MG_UNSUITED;
-    // long rows need the compact level-0 copy (its passes add their own tail sums)
-    if (pp.nr <= MG_COARSE_MAX || (pp.ltail && !pp.ecols) || !pp.vals32 || !pp.fdiag || !(q > 0)) return 0;
+    // long rows need the compact level-0 copy (its passes add their own tail
+    // sums); the finest level's x is the PPE's (the Krylov kernels write the
+    // pre-smoothing there)
+    if (pp.nr <= MG_COARSE_MAX || (pp.ltail && !pp.ecols) || !pp.vals32 || !pp.fdiag || !pp.mgx0 || !(q > 0)) return 0;
     const sph_domain& d = c->dom;
     const double ex = ((double)d.x_max - d.x_min) / q + 3.0, ey = ((double)d.y_max - d.y_min) / q + 3.0,
                  ez = ((double)d.z_max - d.z_min) / q + 3.0;
@@ -1081,7 +1080,7 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         hipLaunchKernelGGL(k_mg_pack16, dim3(grid1(pp.nr)), dim3(BLOCK), 0, c->stream, pp, mg->ev);
         MGTRY(launch_check(c, "mg_pack16"));
     }
-    for (int l = 0; l < nlev; l++) MGTRY(mg_vectors(c, mg->L[l], l == 0, l == nlev - 1));
+    for (int l = 0; l < nlev; l++) MGTRY(mg_vectors(c, mg->L[l], l == 0, l == nlev - 1, pp.mgx0));
     for (int l = 1; l + 1 < nlev; l++) {
         MGLevel& L = mg->L[l];
         int *flag = nullptr, *scan = nullptr;
@@ -1226,12 +1225,12 @@ int mg_apply(Eng& e, int which, int sc_done) {
             SPH_TRY(prof_end(e));                                                                    \
         }                                                                                            \
     } while (0)
-    // The pointwise passes ride on the row passes: a level's pre-smoothing
-    // on the restriction that forms its b (levels below the finest), the
-    // prolongation of a level's correction on the pass that forms it (the
-    // coarse post-smoothing, or the dense solve)
+    // The pointwise passes ride on the passes that form their inputs: the
+    // finest level's pre-smoothing on the Krylov kernel that forms p or s
+    // (k_kr_p, k_kr_s), a coarse level's on the restriction that forms its
+    // b, the prolongation of a level's correction on the pass that forms it
+    // (the coarse post-smoothing, or the dense solve)
     for (int l = 0; l + 1 < P.lmax; l++) {
-        if (l == 0) MGE("mg_pre", 0, P.down_off[l], P.down_cnt[l], k_mg_pre, l, which, sc_done);
         if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
         else MGE("mg_clong", 0, P.clong_off[l], P.clong_cnt[l], k_mg_clong, l, sc_done);
         MGE(l == 0 ? "mg_res0" : "mg_res", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_res, l,

@@ -77,6 +77,7 @@ struct PPEState {
            *shat = nullptr, *t = nullptr;
     float *fr = nullptr, *frhat = nullptr, *fp = nullptr, *fphat = nullptr, *fv = nullptr, *fs = nullptr,
           *fshat = nullptr, *ft = nullptr, *fx = nullptr, *frhs = nullptr;
+    float* mgx0 = nullptr;      // the multigrid V-cycle's finest-level x (its hierarchies use it)
     double* partA = nullptr;
     double* partB = nullptr;
     int nb = 0;
@@ -624,6 +625,8 @@ __global__ void __launch_bounds__(BLOCK) k_kr_p(Seg sg) {
     K.p[row] = pv;
     if (sizeof(X) == 8 && cd.ctl[CTL_MG] != 0.0) pp.fp[row] = (float)pv;   // the V-cycle's input
     else K.phat[row] = (X)((double)pv / (double)kdiag(pp, (X*)nullptr)[row]);
+    // the V-cycle's finest-level pre-smoothing of its input (float p)
+    if (pp.mgx0) pp.mgx0[row] = (float)(MG_OMEGA * (double)(float)pv / (double)pp.fdiag[row]);
 }
 
 // FP64 solve with the multigrid: p-hat (which = 0) or s-hat (1) from the
@@ -689,6 +692,8 @@ __global__ void __launch_bounds__(BLOCK) k_kr_s(Seg sg) {
         K.s[row] = sv;
         if (sizeof(X) == 8 && cd.ctl[CTL_MG] != 0.0) pp.fs[row] = (float)sv;
         else K.shat[row] = (X)((double)sv / (double)kdiag(pp, (X*)nullptr)[row]);
+        // the V-cycle's finest-level pre-smoothing of its input (float s)
+        if (pp.mgx0) pp.mgx0[row] = (float)(MG_OMEGA * (double)(float)sv / (double)pp.fdiag[row]);
         a = (double)sv * (double)sv;
     }
     double ta = block_canon(a, sh);
@@ -845,7 +850,7 @@ void ppe_free(sph_ctx* c) {
     if (!P) return;
     void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->ecols, P->gbase, P->slotmap, P->vals64, P->vals32,
                     P->diag, P->fdiag, P->rhs, P->x, P->pnew, P->r, P->rhat, P->p, P->phat, P->v, P->s, P->shat, P->t, P->fr, P->frhat, P->fp,
-                    P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->partA, P->partB, P->ltail, P->lrows, P->tail_off, P->tail_cols, P->tail_vals, P->lsum};
+                    P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->mgx0, P->partA, P->partB, P->ltail, P->lrows, P->tail_off, P->tail_cols, P->tail_vals, P->lsum};
     for (void* q : ptrs) dfree(c, q);
     mg_free(c, P->mg);
     delete P;
@@ -1138,6 +1143,9 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
                 for (auto q : fv) SPH_TRY(dalloc(c, q, nr));
             }
         }
+        // the V-cycle's finest-level x, filled with its pre-smoothing by the
+        // Krylov kernels that form p and s
+        if (c->prm.ppe_precond == SPH_PRECOND_MULTIGRID && nr > MG_COARSE_MAX) SPH_TRY(dalloc(c, &P->mgx0, nr));
     }
     return 0;
 }
@@ -1155,6 +1163,7 @@ PPEView ppe_view(const sph_ctx* c) {
     v.r = P->r; v.rhat = P->rhat; v.p = P->p; v.phat = P->phat; v.v = P->v; v.s = P->s; v.shat = P->shat; v.t = P->t;
     v.fr = P->fr; v.frhat = P->frhat; v.fp = P->fp; v.fphat = P->fphat; v.fv = P->fv; v.fs = P->fs;
     v.fshat = P->fshat; v.ft = P->ft; v.fx = P->fx; v.frhs = P->frhs;
+    v.mgx0 = P->mgx0;
     v.partA = P->partA; v.partB = P->partB;
     v.ltail = P->ltail; v.lrows = P->lrows; v.tail_off = P->tail_off; v.tail_cols = P->tail_cols;
     v.tail_vals = P->tail_vals; v.nlong = P->nlong; v.lsum = P->lsum;

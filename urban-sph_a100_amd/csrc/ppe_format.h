@@ -5,6 +5,11 @@
 
 namespace sph {
 
+// V-cycle weights of the multigrid preconditioner: damped Jacobi smoothing
+// x = w b / diag, and the over-correction of unsmoothed aggregation
+constexpr double MG_OMEGA = 0.9;
+constexpr double MG_SCALE = 1.5;
+
 // 16-bit column entries: 10 offset bits, up to 64 window groups per slice
 constexpr int EC_OFF_BITS = 10;
 constexpr int EC_OFF_MASK = (1 << EC_OFF_BITS) - 1;

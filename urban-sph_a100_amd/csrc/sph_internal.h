@@ -110,6 +110,8 @@ struct PPEView {
     float* fdiag;                 // MIXED: float diagonal of the inner (float) operator
     double *r, *rhat, *p, *phat, *v, *s, *shat, *t;
     float *fr, *frhat, *fp, *fphat, *fv, *fs, *fshat, *ft, *fx, *frhs;
+    float* mgx0;                  // the multigrid V-cycle's finest-level x (allocated with the PPE when the
+                                  // multigrid may run), or null: k_kr_p / k_kr_s write its pre-smoothing
     double *partA, *partB;
     // MIXED: a row of more than LONG_ROW entries (a coarse row beside
     // refined particles) keeps its first LONG_ROW entries in the SELL slices
