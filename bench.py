@@ -380,7 +380,15 @@ def run_serving(cases, abi, urbansph, dt, tol, budget=240e9, n_req=16, rate=4.0,
     st = srv.status()
     span = max(i.t_finish for i in infos) - (t0 + reqs[0]["t_arrival"])
     offered = sum(len(d[0]) * r["steps"] for d, r in zip(decks, reqs)) / max(reqs[-1]["t_arrival"], 1e-9)
-    out = dict(requests=n_req, rate_per_s=rate, failed=sum(i.state != abi.REQ_DONE for i in infos),
+    rc_name = {abi.E_NOCONV: "no PPE convergence", abi.E_NONFINITE: "state went non-finite"}
+    failed = [dict(request=k, particles=i.n, steps_done=i.steps_done, steps=i.steps,
+                   why=rc_name.get(i.rc, f"rc {i.rc}"), L=reqs[k]["L"], u_scale=round(reqs[k]["u_scale"], 3),
+                   building_height=round(reqs[k]["bh"], 3))
+              for k, i in enumerate(infos) if i.state != abi.REQ_DONE]
+    for f in failed:
+        log(f"[bench] serving: request {f['request']} (n={f['particles']}) failed after {f['steps_done']} of "
+            f"{f['steps']} steps: {f['why']}")
+    out = dict(requests=n_req, rate_per_s=rate, failed=len(failed), failed_requests=failed,
                offered_particle_steps_per_s=offered, span_s=span, lo=lo, hi=hi, steps_lo=steps_lo, steps_hi=steps_hi,
                latencies_s=sorted(round(float(x), 3) for x in lat),
                p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
@@ -723,8 +731,10 @@ def main():
             "offered_particle_steps_per_s": round(sv["offered_particle_steps_per_s"], 1),
             "span_s": round(sv["span_s"], 2), "latencies_s": sv["latencies_s"],
             "ticks": sv["ticks"], "max_running": sv["max_running"], "failed": sv["failed"],
+            "failed_requests": sv["failed_requests"],
             "note": "the offered load (particle-steps per second of arrivals) exceeds one GPU's throughput, so "
-                    "latency includes the time requests share the device with those admitted before them"}
+                    "latency includes the time requests share the device with those admitted before them; a "
+                    "request whose state goes non-finite is retired as failed at that step while the others go on"}
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         line["cpu_baseline"] = cpu_baseline(cases, abi, P, nrm, geom, args.dt, args.tol)

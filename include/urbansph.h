@@ -237,6 +237,14 @@ typedef struct sph_params {
                                    neighbours; 0 the 128-byte double records
                                    and field arrays.  The same results
                                    bitwise; 0 is for checking that           */
+    int    ppe_stencil;         /* MIXED, lattice row order, fixed particles:
+                                   1 (default) stores a 64-row slice whose
+                                   rows' column offsets (col - row) have a
+                                   union of at most 5/4 of its widest row
+                                   over that union, without columns (an
+                                   offset a row lacks holds the value 0); the
+                                   products read values only.  The same
+                                   results bitwise; 0 stores every column    */
 } sph_params;
 
 /* Sources of the IB marker normals (ib_normals). */
@@ -331,6 +339,9 @@ typedef struct sph_ppe_report {
                                    tile_halo_max (memory gathers)            */
     int    tile_halo_largest;   /* largest staged halo (records)             */
     long long tile_halo_total;  /* staged records of all workgroups          */
+    int    stencil_slices;      /* 64-row slices stored over one set of
+                                   column offsets (ppe_stencil)              */
+    long long stencil_nnz;      /* off-diagonal entries in those slices     */
 } sph_ppe_report;
 
 /* Why the multigrid preconditioner was not used (sph_ppe_report.mg_status). */

@@ -1624,7 +1624,7 @@ void oracle_params_default(sph_params* p) {
     p->ib_face_xlo = 0.764; p->ib_face_xhi = 0.836; p->ib_face_ylo = -0.036; p->ib_face_yhi = 0.036;
     p->ib_face_zlo = 0.0; p->ib_face_zhi = 0.152; p->ib_face_tol = 1e-5;
     // the library's LDS-staged record passes (the oracle has no such choice)
-    p->tile_cells = 0; p->tile_particles = 128; p->tile_halo_max = 1536; p->compact_records = 1;
+    p->tile_cells = 0; p->tile_particles = 128; p->tile_halo_max = 1536; p->compact_records = 1; p->ppe_stencil = 1;
 }
 
 void* oracle_create(const sph_params* prm, int n) {

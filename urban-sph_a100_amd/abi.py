@@ -93,7 +93,8 @@ class Params(C.Structure):
             "ib_face_xlo", "ib_face_xhi", "ib_face_ylo", "ib_face_yhi", "ib_face_zlo", "ib_face_zhi",
             "ib_face_tol")]
         + [("mg_sort_all", C.c_int), ("tile_cells", C.c_int), ("tile_particles", C.c_int),
-           ("tile_halo_max", C.c_int), ("compact_records", C.c_int)]
+           ("tile_halo_max", C.c_int), ("compact_records", C.c_int),
+           ("ppe_stencil", C.c_int)]
     )
 
     def update(self, **kw):
@@ -121,7 +122,8 @@ class PPEReport(C.Structure):
                 ("precond", C.c_int), ("mg_levels", C.c_int), ("mg_coarsest", C.c_int),
                 ("mg_status", C.c_int), ("mg_sorted_rows", C.c_longlong), ("mg_long_rows", C.c_int),
                 ("heavy_lists", C.c_int), ("tile_chunks", C.c_int), ("tile_raw_chunks", C.c_int),
-                ("tile_halo_largest", C.c_int), ("tile_halo_total", C.c_longlong)]
+                ("tile_halo_largest", C.c_int), ("tile_halo_total", C.c_longlong),
+                ("stencil_slices", C.c_int), ("stencil_nnz", C.c_longlong)]
 
 
 ORDER_CELL, ORDER_LATTICE = 0, 1

@@ -210,7 +210,7 @@ void sph_params_default(sph_params* p) {
     p->ib_face_xlo = 0.764; p->ib_face_xhi = 0.836; p->ib_face_ylo = -0.036; p->ib_face_yhi = 0.036;
     p->ib_face_zlo = 0.0; p->ib_face_zhi = 0.152; p->ib_face_tol = 1e-5;
     // LDS-staged record passes: off (measured slower than the per-particle gathers)
-    p->tile_cells = 0; p->tile_particles = 128; p->tile_halo_max = 1536; p->compact_records = 1;
+    p->tile_cells = 0; p->tile_particles = 128; p->tile_halo_max = 1536; p->compact_records = 1; p->ppe_stencil = 1;
 }
 
 static int validate(const sph_params* p, std::string& why) {
@@ -242,6 +242,7 @@ static int validate(const sph_params* p, std::string& why) {
         why = "tile_particles: 64, 128, 192 or 256"; return SPH_E_ARG;
     }
     if (p->tile_halo_max < 0 || p->tile_halo_max > 2048) { why = "tile_halo_max: 1..2048"; return SPH_E_ARG; }
+    if (p->ppe_stencil != 0 && p->ppe_stencil != 1) { why = "ppe_stencil: 0 or 1"; return SPH_E_ARG; }
     return 0;
 }
 

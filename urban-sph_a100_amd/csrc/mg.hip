@@ -299,11 +299,14 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
         if (BIG && lane == 0) moff[t - t0] = pos;
         for (int k = lane - 1; k < c; k += SLICE) {
             const int q = pos + k + 1;
-            const int J = k < 0 ? I : agg[op.col(m, k)];
+            const float v = k < 0 ? op.dg(m) : -op.val(m, k);
+            // a stored zero (a stencil slot the row lacks) joins the
+            // diagonal's sum, where it adds nothing
+            const int J = (k < 0 || v == 0.0f) ? I : agg[op.col(m, k)];
             sk[q] = (unsigned)J;
             jmin = min(jmin, J);
             jmax = max(jmax, J);
-            if (!BIG) sv[q] = k < 0 ? op.dg(m) : -op.val(m, k);
+            if (!BIG) sv[q] = v;
         }
         pos += c + 1;
     }
@@ -414,8 +417,8 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap_hash(OP op, int n1, const int*
         const int m = mem[t];
         const int c = op.cnt(m);
         for (int k = lane - 1; k < c && !full; k += SLICE) {
-            const int J = k < 0 ? I : agg[op.col(m, k)];
             const double v = k < 0 ? (double)op.dg(m) : -(double)op.val(m, k);
+            const int J = (k < 0 || v == 0.0) ? I : agg[op.col(m, k)];   // stored zeros: no column of their own
             const long long f = llrint(v * scale);
             int h = (int)(((unsigned)J * 2654435761u) >> 23) & (MG_HASH - 1);
             int probe = 0;
@@ -557,17 +560,26 @@ __global__ void __launch_bounds__(BLOCK) k_mg_tofloat(long long n, const double*
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
-// the V-cycle's compact level-0 operator: column word | half(value / diag) << 16
+// The V-cycle's compact level-0 operator: per stored entry its column word
+// | half(value / diag) << 16.  A stencil slice needs no columns: its halves
+// alone, packed into the first half of the slice's words (entry k of row r
+// at half-word 2 sl_off[r / 64] + 64 k + r % 64).
+__device__ __forceinline__ const unsigned short* ev_halves(const PPEView& pp, const unsigned* ev, int r) {
+    return (const unsigned short*)ev + 2 * pp.sl_off[r >> 6] + (r & 63);
+}
 __global__ void __launch_bounds__(BLOCK) k_mg_pack16(PPEView pp, unsigned* ev) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= pp.nr) return;
     const long long base = prow(pp.sl_off, r);
     const float d = pp.fdiag[r];
     const int m = pp.cnt[r];
+    const bool st = pp.spat && pp.spat[r >> 6] >= 0;
+    unsigned short* hv = const_cast<unsigned short*>(ev_halves(pp, ev, r));
     for (int k = 0; k < m; k++) {
         const long long e = base + pslot(k);
-        const _Float16 h = (_Float16)(pp.vals32[e] / d);
-        ev[e] = (unsigned)pp.ecols[e] | ((unsigned)__builtin_bit_cast(unsigned short, h) << 16);
+        const unsigned short h = __builtin_bit_cast(unsigned short, (_Float16)(pp.vals32[e] / d));
+        if (st) hv[pslot(k)] = h;
+        else ev[e] = (unsigned)pp.ecols[e] | ((unsigned)h << 16);
     }
 }
 
@@ -607,6 +619,30 @@ __device__ __forceinline__ double coarse_row(const CoarseOp& o, int r, const flo
 // order, accumulated in double; four independent loads in flight per lane
 __device__ __forceinline__ double mg_row16(const PPEView& pp, const unsigned* ev, int r, const float* x,
                                            const int* wb) {
+    const int st = stencil_of(pp, r);
+    if (st >= 0) {   // a stencil slice: halves only, columns row + offset
+        const gmem<unsigned short> hp = (gmem<unsigned short>)ev_halves(pp, ev, r);
+        const cmem<int> off = (cmem<int>)(pp.poff + st);
+        const gmem<float> xr = (gmem<float>)(x + r);
+        const int m = __builtin_amdgcn_readfirstlane(pp.cnt[r]);
+        auto hterm = [&](unsigned short u, int o) {
+            return (double)(float)__builtin_bit_cast(_Float16, u) * (double)xr[o];
+        };
+        double s = 0.0;
+        int k = 0;
+        for (; k + 4 <= m; k += 4) {
+            const unsigned short u0 = __builtin_nontemporal_load(hp + pslot(k + 0));
+            const unsigned short u1 = __builtin_nontemporal_load(hp + pslot(k + 1));
+            const unsigned short u2 = __builtin_nontemporal_load(hp + pslot(k + 2));
+            const unsigned short u3 = __builtin_nontemporal_load(hp + pslot(k + 3));
+            s += hterm(u0, off[k + 0]);
+            s += hterm(u1, off[k + 1]);
+            s += hterm(u2, off[k + 2]);
+            s += hterm(u3, off[k + 3]);
+        }
+        for (; k < m; k++) s += hterm(__builtin_nontemporal_load(hp + pslot(k)), off[k]);
+        return s;
+    }
     const unsigned* p = ev + prow(pp.sl_off, r);
     const int m = pp.cnt[r];
     auto term = [&](unsigned u) {

@@ -65,6 +65,12 @@ struct PPEState {
     unsigned short* ecols = nullptr;
     int* gbase = nullptr;
     unsigned char* slotmap = nullptr;   // lattice row order: slot of each entry in list order
+    // stencil slices (see k_stencil_union): per slice the first of its
+    // column offsets in poff, or -1 where the slice stores its columns
+    int* spat = nullptr;
+    int* poff = nullptr;
+    int nstencil = 0;           // stencil slices
+    long long nnz_stencil = 0;  // real entries in them
     double* vals64 = nullptr;
     float* vals32 = nullptr;
     double* diag = nullptr;
@@ -386,6 +392,131 @@ __global__ void __launch_bounds__(SLICE) k_ecols_build(int nr, const long long* 
         }
         ep[pslot(k2)] = (unsigned short)((lo << EC_OFF_BITS) | (c & EC_OFF_MASK));
     }
+}
+
+// Stencil slices.  With fixed particles on a lattice and rows in lattice
+// order, the 64 rows of a slice have nearly the same column offsets
+// (col - row): the same lattice stencil, cut by the domain's walls, with
+// the neighbours at exactly the search radius kept or dropped by rounding.
+// A slice whose rows' offsets have a union of at most 5/4 of its widest row
+// (and at most ST_CAP) stores every row over that union, in ascending
+// order, an offset the row lacks holding the value 0: its products then
+// read no columns, and the wave's x gathers of one slot are the contiguous
+// x[row + offset].  The nonzero entries keep their ascending column order,
+// so every sum adds the same terms in the same order (plus exact zeros).
+constexpr int ST_CAP = SORT_CAP;
+constexpr int ST_HASH = 512;
+static_assert(ST_CAP <= 255, "stencil slots are numbered in the 8-bit slot map");
+// The union of one slice's offsets, ascending, into ptmp[ST_CAP * slice];
+// wp[slice] = its size, or 0 when the slice keeps its columns.
+__global__ void __launch_bounds__(SLICE) k_stencil_union(int nr, const long long* sl_off, const int* cnt,
+                                                         const int* cols, int* wp, int* ptmp) {
+    __shared__ int hk[ST_HASH];
+    __shared__ int uo[ST_CAP];
+    __shared__ int nd, wmax, bad;
+    const int s = blockIdx.x, l = threadIdx.x, r = s * SLICE + l;
+    for (int h = l; h < ST_HASH; h += SLICE) hk[h] = INT_MIN;
+    if (l == 0) { nd = 0; wmax = 0; bad = 0; }
+    __syncthreads();
+    const int m = r < nr ? cnt[r] : 0;
+    atomicMax(&wmax, m);
+    const int* cp = cols + sl_off[s] + l;
+    for (int k = 0; k < m; k++) {
+        const int d = cp[pslot(k)] - r;
+        int h = (int)(((unsigned)d * 2654435761u) >> 23) & (ST_HASH - 1);
+        for (int probe = 0; probe < ST_HASH; probe++) {
+            const int old = atomicCAS(&hk[h], INT_MIN, d);
+            if (old == INT_MIN) {
+                const int u = atomicAdd(&nd, 1);
+                if (u < ST_CAP) uo[u] = d;
+                break;
+            }
+            if (old == d) break;
+            h = (h + 1) & (ST_HASH - 1);
+        }
+    }
+    __syncthreads();
+    const int n = nd, w = wmax;
+    if (n > ST_CAP || n == 0 || 4 * n > 5 * w) {
+        if (l == 0) wp[s] = 0;
+        return;
+    }
+    // ascending: each distinct offset's rank
+    int* out = ptmp + (size_t)s * ST_CAP;
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int i = l; i < n; i += SLICE) {
+        const int v = uo[i];
+        int rank = 0;
+        for (int j = 0; j < n; j++) rank += uo[j] < v ? 1 : 0;
+        out[rank] = v;
+        lo = min(lo, v);
+        hi = max(hi, v);
+    }
+    // every row's columns row + offset stay inside the operator
+    __shared__ int slo, shi;
+    if (l == 0) { slo = INT_MAX; shi = INT_MIN; }
+    __syncthreads();
+    atomicMin(&slo, lo);
+    atomicMax(&shi, hi);
+    __syncthreads();
+    if (r < nr && ((long long)r + slo < 0 || (long long)r + shi >= nr)) atomicOr(&bad, 1);
+    __syncthreads();
+    if (l == 0) wp[s] = bad ? 0 : n;
+}
+// Entries per row of the new layout: a stencil slice's rows all hold its union.
+__global__ void __launch_bounds__(BLOCK) k_stencil_cnt(int nr, const int* cnt, const int* wp, int* cnt2) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    const int n = wp[r >> 6];
+    cnt2[r] = n > 0 ? n : cnt[r];
+}
+// The new layout (one 64-thread block per slice): a stencil slice's columns
+// row + offset for every slot and its slot map pointed at the offsets' slots;
+// other slices copied.  spat[slice] = the first of its offsets in poff, or -1.
+__global__ void __launch_bounds__(SLICE) k_stencil_relayout(int nr, const long long* sl_off, const int* cnt,
+                                                            const int* cols, const unsigned char* slotmap,
+                                                            const int* wp, const int* ptmp,
+                                                            const long long* sl_off2, int* cols2,
+                                                            unsigned char* slotmap2, int* spat, int* poff) {
+    __shared__ int off[ST_CAP];
+    const int s = blockIdx.x, l = threadIdx.x, r = s * SLICE + l;
+    const int n = wp[s];
+    const long long b2 = sl_off2[s];
+    if (l == 0) spat[s] = n > 0 ? (int)(b2 / SLICE) : -1;
+    for (int k = l; k < n; k += SLICE) {
+        off[k] = ptmp[(size_t)s * ST_CAP + k];
+        poff[b2 / SLICE + k] = off[k];
+    }
+    __syncthreads();
+    if (r >= nr) return;
+    const int m = cnt[r];
+    const int* cp = cols + sl_off[s] + l;
+    const unsigned char* sm = slotmap + sl_off[s] + l;
+    int* cq = cols2 + b2 + l;
+    unsigned char* sq = slotmap2 + b2 + l;
+    if (n > 0) {
+        for (int k = 0; k < n; k++) cq[pslot(k)] = r + off[k];
+        for (int t = 0; t < m; t++) {   // t: rank in the neighbour list
+            const int d = cp[pslot(sm[pslot(t)])] - r;
+            int lo = 0, hi = n - 1;   // off is ascending and holds d
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (off[mid] < d) lo = mid + 1;
+                else hi = mid;
+            }
+            sq[pslot(t)] = (unsigned char)lo;
+        }
+    } else {
+        for (int k = 0; k < m; k++) {
+            cq[pslot(k)] = cp[pslot(k)];
+            sq[pslot(k)] = sm[pslot(k)];
+        }
+    }
+}
+// real entries of the stencil slices, per row (for the byte counts)
+__global__ void __launch_bounds__(BLOCK) k_stencil_real(int nr, const int* cnt, const int* wp, long long* out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nr) out[r] = wp[r >> 6] > 0 ? cnt[r] : 0;
 }
 
 // This thread's PPE row: case descriptor `cd`, row index `row` in the case
@@ -848,7 +979,8 @@ __global__ void __launch_bounds__(BLOCK) k_zero(long long n, T* a) {
 void ppe_free(sph_ctx* c) {
     PPEState* P = c->ppe;
     if (!P) return;
-    void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->ecols, P->gbase, P->slotmap, P->vals64, P->vals32,
+    void* ptrs[] = {P->rowof, P->rows, P->cnt, P->sl_off, P->cols, P->ecols, P->gbase, P->slotmap, P->spat, P->poff,
+                    P->vals64, P->vals32,
                     P->diag, P->fdiag, P->rhs, P->x, P->pnew, P->r, P->rhat, P->p, P->phat, P->v, P->s, P->shat, P->t, P->fr, P->frhat, P->fp,
                     P->fphat, P->fv, P->fs, P->fshat, P->ft, P->fx, P->frhs, P->mgx0, P->partA, P->partB, P->ltail, P->lrows, P->tail_off, P->tail_cols, P->tail_vals, P->lsum};
     for (void* q : ptrs) dfree(c, q);
@@ -1026,6 +1158,77 @@ static int ppe_nnz(sph_ctx* c, PPEState* P) {
     return 0;
 }
 
+// Lattice-ordered structure (sorted columns and slot map in place) ->
+// stencil slices where they pay (k_stencil_union); the slices' layout, the
+// row counts, columns and slot map are replaced.  Runs before the 16-bit
+// columns and the values are allocated.
+static int stencil_slices(sph_ctx* c, PPEState* P) {
+    const int nr = P->nr, ns = P->nslices;
+    int *wp = nullptr, *ptmp = nullptr;
+    SPH_TRY(dalloc(c, &wp, ns));
+    SPH_TRY(dalloc(c, &ptmp, (size_t)ns * ST_CAP));
+    hipLaunchKernelGGL(k_stencil_union, dim3(ns), dim3(SLICE), 0, c->stream, nr, (const long long*)P->sl_off,
+                       (const int*)P->cnt, (const int*)P->cols, wp, ptmp);
+    SPH_TRY(launch_check(c, "stencil_union"));
+    long long* real = nullptr;
+    long long* tmp = nullptr;
+    SPH_TRY(dalloc(c, &real, nr));
+    SPH_TRY(dalloc(c, &tmp, nr));
+    hipLaunchKernelGGL(k_stencil_real, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->cnt,
+                       (const int*)wp, real);
+    SPH_TRY(launch_check(c, "stencil_real"));
+    long long nnz_st = 0;
+    SPH_TRY(exclusive_scan_i64(c, real, tmp, nr, &nnz_st));
+    dfree(c, real);
+    dfree(c, tmp);
+    if (nnz_st == 0) {   // no slice pays: the layout stays
+        dfree(c, wp);
+        dfree(c, ptmp);
+        return 0;
+    }
+    int* cnt2 = nullptr;
+    long long *w = nullptr, *sl_off2 = nullptr, slots2 = 0;
+    SPH_TRY(dalloc(c, &cnt2, nr));
+    hipLaunchKernelGGL(k_stencil_cnt, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->cnt,
+                       (const int*)wp, cnt2);
+    SPH_TRY(launch_check(c, "stencil_cnt"));
+    SPH_TRY(dalloc(c, &w, ns + 1));
+    SPH_TRY(dalloc(c, &sl_off2, ns + 1));
+    hipLaunchKernelGGL(k_row_slice_width, dim3(grid1(ns)), dim3(BLOCK), 0, c->stream, nr, ns, (const int*)cnt2, w);
+    SPH_TRY(launch_check(c, "stencil_width"));
+    SPH_TRY(exclusive_scan_i64(c, w, sl_off2, ns, &slots2));
+    dfree(c, w);
+    int* cols2 = nullptr;
+    unsigned char* slotmap2 = nullptr;
+    SPH_TRY(dalloc(c, &cols2, slots2 + 1));
+    SPH_TRY(dalloc(c, &slotmap2, slots2 + 1));
+    SPH_TRY(dalloc(c, &P->spat, ns));
+    SPH_TRY(dalloc(c, &P->poff, slots2 / SLICE + 1));
+    hipLaunchKernelGGL(k_zero<int>, dim3(grid1(slots2 + 1)), dim3(BLOCK), 0, c->stream, slots2 + 1, cols2);
+    hipLaunchKernelGGL(k_stencil_relayout, dim3(ns), dim3(SLICE), 0, c->stream, nr, (const long long*)P->sl_off,
+                       (const int*)P->cnt, (const int*)P->cols, (const unsigned char*)P->slotmap, (const int*)wp,
+                       (const int*)ptmp, (const long long*)sl_off2, cols2, slotmap2, P->spat, P->poff);
+    SPH_TRY(launch_check(c, "stencil_relayout"));
+    std::vector<int> hwp(ns);
+    SPH_HIP(c, hipMemcpyAsync(hwp.data(), wp, sizeof(int) * ns, hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    P->nstencil = 0;
+    for (int v : hwp) P->nstencil += v > 0 ? 1 : 0;
+    P->nnz_stencil = nnz_st;
+    dfree(c, wp);
+    dfree(c, ptmp);
+    dfree(c, P->cnt);
+    dfree(c, P->sl_off);
+    dfree(c, P->cols);
+    dfree(c, P->slotmap);
+    P->cnt = cnt2;
+    P->sl_off = sl_off2;
+    P->cols = cols2;
+    P->slotmap = slotmap2;
+    P->nnz_slots = slots2;
+    return 0;
+}
+
 // Rows and column structure of a case's PPE from its neighbour list.
 static int ppe_build_ordered(sph_ctx* c, bool allow_lattice);
 int ppe_build(sph_ctx* c) { return ppe_build_ordered(c, true); }
@@ -1065,6 +1268,7 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
     // MIXED keeps the SELL slices at most LONG_ROW wide (FP64 keeps each row
     // whole: its products add the entries in the reference's order)
     SPH_TRY(ppe_count_and_slices(c, P, mixed));
+    SPH_TRY(ppe_nnz(c, P));   // real entries (stencil slices add stored zeros)
     SPH_TRY(dalloc(c, &P->cols, P->nnz_slots + 1));
     hipLaunchKernelGGL(k_zero<int>, dim3(grid1(P->nnz_slots + 1)), dim3(BLOCK), 0, c->stream, P->nnz_slots + 1, P->cols);
     if (lattice) {
@@ -1087,6 +1291,8 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
             return ppe_build_ordered(c, false);
         }
         P->ordering = SPH_ORDER_LATTICE;
+        // fixed particles keep their lattice stencils from step to step
+        if (c->prm.ppe_stencil && !c->prm.ale_positions) SPH_TRY(stencil_slices(c, P));
     } else {
         hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
                            (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols,
@@ -1115,13 +1321,19 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
             dfree(c, P->gbase); P->gbase = nullptr;
         }
     }
-    SPH_TRY(ppe_nnz(c, P));
     if (P->ecols) { dfree(c, P->cols); P->cols = nullptr; }   // the 16-bit copy replaces them
     // FP64 with the multigrid: a float copy of the operator builds the
     // hierarchy, and float copies of p and s feed the V-cycle
     const bool fp64_mg = !mixed && c->prm.ppe_precond == SPH_PRECOND_MULTIGRID &&
                          c->prm.ppe_solver == SPH_PPE_BICGSTAB && nr > MG_COARSE_MAX;
-    if (mixed || fp64_mg) SPH_TRY(dalloc(c, &P->vals32, P->nnz_slots + 1));
+    if (mixed || fp64_mg) {
+        // the assembly writes the real entries; the stored zeros of stencil
+        // slices keep these
+        SPH_TRY(dalloc(c, &P->vals32, P->nnz_slots + 1));
+        hipLaunchKernelGGL(k_zero<float>, dim3(grid1(P->nnz_slots + 1)), dim3(BLOCK), 0, c->stream, P->nnz_slots + 1,
+                           P->vals32);
+        SPH_TRY(launch_check(c, "ppe_zero_vals"));
+    }
     if (!mixed) SPH_TRY(dalloc(c, &P->vals64, P->nnz_slots + 1));
     SPH_TRY(dalloc(c, &P->diag, nr));
     if (mixed || fp64_mg) SPH_TRY(dalloc(c, &P->fdiag, nr));
@@ -1159,6 +1371,7 @@ PPEView ppe_view(const sph_ctx* c) {
     v.nr = P->nr; v.nb = P->nb;
     v.rows = P->rows; v.rowof = P->rowof; v.cnt = P->cnt; v.sl_off = P->sl_off; v.cols = P->cols;
     v.ecols = P->ecols; v.gbase = P->gbase; v.fdiag = P->fdiag; v.slotmap = P->slotmap;
+    v.spat = P->spat; v.poff = P->poff;
     v.vals64 = P->vals64; v.vals32 = P->vals32; v.diag = P->diag; v.rhs = P->rhs; v.x = P->x; v.pnew = P->pnew;
     v.r = P->r; v.rhat = P->rhat; v.p = P->p; v.phat = P->phat; v.v = P->v; v.s = P->s; v.shat = P->shat; v.t = P->t;
     v.fr = P->fr; v.frhat = P->frhat; v.fp = P->fp; v.fphat = P->fphat; v.fv = P->fv; v.fs = P->fs;
@@ -1309,7 +1522,7 @@ static int kr_graph(Eng& e, int gi, double spmv_bytes, bool mg, hipGraphExec_t* 
 template <class V, class X>
 static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, bool mg = false) {
     const int nc = (int)e.cs.size();
-    double spmv_bytes = 0.0;
+    double spmv_bytes = 0.0, spmv_bytes0 = 0.0;
     int maxit = 0;
     for (int k = 0; k < nc; k++) {
         if (e.h_ctl[k * CTL_COUNT + CTL_SKIP] != 0.0) continue;
@@ -1317,15 +1530,19 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, bool
         // index, value) once, and per row its count, diagonal, own x, the
         // output and the dot partner (plus the slice group bases of 16-bit
         // columns); the x gathers of the neighbours are cache traffic
+        // (stencil slices: values only, their columns implied)
         const PPEState* P = e.cs[k]->ppe;
         const bool i16 = P->ecols != nullptr;
         const double diag_b = sizeof(X) == 4 ? 4.0 : 8.0;
-        spmv_bytes += (double)P->nnz * ((i16 ? 2.0 : 4.0) + sizeof(V)) +
-                      (double)P->nr * (4.0 + diag_b + 3.0 * sizeof(X) + (i16 ? 4.0 : 0.0));
+        const double cols_nnz = (double)(P->nnz - P->nnz_stencil);
+        const double per_row = (double)P->nr * (4.0 + diag_b + 3.0 * sizeof(X) + (i16 ? 4.0 : 0.0));
+        spmv_bytes += (double)P->nnz * sizeof(V) + cols_nnz * (i16 ? 2.0 : 4.0) + per_row;
+        // the V-cycle's level-0 passes: 4 bytes per entry (column word and
+        // half value), 2 in stencil slices
+        spmv_bytes0 += cols_nnz * 4.0 + (double)P->nnz_stencil * 2.0 + per_row;
         maxit = std::max(maxit, (int)e.h_ctl[k * CTL_COUNT + CTL_MAXIT]);
     }
-    // the V-cycle's level-0 passes read 4 bytes per entry (compact copy)
-    if (mg) e.mg_plan.spmv_bytes0 = spmv_bytes * (2.0 / 3.0);
+    if (mg) e.mg_plan.spmv_bytes0 = spmv_bytes0;
     RLONG(V, X, 0);
     RL("kr_init", spmv_bytes, (k_kr_init<V, X>));
     KF(FIN_INIT);
@@ -1392,6 +1609,8 @@ int ppe_solve(Eng& e) {
         c->rep.slots = c->ppe ? c->ppe->nnz_slots : 0;
         c->rep.index16 = c->ppe && c->ppe->ecols ? 1 : 0;
         c->rep.tail_rows = c->ppe ? c->ppe->nlong : 0;
+        c->rep.stencil_slices = c->ppe ? c->ppe->nstencil : 0;
+        c->rep.stencil_nnz = c->ppe ? c->ppe->nnz_stencil : 0;
         c->rep.heavy_lists = c->n_heavy;
         if (e.tile_lanes) {
             c->rep.tile_chunks = c->tl_nchunk;
@@ -1545,7 +1764,12 @@ int ppe_solve(Eng& e) {
         std::vector<int> refine(nc, 0);
         for (int k = 0; k < nc; k++) active[k] = nr_of(e.cs[k]) > 0;
         double res_bytes = 0.0;
-        for (sph_ctx* c : e.cs) res_bytes += (double)c->rep.nnz * 8.0 + (double)c->rep.unknowns * 28.0;
+        for (sph_ctx* c : e.cs) {   // float values, the stored columns, per row x, b, diag, count, r
+            const PPEState* P = c->ppe;
+            if (!P) continue;
+            res_bytes += (double)P->nnz * 4.0 + (double)(P->nnz - P->nnz_stencil) * (P->ecols ? 2.0 : 4.0) +
+                         (double)P->nr * 28.0;
+        }
         for (int outer = 0;; outer++) {
             bool any = false;
             for (int k = 0; k < nc; k++) any |= active[k] != 0;

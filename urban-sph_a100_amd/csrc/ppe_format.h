@@ -79,6 +79,43 @@ __device__ __forceinline__ double offdiag_u16(const unsigned short* ep, const V*
     return s;
 }
 
+// The same over a stencil slice: column row + off[k], off the slice's
+// ascending offsets, so a slot's x gathers are one contiguous run of the
+// wave's rows; the stored zeros add exact zeros.  The offsets are the same
+// for the whole wave and constant during the launch: read through the
+// constant address space they become scalar loads.  Values and x through the
+// global address space.
+template <class T> using gmem = const __attribute__((address_space(1))) T*;
+template <class T> using cmem = const __attribute__((address_space(4))) T*;
+template <class V, class X>
+__device__ __forceinline__ double offdiag_stencil(const V* vals, int m, const int* offs, int r, const X* xv) {
+    const gmem<V> vp = (gmem<V>)vals;
+    const gmem<X> x = (gmem<X>)(xv + r);
+    const cmem<int> off = (cmem<int>)offs;
+    double s = 0.0;
+    int k = 0;
+    for (; k + 4 <= m; k += 4) {
+        const int o0 = off[k + 0], o1 = off[k + 1], o2 = off[k + 2], o3 = off[k + 3];
+        const V v0 = __builtin_nontemporal_load(vp + pslot(k + 0));
+        const V v1 = __builtin_nontemporal_load(vp + pslot(k + 1));
+        const V v2 = __builtin_nontemporal_load(vp + pslot(k + 2));
+        const V v3 = __builtin_nontemporal_load(vp + pslot(k + 3));
+        const X x0 = x[o0], x1 = x[o1], x2 = x[o2], x3 = x[o3];
+        s += (double)v0 * (double)x0;
+        s += (double)v1 * (double)x1;
+        s += (double)v2 * (double)x2;
+        s += (double)v3 * (double)x3;
+    }
+    for (; k < m; k++) s += (double)__builtin_nontemporal_load(vp + pslot(k)) * (double)x[off[k]];
+    return s;
+}
+// Row r's slice: the first of its stencil offsets in pp.poff, or -1.  Every
+// active lane of the calling wave holds a row of the same slice (rows are
+// handed out in whole slices), so the value is read as wave-uniform.
+__device__ __forceinline__ int stencil_of(const PPEView& pp, int r) {
+    return pp.spat ? __builtin_amdgcn_readfirstlane(pp.spat[r >> 6]) : -1;
+}
+
 // Group bases of the block's slices into LDS (every thread of the block
 // takes part; call before the row guard).
 __device__ __forceinline__ void load_group_bases(const PPEView& pp, int first, int* wbase) {
@@ -91,10 +128,17 @@ template <class V, class X, class DG>
 __device__ __forceinline__ double spmv_row(const PPEView& pp, int r, const V* vals, const DG* dg, const X* x,
                                            const int* wbase) {
     const long long base = prow(pp.sl_off, r);
-    const int m = pp.cnt[r];
+    const int st = stencil_of(pp, r);
     double s;
-    if (pp.ecols) s = offdiag_u16<V, X>(pp.ecols + base, vals + base, m, wbase + (threadIdx.x & ~63), x);
-    else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
+    if (st >= 0) {
+        // a stencil slice's rows all hold its offsets: a uniform count
+        const int m = __builtin_amdgcn_readfirstlane(pp.cnt[r]);
+        s = offdiag_stencil<V, X>(vals + base, m, pp.poff + st, r, x);
+    } else {
+        const int m = pp.cnt[r];
+        if (pp.ecols) s = offdiag_u16<V, X>(pp.ecols + base, vals + base, m, wbase + (threadIdx.x & ~63), x);
+        else s = offdiag_i32<V, X>(pp.cols + base, vals + base, m, x);
+    }
     if (pp.ltail) {
         const int u = pp.ltail[r];
         if (u >= 0) s += pp.lsum[u];

@@ -104,6 +104,12 @@ struct PPEView {
     const unsigned short* ecols;  // MIXED: 16-bit columns (window group, offset), or null
     const int* gbase;             // MIXED: first column of each window group, 64 per slice
     const unsigned char* slotmap; // MIXED lattice order: slot of each entry, by its rank in the neighbour list
+    // MIXED lattice order, fixed particles: per 64-row slice the first of its
+    // column offsets in poff, or -1.  A stencil slice stores every row over
+    // the same ascending offsets (col = row + poff[spat + k]; an offset the
+    // row lacks holds the value 0), so its products read no columns.
+    const int* spat;
+    const int* poff;
     double* vals64;
     float* vals32;
     double *diag, *rhs, *x, *pnew;
