@@ -93,7 +93,10 @@ def run_precision(cases, abi, urbansph, P, nrm, geom, precision, steps, warmup, 
                prof=prof, prof_steps=prof_steps, rel_res=rep.rel_residual, refinements=rep.refinements,
                precond=rep.precond, mg_levels=rep.mg_levels, mg_coarsest=rep.mg_coarsest, mg_status=rep.mg_status,
                warmup_s=t_setup, dev_bytes=case.device_bytes(), pres=case.field("PRES"), ux=case.field("UX"),
-               first=first, tuning=case.tuning())
+               first=first, tuning=case.tuning(),
+               ppe=dict(rows=rep.unknowns, nnz=rep.nnz, slots=rep.slots, ordering=["cell", "lattice"][rep.ordering],
+                        index16=rep.index16, stencil_slices=rep.stencil_slices, stencil_nnz=rep.stencil_nnz,
+                        slices=(rep.unknowns + 63) // 64))
     case.close()
     return out
 
@@ -620,7 +623,8 @@ def main():
                                f"MOST L=+50 m, SSM Cs=0.12, dt={args.dt} s, BiCGSTAB tol {args.tol:g}",
                    "particles": n, "precision": "fp64" if main_name == "fp64" else "mixed",
                    "ppe_precond": "multigrid" if m["precond"] == abi.PRECOND_MULTIGRID else "jacobi",
-                   "mg_levels": m["mg_levels"], "mg_coarsest": m["mg_coarsest"], "mg_status": m["mg_status"]},
+                   "mg_levels": m["mg_levels"], "mg_coarsest": m["mg_coarsest"], "mg_status": m["mg_status"],
+                   "ppe": m["ppe"]},
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                             "p99": round(float(np.percentile(lat, 99)), 3),
                             "what": "per-step latency of the resident 2M case"},
