@@ -573,6 +573,7 @@ __device__ __forceinline__ void assemble_body(const CaseDesc& cd, int i, const N
     // with fixed particles the values of a search are written by its first
     // step only; later steps compute the right-hand side
     V* out = (jacobi || cd.vals_fresh) ? nullptr : wvals_of(pp, (V*)nullptr) + prow(pp.sl_off, r);
+    const bool full = jacobi || out != nullptr;   // the coefficients of every pair are needed
     const unsigned char* smap = pp.slotmap ? pp.slotmap + prow(pp.sl_off, r) : nullptr;
     int slot = 0;
     // read once: the stores of the loop could alias the descriptor
@@ -617,20 +618,27 @@ __device__ __forceinline__ void assemble_body(const CaseDesc& cd, int i, const N
         bix += (double)(mj / rhoj * (R)(uxj - uxi) * tdwxc * hm);
         biy += (double)(mj / rhoj * (R)(uyj - uyi) * tdwyc * hm);
         biz += (double)(mj / rhoj * (R)(uzj - uzi) * tdwzc * hm);
-        R cc = (R)2.0 * mj / rhoj * (dx * tdwxc + dy * tdwyc + dz * tdwzc) / tdist / tdist;
-        double ccd = (double)cc;
-        Aij += ccd;
-        if (jacobi) AijPij += ccd * qj;
-        if (!src.self(j, i)) {
-            if (jrow) {
-                if (out) {
-                    if (slot < LONG_ROW || !pp.ltail) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
-                    else pp.tail_vals[pp.tail_off[pp.ltail[r]] + (slot - LONG_ROW)] = (float)cc;
+        auto coef = [&]() { return (R)2.0 * mj / rhoj * (dx * tdwxc + dy * tdwyc + dz * tdwzc) / tdist / tdist; };
+        if (full) {
+            R cc = coef();
+            double ccd = (double)cc;
+            Aij += ccd;
+            if (jacobi) AijPij += ccd * qj;
+            if (!src.self(j, i)) {
+                if (jrow) {
+                    if (out) {
+                        if (slot < LONG_ROW || !pp.ltail) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
+                        else pp.tail_vals[pp.tail_off[pp.ltail[r]] + (slot - LONG_ROW)] = (float)cc;
+                    }
+                    slot++;
+                } else {
+                    kn += ccd * qj;
                 }
-                slot++;
-            } else {
-                kn += ccd * qj;
             }
+        } else if (!jrow && !src.self(j, i)) {
+            // values and diagonal kept from the search's first step: only the
+            // known neighbours' share of the right-hand side needs the coefficient
+            kn += (double)coef() * qj;
         }
     }
     double bi = 1.0 * (bix + biy + biz + 0.0) / dt;
