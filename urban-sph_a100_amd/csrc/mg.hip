@@ -76,6 +76,8 @@ struct MGLevel {
     double* lsum = nullptr;
 };
 constexpr int MG_CLONG = 128;
+constexpr int MG_G = 8;   // lanes per row of the coarse residual and post-smoothing passes
+static_assert(BLOCK % MG_G == 0 && SLICE % MG_G == 0, "a row's lanes sit in one wave");
 struct MGState {
     long long id = 0;        // distinct for every hierarchy built (table signatures)
     int nlev = 0;            // levels, the PPE and the coarsest included
@@ -605,15 +607,6 @@ struct MGLv {
     const unsigned* ev;           // level 0: compact operator, or null
     double* lsum;                 // level 0: tail sums of the long rows; coarse: long rows' products
 };
-__device__ __forceinline__ CoarseOp lv_op(const MGLv& L) { return CoarseOp{L.sl_off, L.cnt, L.cols, L.vals, L.diag}; }
-
-__device__ __forceinline__ double coarse_row(const CoarseOp& o, int r, const float* x) {
-    const long long base = prow(o.sl_off, r);
-    const int m = o.cnt_[r];
-    double s = 0.0;
-    for (int k = 0; k < m; k++) s += (double)o.vals[base + pslot(k)] * (double)x[o.cols[base + pslot(k)]];
-    return (double)o.diag[r] * (double)x[r] - s;
-}
 
 // sum_k (a_rk / d_r) x[col_k] over the compact level-0 operator, in entry
 // order, accumulated in double; four independent loads in flight per lane
@@ -665,14 +658,6 @@ __device__ __forceinline__ double mg_row16(const PPEView& pp, const unsigned* ev
     return s;
 }
 
-// (A x)_row of a coarse level over its x; a long row's product comes from
-// k_mg_clong
-__device__ __forceinline__ double coarse_ax(const MGLv& L, int row) {
-    const int u = L.lidx ? L.lidx[row] : -1;
-    if (u >= 0) return (double)L.diag[row] * (double)L.x[row] - L.lsum[u];
-    return coarse_row(lv_op(L), row, L.x);
-}
-
 // block -> (case, first row of the level); the block returns when the
 // case's solve is done
 #define MG_HEAD(tab, lvl)                                      \
@@ -691,12 +676,12 @@ __device__ __forceinline__ void mg_prolong_members(const MGLv& F, int a, float z
         F.x[m] = (float)((double)F.x[m] + MG_SCALE * (double)z);
     }
 }
-// r = b - A x
+// level 0 (l = 0): r = b - A x
 __global__ void __launch_bounds__(BLOCK) k_mg_res(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
                                                   int which, int sc_done) {
     __shared__ int wbase[BLOCK];
     MG_HEAD(tab, l);
-    if (l == 0) {
+    {
         const PPEView& pp = cd.ppe;
         load_group_bases(pp, te_.y, wbase);
         if (row >= L.n) return;
@@ -709,17 +694,14 @@ __global__ void __launch_bounds__(BLOCK) k_mg_res(const CaseDesc* cds, const MGL
         } else {
             L.r[row] = (float)((double)b[row] - spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, L.x, wbase));
         }
-    } else {
-        if (row >= L.n) return;
-        L.r[row] = (float)((double)L.b[row] - coarse_ax(L, row));
     }
 }
-// z = x + w (b - A x) / diag  (level 0: into p-hat or s-hat)
+// level 0 (l = 0): z = x + w (b - A x) / diag into p-hat or s-hat
 __global__ void __launch_bounds__(BLOCK) k_mg_post(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
                                                    int which, int sc_done) {
     __shared__ int wbase[BLOCK];
     MG_HEAD(tab, l);
-    if (l == 0) {
+    {
         const PPEView& pp = cd.ppe;
         load_group_bases(pp, te_.y, wbase);
         if (row >= L.n) return;
@@ -734,12 +716,55 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(const CaseDesc* cds, const MG
             const double ax = spmv_row(pp, row, (const float*)pp.vals32, pp.fdiag, L.x, wbase);
             out[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)b[row] - ax) / d);
         }
-    } else {
-        if (row >= L.n) return;
-        const double ax = coarse_ax(L, row);
-        const float y = (float)((double)L.x[row] + MG_OMEGA * ((double)L.b[row] - ax) / (double)L.diag[row]);
-        L.y[row] = y;
-        mg_prolong_members(lv[te_.x * MG_MAX_LEVELS + l - 1], row, y);   // into level l - 1
+    }
+}
+// Coarse levels: MG_G lanes per row.  A coarse row holds a few dozen
+// entries, and a level of a few thousand rows gives a thread per row too
+// little company to hide its chain of dependent gathers.  Lane g adds
+// entries g, g + MG_G, ... in order, then a fixed shuffle tree adds the
+// group's sums, so the result does not depend on the launch.  A long row's
+// product comes from k_mg_clong.  tab: BLOCK / MG_G rows per block; every
+// lane of a group has the same row, so a group returns or stays whole.
+__device__ __forceinline__ double coarse_ax_g(const MGLv& L, int row, int g) {
+    const int u = L.lidx ? L.lidx[row] : -1;
+    double s = 0.0;
+    if (u < 0) {
+        const long long base = prow(L.sl_off, row);
+        const int m = L.cnt[row];
+        for (int k = g; k < m; k += MG_G) s += (double)L.vals[base + pslot(k)] * (double)L.x[L.cols[base + pslot(k)]];
+    }
+#pragma unroll
+    for (int off = MG_G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return (double)L.diag[row] * (double)L.x[row] - (u >= 0 ? L.lsum[u] : s);
+}
+#define MG_HEAD_G(tab, lvl)                                    \
+    const int2 te_ = (tab)[blockIdx.x];                        \
+    const CaseDesc& cd = cds[te_.x];                           \
+    if (cd.sc[sc_done] != 0.0) return;                         \
+    const MGLv& L = lv[te_.x * MG_MAX_LEVELS + (lvl)];         \
+    const int row = te_.y + (int)threadIdx.x / MG_G;           \
+    const int g = (int)threadIdx.x % MG_G;
+// r = b - A x on a coarse level
+__global__ void __launch_bounds__(BLOCK) k_mg_res_g(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                    int sc_done) {
+    MG_HEAD_G(tab, l);
+    if (row >= L.n) return;
+    const double ax = coarse_ax_g(L, row, g);
+    if (g == 0) L.r[row] = (float)((double)L.b[row] - ax);
+}
+// y = x + w (b - A x) / diag on a coarse level, and its prolongation into
+// level l - 1 (x += s y over the aggregate's members, a lane each)
+__global__ void __launch_bounds__(BLOCK) k_mg_post_g(const CaseDesc* cds, const MGLv* lv, const int2* tab, int l,
+                                                     int sc_done) {
+    MG_HEAD_G(tab, l);
+    if (row >= L.n) return;
+    const double ax = coarse_ax_g(L, row, g);
+    const float y = (float)((double)L.x[row] + MG_OMEGA * ((double)L.b[row] - ax) / (double)L.diag[row]);
+    if (g == 0) L.y[row] = y;
+    const MGLv& F = lv[te_.x * MG_MAX_LEVELS + l - 1];
+    for (int t = F.mstart[row] + g; t < F.mstart[row + 1]; t += MG_G) {
+        const int m = F.mem[t];
+        F.x[m] = (float)((double)F.x[m] + MG_SCALE * (double)y);
     }
 }
 // b of level l + 1: the sum of r over each aggregate's members, in member
@@ -1185,13 +1210,16 @@ int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool
         }
     }
     auto rows_table = [&](int l, bool below, int* off, int* cnt) {
+        // the coarse levels' residual and post-smoothing passes run MG_G
+        // lanes per row
+        const int per = (below && l > 0) ? BLOCK / MG_G : BLOCK;
         *off = (int)tab.size();
         for (int k = 0; k < nc; k++) {
             const MGState* mg = mgs[k];
             if (!mg) continue;
             const int lc = mg->nlev - 1;
             if (below ? !(l < lc) : !(l <= lc)) continue;
-            for (int f = 0; f < mg->L[l].n; f += BLOCK) tab.push_back(make_int2(k, f));
+            for (int f = 0; f < mg->L[l].n; f += per) tab.push_back(make_int2(k, f));
         }
         *cnt = (int)tab.size() - *off;
     };
@@ -1269,16 +1297,16 @@ int mg_apply(Eng& e, int which, int sc_done) {
     for (int l = 0; l + 1 < P.lmax; l++) {
         if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
         else MGE("mg_clong", 0, P.clong_off[l], P.clong_cnt[l], k_mg_clong, l, sc_done);
-        MGE(l == 0 ? "mg_res0" : "mg_res", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_res, l,
-            which, sc_done);
+        if (l == 0) MGE("mg_res0", P.spmv_bytes0, P.down_off[l], P.down_cnt[l], k_mg_res, l, which, sc_done);
+        else MGE("mg_res", 0, P.down_off[l], P.down_cnt[l], k_mg_res_g, l, sc_done);
         MGE("mg_restrict", 0, P.lvl_off[l + 1], P.lvl_cnt[l + 1], k_mg_restrict, l, sc_done);
     }
     MGE("mg_dense", 0, P.dense_off, P.dense_cnt, k_mg_dense, sc_done);
     for (int l = P.lmax - 2; l >= 0; l--) {
         if (l == 0) MGE("mg_long", 0, P.long_off, P.long_cnt, k_mg_long, sc_done);
         else MGE("mg_clong", 0, P.clong_off[l], P.clong_cnt[l], k_mg_clong, l, sc_done);
-        MGE(l == 0 ? "mg_post0" : "mg_post", l == 0 ? P.spmv_bytes0 : 0.0, P.down_off[l], P.down_cnt[l], k_mg_post,
-            l, which, sc_done);
+        if (l == 0) MGE("mg_post0", P.spmv_bytes0, P.down_off[l], P.down_cnt[l], k_mg_post, l, which, sc_done);
+        else MGE("mg_post", 0, P.down_off[l], P.down_cnt[l], k_mg_post_g, l, sc_done);
     }
 #undef MGE
     return 0;
