@@ -76,7 +76,7 @@ struct MGLevel {
     double* lsum = nullptr;
 };
 constexpr int MG_CLONG = 128;
-constexpr int MG_G = 8;   // lanes per row of the coarse residual and post-smoothing passes
+constexpr int MG_G = 4;   // lanes per row of the coarse residual and post-smoothing passes
 static_assert(BLOCK % MG_G == 0 && SLICE % MG_G == 0, "a row's lanes sit in one wave");
 struct MGState {
     long long id = 0;        // distinct for every hierarchy built (table signatures)
