@@ -716,19 +716,21 @@ __global__ void __launch_bounds__(BLOCK) k_kr_long(Seg sg) {
     if (lane == 0) pp.lsum[u] = s;
 }
 
-// r = rhs - A x ; rhat = r ; partials rhs.rhs (A) and r.r (B)
-template <class V, class X>
+// r = rhs - A x ; rhat = r ; partials rhs.rhs (A) and r.r (B).  ZERO: x is
+// zero (the inner solves of the mixed mode start from it), so r = rhs and
+// the product is not formed
+template <class V, class X, bool ZERO>
 __global__ void __launch_bounds__(BLOCK) k_kr_init(Seg sg) {
     __shared__ double sh[2 * BLOCK];
     __shared__ int wbase[BLOCK];
     SEG_ROW(sg);
     if (SKIPPED(cd)) return;
     const PPEView pp = cd.ppe;
-    load_group_bases(pp, first_, wbase);
+    if (!ZERO) load_group_bases(pp, first_, wbase);
     const KSet<X> K = kset(pp, (X*)nullptr);
     double a = 0.0, b = 0.0;
     if (row < pp.nr) {
-        double y = spmv_row(pp, row, vals_of(pp, (V*)nullptr), kdiag(pp, (X*)nullptr), K.x, wbase);
+        double y = ZERO ? 0.0 : spmv_row(pp, row, vals_of(pp, (V*)nullptr), kdiag(pp, (X*)nullptr), K.x, wbase);
         X rr = (X)((double)K.rhs[row] - y);
         K.r[row] = rr;
         K.rhat[row] = rr;
@@ -1528,7 +1530,7 @@ static int kr_graph(Eng& e, int gi, double spmv_bytes, bool mg, hipGraphExec_t* 
 // with the tolerances and iteration caps already in the controls.
 // iters[k] receives case k's iteration count.
 template <class V, class X>
-static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, bool mg = false) {
+static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, bool mg = false, bool x_zero = false) {
     const int nc = (int)e.cs.size();
     double spmv_bytes = 0.0, spmv_bytes0 = 0.0;
     int maxit = 0;
@@ -1551,8 +1553,12 @@ static int bicgstab(Eng& e, int gi, int pred_slot, std::vector<int>& iters, bool
         maxit = std::max(maxit, (int)e.h_ctl[k * CTL_COUNT + CTL_MAXIT]);
     }
     if (mg) e.mg_plan.spmv_bytes0 = spmv_bytes0;
-    RLONG(V, X, 0);
-    RL("kr_init", spmv_bytes, (k_kr_init<V, X>));
+    if (x_zero) {
+        RL("kr_init", 0, (k_kr_init<V, X, true>));
+    } else {
+        RLONG(V, X, 0);
+        RL("kr_init", spmv_bytes, (k_kr_init<V, X, false>));
+    }
     KF(FIN_INIT);
     RL("kr_zerox", 0, k_kr_zerox<X>);
     SPH_TRY(read_scalars(e));
@@ -1817,7 +1823,7 @@ int ppe_solve(Eng& e) {
             SPH_TRY(ctl_upload(e));
             if (!any) break;
             RL("zero_fx", 0, k_zero_fx);
-            SPH_TRY((bicgstab<float, float>(e, any_mg ? 2 : 1, outer, iters, any_mg)));
+            SPH_TRY((bicgstab<float, float>(e, any_mg ? 2 : 1, outer, iters, any_mg, true)));
             bool stalled = false;
             for (int k = 0; k < nc; k++) {
                 if (!active[k]) continue;
