@@ -88,6 +88,16 @@ struct MGState {
     // its row's diagonal as a half float in the high half, in the PPE's
     // slot layout; null: the V-cycle reads the PPE's float values
     unsigned* ev = nullptr;
+    // its stencil slices: the half values alone, over the slice's offsets
+    // whose halves are not all zero (the neighbours at the search radius
+    // carry entries of ~1e-39, zero as halves): per slice the kept width
+    // (-1: the slice keeps its words in ev), first half-word in eh and first
+    // offset in hoff
+    int* hw = nullptr;
+    long long* hbase = nullptr;
+    int* hpat = nullptr;
+    unsigned short* eh = nullptr;
+    int* hoff = nullptr;
     double* lsum = nullptr;  // level 0: tail sums of the long rows for the next pass
     long long sorted_rows = 0;   // coarse rows the sorting Galerkin kernels formed (all levels)
 };
@@ -563,26 +573,53 @@ __global__ void __launch_bounds__(BLOCK) k_mg_tofloat(long long n, const double*
     if (i < n) b[i] = (float)a[i];
 }
 // The V-cycle's compact level-0 operator: per stored entry its column word
-// | half(value / diag) << 16.  A stencil slice needs no columns: its halves
-// alone, packed into the first half of the slice's words (entry k of row r
-// at half-word 2 sl_off[r / 64] + 64 k + r % 64).
-__device__ __forceinline__ const unsigned short* ev_halves(const PPEView& pp, const unsigned* ev, int r) {
-    return (const unsigned short*)ev + 2 * pp.sl_off[r >> 6] + (r & 63);
-}
+// | half(value / diag) << 16; stencil slices are packed by k_mg_hpack.
 __global__ void __launch_bounds__(BLOCK) k_mg_pack16(PPEView pp, unsigned* ev) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= pp.nr) return;
+    if (pp.spat && pp.spat[r >> 6] >= 0) return;
     const long long base = prow(pp.sl_off, r);
     const float d = pp.fdiag[r];
     const int m = pp.cnt[r];
-    const bool st = pp.spat && pp.spat[r >> 6] >= 0;
-    unsigned short* hv = const_cast<unsigned short*>(ev_halves(pp, ev, r));
     for (int k = 0; k < m; k++) {
         const long long e = base + pslot(k);
         const unsigned short h = __builtin_bit_cast(unsigned short, (_Float16)(pp.vals32[e] / d));
-        if (st) hv[pslot(k)] = h;
-        else ev[e] = (unsigned)pp.ecols[e] | ((unsigned)h << 16);
+        ev[e] = (unsigned)pp.ecols[e] | ((unsigned)h << 16);
     }
+}
+// Stencil slices of the level-0 copy, one 64-thread block per slice: a slot
+// is kept when some row of the slice has a nonzero half there (a wave
+// ballot, so every lane agrees).  PACK = false: the kept width of each
+// slice (hw, -1 for the other slices; wh = 64 x width and wp = width for
+// the scans); PACK = true: the halves of the kept slots at hbase[s] + 64 k'
+// + lane and their offsets at hoff[hpat[s] + k'], k' in slot order.
+template <bool PACK>
+__global__ void __launch_bounds__(SLICE) k_mg_hpack(PPEView pp, int* hw, long long* wh, int* wp,
+                                                     const long long* hbase, const int* hpat, unsigned short* eh,
+                                                     int* hoff) {
+    const int s = blockIdx.x, l = threadIdx.x, r = s * SLICE + l;
+    const int st = pp.spat[s];
+    if (st < 0) {
+        if (!PACK && l == 0) { hw[s] = -1; wh[s] = 0; wp[s] = 0; }
+        return;
+    }
+    const bool live = r < pp.nr;
+    const int n = pp.cnt[s * SLICE];   // every row of a stencil slice holds its offsets
+    const long long base = pp.sl_off[s] + l;
+    const float d = live ? pp.fdiag[r] : 1.0f;
+    int kept = 0;
+    for (int k = 0; k < n; k++) {
+        const unsigned short h =
+            live ? __builtin_bit_cast(unsigned short, (_Float16)(pp.vals32[base + pslot(k)] / d)) : (unsigned short)0;
+        // +0 and -0 halves add exact zeros
+        if (__ballot((h & 0x7fffu) != 0) == 0) continue;
+        if (PACK) {
+            eh[hbase[s] + pslot(kept) + l] = h;
+            if (l == 0) hoff[hpat[s] + kept] = pp.poff[st + k];
+        }
+        kept++;
+    }
+    if (!PACK && l == 0) { hw[s] = kept; wh[s] = (long long)kept * SLICE; wp[s] = kept; }
 }
 
 // ------------------------------------------------------------------------
@@ -605,25 +642,31 @@ struct MGLv {
     float *b, *x, *y, *r;
     const float* ainv;            // coarsest level: dense inverse
     const unsigned* ev;           // level 0: compact operator, or null
+    const int* hw;                // level 0, stencil slices: kept width (or -1), first half-word, first offset
+    const long long* hbase;
+    const int* hpat;
+    const unsigned short* eh;
+    const int* hoff;
     double* lsum;                 // level 0: tail sums of the long rows; coarse: long rows' products
 };
 
 // sum_k (a_rk / d_r) x[col_k] over the compact level-0 operator, in entry
 // order, accumulated in double; four independent loads in flight per lane
-__device__ __forceinline__ double mg_row16(const PPEView& pp, const unsigned* ev, int r, const float* x,
+__device__ __forceinline__ double mg_row16(const PPEView& pp, const MGLv& L, int r, const float* x,
                                            const int* wb) {
-    const int st = stencil_of(pp, r);
-    if (st >= 0) {   // a stencil slice: halves only, columns row + offset
-        const gmem<unsigned short> hp = (gmem<unsigned short>)ev_halves(pp, ev, r);
-        const cmem<int> off = (cmem<int>)(pp.poff + st);
+    const unsigned* ev = L.ev;
+    const int w = L.hw ? __builtin_amdgcn_readfirstlane(L.hw[r >> 6]) : -1;
+    if (w >= 0) {   // a stencil slice: the kept halves only, columns row + offset
+        const int s6 = r >> 6;
+        const gmem<unsigned short> hp = (gmem<unsigned short>)(L.eh + L.hbase[s6] + (r & 63));
+        const cmem<int> off = (cmem<int>)(L.hoff + L.hpat[s6]);
         const gmem<float> xr = (gmem<float>)(x + r);
-        const int m = __builtin_amdgcn_readfirstlane(pp.cnt[r]);
         auto hterm = [&](unsigned short u, int o) {
             return (double)(float)__builtin_bit_cast(_Float16, u) * (double)xr[o];
         };
         double s = 0.0;
         int k = 0;
-        for (; k + 4 <= m; k += 4) {
+        for (; k + 4 <= w; k += 4) {
             const unsigned short u0 = __builtin_nontemporal_load(hp + pslot(k + 0));
             const unsigned short u1 = __builtin_nontemporal_load(hp + pslot(k + 1));
             const unsigned short u2 = __builtin_nontemporal_load(hp + pslot(k + 2));
@@ -633,7 +676,7 @@ __device__ __forceinline__ double mg_row16(const PPEView& pp, const unsigned* ev
             s += hterm(u2, off[k + 2]);
             s += hterm(u3, off[k + 3]);
         }
-        for (; k < m; k++) s += hterm(__builtin_nontemporal_load(hp + pslot(k)), off[k]);
+        for (; k < w; k++) s += hterm(__builtin_nontemporal_load(hp + pslot(k)), off[k]);
         return s;
     }
     const unsigned* p = ev + prow(pp.sl_off, r);
@@ -688,7 +731,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_res(const CaseDesc* cds, const MGL
         const float* b = which ? pp.fs : pp.fp;
         if (L.ev) {
             const double d = (double)pp.fdiag[row];
-            double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
+            double s = mg_row16(pp, L, row, L.x, wbase + (threadIdx.x & ~63));
             if (pp.ltail && pp.ltail[row] >= 0) s += L.lsum[pp.ltail[row]] / d;
             L.r[row] = (float)((double)b[row] - d * ((double)L.x[row] - s));
         } else {
@@ -709,7 +752,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(const CaseDesc* cds, const MG
         float* out = which ? pp.fshat : pp.fphat;
         const double d = (double)pp.fdiag[row];
         if (L.ev) {
-            double s = mg_row16(pp, L.ev, row, L.x, wbase + (threadIdx.x & ~63));
+            double s = mg_row16(pp, L, row, L.x, wbase + (threadIdx.x & ~63));
             if (pp.ltail && pp.ltail[row] >= 0) s += L.lsum[pp.ltail[row]] / d;
             out[row] = (float)((double)L.x[row] + MG_OMEGA * ((double)b[row] / d - (double)L.x[row] + s));
         } else {
@@ -870,6 +913,8 @@ void mg_free(sph_ctx* c, MGState* mg) {
     }
     dfree(c, mg->ainv);
     dfree(c, mg->ev);
+    void* hp[] = {mg->hw, mg->hbase, mg->hpat, mg->eh, mg->hoff};
+    for (void* q : hp) dfree(c, q);
     dfree(c, mg->lsum);
     delete mg;
 }
@@ -1140,6 +1185,31 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
         MGTRY(dalloc(c, &mg->ev, (size_t)std::max(1LL, slots)));
         hipLaunchKernelGGL(k_mg_pack16, dim3(grid1(pp.nr)), dim3(BLOCK), 0, c->stream, pp, mg->ev);
         MGTRY(launch_check(c, "mg_pack16"));
+        if (pp.spat) {   // stencil slices: their kept halves
+            const int ns = (pp.nr + SLICE - 1) / SLICE;
+            long long* wh = nullptr;
+            int* wp = nullptr;
+            long long nh = 0;
+            int no = 0;
+            MGTRY(dalloc(c, &mg->hw, ns));
+            MGTRY(dalloc(c, &mg->hbase, ns + 1));
+            MGTRY(dalloc(c, &mg->hpat, ns + 1));
+            MGTRY(dalloc(c, &wh, ns));
+            MGTRY(dalloc(c, &wp, ns));
+            hipLaunchKernelGGL(k_mg_hpack<false>, dim3(ns), dim3(SLICE), 0, c->stream, pp, mg->hw, wh, wp,
+                               (const long long*)nullptr, (const int*)nullptr, (unsigned short*)nullptr, (int*)nullptr);
+            MGTRY(launch_check(c, "mg_hwidth"));
+            MGTRY(exclusive_scan_i64(c, wh, mg->hbase, ns, &nh));
+            MGTRY(exclusive_scan_i32(c, wp, mg->hpat, ns, &no));
+            dfree(c, wh);
+            dfree(c, wp);
+            MGTRY(dalloc(c, &mg->eh, (size_t)std::max(1LL, nh)));
+            MGTRY(dalloc(c, &mg->hoff, (size_t)std::max(1, no)));
+            hipLaunchKernelGGL(k_mg_hpack<true>, dim3(ns), dim3(SLICE), 0, c->stream, pp, (int*)nullptr,
+                               (long long*)nullptr, (int*)nullptr, (const long long*)mg->hbase, (const int*)mg->hpat,
+                               mg->eh, mg->hoff);
+            MGTRY(launch_check(c, "mg_hpack"));
+        }
     }
     for (int l = 0; l < nlev; l++) MGTRY(mg_vectors(c, mg->L[l], l == 0, l == nlev - 1, pp.mgx0));
     for (int l = 1; l + 1 < nlev; l++) {
@@ -1203,6 +1273,7 @@ int mg_prepare(Eng& e, const std::vector<const MGState*>& mgs, int sc_done, bool
             D.b = S.b; D.x = S.x; D.y = S.y; D.r = S.r;
             D.ainv = l == mg->nlev - 1 ? mg->ainv : nullptr;
             D.ev = l == 0 ? mg->ev : nullptr;
+            if (l == 0) { D.hw = mg->hw; D.hbase = mg->hbase; D.hpat = mg->hpat; D.eh = mg->eh; D.hoff = mg->hoff; }
             D.lsum = l == 0 ? mg->lsum : S.lsum;
             D.nlong = l == 0 ? 0 : S.nlong;
             D.lrows = S.lrows;
