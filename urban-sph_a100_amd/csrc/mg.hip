@@ -91,8 +91,8 @@ struct MGState {
     // its stencil slices: the half values alone, over the slice's offsets
     // whose halves are not all zero (the neighbours at the search radius
     // carry entries of ~1e-39, zero as halves): per slice the kept width
-    // (-1: the slice keeps its words in ev), first half-word in eh and first
-    // offset in hoff
+    // (-1: the slice keeps its words in ev), first slot (64 half-words) in eh
+    // and first offset in hoff
     int* hw = nullptr;
     long long* hbase = nullptr;
     int* hpat = nullptr;
@@ -590,9 +590,9 @@ __global__ void __launch_bounds__(BLOCK) k_mg_pack16(PPEView pp, unsigned* ev) {
 // Stencil slices of the level-0 copy, one 64-thread block per slice: a slot
 // is kept when some row of the slice has a nonzero half there (a wave
 // ballot, so every lane agrees).  PACK = false: the kept width of each
-// slice (hw, -1 for the other slices; wh = 64 x width and wp = width for
-// the scans); PACK = true: the halves of the kept slots at hbase[s] + 64 k'
-// + lane and their offsets at hoff[hpat[s] + k'], k' in slot order.
+// slice (hw, -1 for the other slices; wh and wp = the width for the scans);
+// PACK = true: the halves of the kept slots at 64 (hbase[s] + k') + lane and
+// their offsets at hoff[hpat[s] + k'], k' in slot order.
 template <bool PACK>
 __global__ void __launch_bounds__(SLICE) k_mg_hpack(PPEView pp, int* hw, long long* wh, int* wp,
                                                      const long long* hbase, const int* hpat, unsigned short* eh,
@@ -614,12 +614,12 @@ __global__ void __launch_bounds__(SLICE) k_mg_hpack(PPEView pp, int* hw, long lo
         // +0 and -0 halves add exact zeros
         if (__ballot((h & 0x7fffu) != 0) == 0) continue;
         if (PACK) {
-            eh[hbase[s] + pslot(kept) + l] = h;
+            eh[pslot(hbase[s] + kept) + l] = h;
             if (l == 0) hoff[hpat[s] + kept] = pp.poff[st + k];
         }
         kept++;
     }
-    if (!PACK && l == 0) { hw[s] = kept; wh[s] = (long long)kept * SLICE; wp[s] = kept; }
+    if (!PACK && l == 0) { hw[s] = kept; wh[s] = kept; wp[s] = kept; }
 }
 
 // ------------------------------------------------------------------------
@@ -657,9 +657,12 @@ __device__ __forceinline__ double mg_row16(const PPEView& pp, const MGLv& L, int
     const unsigned* ev = L.ev;
     const int w = L.hw ? __builtin_amdgcn_readfirstlane(L.hw[r >> 6]) : -1;
     if (w >= 0) {   // a stencil slice: the kept halves only, columns row + offset
+        // the slice's first slot and offset, wave-uniform (scalar offset loads)
         const int s6 = r >> 6;
-        const gmem<unsigned short> hp = (gmem<unsigned short>)(L.eh + L.hbase[s6] + (r & 63));
-        const cmem<int> off = (cmem<int>)(L.hoff + L.hpat[s6]);
+        const int hb = __builtin_amdgcn_readfirstlane((int)L.hbase[s6]);
+        const int hq = __builtin_amdgcn_readfirstlane(L.hpat[s6]);
+        const gmem<unsigned short> hp = (gmem<unsigned short>)(L.eh + pslot(hb) + (r & 63));
+        const cmem<int> off = (cmem<int>)(L.hoff + hq);
         const gmem<float> xr = (gmem<float>)(x + r);
         auto hterm = [&](unsigned short u, int o) {
             return (double)(float)__builtin_bit_cast(_Float16, u) * (double)xr[o];
@@ -1203,7 +1206,8 @@ int mg_build(sph_ctx* c, const PPEView& pp, long long slots, double q, MGState**
             MGTRY(exclusive_scan_i32(c, wp, mg->hpat, ns, &no));
             dfree(c, wh);
             dfree(c, wp);
-            MGTRY(dalloc(c, &mg->eh, (size_t)std::max(1LL, nh)));
+            if (nh > INT_MAX) { *why = SPH_MG_TERMS; return fail_out(0); }   // slot numbers are read as int
+            MGTRY(dalloc(c, &mg->eh, (size_t)std::max(1LL, nh * SLICE)));
             MGTRY(dalloc(c, &mg->hoff, (size_t)std::max(1, no)));
             hipLaunchKernelGGL(k_mg_hpack<true>, dim3(ns), dim3(SLICE), 0, c->stream, pp, (int*)nullptr,
                                (long long*)nullptr, (int*)nullptr, (const long long*)mg->hbase, (const int*)mg->hpat,
