@@ -370,15 +370,19 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_boundary3d(Seg sg) {
     }
 }
 
+// The boundary passes of MIXED run a 64-lane wave per boundary particle,
+// four waves (four particles) to a workgroup, a quarter of the workgroups
+// of one wave each.  The waves share nothing.
+constexpr int WAVE_BLOCK = 256;
 // The same in MIXED, a 64-lane wave per boundary particle (walls are a few
 // per cent of the case): lane sums over entries lane, lane + 64, ..., then
 // a fixed butterfly; only the particles the serial kernel writes (p_type 0
 // and 9) are visited.
-__global__ void __launch_bounds__(64) k_boundary3d_wave(Seg sg) {
+__global__ void __launch_bounds__(WAVE_BLOCK) k_boundary3d_wave(Seg sg) {
     int first_;
     const int cid_ = seg_case(sg, first_);
     const CaseDesc& cd = sg.cd[cid_];
-    const int u = first_ >> 6, lane = threadIdx.x;
+    const int u = (first_ + (int)threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (u >= cd.nm_count) return;
     const sph_params pr = cd.pr;
     if (pr.noslip_bc != 1) return;   // ISPH_Calc.cuh:83
@@ -1099,11 +1103,11 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_freeslip(Seg sg, int which) {
 // the device nearly idle): lane l adds entries l, l + 64, ... of the list,
 // then a fixed butterfly adds the lane sums, so a case gives the same bits
 // alone and in a batch.
-__global__ void __launch_bounds__(64) k_freeslip_wave(Seg sg, int which) {
+__global__ void __launch_bounds__(WAVE_BLOCK) k_freeslip_wave(Seg sg, int which) {
     int first_;
     const int cid_ = seg_case(sg, first_);
     const CaseDesc& cd = sg.cd[cid_];
-    const int u = first_ >> 6, lane = threadIdx.x;
+    const int u = (first_ + (int)threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (u >= cd.nm_count) return;
     const PView A = which ? cd.P : cd.S;
     const int i = cd.nm_list[u];
@@ -1324,11 +1328,11 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_ibm_force(Seg sg) {
 // The marker force in MIXED, a 64-lane wave per marker (markers are in the
 // boundary list): lane sums over entries lane, lane + 64, ..., then a fixed
 // butterfly; lane 0 forms the force from the sums as the serial kernel does.
-__global__ void __launch_bounds__(64) k_ibm_force_wave(Seg sg) {
+__global__ void __launch_bounds__(WAVE_BLOCK) k_ibm_force_wave(Seg sg) {
     int first_;
     const int cid_ = seg_case(sg, first_);
     const CaseDesc& cd = sg.cd[cid_];
-    const int u = first_ >> 6, lane = threadIdx.x;
+    const int u = (first_ + (int)threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (u >= cd.nm_count) return;
     const PView S = cd.S;
     const int i = cd.nm_list[u];
@@ -1527,10 +1531,10 @@ __global__ void k_stats_final(const unsigned* in, float* out) {
 #define WAVEL(name, kern, ...)                                                                        \
     do {                                                                                            \
         Shape sh_;                                                                                  \
-        SPH_TRY(shape(e, 3, 64, &sh_));                                                             \
+        SPH_TRY(shape(e, 3, WAVE_BLOCK, &sh_));                                                     \
         if (sh_.grid > 0) {                                                                         \
             SPH_TRY(prof_begin(e, name, 0.0));                                                      \
-            hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(64), 0, e.st, Seg{e.d_desc, sh_.tab}, ##__VA_ARGS__); \
+            hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(WAVE_BLOCK), 0, e.st, Seg{e.d_desc, sh_.tab}, ##__VA_ARGS__); \
             SPH_TRY(launch_check(&e, name));                                                        \
             SPH_TRY(prof_end(e));                                                                   \
         }                                                                                           \
