@@ -3,9 +3,11 @@
 With fixed particles on a lattice and the rows in lattice order, a 64-row
 slice whose rows' column offsets (col - row) have a small union stores every
 row over that union, an offset a row lacks holding the value 0, and its
-products read no columns (ppe_stencil = 1, the default).  The nonzero
-entries keep their ascending column order and the stored zeros add exact
-zeros, so every field and the PPE report come out bitwise the same as with
+products read no columns (ppe_stencil = 1, the default); after the first
+assembly of a search the slices drop the offsets whose values are zero in
+all their rows.  The nonzero entries keep their ascending column order and
+the stored zeros add exact zeros, so every field and the PPE report come out
+bitwise the same as with
 ppe_stencil = 0: on the canyon with the multigrid and with Jacobi over
 several steps (graph replay), on an APR-refined canyon, and in a batch.
 """
@@ -50,8 +52,11 @@ def test_stencil_slices_equal_stored_columns(precond):
     assert st["ordering"] == abi.ORDER_LATTICE
     assert ex["slices"] == 0 and ex["nnz_st"] == 0
     assert st["slices"] > nslices // 2 and 0 < st["nnz_st"] <= st["nnz"]
-    assert st["nnz"] == ex["nnz"]                     # the real entries
-    assert ex["slots"] <= st["slots"] <= 5 * ex["slots"] // 4 + 64   # the stored zeros
+    # after the first assembly the stencil slices drop the offsets whose
+    # values are zero (the search-radius neighbours); the rest of the union's
+    # slots are stored zeros
+    assert st["nnz"] < ex["nnz"]
+    assert st["slots"] <= 5 * ex["slots"] // 4 + 64
     _same(got, ref, f"stencil slices, precond {precond}")
 
 

@@ -70,7 +70,8 @@ struct PPEState {
     int* spat = nullptr;
     int* poff = nullptr;
     int nstencil = 0;           // stencil slices
-    long long nnz_stencil = 0;  // real entries in them
+    long long nnz_stencil = 0;  // real entries in them (after the squeeze: the nonzero ones)
+    bool squeezed = false;      // the stencil slices dropped their zero offsets: values final
     double* vals64 = nullptr;
     float* vals32 = nullptr;
     double* diag = nullptr;
@@ -278,6 +279,10 @@ __global__ void __launch_bounds__(BLOCK) k_long_index(int nr, int* cnt, const in
     } else {
         ltail[r] = -1;
     }
+}
+__global__ void __launch_bounds__(BLOCK) k_scale_i64(long long n, long long* a, long long f) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] *= f;
 }
 __global__ void __launch_bounds__(BLOCK) k_widen(int n, const int* a, long long* b) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -518,6 +523,69 @@ __global__ void __launch_bounds__(BLOCK) k_stencil_real(int nr, const int* cnt, 
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < nr) out[r] = wp[r >> 6] > 0 ? cnt[r] : 0;
 }
+// After the first assembly of a search (with fixed particles the values are
+// final until the next search): a stencil slice drops the offsets whose
+// values are zero in every one of its rows (the search-radius neighbours,
+// see stored_value), one 64-thread block per slice; a wave ballot decides,
+// so every lane agrees.  MOVE = false: the new width of each slice (w2; the
+// old one for slices that store columns) and the stored nonzero values of
+// the stencil slices (nzs); MOVE = true: values, columns and offsets into
+// the new layout, and spat pointed at the slice's offsets there.
+template <bool MOVE>
+__global__ void __launch_bounds__(SLICE) k_stencil_squeeze(int nr, const long long* sl_off, const int* cnt,
+                                                           int* spat, const int* poff, const float* vals,
+                                                           const unsigned short* ecols, const int* cols,
+                                                           long long* w2, long long* nzs, const long long* sl_off2,
+                                                           float* vals2, unsigned short* ecols2, int* cols2,
+                                                           int* poff2, int* cnt2) {
+    const int s = blockIdx.x, l = threadIdx.x, r = s * SLICE + l;
+    const int st = spat[s];
+    const bool live = r < nr;
+    const long long b = sl_off[s] + l;
+    if (st < 0) {   // columns stored: copied as they are
+        if (!MOVE) {
+            int m = live ? cnt[r] : 0;
+            for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+            if (l == 0) { w2[s] = m; nzs[s] = 0; }
+            return;
+        }
+        if (!live) return;
+        const int m = cnt[r];
+        const long long b2 = sl_off2[s] + l;
+        for (int k = 0; k < m; k++) {
+            vals2[b2 + pslot(k)] = vals[b + pslot(k)];
+            if (ecols) ecols2[b2 + pslot(k)] = ecols[b + pslot(k)];
+            if (cols) cols2[b2 + pslot(k)] = cols[b + pslot(k)];
+        }
+        cnt2[r] = m;
+        return;
+    }
+    const int n = cnt[s * SLICE];   // every row of a stencil slice holds its offsets
+    int kept = 0, nz = 0;
+    for (int k = 0; k < n; k++) {
+        const float v = live ? vals[b + pslot(k)] : 0.0f;
+        const bool keep = __ballot(v != 0.0f) != 0;
+        if (MOVE && keep) {
+            const long long e2 = sl_off2[s] + pslot(kept) + l;
+            if (live) {
+                vals2[e2] = v;
+                if (ecols) ecols2[e2] = ecols[b + pslot(k)];
+                if (cols) cols2[e2] = cols[b + pslot(k)];
+            }
+            if (l == 0) poff2[sl_off2[s] / SLICE + kept] = poff[st + k];
+        }
+        nz += v != 0.0f ? 1 : 0;
+        kept += keep ? 1 : 0;
+    }
+    if (!MOVE) {
+        for (int o = 32; o > 0; o >>= 1) nz += __shfl_xor(nz, o, 64);
+        if (l == 0) { w2[s] = kept; nzs[s] = nz; }
+        return;
+    }
+    // (one wave: every lane read spat[s] above, before this store)
+    if (l == 0) spat[s] = (int)(sl_off2[s] / SLICE);
+    if (live) cnt2[r] = kept;
+}
 
 // This thread's PPE row: case descriptor `cd`, row index `row` in the case
 // and the case-local 256-row chunk `chunk` of the block.
@@ -549,6 +617,21 @@ __device__ __forceinline__ KSet<float> kset(const PPEView& v, float*) {
 
 // ------------------------------------------------------------------------
 // per-step assembly, functions_PPE.cuh:2-160
+// The stored PPE coefficient.  Float values (the mixed mode) of magnitude
+// below 1e-30 are stored as 0: the neighbours at exactly the search radius,
+// where the kernel gradient vanishes, give coefficients of about 1e-39,
+// which no float or double sum of this operator can feel (its entries are
+// O(1 / dx^2)), and as zeros they can leave the stencil slices.  FP64 stores
+// every value as computed.
+template <class V, class R>
+__device__ __forceinline__ V stored_value(R cc) {
+    if constexpr (sizeof(V) == 4) {
+        const float v = (float)cc;
+        return fabsf(v) < 1e-30f ? 0.0f : v;
+    } else {
+        return (V)cc;
+    }
+}
 template <class R, class V, class Src>
 __device__ __forceinline__ void assemble_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src,
                                               int jacobi) {
@@ -627,8 +710,9 @@ __device__ __forceinline__ void assemble_body(const CaseDesc& cd, int i, const N
             if (!src.self(j, i)) {
                 if (jrow) {
                     if (out) {
-                        if (slot < LONG_ROW || !pp.ltail) out[pslot(smap ? smap[pslot(slot)] : slot)] = (V)cc;
-                        else pp.tail_vals[pp.tail_off[pp.ltail[r]] + (slot - LONG_ROW)] = (float)cc;
+                        const V v = stored_value<V>(cc);
+                        if (slot < LONG_ROW || !pp.ltail) out[pslot(smap ? smap[pslot(slot)] : slot)] = v;
+                        else pp.tail_vals[pp.tail_off[pp.ltail[r]] + (slot - LONG_ROW)] = (float)v;
                     }
                     slot++;
                 } else {
@@ -1239,6 +1323,70 @@ static int stencil_slices(sph_ctx* c, PPEState* P) {
     return 0;
 }
 
+// After the first assembly of a search: the stencil slices drop the offsets
+// whose values are zero in all their rows (k_stencil_squeeze).  With fixed
+// particles the values are final until the next search, which builds a new
+// structure, so the slot map (needed only to write them) goes.
+static int stencil_squeeze(sph_ctx* c, PPEState* P) {
+    const int nr = P->nr, ns = P->nslices;
+    long long *w2 = nullptr, *nzs = nullptr, *tmp = nullptr, *sl_off2 = nullptr, slots2 = 0, nz = 0;
+    SPH_TRY(dalloc(c, &w2, ns + 1));
+    SPH_TRY(dalloc(c, &nzs, ns));
+    SPH_TRY(dalloc(c, &tmp, ns));
+    SPH_TRY(dalloc(c, &sl_off2, ns + 1));
+    hipLaunchKernelGGL(k_stencil_squeeze<false>, dim3(ns), dim3(SLICE), 0, c->stream, nr, (const long long*)P->sl_off,
+                       (const int*)P->cnt, P->spat, (const int*)P->poff, (const float*)P->vals32,
+                       (const unsigned short*)P->ecols, (const int*)P->cols, w2, nzs, (const long long*)nullptr,
+                       (float*)nullptr, (unsigned short*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr);
+    SPH_TRY(launch_check(c, "stencil_widths"));
+    SPH_TRY(exclusive_scan_i64(c, nzs, tmp, ns, &nz));
+    SPH_TRY(exclusive_scan_i64(c, w2, sl_off2, ns, &slots2));
+    slots2 *= SLICE;
+    hipLaunchKernelGGL(k_scale_i64, dim3(grid1(ns)), dim3(BLOCK), 0, c->stream, (long long)ns, sl_off2, (long long)SLICE);
+    SPH_TRY(launch_check(c, "stencil_offsets"));
+    dfree(c, w2);
+    dfree(c, nzs);
+    dfree(c, tmp);
+    float* vals2 = nullptr;
+    unsigned short* ecols2 = nullptr;
+    int *cols2 = nullptr, *poff2 = nullptr, *cnt2 = nullptr;
+    SPH_TRY(dalloc(c, &vals2, slots2 + 1));
+    hipLaunchKernelGGL(k_zero<float>, dim3(grid1(slots2 + 1)), dim3(BLOCK), 0, c->stream, slots2 + 1, vals2);
+    if (P->ecols) SPH_TRY(dalloc(c, &ecols2, slots2 + 1));
+    if (P->cols) SPH_TRY(dalloc(c, &cols2, slots2 + 1));
+    SPH_TRY(dalloc(c, &poff2, slots2 / SLICE + 1));
+    SPH_TRY(dalloc(c, &cnt2, nr));
+    hipLaunchKernelGGL(k_stencil_squeeze<true>, dim3(ns), dim3(SLICE), 0, c->stream, nr, (const long long*)P->sl_off,
+                       (const int*)P->cnt, P->spat, (const int*)P->poff, (const float*)P->vals32,
+                       (const unsigned short*)P->ecols, (const int*)P->cols, (long long*)nullptr, (long long*)nullptr,
+                       (const long long*)sl_off2, vals2, ecols2, cols2, poff2, cnt2);
+    SPH_TRY(launch_check(c, "stencil_squeeze"));
+    SPH_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(c, P->vals32);
+    dfree(c, P->ecols);
+    dfree(c, P->cols);
+    dfree(c, P->poff);
+    dfree(c, P->cnt);
+    dfree(c, P->sl_off);
+    dfree(c, P->slotmap);
+    P->vals32 = vals2;
+    P->ecols = ecols2;
+    P->cols = cols2;
+    P->poff = poff2;
+    P->cnt = cnt2;
+    P->sl_off = sl_off2;
+    P->slotmap = nullptr;
+    P->nnz_slots = slots2;
+    P->nnz = P->nnz - P->nnz_stencil + nz;
+    P->nnz_stencil = nz;
+    P->squeezed = true;
+    // the old layout's blocks would stay in the context's cache, counted in
+    // its device memory (the admission estimate bounds that); nothing in
+    // flight uses them after the synchronisation above
+    dcache_release(c);
+    return 0;
+}
+
 // Rows and column structure of a case's PPE from its neighbour list.
 static int ppe_build_ordered(sph_ctx* c, bool allow_lattice);
 int ppe_build(sph_ctx* c) { return ppe_build_ordered(c, true); }
@@ -1616,6 +1764,10 @@ int ppe_solve(Eng& e) {
     double asm_bytes = 0.0;
     long long rows_total = 0;
     for (sph_ctx* c : e.cs) {
+        // a squeezed operator (stencil slices without their zero offsets)
+        // keeps its values until the next search builds a new structure
+        if (c->ppe && c->ppe->squeezed && (c->prm.ale_positions || c->vals_gen != c->generation))
+            return fail(&e, SPH_E_STATE, "PPE values of a squeezed operator would be rewritten");
         c->rep = sph_ppe_report{};
         c->rep.unknowns = nr_of(c);
         c->rep.nnz = c->ppe ? c->ppe->nnz : 0;
@@ -1664,6 +1816,28 @@ int ppe_solve(Eng& e) {
         SPH_TRY(launch_check(&e, "ppe_assemble"));
         SPH_TRY(tune_end(e));
         SPH_TRY(prof_end(e));
+    }
+    // the values of a search's first step are written: its stencil slices
+    // drop their zero offsets, and the descriptors take the new layout
+    if (mixed && !jacobi) {
+        bool any = false;
+        for (int k = 0; k < nc; k++) {
+            sph_ctx* c = e.cs[k];
+            PPEState* P = c->ppe;
+            if (!P || !P->spat || P->squeezed || (!c->prm.ale_positions && c->vals_gen == c->generation)) continue;
+            if (!any) SPH_HIP(&e, hipStreamSynchronize(e.st));   // the assembly's values
+            any = true;
+            SPH_TRY(stencil_squeeze(c, P));
+            c->rep.nnz = P->nnz;
+            c->rep.slots = P->nnz_slots;
+            c->rep.stencil_nnz = P->nnz_stencil;
+        }
+        if (any) {
+            SPH_HIP(&e, hipEventSynchronize(e.up_ev));
+            for (int k = 0; k < nc; k++) e.h_desc[k].ppe = ppe_view(e.cs[k]);
+            SPH_HIP(&e, hipMemcpyAsync(e.d_desc, e.h_desc, sizeof(CaseDesc) * nc, hipMemcpyHostToDevice, e.st));
+            SPH_HIP(&e, hipEventRecord(e.up_ev, e.st));
+        }
     }
     if (jacobi) {
         RL("ppe_scatter", 0, k_ppe_scatter, 1);
