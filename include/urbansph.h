@@ -243,8 +243,12 @@ typedef struct sph_params {
                                    union of at most 5/4 of its widest row
                                    over that union, without columns (an
                                    offset a row lacks holds the value 0); the
-                                   products read values only.  The same
-                                   results bitwise; 0 stores every column    */
+                                   products read values only.  After the
+                                   first assembly of a search the slice drops
+                                   the offsets whose values are zero in all
+                                   its rows (MIXED stores coefficients below
+                                   1e-30 as 0).  The same results bitwise; 0
+                                   stores every column                       */
 } sph_params;
 
 /* Sources of the IB marker normals (ib_normals). */
