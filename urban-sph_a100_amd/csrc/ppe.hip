@@ -292,50 +292,65 @@ __global__ void __launch_bounds__(BLOCK) k_rowof_from_rows(int nr, const int* ro
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < nr) rowof[rows[r]] = r;
 }
-// Columns of each row in ascending (lattice) order, sorted in LDS, and the
-// slot each entry takes, indexed by its rank in the neighbour list (the
-// order in which the assembly visits it).  Rows longer than SORT_CAP set
-// *overflow (the case then keeps cell order).
+// Columns of each row in ascending (lattice) order, and the slot each entry
+// takes, indexed by its rank in the neighbour list (the order in which the
+// assembly visits it).  One wave per row: the lanes take the list 64 entries
+// at a time and place the selected ones at their ranks (a ballot prefix),
+// then a bitonic sort of (column, rank) keys in LDS orders them.  Rows
+// longer than SORT_CAP set *overflow (the case then keeps cell order).
 constexpr int SORT_CAP = 160;
+constexpr int SORT_KEYS = 256;   // the bitonic sort's width (>= SORT_CAP)
+static_assert(SORT_KEYS >= SORT_CAP && (SORT_KEYS & (SORT_KEYS - 1)) == 0, "bitonic width");
 // lattice-ordered rows fit the sort, so they never have tails
 static_assert(LONG_ROW >= SORT_CAP, "a lattice-ordered row must fit the SELL part");
-__global__ void __launch_bounds__(SLICE) k_ppe_struct_sorted(int nr, const int* rows, const int* rowof, PView S,
+__global__ void __launch_bounds__(BLOCK) k_ppe_struct_sorted(int nr, const int* rows, const int* rowof, PView S,
                                                              NList nl, const int* cnt, const long long* sl_off,
                                                              int* cols, unsigned char* slotmap, int* overflow) {
-    __shared__ int sc[SLICE][SORT_CAP + 1];
-    __shared__ unsigned char so[SLICE][SORT_CAP];
-    const int t = threadIdx.x;
-    const int r = blockIdx.x * SLICE + t;
-    if (r >= nr) return;
+    __shared__ unsigned long long keys[BLOCK / SLICE][SORT_KEYS];
+    const int w = threadIdx.x / SLICE, lane = threadIdx.x % SLICE;
+    const int r = blockIdx.x * (BLOCK / SLICE) + w;
+    if (r >= nr) return;   // the whole wave
     const int m = cnt[r];
-    if (m > SORT_CAP) { *overflow = 1; return; }
+    if (m > SORT_CAP) {
+        if (lane == 0) *overflow = 1;
+        return;
+    }
+    unsigned long long* key = keys[w];
     const int i = rows[r];
     const unsigned* row = nl.ent + nl.slice_off[i >> 6] + (i & 63);
     const int ml = nl.cnt[i];
-    int e = 0;
-    for (int k = 0; k < ml; k++) {
-        unsigned en = row[(size_t)k * SLICE];
-        if (!(en & F_DP)) continue;
-        int j = en & NB_IDX;
-        if (S.p_type[j] >= 1000 || S.i_type[j] == 4) continue;
-        if (j == i) continue;
-        int rj = rowof[j];
-        if (rj < 0) continue;
-        int pos = e;
-        while (pos > 0 && sc[t][pos - 1] > rj) {
-            sc[t][pos] = sc[t][pos - 1];
-            so[t][pos] = so[t][pos - 1];
-            pos--;
+    int e = 0;   // selected entries so far, in list order
+    for (int k0 = 0; k0 < ml; k0 += SLICE) {
+        const int k = k0 + lane;
+        int rj = -1;
+        if (k < ml) {
+            const unsigned en = row[(size_t)k * SLICE];
+            const int j = en & NB_IDX;
+            if ((en & F_DP) && S.p_type[j] < 1000 && S.i_type[j] != 4 && j != i) rj = rowof[j];
         }
-        sc[t][pos] = rj;
-        so[t][pos] = (unsigned char)e;
-        e++;
+        const unsigned long long sel = __ballot(rj >= 0);
+        const int pos = e + __popcll(sel & ((1ull << lane) - 1ull));
+        if (rj >= 0 && pos < SORT_KEYS) key[pos] = ((unsigned long long)(unsigned)rj << 32) | (unsigned)pos;
+        e += __popcll(sel);
     }
-    int* out = cols + prow(sl_off, r);
-    unsigned char* sm = slotmap + prow(sl_off, r);
-    for (int p2 = 0; p2 < e; p2++) {
-        out[pslot(p2)] = sc[t][p2];
-        sm[pslot(so[t][p2])] = (unsigned char)p2;
+    for (int p = e + lane; p < SORT_KEYS; p += SLICE) key[p] = ~0ull;
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 2; k <= SORT_KEYS; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = lane; t < SORT_KEYS; t += SLICE) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const unsigned long long x = key[t], y = key[u];
+                    if ((x > y) == ((t & k) == 0)) { key[t] = y; key[u] = x; }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    const long long base = prow(sl_off, r);
+    for (int p = lane; p < m; p += SLICE) {
+        const unsigned long long x = key[p];
+        cols[base + pslot(p)] = (int)(x >> 32);
+        slotmap[base + pslot((int)(x & 0xffffffffu))] = (unsigned char)p;
     }
 }
 
@@ -1434,7 +1449,8 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
         SPH_TRY(dalloc(c, &P->slotmap, P->nnz_slots + 1));
         SPH_TRY(dalloc(c, &ovf, 1));
         hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(BLOCK), 0, c->stream, 1LL, ovf);
-        hipLaunchKernelGGL(k_ppe_struct_sorted, dim3(P->nslices), dim3(SLICE), 0, c->stream, nr, (const int*)P->rows,
+        hipLaunchKernelGGL(k_ppe_struct_sorted, dim3((nr + BLOCK / SLICE - 1) / (BLOCK / SLICE)), dim3(BLOCK), 0,
+                           c->stream, nr, (const int*)P->rows,
                            (const int*)P->rowof, S, c->nlist(), (const int*)P->cnt, (const long long*)P->sl_off,
                            P->cols, P->slotmap, ovf);
         SPH_TRY(launch_check(c, "ppe_cols_sorted"));
