@@ -291,7 +291,7 @@ def run_apr(cases, abi, urbansph, steps, warmup, dt, tol, r1=0.05, r2=0.0125):
     out = dict(value=live * steps / wall, ms_per_step=wall / steps * 1e3, particles_live=live, particles_total=n,
                coarse=len(P), refine_s=t_apr, iters=iters, rel_res=rep.rel_residual, converged=rep.converged,
                ppe_rows=rep.unknowns, ppe_nnz=rep.nnz, ppe_slots=rep.slots, ppe_ordering=rep.ordering,
-               ppe_index16=rep.index16, ppe_tail_rows=rep.tail_rows,
+               ppe_index16=rep.index16, ppe_tail_rows=rep.tail_rows, heavy_lists=rep.heavy_lists,
                ppe_precond="multigrid" if rep.precond == abi.PRECOND_MULTIGRID else "jacobi", mg_status=rep.mg_status,
                kernel_ms=dict(sorted(((k, round(v[1], 2)) for k, v in prof.items()), key=lambda kv: -kv[1])[:12]))
     log(f"[bench] apr: {out['value'] / 1e6:.2f} M particle-steps/s, {out['ms_per_step']:.1f} ms/step, iters {iters}")
@@ -710,7 +710,8 @@ def main():
             "converged": ar["converged"], "kernel_ms_one_step": ar["kernel_ms"], "ppe_precond": ar["ppe_precond"],
             "mg_status": ar["mg_status"],
             "ppe": {"rows": ar["ppe_rows"], "nnz": ar["ppe_nnz"], "sell_slots": ar["ppe_slots"],
-                    "ordering": ["cell", "lattice"][ar["ppe_ordering"]], "index16": ar["ppe_index16"], "tail_rows": ar["ppe_tail_rows"]}}
+                    "ordering": ["cell", "lattice"][ar["ppe_ordering"]], "index16": ar["ppe_index16"], "tail_rows": ar["ppe_tail_rows"]},
+            "heavy_lists": ar["heavy_lists"]}
     if not args.no_ale:
         al = run_ale(cases, abi, urbansph, P, nrm, geom, 3, 2, args.dt, args.tol)
         line["ale"] = {

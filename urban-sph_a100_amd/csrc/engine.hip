@@ -239,15 +239,19 @@ static int shape_device(Eng& e, int block, long long total, const U& units, Eng:
     return 0;
 }
 
+int heavy_waves(const sph_ctx* c) { return c->prm.precision == SPH_MIXED && c->porder ? c->n_heavy : 0; }
+
 int shape(Eng& e, int kind, int block, Shape* out) {
     const int nc = (int)e.cs.size();
     auto units = [&](int k) -> long long {
         const sph_ctx* c = e.cs[k];
         // kind 0: particles; 1: PPE rows; 2: one 64-lane wave per long PPE row;
         // 3: one 64-lane wave per boundary particle of the Neumann pass;
-        // 4: the lanes of the cell-tile chunks (a block per chunk)
+        // 4: the lanes of the cell-tile chunks (a block per chunk); 5: one
+        // 64-lane wave per particle of a heavy neighbour list
         return kind == 0 ? c->n : kind == 1 ? ppe_view(c).nr : kind == 2 ? 64LL * ppe_view(c).nlong
-             : kind == 3 ? 64LL * c->nm_count : (long long)c->tl_nchunk * c->tl_lanes;
+             : kind == 3 ? 64LL * c->nm_count : kind == 5 ? 64LL * heavy_waves(c)
+             : (long long)c->tl_nchunk * c->tl_lanes;
     };
     out->block = block;
     if (nc == 1 && kind != 0) {
@@ -526,6 +530,7 @@ static void fill_desc(Eng& e, int k, sph_ctx* c, int count, double dt, double ti
     d.g = c->grid();
     d.nl = c->nlist();
     d.porder = c->porder;
+    d.n_heavy = heavy_waves(c);
     d.keys = c->keys;
     for (int f = 0; f < 5; f++) d.snap[f] = c->snap[f];
     d.ppe = ppe_view(c);

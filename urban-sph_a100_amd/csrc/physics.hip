@@ -30,6 +30,24 @@ namespace sph {
     const CaseDesc& cd = (sg).cd[cid_];           \
     const int i = first_ + (int)threadIdx.x;      \
     if (i >= cd.n) return;
+// prep, advection and the pressure force: the particles of short lists, a
+// thread each in the pass order; in MIXED the last cd.n_heavy of that order
+// (the heavy lists) are left to a wave each (SEG_HEAVY)
+#define SEG_PARTICLE_LIGHT(sg)                            \
+    int first_;                                           \
+    const int cid_ = seg_case((sg), first_);              \
+    const CaseDesc& cd = (sg).cd[cid_];                   \
+    const int t_ = first_ + (int)threadIdx.x;             \
+    if (t_ >= cd.n - cd.n_heavy) return;                  \
+    const int i = cd.porder ? cd.porder[t_] : t_;
+#define SEG_HEAVY(sg)                                         \
+    int first_;                                               \
+    const int cid_ = seg_case((sg), first_);                  \
+    const CaseDesc& cd = (sg).cd[cid_];                       \
+    const int u_ = (first_ + (int)threadIdx.x) >> 6;          \
+    const int lane_ = threadIdx.x & 63;                       \
+    if (u_ >= cd.n_heavy) return;                             \
+    const int i = cd.porder[cd.n - cd.n_heavy + u_];
 // The same through the case's pass order when it has one (long neighbour
 // lists last): for the record-gathering passes, where it measured faster.
 #define SEG_PARTICLE_ORDERED(sg)                          \
@@ -214,8 +232,11 @@ __global__ void __launch_bounds__(BLOCK) k_pack_records(Seg sg) {
     r[6] = make_double2(S.temp[i], S.concn[i]);
 }
 
-template <class R, class Src>
-__device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src) {
+// W = 64 (MIXED, a heavy list): the calling wave's lanes take entries lane,
+// lane + 64, ... and wave_fold adds their sums; lane 0 writes.
+template <class R, class Src, int W = 1>
+__device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src,
+                                          int lane = 0) {
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const PView S = cd.S;
@@ -241,7 +262,7 @@ __device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList
     const bool les = (pr.fv_solve != 0) && (pr.turbulence_model != SPH_LAMINAR);
     const bool compact = Src::lds || (sizeof(R) == 4 && src.ok());
     const NLRow row = nl_row(nl, li);
-    for (int k = 0; k < row.cnt; k++) {
+    for (int k = W > 1 ? lane : 0; k < row.cnt; k += W) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_HP)) continue;
         int j = e & NB_IDX;
@@ -289,6 +310,11 @@ __device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList
             }
         }
     }
+    if constexpr (W > 1) {
+        wave_fold(tmp_flt); wave_fold(tmp_sigma);
+        wave_fold(st11); wave_fold(st12); wave_fold(st22); wave_fold(st23); wave_fold(st31); wave_fold(st33);
+        if (lane != 0) return;
+    }
     if (les) {
         double tmp_SR = 2 * (st11 * st11 + st22 * st22 + st33 * st33 + 2 * st12 * st12 + 2 * st31 * st31 + 2 * st23 * st23);
         tmp_SR = sqrt(tmp_SR);
@@ -315,8 +341,12 @@ __device__ __forceinline__ void prep_body(const CaseDesc& cd, int i, const NList
 }
 template <class R>
 __global__ void __launch_bounds__(MAX_BLOCK) k_prep(Seg sg) {
-    SEG_PARTICLE(sg);
+    SEG_PARTICLE_LIGHT(sg);
     prep_body<R>(cd, i, cd.nl, i, GSrc<CREC_ADV_FLOATS>{sizeof(R) == 4 ? &cd.crec_adv : nullptr});
+}
+__global__ void __launch_bounds__(WAVE_BLOCK) k_prep_heavy(Seg sg) {
+    SEG_HEAVY(sg);
+    prep_body<float, GSrc<CREC_ADV_FLOATS>, 64>(cd, i, cd.nl, i, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv}, lane_);
 }
 __global__ void __launch_bounds__(TILE_MAX) k_prep_tile(Seg sg) {
     TILE_PARTICLE(sg, cd.crec_adv, CREC_ADV_FLOATS);
@@ -370,10 +400,6 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_boundary3d(Seg sg) {
     }
 }
 
-// The boundary passes of MIXED run a 64-lane wave per boundary particle,
-// four waves (four particles) to a workgroup, a quarter of the workgroups
-// of one wave each.  The waves share nothing.
-constexpr int WAVE_BLOCK = 256;
 // The same in MIXED, a 64-lane wave per boundary particle (walls are a few
 // per cent of the case): lane sums over entries lane, lane + 64, ..., then
 // a fixed butterfly; only the particles the serial kernel writes (p_type 0
@@ -803,8 +829,9 @@ __global__ void __launch_bounds__(64) k_neumann(Seg sg, int which, int only_coun
 // SCAL: some case of the launch transports temperature or concentration or
 // uses the Boussinesq buoyancy; without them (the canyon) those terms and
 // their registers are compiled out (130 -> fewer VGPRs, more waves per SIMD)
-template <class R, bool SCAL, class Src>
-__device__ __forceinline__ void advection_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src) {
+template <class R, bool SCAL, class Src, int W = 1>
+__device__ __forceinline__ void advection_body(const CaseDesc& cd, int i, const NList& nl, int li, const Src src,
+                                               int lane = 0) {
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const bool con_solve = SCAL && pr.con_solve, concn_solve = SCAL && pr.concn_solve;
@@ -836,7 +863,7 @@ __device__ __forceinline__ void advection_body(const CaseDesc& cd, int i, const 
     const R ome = (R)(1 - eli);
     const bool compact = Src::lds || (sizeof(R) == 4 && src.ok());
     const NLRow row = nl_row(nl, li);
-    for (int k = 0; k < row.cnt; k++) {
+    for (int k = W > 1 ? lane : 0; k < row.cnt; k += W) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_DP)) continue;
         int j = e & NB_IDX;
@@ -944,6 +971,12 @@ __device__ __forceinline__ void advection_body(const CaseDesc& cd, int i, const 
             }
         }
     }
+    if constexpr (W > 1) {
+        wave_fold(tmpx); wave_fold(tmpy); wave_fold(tmpz); wave_fold(tmp_Rc); wave_fold(tmp_Rd);
+        wave_fold(eulerx); wave_fold(eulery); wave_fold(eulerz); wave_fold(eulert);
+        wave_fold(tmp_temp); wave_fold(tmp_flt);
+        if (lane != 0) return;
+    }
     if (pr.fg_solve) tmpz += -pr.gravity;
     if ((boussinesq_solve) && (ptypei > 0) && (ptypei < 1000))
         tmpz = betai * pr.gravity * (tmp_temp / tmp_flt - pr.tempb);
@@ -957,8 +990,13 @@ __device__ __forceinline__ void advection_body(const CaseDesc& cd, int i, const 
 template <class R, bool SCAL>
 __global__ void __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(SCAL ? 1 : 4)))
 k_advection(Seg sg) {
-    SEG_PARTICLE_ORDERED(sg);
+    SEG_PARTICLE_LIGHT(sg);
     advection_body<R, SCAL>(cd, i, cd.nl, i, GSrc<CREC_ADV_FLOATS>{sizeof(R) == 4 ? &cd.crec_adv : nullptr});
+}
+template <bool SCAL>
+__global__ void __launch_bounds__(WAVE_BLOCK) k_advection_heavy(Seg sg) {
+    SEG_HEAVY(sg);
+    advection_body<float, SCAL, GSrc<CREC_ADV_FLOATS>, 64>(cd, i, cd.nl, i, GSrc<CREC_ADV_FLOATS>{&cd.crec_adv}, lane_);
 }
 template <bool SCAL>
 __global__ void __launch_bounds__(TILE_MAX) k_advection_tile(Seg sg) {
@@ -1188,9 +1226,9 @@ __global__ void __launch_bounds__(MAX_BLOCK) k_outlet(Seg sg, int which) {
 // ------------------------------------------------------------------------
 // KERNEL_pressureforce3D, functions_PPE.cuh:482-647.  row: i's neighbour
 // entries; src: where their records are (compact records in MIXED).
-template <class R, class Src>
+template <class R, class Src, int W = 1>
 __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, const NList& nl, int li,
-                                                    const Src src) {
+                                                    const Src src, int lane = 0) {
     const KConst kc = cd.kc;
     const sph_params pr = cd.pr;
     const PView S = cd.S;
@@ -1210,7 +1248,7 @@ __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, c
     const NLRow row = nl_row(nl, li);
     if (sizeof(R) == 4 && src.ok()) {
         // the same sums from the compact float records
-        for (int k = 0; k < row.cnt; k++) {
+        for (int k = W > 1 ? lane : 0; k < row.cnt; k += W) {
             unsigned e = NB_AT(row, k);
             if (!(e & F_DM)) continue;
             const float* C = src.at(e & NB_IDX);
@@ -1232,11 +1270,15 @@ __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, c
                 tmpx += (double)(C_p * tdwx); tmpy += (double)(C_p * tdwy); tmpz += (double)(C_p * tdwz);
             }
         }
+        if constexpr (W > 1) {
+            wave_fold(tmpx); wave_fold(tmpy); wave_fold(tmpz);
+            if (lane != 0) return;
+        }
         Q.fpx[i] = tmpx; Q.fpy[i] = tmpy; Q.fpz[i] = tmpz;
         return;
     }
     if constexpr (Src::lds) return;   // staged records are compact
-    for (int k = 0; k < row.cnt; k++) {
+    for (int k = W > 1 ? lane : 0; k < row.cnt; k += W) {
         unsigned e = NB_AT(row, k);
         if (!(e & F_DM)) continue;
         int j = e & NB_IDX;
@@ -1260,12 +1302,20 @@ __device__ __forceinline__ void pressure_force_body(const CaseDesc& cd, int i, c
             tmpx += (double)(C_p * tdwx); tmpy += (double)(C_p * tdwy); tmpz += (double)(C_p * tdwz);
         }
     }
+    if constexpr (W > 1) {
+        wave_fold(tmpx); wave_fold(tmpy); wave_fold(tmpz);
+        if (lane != 0) return;
+    }
     Q.fpx[i] = tmpx; Q.fpy[i] = tmpy; Q.fpz[i] = tmpz;
 }
 template <class R>
 __global__ void __launch_bounds__(MAX_BLOCK) k_pressure_force(Seg sg) {
-    SEG_PARTICLE_ORDERED(sg);
+    SEG_PARTICLE_LIGHT(sg);
     pressure_force_body<R>(cd, i, cd.nl, i, GSrc<CREC_FLOATS>{sizeof(R) == 4 ? &cd.crec : nullptr});
+}
+__global__ void __launch_bounds__(WAVE_BLOCK) k_pressure_force_heavy(Seg sg) {
+    SEG_HEAVY(sg);
+    pressure_force_body<float, GSrc<CREC_FLOATS>, 64>(cd, i, cd.nl, i, GSrc<CREC_FLOATS>{&cd.crec}, lane_);
 }
 __global__ void __launch_bounds__(TILE_MAX) k_pressure_force_tile(Seg sg) {
     TILE_PARTICLE(sg, cd.crec, CREC_FLOATS);
@@ -1532,6 +1582,18 @@ __global__ void k_stats_final(const unsigned* in, float* out) {
     do {                                                                                            \
         Shape sh_;                                                                                  \
         SPH_TRY(shape(e, 3, WAVE_BLOCK, &sh_));                                                     \
+        if (sh_.grid > 0) {                                                                         \
+            SPH_TRY(prof_begin(e, name, 0.0));                                                      \
+            hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(WAVE_BLOCK), 0, e.st, Seg{e.d_desc, sh_.tab}, ##__VA_ARGS__); \
+            SPH_TRY(launch_check(&e, name));                                                        \
+            SPH_TRY(prof_end(e));                                                                   \
+        }                                                                                           \
+    } while (0)
+// the heavy-list share of a MIXED record pass: a wave per such particle
+#define HEAVYL(name, kern, ...)                                                                       \
+    do {                                                                                            \
+        Shape sh_;                                                                                  \
+        SPH_TRY(shape(e, 5, WAVE_BLOCK, &sh_));                                                     \
         if (sh_.grid > 0) {                                                                         \
             SPH_TRY(prof_begin(e, name, 0.0));                                                      \
             hipLaunchKernelGGL(kern, dim3(sh_.grid), dim3(WAVE_BLOCK), 0, e.st, Seg{e.d_desc, sh_.tab}, ##__VA_ARGS__); \
@@ -1822,8 +1884,10 @@ int step_physics_front(Eng& e) {
     }
     SPH_TRY(pack_records(e, REC_PRE));   // compact records for the strain pass
     if (mixed && e.tile_lanes) TILEL("prep", k_prep_tile, CREC_ADV_FLOATS);
-    else if (mixed) SEGL("prep", TK_PREP, k_prep<float>);
-    else SEGL("prep", TK_PREP, k_prep<double>);
+    else if (mixed) {
+        SEGL("prep", TK_PREP, k_prep<float>);
+        HEAVYL("prep_heavy", k_prep_heavy);
+    } else SEGL("prep", TK_PREP, k_prep<double>);
     if (mixed) WAVEL("boundary3d", k_boundary3d_wave);
     else SEGL("boundary3d", TK_BOUNDARY, k_boundary3d<double>);
     SEGL("most_self", -1, k_most_self);
@@ -1836,8 +1900,13 @@ int step_physics_front(Eng& e) {
         if (e.tile_lanes) {
             if (scal) TILEL("advection", k_advection_tile<true>, CREC_ADV_FLOATS);
             else TILEL("advection", k_advection_tile<false>, CREC_ADV_FLOATS);
-        } else if (scal) SEGL("advection", TK_ADVECTION, (k_advection<float, true>));
-        else SEGL("advection", TK_ADVECTION, (k_advection<float, false>));
+        } else if (scal) {
+            SEGL("advection", TK_ADVECTION, (k_advection<float, true>));
+            HEAVYL("advection_heavy", k_advection_heavy<true>);
+        } else {
+            SEGL("advection", TK_ADVECTION, (k_advection<float, false>));
+            HEAVYL("advection_heavy", k_advection_heavy<false>);
+        }
     } else {
         if (scal) SEGL("advection", TK_ADVECTION, (k_advection<double, true>));
         else SEGL("advection", TK_ADVECTION, (k_advection<double, false>));
@@ -1855,7 +1924,10 @@ int step_physics_back(Eng& e) {
     SPH_TRY(run_outlet(e, 0));
     SPH_TRY(pack_records(e, REC_PQ));   // pressures after the solve and the boundary passes
     if (mixed && e.tile_lanes) TILEL("pressure_force", k_pressure_force_tile, CREC_FLOATS);
-    else if (mixed) SEGL("pressure_force", TK_PRESSURE, k_pressure_force<float>);
+    else if (mixed) {
+        SEGL("pressure_force", TK_PRESSURE, k_pressure_force<float>);
+        HEAVYL("pressure_force_heavy", k_pressure_force_heavy);
+    }
     else SEGL("pressure_force", TK_PRESSURE, k_pressure_force<double>);
     if (mixed) WAVEL("ibm_force", k_ibm_force_wave);
     else SEGL("ibm_force", TK_IBM_FORCE, k_ibm_force<double>);

@@ -141,6 +141,8 @@ struct CaseDesc {
     int gc_fresh;        // the correction matrices inv_cm hold this search's values (fixed positions)
     int vals_fresh;      // the PPE values and diagonal hold this search's values (fixed positions)
     const int* porder;   // particle passes: thread -> particle (long neighbour lists last), or null
+    int n_heavy;         // MIXED: the last n_heavy particles of porder (lists of more than 512 entries),
+                         // which prep, advection and the pressure force give a wave each; 0 otherwise
     const int* nm_list;  // particles the Neumann pass updates (p_type != 1), in sorted order
     int nm_count;
     const long long* nmp_off;   // per nm_list particle: its first Neumann pair (nm_count + 1)
@@ -187,6 +189,15 @@ struct Seg {
     const CaseDesc* cd;
     const int2* tab;
 };
+// The passes that give a boundary particle or a heavy list a 64-lane wave
+// run four waves (four particles) to a workgroup.  The waves share nothing.
+constexpr int WAVE_BLOCK = 256;
+// The lane sums of a wave added by a fixed butterfly (every lane gets the
+// total), so a particle's sums do not depend on the launch.
+__device__ __forceinline__ void wave_fold(double& v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+}
 // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
 // b mod 8).  Logical block order is remapped so that each XCD works on one
 // contiguous range of blocks: neighbouring rows, whose gathers touch the
@@ -618,6 +629,10 @@ int neumann_pairs(sph_ctx* c);
 int ibm_near_build(sph_ctx* c);
 // tiles.hip: the cell tiles of a case's current search
 int tile_build(sph_ctx* c);
+// MIXED: the particles whose neighbour lists the record passes give a wave
+// each (the heavy ones at the end of the pass order); 0 in FP64, whose sums
+// keep the serial order
+int heavy_waves(const sph_ctx* c);
 void tile_release(sph_ctx* c);
 // aggregation multigrid preconditioner of the MIXED inner solve (mg.hip)
 struct MGState;
