@@ -295,62 +295,119 @@ __global__ void __launch_bounds__(BLOCK) k_rowof_from_rows(int nr, const int* ro
 // Columns of each row in ascending (lattice) order, and the slot each entry
 // takes, indexed by its rank in the neighbour list (the order in which the
 // assembly visits it).  One wave per row: the lanes take the list 64 entries
-// at a time and place the selected ones at their ranks (a ballot prefix),
-// then a bitonic sort of (column, rank) keys in LDS orders them.  Rows
-// longer than SORT_CAP set *overflow (the case then keeps cell order).
+// at a time and place the selected ones' (column, rank) keys at their ranks
+// in LDS (a ballot prefix).  A key's place in the sorted row is the number of
+// the row's keys below it: each lane counts that for its keys against all of
+// them, read four at a time with one broadcast LDS load.  The keys are
+// distinct (the rank breaks ties), so the places are a permutation.  Keys are
+// 32-bit (column << 8 | rank) while the columns fit 24 bits, else 64-bit.
+// Rows longer than SORT_CAP set *overflow (the case then keeps cell order).
 constexpr int SORT_CAP = 160;
-constexpr int SORT_KEYS = 256;   // the bitonic sort's width (>= SORT_CAP)
-static_assert(SORT_KEYS >= SORT_CAP && (SORT_KEYS & (SORT_KEYS - 1)) == 0, "bitonic width");
+constexpr int SORT_KEYS = 256;   // key room per row (>= SORT_CAP + 3; ranks fit 8 bits)
+static_assert(SORT_KEYS >= SORT_CAP + 3 && SORT_KEYS <= 256, "key room");
 // lattice-ordered rows fit the sort, so they never have tails
 static_assert(LONG_ROW >= SORT_CAP, "a lattice-ordered row must fit the SELL part");
+constexpr int SORT_NARROW_ROWS = 1 << 24;   // rows whose columns fit a 32-bit key
+template <class K> struct SortKey;
+template <> struct SortKey<unsigned> {
+    static constexpr int SHIFT = 8;
+    using Quad = uint4;
+    __device__ static void split(const Quad& v, unsigned (&k)[4]) { k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w; }
+};
+template <> struct SortKey<unsigned long long> {
+    static constexpr int SHIFT = 32;
+    struct Quad { ulonglong2 a, b; };
+    __device__ static void split(const Quad& v, unsigned long long (&k)[4]) {
+        k[0] = v.a.x; k[1] = v.a.y; k[2] = v.b.x; k[3] = v.b.y;
+    }
+};
+template <class K>
 __global__ void __launch_bounds__(BLOCK) k_ppe_struct_sorted(int nr, const int* rows, const int* rowof, PView S,
                                                              NList nl, const int* cnt, const long long* sl_off,
                                                              int* cols, unsigned char* slotmap, int* overflow) {
-    __shared__ unsigned long long keys[BLOCK / SLICE][SORT_KEYS];
+    using SK = SortKey<K>;
+    constexpr int SH = SK::SHIFT;
+    constexpr int KPL = (SORT_CAP + SLICE - 1) / SLICE;   // keys per lane
+    __shared__ __align__(16) K keys[BLOCK / SLICE][SORT_KEYS];
     const int w = threadIdx.x / SLICE, lane = threadIdx.x % SLICE;
-    const int r = blockIdx.x * (BLOCK / SLICE) + w;
+    // a slice's rows on one XCD: their scattered slot writes fill whole
+    // lines in one L2
+    const int r = (int)xcd_block() * (BLOCK / SLICE) + w;
     if (r >= nr) return;   // the whole wave
     const int m = cnt[r];
     if (m > SORT_CAP) {
         if (lane == 0) *overflow = 1;
         return;
     }
-    unsigned long long* key = keys[w];
+    K* key = keys[w];
     const int i = rows[r];
     const unsigned* row = nl.ent + nl.slice_off[i >> 6] + (i & 63);
     const int ml = nl.cnt[i];
     int e = 0;   // selected entries so far, in list order
-    for (int k0 = 0; k0 < ml; k0 += SLICE) {
-        const int k = k0 + lane;
-        int rj = -1;
-        if (k < ml) {
-            const unsigned en = row[(size_t)k * SLICE];
-            const int j = en & NB_IDX;
-            if ((en & F_DP) && S.p_type[j] < 1000 && S.i_type[j] != 4 && j != i) rj = rowof[j];
-        }
-        const unsigned long long sel = __ballot(rj >= 0);
-        const int pos = e + __popcll(sel & ((1ull << lane) - 1ull));
-        if (rj >= 0 && pos < SORT_KEYS) key[pos] = ((unsigned long long)(unsigned)rj << 32) | (unsigned)pos;
-        e += __popcll(sel);
-    }
-    for (int p = e + lane; p < SORT_KEYS; p += SLICE) key[p] = ~0ull;
-    __builtin_amdgcn_wave_barrier();
-    for (int k = 2; k <= SORT_KEYS; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = lane; t < SORT_KEYS; t += SLICE) {
-                const int u = t ^ j;
-                if (u > t) {
-                    const unsigned long long x = key[t], y = key[u];
-                    if ((x > y) == ((t & k) == 0)) { key[t] = y; key[u] = x; }
-                }
+    // GATHER chunks of 64 entries at a time, their loads independent of one
+    // another (the type and row gathers are not chained behind each other's
+    // tests), so a wave keeps many in flight
+    constexpr int GATHER = 4;
+    for (int k0 = 0; k0 < ml; k0 += GATHER * SLICE) {
+        int rj[GATHER];
+#pragma unroll
+        for (int g = 0; g < GATHER; g++) {
+            const int k = k0 + g * SLICE + lane;
+            rj[g] = -1;
+            if (k < ml) {
+                const unsigned en = row[(size_t)k * SLICE];
+                const int j = en & NB_IDX;
+                const int pt = S.p_type[j], it = S.i_type[j], rw = rowof[j];
+                if ((en & F_DP) && pt < 1000 && it != 4 && j != i) rj[g] = rw;
             }
-            __builtin_amdgcn_wave_barrier();
         }
+#pragma unroll
+        for (int g = 0; g < GATHER; g++) {
+            const unsigned long long sel = __ballot(rj[g] >= 0);
+            const int pos = e + __popcll(sel & ((1ull << lane) - 1ull));
+            if (rj[g] >= 0 && pos < SORT_CAP) key[pos] = ((K)(unsigned)rj[g] << SH) | (K)(unsigned)pos;
+            e += __popcll(sel);
+        }
+    }
+    // e (a sum of ballots) is wave-uniform; the count pass selected the same
+    // entries, so e == m: anything else keeps cell order rather than write
+    // past the row
+    if (e != m) {
+        if (lane == 0) *overflow = 1;
+        return;
+    }
+    const int e4 = (e + 3) & ~3;
+    if (lane < e4 - e) key[e + lane] = ~(K)0;   // pads of the last quad, above every key
+    // the wave's LDS writes before its reads of other lanes' keys
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    K mine[KPL];
+    int place[KPL];
+#pragma unroll
+    for (int q = 0; q < KPL; q++) {
+        const int t = lane + q * SLICE;
+        mine[q] = t < e ? key[t] : ~(K)0;
+        place[q] = 0;
+    }
+    const typename SK::Quad* quads = (const typename SK::Quad*)key;
+    for (int p = 0; p < e4 / 4; p++) {
+        K k4[4];
+        SK::split(quads[p], k4);   // the same address in every lane: a broadcast
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int q = 0; q < KPL; q++) place[q] += k4[u] < mine[q] ? 1 : 0;
+    }
     const long long base = prow(sl_off, r);
-    for (int p = lane; p < m; p += SLICE) {
-        const unsigned long long x = key[p];
-        cols[base + pslot(p)] = (int)(x >> 32);
-        slotmap[base + pslot((int)(x & 0xffffffffu))] = (unsigned char)p;
+    constexpr K RANK = ((K)1 << SH) - 1;
+#pragma unroll
+    for (int q = 0; q < KPL; q++) {
+        const int t = lane + q * SLICE;
+        if (t < e) {
+            cols[base + pslot(place[q])] = (int)(mine[q] >> SH);
+            slotmap[base + pslot((int)(mine[q] & RANK))] = (unsigned char)place[q];
+        }
     }
 }
 
@@ -1207,10 +1264,12 @@ static int lattice_order(sph_ctx* c, PPEState* P, bool* used) {
 // numbering.
 static int ppe_count_and_slices(sph_ctx* c, PPEState* P, bool tails) {
     const int nr = P->nr;
+    SPH_TRY(prof_begin(c, "ppe_count", 0));
     hipLaunchKernelGGL(k_ppe_struct<false>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
                        (const int*)P->rowof, c->sview(), c->nlist(), P->cnt, (const long long*)nullptr, (int*)nullptr,
                        (const int*)nullptr, (const long long*)nullptr, (int*)nullptr);
     SPH_TRY(launch_check(c, "ppe_count"));
+    SPH_TRY(prof_end(c));
     if (tails) {
         // rows past LONG_ROW entries keep the rest of their entries in a tail
         int *flag = nullptr, *scan = nullptr, nlong = 0;
@@ -1449,11 +1508,18 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
         SPH_TRY(dalloc(c, &P->slotmap, P->nnz_slots + 1));
         SPH_TRY(dalloc(c, &ovf, 1));
         hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(BLOCK), 0, c->stream, 1LL, ovf);
-        hipLaunchKernelGGL(k_ppe_struct_sorted, dim3((nr + BLOCK / SLICE - 1) / (BLOCK / SLICE)), dim3(BLOCK), 0,
-                           c->stream, nr, (const int*)P->rows,
-                           (const int*)P->rowof, S, c->nlist(), (const int*)P->cnt, (const long long*)P->sl_off,
-                           P->cols, P->slotmap, ovf);
+        SPH_TRY(prof_begin(c, "ppe_cols_sorted", 0));
+        const dim3 sgrid((nr + BLOCK / SLICE - 1) / (BLOCK / SLICE));
+        if (nr < SORT_NARROW_ROWS)
+            hipLaunchKernelGGL(k_ppe_struct_sorted<unsigned>, sgrid, dim3(BLOCK), 0, c->stream, nr,
+                               (const int*)P->rows, (const int*)P->rowof, S, c->nlist(), (const int*)P->cnt,
+                               (const long long*)P->sl_off, P->cols, P->slotmap, ovf);
+        else
+            hipLaunchKernelGGL(k_ppe_struct_sorted<unsigned long long>, sgrid, dim3(BLOCK), 0, c->stream, nr,
+                               (const int*)P->rows, (const int*)P->rowof, S, c->nlist(), (const int*)P->cnt,
+                               (const long long*)P->sl_off, P->cols, P->slotmap, ovf);
         SPH_TRY(launch_check(c, "ppe_cols_sorted"));
+        SPH_TRY(prof_end(c));
         int hovf = 0;
         SPH_HIP(c, hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         SPH_HIP(c, hipStreamSynchronize(c->stream));
@@ -1468,10 +1534,12 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
         // fixed particles keep their lattice stencils from step to step
         if (c->prm.ppe_stencil && !c->prm.ale_positions) SPH_TRY(stencil_slices(c, P));
     } else {
+        SPH_TRY(prof_begin(c, "ppe_cols", 0));
         hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
                            (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols,
                            (const int*)P->ltail, (const long long*)P->tail_off, P->tail_cols);
         SPH_TRY(launch_check(c, "ppe_cols"));
+        SPH_TRY(prof_end(c));
     }
     P->nb = (nr + BLOCK - 1) / BLOCK;
     if (c->prm.ppe_index16 && (long long)nr <= 65536LL * BLOCK) {   // the window bitmap covers 16.7M columns
@@ -1483,9 +1551,11 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
         hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(BLOCK), 0, c->stream, 1LL, ovf);
         hipLaunchKernelGGL(k_zero<int>, dim3(grid1((long long)P->nb * BLOCK)), dim3(BLOCK), 0, c->stream,
                            (long long)P->nb * BLOCK, P->gbase);
+        SPH_TRY(prof_begin(c, "ecols_build", 0));
         hipLaunchKernelGGL(k_ecols_build, dim3(P->nslices), dim3(SLICE), 0, c->stream, nr, (const long long*)P->sl_off,
                            (const int*)P->cnt, (const int*)P->cols, P->ecols, P->gbase, ovf);
         SPH_TRY(launch_check(c, "ecols_build"));
+        SPH_TRY(prof_end(c));
         int hovf = 0;
         SPH_HIP(c, hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         SPH_HIP(c, hipStreamSynchronize(c->stream));
