@@ -357,7 +357,9 @@ __device__ __forceinline__ int nl_wave(int i, const Grid& g, const double* x, co
 }
 __global__ void __launch_bounds__(BLOCK) k_nl_count(int n, Grid g, const double* x, const double* y, const double* z,
                                                     const double* h, const int* ncell, double kappa, int* cnt) {
-    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    // xcd_block: a list slice's 64 particles on one XCD, so their
+    // interleaved entries fill whole lines in one L2
+    const int i = (int)(((long long)xcd_block() * blockDim.x + threadIdx.x) >> 6);
     if (i >= n) return;   // whole waves
     const int m = nl_wave<false>(i, g, x, y, z, h, ncell, kappa, nullptr, 0);
     if ((threadIdx.x & 63) == 0) cnt[i] = m;
@@ -368,7 +370,9 @@ __global__ void __launch_bounds__(BLOCK) k_nl_fill(int n, Grid g, const double* 
                                                    const double* h, const int* ncell, double kappa,
                                                    const long long* slice_off, long long total, unsigned* ent, int* cnt,
                                                    int* over) {
-    const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    // xcd_block: a list slice's 64 particles on one XCD, so their
+    // interleaved entries fill whole lines in one L2
+    const int i = (int)(((long long)xcd_block() * blockDim.x + threadIdx.x) >> 6);
     if (i >= n) return;   // whole waves
     const int s = i >> 6;
     const long long end = s + 1 < (n + SLICE - 1) / SLICE ? slice_off[s + 1] : total;
