@@ -293,152 +293,127 @@ __global__ void __launch_bounds__(BLOCK) k_rowof_from_rows(int nr, const int* ro
     if (r < nr) rowof[rows[r]] = r;
 }
 // Columns of each row in ascending (lattice) order, and the slot each entry
-// takes, indexed by its rank in the neighbour list (the order in which the
-// assembly visits it).  One workgroup per 64-row slice, one wave per row at
-// a time: the lanes take the list 64 entries at a time and place the
-// selected ones' (column, rank) keys at their ranks in LDS (a ballot
-// prefix).  A key's place in the sorted row is the number of the row's keys
-// below it: each lane counts that for its keys against all of them, read
-// four at a time with one broadcast LDS load.  The keys are distinct (the
-// rank breaks ties), so the places are a permutation.  The sorted columns and
-// the slot map of the whole slice are staged in LDS and written slot by
-// slot, 64 rows side by side: whole cache lines.  Keys are 32-bit
-// (column << 8 | rank) while the columns fit 24 bits, else 64-bit.  Rows
-// longer than SORT_CAP set *overflow (the case then keeps cell order).
+// takes, indexed by its position in the neighbour list (the order in which
+// the assembly visits it).  The fill pass has written every row's columns in
+// list order, a thread per row; this pass sorts them in place, one workgroup
+// per 64-row slice.  The slice's columns are read slot by slot (whole lines)
+// into LDS; then one wave per row sorts the row's (column, position) keys,
+// held in registers, by a bitonic network over 64, 128 or 256 keys (the keys
+// are distinct, so the order is unique); then the sorted columns and the
+// slot map are written slot by slot.  Keys are 32-bit
+// (column << 8 | position) while the columns fit 24 bits, else 64-bit.  A
+// slice with a row longer than SORT_CAP sets *overflow and writes nothing
+// (the case then keeps cell order).
 constexpr int SORT_CAP = 160;
-constexpr int SORT_KEYS = 164;   // key room per row: SORT_CAP + the pads of the last quad
-static_assert(SORT_KEYS >= SORT_CAP + 3 && SORT_KEYS % 4 == 0 && SORT_CAP <= 256, "key room; ranks fit 8 bits");
+static_assert(SORT_CAP <= 4 * 64, "list positions fit 8 bits; a row fits the widest register sort");
 // lattice-ordered rows fit the sort, so they never have tails
 static_assert(LONG_ROW >= SORT_CAP, "a lattice-ordered row must fit the SELL part");
 constexpr int SORT_NARROW_ROWS = 1 << 24;   // rows whose columns fit a 32-bit key
+constexpr int SORT_BLOCK = 1024;            // 16 waves; LDS 62 KB a workgroup
 template <class K> struct SortKey;
-template <> struct SortKey<unsigned> {
-    static constexpr int SHIFT = 8;
-    static constexpr int BLOCK_ = 1024;   // 16 waves: LDS 63 KB a workgroup
-    using Quad = uint4;
-    __device__ static void split(const Quad& v, unsigned (&k)[4]) { k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w; }
-};
-template <> struct SortKey<unsigned long long> {
-    static constexpr int SHIFT = 32;
-    static constexpr int BLOCK_ = 512;    // 8 waves: the same LDS
-    struct Quad { ulonglong2 a, b; };
-    __device__ static void split(const Quad& v, unsigned long long (&k)[4]) {
-        k[0] = v.a.x; k[1] = v.a.y; k[2] = v.b.x; k[3] = v.b.y;
+template <> struct SortKey<unsigned> { static constexpr int SHIFT = 8; };
+template <> struct SortKey<unsigned long long> { static constexpr int SHIFT = 32; };
+// Bitonic sort of a wave's Q * 64 keys held in registers, key t = lane +
+// 64 q ascending: partners in other lanes through cross-lane moves, partners
+// in the same lane (distance 64 and more) by register.
+template <class K, int Q>
+__device__ __forceinline__ void bitonic_wave(K (&v)[Q], int lane) {
+    constexpr int W = Q * SLICE;
+#pragma unroll
+    for (int k = 2; k <= W; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= SLICE) {
+                const int qj = j / SLICE;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    if ((q & qj) == 0) {
+                        const bool up = ((lane + q * SLICE) & k) == 0;
+                        const K a = v[q], b = v[q | qj];
+                        const bool sw = (a > b) == up;
+                        v[q] = sw ? b : a;
+                        v[q | qj] = sw ? a : b;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const K o = __shfl_xor(v[q], j, SLICE);
+                    const bool up = ((lane + q * SLICE) & k) == 0;
+                    const bool take_min = ((lane & j) == 0) == up;
+                    v[q] = take_min ? (o < v[q] ? o : v[q]) : (o > v[q] ? o : v[q]);
+                }
+            }
+        }
     }
-};
-// the wave's LDS writes before its reads of other lanes' values
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Row l of the staged slice (m <= Q * 64 entries) sorted by (column, list
+// position): the slot map (position -> place) and its inverse into LDS.
+template <class K, int Q>
+__device__ __forceinline__ void sort_row(const int (*col_s)[SLICE + 1], unsigned char (*map_s)[SLICE + 4],
+                                         unsigned char (*inv_s)[SLICE + 4], int l, int m, int lane) {
+    constexpr int SH = SortKey<K>::SHIFT;
+    constexpr K POS = ((K)1 << SH) - 1;
+    K v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int t = lane + q * SLICE;
+        v[q] = t < m ? ((K)(unsigned)col_s[t][l] << SH) | (K)(unsigned)t : ~(K)0;   // pads sort last
+    }
+    bitonic_wave<K, Q>(v, lane);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int t = lane + q * SLICE;
+        if (t < m) {
+            const int pos = (int)(v[q] & POS);
+            map_s[pos][l] = (unsigned char)t;
+            inv_s[t][l] = (unsigned char)pos;
+        }
+    }
 }
 template <class K>
-__global__ void __launch_bounds__(SortKey<K>::BLOCK_) k_ppe_struct_sorted(int nr, const int* rows, const int* rowof,
-                                                                          PView S, NList nl, const int* cnt,
-                                                                          const long long* sl_off, int* cols,
-                                                                          unsigned char* slotmap, int* overflow) {
-    using SK = SortKey<K>;
-    constexpr int SH = SK::SHIFT;
-    constexpr int WAVES = SK::BLOCK_ / SLICE;
-    constexpr int KPL = (SORT_CAP + SLICE - 1) / SLICE;   // keys per lane
-    __shared__ __align__(16) K keys[WAVES][SORT_KEYS];
-    // the slice's sorted columns and slot map, slot-major; rows padded so a
-    // wave's writes of one row (one slot each) spread over the banks
+__global__ void __launch_bounds__(SORT_BLOCK) k_ppe_sort_slices(int nr, const int* cnt, const long long* sl_off,
+                                                                int* cols, unsigned char* slotmap, int* overflow) {
+    constexpr int WAVES = SORT_BLOCK / SLICE;
+    // slot-major like the matrix; rows padded so a wave's accesses to one
+    // row (one slot per lane) spread over the banks
     __shared__ int col_s[SORT_CAP][SLICE + 1];
-    __shared__ unsigned char map_s[SORT_CAP][SLICE + 4];
+    __shared__ unsigned char map_s[SORT_CAP][SLICE + 4];   // list position -> place
+    __shared__ unsigned char inv_s[SORT_CAP][SLICE + 4];   // place -> list position
     __shared__ int m_s[SLICE];
     __shared__ int bad;
     const int w = threadIdx.x / SLICE, lane = threadIdx.x % SLICE;
-    // consecutive slices on one XCD: neighbouring rows gather the same
-    // particles' list entries
+    // consecutive slices on one XCD
     const int sl = (int)xcd_block();
+    const long long base = sl_off[sl];
     if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    for (int l = w; l < SLICE; l += WAVES) {
-        const int r = sl * SLICE + l;
-        const int m = r < nr ? cnt[r] : 0;
-        if (lane == 0) m_s[l] = m;
-        if (m == 0) continue;
-        if (m > SORT_CAP) {
-            if (lane == 0) bad = 1;
-            continue;
-        }
-        K* key = keys[w];
-        const int i = rows[r];
-        const unsigned* row = nl.ent + nl.slice_off[i >> 6] + (i & 63);
-        const int ml = nl.cnt[i];
-        int e = 0;   // selected entries so far, in list order
-        // GATHER chunks of 64 entries at a time, their loads independent of
-        // one another, so a wave keeps many in flight
-        constexpr int GATHER = 4;
-        for (int k0 = 0; k0 < ml; k0 += GATHER * SLICE) {
-            int rj[GATHER];
-#pragma unroll
-            for (int g = 0; g < GATHER; g++) {
-                const int k = k0 + g * SLICE + lane;
-                rj[g] = -1;
-                if (k < ml) {
-                    const unsigned en = row[(size_t)k * SLICE];
-                    const int j = en & NB_IDX;
-                    const int pt = S.p_type[j], it = S.i_type[j], rw = rowof[j];
-                    if ((en & F_DP) && pt < 1000 && it != 4 && j != i) rj[g] = rw;
-                }
-            }
-#pragma unroll
-            for (int g = 0; g < GATHER; g++) {
-                const unsigned long long sel = __ballot(rj[g] >= 0);
-                const int pos = e + __popcll(sel & ((1ull << lane) - 1ull));
-                if (rj[g] >= 0 && pos < SORT_CAP) key[pos] = ((K)(unsigned)rj[g] << SH) | (K)(unsigned)pos;
-                e += __popcll(sel);
-            }
-        }
-        // e (a sum of ballots) is wave-uniform; the count pass selected the
-        // same entries, so e == m: anything else keeps cell order
-        if (e != m) {
-            if (lane == 0) bad = 1;
-            continue;
-        }
-        const int e4 = (e + 3) & ~3;
-        if (lane < e4 - e) key[e + lane] = ~(K)0;   // pads of the last quad, above every key
-        wave_lds_sync();
-        K mine[KPL];
-        int place[KPL];
-#pragma unroll
-        for (int q = 0; q < KPL; q++) {
-            const int t = lane + q * SLICE;
-            mine[q] = t < e ? key[t] : ~(K)0;
-            place[q] = 0;
-        }
-        const typename SK::Quad* quads = (const typename SK::Quad*)key;
-        for (int p = 0; p < e4 / 4; p++) {
-            K k4[4];
-            SK::split(quads[p], k4);   // the same address in every lane: a broadcast
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-#pragma unroll
-                for (int q = 0; q < KPL; q++) place[q] += k4[u] < mine[q] ? 1 : 0;
-        }
-        constexpr K RANK = ((K)1 << SH) - 1;
-#pragma unroll
-        for (int q = 0; q < KPL; q++) {
-            const int t = lane + q * SLICE;
-            if (t < e) {
-                col_s[place[q]][l] = (int)(mine[q] >> SH);
-                map_s[(int)(mine[q] & RANK)][l] = (unsigned char)place[q];
-            }
-        }
-        wave_lds_sync();   // the key reads done before the next row's key writes
+    if (threadIdx.x < SLICE) {
+        const int r = sl * SLICE + (int)threadIdx.x;
+        m_s[threadIdx.x] = r < nr ? cnt[r] : 0;
     }
     __syncthreads();
-    if (bad) {   // the case keeps cell order; nothing of this slice is written
+    if (threadIdx.x < SLICE && m_s[threadIdx.x] > SORT_CAP) bad = 1;
+    for (int t = threadIdx.x; t < SORT_CAP * SLICE; t += SORT_BLOCK) {
+        const int k = t / SLICE, l = t % SLICE;
+        if (k < m_s[l]) col_s[k][l] = cols[base + pslot(k) + l];
+    }
+    __syncthreads();
+    if (bad) {
         if (threadIdx.x == 0) *overflow = 1;
         return;
     }
-    const long long base = sl_off[sl];
-    for (int t = threadIdx.x; t < SORT_CAP * SLICE; t += SK::BLOCK_) {
+    for (int l = w; l < SLICE; l += WAVES) {
+        const int m = m_s[l];   // uniform in the wave
+        if (m == 0) continue;
+        if (m <= SLICE) sort_row<K, 1>(col_s, map_s, inv_s, l, m, lane);
+        else if (m <= 2 * SLICE) sort_row<K, 2>(col_s, map_s, inv_s, l, m, lane);
+        else sort_row<K, 4>(col_s, map_s, inv_s, l, m, lane);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < SORT_CAP * SLICE; t += SORT_BLOCK) {
         const int k = t / SLICE, l = t % SLICE;
         if (k < m_s[l]) {
-            cols[base + pslot(k) + l] = col_s[k][l];
+            cols[base + pslot(k) + l] = col_s[inv_s[k][l]][l];
             slotmap[base + pslot(k) + l] = map_s[k][l];
         }
     }
@@ -1541,17 +1516,22 @@ static int ppe_build_ordered(sph_ctx* c, bool allow_lattice) {
         SPH_TRY(dalloc(c, &P->slotmap, P->nnz_slots + 1));
         SPH_TRY(dalloc(c, &ovf, 1));
         hipLaunchKernelGGL(k_zero<int>, dim3(1), dim3(BLOCK), 0, c->stream, 1LL, ovf);
+        // the columns in list order, a thread per row, then sorted in place
+        SPH_TRY(prof_begin(c, "ppe_cols", 0));
+        hipLaunchKernelGGL(k_ppe_struct<true>, dim3(grid1(nr)), dim3(BLOCK), 0, c->stream, nr, (const int*)P->rows,
+                           (const int*)P->rowof, S, c->nlist(), P->cnt, (const long long*)P->sl_off, P->cols,
+                           (const int*)P->ltail, (const long long*)P->tail_off, P->tail_cols);
+        SPH_TRY(launch_check(c, "ppe_cols"));
+        SPH_TRY(prof_end(c));
         SPH_TRY(prof_begin(c, "ppe_cols_sorted", 0));
         // one workgroup per 64-row slice
         if (nr < SORT_NARROW_ROWS)
-            hipLaunchKernelGGL(k_ppe_struct_sorted<unsigned>, dim3(P->nslices), dim3(SortKey<unsigned>::BLOCK_), 0,
-                               c->stream, nr, (const int*)P->rows, (const int*)P->rowof, S, c->nlist(),
+            hipLaunchKernelGGL(k_ppe_sort_slices<unsigned>, dim3(P->nslices), dim3(SORT_BLOCK), 0, c->stream, nr,
                                (const int*)P->cnt, (const long long*)P->sl_off, P->cols, P->slotmap, ovf);
         else
-            hipLaunchKernelGGL(k_ppe_struct_sorted<unsigned long long>, dim3(P->nslices),
-                               dim3(SortKey<unsigned long long>::BLOCK_), 0, c->stream, nr, (const int*)P->rows,
-                               (const int*)P->rowof, S, c->nlist(), (const int*)P->cnt, (const long long*)P->sl_off,
-                               P->cols, P->slotmap, ovf);
+            hipLaunchKernelGGL(k_ppe_sort_slices<unsigned long long>, dim3(P->nslices), dim3(SORT_BLOCK), 0,
+                               c->stream, nr, (const int*)P->cnt, (const long long*)P->sl_off, P->cols, P->slotmap,
+                               ovf);
         SPH_TRY(launch_check(c, "ppe_cols_sorted"));
         SPH_TRY(prof_end(c));
         int hovf = 0;
