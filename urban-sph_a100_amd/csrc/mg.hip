@@ -398,6 +398,23 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap(OP op, int n1, const int* agg,
 // of insertion) is flagged in redo, and counted in nredo, for the sorting
 // kernels above.
 constexpr int MG_HASH = 512;
+static_assert(MG_HASH / 2 <= 4 * SLICE, "a row's distinct columns fit the widest register sort");
+// key[0 .. total) ascending, by one wave, through registers
+template <int Q>
+__device__ __forceinline__ void sort_keys_in_place(unsigned long long* key, int total, int lane) {
+    unsigned long long v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int t = lane + q * SLICE;
+        v[q] = t < total ? key[t] : ~0ull;   // pads sort last
+    }
+    bitonic_wave<unsigned long long, Q>(v, lane);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int t = lane + q * SLICE;
+        if (t < total) key[t] = v[q];
+    }
+}
 template <class OP>
 __global__ void __launch_bounds__(SLICE) k_mg_rap_hash(OP op, int n1, const int* agg, const int* mstart,
                                                        const int* mem, double scale, int* redo, int* nredo,
@@ -456,23 +473,12 @@ __global__ void __launch_bounds__(SLICE) k_mg_rap_hash(OP op, int n1, const int*
     const int total = wave_sum(own);
     for (int h = lane; h < MG_HASH; h += SLICE)
         if (hk[h] >= 0) key[w++] = ((unsigned long long)(unsigned)hk[h] << 32) | (unsigned)h;
-    int np2 = 1;
-    while (np2 < total) np2 <<= 1;
-    for (int p = total + lane; p < np2; p += SLICE) key[p] = ~0ull;
     __syncthreads();
-    for (int k = 2; k <= np2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = lane; i < np2; i += SLICE) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const unsigned long long a = key[i], b = key[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) { key[i] = b; key[ixj] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    }
+    // sorted in registers (total <= MG_HASH / 2 = 4 * 64 keys), back to LDS
+    if (total <= SLICE) sort_keys_in_place<1>(key, total, lane);
+    else if (total <= 2 * SLICE) sort_keys_in_place<2>(key, total, lane);
+    else sort_keys_in_place<4>(key, total, lane);
+    __syncthreads();
     int below = 0;
     for (int p = lane; p < total; p += SLICE) below += (int)(key[p] >> 32) < I ? 1 : 0;
     const int sI = wave_sum(below);

@@ -116,6 +116,40 @@ __device__ __forceinline__ int stencil_of(const PPEView& pp, int r) {
     return pp.spat ? __builtin_amdgcn_readfirstlane(pp.spat[r >> 6]) : -1;
 }
 
+// Bitonic sort of a wave's Q * 64 keys held in registers, key t = lane +
+// 64 q ascending: partners in other lanes through cross-lane moves, partners
+// in the same lane (distance 64 and more) by register.
+template <class K, int Q>
+__device__ __forceinline__ void bitonic_wave(K (&v)[Q], int lane) {
+    constexpr int W = Q * SLICE;
+#pragma unroll
+    for (int k = 2; k <= W; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= SLICE) {
+                const int qj = j / SLICE;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    if ((q & qj) == 0) {
+                        const bool up = ((lane + q * SLICE) & k) == 0;
+                        const K a = v[q], b = v[q | qj];
+                        const bool sw = (a > b) == up;
+                        v[q] = sw ? b : a;
+                        v[q | qj] = sw ? a : b;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const K o = __shfl_xor(v[q], j, SLICE);
+                    const bool up = ((lane + q * SLICE) & k) == 0;
+                    const bool take_min = ((lane & j) == 0) == up;
+                    v[q] = take_min ? (o < v[q] ? o : v[q]) : (o > v[q] ? o : v[q]);
+                }
+            }
+        }
+    }
+}
 // Group bases of the block's slices into LDS (every thread of the block
 // takes part; call before the row guard).
 __device__ __forceinline__ void load_group_bases(const PPEView& pp, int first, int* wbase) {
